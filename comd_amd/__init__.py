@@ -1,0 +1,429 @@
+"""comd_amd -- MI355X (gfx950) link-cell force and timestep path of CoMD.
+
+Python view of the C-ABI in ``include/comd_hip.h`` (``libcomd_hip.so``) and
+of the setup helpers of ``include/comd_host.h`` (``libcomd_host.so``).  All
+force, rebinning and integration work runs in the HIP library; loading fails
+with an error when the built libraries are missing (there is no CPU path).
+
+Main objects:
+
+* :class:`EAMPotential` / :class:`LJPotential` -- potentials in the
+  reference's form (tables of src-flat/eam.c:130-398, Chebyshev fit of
+  src-flat/cheby.c:99-270, LJ parameters of src-flat/ljForce.c:75-86).
+* :class:`Simulation` -- one device context: binning, force, rebinning,
+  ``nsteps`` (src-flat/timestep.c:114-140), energies, download in the
+  reference's slot layout (``c*64+i``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+POT_DIR = os.path.join(PKG_DIR, "pots")
+MAXATOMS = 64
+
+OK = 0
+ERRORS = {
+    1: "bad argument", 2: "HIP error", 3: "out of device memory", 4: "link cell overflow (64 atoms)",
+    5: "atom outside the cell grid", 6: "grid has fewer than one cutoff-wide cell", 7: "no HIP device",
+    8: "call out of order",
+}
+POT_EAM, POT_LJ = 1, 2
+EVAL_TABLE, EVAL_CHEB = 0, 1
+PREC_FP64, PREC_MIXED = 0, 1
+K_REBIN, K_DENSITY, K_EMBED_FORCE, K_REDUCE, K_NCLASS = 0, 1, 2, 3, 4
+
+# unit constants of src-flat/include/constants.h:61-65
+BOHR_PER_ATU_TO_A_PER_S = 0.529 / 2.418e-17
+AMU_TO_ME = 1822.83
+HARTREE_EV = 27.211386
+KB_HARTREE = 3.166811563e-6
+
+
+class ComdError(RuntimeError):
+    def __init__(self, code, msg=""):
+        super().__init__(f"comd error {code} ({ERRORS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class Params(C.Structure):
+    """struct comd_params of include/comd_hip.h"""
+    _fields_ = [
+        ("potential", C.c_int), ("eval", C.c_int), ("precision", C.c_int), ("periodic", C.c_int),
+        ("cutoff", C.c_double), ("mass", C.c_double), ("lj_sigma", C.c_double), ("lj_epsilon", C.c_double),
+        ("tab_n", C.c_int * 3), ("tab_x0", C.c_double * 3), ("tab_xn", C.c_double * 3),
+        ("tab_invdx", C.c_double * 3), ("tab_values", _dp * 3),
+        ("cheb_n", C.c_int), ("cheb_coef", _dp), ("cheb_x0", C.c_double * 3), ("cheb_xn", C.c_double * 3),
+        ("nbx", C.c_int * 3), ("bounds", C.c_double * 3), ("boxsize", C.c_double * 3),
+    ]
+
+
+class PotArray(C.Structure):
+    """potentialarray_t (src-flat/include/eamTypes.h:68-74)"""
+    _fields_ = [("n", C.c_int), ("x0", C.c_double), ("xn", C.c_double), ("invDx", C.c_double), ("values", _dp)]
+
+
+class EamPotT(C.Structure):
+    """eampotential_t (src-flat/include/eamTypes.h:90-99)"""
+    _fields_ = [("cutoff", C.c_double), ("mass", C.c_double), ("force", C.c_void_p), ("destroy", C.c_void_p),
+                ("phi", C.POINTER(PotArray)), ("rho", C.POINTER(PotArray)), ("f", C.POINTER(PotArray)),
+                ("lat", C.c_double)]
+
+
+class ChebT(C.Structure):
+    """eam_cheby_t (src-flat/include/eamTypes.h:102-112)"""
+    _fields_ = [("cutoff", C.c_double), ("mass", C.c_double)] + [
+        (n, C.POINTER(PotArray)) for n in ("phi", "rho", "f", "dphi", "drho", "df")] + [("lat", C.c_double)]
+
+
+_HIP = None
+_HOST = None
+
+
+def _load():
+    global _HIP, _HOST
+    if _HIP is not None:
+        return _HIP, _HOST
+    hip_path = os.path.join(PKG_DIR, "libcomd_hip.so")
+    host_path = os.path.join(PKG_DIR, "libcomd_host.so")
+    for p in (hip_path, host_path):
+        if not os.path.exists(p):
+            raise ImportError(f"{p} is not built; run __graft_entry__.build() or make -C comd_amd")
+    hip = C.CDLL(hip_path, mode=C.RTLD_GLOBAL)
+    host = C.CDLL(host_path, mode=C.RTLD_GLOBAL)
+    vp = C.c_void_p
+    hip.comd_grid_for.argtypes = [C.POINTER(Params), C.c_double * 3, C.c_double, C.c_double]
+    hip.comd_create.argtypes = [C.POINTER(Params), C.POINTER(vp)]
+    hip.comd_destroy.argtypes = [vp]
+    hip.comd_sizes.argtypes = [vp, _ip, _ip]
+    hip.comd_upload.argtypes = [vp, _ip, _ip, _ip, _dp, _dp]
+    hip.comd_download.argtypes = [vp, _ip, _ip, _ip, _dp, _dp, _dp, _dp, _dp]
+    hip.comd_bin_atoms.argtypes = [vp, C.c_int, _dp, _dp, C.c_int]
+    hip.comd_bin_atoms_device.argtypes = [vp, C.c_int, vp, vp, C.c_int]
+    hip.comd_rebox.argtypes = [vp]
+    hip.comd_force.argtypes = [vp]
+    hip.comd_nsteps.argtypes = [vp, C.c_int, C.c_double, _dp]
+    hip.comd_energy.argtypes = [vp, _dp, _dp, _dp]
+    hip.comd_advance_positions.argtypes = [vp, C.c_double]
+    hip.comd_advance_velocity.argtypes = [vp, C.c_double]
+    hip.comd_set_precision.argtypes = [vp, C.c_int]
+    hip.comd_set_eval.argtypes = [vp, C.c_int]
+    hip.comd_sync.argtypes = [vp]
+    hip.comd_device_info.argtypes = [_ip, C.c_char_p, C.c_int]
+    hip.comd_last_error.argtypes = [vp]
+    hip.comd_last_error.restype = C.c_char_p
+    hip.comd_timing.argtypes = [vp, C.c_int]
+    hip.comd_timing_read.argtypes = [vp, _dp, _ip]
+    host.setEamPot.argtypes = [C.c_char_p, C.c_char_p]
+    host.setEamPot.restype = C.POINTER(EamPotT)
+    host.setChebPot.argtypes = [C.POINTER(EamPotT), C.c_int]
+    host.setChebPot.restype = C.POINTER(ChebT)
+    host.comd_fcc_positions.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, _dp, C.c_int]
+    host.comd_seed_state.argtypes = [C.c_ulonglong, C.c_int, C.c_double, C.c_double, C.c_double, _dp, _dp]
+    host.comd_write_gz.argtypes = [C.c_char_p, C.c_int, C.c_double * 3, _dp, _dp]
+    host.comd_read_gz.argtypes = [C.c_char_p, _ip, C.c_double * 3, C.POINTER(_dp), C.POINTER(_dp)]
+    host.comd_host_error.restype = C.c_char_p
+    _HIP, _HOST = hip, host
+    return hip, host
+
+
+def libraries():
+    """(hip, host) ctypes handles; raises ImportError when not built."""
+    return _load()
+
+
+def device_info():
+    """(number of HIP devices, name of the current one)."""
+    hip, _ = _load()
+    n = C.c_int()
+    buf = C.create_string_buffer(256)
+    hip.comd_device_info(C.byref(n), buf, 256)
+    return n.value, buf.value.decode()
+
+
+def _ptr(a, ct=C.c_double):
+    return a.ctypes.data_as(C.POINTER(ct)) if a is not None else None
+
+
+# ---------------------------------------------------------------------------
+# potentials
+
+class EAMPotential:
+    """EAM tables read by the host library (src-flat/eam.c:347-398) and the
+    32-term Chebyshev fit of src-flat/cheby.c:99-121."""
+
+    def __init__(self, name="cu500", directory=POT_DIR, ncheb=32):
+        _, host = _load()
+        p = host.setEamPot(directory.encode(), name.encode())
+        if not p:
+            raise ComdError(1, f"cannot read EAM potential {directory}/{name}: {host.comd_host_error().decode()}")
+        self._p = p
+        pot = p.contents
+        self.name = name
+        self.cutoff = pot.cutoff
+        self.mass = pot.mass
+        self.lat = pot.lat
+        self.tables = []
+        for arr in (pot.phi, pot.rho, pot.f):
+            a = arr.contents
+            vals = np.ctypeslib.as_array(C.cast(C.addressof(a.values.contents) - 8, _dp), shape=(a.n + 3,)).copy()
+            self.tables.append(dict(n=a.n, x0=a.x0, xn=a.xn, invdx=a.invDx, values=vals))
+        ch = host.setChebPot(p, ncheb).contents
+        self.ncheb = ncheb
+        self.cheb = np.zeros((6, ncheb))
+        self.cheb_dom = np.zeros((3, 2))
+        for k, arr in enumerate((ch.phi, ch.rho, ch.f, ch.dphi, ch.drho, ch.df)):
+            a = arr.contents
+            self.cheb[k] = np.ctypeslib.as_array(a.values, shape=(ncheb,))
+            if k < 3:
+                self.cheb_dom[k] = (a.x0, a.xn)
+        self.kind = POT_EAM
+
+    def fill(self, prm: Params):
+        prm.potential = POT_EAM
+        prm.cutoff = self.cutoff
+        prm.mass = self.mass
+        self._keep = [np.ascontiguousarray(t["values"]) for t in self.tables]
+        for k, t in enumerate(self.tables):
+            prm.tab_n[k] = t["n"]
+            prm.tab_x0[k] = t["x0"]
+            prm.tab_xn[k] = t["xn"]
+            prm.tab_invdx[k] = t["invdx"]
+            prm.tab_values[k] = _ptr(self._keep[k])
+        self._cheb_keep = np.ascontiguousarray(self.cheb.reshape(-1))
+        prm.cheb_n = self.ncheb
+        prm.cheb_coef = _ptr(self._cheb_keep)
+        for k in range(3):
+            prm.cheb_x0[k], prm.cheb_xn[k] = self.cheb_dom[k]
+
+    def with_cheb(self, coef):
+        """A copy whose Chebyshev coefficients are replaced by coef (6 x n)."""
+        import copy
+        q = copy.copy(self)
+        q.cheb = np.array(coef, dtype=np.float64).reshape(6, self.ncheb)
+        return q
+
+
+class LJPotential:
+    """12-6 Lennard-Jones with settable parameters (src-flat/ljForce.c:75-86
+    hard-codes sigma=1.53, eps=0.0085, rc=3 sigma, m=1).  sigma and cutoff
+    in A, epsilon in hartree, mass in amu."""
+
+    def __init__(self, sigma=1.53, epsilon=0.0085, cutoff=None, mass=1.0):
+        self.sigma = float(sigma)
+        self.epsilon = float(epsilon)
+        self.cutoff = float(cutoff) if cutoff is not None else 3.0 * self.sigma
+        self.mass = float(mass)
+        self.kind = POT_LJ
+
+    def fill(self, prm: Params):
+        prm.potential = POT_LJ
+        prm.cutoff = self.cutoff
+        prm.mass = self.mass
+        prm.lj_sigma = self.sigma
+        prm.lj_epsilon = self.epsilon
+
+
+def copper_lj():
+    """Lennard-Jones copper of the 32 000-atom benchmark case: sigma 2.315 A,
+    eps 0.167 eV, rc 2.5 sigma, Cu mass; lattice sqrt(2) 2^(1/6) sigma."""
+    s = 2.315
+    pot = LJPotential(sigma=s, epsilon=0.167 / HARTREE_EV, cutoff=2.5 * s, mass=63.55)
+    pot.lat = np.sqrt(2.0) * 2.0 ** (1.0 / 6.0) * s
+    return pot
+
+
+# ---------------------------------------------------------------------------
+# initial states
+
+def fcc_positions(nx, ny, nz, lat, defgrad=1.0):
+    """Atom positions in the order of create_fcc_lattice (src-flat/ic_fcc.c:115-132)."""
+    _, host = _load()
+    n = host.comd_fcc_positions(nx, ny, nz, lat, defgrad, None, 0)
+    xyz = np.zeros((n, 3))
+    host.comd_fcc_positions(nx, ny, nz, lat, defgrad, _ptr(xyz), n)
+    return xyz
+
+
+def seed_state(xyz, mass_amu, temp, disp, seed):
+    """Seeded Gaussian displacements and Maxwell-Boltzmann momenta (zero net
+    momentum); returns (xyz_displaced, p)."""
+    _, host = _load()
+    x = np.ascontiguousarray(xyz, dtype=np.float64).copy()
+    p = np.zeros_like(x)
+    host.comd_seed_state(int(seed), x.shape[0], float(mass_amu), float(temp), float(disp), _ptr(x), _ptr(p))
+    return x, p
+
+
+def write_gz(path, bounds, xyz, p=None):
+    _, host = _load()
+    b = (C.c_double * 3)(*bounds)
+    x = np.ascontiguousarray(xyz, dtype=np.float64)
+    pp = np.ascontiguousarray(p, dtype=np.float64) if p is not None else None
+    rc = host.comd_write_gz(str(path).encode(), x.shape[0], b, _ptr(x), _ptr(pp))
+    if rc:
+        raise ComdError(rc, host.comd_host_error().decode())
+
+
+def read_gz(path):
+    """Reference gz input (src-flat/read.c:155-241) -> (bounds, xyz, p)."""
+    _, host = _load()
+    n = C.c_int()
+    b = (C.c_double * 3)()
+    xp, pp = _dp(), _dp()
+    rc = host.comd_read_gz(str(path).encode(), C.byref(n), b, C.byref(xp), C.byref(pp))
+    if rc:
+        raise ComdError(rc, host.comd_host_error().decode())
+    m = n.value
+    xyz = np.ctypeslib.as_array(xp, shape=(max(m, 1), 3))[:m].copy()
+    p = np.ctypeslib.as_array(pp, shape=(max(m, 1), 3))[:m].copy()
+    libc = C.CDLL(None)
+    libc.free(C.cast(xp, C.c_void_p))
+    libc.free(C.cast(pp, C.c_void_p))
+    return list(b), xyz, p
+
+
+# ---------------------------------------------------------------------------
+
+class Simulation:
+    """One device context (comd_create) for a potential on a link-cell grid.
+
+    The grid follows allocDomains (src-flat/domains.c:155-200): pass
+    ``defgrad`` as the reference would see it (the FCC path keeps the
+    stretch, the file path uses 0)."""
+
+    def __init__(self, pot, bounds, defgrad=1.0, bf=1.0, periodic=True, precision="fp64", evaluation="table"):
+        hip, _ = _load()
+        self._hip = hip
+        self.pot = pot
+        prm = Params()
+        pot.fill(prm)
+        prm.periodic = 1 if periodic else 0
+        prm.precision = PREC_MIXED if precision == "mixed" else PREC_FP64
+        prm.eval = EVAL_CHEB if evaluation == "cheby" else EVAL_TABLE
+        rc = hip.comd_grid_for(C.byref(prm), (C.c_double * 3)(*bounds), float(defgrad), float(bf))
+        if rc:
+            raise ComdError(rc, "grid")
+        self.prm = prm
+        self.nbx = tuple(prm.nbx)
+        self.ncells = int(np.prod(self.nbx))
+        self.bounds = tuple(prm.bounds)
+        self.boxsize = tuple(prm.boxsize)
+        ctx = C.c_void_p()
+        rc = hip.comd_create(C.byref(prm), C.byref(ctx))
+        if rc:
+            raise ComdError(rc, hip.comd_last_error(None).decode())
+        self.ctx = ctx
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self._hip.comd_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc):
+        if rc:
+            raise ComdError(rc, self._hip.comd_last_error(self.ctx).decode())
+
+    @property
+    def nslots(self):
+        return self.ncells * MAXATOMS
+
+    def bin_atoms(self, xyz, p=None, itype=0):
+        x = np.ascontiguousarray(xyz, dtype=np.float64)
+        pp = np.ascontiguousarray(p, dtype=np.float64) if p is not None else None
+        self._check(self._hip.comd_bin_atoms(self.ctx, x.shape[0], _ptr(x), _ptr(pp), int(itype)))
+
+    def upload(self, natoms, ids, itype, r, p):
+        na = np.ascontiguousarray(natoms, dtype=np.int32)
+        i = np.ascontiguousarray(ids, dtype=np.int32)
+        t = np.ascontiguousarray(itype, dtype=np.int32)
+        rr = np.ascontiguousarray(r, dtype=np.float64)
+        pp = np.ascontiguousarray(p, dtype=np.float64)
+        self._check(self._hip.comd_upload(self.ctx, _ptr(na, C.c_int), _ptr(i, C.c_int), _ptr(t, C.c_int),
+                                          _ptr(rr), _ptr(pp)))
+
+    def download(self):
+        """Slot-layout state: natoms (C), id, itype (S), r, p (S,3), f (S,4), rho, fi (S)."""
+        S, Cn = self.nslots, self.ncells
+        out = dict(natoms=np.zeros(Cn, np.int32), id=np.zeros(S, np.int32), itype=np.zeros(S, np.int32),
+                   r=np.zeros((S, 3)), p=np.zeros((S, 3)), f=np.zeros((S, 4)), rho=np.zeros(S), fi=np.zeros(S))
+        self._check(self._hip.comd_download(
+            self.ctx, _ptr(out["natoms"], C.c_int), _ptr(out["id"], C.c_int), _ptr(out["itype"], C.c_int),
+            _ptr(out["r"]), _ptr(out["p"]), _ptr(out["f"]), _ptr(out["rho"]), _ptr(out["fi"])))
+        return out
+
+    def rebox(self):
+        self._check(self._hip.comd_rebox(self.ctx))
+
+    def force(self):
+        self._check(self._hip.comd_force(self.ctx))
+
+    def nsteps(self, n, dt=1.0e-15):
+        pe = C.c_double()
+        self._check(self._hip.comd_nsteps(self.ctx, int(n), float(dt), C.byref(pe)))
+        return pe.value
+
+    def advance_positions(self, dt_internal):
+        self._check(self._hip.comd_advance_positions(self.ctx, float(dt_internal)))
+
+    def advance_velocity(self, dt_internal):
+        self._check(self._hip.comd_advance_velocity(self.ctx, float(dt_internal)))
+
+    def energy(self):
+        """(potential energy, kinetic energy, virial 'stress') in hartree."""
+        pe, ke, st = C.c_double(), C.c_double(), C.c_double()
+        self._check(self._hip.comd_energy(self.ctx, C.byref(pe), C.byref(ke), C.byref(st)))
+        return pe.value, ke.value, st.value
+
+    def set_precision(self, precision):
+        self._check(self._hip.comd_set_precision(self.ctx, PREC_MIXED if precision == "mixed" else PREC_FP64))
+
+    def set_eval(self, evaluation):
+        self._check(self._hip.comd_set_eval(self.ctx, EVAL_CHEB if evaluation == "cheby" else EVAL_TABLE))
+
+    def sync(self):
+        self._check(self._hip.comd_sync(self.ctx))
+
+    def timing(self, on=True):
+        self._check(self._hip.comd_timing(self.ctx, 1 if on else 0))
+
+    def timing_read(self):
+        ms = (C.c_double * K_NCLASS)()
+        ln = (C.c_int * K_NCLASS)()
+        self._check(self._hip.comd_timing_read(self.ctx, ms, ln))
+        return list(ms), list(ln)
+
+
+def fcc_simulation(pot, n, lat=None, temp=0.0, disp=0.0, seed=1, precision="fp64", evaluation="table",
+                   defgrad=1.0, bf=1.0, periodic=True):
+    """FCC n^3 crystal binned on the GPU in create_fcc_lattice order, with
+    optional seeded displacements and momenta.  Returns (sim, xyz, p)."""
+    lat = lat if lat is not None else pot.lat
+    xyz = fcc_positions(n, n, n, lat, defgrad)
+    p = np.zeros_like(xyz)
+    if temp > 0 or disp > 0:
+        xyz, p = seed_state(xyz, pot.mass, temp, disp, seed)
+    bounds = (n * lat * defgrad, n * lat, n * lat)
+    sim = Simulation(pot, bounds, defgrad=defgrad, bf=bf, periodic=periodic, precision=precision,
+                     evaluation=evaluation)
+    sim.bin_atoms(xyz, p, itype=1)
+    return sim, xyz, p

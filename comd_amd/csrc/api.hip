@@ -1,0 +1,676 @@
+// api.hip -- context management and the C-ABI entry points of
+// include/comd_hip.h.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "comd_internal.h"
+
+// src-flat/include/constants.h:61-65
+static const double kBohrPerAtuToAPerS = (double)0.529 / (double)2.418e-17;
+static const double kAmuToMe = (double)1822.83;
+
+static char g_create_err[512] = "";
+
+struct TimedEvent {
+    int cls;
+    hipEvent_t a, b;
+};
+
+struct comd_ctx {
+    comd_params prm;
+    int ncells;
+    long nslots;
+    int ntot;
+    int cur;               // which of the two moving-state buffers is current
+    int have_atoms;
+    int device;
+    hipStream_t stream;
+    // moving state, two buffers
+    int *natoms[2], *id[2], *itype[2];
+    double *rx[2], *ry[2], *rz[2], *px[2], *py[2], *pz[2];
+    // outputs
+    double *fx, *fy, *fz, *fe, *rho, *fi, *vir;
+    // scratch
+    int *dest, *rank, *meta;
+    double *rnx, *rny, *rnz;
+    uint8_t *perm;
+    unsigned *flags;
+    double *partials;
+    int nparts;
+    double *red_out;       // 3 doubles, device
+    double *host_out;      // pinned 4 doubles
+    unsigned *host_flags;  // pinned
+    // potential
+    PotDev pot;
+    double *d_tab[3];
+    float *d_coef32[2];
+    double *d_cheb;
+    float *d_cheb32;
+    int force_blocks;
+    // timing
+    int timing;
+    std::vector<TimedEvent> events;
+    double ms[COMD_K_NCLASS];
+    int launches[COMD_K_NCLASS];
+    std::string err;
+};
+
+static int set_err(comd_ctx *ctx, int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (ctx) ctx->err = buf;
+    else snprintf(g_create_err, sizeof g_create_err, "%s", buf);
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                                  \
+    do {                                                                                                   \
+        hipError_t e_ = (call);                                                                            \
+        if (e_ != hipSuccess) return set_err((ctx), COMD_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+static DevState dev_state(const comd_ctx *c)
+{
+    DevState d;
+    const int a = c->cur, b = 1 - c->cur;
+    d.nbx0 = c->prm.nbx[0];
+    d.nbx1 = c->prm.nbx[1];
+    d.nbx2 = c->prm.nbx[2];
+    d.ncells = c->ncells;
+    d.periodic = c->prm.periodic;
+    d.bs0 = c->prm.boxsize[0];
+    d.bs1 = c->prm.boxsize[1];
+    d.bs2 = c->prm.boxsize[2];
+    d.L0 = c->prm.bounds[0];
+    d.L1 = c->prm.bounds[1];
+    d.L2 = c->prm.bounds[2];
+    d.natoms = c->natoms[a];
+    d.natoms_n = c->natoms[b];
+    d.id = c->id[a];
+    d.id_n = c->id[b];
+    d.itype = c->itype[a];
+    d.itype_n = c->itype[b];
+    d.rx = c->rx[a]; d.ry = c->ry[a]; d.rz = c->rz[a];
+    d.rx_n = c->rx[b]; d.ry_n = c->ry[b]; d.rz_n = c->rz[b];
+    d.px = c->px[a]; d.py = c->py[a]; d.pz = c->pz[a];
+    d.px_n = c->px[b]; d.py_n = c->py[b]; d.pz_n = c->pz[b];
+    d.fx = c->fx; d.fy = c->fy; d.fz = c->fz; d.fe = c->fe;
+    d.rho = c->rho; d.fi = c->fi; d.vir = c->vir;
+    d.dest = c->dest;
+    d.rank = c->rank;
+    d.rnx = c->rnx; d.rny = c->rny; d.rnz = c->rnz;
+    d.perm = c->perm;
+    d.meta = c->meta;
+    d.flags = c->flags;
+    return d;
+}
+
+// ---- timing helpers
+static void tbegin(comd_ctx *c, int cls, hipEvent_t *a)
+{
+    if (!c->timing) return;
+    hipEventCreate(a);
+    hipEventRecord(*a, c->stream);
+    (void)cls;
+}
+static void tend(comd_ctx *c, int cls, hipEvent_t a)
+{
+    if (!c->timing) return;
+    hipEvent_t b;
+    hipEventCreate(&b);
+    hipEventRecord(b, c->stream);
+    c->events.push_back({cls, a, b});
+}
+
+// ---- library operations on the device (stream-ordered, no host sync)
+
+static void op_rebox(comd_ctx *c)
+{
+    hipEvent_t ev = nullptr;
+    tbegin(c, COMD_K_REBIN, &ev);
+    DevState d = dev_state(c);
+    cm_launch_rebox(d, c->stream);
+    tend(c, COMD_K_REBIN, ev);
+    c->cur = 1 - c->cur;
+}
+
+static void op_move(comd_ctx *c, double dtk, double dtd, int kick, int drift)
+{
+    hipEvent_t ev = nullptr;
+    tbegin(c, COMD_K_REBIN, &ev);
+    DevState d = dev_state(c);
+    cm_launch_move(d, c->stream, dtk, dtd, kAmuToMe * c->prm.mass, kick, drift);
+    tend(c, COMD_K_REBIN, ev);
+}
+
+static void op_force(comd_ctx *c)
+{
+    DevState d = dev_state(c);
+    const int mixed = c->prm.precision == COMD_PREC_MIXED;
+    c->pot.eval = c->prm.eval;
+    if (c->prm.potential == COMD_POT_LJ) {
+        hipEvent_t ev = nullptr;
+        tbegin(c, COMD_K_DENSITY, &ev);
+        cm_launch_force(d, c->pot, CM_KIND_LJ, mixed, c->force_blocks, c->stream);
+        tend(c, COMD_K_DENSITY, ev);
+    } else {
+        hipEvent_t ev = nullptr;
+        tbegin(c, COMD_K_DENSITY, &ev);
+        cm_launch_force(d, c->pot, CM_KIND_EAM1, mixed, c->force_blocks, c->stream);
+        tend(c, COMD_K_DENSITY, ev);
+        tbegin(c, COMD_K_EMBED_FORCE, &ev);
+        cm_launch_force(d, c->pot, CM_KIND_EAM3, mixed, c->force_blocks, c->stream);
+        tend(c, COMD_K_EMBED_FORCE, ev);
+    }
+}
+
+static int check_flags(comd_ctx *c)
+{
+    HIPCHK(c, hipMemcpyAsync(c->host_flags, c->flags, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const unsigned f = *c->host_flags;
+    if (f & CM_FLAG_OVERFLOW) return set_err(c, COMD_ERR_OVERFLOW, "a link cell reached %d atoms", CM_SLOTS);
+    if (f & CM_FLAG_OUTSIDE) return set_err(c, COMD_ERR_OUTSIDE, "an atom fell outside the cell grid");
+    return COMD_OK;
+}
+
+// ---- C-ABI
+
+extern "C" {
+
+int comd_grid_for(comd_params *p, const double bounds[3], double defgrad, double bf)
+{
+    if (!p || !bounds || !(p->cutoff > 0.0)) return COMD_ERR_ARG;
+    double strain[3] = {defgrad, 1.0, 1.0};
+    if (strain[0] == 0.0) strain[0] = 1.0;
+    const double boxfactor = bf == 0.0 ? 1.0 : bf;
+    for (int j = 0; j < 3; j++) {
+        p->bounds[j] = bounds[j];
+        p->nbx[j] = (int)floor(bounds[j] / (p->cutoff * boxfactor * strain[j]));
+        if (p->nbx[j] < 1) return COMD_ERR_GRID;
+        p->boxsize[j] = bounds[j] / (double)p->nbx[j];
+    }
+    return COMD_OK;
+}
+
+static void free_all(comd_ctx *c)
+{
+    void *ptrs[] = {c->fx, c->fy, c->fz, c->fe, c->rho, c->fi, c->vir, c->dest, c->rank, c->meta,
+                    c->rnx, c->rny, c->rnz, c->perm, c->flags, c->partials, c->red_out,
+                    c->d_tab[0], c->d_tab[1], c->d_tab[2], c->d_coef32[0], c->d_coef32[1], c->d_cheb, c->d_cheb32};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+    for (int b = 0; b < 2; b++) {
+        void *q[] = {c->natoms[b], c->id[b], c->itype[b], c->rx[b], c->ry[b], c->rz[b], c->px[b], c->py[b], c->pz[b]};
+        for (void *p : q)
+            if (p) hipFree(p);
+    }
+    if (c->host_out) hipHostFree(c->host_out);
+    if (c->host_flags) hipHostFree(c->host_flags);
+    for (auto &e : c->events) {
+        hipEventDestroy(e.a);
+        hipEventDestroy(e.b);
+    }
+    if (c->stream) hipStreamDestroy(c->stream);
+}
+
+// Per-interval coefficients of the table formula (src-flat/eam.c:418-433),
+// evaluated in double precision and stored as float for the mixed path.
+static std::vector<float> table_coefs32(int n, double invdx, const double *vals /* values[-1] first */)
+{
+    std::vector<float> out((size_t)8 * n, 0.0f);
+    const double *v = vals + 1;
+    for (int i = 0; i < n; i++) {
+        const double vm = v[i - 1], v0 = v[i], v1 = v[i + 1], v2 = v[i + 2];
+        const double g0 = v1 - vm, g1 = v2 - v0;
+        const double A = v1 + vm - 2.0 * v0;
+        out[8 * i + 0] = (float)v0;
+        out[8 * i + 1] = (float)(0.5 * g0);
+        out[8 * i + 2] = (float)(0.5 * A);
+        out[8 * i + 3] = i <= 0 ? 0.0f : (float)(0.5 * g0 * invdx);
+        out[8 * i + 4] = i <= 0 ? 0.0f : (float)(0.5 * (g1 - g0) * invdx);
+    }
+    return out;
+}
+
+int comd_create(const comd_params *p, comd_ctx **out)
+{
+    if (!p || !out) return set_err(nullptr, COMD_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (p->potential != COMD_POT_EAM && p->potential != COMD_POT_LJ)
+        return set_err(nullptr, COMD_ERR_ARG, "unknown potential %d", p->potential);
+    for (int j = 0; j < 3; j++)
+        if (p->nbx[j] < 1 || !(p->boxsize[j] > 0) || !(p->bounds[j] > 0))
+            return set_err(nullptr, COMD_ERR_GRID, "bad grid in dimension %d", j);
+    if (p->potential == COMD_POT_EAM) {
+        for (int k = 0; k < 3; k++)
+            if (p->tab_n[k] < 2 || !p->tab_values[k]) return set_err(nullptr, COMD_ERR_ARG, "EAM table %d missing", k);
+        if (p->eval == COMD_EVAL_CHEB && (p->cheb_n < 2 || p->cheb_n > 64 || !p->cheb_coef))
+            return set_err(nullptr, COMD_ERR_ARG, "Chebyshev coefficients missing");
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_err(nullptr, COMD_ERR_NODEVICE, "no HIP device");
+
+    comd_ctx *c = new comd_ctx();
+    c->prm = *p;
+    for (int k = 0; k < 3; k++) c->prm.tab_values[k] = nullptr;
+    c->prm.cheb_coef = nullptr;
+    const long ncl = (long)p->nbx[0] * p->nbx[1] * p->nbx[2];
+    if (ncl > (1L << 26)) {
+        delete c;
+        return set_err(nullptr, COMD_ERR_ARG, "too many cells");
+    }
+    c->ncells = (int)ncl;
+    c->nslots = ncl * CM_SLOTS;
+    hipGetDevice(&c->device);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return set_err(nullptr, COMD_ERR_HIP, "stream creation failed");
+    }
+    bool ok = true;
+    auto alloc = [&](void **ptr, size_t bytes) {
+        if (!ok) return;
+        if (hipMalloc(ptr, bytes < 16 ? 16 : bytes) != hipSuccess) ok = false;
+        else hipMemsetAsync(*ptr, 0, bytes < 16 ? 16 : bytes, c->stream);
+    };
+    const size_t S = (size_t)c->nslots, C = (size_t)c->ncells;
+    for (int b = 0; b < 2; b++) {
+        alloc((void **)&c->natoms[b], C * 4);
+        alloc((void **)&c->id[b], S * 4);
+        alloc((void **)&c->itype[b], S * 4);
+        alloc((void **)&c->rx[b], S * 8);
+        alloc((void **)&c->ry[b], S * 8);
+        alloc((void **)&c->rz[b], S * 8);
+        alloc((void **)&c->px[b], S * 8);
+        alloc((void **)&c->py[b], S * 8);
+        alloc((void **)&c->pz[b], S * 8);
+    }
+    alloc((void **)&c->fx, S * 8);
+    alloc((void **)&c->fy, S * 8);
+    alloc((void **)&c->fz, S * 8);
+    alloc((void **)&c->fe, S * 8);
+    alloc((void **)&c->rho, S * 8);
+    alloc((void **)&c->fi, S * 8);
+    alloc((void **)&c->vir, S * 8);
+    alloc((void **)&c->dest, S * 4);
+    alloc((void **)&c->rank, S * 4);
+    alloc((void **)&c->meta, C * 16);
+    alloc((void **)&c->rnx, S * 8);
+    alloc((void **)&c->rny, S * 8);
+    alloc((void **)&c->rnz, S * 8);
+    alloc((void **)&c->perm, S);
+    alloc((void **)&c->flags, 16);
+    c->nparts = 1024;
+    alloc((void **)&c->partials, (size_t)c->nparts * 3 * 8);
+    alloc((void **)&c->red_out, 4 * 8);
+    if (ok && hipHostMalloc((void **)&c->host_out, 4 * 8, 0) != hipSuccess) ok = false;
+    if (ok && hipHostMalloc((void **)&c->host_flags, 16, 0) != hipSuccess) ok = false;
+
+    // potential
+    PotDev &P = c->pot;
+    memset(&P, 0, sizeof P);
+    P.kind = p->potential;
+    P.eval = p->eval;
+    P.rc2 = p->cutoff * p->cutoff;
+    P.mass_amu = p->mass;
+    if (p->potential == COMD_POT_LJ) {
+        const double sg = p->lj_sigma;
+        P.lj_s6 = sg * sg * sg * sg * sg * sg;  // src-flat/ljForce.c:143
+        P.lj_eps = p->lj_epsilon;
+    } else if (ok) {
+        for (int k = 0; k < 3; k++) {
+            const int n = p->tab_n[k];
+            alloc((void **)&c->d_tab[k], (size_t)(n + 3) * 8);
+            if (!ok) break;
+            hipMemcpyAsync(c->d_tab[k], p->tab_values[k], (size_t)(n + 3) * 8, hipMemcpyHostToDevice, c->stream);
+            P.tab[k] = c->d_tab[k];
+            P.x0[k] = p->tab_x0[k];
+            P.xn[k] = p->tab_xn[k];
+            P.invdx[k] = p->tab_invdx[k];
+            P.n[k] = n;
+        }
+        for (int k = 0; ok && k < 2; k++) {
+            std::vector<float> cf = table_coefs32(p->tab_n[k], p->tab_invdx[k], p->tab_values[k]);
+            alloc((void **)&c->d_coef32[k], cf.size() * 4);
+            if (!ok) break;
+            hipMemcpyAsync(c->d_coef32[k], cf.data(), cf.size() * 4, hipMemcpyHostToDevice, c->stream);
+            hipStreamSynchronize(c->stream);
+            P.coef32[k] = c->d_coef32[k];
+            P.x0f[k] = (float)p->tab_x0[k];
+            P.xnf[k] = (float)p->tab_xn[k];
+            P.invdxf[k] = (float)p->tab_invdx[k];
+        }
+        if (ok && p->cheb_n >= 2 && p->cheb_coef) {
+            const int m = p->cheb_n;
+            alloc((void **)&c->d_cheb, (size_t)6 * m * 8);
+            alloc((void **)&c->d_cheb32, (size_t)6 * m * 4);
+            if (ok) {
+                std::vector<float> c32(6 * m);
+                for (int i = 0; i < 6 * m; i++) c32[i] = (float)p->cheb_coef[i];
+                hipMemcpyAsync(c->d_cheb, p->cheb_coef, (size_t)6 * m * 8, hipMemcpyHostToDevice, c->stream);
+                hipMemcpyAsync(c->d_cheb32, c32.data(), (size_t)6 * m * 4, hipMemcpyHostToDevice, c->stream);
+                hipStreamSynchronize(c->stream);
+                P.cheb = c->d_cheb;
+                P.cheb32 = c->d_cheb32;
+                P.cn = m;
+                for (int k = 0; k < 3; k++) {
+                    P.ca[k] = p->cheb_x0[k];
+                    P.cb[k] = p->cheb_xn[k];
+                }
+            }
+        }
+    }
+    if (!ok) {
+        free_all(c);
+        delete c;
+        return set_err(nullptr, COMD_ERR_NOMEM, "device allocation failed");
+    }
+    hipDeviceProp_t prop;
+    int ncu = 256;
+    if (hipGetDeviceProperties(&prop, c->device) == hipSuccess && prop.multiProcessorCount > 0) ncu = prop.multiProcessorCount;
+    c->force_blocks = ncu * 16;
+    if (hipStreamSynchronize(c->stream) != hipSuccess) {
+        free_all(c);
+        delete c;
+        return set_err(nullptr, COMD_ERR_HIP, "initialisation failed");
+    }
+    *out = c;
+    return COMD_OK;
+}
+
+int comd_destroy(comd_ctx *ctx)
+{
+    if (!ctx) return COMD_OK;
+    hipStreamSynchronize(ctx->stream);
+    free_all(ctx);
+    delete ctx;
+    return COMD_OK;
+}
+
+const char *comd_last_error(const comd_ctx *ctx)
+{
+    return ctx ? ctx->err.c_str() : g_create_err;
+}
+
+int comd_sizes(const comd_ctx *ctx, int *ncells, int *natoms_total)
+{
+    if (!ctx) return COMD_ERR_ARG;
+    if (ncells) *ncells = ctx->ncells;
+    if (natoms_total) *natoms_total = ctx->ntot;
+    return COMD_OK;
+}
+
+int comd_upload(comd_ctx *c, const int *natoms, const int *id, const int *itype, const double *r, const double *p)
+{
+    if (!c || !natoms || !id || !r) return COMD_ERR_ARG;
+    const size_t S = (size_t)c->nslots, C = (size_t)c->ncells;
+    int tot = 0;
+    for (size_t b = 0; b < C; b++) {
+        if (natoms[b] < 0 || natoms[b] > CM_SLOTS) return set_err(c, COMD_ERR_ARG, "cell %zu holds %d atoms", b, natoms[b]);
+        tot += natoms[b];
+    }
+    std::vector<double> buf(S);
+    std::vector<int> it(S, 0);
+    const int a = c->cur;
+    HIPCHK(c, hipMemcpyAsync(c->natoms[a], natoms, C * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->id[a], id, S * 4, hipMemcpyHostToDevice, c->stream));
+    if (itype) memcpy(it.data(), itype, S * 4);
+    HIPCHK(c, hipMemcpyAsync(c->itype[a], it.data(), S * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double *dst[6] = {c->rx[a], c->ry[a], c->rz[a], c->px[a], c->py[a], c->pz[a]};
+    for (int k = 0; k < 6; k++) {
+        const double *src = k < 3 ? r : p;
+        if (!src) {
+            std::fill(buf.begin(), buf.end(), 0.0);
+        } else {
+            for (size_t s = 0; s < S; s++) buf[s] = src[3 * s + (k % 3)];
+        }
+        HIPCHK(c, hipMemcpyAsync(dst[k], buf.data(), S * 8, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    c->ntot = tot;
+    c->have_atoms = 1;
+    return COMD_OK;
+}
+
+int comd_download(comd_ctx *c, int *natoms, int *id, int *itype, double *r, double *p, double *f, double *rho, double *fi)
+{
+    if (!c) return COMD_ERR_ARG;
+    const size_t S = (size_t)c->nslots, C = (size_t)c->ncells;
+    const int a = c->cur;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (natoms) HIPCHK(c, hipMemcpy(natoms, c->natoms[a], C * 4, hipMemcpyDeviceToHost));
+    if (id) HIPCHK(c, hipMemcpy(id, c->id[a], S * 4, hipMemcpyDeviceToHost));
+    if (itype) HIPCHK(c, hipMemcpy(itype, c->itype[a], S * 4, hipMemcpyDeviceToHost));
+    std::vector<double> buf(S);
+    auto gather = [&](double *dst, int stride, int k, const double *src) -> int {
+        if (hipMemcpy(buf.data(), src, S * 8, hipMemcpyDeviceToHost) != hipSuccess) return COMD_ERR_HIP;
+        for (size_t s = 0; s < S; s++) dst[stride * s + k] = buf[s];
+        return COMD_OK;
+    };
+    int rc = COMD_OK;
+    if (r) {
+        rc |= gather(r, 3, 0, c->rx[a]);
+        rc |= gather(r, 3, 1, c->ry[a]);
+        rc |= gather(r, 3, 2, c->rz[a]);
+    }
+    if (p) {
+        rc |= gather(p, 3, 0, c->px[a]);
+        rc |= gather(p, 3, 1, c->py[a]);
+        rc |= gather(p, 3, 2, c->pz[a]);
+    }
+    if (f) {
+        rc |= gather(f, 4, 0, c->fx);
+        rc |= gather(f, 4, 1, c->fy);
+        rc |= gather(f, 4, 2, c->fz);
+        rc |= gather(f, 4, 3, c->fe);
+    }
+    if (rho) HIPCHK(c, hipMemcpy(rho, c->rho, S * 8, hipMemcpyDeviceToHost));
+    if (fi) HIPCHK(c, hipMemcpy(fi, c->fi, S * 8, hipMemcpyDeviceToHost));
+    if (rc) return set_err(c, COMD_ERR_HIP, "download failed");
+    return COMD_OK;
+}
+
+int comd_bin_atoms_device(comd_ctx *c, int n, const double *d_xyz, const double *d_pxyz, int itype)
+{
+    if (!c || n < 0 || (n > 0 && !d_xyz)) return COMD_ERR_ARG;
+    if ((long)n > c->nslots) return set_err(c, COMD_ERR_OVERFLOW, "more atoms than slots");
+    HIPCHK(c, hipMemsetAsync(c->flags, 0, 4, c->stream));
+    DevState d = dev_state(c);
+    cm_launch_bin(d, c->stream, n, d_xyz, d_pxyz, itype, c->meta, c->rank);
+    int rc = check_flags(c);
+    if (rc) return rc;
+    c->ntot = n;
+    c->have_atoms = 1;
+    return COMD_OK;
+}
+
+int comd_bin_atoms(comd_ctx *c, int n, const double *xyz, const double *pxyz, int itype)
+{
+    if (!c || n < 0 || (n > 0 && !xyz)) return COMD_ERR_ARG;
+    double *dx = nullptr, *dp = nullptr;
+    const size_t bytes = (size_t)(n > 0 ? n : 1) * 3 * 8;
+    if (hipMalloc((void **)&dx, bytes) != hipSuccess) return set_err(c, COMD_ERR_NOMEM, "no memory for input");
+    if (pxyz && hipMalloc((void **)&dp, bytes) != hipSuccess) {
+        hipFree(dx);
+        return set_err(c, COMD_ERR_NOMEM, "no memory for input");
+    }
+    int rc = COMD_OK;
+    if (n > 0) {
+        if (hipMemcpy(dx, xyz, (size_t)n * 24, hipMemcpyHostToDevice) != hipSuccess ||
+            (dp && hipMemcpy(dp, pxyz, (size_t)n * 24, hipMemcpyHostToDevice) != hipSuccess))
+            rc = set_err(c, COMD_ERR_HIP, "input copy failed");
+    }
+    if (!rc) rc = comd_bin_atoms_device(c, n, dx, dp, itype);
+    hipStreamSynchronize(c->stream);
+    hipFree(dx);
+    if (dp) hipFree(dp);
+    return rc;
+}
+
+int comd_rebox(comd_ctx *c)
+{
+    if (!c) return COMD_ERR_ARG;
+    if (!c->have_atoms) return set_err(c, COMD_ERR_STATE, "no atoms");
+    HIPCHK(c, hipMemsetAsync(c->flags, 0, 4, c->stream));
+    op_move(c, 0.0, 0.0, 0, 0);
+    op_rebox(c);
+    return check_flags(c);
+}
+
+int comd_force(comd_ctx *c)
+{
+    if (!c) return COMD_ERR_ARG;
+    if (!c->have_atoms) return set_err(c, COMD_ERR_STATE, "no atoms");
+    op_force(c);
+    HIPCHK(c, hipGetLastError());
+    return COMD_OK;
+}
+
+int comd_energy(comd_ctx *c, double *pe, double *ke, double *stress)
+{
+    if (!c) return COMD_ERR_ARG;
+    if (!c->have_atoms) return set_err(c, COMD_ERR_STATE, "no atoms");
+    hipEvent_t ev = nullptr;
+    tbegin(c, COMD_K_REDUCE, &ev);
+    DevState d = dev_state(c);
+    cm_launch_reduce(d, c->pot, c->partials, c->nparts, c->stream);
+    cm_launch_reduce_final(c->partials, c->nparts, c->red_out, c->stream);
+    tend(c, COMD_K_REDUCE, ev);
+    HIPCHK(c, hipMemcpyAsync(c->host_out, c->red_out, 3 * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double e = c->host_out[0];
+    if (c->prm.potential == COMD_POT_LJ) e = e * 4.0 * c->prm.lj_epsilon * c->pot.lj_s6;  // ljForce.c:217
+    if (pe) *pe = e;
+    if (ke) *ke = c->host_out[1];
+    if (stress) *stress = c->host_out[2] / (c->prm.bounds[0] * c->prm.bounds[1] * c->prm.bounds[2]);
+    return COMD_OK;
+}
+
+int comd_nsteps(comd_ctx *c, int n, double dt_seconds, double *pe_out)
+{
+    if (!c || n < 0) return COMD_ERR_ARG;
+    if (!c->have_atoms) return set_err(c, COMD_ERR_STATE, "no atoms");
+    HIPCHK(c, hipMemsetAsync(c->flags, 0, 4, c->stream));
+    const double dt = dt_seconds * kBohrPerAtuToAPerS;  // timestep.c:126
+    for (int i = 0; i < n; i++) {
+        op_move(c, 0.0, dt / 2.0, 0, 1);
+        op_rebox(c);
+        op_force(c);
+        op_move(c, dt, dt / 2.0, 1, 1);
+        op_rebox(c);
+    }
+    op_force(c);  // timestep.c:136
+    HIPCHK(c, hipGetLastError());
+    int rc = check_flags(c);
+    if (rc) return rc;
+    if (pe_out) return comd_energy(c, pe_out, nullptr, nullptr);
+    return COMD_OK;
+}
+
+int comd_advance_positions(comd_ctx *c, double dt_internal)
+{
+    if (!c) return COMD_ERR_ARG;
+    if (!c->have_atoms) return set_err(c, COMD_ERR_STATE, "no atoms");
+    HIPCHK(c, hipMemsetAsync(c->flags, 0, 4, c->stream));
+    op_move(c, 0.0, dt_internal, 0, 1);
+    op_rebox(c);
+    return check_flags(c);
+}
+
+int comd_advance_velocity(comd_ctx *c, double dt_internal)
+{
+    if (!c) return COMD_ERR_ARG;
+    if (!c->have_atoms) return set_err(c, COMD_ERR_STATE, "no atoms");
+    // kick only: k_move with drift off leaves r untouched; its destination
+    // scratch is rewritten before the next rebin
+    op_move(c, dt_internal, 0.0, 1, 0);
+    HIPCHK(c, hipGetLastError());
+    return COMD_OK;
+}
+
+int comd_set_precision(comd_ctx *c, int precision)
+{
+    if (!c || (precision != COMD_PREC_FP64 && precision != COMD_PREC_MIXED)) return COMD_ERR_ARG;
+    c->prm.precision = precision;
+    return COMD_OK;
+}
+
+int comd_set_eval(comd_ctx *c, int eval)
+{
+    if (!c || (eval != COMD_EVAL_TABLE && eval != COMD_EVAL_CHEB)) return COMD_ERR_ARG;
+    if (eval == COMD_EVAL_CHEB && !c->pot.cheb) return set_err(c, COMD_ERR_ARG, "no Chebyshev coefficients");
+    c->prm.eval = eval;
+    return COMD_OK;
+}
+
+int comd_device_info(int *count, char *name, int name_len)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    if (count) *count = n;
+    if (name && name_len > 0) {
+        name[0] = 0;
+        int dev = 0;
+        hipDeviceProp_t prop;
+        if (n > 0 && hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+            snprintf(name, (size_t)name_len, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+    }
+    return n > 0 ? COMD_OK : COMD_ERR_NODEVICE;
+}
+
+int comd_sync(comd_ctx *c)
+{
+    if (!c) return COMD_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return COMD_OK;
+}
+
+int comd_timing(comd_ctx *c, int on)
+{
+    if (!c) return COMD_ERR_ARG;
+    hipStreamSynchronize(c->stream);
+    for (auto &e : c->events) {
+        hipEventDestroy(e.a);
+        hipEventDestroy(e.b);
+    }
+    c->events.clear();
+    for (int k = 0; k < COMD_K_NCLASS; k++) {
+        c->ms[k] = 0;
+        c->launches[k] = 0;
+    }
+    c->timing = on ? 1 : 0;
+    return COMD_OK;
+}
+
+int comd_timing_read(comd_ctx *c, double *ms, int *launches)
+{
+    if (!c) return COMD_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (auto &e : c->events) {
+        float t = 0;
+        hipEventElapsedTime(&t, e.a, e.b);
+        c->ms[e.cls] += t;
+        c->launches[e.cls] += 1;
+        hipEventDestroy(e.a);
+        hipEventDestroy(e.b);
+    }
+    c->events.clear();
+    for (int k = 0; k < COMD_K_NCLASS; k++) {
+        if (ms) ms[k] = c->ms[k];
+        if (launches) launches[k] = c->launches[k];
+    }
+    return COMD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,92 @@
+// comd_common.h -- device-state layout and shared helpers of the gfx950
+// link-cell library (internal to comd_amd/csrc).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/comd_hip.h"
+
+#define CM_SLOTS 64  // slots per link cell (src-flat/include/pmdTypes.h:57)
+
+// Error bits raised by kernels into the context's device flag word.
+enum {
+    CM_FLAG_OVERFLOW = 1,  // a cell would reach 64 atoms (src-flat/domains.c:229)
+    CM_FLAG_OUTSIDE = 2,   // atom outside the grid when binned
+    CM_FLAG_MULTIHOP = 4,  // an arrival would move again in the same rebin pass
+};
+
+// Plain view of the device arrays passed to kernels by value.
+struct DevState {
+    int nbx0, nbx1, nbx2, ncells;
+    int periodic;
+    double bs0, bs1, bs2;      // boxsize
+    double L0, L1, L2;         // bounds
+    // current (cur) and next buffers of the moving state
+    int *natoms, *natoms_n;
+    int *id, *id_n;
+    int *itype, *itype_n;
+    double *rx, *ry, *rz, *rx_n, *ry_n, *rz_n;
+    double *px, *py, *pz, *px_n, *py_n, *pz_n;
+    // force outputs (slot layout)
+    double *fx, *fy, *fz, *fe, *rho, *fi, *vir;
+    // rebin scratch
+    int *dest;          // per slot: destination cell, or -1 to stay
+    int *rank;          // per slot: departure rank within its source cell
+    double *rnx, *rny, *rnz;  // per slot: shifted coordinates of a moving atom
+    uint8_t *perm;      // per cell: survivor arrangement after the scan
+    int *meta;          // per cell: (len, nlow, nhigh, n0)
+    unsigned *flags;    // error flag word
+};
+
+__device__ __forceinline__ void cell_xyz(const DevState &d, int c, int &ix, int &iy, int &iz)
+{
+    ix = c % d.nbx0;
+    iy = (c / d.nbx0) % d.nbx1;
+    iz = (c / d.nbx0 / d.nbx1) % d.nbx2;
+}
+
+// Periodic wrap of a cell triple, or -1 outside a non-periodic box
+// (src-flat/domains.c:297-325).
+__device__ __forceinline__ int cell_wrap(const DevState &d, int ix, int iy, int iz)
+{
+    if (!d.periodic) {
+        if (ix < 0 || ix >= d.nbx0 || iy < 0 || iy >= d.nbx1 || iz < 0 || iz >= d.nbx2) return -1;
+    }
+    if (ix < 0) ix += d.nbx0; else if (ix >= d.nbx0) ix -= d.nbx0;
+    if (iy < 0) iy += d.nbx1; else if (iy >= d.nbx1) iy -= d.nbx1;
+    if (iz < 0) iz += d.nbx2; else if (iz >= d.nbx2) iz -= d.nbx2;
+    return ix + iy * d.nbx0 + iz * d.nbx0 * d.nbx1;
+}
+
+// Stencil entry e (0..26) of cell c in the reference's order: x outer, y,
+// z inner (src-flat/domains.c:419-426); -1 outside a non-periodic box.
+__device__ __forceinline__ int stencil_cell(const DevState &d, int c, int e)
+{
+    if (d.ncells == 1) return e == 0 ? 0 : -1;
+    int ix, iy, iz;
+    cell_xyz(d, c, ix, iy, iz);
+    int ox = e / 9 - 1, oy = (e / 3) % 3 - 1, oz = e % 3 - 1;
+    return cell_wrap(d, ix + ox, iy + oy, iz + oz);
+}
+
+// dcenter[c][k] = boxsize[k] * ix_k (src-flat/domains.c:191-197).
+__device__ __forceinline__ void cell_corner(const DevState &d, int c, double &x, double &y, double &z)
+{
+    int ix, iy, iz;
+    cell_xyz(d, c, ix, iy, iz);
+    x = d.bs0 * (double)ix;
+    y = d.bs1 * (double)iy;
+    z = d.bs2 * (double)iz;
+}
+
+// Minimum-imaged corner difference dcenter[i]-dcenter[j]
+// (src-flat/eam.c:495-502).
+__device__ __forceinline__ double image1(double dd, double L, int periodic)
+{
+    if (periodic) {
+        if (dd < -0.5 * L) dd += L;
+        else if (dd > 0.5 * L) dd -= L;
+    }
+    return dd;
+}

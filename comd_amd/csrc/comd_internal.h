@@ -1,0 +1,39 @@
+// comd_internal.h -- declarations shared by the translation units of the
+// gfx950 library (not part of the C-ABI).
+#pragma once
+
+#include "comd_common.h"
+
+// Potential description as the kernels see it.
+struct PotDev {
+    int kind;     // COMD_POT_EAM / COMD_POT_LJ
+    int eval;     // COMD_EVAL_TABLE / COMD_EVAL_CHEB
+    double rc2;   // cutoff squared
+    // tables phi, rho, F: values[-1..n+1] at tab[k] (tab[k][0] is values[-1])
+    const double *tab[3];
+    double x0[3], xn[3], invdx[3];
+    int n[3];
+    // per-interval single-precision coefficients of phi and rho for the
+    // mixed path: 8 floats per interval (c0 c1 c2 d0 d1 pad pad pad)
+    const float *coef32[2];
+    float x0f[2], xnf[2], invdxf[2];
+    // Chebyshev: 6 series of cn coefficients (phi rho F dphi drho dF)
+    const double *cheb;
+    const float *cheb32;
+    int cn;
+    double ca[3], cb[3];   // domains of phi, rho, F
+    // Lennard-Jones
+    double lj_s6, lj_eps;
+    double mass_amu;
+};
+
+enum { CM_KIND_EAM1 = 0, CM_KIND_EAM3 = 1, CM_KIND_LJ = 2 };
+
+void cm_launch_move(const DevState &d, hipStream_t st, double dtk, double dtd, double rmass, int kick, int drift);
+void cm_launch_rebox(const DevState &d, hipStream_t st);
+void cm_launch_bin(const DevState &d, hipStream_t st, int n, const double *xyz, const double *pxyz, int itype,
+                   int *cnt, int *tmp);
+void cm_launch_fill_int(int *p, int v, long n, hipStream_t st);
+void cm_launch_force(const DevState &d, const PotDev &pot, int kind, int mixed, int nblocks, hipStream_t st);
+void cm_launch_reduce(const DevState &d, const PotDev &pot, double *partials, int nparts, hipStream_t st);
+void cm_launch_reduce_final(const double *partials, int nparts, double *out, hipStream_t st);

@@ -1,0 +1,166 @@
+/*
+ * comd_hip.h -- C-ABI of the MI355X (gfx950) link-cell force and timestep
+ * library (comd_amd/libcomd_hip.so).
+ *
+ * The library replaces the hot path of CoMD src-flat: the potential's force
+ * slot `int (*force)(void *s)` (src-flat/include/pmdTypes.h:77, set at
+ * src-flat/eam.c:394 and src-flat/ljForce.c:79, called from computeForce,
+ * src-flat/timestep.c:110-112), the timestep driver `double nTimeSteps(int n,
+ * simflat_t *s, real_t dt)` (src-flat/timestep.c:114-140) and the rebinning
+ * `void reBoxAll(simflat_t *s)` (src-flat/domains.c:347-389).
+ *
+ * Only plain C types cross this boundary.  Host arrays use the reference's
+ * slot layout: slot i of link cell c is index c*64+i (arrayAtBox,
+ * src-flat/include/pmdTypes.h:100); r and p are 3 doubles per slot, f is 4
+ * doubles per slot (fx, fy, fz, per-atom energy).  Every entry point returns
+ * COMD_OK or a positive COMD_ERR_* code and never exits the process (the
+ * reference calls exit() through simulationAbort, src-flat/utility.c:96-109).
+ * A context owns one HIP stream; contexts are independent and reentrant.
+ */
+#ifndef COMD_HIP_H
+#define COMD_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define COMD_MAXATOMS 64 /* slots per link cell, src-flat/include/pmdTypes.h:57 */
+
+enum comd_status {
+    COMD_OK = 0,
+    COMD_ERR_ARG = 1,       /* bad argument or parameter */
+    COMD_ERR_HIP = 2,       /* a HIP runtime call failed */
+    COMD_ERR_NOMEM = 3,     /* device allocation failed */
+    COMD_ERR_OVERFLOW = 4,  /* a link cell reached 64 atoms (src-flat/domains.c:229) */
+    COMD_ERR_OUTSIDE = 5,   /* an atom fell outside the cell grid when binned */
+    COMD_ERR_GRID = 6,      /* fewer than one cutoff-wide cell (src-flat/domains.c:180) */
+    COMD_ERR_NODEVICE = 7,  /* no HIP device */
+    COMD_ERR_STATE = 8,     /* call out of order (e.g. no atoms uploaded) */
+};
+
+enum comd_potential { COMD_POT_EAM = 1, COMD_POT_LJ = 2 };
+enum comd_eval { COMD_EVAL_TABLE = 0, COMD_EVAL_CHEB = 1 };
+enum comd_precision { COMD_PREC_FP64 = 0, COMD_PREC_MIXED = 1 };
+
+/* Description of one simulation: potential and link-cell grid.
+ * The grid fields are the values allocDomains computes
+ * (src-flat/domains.c:155-200); comd_grid_for() fills them. */
+typedef struct comd_params {
+    int potential;          /* comd_potential */
+    int eval;               /* comd_eval (EAM only) */
+    int precision;          /* comd_precision */
+    int periodic;           /* the reference's global PERIODIC (mycommand.c:67) */
+    double cutoff;          /* A */
+    double mass;            /* amu */
+    /* Lennard-Jones (src-flat/ljForce.c:75-86) */
+    double lj_sigma, lj_epsilon;
+    /* EAM tables in the reference's potentialarray_t form
+     * (src-flat/include/eamTypes.h:68-74): index 0 phi, 1 rho, 2 F.
+     * x0 is the shifted origin (src-flat/eam.c:150); values points at n+3
+     * doubles starting with the guard slot values[-1]. */
+    int tab_n[3];
+    double tab_x0[3], tab_xn[3], tab_invdx[3];
+    const double *tab_values[3];
+    /* Chebyshev series (src-flat/cheby.c:99-121): cheb_n coefficients each
+     * for phi, rho, F, dphi, drho, dF (6*cheb_n doubles), with the domain
+     * [cheb_x0[k], cheb_xn[k]] of phi, rho, F. */
+    int cheb_n;
+    const double *cheb_coef;
+    double cheb_x0[3], cheb_xn[3];
+    /* link-cell grid */
+    int nbx[3];
+    double bounds[3];
+    double boxsize[3];
+} comd_params;
+
+typedef struct comd_ctx comd_ctx;
+
+/* Grid of allocDomains (src-flat/domains.c:155-200): nbx = floor(bounds /
+ * (cutoff*bf*strain)), strain = (defgrad,1,1), 0 read as 1 for defgrad and bf. */
+int comd_grid_for(comd_params *p, const double bounds[3], double defgrad, double bf);
+
+/* Create a context on the current HIP device; allocates device state for
+ * nbx[0]*nbx[1]*nbx[2]*64 slots. */
+int comd_create(const comd_params *p, comd_ctx **out);
+int comd_destroy(comd_ctx *ctx);
+
+/* Number of link cells and of atoms currently held. */
+int comd_sizes(const comd_ctx *ctx, int *ncells, int *natoms_total);
+
+/* Copy a slot-layout state to the device (the simflat_t arrays natoms, id,
+ * iType, r, p of src-flat/include/pmdTypes.h:104-141). */
+int comd_upload(comd_ctx *ctx, const int *natoms, const int *id, const int *itype,
+                const double *r, const double *p);
+
+/* Copy the device state back into slot-layout arrays; any pointer may be
+ * NULL.  f, rho and fi are those of the last force evaluation. */
+int comd_download(comd_ctx *ctx, int *natoms, int *id, int *itype, double *r, double *p,
+                  double *f, double *rho, double *fi);
+
+/* Bin n atoms given in world coordinates, in input order, with ids 0..n-1:
+ * the sequence of putAtomInBox calls of src-flat/read.c:222 and
+ * src-flat/ic_fcc.c:125 (wrap once into the box, cell = floor(x/boxsize),
+ * append), done on the GPU as a stable counting sort.  xyz, pxyz: 3n doubles
+ * in host memory.  itype is stored for every atom. */
+int comd_bin_atoms(comd_ctx *ctx, int n, const double *xyz, const double *pxyz, int itype);
+
+/* Same as comd_bin_atoms with the input already in device memory. */
+int comd_bin_atoms_device(comd_ctx *ctx, int n, const double *d_xyz, const double *d_pxyz, int itype);
+
+/* reBoxAll (src-flat/domains.c:347-389): same cell assignment and the same
+ * in-cell order as the reference's sequential scan. */
+int comd_rebox(comd_ctx *ctx);
+
+/* One force evaluation, the potential's force slot (src-flat/eam.c:438-660,
+ * src-flat/ljForce.c:89-224).  Results stay on the device. */
+int comd_force(comd_ctx *ctx);
+
+/* nTimeSteps (src-flat/timestep.c:114-140): n position-Verlet steps of dt
+ * seconds, then one more force evaluation.  *pe_out (may be NULL) receives
+ * the potential energy of that final force, as the reference returns s->e. */
+int comd_nsteps(comd_ctx *ctx, int n, double dt_seconds, double *pe_out);
+
+/* The two halves of a step, for callers that drive the integrator
+ * themselves: advancePositions (src-flat/timestep.c:76-108: r += (dt p)/m,
+ * then reBoxAll) and advanceVelocity (timestep.c:62-74: p -= dt f), with dt
+ * in the reference's internal unit (seconds * 0.529/2.418e-17). */
+int comd_advance_positions(comd_ctx *ctx, double dt_internal);
+int comd_advance_velocity(comd_ctx *ctx, double dt_internal);
+
+/* Totals of the last force evaluation: potential energy (s->e), the
+ * reference's scalar virial "stress" (src-flat/eam.c:649-657,
+ * src-flat/ljForce.c:175-221), and the kinetic energy sum |p|^2/(2 m) in
+ * hartree (which the reference never computes). */
+int comd_energy(comd_ctx *ctx, double *pe, double *ke, double *stress);
+
+int comd_set_precision(comd_ctx *ctx, int precision);
+int comd_set_eval(comd_ctx *ctx, int eval);
+
+/* Number of visible HIP devices and the name of the current one. */
+int comd_device_info(int *count, char *name, int name_len);
+
+/* Block until the context's stream is idle. */
+int comd_sync(comd_ctx *ctx);
+
+/* Human-readable text of the last error on this context (or of the last
+ * failed comd_create when ctx is NULL). */
+const char *comd_last_error(const comd_ctx *ctx);
+
+/* Device-side timing of the kernels launched on the context's stream:
+ * enables (on != 0) or disables per-kernel HIP-event timing, and reads the
+ * accumulated milliseconds and launch counts per kernel class
+ * (COMD_KCLASS_*). */
+enum comd_kclass {
+    COMD_K_REBIN = 0,   /* drift/kick + rebinning kernels */
+    COMD_K_DENSITY = 1, /* EAM pass 1 (+ embedding) or LJ */
+    COMD_K_EMBED_FORCE = 2, /* EAM pass 3 */
+    COMD_K_REDUCE = 3,
+    COMD_K_NCLASS = 4
+};
+int comd_timing(comd_ctx *ctx, int on);
+int comd_timing_read(comd_ctx *ctx, double *ms /* COMD_K_NCLASS */, int *launches /* COMD_K_NCLASS */);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
