@@ -1,0 +1,230 @@
+"""Benchmark of the link-cell EAM timestep on one MI355X.
+
+Workload: EAM Cu (cu500 tables, 3-point table interpolation), FCC 80^3 =
+2,048,000 atoms, a = 3.615 A, periodic, Maxwell-Boltzmann velocities at
+300 K from a fixed seed, dt = 1 fs.  One step = half drift + rebin, force
+(density + embedding pass, embedding-force pass), kick, half drift + rebin,
+all on the GPU with the state resident in HBM.  The headline value is the
+mixed-precision path (FP32 pair arithmetic, FP64 sums and integration); the
+FP64 path is timed in the same run.
+
+Prints one JSON line (rank 0).  --gpus above 1 is refused: this build runs
+on one GPU.
+"""
+import argparse
+import json
+import os
+import re
+import shutil
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import comd_amd as CA  # noqa: E402
+
+# MI355X published peaks
+PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+# Algorithmic work per atom-step (EAM table; +,-,*,/,sqrt = 1 FLOP each,
+# rejected candidate pairs not counted; P_half = half-pairs per atom):
+#   density pass: 66 FLOP per half-pair + 20 per atom (embedding)
+#   embedding-force pass: 49 FLOP per half-pair
+#   drift/kick: 24 FLOP per atom
+# Bytes per atom-step (FP64 state, compulsory): 232.
+FLOP_PAIR_DENSITY = 66
+FLOP_PAIR_EMBED_FORCE = 49
+FLOP_ATOM_EMBED = 20
+FLOP_ATOM_VERLET = 24
+BYTES_ATOM_STEP = 232
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=80, help="FCC unit cells per edge")
+    ap.add_argument("--seed", type=int, default=2048)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU reference timing")
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    return ap.parse_args()
+
+
+def run_precision(sim, precision, steps, warmup, n_atoms):
+    sim.set_precision(precision)
+    sim.steps(warmup)
+    sim.sync()
+    sim.force()
+    pe0, ke0, _ = sim.energy()
+    p0 = sim.pair_count() / n_atoms
+    sim.timing(True)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.steps(steps)
+    sim.sync()
+    t1 = time.perf_counter()
+    ms, launches = sim.timing_read()
+    sim.timing(False)
+    sim.force()
+    pe1, ke1, _ = sim.energy()
+    p1 = sim.pair_count() / n_atoms
+    wall = t1 - t0
+    return dict(
+        wall=wall,
+        atom_steps_per_s=n_atoms * steps / wall,
+        ms_per_step=1e3 * wall / steps,
+        kernel_ms=ms,
+        launches=launches,
+        p_half=0.5 * (p0 + p1),
+        e0=pe0 + ke0,
+        e1=pe1 + ke1,
+    )
+
+
+def cpu_baseline(n, steps):
+    """Compiled reference (oracle/_ref/ref_table, built from the reference's
+    own sources) on the same FCC lattice, one thread; falls back to the
+    oracle restatement when the compiled reference is absent."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_table")
+    cores = 1
+    sample = (f"EAM cu500 table, FCC {n}^3 ({4 * n ** 3} atoms) perfect lattice, {steps} steps after one force, "
+              f"extra end-of-call force excluded, 1 thread")
+    if os.path.exists(exe):
+        pin = ["taskset", "-c", "0"] if shutil.which("taskset") else []
+        out = subprocess.run(pin + [exe, "--eam", CA.POT_DIR, "cu500", "--fcc", str(n), str(n),
+                              str(n), "3.615", "1.0", "--steps", str(steps), "--dumps", "1", "--time"],
+                             capture_output=True, text=True, check=True).stdout
+        m = re.search(r"atom_steps_per_s=([0-9.eE+-]+)", out)
+        return dict(value=float(m.group(1)), unit="atom-steps/s", cores=cores, kind="reference", sample=sample)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py as O
+
+    pot = O.OraclePot.eam(CA.POT_DIR, "cu500")
+    s = O.OracleSim.fcc(pot, n, 3.615)
+    s.rebox()
+    t0 = time.perf_counter()
+    s.force()
+    tf = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    s.nsteps(steps)
+    t = time.perf_counter() - t0 - tf
+    return dict(value=4 * n ** 3 * steps / t, unit="atom-steps/s", cores=cores, kind="port", sample=sample)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    ndev, devname = CA.device_info()
+    if ndev < 1:
+        print("no HIP device", file=sys.stderr)
+        sys.exit(1)
+    n = args.n
+    pot = CA.EAMPotential("cu500")
+    t_setup = time.perf_counter()
+    sim, xyz, p = CA.fcc_simulation(pot, n, temp=300.0, disp=0.0, seed=args.seed)
+    n_atoms = xyz.shape[0]
+    del xyz, p
+    sim.rebox()
+    sim.force()
+    t_setup = time.perf_counter() - t_setup
+
+    res = {}
+    for prec in ("fp64", "mixed"):
+        res[prec] = run_precision(sim, prec, args.steps, args.warmup, n_atoms)
+        print(f"[bench] {prec}: {res[prec]['atom_steps_per_s'] / 1e9:.3f} G atom-steps/s, "
+              f"{res[prec]['ms_per_step']:.3f} ms/step, kernel ms {['%.2f' % v for v in res[prec]['kernel_ms']]}",
+              file=sys.stderr, flush=True)
+    sim.close()
+
+    mixed = res["mixed"]
+    fp64 = res["fp64"]
+    # dominant kernel: the density pass (pair energy, pair force, density,
+    # embedding) of the mixed path, timed with HIP events on its stream
+    k = CA.K_DENSITY
+    avg_s = mixed["kernel_ms"][k] / max(mixed["launches"][k], 1) * 1e-3
+    flop_launch = (FLOP_PAIR_DENSITY * mixed["p_half"] + FLOP_ATOM_EMBED) * n_atoms
+    achieved = flop_launch / avg_s / 1e12
+    k64 = fp64["kernel_ms"][k] / max(fp64["launches"][k], 1) * 1e-3
+    flop64 = (FLOP_PAIR_DENSITY * fp64["p_half"] + FLOP_ATOM_EMBED) * n_atoms
+
+    f_alg = (FLOP_PAIR_DENSITY + FLOP_PAIR_EMBED_FORCE) * mixed["p_half"] + FLOP_ATOM_EMBED + FLOP_ATOM_VERLET
+    ceil_mixed = min(PEAK_FP32_TFLOPS * 1e12 / f_alg, PEAK_HBM_GBS * 1e9 / BYTES_ATOM_STEP)
+    ceil_fp64 = min(PEAK_FP64_TFLOPS * 1e12 / f_alg, PEAK_HBM_GBS * 1e9 / BYTES_ATOM_STEP)
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(n, args.cpu_steps)
+        except Exception as exc:  # reported, not fatal
+            print(f"[bench] cpu baseline failed: {exc}", file=sys.stderr)
+
+    out = {
+        "metric": "atom-steps/s, EAM Cu 2.05M atoms: mixed-prec vs FP64 on 1 GPU; % roofline",
+        "value": mixed["atom_steps_per_s"],
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": mixed["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 pair arithmetic, f64 accumulation and integration",
+        "data": "synthetic: FCC lattice, seeded Maxwell-Boltzmann velocities at 300 K",
+        "config": {
+            "workload": f"EAM Cu cu500 table, FCC {n}^3 = {n_atoms} atoms, a=3.615 A, 300 K, dt=1 fs, "
+                        f"link cells {'x'.join(map(str, sim.nbx))}",
+            "atoms": n_atoms,
+            "precision": "mixed",
+            "parallelism": "single GPU",
+        },
+        "fp64": {
+            "value": fp64["atom_steps_per_s"],
+            "ms_per_step": fp64["ms_per_step"],
+            "density_kernel_ms": k64 * 1e3,
+            "density_kernel_tflops": flop64 / k64 / 1e12,
+            "step_roofline_frac": fp64["atom_steps_per_s"] / ceil_fp64,
+            "energy_drift_per_atom_ha": (fp64["e1"] - fp64["e0"]) / n_atoms,
+        },
+        "mixed_over_fp64": mixed["atom_steps_per_s"] / fp64["atom_steps_per_s"],
+        "roofline": {
+            "bound": "valu",
+            "kernel": "k_pair<EAM density pass, table, mixed>",
+            "achieved": achieved,
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32_TFLOPS,
+            "traffic": None,
+            "avg_launch_ms": avg_s * 1e3,
+            "flop_per_launch": flop_launch,
+            "p_half": mixed["p_half"],
+        },
+        "step_roofline": {
+            "f_alg_flop_per_atom_step": f_alg,
+            "b_alg_bytes_per_atom_step": BYTES_ATOM_STEP,
+            "ceiling_atom_steps_per_s": ceil_mixed,
+            "frac": mixed["atom_steps_per_s"] / ceil_mixed,
+        },
+        "kernel_ms_per_step": {
+            name: mixed["kernel_ms"][i] / args.steps
+            for i, name in enumerate(("move+rebin", "density", "embed_force", "reduce"))
+        },
+        "energy_drift_per_atom_ha": (mixed["e1"] - mixed["e0"]) / n_atoms,
+        "device": devname,
+        "setup_s": t_setup,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

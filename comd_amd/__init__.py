@@ -35,6 +35,7 @@ POT_EAM, POT_LJ = 1, 2
 EVAL_TABLE, EVAL_CHEB = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 K_REBIN, K_DENSITY, K_EMBED_FORCE, K_REDUCE, K_NCLASS = 0, 1, 2, 3, 4
+TUNE_WAVES, TUNE_BLOCKS_PER_CU, TUNE_TABLES_LDS = 1, 2, 3
 
 # unit constants of src-flat/include/constants.h:61-65
 BOHR_PER_ATU_TO_A_PER_S = 0.529 / 2.418e-17
@@ -110,6 +111,9 @@ def _load():
     hip.comd_rebox.argtypes = [vp]
     hip.comd_force.argtypes = [vp]
     hip.comd_nsteps.argtypes = [vp, C.c_int, C.c_double, _dp]
+    hip.comd_steps.argtypes = [vp, C.c_int, C.c_double]
+    hip.comd_pair_count.argtypes = [vp, C.POINTER(C.c_longlong)]
+    hip.comd_tune.argtypes = [vp, C.c_int, C.c_int]
     hip.comd_energy.argtypes = [vp, _dp, _dp, _dp]
     hip.comd_advance_positions.argtypes = [vp, C.c_double]
     hip.comd_advance_velocity.argtypes = [vp, C.c_double]
@@ -382,6 +386,10 @@ class Simulation:
         self._check(self._hip.comd_nsteps(self.ctx, int(n), float(dt), C.byref(pe)))
         return pe.value
 
+    def steps(self, n, dt=1.0e-15):
+        """n DKD steps without the trailing force of nTimeSteps; asynchronous."""
+        self._check(self._hip.comd_steps(self.ctx, int(n), float(dt)))
+
     def advance_positions(self, dt_internal):
         self._check(self._hip.comd_advance_positions(self.ctx, float(dt_internal)))
 
@@ -393,6 +401,19 @@ class Simulation:
         pe, ke, st = C.c_double(), C.c_double(), C.c_double()
         self._check(self._hip.comd_energy(self.ctx, C.byref(pe), C.byref(ke), C.byref(st)))
         return pe.value, ke.value, st.value
+
+    def tune(self, waves_per_block=None, blocks_per_cu=None, tables_in_lds=None):
+        """Launch shape of the force kernels (comd_tune); results do not change."""
+        for key, val in ((TUNE_WAVES, waves_per_block), (TUNE_BLOCKS_PER_CU, blocks_per_cu),
+                         (TUNE_TABLES_LDS, tables_in_lds)):
+            if val is not None:
+                self._check(self._hip.comd_tune(self.ctx, key, int(val)))
+
+    def pair_count(self):
+        """Half-pairs inside the cutoff found by the last density/LJ pass."""
+        n = C.c_longlong()
+        self._check(self._hip.comd_pair_count(self.ctx, C.byref(n)))
+        return n.value
 
     def set_precision(self, precision):
         self._check(self._hip.comd_set_precision(self.ctx, PREC_MIXED if precision == "mixed" else PREC_FP64))

@@ -36,9 +36,11 @@ struct comd_ctx {
     double *rx[2], *ry[2], *rz[2], *px[2], *py[2], *pz[2];
     // outputs
     double *fx, *fy, *fz, *fe, *rho, *fi, *vir;
+    int *nnb;
     // scratch
-    int *dest, *rank, *meta;
-    double *rnx, *rny, *rnz;
+    int *dest, *mvidx, *meta, *counts, *touched, *tc, *arr_cnt, *arr;
+    int *mv_slot, *mv_rank, *mv_id, *mv_itype;
+    double *mv_rx, *mv_ry, *mv_rz, *mv_px, *mv_py, *mv_pz;
     uint8_t *perm;
     unsigned *flags;
     double *partials;
@@ -49,13 +51,17 @@ struct comd_ctx {
     // potential
     PotDev pot;
     double *d_tab[3];
-    float *d_coef32[2];
+    float *d_c4[2], *d_d1[2];
     double *d_cheb;
     float *d_cheb32;
-    int force_blocks;
+    // force-kernel launch shape (comd_tune)
+    int ncu, max_lds;
+    int force_waves, force_blocks_per_cu, tab_lds;
     // timing
     int timing;
     std::vector<TimedEvent> events;
+    std::vector<hipEvent_t> pool;  // reusable events
+    size_t pool_used;
     double ms[COMD_K_NCLASS];
     int launches[COMD_K_NCLASS];
     std::string err;
@@ -82,7 +88,7 @@ static int set_err(comd_ctx *ctx, int code, const char *fmt, ...)
 static DevState dev_state(const comd_ctx *c)
 {
     DevState d;
-    const int a = c->cur, b = 1 - c->cur;
+    const int a = c->cur;
     d.nbx0 = c->prm.nbx[0];
     d.nbx1 = c->prm.nbx[1];
     d.nbx2 = c->prm.nbx[2];
@@ -95,39 +101,51 @@ static DevState dev_state(const comd_ctx *c)
     d.L1 = c->prm.bounds[1];
     d.L2 = c->prm.bounds[2];
     d.natoms = c->natoms[a];
-    d.natoms_n = c->natoms[b];
     d.id = c->id[a];
-    d.id_n = c->id[b];
     d.itype = c->itype[a];
-    d.itype_n = c->itype[b];
     d.rx = c->rx[a]; d.ry = c->ry[a]; d.rz = c->rz[a];
-    d.rx_n = c->rx[b]; d.ry_n = c->ry[b]; d.rz_n = c->rz[b];
     d.px = c->px[a]; d.py = c->py[a]; d.pz = c->pz[a];
-    d.px_n = c->px[b]; d.py_n = c->py[b]; d.pz_n = c->pz[b];
     d.fx = c->fx; d.fy = c->fy; d.fz = c->fz; d.fe = c->fe;
     d.rho = c->rho; d.fi = c->fi; d.vir = c->vir;
+    d.nnb = c->nnb;
     d.dest = c->dest;
-    d.rank = c->rank;
-    d.rnx = c->rnx; d.rny = c->rny; d.rnz = c->rnz;
+    d.mvidx = c->mvidx;
+    d.counts = c->counts;
+    d.touched = c->touched;
+    d.tc = c->tc;
+    d.arr_cnt = c->arr_cnt;
+    d.arr = c->arr;
+    d.mv_slot = c->mv_slot; d.mv_rank = c->mv_rank; d.mv_id = c->mv_id; d.mv_itype = c->mv_itype;
+    d.mv_rx = c->mv_rx; d.mv_ry = c->mv_ry; d.mv_rz = c->mv_rz;
+    d.mv_px = c->mv_px; d.mv_py = c->mv_py; d.mv_pz = c->mv_pz;
     d.perm = c->perm;
     d.meta = c->meta;
     d.flags = c->flags;
     return d;
 }
 
-// ---- timing helpers
+// ---- timing helpers: HIP events recorded on the context's stream around
+// each kernel group, taken from a reusable pool
+static hipEvent_t pool_event(comd_ctx *c)
+{
+    if (c->pool_used == c->pool.size()) {
+        hipEvent_t e;
+        hipEventCreate(&e);
+        c->pool.push_back(e);
+    }
+    return c->pool[c->pool_used++];
+}
 static void tbegin(comd_ctx *c, int cls, hipEvent_t *a)
 {
-    if (!c->timing) return;
-    hipEventCreate(a);
-    hipEventRecord(*a, c->stream);
     (void)cls;
+    if (!c->timing) return;
+    *a = pool_event(c);
+    hipEventRecord(*a, c->stream);
 }
 static void tend(comd_ctx *c, int cls, hipEvent_t a)
 {
     if (!c->timing) return;
-    hipEvent_t b;
-    hipEventCreate(&b);
+    hipEvent_t b = pool_event(c);
     hipEventRecord(b, c->stream);
     c->events.push_back({cls, a, b});
 }
@@ -141,16 +159,32 @@ static void op_rebox(comd_ctx *c)
     DevState d = dev_state(c);
     cm_launch_rebox(d, c->stream);
     tend(c, COMD_K_REBIN, ev);
-    c->cur = 1 - c->cur;
 }
 
-static void op_move(comd_ctx *c, double dtk, double dtd, int kick, int drift)
+// rebin: record movers for the op_rebox that follows (0 for a lone kick)
+static void op_move(comd_ctx *c, double dtk, double dtd, int kick, int drift, int rebin = 1)
 {
     hipEvent_t ev = nullptr;
     tbegin(c, COMD_K_REBIN, &ev);
     DevState d = dev_state(c);
-    cm_launch_move(d, c->stream, dtk, dtd, kAmuToMe * c->prm.mass, kick, drift);
+    cm_launch_move(d, c->stream, dtk, dtd, kAmuToMe * c->prm.mass, kick, drift, rebin);
     tend(c, COMD_K_REBIN, ev);
+}
+
+static ForceCfg force_cfg(const comd_ctx *c, int kind, int mixed)
+{
+    ForceCfg f;
+    f.waves = c->force_waves;
+    f.tab_lds = c->tab_lds;
+    f.max_lds = c->max_lds;
+    f.ncu = c->ncu;
+    // fewer waves per block when the per-wave LDS does not fit
+    while (f.waves > 1 && cm_force_lds_bytes(c->pot, kind, mixed, f.waves, 0) > f.max_lds) f.waves--;
+    long b = (long)c->ncu * c->force_blocks_per_cu;
+    const long need = ((long)c->ncells + f.waves - 1) / f.waves;
+    f.blocks = (int)(b < need ? b : need);
+    if (f.blocks < 1) f.blocks = 1;
+    return f;
 }
 
 static void op_force(comd_ctx *c)
@@ -161,15 +195,15 @@ static void op_force(comd_ctx *c)
     if (c->prm.potential == COMD_POT_LJ) {
         hipEvent_t ev = nullptr;
         tbegin(c, COMD_K_DENSITY, &ev);
-        cm_launch_force(d, c->pot, CM_KIND_LJ, mixed, c->force_blocks, c->stream);
+        cm_launch_force(d, c->pot, CM_KIND_LJ, mixed, force_cfg(c, CM_KIND_LJ, mixed), c->stream);
         tend(c, COMD_K_DENSITY, ev);
     } else {
         hipEvent_t ev = nullptr;
         tbegin(c, COMD_K_DENSITY, &ev);
-        cm_launch_force(d, c->pot, CM_KIND_EAM1, mixed, c->force_blocks, c->stream);
+        cm_launch_force(d, c->pot, CM_KIND_EAM1, mixed, force_cfg(c, CM_KIND_EAM1, mixed), c->stream);
         tend(c, COMD_K_DENSITY, ev);
         tbegin(c, COMD_K_EMBED_FORCE, &ev);
-        cm_launch_force(d, c->pot, CM_KIND_EAM3, mixed, c->force_blocks, c->stream);
+        cm_launch_force(d, c->pot, CM_KIND_EAM3, mixed, force_cfg(c, CM_KIND_EAM3, mixed), c->stream);
         tend(c, COMD_K_EMBED_FORCE, ev);
     }
 }
@@ -205,9 +239,12 @@ int comd_grid_for(comd_params *p, const double bounds[3], double defgrad, double
 
 static void free_all(comd_ctx *c)
 {
-    void *ptrs[] = {c->fx, c->fy, c->fz, c->fe, c->rho, c->fi, c->vir, c->dest, c->rank, c->meta,
-                    c->rnx, c->rny, c->rnz, c->perm, c->flags, c->partials, c->red_out,
-                    c->d_tab[0], c->d_tab[1], c->d_tab[2], c->d_coef32[0], c->d_coef32[1], c->d_cheb, c->d_cheb32};
+    void *ptrs[] = {c->fx, c->fy, c->fz, c->fe, c->rho, c->fi, c->vir, c->nnb, c->dest, c->mvidx, c->meta,
+                    c->counts, c->touched, c->tc, c->arr_cnt, c->arr, c->mv_slot, c->mv_rank, c->mv_id,
+                    c->mv_itype, c->mv_rx, c->mv_ry, c->mv_rz, c->mv_px, c->mv_py, c->mv_pz,
+                    c->perm, c->flags, c->partials, c->red_out,
+                    c->d_tab[0], c->d_tab[1], c->d_tab[2], c->d_c4[0], c->d_c4[1], c->d_d1[0], c->d_d1[1],
+                    c->d_cheb, c->d_cheb32};
     for (void *p : ptrs)
         if (p) hipFree(p);
     for (int b = 0; b < 2; b++) {
@@ -217,30 +254,30 @@ static void free_all(comd_ctx *c)
     }
     if (c->host_out) hipHostFree(c->host_out);
     if (c->host_flags) hipHostFree(c->host_flags);
-    for (auto &e : c->events) {
-        hipEventDestroy(e.a);
-        hipEventDestroy(e.b);
-    }
+    for (auto e : c->pool) hipEventDestroy(e);
     if (c->stream) hipStreamDestroy(c->stream);
 }
 
 // Per-interval coefficients of the table formula (src-flat/eam.c:418-433),
-// evaluated in double precision and stored as float for the mixed path.
-static std::vector<float> table_coefs32(int n, double invdx, const double *vals /* values[-1] first */)
+// evaluated in double precision and stored as float for the mixed path:
+// c4[i] = (c0, c1, c2, d0), d1[i], with value c0 + t(c1 + t c2) and
+// derivative d0 + t d1 on interval i (the derivative is 0 for i <= 0).
+static void table_coefs32(int n, double invdx, const double *vals /* values[-1] first */, std::vector<float> &c4,
+                          std::vector<float> &d1)
 {
-    std::vector<float> out((size_t)8 * n, 0.0f);
+    c4.assign((size_t)4 * n, 0.0f);
+    d1.assign((size_t)n, 0.0f);
     const double *v = vals + 1;
     for (int i = 0; i < n; i++) {
         const double vm = v[i - 1], v0 = v[i], v1 = v[i + 1], v2 = v[i + 2];
         const double g0 = v1 - vm, g1 = v2 - v0;
         const double A = v1 + vm - 2.0 * v0;
-        out[8 * i + 0] = (float)v0;
-        out[8 * i + 1] = (float)(0.5 * g0);
-        out[8 * i + 2] = (float)(0.5 * A);
-        out[8 * i + 3] = i <= 0 ? 0.0f : (float)(0.5 * g0 * invdx);
-        out[8 * i + 4] = i <= 0 ? 0.0f : (float)(0.5 * (g1 - g0) * invdx);
+        c4[4 * i + 0] = (float)v0;
+        c4[4 * i + 1] = (float)(0.5 * g0);
+        c4[4 * i + 2] = (float)(0.5 * A);
+        c4[4 * i + 3] = i <= 0 ? 0.0f : (float)(0.5 * g0 * invdx);
+        d1[i] = i <= 0 ? 0.0f : (float)(0.5 * (g1 - g0) * invdx);
     }
-    return out;
 }
 
 int comd_create(const comd_params *p, comd_ctx **out)
@@ -284,7 +321,7 @@ int comd_create(const comd_params *p, comd_ctx **out)
         else hipMemsetAsync(*ptr, 0, bytes < 16 ? 16 : bytes, c->stream);
     };
     const size_t S = (size_t)c->nslots, C = (size_t)c->ncells;
-    for (int b = 0; b < 2; b++) {
+    for (int b = 0; b < 1; b++) {
         alloc((void **)&c->natoms[b], C * 4);
         alloc((void **)&c->id[b], S * 4);
         alloc((void **)&c->itype[b], S * 4);
@@ -302,12 +339,26 @@ int comd_create(const comd_params *p, comd_ctx **out)
     alloc((void **)&c->rho, S * 8);
     alloc((void **)&c->fi, S * 8);
     alloc((void **)&c->vir, S * 8);
+    alloc((void **)&c->nnb, S * 4);
     alloc((void **)&c->dest, S * 4);
-    alloc((void **)&c->rank, S * 4);
+    alloc((void **)&c->mvidx, S * 4);
     alloc((void **)&c->meta, C * 16);
-    alloc((void **)&c->rnx, S * 8);
-    alloc((void **)&c->rny, S * 8);
-    alloc((void **)&c->rnz, S * 8);
+    alloc((void **)&c->counts, 16);
+    alloc((void **)&c->touched, C * 4);
+    alloc((void **)&c->tc, C * 4);
+    alloc((void **)&c->arr_cnt, C * 4);
+    alloc((void **)&c->arr, S * 4);
+    // one mover record per slot: a pass can never overflow the list
+    alloc((void **)&c->mv_slot, S * 4);
+    alloc((void **)&c->mv_rank, S * 4);
+    alloc((void **)&c->mv_id, S * 4);
+    alloc((void **)&c->mv_itype, S * 4);
+    alloc((void **)&c->mv_rx, S * 8);
+    alloc((void **)&c->mv_ry, S * 8);
+    alloc((void **)&c->mv_rz, S * 8);
+    alloc((void **)&c->mv_px, S * 8);
+    alloc((void **)&c->mv_py, S * 8);
+    alloc((void **)&c->mv_pz, S * 8);
     alloc((void **)&c->perm, S);
     alloc((void **)&c->flags, 16);
     c->nparts = 1024;
@@ -340,15 +391,16 @@ int comd_create(const comd_params *p, comd_ctx **out)
             P.n[k] = n;
         }
         for (int k = 0; ok && k < 2; k++) {
-            std::vector<float> cf = table_coefs32(p->tab_n[k], p->tab_invdx[k], p->tab_values[k]);
-            alloc((void **)&c->d_coef32[k], cf.size() * 4);
+            std::vector<float> c4, d1;
+            table_coefs32(p->tab_n[k], p->tab_invdx[k], p->tab_values[k], c4, d1);
+            alloc((void **)&c->d_c4[k], c4.size() * 4);
+            alloc((void **)&c->d_d1[k], d1.size() * 4);
             if (!ok) break;
-            hipMemcpyAsync(c->d_coef32[k], cf.data(), cf.size() * 4, hipMemcpyHostToDevice, c->stream);
+            hipMemcpyAsync(c->d_c4[k], c4.data(), c4.size() * 4, hipMemcpyHostToDevice, c->stream);
+            hipMemcpyAsync(c->d_d1[k], d1.data(), d1.size() * 4, hipMemcpyHostToDevice, c->stream);
             hipStreamSynchronize(c->stream);
-            P.coef32[k] = c->d_coef32[k];
-            P.x0f[k] = (float)p->tab_x0[k];
-            P.xnf[k] = (float)p->tab_xn[k];
-            P.invdxf[k] = (float)p->tab_invdx[k];
+            P.coef32[k] = (const float4 *)c->d_c4[k];
+            P.coef32d1[k] = c->d_d1[k];
         }
         if (ok && p->cheb_n >= 2 && p->cheb_coef) {
             const int m = p->cheb_n;
@@ -376,9 +428,16 @@ int comd_create(const comd_params *p, comd_ctx **out)
         return set_err(nullptr, COMD_ERR_NOMEM, "device allocation failed");
     }
     hipDeviceProp_t prop;
-    int ncu = 256;
-    if (hipGetDeviceProperties(&prop, c->device) == hipSuccess && prop.multiProcessorCount > 0) ncu = prop.multiProcessorCount;
-    c->force_blocks = ncu * 16;
+    c->ncu = 256;
+    c->max_lds = 65536;
+    if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) {
+        if (prop.multiProcessorCount > 0) c->ncu = prop.multiProcessorCount;
+        if (prop.sharedMemPerBlock > 0 && prop.sharedMemPerBlock < (size_t)c->max_lds) c->max_lds = (int)prop.sharedMemPerBlock;
+    }
+    // measured on MI355X (2.05M-atom EAM): single-wave blocks, 8 per CU
+    c->force_waves = 1;
+    c->force_blocks_per_cu = 8;
+    c->tab_lds = 1;
     if (hipStreamSynchronize(c->stream) != hipSuccess) {
         free_all(c);
         delete c;
@@ -487,7 +546,7 @@ int comd_bin_atoms_device(comd_ctx *c, int n, const double *d_xyz, const double 
     if ((long)n > c->nslots) return set_err(c, COMD_ERR_OVERFLOW, "more atoms than slots");
     HIPCHK(c, hipMemsetAsync(c->flags, 0, 4, c->stream));
     DevState d = dev_state(c);
-    cm_launch_bin(d, c->stream, n, d_xyz, d_pxyz, itype, c->meta, c->rank);
+    cm_launch_bin(d, c->stream, n, d_xyz, d_pxyz, itype, c->meta, c->arr);
     int rc = check_flags(c);
     if (rc) return rc;
     c->ntot = n;
@@ -578,6 +637,23 @@ int comd_nsteps(comd_ctx *c, int n, double dt_seconds, double *pe_out)
     return COMD_OK;
 }
 
+int comd_steps(comd_ctx *c, int n, double dt_seconds)
+{
+    if (!c || n < 0) return COMD_ERR_ARG;
+    if (!c->have_atoms) return set_err(c, COMD_ERR_STATE, "no atoms");
+    HIPCHK(c, hipMemsetAsync(c->flags, 0, 4, c->stream));
+    const double dt = dt_seconds * kBohrPerAtuToAPerS;
+    for (int i = 0; i < n; i++) {
+        op_move(c, 0.0, dt / 2.0, 0, 1);
+        op_rebox(c);
+        op_force(c);
+        op_move(c, dt, dt / 2.0, 1, 1);
+        op_rebox(c);
+    }
+    HIPCHK(c, hipGetLastError());
+    return COMD_OK;
+}
+
 int comd_advance_positions(comd_ctx *c, double dt_internal)
 {
     if (!c) return COMD_ERR_ARG;
@@ -594,7 +670,7 @@ int comd_advance_velocity(comd_ctx *c, double dt_internal)
     if (!c->have_atoms) return set_err(c, COMD_ERR_STATE, "no atoms");
     // kick only: k_move with drift off leaves r untouched; its destination
     // scratch is rewritten before the next rebin
-    op_move(c, dt_internal, 0.0, 1, 0);
+    op_move(c, dt_internal, 0.0, 1, 0, 0);
     HIPCHK(c, hipGetLastError());
     return COMD_OK;
 }
@@ -629,22 +705,54 @@ int comd_device_info(int *count, char *name, int name_len)
     return n > 0 ? COMD_OK : COMD_ERR_NODEVICE;
 }
 
+int comd_tune(comd_ctx *c, int key, int value)
+{
+    if (!c) return COMD_ERR_ARG;
+    switch (key) {
+    case COMD_TUNE_WAVES_PER_BLOCK:
+        if (value < 1 || value > 4) return set_err(c, COMD_ERR_ARG, "waves per block must be 1..4");
+        c->force_waves = value;
+        return COMD_OK;
+    case COMD_TUNE_BLOCKS_PER_CU:
+        if (value < 1 || value > 64) return set_err(c, COMD_ERR_ARG, "blocks per CU must be 1..64");
+        c->force_blocks_per_cu = value;
+        return COMD_OK;
+    case COMD_TUNE_TABLES_IN_LDS:
+        c->tab_lds = value ? 1 : 0;
+        return COMD_OK;
+    default:
+        return set_err(c, COMD_ERR_ARG, "unknown tuning key %d", key);
+    }
+}
+
+int comd_pair_count(comd_ctx *c, long long *half_pairs)
+{
+    if (!c || !half_pairs) return COMD_ERR_ARG;
+    const size_t S = (size_t)c->nslots, C = (size_t)c->ncells;
+    std::vector<int> nn(S), na(C);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(nn.data(), c->nnb, S * 4, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(na.data(), c->natoms[c->cur], C * 4, hipMemcpyDeviceToHost));
+    long long tot = 0;
+    for (size_t b = 0; b < C; b++)
+        for (int i = 0; i < na[b]; i++) tot += nn[b * CM_SLOTS + i];
+    *half_pairs = tot / 2;
+    return COMD_OK;
+}
+
 int comd_sync(comd_ctx *c)
 {
     if (!c) return COMD_ERR_ARG;
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    return COMD_OK;
+    return check_flags(c);
 }
 
 int comd_timing(comd_ctx *c, int on)
 {
     if (!c) return COMD_ERR_ARG;
     hipStreamSynchronize(c->stream);
-    for (auto &e : c->events) {
-        hipEventDestroy(e.a);
-        hipEventDestroy(e.b);
-    }
     c->events.clear();
+    c->pool_used = 0;
     for (int k = 0; k < COMD_K_NCLASS; k++) {
         c->ms[k] = 0;
         c->launches[k] = 0;
@@ -662,10 +770,9 @@ int comd_timing_read(comd_ctx *c, double *ms, int *launches)
         hipEventElapsedTime(&t, e.a, e.b);
         c->ms[e.cls] += t;
         c->launches[e.cls] += 1;
-        hipEventDestroy(e.a);
-        hipEventDestroy(e.b);
     }
     c->events.clear();
+    c->pool_used = 0;
     for (int k = 0; k < COMD_K_NCLASS; k++) {
         if (ms) ms[k] = c->ms[k];
         if (launches) launches[k] = c->launches[k];

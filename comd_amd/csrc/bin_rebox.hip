@@ -1,11 +1,11 @@
-// bin_rebox.hip -- atom binning (GPU counting sort) and the parallel
-// emulation of reBoxAll, plus the fused kick/drift that precedes every
-// rebin in the position-Verlet step.
+// bin_rebox.hip -- atom binning (GPU counting sort) and a parallel, in-place
+// emulation of reBoxAll, with the kick/drift that precedes every rebin of the
+// position-Verlet step fused in.
 //
 // reBoxAll (src-flat/domains.c:347-389) scans cells in ascending order; an
 // atom that left its cell is appended to the destination (moveAtom,
 // domains.c:224-236) and the source's last atom fills the hole, which is
-// examined next.  Its result is reproduced here without a serial scan:
+// examined next.  Its result is reproduced without a serial scan:
 //   * whether an atom leaves, and where to, depends only on its own r;
 //   * the order of departures from cell c depends only on c's own list and
 //     on the NUMBER n_low of atoms that arrived from lower-indexed cells
@@ -13,16 +13,31 @@
 //   * arrivals into c come in (source cell, departure rank) order; those
 //     from lower cells fill the n_low placeholder positions, those from
 //     higher cells are appended after c's scan.
-// That holds while no arrival into a higher-indexed cell would move again in
-// the same pass (k_move flags it).  Then a serial kernel redoes the pass
-// exactly as the reference does.
+// Only cells that lose or gain atoms change, so the pass works on the list of
+// movers and of touched cells (a few per thousand atoms per half step) and
+// updates the slot arrays in place.  The argument holds while no arrival
+// into a higher-indexed cell would move again in the same pass; k_move flags
+// that case and a serial kernel then redoes the pass exactly as the reference
+// does.
 #include "comd_internal.h"
+
+#define CM_ACAP 64  // arrivals recorded per cell and pass
+
+__device__ static inline void mark_touched(const DevState &d, int c)
+{
+    if (atomicExch(&d.touched[c], 1) == 0) {
+        const int i = atomicAdd(&d.counts[1], 1);
+        d.tc[i] = c;
+    }
+}
 
 // ---------------------------------------------------------------------------
 // kick (p -= dtk f, src-flat/timestep.c:62-74), drift (r += (dtd p)/rmass,
 // timestep.c:76-104) and the per-atom destination of reBoxAll
-// (domains.c:356-377).  One block of 64 threads per cell.
-__global__ __launch_bounds__(64) void k_move(DevState d, double dtk, double dtd, double rmass, int kick, int drift)
+// (domains.c:356-377); departing atoms are recorded as movers.  One block of
+// 64 threads per cell.
+__global__ __launch_bounds__(64) void k_move(DevState d, double dtk, double dtd, double rmass, int kick, int drift,
+                                             int rebin)
 {
     const int c = blockIdx.x;
     const int t = threadIdx.x;
@@ -30,25 +45,24 @@ __global__ __launch_bounds__(64) void k_move(DevState d, double dtk, double dtd,
     if (t >= n) return;
     const int s = c * CM_SLOTS + t;
     double x = d.rx[s], y = d.ry[s], z = d.rz[s];
-    if (kick || drift) {
-        double px = d.px[s], py = d.py[s], pz = d.pz[s];
-        if (kick) {
-            px = px - dtk * d.fx[s];
-            py = py - dtk * d.fy[s];
-            pz = pz - dtk * d.fz[s];
-            d.px[s] = px;
-            d.py[s] = py;
-            d.pz[s] = pz;
-        }
-        if (drift) {
-            x = x + dtd * px / rmass;
-            y = y + dtd * py / rmass;
-            z = z + dtd * pz / rmass;
-            d.rx[s] = x;
-            d.ry[s] = y;
-            d.rz[s] = z;
-        }
+    double px = d.px[s], py = d.py[s], pz = d.pz[s];
+    if (kick) {
+        px = px - dtk * d.fx[s];
+        py = py - dtk * d.fy[s];
+        pz = pz - dtk * d.fz[s];
+        d.px[s] = px;
+        d.py[s] = py;
+        d.pz[s] = pz;
     }
+    if (drift) {
+        x = x + dtd * px / rmass;
+        y = y + dtd * py / rmass;
+        z = z + dtd * pz / rmass;
+        d.rx[s] = x;
+        d.ry[s] = y;
+        d.rz[s] = z;
+    }
+    if (!rebin) return;
     int ix, iy, iz;
     cell_xyz(d, c, ix, iy, iz);
     int nx = ix, ny = iy, nz = iz;
@@ -56,211 +70,207 @@ __global__ __launch_bounds__(64) void k_move(DevState d, double dtk, double dtd,
     if (x < 0.0) { nx = ix - 1; ax = x + d.bs0; } else if (x >= d.bs0) { nx = ix + 1; ax = x - d.bs0; }
     if (y < 0.0) { ny = iy - 1; ay = y + d.bs1; } else if (y >= d.bs1) { ny = iy + 1; ay = y - d.bs1; }
     if (z < 0.0) { nz = iz - 1; az = z + d.bs2; } else if (z >= d.bs2) { nz = iz + 1; az = z - d.bs2; }
-    int j = cell_wrap(d, nx, ny, nz);
-    int dst = (j < 0 || j == c) ? -1 : j;
+    const int j = cell_wrap(d, nx, ny, nz);
+    const int dst = (j < 0 || j == c) ? -1 : j;
     d.dest[s] = dst;
-    if (dst >= 0) {
-        d.rnx[s] = ax;
-        d.rny[s] = ay;
-        d.rnz[s] = az;
-        if (dst > c) {
-            // the arrival is examined again when dst is scanned
-            int jx, jy, jz;
-            cell_xyz(d, dst, jx, jy, jz);
-            int mx = jx, my = jy, mz = jz;
-            if (ax < 0.0) mx = jx - 1; else if (ax >= d.bs0) mx = jx + 1;
-            if (ay < 0.0) my = jy - 1; else if (ay >= d.bs1) my = jy + 1;
-            if (az < 0.0) mz = jz - 1; else if (az >= d.bs2) mz = jz + 1;
-            int j2 = cell_wrap(d, mx, my, mz);
-            if (j2 >= 0 && j2 != dst) atomicOr(d.flags, (unsigned)CM_FLAG_MULTIHOP);
-        }
+    if (dst < 0) return;
+    if (dst > c) {
+        // the arrival is examined again when dst is scanned
+        int jx, jy, jz;
+        cell_xyz(d, dst, jx, jy, jz);
+        int mx = jx, my = jy, mz = jz;
+        if (ax < 0.0) mx = jx - 1; else if (ax >= d.bs0) mx = jx + 1;
+        if (ay < 0.0) my = jy - 1; else if (ay >= d.bs1) my = jy + 1;
+        if (az < 0.0) mz = jz - 1; else if (az >= d.bs2) mz = jz + 1;
+        const int j2 = cell_wrap(d, mx, my, mz);
+        if (j2 >= 0 && j2 != dst) atomicOr(d.flags, (unsigned)CM_FLAG_MULTIHOP);
     }
+    const int m = atomicAdd(&d.counts[0], 1);  // capacity: one record per slot
+    d.mv_slot[m] = s;
+    d.mv_id[m] = d.id[s];
+    d.mv_itype[m] = d.itype[s];
+    d.mv_rx[m] = ax;
+    d.mv_ry[m] = ay;
+    d.mv_rz[m] = az;
+    d.mv_px[m] = px;
+    d.mv_py[m] = py;
+    d.mv_pz[m] = pz;
+    d.mvidx[s] = m;
+    const int k = atomicAdd(&d.arr_cnt[dst], 1);
+    if (k < CM_ACAP) d.arr[dst * CM_ACAP + k] = m;
+    mark_touched(d, c);
+    mark_touched(d, dst);
 }
 
-// Distinct cells (other than c) that an atom can leave to reach c: the 26
-// neighbours, wrapped and de-duplicated, ascending.
-__device__ static int source_cells(const DevState &d, int c, int *src)
-{
-    int n = 0;
-    if (d.ncells == 1) return 0;
-    int ix, iy, iz;
-    cell_xyz(d, c, ix, iy, iz);
-    for (int e = 0; e < 27; e++) {
-        if (e == 13) continue;
-        int b = cell_wrap(d, ix + e / 9 - 1, iy + (e / 3) % 3 - 1, iz + e % 3 - 1);
-        if (b < 0 || b == c) continue;
-        int k = n;
-        bool dup = false;
-        for (int q = 0; q < n; q++) dup |= (src[q] == b);
-        if (dup) continue;
-        while (k > 0 && src[k - 1] > b) { src[k] = src[k - 1]; k--; }
-        src[k] = b;
-        n++;
-    }
-    return n;
-}
-
-// Per cell: count arrivals from lower/higher cells, then run the cell's own
-// swap-with-last scan with n_low never-departing placeholders appended,
-// recording each departure's rank and the surviving arrangement.
+// Per touched cell (one thread each): count arrivals from lower/higher cells,
+// then run the cell's own swap-with-last scan with n_low never-departing
+// placeholders appended, recording each departure's rank and the resulting
+// arrangement.
 __global__ __launch_bounds__(64) void k_rebox_scan(DevState d)
 {
     __shared__ uint8_t sperm[64][128];
-    const int c = blockIdx.x * 64 + threadIdx.x;
-    if (c >= d.ncells) return;
+    if (*d.flags & CM_FLAG_MULTIHOP) return;
     uint8_t *perm = sperm[threadIdx.x];
-    const int n0 = d.natoms[c];
-    int src[26];
-    const int nsrc = source_cells(d, c, src);
-    int nlow = 0, nhigh = 0;
-    for (int q = 0; q < nsrc; q++) {
-        const int b = src[q];
-        const int nb = d.natoms[b];
-        int k = 0;
-        for (int s = 0; s < nb; s++) k += (d.dest[b * CM_SLOTS + s] == c);
-        if (b < c) nlow += k; else nhigh += k;
-    }
-    unsigned fl = 0;
-    if (nlow > 0 && n0 + nlow >= CM_SLOTS) fl |= CM_FLAG_OVERFLOW;
-    int len = n0 + nlow;
-    if (len > 127) len = 127;
-    for (int i = 0; i < len; i++) perm[i] = (uint8_t)(i < n0 ? i : CM_SLOTS + (i - n0));
-    int i = 0, rk = 0;
-    while (i < len) {
-        const int e = perm[i];
-        const bool dep = e < CM_SLOTS && d.dest[c * CM_SLOTS + e] >= 0;
-        if (!dep) {
-            i++;
-        } else {
-            d.rank[c * CM_SLOTS + e] = rk++;
-            len--;
-            perm[i] = perm[len];
+    const int ntc = d.counts[1];
+    for (int i = blockIdx.x * 64 + threadIdx.x; i < ntc; i += gridDim.x * 64) {
+        const int c = d.tc[i];
+        const int n0 = d.natoms[c];
+        int na = d.arr_cnt[c];
+        unsigned fl = 0;
+        if (na > CM_ACAP) {
+            fl |= CM_FLAG_OVERFLOW;
+            na = CM_ACAP;
         }
+        int nlow = 0;
+        for (int k = 0; k < na; k++) nlow += (d.mv_slot[d.arr[c * CM_ACAP + k]] / CM_SLOTS) < c;
+        const int nhigh = na - nlow;
+        if (nlow > 0 && n0 + nlow >= CM_SLOTS) fl |= CM_FLAG_OVERFLOW;
+        int len = n0 + nlow;
+        if (len > 127) len = 127;
+        for (int k = 0; k < len; k++) perm[k] = (uint8_t)(k < n0 ? k : CM_SLOTS + (k - n0));
+        int p = 0, rk = 0;
+        while (p < len) {
+            const int e = perm[p];
+            const int o = c * CM_SLOTS + e;
+            if (e < CM_SLOTS && d.dest[o] >= 0) {
+                d.mv_rank[d.mvidx[o]] = rk++;
+                len--;
+                perm[p] = perm[len];
+            } else {
+                p++;
+            }
+        }
+        if (nhigh > 0 && len + nhigh >= CM_SLOTS) fl |= CM_FLAG_OVERFLOW;
+        const int keep = len < CM_SLOTS ? len : CM_SLOTS;
+        for (int k = 0; k < keep; k++) d.perm[c * CM_SLOTS + k] = perm[k];
+        d.meta[4 * c + 0] = len;
+        d.meta[4 * c + 1] = nlow;
+        d.meta[4 * c + 2] = nhigh;
+        d.meta[4 * c + 3] = n0;
+        if (fl) atomicOr(d.flags, fl);
     }
-    if (nhigh > 0 && len + nhigh >= CM_SLOTS) fl |= CM_FLAG_OVERFLOW;
-    const int keep = len < CM_SLOTS ? len : CM_SLOTS;
-    for (int k = 0; k < keep; k++) d.perm[c * CM_SLOTS + k] = perm[k];
-    d.meta[4 * c + 0] = len;
-    d.meta[4 * c + 1] = nlow;
-    d.meta[4 * c + 2] = nhigh;
-    d.meta[4 * c + 3] = n0;
-    if (fl) atomicOr(d.flags, fl);
 }
 
-__device__ static inline void copy_slot_to_next(const DevState &d, int from, int to, bool moved)
-{
-    d.id_n[to] = d.id[from];
-    d.itype_n[to] = d.itype[from];
-    if (moved) {
-        d.rx_n[to] = d.rnx[from];
-        d.ry_n[to] = d.rny[from];
-        d.rz_n[to] = d.rnz[from];
-    } else {
-        d.rx_n[to] = d.rx[from];
-        d.ry_n[to] = d.ry[from];
-        d.rz_n[to] = d.rz[from];
-    }
-    d.px_n[to] = d.px[from];
-    d.py_n[to] = d.py[from];
-    d.pz_n[to] = d.pz[from];
-}
-
-// Per cell: write the final list into the next buffers -- survivors and
-// low arrivals in the scanned arrangement, then high arrivals in
-// (source cell, rank) order.
+// Per touched cell (one wave each, lane = output slot): order the arrivals by
+// (source cell, rank), then write survivors and low arrivals in the scanned
+// arrangement followed by the high arrivals.  All lanes read before any lane
+// writes, so the cell is updated in place.  Clears the cell's pass state.
 __global__ __launch_bounds__(64) void k_rebox_gather(DevState d)
 {
-    __shared__ uint8_t sph[64][64];
-    const int c = blockIdx.x * 64 + threadIdx.x;
-    if (c >= d.ncells) return;
-    uint8_t *phpos = sph[threadIdx.x];
-    const int len = d.meta[4 * c + 0];
-    const int nhigh = d.meta[4 * c + 2];
-    const int keep = len < CM_SLOTS ? len : CM_SLOTS;
-    for (int k = 0; k < keep; k++) {
-        const int e = d.perm[c * CM_SLOTS + k];
-        if (e < CM_SLOTS) copy_slot_to_next(d, c * CM_SLOTS + e, c * CM_SLOTS + k, false);
-        else if (e - CM_SLOTS < CM_SLOTS) phpos[e - CM_SLOTS] = (uint8_t)k;
-    }
-    int src[26];
-    const int nsrc = source_cells(d, c, src);
-    int jl = 0, jh = 0;
-    for (int q = 0; q < nsrc; q++) {
-        const int b = src[q];
-        const int nb = d.natoms[b];
-        int last = -1;
-        for (;;) {
-            int best = -1, bestrank = 0x7fffffff;
-            for (int s = 0; s < nb; s++) {
-                const int o = b * CM_SLOTS + s;
-                if (d.dest[o] == c) {
-                    const int r = d.rank[o];
-                    if (r > last && r < bestrank) { bestrank = r; best = o; }
+    __shared__ int sm[2][CM_SLOTS];  // movers by low / high arrival order
+    const int L = threadIdx.x;
+    const bool skip = (*d.flags & CM_FLAG_MULTIHOP) != 0;
+    const int ntc = d.counts[1];
+    for (int i = blockIdx.x; i < ntc; i += gridDim.x) {
+        const int c = d.tc[i];
+        if (!skip) {
+            const int len = d.meta[4 * c + 0];
+            const int nhigh = d.meta[4 * c + 2];
+            int na = d.arr_cnt[c];
+            na = na < CM_ACAP ? na : CM_ACAP;
+            int m = -1, src = 0, rk = 0;
+            if (L < na) {
+                m = d.arr[c * CM_ACAP + L];
+                src = d.mv_slot[m] / CM_SLOTS;
+                rk = d.mv_rank[m];
+            }
+            // rank among arrivals of the same class, by (source cell, rank)
+            const bool low = src < c;
+            int pos = 0;
+            for (int u = 0; u < na; u++) {
+                const int su = __shfl(src, u, 64);
+                const int ru = __shfl(rk, u, 64);
+                const bool lu = su < c;
+                pos += (lu == low) && (su < src || (su == src && ru < rk));
+            }
+            if (L < na && pos < CM_SLOTS) sm[low ? 0 : 1][pos] = m;
+            __syncthreads();
+            // gather this lane's output atom
+            const int tot = len + nhigh;
+            const int nout = tot < CM_SLOTS ? tot : CM_SLOTS;
+            const int keep = len < CM_SLOTS ? len : CM_SLOTS;
+            int gid = 0, git = 0;
+            double gx = 0, gy = 0, gz = 0, gpx = 0, gpy = 0, gpz = 0;
+            if (L < nout) {
+                int mv = -1, own = -1;
+                if (L < keep) {
+                    const int e = d.perm[c * CM_SLOTS + L];
+                    if (e < CM_SLOTS) own = c * CM_SLOTS + e;
+                    else mv = sm[0][(e - CM_SLOTS) & (CM_SLOTS - 1)];
+                } else {
+                    mv = sm[1][(L - len) & (CM_SLOTS - 1)];
+                }
+                if (own >= 0) {
+                    gid = d.id[own];
+                    git = d.itype[own];
+                    gx = d.rx[own];
+                    gy = d.ry[own];
+                    gz = d.rz[own];
+                    gpx = d.px[own];
+                    gpy = d.py[own];
+                    gpz = d.pz[own];
+                } else if (mv >= 0) {
+                    gid = d.mv_id[mv];
+                    git = d.mv_itype[mv];
+                    gx = d.mv_rx[mv];
+                    gy = d.mv_ry[mv];
+                    gz = d.mv_rz[mv];
+                    gpx = d.mv_px[mv];
+                    gpy = d.mv_py[mv];
+                    gpz = d.mv_pz[mv];
                 }
             }
-            if (best < 0) break;
-            last = bestrank;
-            if (b < c) {
-                if (jl < CM_SLOTS && jl < len) {
-                    // placeholder jl sits at phpos[jl] unless it was cut off by overflow
-                    const int k = phpos[jl];
-                    if (k < keep) copy_slot_to_next(d, best, c * CM_SLOTS + k, true);
-                }
-                jl++;
-            } else {
-                const int k = len + jh;
-                if (k < CM_SLOTS) copy_slot_to_next(d, best, c * CM_SLOTS + k, true);
-                jh++;
+            __syncthreads();  // every lane has read its source before any slot is overwritten
+            if (L < nout) {
+                const int o = c * CM_SLOTS + L;
+                d.id[o] = gid;
+                d.itype[o] = git;
+                d.rx[o] = gx;
+                d.ry[o] = gy;
+                d.rz[o] = gz;
+                d.px[o] = gpx;
+                d.py[o] = gpy;
+                d.pz[o] = gpz;
             }
+            if (L == 0) d.natoms[c] = nout;
         }
-    }
-    int tot = len + nhigh;
-    d.natoms_n[c] = tot < CM_SLOTS ? tot : CM_SLOTS;
-}
-
-// Fallback for a pass in which an arrival would move again: the whole pass
-// is redone serially on the next buffers, exactly as reBoxAll does.
-__global__ void k_rebox_serial_copy(DevState d)
-{
-    if (!(*d.flags & CM_FLAG_MULTIHOP)) return;
-    const long ns = (long)d.ncells * CM_SLOTS;
-    for (long s = blockIdx.x * (long)blockDim.x + threadIdx.x; s < ns; s += (long)gridDim.x * blockDim.x) {
-        d.id_n[s] = d.id[s];
-        d.itype_n[s] = d.itype[s];
-        d.rx_n[s] = d.rx[s];
-        d.ry_n[s] = d.ry[s];
-        d.rz_n[s] = d.rz[s];
-        d.px_n[s] = d.px[s];
-        d.py_n[s] = d.py[s];
-        d.pz_n[s] = d.pz[s];
-        if ((s & (CM_SLOTS - 1)) == 0) d.natoms_n[s / CM_SLOTS] = d.natoms[s / CM_SLOTS];
+        if (L == 0) {
+            d.arr_cnt[c] = 0;
+            d.touched[c] = 0;
+        }
+        __syncthreads();
     }
 }
 
+// Serial pass, run only when k_move found an arrival that would move again:
+// reBoxAll exactly as the reference runs it, in place.  Always clears the
+// pass counters.
 __device__ static void serial_copy(const DevState &d, int from, int to)
 {
-    d.id_n[to] = d.id_n[from];
-    d.itype_n[to] = d.itype_n[from];
-    d.rx_n[to] = d.rx_n[from];
-    d.ry_n[to] = d.ry_n[from];
-    d.rz_n[to] = d.rz_n[from];
-    d.px_n[to] = d.px_n[from];
-    d.py_n[to] = d.py_n[from];
-    d.pz_n[to] = d.pz_n[from];
+    d.id[to] = d.id[from];
+    d.itype[to] = d.itype[from];
+    d.rx[to] = d.rx[from];
+    d.ry[to] = d.ry[from];
+    d.rz[to] = d.rz[from];
+    d.px[to] = d.px[from];
+    d.py[to] = d.py[from];
+    d.pz[to] = d.pz[from];
 }
 
 __global__ void k_rebox_serial(DevState d)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    d.counts[0] = 0;
+    d.counts[1] = 0;
     if (!(*d.flags & CM_FLAG_MULTIHOP)) return;
     unsigned fl = 0;
     for (int c = 0; c < d.ncells; c++) {
         int ix, iy, iz;
         cell_xyz(d, c, ix, iy, iz);
         int i = 0;
-        while (i < d.natoms_n[c]) {
+        while (i < d.natoms[c]) {
             const int o = c * CM_SLOTS + i;
-            double x = d.rx_n[o], y = d.ry_n[o], z = d.rz_n[o];
+            const double x = d.rx[o], y = d.ry[o], z = d.rz[o];
             int nx = ix, ny = iy, nz = iz;
             double ax = x, ay = y, az = z;
             if (x < 0.0) { nx = ix - 1; ax = x + d.bs0; } else if (x >= d.bs0) { nx = ix + 1; ax = x - d.bs0; }
@@ -271,19 +281,19 @@ __global__ void k_rebox_serial(DevState d)
                 i++;
                 continue;
             }
-            d.rx_n[o] = ax;
-            d.ry_n[o] = ay;
-            d.rz_n[o] = az;
-            const int nj = d.natoms_n[j];
+            d.rx[o] = ax;
+            d.ry[o] = ay;
+            d.rz[o] = az;
+            const int nj = d.natoms[j];
             if (nj >= CM_SLOTS) {  // no room: the reference aborts here
                 fl |= CM_FLAG_OVERFLOW;
                 i++;
                 continue;
             }
             serial_copy(d, o, j * CM_SLOTS + nj);
-            d.natoms_n[j] = nj + 1;
+            d.natoms[j] = nj + 1;
             if (nj + 1 >= CM_SLOTS) fl |= CM_FLAG_OVERFLOW;
-            const int ni = --d.natoms_n[c];
+            const int ni = --d.natoms[c];
             if (ni) serial_copy(d, c * CM_SLOTS + ni, o);
         }
     }
@@ -363,17 +373,19 @@ __global__ void k_fill_int(int *p, int v, long n)
 // ---------------------------------------------------------------------------
 // host launchers
 
-void cm_launch_move(const DevState &d, hipStream_t st, double dtk, double dtd, double rmass, int kick, int drift)
+void cm_launch_move(const DevState &d, hipStream_t st, double dtk, double dtd, double rmass, int kick, int drift,
+                    int rebin)
 {
-    hipLaunchKernelGGL(k_move, dim3(d.ncells), dim3(64), 0, st, d, dtk, dtd, rmass, kick, drift);
+    hipLaunchKernelGGL(k_move, dim3(d.ncells), dim3(64), 0, st, d, dtk, dtd, rmass, kick, drift, rebin);
 }
 
 void cm_launch_rebox(const DevState &d, hipStream_t st)
 {
-    const int nb = (d.ncells + 63) / 64;
+    int nb = (d.ncells + 63) / 64;
+    nb = nb < 1024 ? nb : 1024;
     hipLaunchKernelGGL(k_rebox_scan, dim3(nb), dim3(64), 0, st, d);
-    hipLaunchKernelGGL(k_rebox_gather, dim3(nb), dim3(64), 0, st, d);
-    hipLaunchKernelGGL(k_rebox_serial_copy, dim3(1024), dim3(256), 0, st, d);
+    const int ng = d.ncells < 4096 ? d.ncells : 4096;
+    hipLaunchKernelGGL(k_rebox_gather, dim3(ng), dim3(64), 0, st, d);
     hipLaunchKernelGGL(k_rebox_serial, dim3(1), dim3(64), 0, st, d);
 }
 

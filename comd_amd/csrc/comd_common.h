@@ -22,20 +22,29 @@ struct DevState {
     int periodic;
     double bs0, bs1, bs2;      // boxsize
     double L0, L1, L2;         // bounds
-    // current (cur) and next buffers of the moving state
-    int *natoms, *natoms_n;
-    int *id, *id_n;
-    int *itype, *itype_n;
-    double *rx, *ry, *rz, *rx_n, *ry_n, *rz_n;
-    double *px, *py, *pz, *px_n, *py_n, *pz_n;
+    // moving state (slot layout, updated in place by the rebin)
+    int *natoms;
+    int *id;
+    int *itype;
+    double *rx, *ry, *rz;
+    double *px, *py, *pz;
     // force outputs (slot layout)
     double *fx, *fy, *fz, *fe, *rho, *fi, *vir;
+    int *nnb;           // per slot: neighbours inside the cutoff (last density/LJ pass)
     // rebin scratch
     int *dest;          // per slot: destination cell, or -1 to stay
-    int *rank;          // per slot: departure rank within its source cell
-    double *rnx, *rny, *rnz;  // per slot: shifted coordinates of a moving atom
-    uint8_t *perm;      // per cell: survivor arrangement after the scan
+    int *mvidx;         // per slot: index of its mover record (departing atoms)
+    uint8_t *perm;      // per cell: arrangement after the cell's own scan
     int *meta;          // per cell: (len, nlow, nhigh, n0)
+    int *counts;        // [0] movers, [1] touched cells of the current pass
+    int *touched;       // per cell: 1 while listed in tc
+    int *tc;            // touched cells of the current pass
+    int *arr_cnt;       // per cell: arrivals of the current pass
+    int *arr;           // per cell: up to 64 arriving mover indices
+    // mover records: source slot, departure rank, and the atom with its
+    // shifted coordinates
+    int *mv_slot, *mv_rank, *mv_id, *mv_itype;
+    double *mv_rx, *mv_ry, *mv_rz, *mv_px, *mv_py, *mv_pz;
     unsigned *flags;    // error flag word
 };
 

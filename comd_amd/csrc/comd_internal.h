@@ -14,9 +14,10 @@ struct PotDev {
     double x0[3], xn[3], invdx[3];
     int n[3];
     // per-interval single-precision coefficients of phi and rho for the
-    // mixed path: 8 floats per interval (c0 c1 c2 d0 d1 pad pad pad)
-    const float *coef32[2];
-    float x0f[2], xnf[2], invdxf[2];
+    // mixed path: (c0, c1, c2, d0) and d1 (value c0 + t(c1 + t c2),
+    // derivative d0 + t d1)
+    const float4 *coef32[2];
+    const float *coef32d1[2];
     // Chebyshev: 6 series of cn coefficients (phi rho F dphi drho dF)
     const double *cheb;
     const float *cheb32;
@@ -29,11 +30,22 @@ struct PotDev {
 
 enum { CM_KIND_EAM1 = 0, CM_KIND_EAM3 = 1, CM_KIND_LJ = 2 };
 
-void cm_launch_move(const DevState &d, hipStream_t st, double dtk, double dtd, double rmass, int kick, int drift);
+// Launch shape of the force kernels (chosen by the tuner).
+struct ForceCfg {
+    int blocks;   // persistent grid size
+    int waves;    // waves per block (cells processed concurrently per block)
+    int tab_lds;  // stage the potential tables in LDS when they fit
+    int max_lds;  // LDS bytes allowed per block
+    int ncu;      // compute units of the device
+};
+int cm_force_lds_bytes(const PotDev &P, int kind, int mixed, int waves, int tab_lds);
+
+void cm_launch_move(const DevState &d, hipStream_t st, double dtk, double dtd, double rmass, int kick, int drift,
+                    int rebin);
 void cm_launch_rebox(const DevState &d, hipStream_t st);
 void cm_launch_bin(const DevState &d, hipStream_t st, int n, const double *xyz, const double *pxyz, int itype,
                    int *cnt, int *tmp);
 void cm_launch_fill_int(int *p, int v, long n, hipStream_t st);
-void cm_launch_force(const DevState &d, const PotDev &pot, int kind, int mixed, int nblocks, hipStream_t st);
+void cm_launch_force(const DevState &d, const PotDev &pot, int kind, int mixed, const ForceCfg &cfg, hipStream_t st);
 void cm_launch_reduce(const DevState &d, const PotDev &pot, double *partials, int nparts, hipStream_t st);
 void cm_launch_reduce_final(const double *partials, int nparts, double *out, hipStream_t st);

@@ -4,30 +4,77 @@
 //
 // Every kernel is full-shell: each atom sums over all of its neighbours, so
 // no atom is written by two waves and nothing needs atomics; the reference's
-// half-shell sums give the same totals (its pair loop with jBox >= iBox,
-// eam.c:494,609, counts each pair once and adds to both atoms).
+// half-shell loops (jBox >= iBox, eam.c:494,609) count each pair once and add
+// to both atoms, which gives the same sums.
 //
-// Work mapping.  One wave (one 64-thread block) owns one home cell at a
-// time (persistent grid).  The atoms of the 27 stencil cells (the reference's
-// order, src-flat/domains.c:419-426, duplicates kept as it keeps them when a
-// dimension has fewer than 3 cells) are staged in LDS shifted into the home
-// cell's frame.  The wave's lanes are split into S = 64/n_home groups; lane
-// (a, s) tests home atom a against halo atoms s, s+S, ...  Accepted
-// neighbours are pushed into the lane's own LDS queue, and the queues are
-// drained together, so the expensive pair arithmetic runs on dense lanes
-// rather than on the ~15% of candidates inside the cutoff.  The S partial
-// sums of each home atom are then added in a fixed order.
+// Work mapping.  A block of W waves shares the potential tables in LDS; each
+// wave owns one home cell at a time (persistent grid, cells of a z-slab per
+// XCD so that neighbouring cells are processed under the same L2).  The atoms
+// of the 27 stencil cells (the reference's order, src-flat/domains.c:419-426,
+// with the duplicates it keeps when a dimension has fewer than 3 cells) are
+// staged in the wave's LDS, shifted into the home cell's frame.  The lanes are
+// split into S = 64/n_home groups; lane (a, s) tests home atom a against halo
+// atoms s, s+S, ...  Accepted neighbours go into the lane's own LDS queue and
+// the queues are drained together, so the pair arithmetic runs on dense lanes
+// rather than on the ~15% of candidates inside the cutoff.  The S partial sums
+// of each home atom are added in a fixed order (deterministic results).
 #include "comd_internal.h"
 
-#define HMAX 384  // halo atoms staged per batch
-#define QCAP 32   // queue entries per lane
+#define HMAX 288    // halo atoms staged per batch
+#define QCAP 24     // queue entries per lane
+#define UNROLL 4    // candidates tested per lane between queue checks
+
+// ---------------------------------------------------------------------------
+// per-wave LDS layout (bytes): halo x,y,z (HMAX doubles each), halo F' (EAM
+// pass 3), queue (QCAP x 64 uint16), stencil cell ids / offsets / shifts.
+// The reduction scratch (7 x 64 values) reuses the halo area.
+struct WaveLds {
+    double *hx, *hy, *hz, *hf;
+    uint16_t *q;
+    int *cell, *off;
+    double *drb;
+};
+
+__host__ __device__ constexpr int wave_lds_bytes(int kind)
+{
+    return HMAX * 8 * (kind == CM_KIND_EAM3 ? 4 : 3) + QCAP * 64 * 2 + 27 * 4 + 28 * 4 + 27 * 3 * 8;
+}
+
+__device__ __forceinline__ WaveLds carve(char *base, int kind)
+{
+    WaveLds w;
+    w.hx = (double *)base;
+    w.hy = w.hx + HMAX;
+    w.hz = w.hy + HMAX;
+    double *p = w.hz + HMAX;
+    if (kind == CM_KIND_EAM3) {
+        w.hf = p;
+        p += HMAX;
+    } else {
+        w.hf = nullptr;
+    }
+    w.drb = p;
+    w.q = (uint16_t *)(w.drb + 27 * 3);
+    w.cell = (int *)(w.q + QCAP * 64);
+    w.off = w.cell + 27;
+    return w;
+}
+
+// Completes this wave's LDS writes before its lanes read them (the waves of
+// a block work independently, so no block barrier is used in the cell loop).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // ---------------------------------------------------------------------------
 // potential evaluation
 
 // 3-point quadratic value and interpolated central-difference derivative,
 // the formula of src-flat/eam.c:401-434 in double precision.  v points at
-// values[-1].
+// values[-1] (global memory or LDS).
 __device__ __forceinline__ void interp64(const double *v, double x0, double xn, double invdx, int n, double r,
                                          double &val, double &der)
 {
@@ -49,23 +96,23 @@ __device__ __forceinline__ void interp64(const double *v, double x0, double xn, 
     der = (i <= 0) ? 0.0 : 0.5 * (g0 + r * (g1 - g0)) * invdx;
 }
 
-// Same function from per-interval single-precision coefficients computed
-// in double precision on the host (value c0 + t(c1 + t c2), derivative
-// d0 + t d1), so no difference of nearby table values is taken in FP32.
-__device__ __forceinline__ void interp32(const float *coef, float x0, float xn, float invdx, int n, float r,
-                                         float &val, float &der)
+// Mixed path: the interval and its coordinate t come from r in double
+// precision; the polynomials use per-interval coefficients computed in double
+// precision on the host and stored as float (value c0 + t(c1 + t c2),
+// derivative d0 + t d1), so no difference of nearby table values is formed
+// in FP32.  c4 holds (c0, c1, c2, d0), d1 the derivative slope.
+__device__ __forceinline__ void interp_mixed(const float4 *c4, const float *d1, double x0, double xn, double invdx,
+                                             int n, double r, float &val, float &der)
 {
-    r = fminf(fmaxf(r, x0), xn);
-    const float u = (r - x0) * invdx;
-    const float fl = floorf(u);
-    int i = (int)fl;
+    r = r < x0 ? x0 : (r > xn ? xn : r);
+    const double u = (r - x0) * invdx;
+    int i = (int)u;
     i = i < n - 1 ? i : n - 1;
     i = i > 0 ? i : 0;
-    const float t = u - (float)i;
-    const float4 c = *reinterpret_cast<const float4 *>(coef + 8 * i);
-    const float d1 = coef[8 * i + 4];
+    const float t = (float)(u - (double)i);
+    const float4 c = c4[i];
     val = fmaf(t, fmaf(t, c.z, c.y), c.x);
-    der = fmaf(t, d1, c.w);
+    der = fmaf(t, d1[i], c.w);
 }
 
 // Clenshaw sum with the lower clamp only, src-flat/cheby.c:276-310.
@@ -97,6 +144,26 @@ __device__ __forceinline__ float cheb32(const float *c, int m, float a, float b,
     return fmaf(y, d, 0.5f * c[0] - dd);
 }
 
+// 1/sqrt(x) in double precision: hardware estimate and two Newton steps.
+__device__ __forceinline__ double rsqrt64(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+    double h = x * y;
+    double e = fma(-h, y, 1.0);
+    y = fma(0.5 * y, e, y);
+    h = x * y;
+    e = fma(-h, y, 1.0);
+    return fma(0.5 * y, e, y);
+}
+
+// 1/sqrt(x): single-precision estimate and one Newton step in double.
+__device__ __forceinline__ double rsqrt_mixed(double x)
+{
+    const double y = (double)__builtin_amdgcn_rsqf((float)x);
+    const double e = fma(-x * y, y, 1.0);
+    return fma(0.5 * y, e, y);
+}
+
 // ---------------------------------------------------------------------------
 // one accepted pair (home atom a, neighbour j): dx = r_a - r_j
 
@@ -104,119 +171,165 @@ struct Acc {
     double x, y, z, e, rho, vir;
 };
 
+struct Tabs {
+    const double *phi, *rho;         // FP64 values[-1..n+1]
+    const float4 *c4phi, *c4rho;     // mixed coefficients
+    const float *d1phi, *d1rho;
+};
+
 template <int KIND, int EVAL, int MIXED>
-__device__ __forceinline__ void pair_term(const PotDev &P, double dx, double dy, double dz, double r2, double fia,
-                                          double fij, Acc &A)
+__device__ __forceinline__ void pair_term(const PotDev &P, const Tabs &T, double dx, double dy, double dz, double r2,
+                                          double fia, double fij, Acc &A)
 {
-    if (!MIXED) {
-        if (KIND == CM_KIND_LJ) {
-            // src-flat/ljForce.c:196-210 (f holds -grad U, energy without 4 eps s6)
-            const double s6 = P.lj_s6;
-            const double r2i = 1.0 / r2;
-            const double r6 = r2i * r2i * r2i;
-            A.e += 0.5 * r6 * (s6 * r6 - 1.0);
-            const double fr = 4.0 * P.lj_eps * s6 * r6 * r2i * (12.0 * s6 * r6 - 6.0);
-            A.x += dx * fr;
-            A.y += dy * fr;
-            A.z += dz * fr;
-            A.vir += fr * dx * dx;
+    if (KIND == CM_KIND_LJ) {
+        // src-flat/ljForce.c:196-210 (f holds -grad U, energy without 4 eps s6)
+        const double s6 = P.lj_s6;
+        double r2i;
+        if (MIXED) {
+            const double y = rsqrt_mixed(r2);
+            r2i = y * y;
         } else {
-            const double r = sqrt(r2);
-            if (KIND == CM_KIND_EAM1) {
-                // src-flat/eam.c:521-545 (f holds +grad U)
-                double phiv, dphi, rhov, drho;
-                if (EVAL == COMD_EVAL_CHEB) {
-                    const int m = P.cn;
-                    phiv = cheb64(P.cheb + 0 * m, m, P.ca[0], P.cb[0], r);
-                    dphi = cheb64(P.cheb + 3 * m, m, P.ca[0], P.cb[0], r);
-                    rhov = cheb64(P.cheb + 1 * m, m, P.ca[1], P.cb[1], r);
-                } else {
-                    interp64(P.tab[0], P.x0[0], P.xn[0], P.invdx[0], P.n[0], r, phiv, dphi);
-                    interp64(P.tab[1], P.x0[1], P.xn[1], P.invdx[1], P.n[1], r, rhov, drho);
-                }
-                A.x += dphi * dx / r;
-                A.y += dphi * dy / r;
-                A.z += dphi * dz / r;
-                A.e += phiv / 2.0;
-                A.rho += rhov;
-            } else {
-                // src-flat/eam.c:633-649
-                double rhov, drho;
-                if (EVAL == COMD_EVAL_CHEB) drho = cheb64(P.cheb + 4 * P.cn, P.cn, P.ca[1], P.cb[1], r);
-                else interp64(P.tab[1], P.x0[1], P.xn[1], P.invdx[1], P.n[1], r, rhov, drho);
-                const double g = (fia + fij) * drho;
-                A.x += g * dx / r;
-                A.y += g * dy / r;
-                A.z += g * dz / r;
-                A.vir += g * dx / r * dx;
-            }
+            r2i = 1.0 / r2;
         }
+        const double r6 = r2i * r2i * r2i;
+        A.e += 0.5 * r6 * (s6 * r6 - 1.0);
+        const double fr = 4.0 * P.lj_eps * s6 * r6 * r2i * (12.0 * s6 * r6 - 6.0);
+        A.x += dx * fr;
+        A.y += dy * fr;
+        A.z += dz * fr;
+        A.vir += fr * dx * dx;
+        return;
+    }
+    const double y = MIXED ? rsqrt_mixed(r2) : rsqrt64(r2);  // 1/r
+    const double r = r2 * y;
+    if (KIND == CM_KIND_EAM1) {
+        // src-flat/eam.c:521-545 (f holds +grad U)
+        double phiv, dphi, rhov;
+        if (EVAL == COMD_EVAL_CHEB) {
+            const int m = P.cn;
+            if (MIXED) {
+                const float rf = (float)r;
+                const float a0 = (float)P.ca[0], b0 = (float)P.cb[0], a1 = (float)P.ca[1], b1 = (float)P.cb[1];
+                phiv = cheb32(P.cheb32 + 0 * m, m, a0, b0, rf);
+                dphi = cheb32(P.cheb32 + 3 * m, m, a0, b0, rf);
+                rhov = cheb32(P.cheb32 + 1 * m, m, a1, b1, rf);
+            } else {
+                phiv = cheb64(P.cheb + 0 * m, m, P.ca[0], P.cb[0], r);
+                dphi = cheb64(P.cheb + 3 * m, m, P.ca[0], P.cb[0], r);
+                rhov = cheb64(P.cheb + 1 * m, m, P.ca[1], P.cb[1], r);
+            }
+        } else if (MIXED) {
+            float pv, pd, rv, rd;
+            interp_mixed(T.c4phi, T.d1phi, P.x0[0], P.xn[0], P.invdx[0], P.n[0], r, pv, pd);
+            interp_mixed(T.c4rho, T.d1rho, P.x0[1], P.xn[1], P.invdx[1], P.n[1], r, rv, rd);
+            phiv = pv;
+            dphi = pd;
+            rhov = rv;
+        } else {
+            double drho;
+            interp64(T.phi, P.x0[0], P.xn[0], P.invdx[0], P.n[0], r, phiv, dphi);
+            interp64(T.rho, P.x0[1], P.xn[1], P.invdx[1], P.n[1], r, rhov, drho);
+        }
+        const double s = dphi * y;
+        A.x += s * dx;
+        A.y += s * dy;
+        A.z += s * dz;
+        A.e += 0.5 * phiv;
+        A.rho += rhov;
     } else {
-        // mixed precision: pair arithmetic in FP32, sums in FP64
-        const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
-        const float r2f = (float)r2;
-        if (KIND == CM_KIND_LJ) {
-            const float s6 = (float)P.lj_s6;
-            const float r2i = 1.0f / r2f;
-            const float r6 = r2i * r2i * r2i;
-            const float sr = s6 * r6;
-            A.e += (double)(0.5f * r6 * (sr - 1.0f));
-            const float fr = (float)(4.0 * P.lj_eps) * sr * r2i * (12.0f * sr - 6.0f);
-            A.x += (double)(fx * fr);
-            A.y += (double)(fy * fr);
-            A.z += (double)(fz * fr);
-            A.vir += (double)(fr * fx * fx);
+        // src-flat/eam.c:633-649
+        double drho;
+        if (EVAL == COMD_EVAL_CHEB) {
+            if (MIXED)
+                drho = cheb32(P.cheb32 + 4 * P.cn, P.cn, (float)P.ca[1], (float)P.cb[1], (float)r);
+            else
+                drho = cheb64(P.cheb + 4 * P.cn, P.cn, P.ca[1], P.cb[1], r);
+        } else if (MIXED) {
+            float rv, rd;
+            interp_mixed(T.c4rho, T.d1rho, P.x0[1], P.xn[1], P.invdx[1], P.n[1], r, rv, rd);
+            drho = rd;
         } else {
-            const float rinv = __builtin_amdgcn_rsqf(r2f);
-            const float r = r2f * rinv;
-            if (KIND == CM_KIND_EAM1) {
-                float phiv, dphi, rhov, drho;
-                if (EVAL == COMD_EVAL_CHEB) {
-                    const int m = P.cn;
-                    const float a0 = (float)P.ca[0], b0 = (float)P.cb[0], a1 = (float)P.ca[1], b1 = (float)P.cb[1];
-                    phiv = cheb32(P.cheb32 + 0 * m, m, a0, b0, r);
-                    dphi = cheb32(P.cheb32 + 3 * m, m, a0, b0, r);
-                    rhov = cheb32(P.cheb32 + 1 * m, m, a1, b1, r);
-                } else {
-                    interp32(P.coef32[0], P.x0f[0], P.xnf[0], P.invdxf[0], P.n[0], r, phiv, dphi);
-                    interp32(P.coef32[1], P.x0f[1], P.xnf[1], P.invdxf[1], P.n[1], r, rhov, drho);
-                }
-                const float s = dphi * rinv;
-                A.x += (double)(s * fx);
-                A.y += (double)(s * fy);
-                A.z += (double)(s * fz);
-                A.e += (double)(0.5f * phiv);
-                A.rho += (double)rhov;
-            } else {
-                float rhov, drho;
-                if (EVAL == COMD_EVAL_CHEB)
-                    drho = cheb32(P.cheb32 + 4 * P.cn, P.cn, (float)P.ca[1], (float)P.cb[1], r);
-                else interp32(P.coef32[1], P.x0f[1], P.xnf[1], P.invdxf[1], P.n[1], r, rhov, drho);
-                const float g = (float)(fia + fij) * drho * rinv;
-                A.x += (double)(g * fx);
-                A.y += (double)(g * fy);
-                A.z += (double)(g * fz);
-                A.vir += (double)(g * fx * fx);
-            }
+            double rhov;
+            interp64(T.rho, P.x0[1], P.xn[1], P.invdx[1], P.n[1], r, rhov, drho);
         }
+        const double g = (fia + fij) * drho * y;
+        A.x += g * dx;
+        A.y += g * dy;
+        A.z += g * dz;
+        A.vir += g * dx * dx;
     }
 }
 
 // ---------------------------------------------------------------------------
 
-template <int KIND, int EVAL, int MIXED>
-__global__ __launch_bounds__(64) void k_pair(DevState d, PotDev P)
+// TLDS: the phi/rho tables (FP64 values, or the mixed path's coefficients)
+// are staged in LDS; otherwise they are read from global memory.
+template <int KIND, int EVAL, int MIXED, int TLDS>
+__global__ __launch_bounds__(256) void k_pair(DevState d, PotDev P)
 {
-    __shared__ double hx[HMAX], hy[HMAX], hz[HMAX];
-    __shared__ double hf[KIND == CM_KIND_EAM3 ? HMAX : 1];
-    __shared__ uint16_t q[QCAP][64];
-    __shared__ int s_cell[27], s_off[28];
-    __shared__ double s_drb[27][3];
-    __shared__ double red[6][64];
+    extern __shared__ __align__(16) char lds[];
+    const int W = blockDim.x >> 6;
+    const int wv = threadIdx.x >> 6;
+    const int L = threadIdx.x & 63;
 
-    const int L = threadIdx.x;
+    // ---- potential tables for the block
+    Tabs T;
+    T.phi = P.tab[0];
+    T.rho = P.tab[1];
+    T.c4phi = P.coef32[0];
+    T.c4rho = P.coef32[1];
+    T.d1phi = P.coef32d1[0];
+    T.d1rho = P.coef32d1[1];
+    int tab_bytes = 0;
+    if (KIND != CM_KIND_LJ && EVAL == COMD_EVAL_TABLE && TLDS) {
+        // the density pass needs phi and rho, the embedding-force pass rho
+        const int nphi = KIND == CM_KIND_EAM1 ? P.n[0] : 0;
+        if (!MIXED) {
+            double *t0 = (double *)lds;
+            double *t1 = t0 + (nphi ? nphi + 3 : 0);
+            for (int i = threadIdx.x; i < (nphi ? nphi + 3 : 0); i += blockDim.x) t0[i] = P.tab[0][i];
+            for (int i = threadIdx.x; i < P.n[1] + 3; i += blockDim.x) t1[i] = P.tab[1][i];
+            if (nphi) T.phi = t0;
+            T.rho = t1;
+            tab_bytes = ((nphi ? nphi + 3 : 0) + P.n[1] + 3) * 8;
+        } else {
+            float4 *c0 = (float4 *)lds;
+            float4 *c1 = c0 + nphi;
+            float *e0 = (float *)(c1 + P.n[1]);
+            float *e1 = e0 + nphi;
+            for (int i = threadIdx.x; i < nphi; i += blockDim.x) {
+                c0[i] = P.coef32[0][i];
+                e0[i] = P.coef32d1[0][i];
+            }
+            for (int i = threadIdx.x; i < P.n[1]; i += blockDim.x) {
+                c1[i] = P.coef32[1][i];
+                e1[i] = P.coef32d1[1][i];
+            }
+            if (nphi) {
+                T.c4phi = c0;
+                T.d1phi = e0;
+            }
+            T.c4rho = c1;
+            T.d1rho = e1;
+            tab_bytes = (nphi + P.n[1]) * 20;
+        }
+        __syncthreads();
+    }
+    tab_bytes = (tab_bytes + 15) & ~15;
+    WaveLds w = carve(lds + tab_bytes + wv * wave_lds_bytes(KIND), KIND);
+
+    // ---- cells of this wave: a z-ordered slab per XCD (blocks are
+    // dispatched round-robin over the 8 XCDs)
+    const int nxcd = gridDim.x < 8 ? (int)gridDim.x : 8;
+    const int xcd = blockIdx.x % nxcd;
+    const int blocks_here = ((int)gridDim.x - xcd + nxcd - 1) / nxcd;
+    const int workers = blocks_here * W;
+    const int worker = (blockIdx.x / nxcd) * W + wv;
+    const int c_begin = (int)((long)d.ncells * xcd / nxcd);
+    const int c_end = (int)((long)d.ncells * (xcd + 1) / nxcd);
     const double rc2 = P.rc2;
-    for (int c = blockIdx.x; c < d.ncells; c += gridDim.x) {
+
+    for (int c = c_begin + worker; c < c_end; c += workers) {
         const int nh = d.natoms[c];
         if (nh == 0) continue;
 
@@ -225,15 +338,15 @@ __global__ __launch_bounds__(64) void k_pair(DevState d, PotDev P)
         int ne = 0;
         if (L < nent) {
             const int b = stencil_cell(d, c, L);
-            s_cell[L] = b;
+            w.cell[L] = b;
             if (b >= 0) {
                 ne = d.natoms[b];
                 double cix, ciy, ciz, cjx, cjy, cjz;
                 cell_corner(d, c, cix, ciy, ciz);
                 cell_corner(d, b, cjx, cjy, cjz);
-                s_drb[L][0] = image1(cix - cjx, d.L0, d.periodic);
-                s_drb[L][1] = image1(ciy - cjy, d.L1, d.periodic);
-                s_drb[L][2] = image1(ciz - cjz, d.L2, d.periodic);
+                w.drb[3 * L + 0] = image1(cix - cjx, d.L0, d.periodic);
+                w.drb[3 * L + 1] = image1(ciy - cjy, d.L1, d.periodic);
+                w.drb[3 * L + 2] = image1(ciz - cjz, d.L2, d.periodic);
             }
         }
         int incl = ne;
@@ -243,8 +356,8 @@ __global__ __launch_bounds__(64) void k_pair(DevState d, PotDev P)
         }
         const int H = __shfl(incl, 31, 64);
         const int nonempty = __popcll(__ballot(ne > 0));
-        if (L < nent) s_off[L] = incl - ne;
-        if (L == 0) s_off[nent] = H;
+        if (L < nent) w.off[L] = incl - ne;
+        if (L == 0) w.off[nent] = H;
 
         // ---- home atoms: lane (a, s)
         const int S = 64 / nh;
@@ -255,69 +368,96 @@ __global__ __launch_bounds__(64) void k_pair(DevState d, PotDev P)
         const double xa = d.rx[hs], ya = d.ry[hs], za = d.rz[hs];
         const double fia = KIND == CM_KIND_EAM3 ? d.fi[hs] : 0.0;
         Acc A = {0, 0, 0, 0, 0, 0};
+        int npair = 0;
+        wave_sync();
 
         for (int h0 = 0; h0 < H; h0 += HMAX) {
             const int hb = H - h0 < HMAX ? H - h0 : HMAX;
-            __syncthreads();
-            for (int h = h0 + L; h < h0 + hb; h += 64) {
+            // stage the batch: lane L takes halo atoms L, L+64, ...; its
+            // stencil entry only moves forward
+            {
                 int e = 0;
-                while (s_off[e + 1] <= h) e++;
-                const int sl = s_cell[e] * CM_SLOTS + (h - s_off[e]);
-                hx[h - h0] = d.rx[sl] - s_drb[e][0];
-                hy[h - h0] = d.ry[sl] - s_drb[e][1];
-                hz[h - h0] = d.rz[sl] - s_drb[e][2];
-                if (KIND == CM_KIND_EAM3) hf[h - h0] = d.fi[sl];
+                for (int h = h0 + L; h < h0 + hb; h += 64) {
+                    while (w.off[e + 1] <= h) e++;
+                    const int sl = w.cell[e] * CM_SLOTS + (h - w.off[e]);
+                    const double x = d.rx[sl], y = d.ry[sl], z = d.rz[sl];
+                    w.hx[h - h0] = x - w.drb[3 * e + 0];
+                    w.hy[h - h0] = y - w.drb[3 * e + 1];
+                    w.hz[h - h0] = z - w.drb[3 * e + 2];
+                    if (KIND == CM_KIND_EAM3) w.hf[h - h0] = d.fi[sl];
+                }
             }
-            __syncthreads();
+            wave_sync();
 
             int cnt = 0;
             auto drain = [&]() {
                 for (int k = 0; __ballot(k < cnt) != 0; k++) {
                     if (k < cnt) {
-                        const int j = q[k][L];
-                        const double dx = xa - hx[j], dy = ya - hy[j], dz = za - hz[j];
+                        const int j = w.q[k * 64 + L];
+                        const double dx = xa - w.hx[j], dy = ya - w.hy[j], dz = za - w.hz[j];
                         const double r2 = dx * dx + dy * dy + dz * dz;
-                        pair_term<KIND, EVAL, MIXED>(P, dx, dy, dz, r2, fia, KIND == CM_KIND_EAM3 ? hf[j] : 0.0, A);
+                        pair_term<KIND, EVAL, MIXED>(P, T, dx, dy, dz, r2, fia,
+                                                     KIND == CM_KIND_EAM3 ? w.hf[j] : 0.0, A);
+                        npair++;
                     }
                 }
                 cnt = 0;
             };
             const int trips = (hb + S - 1) / S;
-            for (int it = 0; it < trips; it++) {
-                const int j = sg + it * S;
-                if (act && j < hb) {
-                    const double dx = xa - hx[j], dy = ya - hy[j], dz = za - hz[j];
+            for (int it0 = 0; it0 < trips; it0 += UNROLL) {
+                bool acc[UNROLL];
+                int jj[UNROLL];
+#pragma unroll
+                for (int u = 0; u < UNROLL; u++) {
+                    const int j = sg + (it0 + u) * S;
+                    const bool valid = act && j < hb;
+                    const int jc = valid ? j : 0;
+                    const double dx = xa - w.hx[jc], dy = ya - w.hy[jc], dz = za - w.hz[jc];
                     const double r2 = dx * dx + dy * dy + dz * dz;
                     const bool in = KIND == CM_KIND_EAM3 ? (r2 < rc2) : (r2 <= rc2);
-                    if (in && r2 > 0.0) {
-                        q[cnt][L] = (uint16_t)j;
+                    acc[u] = valid && in && r2 > 0.0;
+                    jj[u] = j;
+                }
+#pragma unroll
+                for (int u = 0; u < UNROLL; u++)
+                    if (acc[u]) {
+                        w.q[cnt * 64 + L] = (uint16_t)jj[u];
                         cnt++;
                     }
+                if (__ballot(cnt > QCAP - UNROLL) != 0) {
+                    wave_sync();
+                    drain();
                 }
-                if (__ballot(cnt == QCAP) != 0) drain();
             }
+            wave_sync();
             drain();
+            wave_sync();
         }
 
         // ---- combine the S partial sums of each home atom, in s order
-        __syncthreads();
-        red[0][L] = act ? A.x : 0.0;
-        red[1][L] = act ? A.y : 0.0;
-        red[2][L] = act ? A.z : 0.0;
-        red[3][L] = act ? A.e : 0.0;
-        red[4][L] = act ? A.rho : 0.0;
-        red[5][L] = act ? A.vir : 0.0;
-        __syncthreads();
+        // (scratch in the halo area, free once the last batch is drained)
+        double *red = w.hx;
+        int *redn = (int *)(red + 6 * 64);
+        red[0 * 64 + L] = act ? A.x : 0.0;
+        red[1 * 64 + L] = act ? A.y : 0.0;
+        red[2 * 64 + L] = act ? A.z : 0.0;
+        red[3 * 64 + L] = act ? A.e : 0.0;
+        red[4 * 64 + L] = act ? A.rho : 0.0;
+        red[5 * 64 + L] = act ? A.vir : 0.0;
+        redn[L] = act ? npair : 0;
+        wave_sync();
         if (L < nh) {
             double sx = 0, sy = 0, sz = 0, se = 0, sr = 0, sv = 0;
+            int sn = 0;
             for (int s = 0; s < S; s++) {
                 const int l = L + s * nh;
-                sx += red[0][l];
-                sy += red[1][l];
-                sz += red[2][l];
-                se += red[3][l];
-                sr += red[4][l];
-                sv += red[5][l];
+                sx += red[0 * 64 + l];
+                sy += red[1 * 64 + l];
+                sz += red[2 * 64 + l];
+                se += red[3 * 64 + l];
+                sr += red[4 * 64 + l];
+                sv += red[5 * 64 + l];
+                sn += redn[l];
             }
             const int o = c * CM_SLOTS + L;
             if (KIND == CM_KIND_EAM1) {
@@ -335,6 +475,7 @@ __global__ __launch_bounds__(64) void k_pair(DevState d, PotDev P)
                 d.fe[o] = se + fv;
                 d.rho[o] = sr;
                 d.fi[o] = fp;
+                d.nnb[o] = sn;
             } else if (KIND == CM_KIND_EAM3) {
                 d.fx[o] += sx;
                 d.fy[o] += sy;
@@ -349,89 +490,62 @@ __global__ __launch_bounds__(64) void k_pair(DevState d, PotDev P)
                 d.fz[o] = sz;
                 d.fe[o] = se;
                 d.vir[o] = sv - (double)nonempty * (px * px / P.mass_amu);
+                d.nnb[o] = sn;
             }
         }
+        wave_sync();
     }
+}
+
+// LDS bytes of the tables staged by a block (0 when read from global memory).
+static int table_lds_bytes(const PotDev &P, int kind, int mixed)
+{
+    if (kind == CM_KIND_LJ || P.eval != COMD_EVAL_TABLE) return 0;
+    const int nphi = kind == CM_KIND_EAM1 ? P.n[0] : 0;
+    const int b = mixed ? (nphi + P.n[1]) * 20 : ((nphi ? nphi + 3 : 0) + P.n[1] + 3) * 8;
+    return (b + 15) & ~15;
+}
+
+template <int KIND, int EVAL, int MIXED>
+static void launch3(const DevState &d, const PotDev &P, const ForceCfg &cfg, hipStream_t st)
+{
+    const int W = cfg.waves;
+    const int tb = table_lds_bytes(P, KIND, MIXED);
+    const bool tlds = EVAL == COMD_EVAL_TABLE && KIND != CM_KIND_LJ && cfg.tab_lds &&
+                      tb + W * wave_lds_bytes(KIND) <= cfg.max_lds;
+    const size_t lds = (tlds ? tb : 0) + (size_t)W * wave_lds_bytes(KIND);
+    // the grid is persistent: never launch more blocks than can be resident
+    // at once, or the last ones would start only when others finish
+    auto kern = tlds ? k_pair<KIND, EVAL, MIXED, 1> : k_pair<KIND, EVAL, MIXED, 0>;
+    int per_cu = 0;
+    int blocks = cfg.blocks;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * W, lds) == hipSuccess && per_cu > 0 &&
+        cfg.ncu > 0 && blocks > per_cu * cfg.ncu)
+        blocks = per_cu * cfg.ncu;
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * W), lds, st, d, P);
 }
 
 template <int KIND, int EVAL>
-static void launch_kind(const DevState &d, const PotDev &P, int mixed, int nb, hipStream_t st)
+static void launch2(const DevState &d, const PotDev &P, int mixed, const ForceCfg &cfg, hipStream_t st)
 {
-    if (mixed) hipLaunchKernelGGL((k_pair<KIND, EVAL, 1>), dim3(nb), dim3(64), 0, st, d, P);
-    else hipLaunchKernelGGL((k_pair<KIND, EVAL, 0>), dim3(nb), dim3(64), 0, st, d, P);
+    if (mixed) launch3<KIND, EVAL, 1>(d, P, cfg, st);
+    else launch3<KIND, EVAL, 0>(d, P, cfg, st);
 }
 
-void cm_launch_force(const DevState &d, const PotDev &P, int kind, int mixed, int nblocks, hipStream_t st)
+int cm_force_lds_bytes(const PotDev &P, int kind, int mixed, int waves, int tab_lds)
 {
-    int nb = nblocks < d.ncells ? nblocks : d.ncells;
-    if (nb < 1) nb = 1;
+    return (tab_lds ? table_lds_bytes(P, kind, mixed) : 0) + waves * wave_lds_bytes(kind);
+}
+
+void cm_launch_force(const DevState &d, const PotDev &P, int kind, int mixed, const ForceCfg &cfg, hipStream_t st)
+{
     if (kind == CM_KIND_LJ) {
-        launch_kind<CM_KIND_LJ, COMD_EVAL_TABLE>(d, P, mixed, nb, st);
+        launch2<CM_KIND_LJ, COMD_EVAL_TABLE>(d, P, mixed, cfg, st);
     } else if (P.eval == COMD_EVAL_CHEB) {
-        if (kind == CM_KIND_EAM1) launch_kind<CM_KIND_EAM1, COMD_EVAL_CHEB>(d, P, mixed, nb, st);
-        else launch_kind<CM_KIND_EAM3, COMD_EVAL_CHEB>(d, P, mixed, nb, st);
+        if (kind == CM_KIND_EAM1) launch2<CM_KIND_EAM1, COMD_EVAL_CHEB>(d, P, mixed, cfg, st);
+        else launch2<CM_KIND_EAM3, COMD_EVAL_CHEB>(d, P, mixed, cfg, st);
     } else {
-        if (kind == CM_KIND_EAM1) launch_kind<CM_KIND_EAM1, COMD_EVAL_TABLE>(d, P, mixed, nb, st);
-        else launch_kind<CM_KIND_EAM3, COMD_EVAL_TABLE>(d, P, mixed, nb, st);
+        if (kind == CM_KIND_EAM1) launch2<CM_KIND_EAM1, COMD_EVAL_TABLE>(d, P, mixed, cfg, st);
+        else launch2<CM_KIND_EAM3, COMD_EVAL_TABLE>(d, P, mixed, cfg, st);
     }
-}
-
-// ---------------------------------------------------------------------------
-// deterministic two-stage reductions of per-atom energy, kinetic energy and
-// virial: one partial triple per block, then one block sums the partials.
-
-__global__ __launch_bounds__(256) void k_reduce(DevState d, double rmass, double *part)
-{
-    __shared__ double sh[3][256];
-    double pe = 0, ke = 0, vi = 0;
-    const long ns = (long)d.ncells * CM_SLOTS;
-    for (long s = blockIdx.x * 256L + threadIdx.x; s < ns; s += (long)gridDim.x * 256L) {
-        const int c = (int)(s / CM_SLOTS);
-        const int i = (int)(s % CM_SLOTS);
-        if (i < d.natoms[c]) {
-            pe += d.fe[s];
-            vi += d.vir[s];
-            const double px = d.px[s], py = d.py[s], pz = d.pz[s];
-            ke += (px * px + py * py + pz * pz) / (2.0 * rmass);
-        }
-    }
-    sh[0][threadIdx.x] = pe;
-    sh[1][threadIdx.x] = ke;
-    sh[2][threadIdx.x] = vi;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w)
-            for (int k = 0; k < 3; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0)
-        for (int k = 0; k < 3; k++) part[3 * blockIdx.x + k] = sh[k][0];
-}
-
-__global__ __launch_bounds__(256) void k_reduce_final(const double *part, int nparts, double *out)
-{
-    __shared__ double sh[3][256];
-    double a[3] = {0, 0, 0};
-    for (int b = threadIdx.x; b < nparts; b += 256)
-        for (int k = 0; k < 3; k++) a[k] += part[3 * b + k];
-    for (int k = 0; k < 3; k++) sh[k][threadIdx.x] = a[k];
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w)
-            for (int k = 0; k < 3; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0)
-        for (int k = 0; k < 3; k++) out[k] = sh[k][0];
-}
-
-void cm_launch_reduce(const DevState &d, const PotDev &P, double *partials, int nparts, hipStream_t st)
-{
-    const double rmass = 1822.83 * P.mass_amu;
-    hipLaunchKernelGGL(k_reduce, dim3(nparts), dim3(256), 0, st, d, rmass, partials);
-}
-
-void cm_launch_reduce_final(const double *partials, int nparts, double *out, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, st, partials, nparts, out);
 }

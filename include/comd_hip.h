@@ -120,6 +120,13 @@ int comd_force(comd_ctx *ctx);
  * the potential energy of that final force, as the reference returns s->e. */
 int comd_nsteps(comd_ctx *ctx, int n, double dt_seconds, double *pe_out);
 
+/* The loop of nTimeSteps without its trailing force call: n steps of
+ * half drift + rebin, force, kick, half drift + rebin.  Nothing is copied
+ * to the host and the stream is not synchronised; errors raised by the
+ * kernels are reported by the next synchronising call (comd_sync,
+ * comd_energy, comd_nsteps). */
+int comd_steps(comd_ctx *ctx, int n, double dt_seconds);
+
 /* The two halves of a step, for callers that drive the integrator
  * themselves: advancePositions (src-flat/timestep.c:76-108: r += (dt p)/m,
  * then reBoxAll) and advanceVelocity (timestep.c:62-74: p -= dt f), with dt
@@ -135,6 +142,21 @@ int comd_energy(comd_ctx *ctx, double *pe, double *ke, double *stress);
 
 int comd_set_precision(comd_ctx *ctx, int precision);
 int comd_set_eval(comd_ctx *ctx, int eval);
+
+/* Launch shape of the force kernels (no counterpart in the reference, whose
+ * force loops are serial): waves (home cells in flight) per block, persistent
+ * blocks per compute unit, and whether the potential tables are staged in
+ * LDS.  Results do not depend on these settings. */
+enum comd_tune_key {
+    COMD_TUNE_WAVES_PER_BLOCK = 1, /* 1..4, default 1 */
+    COMD_TUNE_BLOCKS_PER_CU = 2,   /* 1..64, default 8 */
+    COMD_TUNE_TABLES_IN_LDS = 3,   /* 0/1, default 1 */
+};
+int comd_tune(comd_ctx *ctx, int key, int value);
+
+/* Number of atom pairs inside the cutoff found by the last EAM density or
+ * LJ pass (each unordered pair once: the reference's half-pair count). */
+int comd_pair_count(comd_ctx *ctx, long long *half_pairs);
 
 /* Number of visible HIP devices and the name of the current one. */
 int comd_device_info(int *count, char *name, int name_len);
