@@ -114,6 +114,7 @@ def _load():
     hip.comd_steps.argtypes = [vp, C.c_int, C.c_double]
     hip.comd_pair_count.argtypes = [vp, C.POINTER(C.c_longlong)]
     hip.comd_tune.argtypes = [vp, C.c_int, C.c_int]
+    hip.comd_download_aux.argtypes = [vp, _ip, _dp]
     hip.comd_energy.argtypes = [vp, _dp, _dp, _dp]
     hip.comd_advance_positions.argtypes = [vp, C.c_double]
     hip.comd_advance_velocity.argtypes = [vp, C.c_double]
@@ -374,6 +375,13 @@ class Simulation:
             self.ctx, _ptr(out["natoms"], C.c_int), _ptr(out["id"], C.c_int), _ptr(out["itype"], C.c_int),
             _ptr(out["r"]), _ptr(out["p"]), _ptr(out["f"]), _ptr(out["rho"]), _ptr(out["fi"])))
         return out
+
+    def download_aux(self):
+        """Per-slot neighbour counts of the last density/LJ pass and per-atom virial."""
+        nnb = np.zeros(self.nslots, np.int32)
+        vir = np.zeros(self.nslots)
+        self._check(self._hip.comd_download_aux(self.ctx, _ptr(nnb, C.c_int), _ptr(vir)))
+        return nnb, vir
 
     def rebox(self):
         self._check(self._hip.comd_rebox(self.ctx))

@@ -51,7 +51,8 @@ struct comd_ctx {
     // potential
     PotDev pot;
     double *d_tab[3];
-    float *d_c4[2], *d_d1[2];
+    double *d_cf64[2];
+    float *d_cf32[2];
     double *d_cheb;
     float *d_cheb32;
     // force-kernel launch shape (comd_tune)
@@ -243,7 +244,7 @@ static void free_all(comd_ctx *c)
                     c->counts, c->touched, c->tc, c->arr_cnt, c->arr, c->mv_slot, c->mv_rank, c->mv_id,
                     c->mv_itype, c->mv_rx, c->mv_ry, c->mv_rz, c->mv_px, c->mv_py, c->mv_pz,
                     c->perm, c->flags, c->partials, c->red_out,
-                    c->d_tab[0], c->d_tab[1], c->d_tab[2], c->d_c4[0], c->d_c4[1], c->d_d1[0], c->d_d1[1],
+                    c->d_tab[0], c->d_tab[1], c->d_tab[2], c->d_cf64[0], c->d_cf64[1], c->d_cf32[0], c->d_cf32[1],
                     c->d_cheb, c->d_cheb32};
     for (void *p : ptrs)
         if (p) hipFree(p);
@@ -258,25 +259,28 @@ static void free_all(comd_ctx *c)
     if (c->stream) hipStreamDestroy(c->stream);
 }
 
-// Per-interval coefficients of the table formula (src-flat/eam.c:418-433),
-// evaluated in double precision and stored as float for the mixed path:
-// c4[i] = (c0, c1, c2, d0), d1[i], with value c0 + t(c1 + t c2) and
-// derivative d0 + t d1 on interval i (the derivative is 0 for i <= 0).
-static void table_coefs32(int n, double invdx, const double *vals /* values[-1] first */, std::vector<float> &c4,
-                          std::vector<float> &d1)
+// Per-interval coefficients of the table formula (src-flat/eam.c:418-433):
+// on interval i the value is c0 + t(c1 + t c2) and the derivative d0 + t d1
+// (0 for i <= 0), with c0 = v[i], c1 = (v[i+1]-v[i-1])/2,
+// c2 = (v[i+1]+v[i-1]-2v[i])/2, d0 = c1 invDx,
+// d1 = ((v[i+2]-v[i]) - (v[i+1]-v[i-1])) invDx / 2.  Layout per interval:
+// (d0, d1, c0, c1, c2, 0) in double, (d0, d1, c0, c1, c2, 0, 0, 0) in float.
+static void table_coefs(int n, double invdx, const double *vals /* values[-1] first */, std::vector<double> &c64,
+                        std::vector<float> &c32)
 {
-    c4.assign((size_t)4 * n, 0.0f);
-    d1.assign((size_t)n, 0.0f);
+    c64.assign((size_t)6 * n, 0.0);
+    c32.assign((size_t)8 * n, 0.0f);
     const double *v = vals + 1;
     for (int i = 0; i < n; i++) {
         const double vm = v[i - 1], v0 = v[i], v1 = v[i + 1], v2 = v[i + 2];
         const double g0 = v1 - vm, g1 = v2 - v0;
         const double A = v1 + vm - 2.0 * v0;
-        c4[4 * i + 0] = (float)v0;
-        c4[4 * i + 1] = (float)(0.5 * g0);
-        c4[4 * i + 2] = (float)(0.5 * A);
-        c4[4 * i + 3] = i <= 0 ? 0.0f : (float)(0.5 * g0 * invdx);
-        d1[i] = i <= 0 ? 0.0f : (float)(0.5 * (g1 - g0) * invdx);
+        const double k[5] = {i <= 0 ? 0.0 : 0.5 * g0 * invdx, i <= 0 ? 0.0 : 0.5 * (g1 - g0) * invdx, v0, 0.5 * g0,
+                             0.5 * A};
+        for (int q = 0; q < 5; q++) {
+            c64[6 * i + q] = k[q];
+            c32[8 * i + q] = (float)k[q];
+        }
     }
 }
 
@@ -391,17 +395,20 @@ int comd_create(const comd_params *p, comd_ctx **out)
             P.n[k] = n;
         }
         for (int k = 0; ok && k < 2; k++) {
-            std::vector<float> c4, d1;
-            table_coefs32(p->tab_n[k], p->tab_invdx[k], p->tab_values[k], c4, d1);
-            alloc((void **)&c->d_c4[k], c4.size() * 4);
-            alloc((void **)&c->d_d1[k], d1.size() * 4);
+            std::vector<double> c64;
+            std::vector<float> c32;
+            table_coefs(p->tab_n[k], p->tab_invdx[k], p->tab_values[k], c64, c32);
+            alloc((void **)&c->d_cf64[k], c64.size() * 8);
+            alloc((void **)&c->d_cf32[k], c32.size() * 4);
             if (!ok) break;
-            hipMemcpyAsync(c->d_c4[k], c4.data(), c4.size() * 4, hipMemcpyHostToDevice, c->stream);
-            hipMemcpyAsync(c->d_d1[k], d1.data(), d1.size() * 4, hipMemcpyHostToDevice, c->stream);
+            hipMemcpyAsync(c->d_cf64[k], c64.data(), c64.size() * 8, hipMemcpyHostToDevice, c->stream);
+            hipMemcpyAsync(c->d_cf32[k], c32.data(), c32.size() * 4, hipMemcpyHostToDevice, c->stream);
             hipStreamSynchronize(c->stream);
-            P.coef32[k] = (const float4 *)c->d_c4[k];
-            P.coef32d1[k] = c->d_d1[k];
+            P.cf64[k] = c->d_cf64[k];
+            P.cf32[k] = c->d_cf32[k];
         }
+        P.same_grid = p->tab_n[0] == p->tab_n[1] && p->tab_x0[0] == p->tab_x0[1] && p->tab_xn[0] == p->tab_xn[1] &&
+                      p->tab_invdx[0] == p->tab_invdx[1];
         if (ok && p->cheb_n >= 2 && p->cheb_coef) {
             const int m = p->cheb_n;
             alloc((void **)&c->d_cheb, (size_t)6 * m * 8);
@@ -720,9 +727,22 @@ int comd_tune(comd_ctx *c, int key, int value)
     case COMD_TUNE_TABLES_IN_LDS:
         c->tab_lds = value ? 1 : 0;
         return COMD_OK;
+    case COMD_TUNE_BALANCED_DRAIN:
+        c->pot.balanced = value ? 1 : 0;
+        return COMD_OK;
     default:
         return set_err(c, COMD_ERR_ARG, "unknown tuning key %d", key);
     }
+}
+
+int comd_download_aux(comd_ctx *c, int *nnb, double *vir)
+{
+    if (!c) return COMD_ERR_ARG;
+    const size_t S = (size_t)c->nslots;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (nnb) HIPCHK(c, hipMemcpy(nnb, c->nnb, S * 4, hipMemcpyDeviceToHost));
+    if (vir) HIPCHK(c, hipMemcpy(vir, c->vir, S * 8, hipMemcpyDeviceToHost));
+    return COMD_OK;
 }
 
 int comd_pair_count(comd_ctx *c, long long *half_pairs)

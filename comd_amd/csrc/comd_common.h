@@ -79,9 +79,14 @@ __device__ __forceinline__ int stencil_cell(const DevState &d, int c, int e)
     return cell_wrap(d, ix + ox, iy + oy, iz + oz);
 }
 
-// dcenter[c][k] = boxsize[k] * ix_k (src-flat/domains.c:191-197).
+// dcenter[c][k] = boxsize[k] * ix_k (src-flat/domains.c:191-197).  The
+// corner arithmetic is never contracted into fma, even in files compiled
+// with contraction: the shift of a cell against itself must be exactly 0
+// (fma(b, i, -b*i) is the rounding error of b*i, not 0), or an atom would
+// see itself as a neighbour at distance ~1e-16.
 __device__ __forceinline__ void cell_corner(const DevState &d, int c, double &x, double &y, double &z)
 {
+#pragma clang fp contract(off)
     int ix, iy, iz;
     cell_xyz(d, c, ix, iy, iz);
     x = d.bs0 * (double)ix;
@@ -93,9 +98,22 @@ __device__ __forceinline__ void cell_corner(const DevState &d, int c, double &x,
 // (src-flat/eam.c:495-502).
 __device__ __forceinline__ double image1(double dd, double L, int periodic)
 {
+#pragma clang fp contract(off)
     if (periodic) {
         if (dd < -0.5 * L) dd += L;
         else if (dd > 0.5 * L) dd -= L;
     }
     return dd;
+}
+
+// dcenter[c] - dcenter[b], minimum-imaged, uncontracted (0 for b == c).
+__device__ __forceinline__ void corner_shift(const DevState &d, int c, int b, double &sx, double &sy, double &sz)
+{
+#pragma clang fp contract(off)
+    double cx, cy, cz, bx, by, bz;
+    cell_corner(d, c, cx, cy, cz);
+    cell_corner(d, b, bx, by, bz);
+    sx = image1(cx - bx, d.L0, d.periodic);
+    sy = image1(cy - by, d.L1, d.periodic);
+    sz = image1(cz - bz, d.L2, d.periodic);
 }

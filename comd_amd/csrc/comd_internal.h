@@ -13,11 +13,13 @@ struct PotDev {
     const double *tab[3];
     double x0[3], xn[3], invdx[3];
     int n[3];
-    // per-interval single-precision coefficients of phi and rho for the
-    // mixed path: (c0, c1, c2, d0) and d1 (value c0 + t(c1 + t c2),
-    // derivative d0 + t d1)
-    const float4 *coef32[2];
-    const float *coef32d1[2];
+    // per-interval coefficients of phi and rho (value c0 + t(c1 + t c2),
+    // derivative d0 + t d1): (d0 d1 c0 c1 c2 0) in double, the same
+    // padded to 8 floats for the mixed path
+    const double *cf64[2];
+    const float *cf32[2];
+    int same_grid;  // phi and rho tables share x0, xn, invDx and n
+    int balanced;   // drain an atom's queues round-robin over its lanes
     // Chebyshev: 6 series of cn coefficients (phi rho F dphi drho dF)
     const double *cheb;
     const float *cheb32;

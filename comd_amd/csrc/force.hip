@@ -2,22 +2,27 @@
 // + embedding (passes 1 and 2 of src-flat/eam.c:438-660, fused), EAM
 // embedding force (pass 3), and Lennard-Jones (src-flat/ljForce.c:89-224).
 //
+// Compiled with floating-point contraction: the pair sums may differ from
+// the reference in the last bits (the parity bar is 1e-10 relative), unlike
+// the integrator and the binning, which round exactly as the reference does.
+//
 // Every kernel is full-shell: each atom sums over all of its neighbours, so
 // no atom is written by two waves and nothing needs atomics; the reference's
 // half-shell loops (jBox >= iBox, eam.c:494,609) count each pair once and add
 // to both atoms, which gives the same sums.
 //
-// Work mapping.  A block of W waves shares the potential tables in LDS; each
-// wave owns one home cell at a time (persistent grid, cells of a z-slab per
-// XCD so that neighbouring cells are processed under the same L2).  The atoms
-// of the 27 stencil cells (the reference's order, src-flat/domains.c:419-426,
-// with the duplicates it keeps when a dimension has fewer than 3 cells) are
-// staged in the wave's LDS, shifted into the home cell's frame.  The lanes are
-// split into S = 64/n_home groups; lane (a, s) tests home atom a against halo
-// atoms s, s+S, ...  Accepted neighbours go into the lane's own LDS queue and
-// the queues are drained together, so the pair arithmetic runs on dense lanes
-// rather than on the ~15% of candidates inside the cutoff.  The S partial sums
-// of each home atom are added in a fixed order (deterministic results).
+// Work mapping.  Each wave owns one home cell at a time (persistent grid,
+// cells of a z-slab per XCD so that neighbouring cells are processed under
+// the same L2).  The atoms of the 27 stencil cells (the reference's order,
+// src-flat/domains.c:419-426, with the duplicates it keeps when a dimension
+// has fewer than 3 cells) are staged in the wave's LDS, shifted into the home
+// cell's frame.  The lanes are split into S = 64/n_home groups; lane (a, s)
+// tests home atom a against halo atoms s, s+S, ... and pushes the accepted
+// ones into its own LDS queue.  The queues of one atom are then drained
+// round-robin by that atom's S lanes, so the pair arithmetic runs on dense,
+// balanced lanes rather than on the ~15% of candidates inside the cutoff.
+// The S partial sums of each home atom are added in a fixed order
+// (deterministic results).
 #include "comd_internal.h"
 
 #define HMAX 288    // halo atoms staged per batch
@@ -25,19 +30,19 @@
 #define UNROLL 4    // candidates tested per lane between queue checks
 
 // ---------------------------------------------------------------------------
-// per-wave LDS layout (bytes): halo x,y,z (HMAX doubles each), halo F' (EAM
-// pass 3), queue (QCAP x 64 uint16), stencil cell ids / offsets / shifts.
-// The reduction scratch (7 x 64 values) reuses the halo area.
+// per-wave LDS: halo x,y,z (HMAX doubles each), halo F' (EAM pass 3), queue
+// (QCAP x 64 uint16), per-lane queue counts, stencil cell ids / offsets /
+// shifts.  The reduction scratch (7 x 64 values) reuses the halo area.
 struct WaveLds {
     double *hx, *hy, *hz, *hf;
     uint16_t *q;
-    int *cell, *off;
+    int *qn, *cell, *off;
     double *drb;
 };
 
 __host__ __device__ constexpr int wave_lds_bytes(int kind)
 {
-    return HMAX * 8 * (kind == CM_KIND_EAM3 ? 4 : 3) + QCAP * 64 * 2 + 27 * 4 + 28 * 4 + 27 * 3 * 8;
+    return HMAX * 8 * (kind == CM_KIND_EAM3 ? 4 : 3) + 27 * 3 * 8 + QCAP * 64 * 2 + 64 * 4 + 27 * 4 + 28 * 4;
 }
 
 __device__ __forceinline__ WaveLds carve(char *base, int kind)
@@ -55,7 +60,8 @@ __device__ __forceinline__ WaveLds carve(char *base, int kind)
     }
     w.drb = p;
     w.q = (uint16_t *)(w.drb + 27 * 3);
-    w.cell = (int *)(w.q + QCAP * 64);
+    w.qn = (int *)(w.q + QCAP * 64);
+    w.cell = w.qn + 64;
     w.off = w.cell + 27;
     return w;
 }
@@ -73,21 +79,19 @@ __device__ __forceinline__ void wave_sync()
 // potential evaluation
 
 // 3-point quadratic value and interpolated central-difference derivative,
-// the formula of src-flat/eam.c:401-434 in double precision.  v points at
-// values[-1] (global memory or LDS).
+// the formula of src-flat/eam.c:401-434 in double precision (used once per
+// atom for F).  v points at values[-1].
 __device__ __forceinline__ void interp64(const double *v, double x0, double xn, double invdx, int n, double r,
                                          double &val, double &der)
 {
-    if (r < x0) r = x0;
-    else if (r > xn) r = xn;
+    r = fmin(fmax(r, x0), xn);
     r = (r - x0) * invdx;
     const double fl = floor(r);
     int i = (int)fl;
     r = r - fl;
     // r <= xn keeps i <= n-1 for a consistent table header; the clamp only
     // keeps a rounding or a bad header inside values[-1..n+1]
-    i = i < n - 1 ? i : n - 1;
-    i = i > 0 ? i : 0;
+    i = min(max(i, 0), n - 1);
     const double *w = v + 1 + i;
     const double vm = w[-1], v0 = w[0], v1 = w[1], v2 = w[2];
     const double g0 = v1 - vm;
@@ -96,24 +100,27 @@ __device__ __forceinline__ void interp64(const double *v, double x0, double xn, 
     der = (i <= 0) ? 0.0 : 0.5 * (g0 + r * (g1 - g0)) * invdx;
 }
 
-// Mixed path: the interval and its coordinate t come from r in double
-// precision; the polynomials use per-interval coefficients computed in double
-// precision on the host and stored as float (value c0 + t(c1 + t c2),
-// derivative d0 + t d1), so no difference of nearby table values is formed
-// in FP32.  c4 holds (c0, c1, c2, d0), d1 the derivative slope.
-__device__ __forceinline__ void interp_mixed(const float4 *c4, const float *d1, double x0, double xn, double invdx,
-                                             int n, double r, float &val, float &der)
+// Interval of the table formula that holds r (clamped to [x0, xn] as in
+// src-flat/eam.c:413-414) and the fraction t inside it.
+__device__ __forceinline__ int table_interval(double r, double x0, double xn, double invdx, int n, double &t)
 {
-    r = r < x0 ? x0 : (r > xn ? xn : r);
+    r = fmin(fmax(r, x0), xn);
     const double u = (r - x0) * invdx;
-    int i = (int)u;
-    i = i < n - 1 ? i : n - 1;
-    i = i > 0 ? i : 0;
-    const float t = (float)(u - (double)i);
-    const float4 c = c4[i];
-    val = fmaf(t, fmaf(t, c.z, c.y), c.x);
-    der = fmaf(t, d1[i], c.w);
+    const double fl = floor(u);
+    t = u - fl;
+    return min(max((int)fl, 0), n - 1);
 }
+
+// Table formula from per-interval coefficients (d0 d1 c0 c1 c2 0):
+// value c0 + t(c1 + t c2), derivative d0 + t d1.
+__device__ __forceinline__ double2 cf_der64(const double *cf, int i) { return ((const double2 *)(cf + 6 * i))[0]; }
+__device__ __forceinline__ double cf_val64(const double *cf, int i, double t)
+{
+    const double2 a = ((const double2 *)(cf + 6 * i))[1];
+    const double c2 = cf[6 * i + 4];
+    return fma(t, fma(t, c2, a.y), a.x);
+}
+__device__ __forceinline__ float4 cf_row32(const float *cf, int i) { return ((const float4 *)(cf + 8 * i))[0]; }
 
 // Clenshaw sum with the lower clamp only, src-flat/cheby.c:276-310.
 __device__ __forceinline__ double cheb64(const double *c, int m, double a, double b, double x)
@@ -148,11 +155,9 @@ __device__ __forceinline__ float cheb32(const float *c, int m, float a, float b,
 __device__ __forceinline__ double rsqrt64(double x)
 {
     double y = __builtin_amdgcn_rsq(x);
-    double h = x * y;
-    double e = fma(-h, y, 1.0);
+    double e = fma(-x * y, y, 1.0);
     y = fma(0.5 * y, e, y);
-    h = x * y;
-    e = fma(-h, y, 1.0);
+    e = fma(-x * y, y, 1.0);
     return fma(0.5 * y, e, y);
 }
 
@@ -171,15 +176,9 @@ struct Acc {
     double x, y, z, e, rho, vir;
 };
 
-struct Tabs {
-    const double *phi, *rho;         // FP64 values[-1..n+1]
-    const float4 *c4phi, *c4rho;     // mixed coefficients
-    const float *d1phi, *d1rho;
-};
-
 template <int KIND, int EVAL, int MIXED>
-__device__ __forceinline__ void pair_term(const PotDev &P, const Tabs &T, double dx, double dy, double dz, double r2,
-                                          double fia, double fij, Acc &A)
+__device__ __forceinline__ void pair_term(const PotDev &P, double dx, double dy, double dz, double r2, double fia,
+                                          double fij, Acc &A)
 {
     if (KIND == CM_KIND_LJ) {
         // src-flat/ljForce.c:196-210 (f holds -grad U, energy without 4 eps s6)
@@ -218,17 +217,27 @@ __device__ __forceinline__ void pair_term(const PotDev &P, const Tabs &T, double
                 dphi = cheb64(P.cheb + 3 * m, m, P.ca[0], P.cb[0], r);
                 rhov = cheb64(P.cheb + 1 * m, m, P.ca[1], P.cb[1], r);
             }
-        } else if (MIXED) {
-            float pv, pd, rv, rd;
-            interp_mixed(T.c4phi, T.d1phi, P.x0[0], P.xn[0], P.invdx[0], P.n[0], r, pv, pd);
-            interp_mixed(T.c4rho, T.d1rho, P.x0[1], P.xn[1], P.invdx[1], P.n[1], r, rv, rd);
-            phiv = pv;
-            dphi = pd;
-            rhov = rv;
         } else {
-            double drho;
-            interp64(T.phi, P.x0[0], P.xn[0], P.invdx[0], P.n[0], r, phiv, dphi);
-            interp64(T.rho, P.x0[1], P.xn[1], P.invdx[1], P.n[1], r, rhov, drho);
+            double t0, t1;
+            const int i0 = table_interval(r, P.x0[0], P.xn[0], P.invdx[0], P.n[0], t0);
+            int i1 = i0;
+            t1 = t0;
+            if (!P.same_grid) i1 = table_interval(r, P.x0[1], P.xn[1], P.invdx[1], P.n[1], t1);
+            if (MIXED) {
+                const float4 a = cf_row32(P.cf32[0], i0);
+                const float a2 = P.cf32[0][8 * i0 + 4];
+                const float4 b = cf_row32(P.cf32[1], i1);
+                const float b2 = P.cf32[1][8 * i1 + 4];
+                const float f0 = (float)t0, f1 = (float)t1;
+                phiv = fmaf(f0, fmaf(f0, a2, a.w), a.z);
+                dphi = fmaf(f0, a.y, a.x);
+                rhov = fmaf(f1, fmaf(f1, b2, b.w), b.z);
+            } else {
+                const double2 dd = cf_der64(P.cf64[0], i0);
+                phiv = cf_val64(P.cf64[0], i0, t0);
+                dphi = fma(t0, dd.y, dd.x);
+                rhov = cf_val64(P.cf64[1], i1, t1);
+            }
         }
         const double s = dphi * y;
         A.x += s * dx;
@@ -244,13 +253,16 @@ __device__ __forceinline__ void pair_term(const PotDev &P, const Tabs &T, double
                 drho = cheb32(P.cheb32 + 4 * P.cn, P.cn, (float)P.ca[1], (float)P.cb[1], (float)r);
             else
                 drho = cheb64(P.cheb + 4 * P.cn, P.cn, P.ca[1], P.cb[1], r);
-        } else if (MIXED) {
-            float rv, rd;
-            interp_mixed(T.c4rho, T.d1rho, P.x0[1], P.xn[1], P.invdx[1], P.n[1], r, rv, rd);
-            drho = rd;
         } else {
-            double rhov;
-            interp64(T.rho, P.x0[1], P.xn[1], P.invdx[1], P.n[1], r, rhov, drho);
+            double t1;
+            const int i1 = table_interval(r, P.x0[1], P.xn[1], P.invdx[1], P.n[1], t1);
+            if (MIXED) {
+                const float2 dd = ((const float2 *)(P.cf32[1] + 8 * i1))[0];
+                drho = fmaf((float)t1, dd.y, dd.x);
+            } else {
+                const double2 dd = cf_der64(P.cf64[1], i1);
+                drho = fma(t1, dd.y, dd.x);
+            }
         }
         const double g = (fia + fij) * drho * y;
         A.x += g * dx;
@@ -262,63 +274,16 @@ __device__ __forceinline__ void pair_term(const PotDev &P, const Tabs &T, double
 
 // ---------------------------------------------------------------------------
 
-// TLDS: the phi/rho tables (FP64 values, or the mixed path's coefficients)
-// are staged in LDS; otherwise they are read from global memory.
-template <int KIND, int EVAL, int MIXED, int TLDS>
+template <int KIND, int EVAL, int MIXED>
 __global__ __launch_bounds__(256) void k_pair(DevState d, PotDev P)
 {
     extern __shared__ __align__(16) char lds[];
     const int W = blockDim.x >> 6;
     const int wv = threadIdx.x >> 6;
     const int L = threadIdx.x & 63;
+    WaveLds w = carve(lds + wv * wave_lds_bytes(KIND), KIND);
 
-    // ---- potential tables for the block
-    Tabs T;
-    T.phi = P.tab[0];
-    T.rho = P.tab[1];
-    T.c4phi = P.coef32[0];
-    T.c4rho = P.coef32[1];
-    T.d1phi = P.coef32d1[0];
-    T.d1rho = P.coef32d1[1];
-    int tab_bytes = 0;
-    if (KIND != CM_KIND_LJ && EVAL == COMD_EVAL_TABLE && TLDS) {
-        // the density pass needs phi and rho, the embedding-force pass rho
-        const int nphi = KIND == CM_KIND_EAM1 ? P.n[0] : 0;
-        if (!MIXED) {
-            double *t0 = (double *)lds;
-            double *t1 = t0 + (nphi ? nphi + 3 : 0);
-            for (int i = threadIdx.x; i < (nphi ? nphi + 3 : 0); i += blockDim.x) t0[i] = P.tab[0][i];
-            for (int i = threadIdx.x; i < P.n[1] + 3; i += blockDim.x) t1[i] = P.tab[1][i];
-            if (nphi) T.phi = t0;
-            T.rho = t1;
-            tab_bytes = ((nphi ? nphi + 3 : 0) + P.n[1] + 3) * 8;
-        } else {
-            float4 *c0 = (float4 *)lds;
-            float4 *c1 = c0 + nphi;
-            float *e0 = (float *)(c1 + P.n[1]);
-            float *e1 = e0 + nphi;
-            for (int i = threadIdx.x; i < nphi; i += blockDim.x) {
-                c0[i] = P.coef32[0][i];
-                e0[i] = P.coef32d1[0][i];
-            }
-            for (int i = threadIdx.x; i < P.n[1]; i += blockDim.x) {
-                c1[i] = P.coef32[1][i];
-                e1[i] = P.coef32d1[1][i];
-            }
-            if (nphi) {
-                T.c4phi = c0;
-                T.d1phi = e0;
-            }
-            T.c4rho = c1;
-            T.d1rho = e1;
-            tab_bytes = (nphi + P.n[1]) * 20;
-        }
-        __syncthreads();
-    }
-    tab_bytes = (tab_bytes + 15) & ~15;
-    WaveLds w = carve(lds + tab_bytes + wv * wave_lds_bytes(KIND), KIND);
-
-    // ---- cells of this wave: a z-ordered slab per XCD (blocks are
+    // ---- cells of this wave: a slab of consecutive cells per XCD (blocks are
     // dispatched round-robin over the 8 XCDs)
     const int nxcd = gridDim.x < 8 ? (int)gridDim.x : 8;
     const int xcd = blockIdx.x % nxcd;
@@ -341,12 +306,11 @@ __global__ __launch_bounds__(256) void k_pair(DevState d, PotDev P)
             w.cell[L] = b;
             if (b >= 0) {
                 ne = d.natoms[b];
-                double cix, ciy, ciz, cjx, cjy, cjz;
-                cell_corner(d, c, cix, ciy, ciz);
-                cell_corner(d, b, cjx, cjy, cjz);
-                w.drb[3 * L + 0] = image1(cix - cjx, d.L0, d.periodic);
-                w.drb[3 * L + 1] = image1(ciy - cjy, d.L1, d.periodic);
-                w.drb[3 * L + 2] = image1(ciz - cjz, d.L2, d.periodic);
+                double sx, sy, sz;
+                corner_shift(d, c, b, sx, sy, sz);
+                w.drb[3 * L + 0] = sx;
+                w.drb[3 * L + 1] = sy;
+                w.drb[3 * L + 2] = sz;
             }
         }
         int incl = ne;
@@ -390,18 +354,48 @@ __global__ __launch_bounds__(256) void k_pair(DevState d, PotDev P)
             wave_sync();
 
             int cnt = 0;
+            // drain all queues: atom a's entries (the concatenation of the
+            // queues of lanes a, a+nh, ...) are taken round-robin by its lanes
             auto drain = [&]() {
-                for (int k = 0; __ballot(k < cnt) != 0; k++) {
-                    if (k < cnt) {
-                        const int j = w.q[k * 64 + L];
+                if (!P.balanced) {
+                    // each lane drains its own queue
+                    for (int k = 0; __ballot(k < cnt) != 0; k++) {
+                        if (k < cnt) {
+                            const int j = w.q[k * 64 + L];
+                            const double dx = xa - w.hx[j], dy = ya - w.hy[j], dz = za - w.hz[j];
+                            const double r2 = dx * dx + dy * dy + dz * dz;
+                            pair_term<KIND, EVAL, MIXED>(P, dx, dy, dz, r2, fia,
+                                                         KIND == CM_KIND_EAM3 ? w.hf[j] : 0.0, A);
+                            npair++;
+                        }
+                    }
+                    cnt = 0;
+                    return;
+                }
+                w.qn[L] = act ? cnt : 0;
+                wave_sync();
+                int tot = 0;
+                if (act)
+                    for (int s = 0; s < S; s++) tot += w.qn[a + s * nh];
+                int v = sg, sp = 0, base = 0;
+                while (__ballot(act && v < tot) != 0) {
+                    if (act && v < tot) {
+                        int cs = w.qn[a + sp * nh];
+                        while (v >= base + cs) {
+                            base += cs;
+                            sp++;
+                            cs = w.qn[a + sp * nh];
+                        }
+                        const int j = w.q[(v - base) * 64 + a + sp * nh];
                         const double dx = xa - w.hx[j], dy = ya - w.hy[j], dz = za - w.hz[j];
                         const double r2 = dx * dx + dy * dy + dz * dz;
-                        pair_term<KIND, EVAL, MIXED>(P, T, dx, dy, dz, r2, fia,
-                                                     KIND == CM_KIND_EAM3 ? w.hf[j] : 0.0, A);
+                        pair_term<KIND, EVAL, MIXED>(P, dx, dy, dz, r2, fia, KIND == CM_KIND_EAM3 ? w.hf[j] : 0.0, A);
                         npair++;
+                        v += S;
                     }
                 }
                 cnt = 0;
+                wave_sync();
             };
             const int trips = (hb + S - 1) / S;
             for (int it0 = 0; it0 < trips; it0 += UNROLL) {
@@ -424,14 +418,9 @@ __global__ __launch_bounds__(256) void k_pair(DevState d, PotDev P)
                         w.q[cnt * 64 + L] = (uint16_t)jj[u];
                         cnt++;
                     }
-                if (__ballot(cnt > QCAP - UNROLL) != 0) {
-                    wave_sync();
-                    drain();
-                }
+                if (__ballot(cnt > QCAP - UNROLL) != 0) drain();
             }
-            wave_sync();
             drain();
-            wave_sync();
         }
 
         // ---- combine the S partial sums of each home atom, in s order
@@ -497,26 +486,14 @@ __global__ __launch_bounds__(256) void k_pair(DevState d, PotDev P)
     }
 }
 
-// LDS bytes of the tables staged by a block (0 when read from global memory).
-static int table_lds_bytes(const PotDev &P, int kind, int mixed)
-{
-    if (kind == CM_KIND_LJ || P.eval != COMD_EVAL_TABLE) return 0;
-    const int nphi = kind == CM_KIND_EAM1 ? P.n[0] : 0;
-    const int b = mixed ? (nphi + P.n[1]) * 20 : ((nphi ? nphi + 3 : 0) + P.n[1] + 3) * 8;
-    return (b + 15) & ~15;
-}
-
 template <int KIND, int EVAL, int MIXED>
 static void launch3(const DevState &d, const PotDev &P, const ForceCfg &cfg, hipStream_t st)
 {
     const int W = cfg.waves;
-    const int tb = table_lds_bytes(P, KIND, MIXED);
-    const bool tlds = EVAL == COMD_EVAL_TABLE && KIND != CM_KIND_LJ && cfg.tab_lds &&
-                      tb + W * wave_lds_bytes(KIND) <= cfg.max_lds;
-    const size_t lds = (tlds ? tb : 0) + (size_t)W * wave_lds_bytes(KIND);
+    const size_t lds = (size_t)W * wave_lds_bytes(KIND);
     // the grid is persistent: never launch more blocks than can be resident
     // at once, or the last ones would start only when others finish
-    auto kern = tlds ? k_pair<KIND, EVAL, MIXED, 1> : k_pair<KIND, EVAL, MIXED, 0>;
+    auto kern = k_pair<KIND, EVAL, MIXED>;
     int per_cu = 0;
     int blocks = cfg.blocks;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * W, lds) == hipSuccess && per_cu > 0 &&
@@ -534,7 +511,10 @@ static void launch2(const DevState &d, const PotDev &P, int mixed, const ForceCf
 
 int cm_force_lds_bytes(const PotDev &P, int kind, int mixed, int waves, int tab_lds)
 {
-    return (tab_lds ? table_lds_bytes(P, kind, mixed) : 0) + waves * wave_lds_bytes(kind);
+    (void)P;
+    (void)mixed;
+    (void)tab_lds;
+    return waves * wave_lds_bytes(kind);
 }
 
 void cm_launch_force(const DevState &d, const PotDev &P, int kind, int mixed, const ForceCfg &cfg, hipStream_t st)

@@ -151,12 +151,18 @@ enum comd_tune_key {
     COMD_TUNE_WAVES_PER_BLOCK = 1, /* 1..4, default 1 */
     COMD_TUNE_BLOCKS_PER_CU = 2,   /* 1..64, default 8 */
     COMD_TUNE_TABLES_IN_LDS = 3,   /* 0/1, default 1 */
+    COMD_TUNE_BALANCED_DRAIN = 4,  /* 0/1: share an atom's accepted pairs among its lanes */
 };
 int comd_tune(comd_ctx *ctx, int key, int value);
 
 /* Number of atom pairs inside the cutoff found by the last EAM density or
  * LJ pass (each unordered pair once: the reference's half-pair count). */
 int comd_pair_count(comd_ctx *ctx, long long *half_pairs);
+
+/* Per-slot results beyond the simflat_t arrays: the number of neighbours
+ * inside the cutoff of the last EAM density or LJ pass, and the per-atom
+ * share of the virial "stress" sum (before division by the volume). */
+int comd_download_aux(comd_ctx *ctx, int *nnb, double *vir);
 
 /* Number of visible HIP devices and the name of the current one. */
 int comd_device_info(int *count, char *name, int name_len);

@@ -210,6 +210,60 @@ def test_lj_force_fp64(gpu, n, temp, seed):
     assert abs(st - oi["stress"]) <= 1e-10 * abs(oi["stress"])
 
 
+def brute_neighbour_counts(xyz_slot_order, bounds, rc):
+    X = np.asarray(xyz_slot_order)
+    L = np.asarray(bounds)
+    d = X[:, None, :] - X[None, :, :]
+    d -= L * np.round(d / L)
+    r2 = (d ** 2).sum(-1)
+    np.fill_diagonal(r2, np.inf)
+    return (r2 <= rc * rc).sum(1)
+
+
+@pytest.mark.parametrize("kind,n,disp,seed", [("eam", 6, 0.05, 1), ("eam", 7, 0.1, 3), ("eam", 10, 0.1, 2),
+                                              ("lj", 5, 0.05, 4), ("lj", 6, 0.05, 5), ("lj", 8, 0.05, 9)])
+def test_neighbour_counts_exact(gpu, cu500, kind, n, disp, seed):
+    """Per-atom number of neighbours inside the cutoff (an integer) equals a
+    brute-force minimum-image count, for several cell occupancies."""
+    pot = cu500 if kind == "eam" else CA.copper_lj()
+    lat = 3.615 if kind == "eam" else pot.lat
+    xyz, p, bounds = seeded_atoms(n, lat, pot.mass, temp=300.0, disp=disp, seed=seed)
+    sim = CA.Simulation(pot, bounds, defgrad=0.0)
+    assert min(sim.nbx) >= 3
+    sim.bin_atoms(xyz, p)
+    sim.rebox()
+    sim.force()
+    st = sim.download()
+    nnb, _ = sim.download_aux()
+    m = (np.arange(64)[None, :] < st["natoms"][:, None]).reshape(-1)
+    ref = brute_neighbour_counts(xyz[st["id"][m]], bounds, pot.cutoff)
+    bad = np.flatnonzero(nnb[m] != ref)
+    print(f"[{kind} n={n}] atoms={m.sum()} occupancy {np.bincount(st['natoms']).nonzero()[0]} mismatches {len(bad)}",
+          flush=True)
+    assert len(bad) == 0
+    assert sim.pair_count() == ref.sum() // 2
+
+
+@pytest.mark.parametrize("n", [2, 3])
+@pytest.mark.parametrize("evaluation", ["table", "cheby"])
+def test_small_grids_match_reference_semantics(gpu, cu500, ocu500, n, evaluation):
+    """Boxes with 1 or 2 cells per dimension: a single cell sees only itself
+    (src-flat/domains.c:406-410); with 2 cells the 27-cell stencil repeats
+    cells and the reference counts those pairs repeatedly (domains.c:419-426).
+    Both are reproduced."""
+    ocu500.set_eval(evaluation == "cheby")
+    xyz, p, bounds = seeded_atoms(n, 3.615, 63.55, temp=300.0, disp=0.05, seed=7)
+    sim, osim = make_pair(cu500, ocu500, xyz, p, bounds, evaluation=evaluation)
+    assert max(sim.nbx) <= 2
+    sim.rebox()
+    osim.rebox()
+    sim.force()
+    osim.force()
+    check_forces(sim.download(), osim.state(), FP64_TOL, f"small-grid n={n} {evaluation}")
+    assert abs(sim.energy()[0] - osim.info()["e"]) <= 1e-12 * abs(osim.info()["e"])
+    ocu500.set_eval(False)
+
+
 # ---------------------------------------------------------------------------
 # timestepping: per step from a shared state
 
