@@ -88,6 +88,21 @@ def run_precision(sim, precision, steps, warmup, n_atoms):
     )
 
 
+def measured_traffic(kname):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 counter
+    summary (FETCH_SIZE and WRITE_SIZE runs, scaled by a copy kernel of known
+    size); (None, reason) when it holds no line for the kernel."""
+    path = os.path.join(ROOT, "profiles", "k_pair_eam_cu_fcc80_hbm_traffic.txt")
+    try:
+        for line in open(path):
+            m = re.match(r"traffic_bytes_per_launch (.+?) = ([0-9]+)", line.strip())
+            if m and m.group(1) == kname:
+                return float(m.group(2)), os.path.relpath(path, ROOT)
+    except OSError:
+        pass
+    return None, "no counter summary for this kernel"
+
+
 def cpu_baseline(n, steps):
     """Compiled reference (oracle/_ref/ref_table, built from the reference's
     own sources) on the same FCC lattice, one thread; falls back to the
@@ -147,14 +162,22 @@ def main():
 
     mixed = res["mixed"]
     fp64 = res["fp64"]
-    # dominant kernel: the density pass (pair energy, pair force, density,
-    # embedding) of the mixed path, timed with HIP events on its stream
-    k = CA.K_DENSITY
+    # dominant kernel of the mixed path (the force pass with the larger
+    # time), timed with HIP events on the stream it runs on
+    passes = {
+        CA.K_DENSITY: ("k_pair<0, 0, 1>", "density + embedding pass (src-flat/eam.c:484-594), table, mixed",
+                       lambda ph: (FLOP_PAIR_DENSITY * ph + FLOP_ATOM_EMBED) * n_atoms),
+        CA.K_EMBED_FORCE: ("k_pair<1, 0, 1>", "embedding-force pass (src-flat/eam.c:595-653), table, mixed",
+                           lambda ph: FLOP_PAIR_EMBED_FORCE * ph * n_atoms),
+    }
+    k = max(passes, key=lambda c: mixed["kernel_ms"][c])
+    kname, kdesc, kflop = passes[k]
     avg_s = mixed["kernel_ms"][k] / max(mixed["launches"][k], 1) * 1e-3
-    flop_launch = (FLOP_PAIR_DENSITY * mixed["p_half"] + FLOP_ATOM_EMBED) * n_atoms
+    flop_launch = kflop(mixed["p_half"])
     achieved = flop_launch / avg_s / 1e12
+    traffic, traffic_src = measured_traffic(kname)
     k64 = fp64["kernel_ms"][k] / max(fp64["launches"][k], 1) * 1e-3
-    flop64 = (FLOP_PAIR_DENSITY * fp64["p_half"] + FLOP_ATOM_EMBED) * n_atoms
+    flop64 = kflop(fp64["p_half"])
 
     f_alg = (FLOP_PAIR_DENSITY + FLOP_PAIR_EMBED_FORCE) * mixed["p_half"] + FLOP_ATOM_EMBED + FLOP_ATOM_VERLET
     ceil_mixed = min(PEAK_FP32_TFLOPS * 1e12 / f_alg, PEAK_HBM_GBS * 1e9 / BYTES_ATOM_STEP)
@@ -190,20 +213,23 @@ def main():
         "fp64": {
             "value": fp64["atom_steps_per_s"],
             "ms_per_step": fp64["ms_per_step"],
-            "density_kernel_ms": k64 * 1e3,
-            "density_kernel_tflops": flop64 / k64 / 1e12,
+            "dominant_kernel_ms": k64 * 1e3,
+            "dominant_kernel_tflops": flop64 / k64 / 1e12,
+            "dominant_kernel_frac_of_fp64_peak": flop64 / k64 / 1e12 / PEAK_FP64_TFLOPS,
             "step_roofline_frac": fp64["atom_steps_per_s"] / ceil_fp64,
             "energy_drift_per_atom_ha": (fp64["e1"] - fp64["e0"]) / n_atoms,
         },
         "mixed_over_fp64": mixed["atom_steps_per_s"] / fp64["atom_steps_per_s"],
         "roofline": {
             "bound": "valu",
-            "kernel": "k_pair<EAM density pass, table, mixed>",
+            "kernel": f"{kname}: {kdesc}",
             "achieved": achieved,
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": traffic_src,
+            "hbm_frac": (traffic / avg_s / 1e9 / PEAK_HBM_GBS) if traffic else None,
             "avg_launch_ms": avg_s * 1e3,
             "flop_per_launch": flop_launch,
             "p_half": mixed["p_half"],

@@ -152,6 +152,8 @@ void orc_sim_get(const orc_sim *s, int *natoms, int *id, int *itype, double *r, 
 void orc_sim_set_rp(orc_sim *s, const double *r, const double *p);
 void orc_pot_get_cheb(const orc_pot *p, double *out /* 6*n */, int *n);
 void orc_pot_get_table(const orc_pot *p, int which, int *n, double *x0xninv /* 3 */, double *vals /* n+3 */);
+void orc_pot_eval(const orc_pot *p, int which, double x, double *val, double *der);
+void orc_pot_set_cheb3(orc_pot *p, const double *coef /* 3*n */, int n);
 
 #ifdef __cplusplus
 }

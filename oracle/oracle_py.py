@@ -71,6 +71,9 @@ def lib():
         L.orc_sim_set_state.argtypes = [vp, _ip, _ip, _ip, _dp, _dp]
         L.orc_eam_force.argtypes = [vp]
         L.orc_lj_force.argtypes = [vp]
+        L.orc_pot_eval.argtypes = [vp, C.c_int, C.c_double, _dp, _dp]
+        L.orc_pot_set_cheb3.argtypes = [vp, _dp, C.c_int]
+        L.orc_neighbor_boxes.restype = C.c_int
         _LIB = L
     return _LIB
 
@@ -129,6 +132,19 @@ class OraclePot:
         vals = np.zeros(n.value + 3)
         lib().orc_pot_get_table(self.h, which, C.byref(n), _p(hdr), _p(vals))
         return dict(n=n.value, x0=hdr[0], xn=hdr[1], invdx=hdr[2], values=vals)
+
+    def eval(self, which, x):
+        """(value, derivative) of phi (0), rho (1) or F (2) at x in the
+        current evaluation mode."""
+        v, d = C.c_double(), C.c_double()
+        lib().orc_pot_eval(self.h, which, float(x), C.byref(v), C.byref(d))
+        return v.value, d.value
+
+    def set_cheb3(self, coef):
+        """Replace the phi, rho, F value series (3 x n) and rebuild the
+        derivative series."""
+        c = np.ascontiguousarray(coef, dtype=np.float64).reshape(3, -1)
+        lib().orc_pot_set_cheb3(self.h, _p(c), c.shape[1])
 
     def __del__(self):
         try:

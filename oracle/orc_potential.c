@@ -278,3 +278,34 @@ void orc_pot_get_table(const orc_pot *p, int which, int *n, double *hdr, double 
     hdr[2] = t->invDx;
     if (vals) memcpy(vals, t->store, (size_t)(t->n + 3) * 8);
 }
+
+/* Value and derivative of function `which` (0 phi, 1 rho, 2 F) at x in the
+ * potential's current evaluation mode: the table formula
+ * (src-flat/eam.c:401-434) or the value and derivative series
+ * (src-flat/eam.c:523-527,582-584,637-639). */
+void orc_pot_eval(const orc_pot *p, int which, double x, double *val, double *der)
+{
+    if (p->eval == ORC_EVAL_CHEB) {
+        const orc_cheb *cv[3] = {&p->cphi, &p->crho, &p->cf};
+        const orc_cheb *cd[3] = {&p->cdphi, &p->cdrho, &p->cdf};
+        *val = orc_cheb_eval(cv[which], x);
+        *der = orc_cheb_eval(cd[which], x);
+    } else {
+        const orc_table *t = which == 0 ? &p->phi : which == 1 ? &p->rho : &p->f;
+        orc_table_interp(t, x, val, der);
+    }
+}
+
+/* Replaces the value series of phi, rho and F by coef[0..3n) (in that
+ * order, on the fitted intervals) and rebuilds the derivative series from
+ * them (src-flat/cheby.c:219-270). */
+void orc_pot_set_cheb3(orc_pot *p, const double *coef, int n)
+{
+    orc_cheb *cs[3] = {&p->cphi, &p->crho, &p->cf};
+    orc_cheb *ds[3] = {&p->cdphi, &p->cdrho, &p->cdf};
+    for (int k = 0; k < 3; k++) {
+        cs[k]->n = n;
+        memcpy(cs[k]->c, coef + k * n, (size_t)n * 8);
+        orc_cheb_deriv(cs[k], ds[k]);
+    }
+}

@@ -1,0 +1,178 @@
+"""Writes the golden fixtures under tests/golden/ from the compiled reference.
+
+Run it in a checkout that sits next to the reference tree
+(/root/reference/src-flat): it builds oracle/_ref/ref_table and
+oracle/_ref/ref_cheby from the reference's own sources (the `ref` target of
+oracle/Makefile), writes the seeded inputs in the reference's gz text format
+(tests/golden/data/*.inp.gz, %.17g, so they read back exactly), runs the two
+binaries through oracle/ref_harness.c and stores, per case, the state in
+cell/slot order after the initial force and after each further
+nTimeSteps(K) call as a compressed .npz:
+
+    s{m}_natoms, s{m}_id, s{m}_r, s{m}_p, s{m}_f, s{m}_rho, s{m}_fi,
+    s{m}_e, s{m}_stress          (m = 0 .. dumps)
+    nbx, bounds, boxsize, steps, dumps, cmd
+
+plus cu500_pot.npz with the table headers, guard slots and the 6 x 32
+Chebyshev coefficients exactly as the reference computed them.  The
+fixtures are data; the tests compare the oracle (and through it the HIP
+path) with them without the reference being present.
+
+Usage: python tests/golden/make_golden.py
+"""
+import os
+import shlex
+import struct
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import comd_amd as CA  # noqa: E402  (seeded inputs and the gz writer)
+import oracle_py as O  # noqa: E402  (dump reader)
+
+DATA = os.path.join(HERE, "data")
+POTS = os.path.join(ROOT, "comd_amd", "pots")
+WORK = os.path.join(ROOT, "_work", "golden_tmp")
+
+LJ_SIGMA, LJ_EPS, LJ_CUT, LJ_MASS = 2.315, 0.167 / 27.211386, 2.5 * 2.315, 63.55
+LJ_LAT = 2.0 ** 0.5 * 2.0 ** (1.0 / 6.0) * LJ_SIGMA
+
+
+def seeded_input(name, n, lat, mass, temp, disp, seed, stretch=1.0):
+    """FCC n^3 with Gaussian displacements and Maxwell-Boltzmann momenta,
+    x positions and the x bound stretched by `stretch`."""
+    path = os.path.join(DATA, name)
+    xyz = CA.fcc_positions(n, n, n, lat)
+    xyz, p = CA.seed_state(xyz, mass, temp, disp, seed)
+    xyz = xyz.copy()
+    xyz[:, 0] *= stretch
+    bounds = (n * lat * stretch, n * lat, n * lat)
+    CA.write_gz(path, bounds, xyz, p)
+    return path
+
+
+def run(exe, args, prefix):
+    cmd = [exe] + args + ["--out", prefix]
+    r = subprocess.run(cmd, check=True, capture_output=True, text=True)
+    return cmd, r.stdout
+
+
+def collect(prefix, dumps, out, cmd, steps, keep=("natoms", "id", "r", "p", "f", "rho", "fi")):
+    arrays = {}
+    for m in range(dumps + 1):
+        d = O.read_dump(f"{prefix}.{m}.bin")
+        for k in keep:
+            arrays[f"s{m}_{k}"] = d[k]
+        arrays[f"s{m}_e"] = np.float64(d["e"])
+        arrays[f"s{m}_stress"] = np.float64(d["stress"])
+    arrays["nbx"] = np.array(d["nbx"], np.int32)
+    arrays["bounds"] = d["bounds"]
+    arrays["boxsize"] = d["boxsize"]
+    arrays["steps"] = np.int32(steps)
+    arrays["dumps"] = np.int32(dumps)
+    rel = [os.path.relpath(c, ROOT) if isinstance(c, str) and c.startswith(ROOT) else c for c in cmd]
+    arrays["cmd"] = np.array(" ".join(shlex.quote(str(c)) for c in rel))
+    np.savez_compressed(os.path.join(HERE, out), **arrays)
+    print(f"wrote {out}: {sum(v.nbytes for v in arrays.values())} bytes raw")
+
+
+def read_pot_bin(path):
+    """cutoff/mass/lat, 3 tables with guards, 6 Chebyshev series
+    (layout of dump_pot in oracle/ref_harness.c)."""
+    raw = open(path, "rb").read()
+    off = 0
+    cut, mass, lat = struct.unpack_from("<3d", raw, off)
+    off += 24
+    out = dict(cutoff=cut, mass=mass, lat=lat)
+    for k, name in enumerate(["phi", "rho", "F", "cphi", "crho", "cF", "cdphi", "cdrho", "cdF"]):
+        (n,) = struct.unpack_from("<i", raw, off)
+        off += 4
+        x0, xn, invdx = struct.unpack_from("<3d", raw, off)
+        off += 24
+        cnt = n + 3 if k < 3 else n
+        vals = np.frombuffer(raw, "<f8", cnt, off).copy()
+        off += 8 * cnt
+        out[name] = dict(n=n, x0=x0, xn=xn, invdx=invdx, vals=vals)
+    return out
+
+
+def main():
+    if not os.path.isdir("/root/reference/src-flat"):
+        sys.exit("the reference tree is not present; the committed fixtures stay as they are")
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
+    os.makedirs(WORK, exist_ok=True)
+    tab, cheb = O.ref_binary(False), O.ref_binary(True)
+    cu = ["--eam", POTS, "cu500"]
+    lj = ["--lj", repr(LJ_SIGMA), repr(LJ_EPS), repr(LJ_CUT), repr(LJ_MASS)]
+
+    # potential: headers, guard slots, Chebyshev coefficients
+    cmd, _ = run(tab, cu + ["--fcc", "2", "2", "2", "3.615", "1.0"], os.path.join(WORK, "pot"))
+    pot = read_pot_bin(os.path.join(WORK, "pot.pot.bin"))
+    arr = dict(cutoff=pot["cutoff"], mass=pot["mass"], lat=pot["lat"])
+    for name in ("phi", "rho", "F"):
+        t = pot[name]
+        arr[f"{name}_hdr"] = np.array([t["n"], t["x0"], t["xn"], t["invdx"]])
+        arr[f"{name}_head"] = t["vals"][:4]    # values[-1..2]
+        arr[f"{name}_tail"] = t["vals"][-4:]   # values[n-2..n+1]
+    arr["cheb"] = np.stack([pot[k]["vals"] for k in ("cphi", "crho", "cF", "cdphi", "cdrho", "cdF")])
+    arr["cheb_ab"] = np.array([[pot[k]["x0"], pot[k]["xn"]] for k in ("cphi", "crho", "cF")])
+    np.savez_compressed(os.path.join(HERE, "cu500_pot.npz"), **arr)
+    print("wrote cu500_pot.npz")
+
+    # two atoms of the reference's own data/2.inp.gz
+    two = os.path.join(DATA, "2.inp.gz")
+    for exe, tag in ((tab, "table"), (cheb, "cheby")):
+        p = os.path.join(WORK, f"two_{tag}")
+        cmd, _ = run(exe, cu + ["--input", two, "--steps", "10", "--dumps", "2"], p)
+        collect(p, 2, f"two_atom_{tag}.npz", cmd, 10)
+
+    # seeded Cu 6^3 (864 atoms, 4^3 cells), sigma 0.1 A, 300 K
+    cu6 = seeded_input("cu6.inp.gz", 6, 3.615, 63.55, 300.0, 0.1, 7)
+    for exe, tag in ((tab, "table"), (cheb, "cheby")):
+        p = os.path.join(WORK, f"cu6_{tag}")
+        cmd, _ = run(exe, cu + ["--input", cu6, "--steps", "5", "--dumps", "2"], p)
+        collect(p, 2, f"cu6_{tag}.npz", cmd, 5)
+
+    # the same, stretched 1.07 in x (a file input: the grid follows the
+    # stretched bounds)
+    cu6s = seeded_input("cu6s.inp.gz", 6, 3.615, 63.55, 300.0, 0.1, 8, stretch=1.07)
+    p = os.path.join(WORK, "cu6s_table")
+    cmd, _ = run(tab, cu + ["--input", cu6s, "--steps", "10", "--dumps", "1"], p)
+    collect(p, 1, "cu6s_table.npz", cmd, 10)
+
+    # Lennard-Jones copper, 6^3 (864 atoms, 3^3 cells), 600 K
+    lj6 = seeded_input("lj6.inp.gz", 6, LJ_LAT, LJ_MASS, 600.0, 0.05, 5)
+    p = os.path.join(WORK, "lj6")
+    cmd, _ = run(tab, lj + ["--input", lj6, "--steps", "5", "--dumps", "2"], p)
+    collect(p, 2, "lj6.npz", cmd, 5)
+
+    # the reference's data/8k.inp.gz with the silver tables, initial force
+    ag = os.path.join(DATA, "8k.inp.gz")
+    p = os.path.join(WORK, "ag8k")
+    cmd, _ = run(tab, ["--eam", POTS, "ag", "--input", ag, "--steps", "1", "--dumps", "0"], p)
+    collect(p, 0, "ag8k.npz", cmd, 0, keep=("natoms", "id", "f", "rho", "fi"))
+
+    # perfect Cu 20^3 lattice: scalars only
+    sc = {}
+    for exe, tag in ((tab, "table"), (cheb, "cheby")):
+        p = os.path.join(WORK, f"fcc20_{tag}")
+        cmd, _ = run(exe, cu + ["--fcc", "20", "20", "20", "3.615", "1.0", "--steps", "1", "--dumps", "0"], p)
+        d = O.read_dump(p + ".0.bin")
+        sc[f"{tag}_e"] = np.float64(d["e"])
+        sc[f"{tag}_stress"] = np.float64(d["stress"])
+        sc[f"{tag}_fmax"] = np.abs(d["f"][:, :3]).max()
+        sc[f"{tag}_ei"] = np.array([d["f"][:, 3].min(), d["f"][:, 3].max()])
+        sc["nbx"] = np.array(d["nbx"], np.int32)
+        sc["ntot"] = np.int32(d["ntot"])
+    np.savez_compressed(os.path.join(HERE, "fcc20.npz"), **sc)
+    print("wrote fcc20.npz")
+
+
+if __name__ == "__main__":
+    main()

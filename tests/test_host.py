@@ -1,0 +1,144 @@
+"""CPU tests of the C-ABI libraries and the host-side setup code: every
+declared entry point is exported, and the host's potential reader,
+Chebyshev fit, lattice, seeded inputs, gz files and grid agree with the
+compiled reference's fixtures and the oracle.  No GPU is needed."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import comd_amd as CA
+import oracle_py as O
+from helpers import GOLDEN, POTS, ROOT
+
+
+def declared_functions(header):
+    """Names of the functions an include/*.h file declares."""
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", "", text)
+    names = set()
+    for m in re.finditer(r"^[A-Za-z_][\w \t\*]*?\b([A-Za-z_]\w*)\s*\(([^;{]*?)\)\s*;", text, flags=re.M):
+        if m.group(1) not in ("if", "while", "for", "return", "sizeof"):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+@pytest.mark.parametrize("header,lib", [("comd_hip.h", 0), ("comd_host.h", 1)])
+def test_libraries_export_every_declared_function(header, lib):
+    names = declared_functions(header)
+    assert len(names) > 10
+    handle = CA.libraries()[lib]
+    missing = [n for n in names if not hasattr(handle, n)]
+    assert not missing, f"{header}: not exported by the library: {missing}"
+
+
+def test_c_abi_has_no_torch_types():
+    for h in ("comd_hip.h", "comd_host.h"):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        for token in ("at::Tensor", "c10::", "torch::", "#include <torch"):
+            assert token not in text
+
+
+def test_host_tables_and_chebyshev_match_reference(cu500):
+    g = np.load(os.path.join(GOLDEN, "cu500_pot.npz"))
+    assert (cu500.cutoff, cu500.mass, cu500.lat) == (float(g["cutoff"]), float(g["mass"]), float(g["lat"]))
+    for t, name in zip(cu500.tables, ("phi", "rho", "F")):
+        n, x0, xn, invdx = g[f"{name}_hdr"]
+        assert (t["n"], t["x0"], t["xn"], t["invdx"]) == (int(n), x0, xn, invdx)
+        np.testing.assert_array_equal(t["values"][:4], g[f"{name}_head"])
+        np.testing.assert_array_equal(t["values"][-4:], g[f"{name}_tail"])
+    ref = g["cheb"]
+    scale = np.abs(ref).max(axis=1, keepdims=True)
+    assert (np.abs(cu500.cheb - ref) <= 1e-14 * scale).all()
+    np.testing.assert_array_equal(cu500.cheb_dom, g["cheb_ab"])
+
+
+def test_host_tables_equal_oracle_tables(cu500, ocu500):
+    for k in range(3):
+        o = ocu500.table(k)
+        np.testing.assert_array_equal(cu500.tables[k]["values"], o["values"])
+
+
+def test_fcc_positions_follow_create_fcc_lattice():
+    """Binning the host lattice gives exactly the oracle's create_fcc_lattice
+    state (same ids, cells, slots and relative positions)."""
+    pot = O.OraclePot.eam(POTS, "cu500")
+    for n, lat, dg in ((5, 3.615, 1.0), (6, 3.615, 1.07)):
+        xyz = CA.fcc_positions(n, n, n, lat, dg)
+        assert xyz.shape == (4 * n ** 3, 3)
+        a = O.OracleSim.fcc(pot, n, lat, defgrad=dg)
+        bounds = a.info()["bounds"]
+        b = O.OracleSim.from_atoms(pot, tuple(bounds), xyz)
+        sa, sb = a.state(), b.state()
+        for k in ("natoms", "id", "r"):
+            np.testing.assert_array_equal(sa[k], sb[k])
+
+
+def test_seeded_state_statistics_and_determinism():
+    n = 8
+    xyz0 = CA.fcc_positions(n, n, n, 3.615)
+    x1, p1 = CA.seed_state(xyz0, 63.55, 300.0, 0.05, 11)
+    x2, p2 = CA.seed_state(xyz0, 63.55, 300.0, 0.05, 11)
+    x3, _ = CA.seed_state(xyz0, 63.55, 300.0, 0.05, 12)
+    np.testing.assert_array_equal(x1, x2)
+    np.testing.assert_array_equal(p1, p2)
+    assert not np.array_equal(x1, x3)
+    rmass = CA.AMU_TO_ME * 63.55
+    assert np.abs(p1.sum(0)).max() < 1e-9 * np.abs(p1).sum()
+    temp = (p1 ** 2).sum() / rmass / (3 * len(p1) * CA.KB_HARTREE)
+    assert abs(temp / 300.0 - 1) < 0.06
+    disp = np.sqrt(((x1 - xyz0) ** 2).mean())
+    assert abs(disp / 0.05 - 1) < 0.06
+
+
+def test_gz_round_trip_is_exact(tmp_path):
+    n = 4
+    xyz, p = CA.seed_state(CA.fcc_positions(n, n, n, 3.615), 63.55, 300.0, 0.05, 3)
+    bounds = (n * 3.615,) * 3
+    path = tmp_path / "s.inp.gz"
+    CA.write_gz(path, bounds, xyz, p)
+    b, x2, p2 = CA.read_gz(path)
+    assert tuple(b) == bounds
+    np.testing.assert_array_equal(x2, xyz)
+    np.testing.assert_array_equal(p2, p)
+    # and the oracle's reader (src-flat/read.c:155-241) sees the same atoms
+    pot = O.OraclePot.eam(POTS, "cu500")
+    a = O.OracleSim.gz(pot, path)
+    bsim = O.OracleSim.from_atoms(pot, bounds, xyz, p)
+    sa, sb = a.state(), bsim.state()
+    for k in ("natoms", "id", "r", "p"):
+        np.testing.assert_array_equal(sa[k], sb[k])
+
+
+def test_read_reference_input_files():
+    b, xyz, p = CA.read_gz(os.path.join(GOLDEN, "data", "2.inp.gz"))
+    assert tuple(b) == (40.0, 40.0, 40.0) and xyz.shape == (2, 3)
+    np.testing.assert_array_equal(xyz[0], [1.0, 39.5, 0.1])
+    b, xyz, p = CA.read_gz(os.path.join(GOLDEN, "data", "8k.inp.gz"))
+    assert xyz.shape[0] == 8192  # first line of the file: "8192 ..."
+
+
+@pytest.mark.parametrize("bounds,defgrad", [((21.69, 21.69, 21.69), 1.0), ((23.2083, 21.69, 21.69), 1.0),
+                                            ((144.6, 144.6, 144.6), 1.0), ((40.0, 40.0, 40.0), 0.0),
+                                            ((21.69, 21.69, 21.69), 1.07)])
+def test_grid_matches_alloc_domains(cu500, bounds, defgrad):
+    """comd_grid_for restates allocDomains (src-flat/domains.c:155-200)."""
+    hip, _ = CA.libraries()
+    prm = CA.Params()
+    cu500.fill(prm)
+    assert hip.comd_grid_for(C.byref(prm), (C.c_double * 3)(*bounds), defgrad, 1.0) == 0
+    pot = O.OraclePot.eam(POTS, "cu500")
+    sim = O.OracleSim(O.lib().orc_sim_create(pot.h, (C.c_double * 3)(*bounds), defgrad, 1.0, 1), pot)
+    info = sim.info()
+    assert tuple(prm.nbx) == tuple(info["nbx"])
+    np.testing.assert_array_equal(np.array(prm.boxsize[:]), info["boxsize"])
+
+
+def test_grid_rejects_box_smaller_than_cutoff(cu500):
+    hip, _ = CA.libraries()
+    prm = CA.Params()
+    cu500.fill(prm)
+    assert hip.comd_grid_for(C.byref(prm), (C.c_double * 3)(3.0, 20.0, 20.0), 1.0, 1.0) == 6
