@@ -126,6 +126,9 @@ def _load():
     hip.comd_last_error.restype = C.c_char_p
     hip.comd_timing.argtypes = [vp, C.c_int]
     hip.comd_timing_read.argtypes = [vp, _dp, _ip]
+    hip.comd_set_cheb.argtypes = [vp, _dp, C.c_int]
+    hip.comd_get_cheb.argtypes = [vp, _dp]
+    hip.comd_eam_cheb_grad.argtypes = [vp, _dp, C.c_int, _dp, C.c_double, C.c_double, _dp, _dp]
     host.setEamPot.argtypes = [C.c_char_p, C.c_char_p]
     host.setEamPot.restype = C.POINTER(EamPotT)
     host.setChebPot.argtypes = [C.POINTER(EamPotT), C.c_int]
@@ -440,6 +443,36 @@ class Simulation:
         ln = (C.c_int * K_NCLASS)()
         self._check(self._hip.comd_timing_read(self.ctx, ms, ln))
         return list(ms), list(ln)
+
+    # ---- Chebyshev coefficients and their adjoint
+
+    def set_cheb(self, coef):
+        """Replace the phi, rho, F value series (3 x n) and rebuild the
+        derivative series (comd_set_cheb)."""
+        c = np.ascontiguousarray(coef, dtype=np.float64).reshape(3, -1)
+        self._check(self._hip.comd_set_cheb(self.ctx, _ptr(c), c.shape[1]))
+
+    def get_cheb(self):
+        """The six series in use: phi, rho, F, dphi, drho, dF (6 x n)."""
+        n = self.prm.cheb_n
+        out = np.zeros((6, n))
+        self._check(self._hip.comd_get_cheb(self.ctx, _ptr(out)))
+        return out
+
+    def cheb_grad(self, coef, f_target, e_target, w_e=1.0):
+        """Force-matching loss sum_i |f_i - t_i|^2 / (3N) + w_e (E - E_t)^2 at
+        the value coefficients `coef` (3 x n) and its gradient (3 x n), from
+        the adjoint kernels (comd_eam_cheb_grad).  f_target: slot layout
+        (S, 3) or (S, 4) as download()['f']."""
+        c = np.ascontiguousarray(coef, dtype=np.float64).reshape(3, -1)
+        ft = np.ascontiguousarray(np.asarray(f_target, dtype=np.float64)[:, :3])
+        if ft.shape[0] != self.nslots:
+            raise ValueError(f"f_target has {ft.shape[0]} rows, the context {self.nslots} slots")
+        loss = C.c_double()
+        grad = np.zeros_like(c)
+        self._check(self._hip.comd_eam_cheb_grad(self.ctx, _ptr(c), c.shape[1], _ptr(ft), float(e_target),
+                                                 float(w_e), C.byref(loss), _ptr(grad)))
+        return loss.value, grad
 
 
 def fcc_simulation(pot, n, lat=None, temp=0.0, disp=0.0, seed=1, precision="fp64", evaluation="table",

@@ -1,0 +1,342 @@
+// adjoint.hip -- gradient of a force-matching loss with respect to the
+// Chebyshev coefficients of phi, rho and F (EAM evaluated through the
+// series of src-flat/cheby.c:99-310, as src-flat/eam.c:523-527,582-584,
+// 637-639 do under USE_CHEBY).
+//
+// Forward quantities (f holds +grad U, as src-flat/eam.c fills it):
+//   rho_i = sum_j rho(r_ij)                          (pairs with r2 <= rc2)
+//   E     = sum_i F(rho_i) + 1/2 sum_i sum_j phi(r_ij)
+//   f_i   = sum_j phi'(r_ij) u_ij                    (r2 <= rc2)
+//         + sum_j (F'(rho_i) + F'(rho_j)) rho'(r_ij) u_ij   (r2 < rc2)
+//   u_ij  = (r_i - r_j) / r_ij
+// with phi' = series(D c_phi), rho' = series(D c_rho), F' = series(D c_F),
+// D the derivative recurrence of src-flat/cheby.c:219-270.
+//
+// Loss L = sum_i g0 |f_i - t_i|^2 + w_E (E - E_t)^2.  With g_i = dL/df_i,
+// w_ij = (g_i - g_j) . u_ij and B_k(x) = T_k(y(x)) (B_0 = 1/2):
+//   pair pass 1: M_phi'[k] = 1/2 sum_ij w_ij B_k(r_ij),  M_phiE[k] = 1/2 sum_ij B_k(r_ij)
+//   pair pass 2: M_rho'[k] = 1/2 sum_ij (F'_i + F'_j) w_ij B_k(r_ij),
+//                lambda_i = sum_j rho'(r_ij) w_ij            (dL/dF'_i through the forces)
+//   atom pass:   mu_i = lambda_i F''(rho_i) + dL/dE F'(rho_i) (dL/drho_i)
+//                M_F[k] = sum_i lambda_i B_k(rho_i), M_FE[k] = sum_i B_k(rho_i)
+//   pair pass 3: M_rho[k] = sum_ij mu_i B_k(r_ij)
+// (sums over ordered pairs, every atom seeing all its neighbours; rho_i is
+// recomputed by the forward force pass, never stored across calls).  The host
+// assembles  dL/dc_phi = D^T M_phi' + dL/dE M_phiE,
+//            dL/dc_rho = D^T M_rho' + M_rho,
+//            dL/dc_F   = D^T M_F + dL/dE M_FE.
+//
+// Every pass is full-shell over the 27-cell stencil with the halo staged in
+// LDS, one wave per cell on a persistent grid; each lane keeps its moments
+// in registers and the waves' partials are summed in a fixed order, so the
+// gradient is deterministic.  MIXED evaluates the Chebyshev basis and rho'
+// in single precision and accumulates in double.
+#include "comd_internal.h"
+
+#define AMAX 32    // coefficients per series supported by the adjoint
+#define AHMAX 320  // halo atoms staged per batch
+#define ANMOM 64   // moments written per wave (two sets of AMAX)
+
+struct AdjDev {
+    const double *gx, *gy, *gz;  // per slot: dL/df
+    double *lam;                 // per slot: lambda (pair pass 2)
+    double *mu;                  // per slot: mu (atom pass)
+    const double *f2;            // F'' series (cn coefficients, FP64)
+    double dlde;                 // dL/dE
+    double *part;                // per block: ANMOM partial moments
+};
+
+// T_0..T_{m-1} at y, each times wa added to a[] (and times wb to b[]).
+template <int MIXED, int TWO>
+__device__ __forceinline__ void add_basis(double y, int m, double wa, double wb, double *a, double *b)
+{
+    if (MIXED) {
+        const float yf = (float)y, y2 = 2.0f * yf;
+        float tm = 1.0f, t = yf;
+        a[0] += wa;
+        if (TWO) b[0] += wb;
+#pragma unroll
+        for (int k = 1; k < AMAX; k++) {
+            if (k < m) {
+                a[k] += wa * (double)t;
+                if (TWO) b[k] += wb * (double)t;
+            }
+            const float tn = y2 * t - tm;
+            tm = t;
+            t = tn;
+        }
+    } else {
+        const double y2 = 2.0 * y;
+        double tm = 1.0, t = y;
+        a[0] += wa;
+        if (TWO) b[0] += wb;
+#pragma unroll
+        for (int k = 1; k < AMAX; k++) {
+            if (k < m) {
+                a[k] += wa * t;
+                if (TWO) b[k] += wb * t;
+            }
+            const double tn = y2 * t - tm;
+            tm = t;
+            t = tn;
+        }
+    }
+}
+
+__device__ __forceinline__ double cheb_arg(double x, double a, double b)
+{
+    if (x < a) x = a;  // lower clamp only (src-flat/cheby.c:290-293)
+    return (2.0 * x - a - b) / (b - a);
+}
+
+// Sum of v over the wave's 64 lanes (result in every lane).
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <int TWO>
+__device__ __forceinline__ void write_moments(const double *a, const double *b, double *part)
+{
+    const int L = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < AMAX; k++) {
+        const double s = wave_sum(a[k]);
+        if (L == 0) part[k] = s;
+        if (TWO) {
+            const double t = wave_sum(b[k]);
+            if (L == 0) part[AMAX + k] = t;
+        } else if (L == 0) {
+            part[AMAX + k] = 0.0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// pair passes 1-3
+
+template <int PASS, int MIXED>
+__global__ __launch_bounds__(64) void k_adj_pair(DevState d, PotDev P, AdjDev A)
+{
+    __shared__ double hx[AHMAX], hy[AHMAX], hz[AHMAX];
+    __shared__ double hgx[PASS == 3 ? 1 : AHMAX], hgy[PASS == 3 ? 1 : AHMAX], hgz[PASS == 3 ? 1 : AHMAX];
+    __shared__ double hf[PASS == 2 ? AHMAX : 1];
+    __shared__ double drb[81];
+    __shared__ double red[64];
+    __shared__ int cell[27], off[28];
+
+    const int L = threadIdx.x;
+    const int m = P.cn;
+    const int dom = PASS == 1 ? 0 : 1;
+    const double ca = P.ca[dom], cb = P.cb[dom];
+    const double rc2 = P.rc2;
+    const double *dser = P.cheb + 4 * m;      // rho' series
+    const float *dser32 = P.cheb32 + 4 * m;
+
+    double m0[AMAX], m1[AMAX];
+#pragma unroll
+    for (int k = 0; k < AMAX; k++) m0[k] = m1[k] = 0.0;
+
+    for (int c = blockIdx.x; c < d.ncells; c += gridDim.x) {
+        const int nh = d.natoms[c];
+        if (nh == 0) continue;
+        const int nent = d.ncells == 1 ? 1 : 27;
+        int ne = 0;
+        if (L < nent) {
+            const int b = stencil_cell(d, c, L);
+            cell[L] = b;
+            if (b >= 0) {
+                ne = d.natoms[b];
+                double sx, sy, sz;
+                corner_shift(d, c, b, sx, sy, sz);
+                drb[3 * L + 0] = sx;
+                drb[3 * L + 1] = sy;
+                drb[3 * L + 2] = sz;
+            }
+        }
+        int incl = ne;
+        for (int o = 1; o < 32; o <<= 1) {
+            const int t = __shfl_up(incl, o, 64);
+            if (L >= o) incl += t;
+        }
+        const int H = __shfl(incl, 31, 64);
+        if (L < nent) off[L] = incl - ne;
+        if (L == 0) off[nent] = H;
+
+        const int S = 64 / nh;
+        const bool act = L < S * nh;
+        const int a = act ? L % nh : 0;
+        const int sg = act ? L / nh : 0;
+        const int hs = c * CM_SLOTS + a;
+        const double xa = d.rx[hs], ya = d.ry[hs], za = d.rz[hs];
+        const double gax = PASS == 3 ? 0.0 : A.gx[hs];
+        const double gay = PASS == 3 ? 0.0 : A.gy[hs];
+        const double gaz = PASS == 3 ? 0.0 : A.gz[hs];
+        const double fia = PASS == 2 ? d.fi[hs] : 0.0;
+        const double mua = PASS == 3 ? A.mu[hs] : 0.0;
+        double lam = 0.0;
+
+        for (int h0 = 0; h0 < H; h0 += AHMAX) {
+            const int hb = H - h0 < AHMAX ? H - h0 : AHMAX;
+            __syncthreads();
+            {
+                int e = 0;
+                for (int h = h0 + L; h < h0 + hb; h += 64) {
+                    while (off[e + 1] <= h) e++;
+                    const int sl = cell[e] * CM_SLOTS + (h - off[e]);
+                    hx[h - h0] = d.rx[sl] - drb[3 * e + 0];
+                    hy[h - h0] = d.ry[sl] - drb[3 * e + 1];
+                    hz[h - h0] = d.rz[sl] - drb[3 * e + 2];
+                    if (PASS != 3) {
+                        hgx[h - h0] = A.gx[sl];
+                        hgy[h - h0] = A.gy[sl];
+                        hgz[h - h0] = A.gz[sl];
+                    }
+                    if (PASS == 2) hf[h - h0] = d.fi[sl];
+                }
+            }
+            __syncthreads();
+            if (act) {
+                for (int j = sg; j < hb; j += S) {
+                    const double dx = xa - hx[j], dy = ya - hy[j], dz = za - hz[j];
+                    const double r2 = dx * dx + dy * dy + dz * dz;
+                    // pair predicates of src-flat/eam.c:519 (phi, rho) and 633 (rho')
+                    const bool in = PASS == 2 ? (r2 < rc2) : (r2 <= rc2);
+                    if (!in || !(r2 > 0.0)) continue;
+                    const double r = sqrt(r2);
+                    const double y = cheb_arg(r, ca, cb);
+                    if (PASS == 3) {
+                        add_basis<MIXED, 0>(y, m, mua, 0.0, m0, m1);
+                    } else {
+                        const double w = ((gax - hgx[j]) * dx + (gay - hgy[j]) * dy + (gaz - hgz[j]) * dz) / r;
+                        if (PASS == 1) {
+                            add_basis<MIXED, 1>(y, m, 0.5 * w, 0.5, m0, m1);
+                        } else {
+                            add_basis<MIXED, 0>(y, m, 0.5 * (fia + hf[j]) * w, 0.0, m0, m1);
+                            const double drho = MIXED ? (double)cheb32(dser32, m, (float)ca, (float)cb, (float)r)
+                                                      : cheb64(dser, m, ca, cb, r);
+                            lam += drho * w;
+                        }
+                    }
+                }
+            }
+        }
+        if (PASS == 2) {
+            // lambda of each home atom: its S partial sums in s order
+            red[L] = act ? lam : 0.0;
+            __syncthreads();
+            if (L < nh) {
+                double s = 0.0;
+                for (int q = 0; q < S; q++) s += red[L + q * nh];
+                A.lam[c * CM_SLOTS + L] = s;
+            }
+        }
+        __syncthreads();
+    }
+    write_moments<PASS == 1>(m0, m1, A.part + (size_t)blockIdx.x * ANMOM);
+}
+
+// ---------------------------------------------------------------------------
+// atom pass: mu_i, M_F, M_FE
+
+template <int MIXED>
+__global__ __launch_bounds__(64) void k_adj_atom(DevState d, PotDev P, AdjDev A)
+{
+    const int L = threadIdx.x;
+    const int m = P.cn;
+    const double ca = P.ca[2], cb = P.cb[2];
+    double m0[AMAX], m1[AMAX];
+#pragma unroll
+    for (int k = 0; k < AMAX; k++) m0[k] = m1[k] = 0.0;
+    for (int c = blockIdx.x; c < d.ncells; c += gridDim.x) {
+        if (L < d.natoms[c]) {
+            const int s = c * CM_SLOTS + L;
+            const double rho = d.rho[s];
+            const double lam = A.lam[s];
+            const double f2 = cheb64(A.f2, m, ca, cb, rho);
+            A.mu[s] = lam * f2 + A.dlde * d.fi[s];
+            add_basis<MIXED, 1>(cheb_arg(rho, ca, cb), m, lam, 1.0, m0, m1);
+        }
+    }
+    write_moments<1>(m0, m1, A.part + (size_t)blockIdx.x * ANMOM);
+}
+
+// ---------------------------------------------------------------------------
+// residual: g_i = gscale (f_i - t_i), and the partial sums of |f_i - t_i|^2
+
+__global__ __launch_bounds__(64) void k_adj_resid(DevState d, const double *ft, double gscale, double *gx,
+                                                  double *gy, double *gz, double *part)
+{
+    const int L = threadIdx.x;
+    double ss = 0.0;
+    for (int c = blockIdx.x; c < d.ncells; c += gridDim.x) {
+        const int s = c * CM_SLOTS + L;
+        if (L < d.natoms[c]) {
+            const double ex = d.fx[s] - ft[3L * s + 0];
+            const double ey = d.fy[s] - ft[3L * s + 1];
+            const double ez = d.fz[s] - ft[3L * s + 2];
+            gx[s] = gscale * ex;
+            gy[s] = gscale * ey;
+            gz[s] = gscale * ez;
+            ss += ex * ex + ey * ey + ez * ez;
+        } else {
+            gx[s] = 0.0;
+            gy[s] = 0.0;
+            gz[s] = 0.0;
+        }
+    }
+    const double t = wave_sum(ss);
+    part[(size_t)blockIdx.x * ANMOM + L] = L == 0 ? t : 0.0;
+}
+
+// Sums the blocks' partial moments in block order: out[k] = sum_b part[b][k].
+__global__ __launch_bounds__(64) void k_adj_final(const double *part, int nblocks, double *out)
+{
+    const int k = threadIdx.x;
+    double s = 0.0;
+    for (int b = 0; b < nblocks; b++) s += part[(size_t)b * ANMOM + k];
+    out[k] = s;
+}
+
+// ---------------------------------------------------------------------------
+
+int cm_adjoint_max_coefs() { return AMAX; }
+int cm_adjoint_moments() { return ANMOM; }
+
+template <int MIXED>
+static void launch_all(const DevState &d, const PotDev &P, const CmAdjoint &J, int stage, hipStream_t st)
+{
+    AdjDev A;
+    A.gx = J.gx;
+    A.gy = J.gy;
+    A.gz = J.gz;
+    A.lam = J.lam;
+    A.mu = J.mu;
+    A.f2 = J.f2;
+    A.dlde = J.dlde;
+    A.part = J.part;
+    const dim3 g(J.nblocks), b(64);
+    double *o = J.out + (size_t)stage * ANMOM;
+    switch (stage) {
+    case 0: hipLaunchKernelGGL((k_adj_pair<1, MIXED>), g, b, 0, st, d, P, A); break;
+    case 1: hipLaunchKernelGGL((k_adj_pair<2, MIXED>), g, b, 0, st, d, P, A); break;
+    case 2: hipLaunchKernelGGL((k_adj_atom<MIXED>), g, b, 0, st, d, P, A); break;
+    case 3: hipLaunchKernelGGL((k_adj_pair<3, MIXED>), g, b, 0, st, d, P, A); break;
+    default: return;
+    }
+    hipLaunchKernelGGL(k_adj_final, dim3(1), dim3(64), 0, st, J.part, J.nblocks, o);
+}
+
+void cm_launch_adj_resid(const DevState &d, const CmAdjoint &J, const double *ft, double gscale, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_adj_resid, dim3(J.nblocks), dim3(64), 0, st, d, ft, gscale, J.gx, J.gy, J.gz, J.part);
+    hipLaunchKernelGGL(k_adj_final, dim3(1), dim3(64), 0, st, J.part, J.nblocks, J.out + (size_t)4 * ANMOM);
+}
+
+void cm_launch_adj_stage(const DevState &d, const PotDev &P, const CmAdjoint &J, int stage, int mixed, hipStream_t st)
+{
+    if (mixed) launch_all<1>(d, P, J, stage, st);
+    else launch_all<0>(d, P, J, stage, st);
+}

@@ -6,6 +6,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -55,6 +56,10 @@ struct comd_ctx {
     float *d_cf32[2];
     double *d_cheb;
     float *d_cheb32;
+    // Chebyshev-coefficient adjoint (allocated on first use)
+    double *adj_g[3], *adj_lam, *adj_mu, *adj_ft, *adj_f2, *adj_part, *adj_out;
+    int adj_nblocks;
+    std::vector<double> cheb_host;  // current value series phi, rho, F (3 x cn)
     // force-kernel launch shape (comd_tune)
     int ncu, max_lds;
     int force_waves, force_blocks_per_cu, tab_lds;
@@ -245,7 +250,8 @@ static void free_all(comd_ctx *c)
                     c->mv_itype, c->mv_rx, c->mv_ry, c->mv_rz, c->mv_px, c->mv_py, c->mv_pz,
                     c->perm, c->flags, c->partials, c->red_out,
                     c->d_tab[0], c->d_tab[1], c->d_tab[2], c->d_cf64[0], c->d_cf64[1], c->d_cf32[0], c->d_cf32[1],
-                    c->d_cheb, c->d_cheb32};
+                    c->d_cheb, c->d_cheb32, c->adj_g[0], c->adj_g[1], c->adj_g[2], c->adj_lam, c->adj_mu,
+                    c->adj_ft, c->adj_f2, c->adj_part, c->adj_out};
     for (void *p : ptrs)
         if (p) hipFree(p);
     for (int b = 0; b < 2; b++) {
@@ -422,6 +428,7 @@ int comd_create(const comd_params *p, comd_ctx **out)
                 P.cheb = c->d_cheb;
                 P.cheb32 = c->d_cheb32;
                 P.cn = m;
+                c->cheb_host.assign(p->cheb_coef, p->cheb_coef + 3 * m);
                 for (int k = 0; k < 3; k++) {
                     P.ca[k] = p->cheb_x0[k];
                     P.cb[k] = p->cheb_xn[k];
@@ -796,6 +803,147 @@ int comd_timing_read(comd_ctx *c, double *ms, int *launches)
     for (int k = 0; k < COMD_K_NCLASS; k++) {
         if (ms) ms[k] = c->ms[k];
         if (launches) launches[k] = c->launches[k];
+    }
+    return COMD_OK;
+}
+
+
+// ---- Chebyshev coefficients and their adjoint
+
+// Derivative series of a Chebyshev series on [a, b]: the recurrence of
+// src-flat/cheby.c:219-270 (last coefficient 0, d[j-1] = d[j+1] + 2 j c[j],
+// scaled by 2/(b-a)).
+static void cheb_derivative(int n, double a, double b, const double *c, double *d)
+{
+    d[n - 1] = 0.0;
+    d[n - 2] = 2.0 * (n - 1) * c[n - 1];
+    for (int j = n - 2; j > 0; j--) d[j - 1] = d[j + 1] + 2 * (j)*c[j];
+    const double con = 2.0 / (b - a);
+    for (int j = 0; j < n; j++) d[j] = d[j] * con;
+}
+
+int comd_set_cheb(comd_ctx *c, const double *coef, int n)
+{
+    if (!c || !coef) return COMD_ERR_ARG;
+    if (c->prm.potential != COMD_POT_EAM || !c->pot.cheb)
+        return set_err(c, COMD_ERR_ARG, "the context has no Chebyshev series");
+    const int m = c->pot.cn;
+    if (n != m) return set_err(c, COMD_ERR_ARG, "expected %d coefficients per series, got %d", m, n);
+    std::vector<double> all((size_t)6 * m);
+    std::vector<float> all32((size_t)6 * m);
+    for (int k = 0; k < 3; k++) {
+        for (int j = 0; j < m; j++) all[(size_t)k * m + j] = coef[(size_t)k * m + j];
+        cheb_derivative(m, c->pot.ca[k], c->pot.cb[k], coef + (size_t)k * m, all.data() + (size_t)(3 + k) * m);
+    }
+    for (int i = 0; i < 6 * m; i++) all32[i] = (float)all[i];
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(c->d_cheb, all.data(), all.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_cheb32, all32.data(), all32.size() * 4, hipMemcpyHostToDevice));
+    c->cheb_host.assign(coef, coef + 3 * m);
+    return COMD_OK;
+}
+
+int comd_get_cheb(comd_ctx *c, double *coef6)
+{
+    if (!c || !coef6) return COMD_ERR_ARG;
+    if (!c->pot.cheb) return set_err(c, COMD_ERR_ARG, "the context has no Chebyshev series");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(coef6, c->d_cheb, (size_t)6 * c->pot.cn * 8, hipMemcpyDeviceToHost));
+    return COMD_OK;
+}
+
+int comd_eam_cheb_grad(comd_ctx *c, const double *coef, int n, const double *f_target, double e_target, double w_e,
+                       double *loss, double *grad)
+{
+    if (!c || !coef || !f_target) return COMD_ERR_ARG;
+    if (!c->have_atoms || c->ntot < 1) return set_err(c, COMD_ERR_STATE, "no atoms");
+    if (n > cm_adjoint_max_coefs())
+        return set_err(c, COMD_ERR_ARG, "the adjoint supports at most %d coefficients per series",
+                       cm_adjoint_max_coefs());
+    int rc = comd_set_cheb(c, coef, n);
+    if (rc) return rc;
+    const int m = n;
+    const int NM = cm_adjoint_moments();
+    const size_t S = (size_t)c->nslots;
+    if (!c->adj_out) {
+        c->adj_nblocks = (int)std::min<long>((long)c->ncells, (long)c->ncu * 8);
+        bool ok = true;
+        auto alloc = [&](void **ptr, size_t bytes) {
+            *ptr = nullptr;
+            if (ok && hipMalloc(ptr, bytes) != hipSuccess) ok = false;
+        };
+        for (int k = 0; k < 3; k++) alloc((void **)&c->adj_g[k], S * 8);
+        alloc((void **)&c->adj_lam, S * 8);
+        alloc((void **)&c->adj_mu, S * 8);
+        alloc((void **)&c->adj_ft, S * 3 * 8);
+        alloc((void **)&c->adj_f2, 64 * 8);
+        alloc((void **)&c->adj_part, (size_t)c->adj_nblocks * NM * 8);
+        alloc((void **)&c->adj_out, (size_t)5 * NM * 8);
+        if (!ok) return set_err(c, COMD_ERR_NOMEM, "adjoint buffers");
+    }
+    // F'' series: the derivative recurrence applied twice
+    std::vector<double> d1(m), d2(m);
+    cheb_derivative(m, c->pot.ca[2], c->pot.cb[2], coef + 2 * m, d1.data());
+    cheb_derivative(m, c->pot.ca[2], c->pot.cb[2], d1.data(), d2.data());
+    HIPCHK(c, hipMemcpy(c->adj_f2, d2.data(), (size_t)m * 8, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->adj_ft, f_target, S * 3 * 8, hipMemcpyHostToDevice));
+
+    // forward: forces and energy through the series
+    const int eval_saved = c->prm.eval;
+    c->prm.eval = COMD_EVAL_CHEB;
+    op_force(c);
+    c->prm.eval = eval_saved;
+    double E = 0.0;
+    rc = comd_energy(c, &E, nullptr, nullptr);
+    if (rc) return rc;
+
+    const int mixed = c->prm.precision == COMD_PREC_MIXED;
+    const double N = (double)c->ntot;
+    CmAdjoint J;
+    J.gx = c->adj_g[0];
+    J.gy = c->adj_g[1];
+    J.gz = c->adj_g[2];
+    J.lam = c->adj_lam;
+    J.mu = c->adj_mu;
+    J.f2 = c->adj_f2;
+    J.dlde = 2.0 * w_e * (E - e_target);
+    J.part = c->adj_part;
+    J.nblocks = c->adj_nblocks;
+    J.out = c->adj_out;
+    PotDev P = c->pot;
+    P.eval = COMD_EVAL_CHEB;
+    DevState d = dev_state(c);
+    cm_launch_adj_resid(d, J, c->adj_ft, 2.0 / (3.0 * N), c->stream);
+    for (int stage = 0; stage < 4; stage++) cm_launch_adj_stage(d, P, J, stage, mixed, c->stream);
+    HIPCHK(c, hipGetLastError());
+    std::vector<double> out((size_t)5 * NM);
+    HIPCHK(c, hipMemcpyAsync(out.data(), c->adj_out, out.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+
+    const double L = out[(size_t)4 * NM] / (3.0 * N) + w_e * (E - e_target) * (E - e_target);
+    if (loss) *loss = L;
+    if (!grad) return COMD_OK;
+    // B_0 = T_0 / 2 in the series convention of src-flat/cheby.c:309
+    auto mom = [&](int stage, int set, int k) { return out[(size_t)stage * NM + set * 32 + k] * (k == 0 ? 0.5 : 1.0); };
+    // columns of the derivative operator: D e_j
+    auto dcol = [&](int fn, int j, std::vector<double> &col) {
+        std::vector<double> e(m, 0.0);
+        e[j] = 1.0;
+        col.assign(m, 0.0);
+        cheb_derivative(m, c->pot.ca[fn], c->pot.cb[fn], e.data(), col.data());
+    };
+    std::vector<double> col;
+    for (int j = 0; j < m; j++) {
+        double gphi = J.dlde * mom(0, 1, j), grho = mom(3, 0, j), gF = J.dlde * mom(2, 1, j);
+        dcol(0, j, col);
+        for (int k = 0; k < m; k++) gphi += col[k] * mom(0, 0, k);
+        dcol(1, j, col);
+        for (int k = 0; k < m; k++) grho += col[k] * mom(1, 0, k);
+        dcol(2, j, col);
+        for (int k = 0; k < m; k++) gF += col[k] * mom(2, 0, k);
+        grad[j] = gphi;
+        grad[m + j] = grho;
+        grad[2 * m + j] = gF;
     }
     return COMD_OK;
 }

@@ -32,6 +32,36 @@ struct PotDev {
 
 enum { CM_KIND_EAM1 = 0, CM_KIND_EAM3 = 1, CM_KIND_LJ = 2 };
 
+// Clenshaw sum with the lower clamp only, src-flat/cheby.c:276-310.
+__device__ __forceinline__ double cheb64(const double *c, int m, double a, double b, double x)
+{
+    if (x < a) x = a;
+    double d = 0.0, dd = 0.0;
+    const double y = (2.0 * x - a - b) / (b - a);
+    const double y2 = 2.0 * y;
+    for (int j = m - 1; j > 0; j--) {
+        const double sv = d;
+        d = y2 * d - dd + c[j];
+        dd = sv;
+    }
+    return y * d - dd + 0.5 * c[0];
+}
+
+__device__ __forceinline__ float cheb32(const float *c, int m, float a, float b, float x)
+{
+    x = fmaxf(x, a);
+    float d = 0.0f, dd = 0.0f;
+    const float y = (2.0f * x - a - b) / (b - a);
+    const float y2 = 2.0f * y;
+    for (int j = m - 1; j > 0; j--) {
+        const float sv = d;
+        d = fmaf(y2, d, c[j] - dd);
+        dd = sv;
+    }
+    return fmaf(y, d, 0.5f * c[0] - dd);
+}
+
+
 // Launch shape of the force kernels (chosen by the tuner).
 struct ForceCfg {
     int blocks;   // persistent grid size
@@ -51,3 +81,21 @@ void cm_launch_fill_int(int *p, int v, long n, hipStream_t st);
 void cm_launch_force(const DevState &d, const PotDev &pot, int kind, int mixed, const ForceCfg &cfg, hipStream_t st);
 void cm_launch_reduce(const DevState &d, const PotDev &pot, double *partials, int nparts, hipStream_t st);
 void cm_launch_reduce_final(const double *partials, int nparts, double *out, hipStream_t st);
+
+// Device buffers and scalars of the Chebyshev-coefficient adjoint
+// (adjoint.hip).  out holds 5 x cm_adjoint_moments() doubles: the moments of
+// stages 0-3 and, at out[4*64], the sum of squared force residuals.
+struct CmAdjoint {
+    double *gx, *gy, *gz;  // per slot: dL/df
+    double *lam, *mu;      // per slot
+    const double *f2;      // F'' series
+    double dlde;
+    double *part;          // nblocks x 64 partial moments
+    int nblocks;
+    double *out;
+};
+int cm_adjoint_max_coefs();
+int cm_adjoint_moments();
+void cm_launch_adj_resid(const DevState &d, const CmAdjoint &J, const double *ft, double gscale, hipStream_t st);
+void cm_launch_adj_stage(const DevState &d, const PotDev &P, const CmAdjoint &J, int stage, int mixed,
+                         hipStream_t st);

@@ -122,35 +122,6 @@ __device__ __forceinline__ double cf_val64(const double *cf, int i, double t)
 }
 __device__ __forceinline__ float4 cf_row32(const float *cf, int i) { return ((const float4 *)(cf + 8 * i))[0]; }
 
-// Clenshaw sum with the lower clamp only, src-flat/cheby.c:276-310.
-__device__ __forceinline__ double cheb64(const double *c, int m, double a, double b, double x)
-{
-    if (x < a) x = a;
-    double d = 0.0, dd = 0.0;
-    const double y = (2.0 * x - a - b) / (b - a);
-    const double y2 = 2.0 * y;
-    for (int j = m - 1; j > 0; j--) {
-        const double sv = d;
-        d = y2 * d - dd + c[j];
-        dd = sv;
-    }
-    return y * d - dd + 0.5 * c[0];
-}
-
-__device__ __forceinline__ float cheb32(const float *c, int m, float a, float b, float x)
-{
-    x = fmaxf(x, a);
-    float d = 0.0f, dd = 0.0f;
-    const float y = (2.0f * x - a - b) / (b - a);
-    const float y2 = 2.0f * y;
-    for (int j = m - 1; j > 0; j--) {
-        const float sv = d;
-        d = fmaf(y2, d, c[j] - dd);
-        dd = sv;
-    }
-    return fmaf(y, d, 0.5f * c[0] - dd);
-}
-
 // 1/sqrt(x) in double precision: hardware estimate and two Newton steps.
 __device__ __forceinline__ double rsqrt64(double x)
 {

@@ -188,6 +188,31 @@ enum comd_kclass {
 int comd_timing(comd_ctx *ctx, int on);
 int comd_timing_read(comd_ctx *ctx, double *ms /* COMD_K_NCLASS */, int *launches /* COMD_K_NCLASS */);
 
+/* ---- Chebyshev coefficients and their adjoint ---- */
+
+/* Replace the value series of phi, rho and F (coef: 3*n doubles, in that
+ * order, on the domains the context was created with) and rebuild the three
+ * derivative series with the recurrence of src-flat/cheby.c:219-270 -- the
+ * runtime form of setChebPot (src-flat/cheby.c:99-121).  n must equal the
+ * context's cheb_n. */
+int comd_set_cheb(comd_ctx *ctx, const double *coef, int n);
+
+/* The six series in use (phi rho F dphi drho dF, 6*cheb_n doubles). */
+int comd_get_cheb(comd_ctx *ctx, double *coef6);
+
+/* Force-matching loss and its gradient with respect to the 3*n value
+ * coefficients (n <= 32).  Sets the coefficients (as comd_set_cheb), runs
+ * the EAM force through the series in the context's precision, and returns
+ *   loss = sum_i |f_i - f_target_i|^2 / (3 N) + w_e (E - e_target)^2
+ * (f the force array of the reference's convention, +grad U, eam.c:529-533,
+ * E the potential energy) and grad[3n] = dloss/dcoef from the adjoint
+ * kernels.  f_target holds 3 doubles per slot in the context's current slot
+ * layout (as comd_download returns f without its 4th component); empty
+ * slots are ignored.  No counterpart in the reference: it replaces fitting
+ * by finite differences over the force call. */
+int comd_eam_cheb_grad(comd_ctx *ctx, const double *coef, int n, const double *f_target, double e_target,
+                       double w_e, double *loss, double *grad);
+
 #ifdef __cplusplus
 }
 #endif

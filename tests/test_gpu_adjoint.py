@@ -1,0 +1,172 @@
+"""GPU tests of the Chebyshev-coefficient adjoint (comd_eam_cheb_grad) and
+its torch.autograd wrapper.
+
+The gradient of the force-matching loss
+    L(c) = sum_i |f_i(c) - t_i|^2 / (3N) + w_e (E(c) - E_t)^2
+with respect to the 3 x 32 value coefficients of phi, rho and F is checked
+against central finite differences with h = 1e-6 relative to each
+coefficient (floored for coefficients near zero), computed on the CPU oracle
+(an independent FP64 forward path) and on the GPU forward path.  Targets come
+from the FP64 forward pass at coefficients perturbed by 1 % Gaussian noise.
+"""
+import numpy as np
+import pytest
+
+import comd_amd as CA
+import oracle_py as O
+from helpers import POTS, seeded_atoms
+
+pytestmark = pytest.mark.gpu
+
+
+def config(cu500, n, seed, precision="fp64"):
+    xyz, p, bounds = seeded_atoms(n, 3.615, 63.55, temp=0.0, disp=0.1, seed=seed)
+    sim = CA.Simulation(cu500, bounds, defgrad=0.0, precision=precision, evaluation="cheby")
+    sim.bin_atoms(xyz, p)
+    sim.rebox()
+    return sim, xyz, bounds
+
+
+def perturbed(c0, seed, rel=0.01):
+    rng = np.random.default_rng(seed)
+    return c0 * (1.0 + rel * rng.standard_normal(c0.shape))
+
+
+def target_of(sim, coef):
+    """FP64 forward at coef: slot-layout forces and the energy."""
+    prec = sim.prm.precision
+    sim.set_precision("fp64")
+    sim.set_cheb(coef)
+    sim.force()
+    f = sim.download()["f"][:, :3].copy()
+    e = sim.energy()[0]
+    sim.set_precision("mixed" if prec == CA.PREC_MIXED else "fp64")
+    return f, e
+
+
+def steps(c0):
+    """h_k = 1e-6 |c_k|, floored at 1e-9 of the largest coefficient of the
+    same function (the high-order coefficients are tiny)."""
+    floor = 1e-3 * np.abs(c0).max(axis=1, keepdims=True)
+    return 1e-6 * np.maximum(np.abs(c0), floor)
+
+
+def oracle_loss(osim, opot, coef, ft, et, w_e, mask):
+    opot.set_cheb3(coef)
+    osim.force()
+    st = osim.state()
+    d = st["f"][:, :3][mask] - ft[mask]
+    n = mask.sum()
+    return (d * d).sum() / (3 * n) + w_e * (osim.info()["e"] - et) ** 2
+
+
+def compare(g, ref, what, tol):
+    err = np.abs(g - ref).max(axis=1) / np.abs(ref).max(axis=1)
+    print(f"[{what}] max |g - ref| / max|ref| per function (phi, rho, F): {err}", flush=True)
+    assert (err <= tol).all(), f"{what}: {err}"
+
+
+@pytest.mark.parametrize("w_e", [0.0, 1.0e-4])
+def test_adjoint_matches_oracle_finite_differences(gpu, cu500, w_e):
+    c0 = cu500.cheb[:3].copy()
+    sim, xyz, bounds = config(cu500, 5, 11)
+    ct = perturbed(c0, 5)
+    ft, et = target_of(sim, ct)
+    loss, g = sim.cheb_grad(c0, ft, et, w_e)
+    st = sim.download()
+    mask = (np.arange(64)[None, :] < st["natoms"][:, None]).reshape(-1)
+
+    opot = O.OraclePot.eam(POTS, "cu500")
+    opot.set_eval(True)
+    osim = O.OracleSim.from_atoms(opot, bounds, xyz)
+    osim.rebox()
+    assert np.array_equal(O.compact(osim.state())["id"], st["id"][mask])
+    l0 = oracle_loss(osim, opot, c0, ft, et, w_e, mask)
+    print(f"[adjoint w_e={w_e}] loss gpu {loss:.12e} oracle {l0:.12e}", flush=True)
+    assert abs(loss - l0) <= 1e-9 * abs(l0)
+    h = steps(c0)
+    fd = np.zeros_like(c0)
+    for k in range(3):
+        for j in range(c0.shape[1]):
+            cp, cm = c0.copy(), c0.copy()
+            cp[k, j] += h[k, j]
+            cm[k, j] -= h[k, j]
+            fd[k, j] = (oracle_loss(osim, opot, cp, ft, et, w_e, mask) -
+                        oracle_loss(osim, opot, cm, ft, et, w_e, mask)) / (2 * h[k, j])
+    compare(g, fd, f"adjoint vs oracle FD, w_e={w_e}", 1e-5)
+
+
+def test_adjoint_matches_gpu_finite_differences_32k(gpu, cu500):
+    """One configuration of the differentiation case: Cu FCC 20^3 (32 000
+    atoms), displacements sigma 0.1 A, targets at 1 %-perturbed coefficients;
+    finite differences of the GPU forward pass on 24 coefficients."""
+    c0 = cu500.cheb[:3].copy()
+    sim, _, _ = config(cu500, 20, 3)
+    ft, et = target_of(sim, perturbed(c0, 9))
+    w_e = 1.0 / 32000 ** 2
+    loss, g = sim.cheb_grad(c0, ft, et, w_e)
+    h = steps(c0)
+    picks = [(k, j) for k in range(3) for j in (0, 1, 2, 3, 5, 8, 13, 21)]
+    for k, j in picks:
+        cp, cm = c0.copy(), c0.copy()
+        cp[k, j] += h[k, j]
+        cm[k, j] -= h[k, j]
+        lp, _ = sim.cheb_grad(cp, ft, et, w_e)
+        lm, _ = sim.cheb_grad(cm, ft, et, w_e)
+        fd = (lp - lm) / (2 * h[k, j])
+        scale = np.abs(g[k]).max()
+        print(f"[32k] coef ({k},{j}) adjoint {g[k, j]:+.10e} fd {fd:+.10e}", flush=True)
+        assert abs(g[k, j] - fd) <= 1e-5 * scale
+
+
+def test_mixed_adjoint_against_fp64(gpu, cu500):
+    c0 = cu500.cheb[:3].copy()
+    sim, xyz, bounds = config(cu500, 6, 2)
+    ft, et = target_of(sim, perturbed(c0, 4))
+    l64, g64 = sim.cheb_grad(c0, ft, et, 1e-4)
+    sim.set_precision("mixed")
+    l32, g32 = sim.cheb_grad(c0, ft, et, 1e-4)
+    rel = np.abs(g32 - g64).max(axis=1) / np.abs(g64).max(axis=1)
+    print(f"[mixed adjoint] loss rel err {abs(l32 - l64) / l64:.3e}, gradient rel err per function {rel}", flush=True)
+    assert abs(l32 - l64) <= 1e-4 * l64
+    assert (rel <= 1e-3).all()
+
+
+def test_adjoint_is_deterministic_and_set_cheb_round_trips(gpu, cu500):
+    c0 = cu500.cheb[:3].copy()
+    sim, _, _ = config(cu500, 6, 8)
+    sim.force()
+    f_ref = sim.download()["f"].copy()
+    sim.set_cheb(c0)
+    # the derivative series rebuilt by the library equal the host fit's
+    np.testing.assert_array_equal(sim.get_cheb(), cu500.cheb)
+    sim.force()
+    np.testing.assert_array_equal(sim.download()["f"], f_ref)
+    ft, et = target_of(sim, perturbed(c0, 1))
+    a = sim.cheb_grad(c0, ft, et, 1e-4)
+    b = sim.cheb_grad(c0, ft, et, 1e-4)
+    assert a[0] == b[0]
+    np.testing.assert_array_equal(a[1], b[1])
+
+
+def test_autograd_function(gpu, cu500):
+    torch = pytest.importorskip("torch")
+    from comd_amd.autograd import force_matching_loss
+
+    c0 = cu500.cheb[:3].copy()
+    sims, targets = [], []
+    for seed in (1, 2):
+        sim, _, _ = config(cu500, 5, seed)
+        targets.append(target_of(sim, perturbed(c0, 20 + seed)))
+        sims.append(sim)
+    coef = torch.tensor(c0.reshape(-1), dtype=torch.float64, requires_grad=True)
+    loss = force_matching_loss(coef, sims, targets, 1e-4)
+    loss.backward()
+    expect = sum(s.cheb_grad(c0, t[0], t[1], 1e-4)[1] for s, t in zip(sims, targets))
+    np.testing.assert_allclose(coef.grad.numpy().reshape(3, -1), expect, rtol=0, atol=0)
+    # torch's own finite-difference check on a few coefficients
+    idx = torch.tensor([0, 1, 33, 34, 64, 65])
+
+    sub = coef.detach()[idx].clone().requires_grad_(True)
+    assert torch.autograd.gradcheck(lambda s: force_matching_loss(
+        coef.detach().index_put((idx,), s), sims, targets, 1e-4), (sub,), eps=1e-7, atol=1e-7, rtol=1e-3)
