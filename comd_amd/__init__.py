@@ -35,7 +35,7 @@ POT_EAM, POT_LJ = 1, 2
 EVAL_TABLE, EVAL_CHEB = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 K_REBIN, K_DENSITY, K_EMBED_FORCE, K_REDUCE, K_NCLASS = 0, 1, 2, 3, 4
-TUNE_WAVES, TUNE_BLOCKS_PER_CU, TUNE_TABLES_LDS = 1, 2, 3
+TUNE_WAVES, TUNE_BLOCKS_PER_CU, TUNE_TABLES_LDS, TUNE_BALANCED, TUNE_GRAPH = 1, 2, 3, 4, 5
 
 # unit constants of src-flat/include/constants.h:61-65
 BOHR_PER_ATU_TO_A_PER_S = 0.529 / 2.418e-17
@@ -126,6 +126,14 @@ def _load():
     hip.comd_last_error.restype = C.c_char_p
     hip.comd_timing.argtypes = [vp, C.c_int]
     hip.comd_timing_read.argtypes = [vp, _dp, _ip]
+    hip.comd_batch_create.argtypes = [C.c_int, C.POINTER(Params), C.POINTER(vp)]
+    hip.comd_batch_info.argtypes = [vp, C.c_int, _ip, _ip, _ip]
+    hip.comd_batch_bin_atoms.argtypes = [vp, C.c_int, C.c_int, _dp, _dp, C.c_int]
+    hip.comd_batch_upload.argtypes = [vp, C.c_int, _ip, _ip, _ip, _dp, _dp]
+    hip.comd_batch_download.argtypes = [vp, C.c_int, _ip, _ip, _ip, _dp, _dp, _dp, _dp, _dp]
+    hip.comd_batch_energy.argtypes = [vp, C.c_int, _dp, _dp, _dp]
+    hip.comd_batch_step.argtypes = [vp, C.c_int, C.c_double]
+    hip.comd_autotune.argtypes = [vp, _ip, _ip, _ip, _dp]
     hip.comd_set_cheb.argtypes = [vp, _dp, C.c_int]
     hip.comd_get_cheb.argtypes = [vp, _dp]
     hip.comd_eam_cheb_grad.argtypes = [vp, _dp, C.c_int, _dp, C.c_double, C.c_double, _dp, _dp]
@@ -413,12 +421,21 @@ class Simulation:
         self._check(self._hip.comd_energy(self.ctx, C.byref(pe), C.byref(ke), C.byref(st)))
         return pe.value, ke.value, st.value
 
-    def tune(self, waves_per_block=None, blocks_per_cu=None, tables_in_lds=None):
-        """Launch shape of the force kernels (comd_tune); results do not change."""
+    def tune(self, waves_per_block=None, blocks_per_cu=None, tables_in_lds=None, balanced=None, graph=None):
+        """Launch shape of the force kernels and step replay as a HIP graph
+        (comd_tune)."""
         for key, val in ((TUNE_WAVES, waves_per_block), (TUNE_BLOCKS_PER_CU, blocks_per_cu),
-                         (TUNE_TABLES_LDS, tables_in_lds)):
+                         (TUNE_TABLES_LDS, tables_in_lds), (TUNE_BALANCED, balanced), (TUNE_GRAPH, graph)):
             if val is not None:
                 self._check(self._hip.comd_tune(self.ctx, key, int(val)))
+
+    def autotune(self):
+        """Time the force passes over launch shapes and keep the fastest
+        (comd_autotune); returns (waves, blocks_per_cu, balanced, ms)."""
+        w, b, d = C.c_int(), C.c_int(), C.c_int()
+        ms = C.c_double()
+        self._check(self._hip.comd_autotune(self.ctx, C.byref(w), C.byref(b), C.byref(d), C.byref(ms)))
+        return w.value, b.value, d.value, ms.value
 
     def pair_count(self):
         """Half-pairs inside the cutoff found by the last density/LJ pass."""
@@ -473,6 +490,71 @@ class Simulation:
         self._check(self._hip.comd_eam_cheb_grad(self.ctx, _ptr(c), c.shape[1], _ptr(ft), float(e_target),
                                                  float(w_e), C.byref(loss), _ptr(grad)))
         return loss.value, grad
+
+
+class Batch(Simulation):
+    """Independent replicas in one context (comd_batch_create): every kernel
+    of a step covers all of them in one launch, and steps replay a captured
+    HIP graph.  `replicas` is a list of dicts with keys pot, bounds and
+    optionally defgrad (grid rule of allocDomains), evaluation, periodic."""
+
+    def __init__(self, replicas, precision="mixed", bf=1.0):
+        hip, _ = _load()
+        self._hip = hip
+        n = len(replicas)
+        arr = (Params * n)()
+        self.pots = []
+        self.nbx_list = []
+        for r, rep in enumerate(replicas):
+            prm = arr[r]
+            rep["pot"].fill(prm)
+            prm.periodic = 1 if rep.get("periodic", True) else 0
+            prm.precision = PREC_MIXED if precision == "mixed" else PREC_FP64
+            prm.eval = EVAL_CHEB if rep.get("evaluation", "table") == "cheby" else EVAL_TABLE
+            rc = hip.comd_grid_for(C.byref(prm), (C.c_double * 3)(*rep["bounds"]), float(rep.get("defgrad", 0.0)),
+                                   float(bf))
+            if rc:
+                raise ComdError(rc, f"grid of replica {r}")
+            self.pots.append(rep["pot"])
+            self.nbx_list.append(tuple(prm.nbx))
+        self._params = arr
+        ctx = C.c_void_p()
+        rc = hip.comd_batch_create(n, arr, C.byref(ctx))
+        if rc:
+            raise ComdError(rc, hip.comd_last_error(None).decode())
+        self.ctx = ctx
+        self.nrep = n
+        self.prm = arr[0]
+        cells = C.c_int()
+        hip.comd_sizes(ctx, C.byref(cells), None)
+        self.ncells = cells.value
+
+    def info(self, r):
+        """(first global cell, cells, atoms) of replica r."""
+        b, nc, na = C.c_int(), C.c_int(), C.c_int()
+        self._check(self._hip.comd_batch_info(self.ctx, r, C.byref(b), C.byref(nc), C.byref(na)))
+        return b.value, nc.value, na.value
+
+    def bin_atoms(self, r, xyz, p=None, itype=0):
+        x = np.ascontiguousarray(xyz, dtype=np.float64)
+        pp = np.ascontiguousarray(p, dtype=np.float64) if p is not None else None
+        self._check(self._hip.comd_batch_bin_atoms(self.ctx, r, x.shape[0], _ptr(x), _ptr(pp), int(itype)))
+
+    def download(self, r):
+        """Slot-layout state of replica r (as Simulation.download)."""
+        _, Cn, _ = self.info(r)
+        S = Cn * MAXATOMS
+        out = dict(natoms=np.zeros(Cn, np.int32), id=np.zeros(S, np.int32), itype=np.zeros(S, np.int32),
+                   r=np.zeros((S, 3)), p=np.zeros((S, 3)), f=np.zeros((S, 4)), rho=np.zeros(S), fi=np.zeros(S))
+        self._check(self._hip.comd_batch_download(
+            self.ctx, r, _ptr(out["natoms"], C.c_int), _ptr(out["id"], C.c_int), _ptr(out["itype"], C.c_int),
+            _ptr(out["r"]), _ptr(out["p"]), _ptr(out["f"]), _ptr(out["rho"]), _ptr(out["fi"])))
+        return out
+
+    def energy_of(self, r):
+        pe, ke, st = C.c_double(), C.c_double(), C.c_double()
+        self._check(self._hip.comd_batch_energy(self.ctx, r, C.byref(pe), C.byref(ke), C.byref(st)))
+        return pe.value, ke.value, st.value
 
 
 def fcc_simulation(pot, n, lat=None, temp=0.0, disp=0.0, seed=1, precision="fp64", evaluation="table",

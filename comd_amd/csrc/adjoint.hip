@@ -142,15 +142,15 @@ __global__ __launch_bounds__(64) void k_adj_pair(DevState d, PotDev P, AdjDev A)
     for (int c = blockIdx.x; c < d.ncells; c += gridDim.x) {
         const int nh = d.natoms[c];
         if (nh == 0) continue;
-        const int nent = d.ncells == 1 ? 1 : 27;
+        const int nent = d.g0.ncells == 1 ? 1 : 27;
         int ne = 0;
         if (L < nent) {
-            const int b = stencil_cell(d, c, L);
+            const int b = stencil_cell(d.g0, c, L);
             cell[L] = b;
             if (b >= 0) {
                 ne = d.natoms[b];
                 double sx, sy, sz;
-                corner_shift(d, c, b, sx, sy, sz);
+                corner_shift(d.g0, c, b, sx, sy, sz);
                 drb[3 * L + 0] = sx;
                 drb[3 * L + 1] = sy;
                 drb[3 * L + 2] = sz;

@@ -1,5 +1,10 @@
 // api.hip -- context management and the C-ABI entry points of
 // include/comd_hip.h.
+//
+// A context holds one or more replicas (independent simulations).  Their
+// link cells are numbered one after the other and share one set of slot
+// arrays, so every kernel of a step covers all replicas in one launch; a
+// single simulation is a context with one replica.
 #include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
@@ -23,22 +28,35 @@ struct TimedEvent {
     hipEvent_t a, b;
 };
 
+// One replica: its parameters, grid, potential and the device memory its
+// potential owns.
+struct Replica {
+    comd_params prm;  // table/series pointers cleared after creation
+    Grid g;
+    PotDev pot;
+    int ntot;
+    std::vector<void *> owned;  // device buffers of the potential
+    double *d_cheb;             // 6 x cn series (FP64), or null
+    float *d_cheb32;
+};
+
 struct comd_ctx {
-    comd_params prm;
-    int ncells;
+    int nrep;
+    std::vector<Replica> reps;
+    int precision;         // COMD_PREC_*, shared by all replicas
+    int ncells;            // cells of all replicas
     long nslots;
     int ntot;
-    int cur;               // which of the two moving-state buffers is current
     int have_atoms;
     int device;
     hipStream_t stream;
-    // moving state, two buffers
-    int *natoms[2], *id[2], *itype[2];
-    double *rx[2], *ry[2], *rz[2], *px[2], *py[2], *pz[2];
+    // moving state (slot layout)
+    int *natoms, *id, *itype;
+    double *rx, *ry, *rz, *px, *py, *pz;
     // outputs
     double *fx, *fy, *fz, *fe, *rho, *fi, *vir;
     int *nnb;
-    // scratch
+    // rebin scratch
     int *dest, *mvidx, *meta, *counts, *touched, *tc, *arr_cnt, *arr;
     int *mv_slot, *mv_rank, *mv_id, *mv_itype;
     double *mv_rx, *mv_ry, *mv_rz, *mv_px, *mv_py, *mv_pz;
@@ -49,20 +67,21 @@ struct comd_ctx {
     double *red_out;       // 3 doubles, device
     double *host_out;      // pinned 4 doubles
     unsigned *host_flags;  // pinned
-    // potential
-    PotDev pot;
-    double *d_tab[3];
-    double *d_cf64[2];
-    float *d_cf32[2];
-    double *d_cheb;
-    float *d_cheb32;
+    // replica descriptors on the device
+    Grid *d_grids;
+    int *d_cell_grid;
+    PotDev *d_pots;
     // Chebyshev-coefficient adjoint (allocated on first use)
     double *adj_g[3], *adj_lam, *adj_mu, *adj_ft, *adj_f2, *adj_part, *adj_out;
     int adj_nblocks;
-    std::vector<double> cheb_host;  // current value series phi, rho, F (3 x cn)
-    // force-kernel launch shape (comd_tune)
+    std::vector<double> cheb_host;  // current value series phi, rho, F (3 x cn) of replica 0
+    // force-kernel launch shape (comd_tune / comd_autotune)
     int ncu, max_lds;
     int force_waves, force_blocks_per_cu, tab_lds;
+    // one DKD step captured as a HIP graph (comd_steps)
+    int use_graph;
+    hipGraphExec_t graph;
+    double graph_dt;
     // timing
     int timing;
     std::vector<TimedEvent> events;
@@ -94,23 +113,16 @@ static int set_err(comd_ctx *ctx, int code, const char *fmt, ...)
 static DevState dev_state(const comd_ctx *c)
 {
     DevState d;
-    const int a = c->cur;
-    d.nbx0 = c->prm.nbx[0];
-    d.nbx1 = c->prm.nbx[1];
-    d.nbx2 = c->prm.nbx[2];
     d.ncells = c->ncells;
-    d.periodic = c->prm.periodic;
-    d.bs0 = c->prm.boxsize[0];
-    d.bs1 = c->prm.boxsize[1];
-    d.bs2 = c->prm.boxsize[2];
-    d.L0 = c->prm.bounds[0];
-    d.L1 = c->prm.bounds[1];
-    d.L2 = c->prm.bounds[2];
-    d.natoms = c->natoms[a];
-    d.id = c->id[a];
-    d.itype = c->itype[a];
-    d.rx = c->rx[a]; d.ry = c->ry[a]; d.rz = c->rz[a];
-    d.px = c->px[a]; d.py = c->py[a]; d.pz = c->pz[a];
+    d.ngrids = c->nrep;
+    d.g0 = c->reps[0].g;
+    d.grids = c->d_grids;
+    d.cell_grid = c->d_cell_grid;
+    d.natoms = c->natoms;
+    d.id = c->id;
+    d.itype = c->itype;
+    d.rx = c->rx; d.ry = c->ry; d.rz = c->rz;
+    d.px = c->px; d.py = c->py; d.pz = c->pz;
     d.fx = c->fx; d.fy = c->fy; d.fz = c->fz; d.fe = c->fe;
     d.rho = c->rho; d.fi = c->fi; d.vir = c->vir;
     d.nnb = c->nnb;
@@ -158,6 +170,12 @@ static void tend(comd_ctx *c, int cls, hipEvent_t a)
 
 // ---- library operations on the device (stream-ordered, no host sync)
 
+static void invalidate_graph(comd_ctx *c)
+{
+    if (c->graph) hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+}
+
 static void op_rebox(comd_ctx *c)
 {
     hipEvent_t ev = nullptr;
@@ -173,19 +191,19 @@ static void op_move(comd_ctx *c, double dtk, double dtd, int kick, int drift, in
     hipEvent_t ev = nullptr;
     tbegin(c, COMD_K_REBIN, &ev);
     DevState d = dev_state(c);
-    cm_launch_move(d, c->stream, dtk, dtd, kAmuToMe * c->prm.mass, kick, drift, rebin);
+    cm_launch_move(d, c->stream, dtk, dtd, kick, drift, rebin);
     tend(c, COMD_K_REBIN, ev);
 }
 
 static ForceCfg force_cfg(const comd_ctx *c, int kind, int mixed)
 {
     ForceCfg f;
-    f.waves = c->force_waves;
+    f.waves = c->nrep > 1 ? 1 : c->force_waves;
     f.tab_lds = c->tab_lds;
     f.max_lds = c->max_lds;
     f.ncu = c->ncu;
     // fewer waves per block when the per-wave LDS does not fit
-    while (f.waves > 1 && cm_force_lds_bytes(c->pot, kind, mixed, f.waves, 0) > f.max_lds) f.waves--;
+    while (f.waves > 1 && cm_force_lds_bytes(c->reps[0].pot, kind, mixed, f.waves, 0) > f.max_lds) f.waves--;
     long b = (long)c->ncu * c->force_blocks_per_cu;
     const long need = ((long)c->ncells + f.waves - 1) / f.waves;
     f.blocks = (int)(b < need ? b : need);
@@ -193,25 +211,80 @@ static ForceCfg force_cfg(const comd_ctx *c, int kind, int mixed)
     return f;
 }
 
+static bool any_eam(const comd_ctx *c)
+{
+    for (const Replica &r : c->reps)
+        if (r.prm.potential == COMD_POT_EAM) return true;
+    return false;
+}
+
 static void op_force(comd_ctx *c)
 {
     DevState d = dev_state(c);
-    const int mixed = c->prm.precision == COMD_PREC_MIXED;
-    c->pot.eval = c->prm.eval;
-    if (c->prm.potential == COMD_POT_LJ) {
-        hipEvent_t ev = nullptr;
+    const int mixed = c->precision == COMD_PREC_MIXED;
+    hipEvent_t ev = nullptr;
+    if (c->nrep > 1) {
         tbegin(c, COMD_K_DENSITY, &ev);
-        cm_launch_force(d, c->pot, CM_KIND_LJ, mixed, force_cfg(c, CM_KIND_LJ, mixed), c->stream);
+        cm_launch_force_batch(d, c->d_pots, 1, mixed, force_cfg(c, CM_KIND_EAM1, mixed), c->stream);
+        tend(c, COMD_K_DENSITY, ev);
+        if (any_eam(c)) {
+            tbegin(c, COMD_K_EMBED_FORCE, &ev);
+            cm_launch_force_batch(d, c->d_pots, 3, mixed, force_cfg(c, CM_KIND_EAM3, mixed), c->stream);
+            tend(c, COMD_K_EMBED_FORCE, ev);
+        }
+        return;
+    }
+    const PotDev &P = c->reps[0].pot;
+    if (P.kind == COMD_POT_LJ) {
+        tbegin(c, COMD_K_DENSITY, &ev);
+        cm_launch_force(d, P, CM_KIND_LJ, mixed, force_cfg(c, CM_KIND_LJ, mixed), c->stream);
         tend(c, COMD_K_DENSITY, ev);
     } else {
-        hipEvent_t ev = nullptr;
         tbegin(c, COMD_K_DENSITY, &ev);
-        cm_launch_force(d, c->pot, CM_KIND_EAM1, mixed, force_cfg(c, CM_KIND_EAM1, mixed), c->stream);
+        cm_launch_force(d, P, CM_KIND_EAM1, mixed, force_cfg(c, CM_KIND_EAM1, mixed), c->stream);
         tend(c, COMD_K_DENSITY, ev);
         tbegin(c, COMD_K_EMBED_FORCE, &ev);
-        cm_launch_force(d, c->pot, CM_KIND_EAM3, mixed, force_cfg(c, CM_KIND_EAM3, mixed), c->stream);
+        cm_launch_force(d, P, CM_KIND_EAM3, mixed, force_cfg(c, CM_KIND_EAM3, mixed), c->stream);
         tend(c, COMD_K_EMBED_FORCE, ev);
     }
+}
+
+// One velocity-Verlet step of src-flat/timestep.c:114-135 without the
+// trailing force: half drift, rebin, force, kick + half drift, rebin.
+static void op_step(comd_ctx *c, double dt)
+{
+    op_move(c, 0.0, dt / 2.0, 0, 1);
+    op_rebox(c);
+    op_force(c);
+    op_move(c, dt, dt / 2.0, 1, 1);
+    op_rebox(c);
+}
+
+// n steps: replays of the captured step graph when graphs are on and the
+// kernels are not being timed, else launched one by one.
+static int run_steps(comd_ctx *c, int n, double dt)
+{
+    if (!c->use_graph || c->timing || n < 2) {
+        for (int i = 0; i < n; i++) op_step(c, dt);
+        HIPCHK(c, hipGetLastError());
+        return COMD_OK;
+    }
+    if (c->graph && c->graph_dt != dt) invalidate_graph(c);
+    if (!c->graph) {
+        hipGraph_t g = nullptr;
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+        op_step(c, dt);
+        HIPCHK(c, hipStreamEndCapture(c->stream, &g));
+        const hipError_t e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            c->graph = nullptr;
+            return set_err(c, COMD_ERR_HIP, "graph instantiation: %s", hipGetErrorString(e));
+        }
+        c->graph_dt = dt;
+    }
+    for (int i = 0; i < n; i++) HIPCHK(c, hipGraphLaunch(c->graph, c->stream));
+    return COMD_OK;
 }
 
 static int check_flags(comd_ctx *c)
@@ -224,7 +297,13 @@ static int check_flags(comd_ctx *c)
     return COMD_OK;
 }
 
-// ---- C-ABI
+static int clear_flags(comd_ctx *c)
+{
+    HIPCHK(c, hipMemsetAsync(c->flags, 0, 4, c->stream));
+    return COMD_OK;
+}
+
+// ---- creation
 
 extern "C" {
 
@@ -243,22 +322,23 @@ int comd_grid_for(comd_params *p, const double bounds[3], double defgrad, double
     return COMD_OK;
 }
 
+}
+
 static void free_all(comd_ctx *c)
 {
-    void *ptrs[] = {c->fx, c->fy, c->fz, c->fe, c->rho, c->fi, c->vir, c->nnb, c->dest, c->mvidx, c->meta,
+    if (c->graph) hipGraphExecDestroy(c->graph);
+    void *ptrs[] = {c->natoms, c->id, c->itype, c->rx, c->ry, c->rz, c->px, c->py, c->pz,
+                    c->fx, c->fy, c->fz, c->fe, c->rho, c->fi, c->vir, c->nnb, c->dest, c->mvidx, c->meta,
                     c->counts, c->touched, c->tc, c->arr_cnt, c->arr, c->mv_slot, c->mv_rank, c->mv_id,
                     c->mv_itype, c->mv_rx, c->mv_ry, c->mv_rz, c->mv_px, c->mv_py, c->mv_pz,
-                    c->perm, c->flags, c->partials, c->red_out,
-                    c->d_tab[0], c->d_tab[1], c->d_tab[2], c->d_cf64[0], c->d_cf64[1], c->d_cf32[0], c->d_cf32[1],
-                    c->d_cheb, c->d_cheb32, c->adj_g[0], c->adj_g[1], c->adj_g[2], c->adj_lam, c->adj_mu,
+                    c->perm, c->flags, c->partials, c->red_out, c->d_grids, c->d_cell_grid, c->d_pots,
+                    c->adj_g[0], c->adj_g[1], c->adj_g[2], c->adj_lam, c->adj_mu,
                     c->adj_ft, c->adj_f2, c->adj_part, c->adj_out};
     for (void *p : ptrs)
         if (p) hipFree(p);
-    for (int b = 0; b < 2; b++) {
-        void *q[] = {c->natoms[b], c->id[b], c->itype[b], c->rx[b], c->ry[b], c->rz[b], c->px[b], c->py[b], c->pz[b]};
-        for (void *p : q)
+    for (Replica &r : c->reps)
+        for (void *p : r.owned)
             if (p) hipFree(p);
-    }
     if (c->host_out) hipHostFree(c->host_out);
     if (c->host_flags) hipHostFree(c->host_flags);
     for (auto e : c->pool) hipEventDestroy(e);
@@ -290,33 +370,136 @@ static void table_coefs(int n, double invdx, const double *vals /* values[-1] fi
     }
 }
 
-int comd_create(const comd_params *p, comd_ctx **out)
+static int check_params(const comd_params *p)
 {
-    if (!p || !out) return set_err(nullptr, COMD_ERR_ARG, "null argument");
-    *out = nullptr;
+    if (!p) return set_err(nullptr, COMD_ERR_ARG, "null parameters");
     if (p->potential != COMD_POT_EAM && p->potential != COMD_POT_LJ)
         return set_err(nullptr, COMD_ERR_ARG, "unknown potential %d", p->potential);
+    if (p->precision != COMD_PREC_FP64 && p->precision != COMD_PREC_MIXED)
+        return set_err(nullptr, COMD_ERR_ARG, "unknown precision %d", p->precision);
     for (int j = 0; j < 3; j++)
         if (p->nbx[j] < 1 || !(p->boxsize[j] > 0) || !(p->bounds[j] > 0))
             return set_err(nullptr, COMD_ERR_GRID, "bad grid in dimension %d", j);
+    if (!(p->cutoff > 0) || !(p->mass > 0)) return set_err(nullptr, COMD_ERR_ARG, "bad cutoff or mass");
     if (p->potential == COMD_POT_EAM) {
         for (int k = 0; k < 3; k++)
             if (p->tab_n[k] < 2 || !p->tab_values[k]) return set_err(nullptr, COMD_ERR_ARG, "EAM table %d missing", k);
         if (p->eval == COMD_EVAL_CHEB && (p->cheb_n < 2 || p->cheb_n > 64 || !p->cheb_coef))
             return set_err(nullptr, COMD_ERR_ARG, "Chebyshev coefficients missing");
     }
+    return COMD_OK;
+}
+
+// Device copy of one replica's potential (tables, per-interval coefficients,
+// Chebyshev series) and its PotDev.
+static bool setup_pot(const comd_params *p, Replica &R)
+{
+    PotDev &P = R.pot;
+    memset(&P, 0, sizeof P);
+    P.kind = p->potential;
+    P.eval = p->eval;
+    P.rc2 = p->cutoff * p->cutoff;
+    P.mass_amu = p->mass;
+    R.d_cheb = nullptr;
+    R.d_cheb32 = nullptr;
+    bool ok = true;
+    auto alloc = [&](size_t bytes) -> void * {
+        void *q = nullptr;
+        if (!ok || hipMalloc(&q, bytes < 16 ? 16 : bytes) != hipSuccess) {
+            ok = false;
+            return nullptr;
+        }
+        R.owned.push_back(q);
+        return q;
+    };
+    if (p->potential == COMD_POT_LJ) {
+        const double sg = p->lj_sigma;
+        P.lj_s6 = sg * sg * sg * sg * sg * sg;  // src-flat/ljForce.c:143
+        P.lj_eps = p->lj_epsilon;
+        return true;
+    }
+    for (int k = 0; k < 3 && ok; k++) {
+        const int n = p->tab_n[k];
+        double *t = (double *)alloc((size_t)(n + 3) * 8);
+        if (!ok) break;
+        ok = hipMemcpy(t, p->tab_values[k], (size_t)(n + 3) * 8, hipMemcpyHostToDevice) == hipSuccess;
+        P.tab[k] = t;
+        P.x0[k] = p->tab_x0[k];
+        P.xn[k] = p->tab_xn[k];
+        P.invdx[k] = p->tab_invdx[k];
+        P.n[k] = n;
+    }
+    for (int k = 0; ok && k < 2; k++) {
+        std::vector<double> c64;
+        std::vector<float> c32;
+        table_coefs(p->tab_n[k], p->tab_invdx[k], p->tab_values[k], c64, c32);
+        double *a = (double *)alloc(c64.size() * 8);
+        float *b = (float *)alloc(c32.size() * 4);
+        if (!ok) break;
+        ok = hipMemcpy(a, c64.data(), c64.size() * 8, hipMemcpyHostToDevice) == hipSuccess &&
+             hipMemcpy(b, c32.data(), c32.size() * 4, hipMemcpyHostToDevice) == hipSuccess;
+        P.cf64[k] = a;
+        P.cf32[k] = b;
+    }
+    P.same_grid = p->tab_n[0] == p->tab_n[1] && p->tab_x0[0] == p->tab_x0[1] && p->tab_xn[0] == p->tab_xn[1] &&
+                  p->tab_invdx[0] == p->tab_invdx[1];
+    if (ok && p->cheb_n >= 2 && p->cheb_coef) {
+        const int m = p->cheb_n;
+        R.d_cheb = (double *)alloc((size_t)6 * m * 8);
+        R.d_cheb32 = (float *)alloc((size_t)6 * m * 4);
+        if (ok) {
+            std::vector<float> c32(6 * m);
+            for (int i = 0; i < 6 * m; i++) c32[i] = (float)p->cheb_coef[i];
+            ok = hipMemcpy(R.d_cheb, p->cheb_coef, (size_t)6 * m * 8, hipMemcpyHostToDevice) == hipSuccess &&
+                 hipMemcpy(R.d_cheb32, c32.data(), (size_t)6 * m * 4, hipMemcpyHostToDevice) == hipSuccess;
+            P.cheb = R.d_cheb;
+            P.cheb32 = R.d_cheb32;
+            P.cn = m;
+            for (int k = 0; k < 3; k++) {
+                P.ca[k] = p->cheb_x0[k];
+                P.cb[k] = p->cheb_xn[k];
+            }
+        }
+    }
+    return ok;
+}
+
+// Device copies of the replicas' grids and potentials.
+static int upload_descriptors(comd_ctx *c)
+{
+    std::vector<Grid> gs(c->nrep);
+    std::vector<PotDev> ps(c->nrep);
+    for (int r = 0; r < c->nrep; r++) {
+        gs[r] = c->reps[r].g;
+        ps[r] = c->reps[r].pot;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(c->d_grids, gs.data(), gs.size() * sizeof(Grid), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_pots, ps.data(), ps.size() * sizeof(PotDev), hipMemcpyHostToDevice));
+    invalidate_graph(c);
+    return COMD_OK;
+}
+
+static int ctx_create(int nrep, const comd_params *prms, comd_ctx **out)
+{
+    if (!prms || !out || nrep < 1) return set_err(nullptr, COMD_ERR_ARG, "null argument");
+    *out = nullptr;
+    long ncl = 0;
+    for (int r = 0; r < nrep; r++) {
+        const int rc = check_params(&prms[r]);
+        if (rc) return rc;
+        if (prms[r].precision != prms[0].precision)
+            return set_err(nullptr, COMD_ERR_ARG, "replicas of one context share their precision");
+        ncl += (long)prms[r].nbx[0] * prms[r].nbx[1] * prms[r].nbx[2];
+    }
+    if (ncl > (1L << 26)) return set_err(nullptr, COMD_ERR_ARG, "too many cells");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_err(nullptr, COMD_ERR_NODEVICE, "no HIP device");
 
     comd_ctx *c = new comd_ctx();
-    c->prm = *p;
-    for (int k = 0; k < 3; k++) c->prm.tab_values[k] = nullptr;
-    c->prm.cheb_coef = nullptr;
-    const long ncl = (long)p->nbx[0] * p->nbx[1] * p->nbx[2];
-    if (ncl > (1L << 26)) {
-        delete c;
-        return set_err(nullptr, COMD_ERR_ARG, "too many cells");
-    }
+    c->nrep = nrep;
+    c->reps.resize(nrep);
+    c->precision = prms[0].precision;
     c->ncells = (int)ncl;
     c->nslots = ncl * CM_SLOTS;
     hipGetDevice(&c->device);
@@ -331,17 +514,15 @@ int comd_create(const comd_params *p, comd_ctx **out)
         else hipMemsetAsync(*ptr, 0, bytes < 16 ? 16 : bytes, c->stream);
     };
     const size_t S = (size_t)c->nslots, C = (size_t)c->ncells;
-    for (int b = 0; b < 1; b++) {
-        alloc((void **)&c->natoms[b], C * 4);
-        alloc((void **)&c->id[b], S * 4);
-        alloc((void **)&c->itype[b], S * 4);
-        alloc((void **)&c->rx[b], S * 8);
-        alloc((void **)&c->ry[b], S * 8);
-        alloc((void **)&c->rz[b], S * 8);
-        alloc((void **)&c->px[b], S * 8);
-        alloc((void **)&c->py[b], S * 8);
-        alloc((void **)&c->pz[b], S * 8);
-    }
+    alloc((void **)&c->natoms, C * 4);
+    alloc((void **)&c->id, S * 4);
+    alloc((void **)&c->itype, S * 4);
+    alloc((void **)&c->rx, S * 8);
+    alloc((void **)&c->ry, S * 8);
+    alloc((void **)&c->rz, S * 8);
+    alloc((void **)&c->px, S * 8);
+    alloc((void **)&c->py, S * 8);
+    alloc((void **)&c->pz, S * 8);
     alloc((void **)&c->fx, S * 8);
     alloc((void **)&c->fy, S * 8);
     alloc((void **)&c->fz, S * 8);
@@ -358,7 +539,7 @@ int comd_create(const comd_params *p, comd_ctx **out)
     alloc((void **)&c->tc, C * 4);
     alloc((void **)&c->arr_cnt, C * 4);
     alloc((void **)&c->arr, S * 4);
-    // one mover record per slot: a pass can never overflow the list
+    // one mover record per slot at most
     alloc((void **)&c->mv_slot, S * 4);
     alloc((void **)&c->mv_rank, S * 4);
     alloc((void **)&c->mv_id, S * 4);
@@ -374,68 +555,47 @@ int comd_create(const comd_params *p, comd_ctx **out)
     c->nparts = 1024;
     alloc((void **)&c->partials, (size_t)c->nparts * 3 * 8);
     alloc((void **)&c->red_out, 4 * 8);
+    alloc((void **)&c->d_grids, (size_t)nrep * sizeof(Grid));
+    alloc((void **)&c->d_pots, (size_t)nrep * sizeof(PotDev));
+    if (nrep > 1) alloc((void **)&c->d_cell_grid, C * 4);
     if (ok && hipHostMalloc((void **)&c->host_out, 4 * 8, 0) != hipSuccess) ok = false;
     if (ok && hipHostMalloc((void **)&c->host_flags, 16, 0) != hipSuccess) ok = false;
+    if (ok) ok = hipStreamSynchronize(c->stream) == hipSuccess;
 
-    // potential
-    PotDev &P = c->pot;
-    memset(&P, 0, sizeof P);
-    P.kind = p->potential;
-    P.eval = p->eval;
-    P.rc2 = p->cutoff * p->cutoff;
-    P.mass_amu = p->mass;
-    if (p->potential == COMD_POT_LJ) {
-        const double sg = p->lj_sigma;
-        P.lj_s6 = sg * sg * sg * sg * sg * sg;  // src-flat/ljForce.c:143
-        P.lj_eps = p->lj_epsilon;
-    } else if (ok) {
-        for (int k = 0; k < 3; k++) {
-            const int n = p->tab_n[k];
-            alloc((void **)&c->d_tab[k], (size_t)(n + 3) * 8);
-            if (!ok) break;
-            hipMemcpyAsync(c->d_tab[k], p->tab_values[k], (size_t)(n + 3) * 8, hipMemcpyHostToDevice, c->stream);
-            P.tab[k] = c->d_tab[k];
-            P.x0[k] = p->tab_x0[k];
-            P.xn[k] = p->tab_xn[k];
-            P.invdx[k] = p->tab_invdx[k];
-            P.n[k] = n;
-        }
-        for (int k = 0; ok && k < 2; k++) {
-            std::vector<double> c64;
-            std::vector<float> c32;
-            table_coefs(p->tab_n[k], p->tab_invdx[k], p->tab_values[k], c64, c32);
-            alloc((void **)&c->d_cf64[k], c64.size() * 8);
-            alloc((void **)&c->d_cf32[k], c32.size() * 4);
-            if (!ok) break;
-            hipMemcpyAsync(c->d_cf64[k], c64.data(), c64.size() * 8, hipMemcpyHostToDevice, c->stream);
-            hipMemcpyAsync(c->d_cf32[k], c32.data(), c32.size() * 4, hipMemcpyHostToDevice, c->stream);
-            hipStreamSynchronize(c->stream);
-            P.cf64[k] = c->d_cf64[k];
-            P.cf32[k] = c->d_cf32[k];
-        }
-        P.same_grid = p->tab_n[0] == p->tab_n[1] && p->tab_x0[0] == p->tab_x0[1] && p->tab_xn[0] == p->tab_xn[1] &&
-                      p->tab_invdx[0] == p->tab_invdx[1];
-        if (ok && p->cheb_n >= 2 && p->cheb_coef) {
-            const int m = p->cheb_n;
-            alloc((void **)&c->d_cheb, (size_t)6 * m * 8);
-            alloc((void **)&c->d_cheb32, (size_t)6 * m * 4);
-            if (ok) {
-                std::vector<float> c32(6 * m);
-                for (int i = 0; i < 6 * m; i++) c32[i] = (float)p->cheb_coef[i];
-                hipMemcpyAsync(c->d_cheb, p->cheb_coef, (size_t)6 * m * 8, hipMemcpyHostToDevice, c->stream);
-                hipMemcpyAsync(c->d_cheb32, c32.data(), (size_t)6 * m * 4, hipMemcpyHostToDevice, c->stream);
-                hipStreamSynchronize(c->stream);
-                P.cheb = c->d_cheb;
-                P.cheb32 = c->d_cheb32;
-                P.cn = m;
-                c->cheb_host.assign(p->cheb_coef, p->cheb_coef + 3 * m);
-                for (int k = 0; k < 3; k++) {
-                    P.ca[k] = p->cheb_x0[k];
-                    P.cb[k] = p->cheb_xn[k];
-                }
-            }
-        }
+    // replicas: grids (cells numbered replica after replica) and potentials
+    std::vector<int> cell_grid;
+    int base = 0;
+    for (int r = 0; ok && r < nrep; r++) {
+        Replica &R = c->reps[r];
+        const comd_params *p = &prms[r];
+        R.prm = *p;
+        for (int k = 0; k < 3; k++) R.prm.tab_values[k] = nullptr;
+        R.prm.cheb_coef = nullptr;
+        R.ntot = 0;
+        Grid &g = R.g;
+        memset(&g, 0, sizeof g);
+        g.nbx0 = p->nbx[0];
+        g.nbx1 = p->nbx[1];
+        g.nbx2 = p->nbx[2];
+        g.ncells = p->nbx[0] * p->nbx[1] * p->nbx[2];
+        g.cell_base = base;
+        g.periodic = p->periodic;
+        g.pot = r;
+        g.bs0 = p->boxsize[0];
+        g.bs1 = p->boxsize[1];
+        g.bs2 = p->boxsize[2];
+        g.L0 = p->bounds[0];
+        g.L1 = p->bounds[1];
+        g.L2 = p->bounds[2];
+        g.rmass = kAmuToMe * p->mass;  // src-flat/timestep.c:90
+        if (nrep > 1) cell_grid.insert(cell_grid.end(), (size_t)g.ncells, r);
+        base += g.ncells;
+        ok = setup_pot(p, R);
+        if (ok && r == 0 && R.pot.cheb) c->cheb_host.assign(p->cheb_coef, p->cheb_coef + 3 * p->cheb_n);
     }
+    if (ok && nrep > 1)
+        ok = hipMemcpy(c->d_cell_grid, cell_grid.data(), cell_grid.size() * 4, hipMemcpyHostToDevice) == hipSuccess;
+    if (ok) ok = upload_descriptors(c) == COMD_OK;
     if (!ok) {
         free_all(c);
         delete c;
@@ -452,6 +612,7 @@ int comd_create(const comd_params *p, comd_ctx **out)
     c->force_waves = 1;
     c->force_blocks_per_cu = 8;
     c->tab_lds = 1;
+    c->use_graph = nrep > 1;
     if (hipStreamSynchronize(c->stream) != hipSuccess) {
         free_all(c);
         delete c;
@@ -460,6 +621,166 @@ int comd_create(const comd_params *p, comd_ctx **out)
     *out = c;
     return COMD_OK;
 }
+
+static int check_rep(comd_ctx *c, int r)
+{
+    if (r < 0 || r >= c->nrep) return set_err(c, COMD_ERR_ARG, "replica %d out of range", r);
+    return COMD_OK;
+}
+
+// ---- state transfer of one replica
+
+static int upload_rep(comd_ctx *c, int r, const int *natoms, const int *id, const int *itype, const double *rr,
+                      const double *p)
+{
+    if (!natoms || !id || !rr) return set_err(c, COMD_ERR_ARG, "null argument");
+    const Grid &g = c->reps[r].g;
+    const size_t C = (size_t)g.ncells, S = C * CM_SLOTS;
+    const size_t c0 = (size_t)g.cell_base, s0 = c0 * CM_SLOTS;
+    int tot = 0;
+    for (size_t b = 0; b < C; b++) {
+        if (natoms[b] < 0 || natoms[b] > CM_SLOTS) return set_err(c, COMD_ERR_ARG, "cell %zu holds %d atoms", b, natoms[b]);
+        tot += natoms[b];
+    }
+    std::vector<double> buf(S);
+    std::vector<int> it(S, 0);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(c->natoms + c0, natoms, C * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->id + s0, id, S * 4, hipMemcpyHostToDevice));
+    if (itype) memcpy(it.data(), itype, S * 4);
+    HIPCHK(c, hipMemcpy(c->itype + s0, it.data(), S * 4, hipMemcpyHostToDevice));
+    double *dst[6] = {c->rx, c->ry, c->rz, c->px, c->py, c->pz};
+    for (int k = 0; k < 6; k++) {
+        const double *src = k < 3 ? rr : p;
+        if (!src) std::fill(buf.begin(), buf.end(), 0.0);
+        else
+            for (size_t s = 0; s < S; s++) buf[s] = src[3 * s + (k % 3)];
+        HIPCHK(c, hipMemcpy(dst[k] + s0, buf.data(), S * 8, hipMemcpyHostToDevice));
+    }
+    c->ntot += tot - c->reps[r].ntot;
+    c->reps[r].ntot = tot;
+    c->have_atoms = 1;
+    return COMD_OK;
+}
+
+static int download_rep(comd_ctx *c, int r, int *natoms, int *id, int *itype, double *rr, double *p, double *f,
+                        double *rho, double *fi)
+{
+    const Grid &g = c->reps[r].g;
+    const size_t C = (size_t)g.ncells, S = C * CM_SLOTS;
+    const size_t c0 = (size_t)g.cell_base, s0 = c0 * CM_SLOTS;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (natoms) HIPCHK(c, hipMemcpy(natoms, c->natoms + c0, C * 4, hipMemcpyDeviceToHost));
+    if (id) HIPCHK(c, hipMemcpy(id, c->id + s0, S * 4, hipMemcpyDeviceToHost));
+    if (itype) HIPCHK(c, hipMemcpy(itype, c->itype + s0, S * 4, hipMemcpyDeviceToHost));
+    std::vector<double> buf(S);
+    auto gather = [&](double *dst, int stride, int k, const double *src) -> int {
+        if (hipMemcpy(buf.data(), src + s0, S * 8, hipMemcpyDeviceToHost) != hipSuccess) return COMD_ERR_HIP;
+        for (size_t s = 0; s < S; s++) dst[stride * s + k] = buf[s];
+        return COMD_OK;
+    };
+    int rc = COMD_OK;
+    if (rr) {
+        rc |= gather(rr, 3, 0, c->rx);
+        rc |= gather(rr, 3, 1, c->ry);
+        rc |= gather(rr, 3, 2, c->rz);
+    }
+    if (p) {
+        rc |= gather(p, 3, 0, c->px);
+        rc |= gather(p, 3, 1, c->py);
+        rc |= gather(p, 3, 2, c->pz);
+    }
+    if (f) {
+        rc |= gather(f, 4, 0, c->fx);
+        rc |= gather(f, 4, 1, c->fy);
+        rc |= gather(f, 4, 2, c->fz);
+        rc |= gather(f, 4, 3, c->fe);
+    }
+    if (rho) HIPCHK(c, hipMemcpy(rho, c->rho + s0, S * 8, hipMemcpyDeviceToHost));
+    if (fi) HIPCHK(c, hipMemcpy(fi, c->fi + s0, S * 8, hipMemcpyDeviceToHost));
+    if (rc) return set_err(c, COMD_ERR_HIP, "download failed");
+    return COMD_OK;
+}
+
+static int bin_rep_device(comd_ctx *c, int r, int n, const double *d_xyz, const double *d_pxyz, int itype)
+{
+    const Grid &g = c->reps[r].g;
+    if (n < 0 || (n > 0 && !d_xyz)) return set_err(c, COMD_ERR_ARG, "bad atom list");
+    if ((long)n > (long)g.ncells * CM_SLOTS) return set_err(c, COMD_ERR_OVERFLOW, "more atoms than slots");
+    int rc = clear_flags(c);
+    if (rc) return rc;
+    DevState d = dev_state(c);
+    cm_launch_bin(d, g, c->stream, n, d_xyz, d_pxyz, itype, c->meta, c->arr);
+    rc = check_flags(c);
+    if (rc) return rc;
+    c->ntot += n - c->reps[r].ntot;
+    c->reps[r].ntot = n;
+    c->have_atoms = 1;
+    return COMD_OK;
+}
+
+static int bin_rep(comd_ctx *c, int r, int n, const double *xyz, const double *pxyz, int itype)
+{
+    if (n < 0 || (n > 0 && !xyz)) return set_err(c, COMD_ERR_ARG, "bad atom list");
+    double *dx = nullptr, *dp = nullptr;
+    const size_t bytes = (size_t)(n > 0 ? n : 1) * 3 * 8;
+    if (hipMalloc((void **)&dx, bytes) != hipSuccess) return set_err(c, COMD_ERR_NOMEM, "no memory for input");
+    if (pxyz && hipMalloc((void **)&dp, bytes) != hipSuccess) {
+        hipFree(dx);
+        return set_err(c, COMD_ERR_NOMEM, "no memory for input");
+    }
+    int rc = COMD_OK;
+    if (n > 0) {
+        if (hipMemcpy(dx, xyz, (size_t)n * 24, hipMemcpyHostToDevice) != hipSuccess ||
+            (dp && hipMemcpy(dp, pxyz, (size_t)n * 24, hipMemcpyHostToDevice) != hipSuccess))
+            rc = set_err(c, COMD_ERR_HIP, "input copy failed");
+    }
+    if (!rc) rc = bin_rep_device(c, r, n, dx, dp, itype);
+    hipStreamSynchronize(c->stream);
+    hipFree(dx);
+    if (dp) hipFree(dp);
+    return rc;
+}
+
+// PE (the reference's s->e), KE and the virial "stress" of one replica.
+static int energy_rep(comd_ctx *c, int r, double *pe, double *ke, double *stress)
+{
+    const Replica &R = c->reps[r];
+    hipEvent_t ev = nullptr;
+    tbegin(c, COMD_K_REDUCE, &ev);
+    DevState d = dev_state(c);
+    cm_launch_reduce(d, R.g, c->partials, c->nparts, c->stream);
+    cm_launch_reduce_final(c->partials, c->nparts, c->red_out, c->stream);
+    tend(c, COMD_K_REDUCE, ev);
+    HIPCHK(c, hipMemcpyAsync(c->host_out, c->red_out, 3 * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double e = c->host_out[0];
+    if (R.prm.potential == COMD_POT_LJ) e = e * 4.0 * R.prm.lj_epsilon * R.pot.lj_s6;  // ljForce.c:217
+    if (pe) *pe = e;
+    if (ke) *ke = c->host_out[1];
+    if (stress) *stress = c->host_out[2] / (R.g.L0 * R.g.L1 * R.g.L2);
+    return COMD_OK;
+}
+
+// Derivative series of a Chebyshev series on [a, b]: the recurrence of
+// src-flat/cheby.c:219-270 (last coefficient 0, d[j-1] = d[j+1] + 2 j c[j],
+// scaled by 2/(b-a)).
+static void cheb_derivative(int n, double a, double b, const double *c, double *d)
+{
+    d[n - 1] = 0.0;
+    d[n - 2] = 2.0 * (n - 1) * c[n - 1];
+    for (int j = n - 2; j > 0; j--) d[j - 1] = d[j + 1] + 2 * (j)*c[j];
+    const double con = 2.0 / (b - a);
+    for (int j = 0; j < n; j++) d[j] = d[j] * con;
+}
+
+// ---- C-ABI
+
+extern "C" {
+
+int comd_create(const comd_params *p, comd_ctx **out) { return ctx_create(1, p, out); }
+
+int comd_batch_create(int n, const comd_params *prms, comd_ctx **out) { return ctx_create(n, prms, out); }
 
 int comd_destroy(comd_ctx *ctx)
 {
@@ -483,119 +804,70 @@ int comd_sizes(const comd_ctx *ctx, int *ncells, int *natoms_total)
     return COMD_OK;
 }
 
+int comd_batch_info(comd_ctx *c, int r, int *cell_base, int *ncells, int *natoms)
+{
+    if (!c) return COMD_ERR_ARG;
+    int rc = check_rep(c, r);
+    if (rc) return rc;
+    if (cell_base) *cell_base = c->reps[r].g.cell_base;
+    if (ncells) *ncells = c->reps[r].g.ncells;
+    if (natoms) *natoms = c->reps[r].ntot;
+    return COMD_OK;
+}
+
 int comd_upload(comd_ctx *c, const int *natoms, const int *id, const int *itype, const double *r, const double *p)
 {
-    if (!c || !natoms || !id || !r) return COMD_ERR_ARG;
-    const size_t S = (size_t)c->nslots, C = (size_t)c->ncells;
-    int tot = 0;
-    for (size_t b = 0; b < C; b++) {
-        if (natoms[b] < 0 || natoms[b] > CM_SLOTS) return set_err(c, COMD_ERR_ARG, "cell %zu holds %d atoms", b, natoms[b]);
-        tot += natoms[b];
-    }
-    std::vector<double> buf(S);
-    std::vector<int> it(S, 0);
-    const int a = c->cur;
-    HIPCHK(c, hipMemcpyAsync(c->natoms[a], natoms, C * 4, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->id[a], id, S * 4, hipMemcpyHostToDevice, c->stream));
-    if (itype) memcpy(it.data(), itype, S * 4);
-    HIPCHK(c, hipMemcpyAsync(c->itype[a], it.data(), S * 4, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    double *dst[6] = {c->rx[a], c->ry[a], c->rz[a], c->px[a], c->py[a], c->pz[a]};
-    for (int k = 0; k < 6; k++) {
-        const double *src = k < 3 ? r : p;
-        if (!src) {
-            std::fill(buf.begin(), buf.end(), 0.0);
-        } else {
-            for (size_t s = 0; s < S; s++) buf[s] = src[3 * s + (k % 3)];
-        }
-        HIPCHK(c, hipMemcpyAsync(dst[k], buf.data(), S * 8, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-    }
-    c->ntot = tot;
-    c->have_atoms = 1;
-    return COMD_OK;
+    if (!c) return COMD_ERR_ARG;
+    return upload_rep(c, 0, natoms, id, itype, r, p);
+}
+
+int comd_batch_upload(comd_ctx *c, int r, const int *natoms, const int *id, const int *itype, const double *rr,
+                      const double *p)
+{
+    if (!c) return COMD_ERR_ARG;
+    int rc = check_rep(c, r);
+    return rc ? rc : upload_rep(c, r, natoms, id, itype, rr, p);
 }
 
 int comd_download(comd_ctx *c, int *natoms, int *id, int *itype, double *r, double *p, double *f, double *rho, double *fi)
 {
     if (!c) return COMD_ERR_ARG;
-    const size_t S = (size_t)c->nslots, C = (size_t)c->ncells;
-    const int a = c->cur;
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (natoms) HIPCHK(c, hipMemcpy(natoms, c->natoms[a], C * 4, hipMemcpyDeviceToHost));
-    if (id) HIPCHK(c, hipMemcpy(id, c->id[a], S * 4, hipMemcpyDeviceToHost));
-    if (itype) HIPCHK(c, hipMemcpy(itype, c->itype[a], S * 4, hipMemcpyDeviceToHost));
-    std::vector<double> buf(S);
-    auto gather = [&](double *dst, int stride, int k, const double *src) -> int {
-        if (hipMemcpy(buf.data(), src, S * 8, hipMemcpyDeviceToHost) != hipSuccess) return COMD_ERR_HIP;
-        for (size_t s = 0; s < S; s++) dst[stride * s + k] = buf[s];
-        return COMD_OK;
-    };
-    int rc = COMD_OK;
-    if (r) {
-        rc |= gather(r, 3, 0, c->rx[a]);
-        rc |= gather(r, 3, 1, c->ry[a]);
-        rc |= gather(r, 3, 2, c->rz[a]);
-    }
-    if (p) {
-        rc |= gather(p, 3, 0, c->px[a]);
-        rc |= gather(p, 3, 1, c->py[a]);
-        rc |= gather(p, 3, 2, c->pz[a]);
-    }
-    if (f) {
-        rc |= gather(f, 4, 0, c->fx);
-        rc |= gather(f, 4, 1, c->fy);
-        rc |= gather(f, 4, 2, c->fz);
-        rc |= gather(f, 4, 3, c->fe);
-    }
-    if (rho) HIPCHK(c, hipMemcpy(rho, c->rho, S * 8, hipMemcpyDeviceToHost));
-    if (fi) HIPCHK(c, hipMemcpy(fi, c->fi, S * 8, hipMemcpyDeviceToHost));
-    if (rc) return set_err(c, COMD_ERR_HIP, "download failed");
-    return COMD_OK;
+    return download_rep(c, 0, natoms, id, itype, r, p, f, rho, fi);
+}
+
+int comd_batch_download(comd_ctx *c, int r, int *natoms, int *id, int *itype, double *rr, double *p, double *f,
+                        double *rho, double *fi)
+{
+    if (!c) return COMD_ERR_ARG;
+    int rc = check_rep(c, r);
+    return rc ? rc : download_rep(c, r, natoms, id, itype, rr, p, f, rho, fi);
 }
 
 int comd_bin_atoms_device(comd_ctx *c, int n, const double *d_xyz, const double *d_pxyz, int itype)
 {
-    if (!c || n < 0 || (n > 0 && !d_xyz)) return COMD_ERR_ARG;
-    if ((long)n > c->nslots) return set_err(c, COMD_ERR_OVERFLOW, "more atoms than slots");
-    HIPCHK(c, hipMemsetAsync(c->flags, 0, 4, c->stream));
-    DevState d = dev_state(c);
-    cm_launch_bin(d, c->stream, n, d_xyz, d_pxyz, itype, c->meta, c->arr);
-    int rc = check_flags(c);
-    if (rc) return rc;
-    c->ntot = n;
-    c->have_atoms = 1;
-    return COMD_OK;
+    if (!c) return COMD_ERR_ARG;
+    return bin_rep_device(c, 0, n, d_xyz, d_pxyz, itype);
 }
 
 int comd_bin_atoms(comd_ctx *c, int n, const double *xyz, const double *pxyz, int itype)
 {
-    if (!c || n < 0 || (n > 0 && !xyz)) return COMD_ERR_ARG;
-    double *dx = nullptr, *dp = nullptr;
-    const size_t bytes = (size_t)(n > 0 ? n : 1) * 3 * 8;
-    if (hipMalloc((void **)&dx, bytes) != hipSuccess) return set_err(c, COMD_ERR_NOMEM, "no memory for input");
-    if (pxyz && hipMalloc((void **)&dp, bytes) != hipSuccess) {
-        hipFree(dx);
-        return set_err(c, COMD_ERR_NOMEM, "no memory for input");
-    }
-    int rc = COMD_OK;
-    if (n > 0) {
-        if (hipMemcpy(dx, xyz, (size_t)n * 24, hipMemcpyHostToDevice) != hipSuccess ||
-            (dp && hipMemcpy(dp, pxyz, (size_t)n * 24, hipMemcpyHostToDevice) != hipSuccess))
-            rc = set_err(c, COMD_ERR_HIP, "input copy failed");
-    }
-    if (!rc) rc = comd_bin_atoms_device(c, n, dx, dp, itype);
-    hipStreamSynchronize(c->stream);
-    hipFree(dx);
-    if (dp) hipFree(dp);
-    return rc;
+    if (!c) return COMD_ERR_ARG;
+    return bin_rep(c, 0, n, xyz, pxyz, itype);
+}
+
+int comd_batch_bin_atoms(comd_ctx *c, int r, int n, const double *xyz, const double *pxyz, int itype)
+{
+    if (!c) return COMD_ERR_ARG;
+    int rc = check_rep(c, r);
+    return rc ? rc : bin_rep(c, r, n, xyz, pxyz, itype);
 }
 
 int comd_rebox(comd_ctx *c)
 {
     if (!c) return COMD_ERR_ARG;
     if (!c->have_atoms) return set_err(c, COMD_ERR_STATE, "no atoms");
-    HIPCHK(c, hipMemsetAsync(c->flags, 0, 4, c->stream));
+    int rc = clear_flags(c);
+    if (rc) return rc;
     op_move(c, 0.0, 0.0, 0, 0);
     op_rebox(c);
     return check_flags(c);
@@ -614,38 +886,42 @@ int comd_energy(comd_ctx *c, double *pe, double *ke, double *stress)
 {
     if (!c) return COMD_ERR_ARG;
     if (!c->have_atoms) return set_err(c, COMD_ERR_STATE, "no atoms");
-    hipEvent_t ev = nullptr;
-    tbegin(c, COMD_K_REDUCE, &ev);
-    DevState d = dev_state(c);
-    cm_launch_reduce(d, c->pot, c->partials, c->nparts, c->stream);
-    cm_launch_reduce_final(c->partials, c->nparts, c->red_out, c->stream);
-    tend(c, COMD_K_REDUCE, ev);
-    HIPCHK(c, hipMemcpyAsync(c->host_out, c->red_out, 3 * 8, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    double e = c->host_out[0];
-    if (c->prm.potential == COMD_POT_LJ) e = e * 4.0 * c->prm.lj_epsilon * c->pot.lj_s6;  // ljForce.c:217
-    if (pe) *pe = e;
-    if (ke) *ke = c->host_out[1];
-    if (stress) *stress = c->host_out[2] / (c->prm.bounds[0] * c->prm.bounds[1] * c->prm.bounds[2]);
+    double a = 0, b = 0, s = 0;
+    for (int r = 0; r < c->nrep; r++) {
+        double x, y, z;
+        int rc = energy_rep(c, r, &x, &y, &z);
+        if (rc) return rc;
+        a += x;
+        b += y;
+        s += z;
+    }
+    if (pe) *pe = a;
+    if (ke) *ke = b;
+    if (stress) *stress = s;
     return COMD_OK;
+}
+
+int comd_batch_energy(comd_ctx *c, int r, double *pe, double *ke, double *stress)
+{
+    if (!c) return COMD_ERR_ARG;
+    int rc = check_rep(c, r);
+    if (rc) return rc;
+    if (!c->have_atoms) return set_err(c, COMD_ERR_STATE, "no atoms");
+    return energy_rep(c, r, pe, ke, stress);
 }
 
 int comd_nsteps(comd_ctx *c, int n, double dt_seconds, double *pe_out)
 {
     if (!c || n < 0) return COMD_ERR_ARG;
     if (!c->have_atoms) return set_err(c, COMD_ERR_STATE, "no atoms");
-    HIPCHK(c, hipMemsetAsync(c->flags, 0, 4, c->stream));
+    int rc = clear_flags(c);
+    if (rc) return rc;
     const double dt = dt_seconds * kBohrPerAtuToAPerS;  // timestep.c:126
-    for (int i = 0; i < n; i++) {
-        op_move(c, 0.0, dt / 2.0, 0, 1);
-        op_rebox(c);
-        op_force(c);
-        op_move(c, dt, dt / 2.0, 1, 1);
-        op_rebox(c);
-    }
+    rc = run_steps(c, n, dt);
+    if (rc) return rc;
     op_force(c);  // timestep.c:136
     HIPCHK(c, hipGetLastError());
-    int rc = check_flags(c);
+    rc = check_flags(c);
     if (rc) return rc;
     if (pe_out) return comd_energy(c, pe_out, nullptr, nullptr);
     return COMD_OK;
@@ -655,24 +931,19 @@ int comd_steps(comd_ctx *c, int n, double dt_seconds)
 {
     if (!c || n < 0) return COMD_ERR_ARG;
     if (!c->have_atoms) return set_err(c, COMD_ERR_STATE, "no atoms");
-    HIPCHK(c, hipMemsetAsync(c->flags, 0, 4, c->stream));
-    const double dt = dt_seconds * kBohrPerAtuToAPerS;
-    for (int i = 0; i < n; i++) {
-        op_move(c, 0.0, dt / 2.0, 0, 1);
-        op_rebox(c);
-        op_force(c);
-        op_move(c, dt, dt / 2.0, 1, 1);
-        op_rebox(c);
-    }
-    HIPCHK(c, hipGetLastError());
-    return COMD_OK;
+    int rc = clear_flags(c);
+    if (rc) return rc;
+    return run_steps(c, n, dt_seconds * kBohrPerAtuToAPerS);
 }
+
+int comd_batch_step(comd_ctx *c, int n, double dt_seconds) { return comd_steps(c, n, dt_seconds); }
 
 int comd_advance_positions(comd_ctx *c, double dt_internal)
 {
     if (!c) return COMD_ERR_ARG;
     if (!c->have_atoms) return set_err(c, COMD_ERR_STATE, "no atoms");
-    HIPCHK(c, hipMemsetAsync(c->flags, 0, 4, c->stream));
+    int rc = clear_flags(c);
+    if (rc) return rc;
     op_move(c, 0.0, dt_internal, 0, 1);
     op_rebox(c);
     return check_flags(c);
@@ -692,16 +963,23 @@ int comd_advance_velocity(comd_ctx *c, double dt_internal)
 int comd_set_precision(comd_ctx *c, int precision)
 {
     if (!c || (precision != COMD_PREC_FP64 && precision != COMD_PREC_MIXED)) return COMD_ERR_ARG;
-    c->prm.precision = precision;
+    if (precision != c->precision) invalidate_graph(c);
+    c->precision = precision;
+    for (Replica &R : c->reps) R.prm.precision = precision;
     return COMD_OK;
 }
 
 int comd_set_eval(comd_ctx *c, int eval)
 {
     if (!c || (eval != COMD_EVAL_TABLE && eval != COMD_EVAL_CHEB)) return COMD_ERR_ARG;
-    if (eval == COMD_EVAL_CHEB && !c->pot.cheb) return set_err(c, COMD_ERR_ARG, "no Chebyshev coefficients");
-    c->prm.eval = eval;
-    return COMD_OK;
+    for (Replica &R : c->reps)
+        if (R.prm.potential == COMD_POT_EAM && eval == COMD_EVAL_CHEB && !R.pot.cheb)
+            return set_err(c, COMD_ERR_ARG, "no Chebyshev coefficients");
+    for (Replica &R : c->reps) {
+        R.prm.eval = eval;
+        R.pot.eval = eval;
+    }
+    return upload_descriptors(c);
 }
 
 int comd_device_info(int *count, char *name, int name_len)
@@ -726,20 +1004,83 @@ int comd_tune(comd_ctx *c, int key, int value)
     case COMD_TUNE_WAVES_PER_BLOCK:
         if (value < 1 || value > 4) return set_err(c, COMD_ERR_ARG, "waves per block must be 1..4");
         c->force_waves = value;
-        return COMD_OK;
+        break;
     case COMD_TUNE_BLOCKS_PER_CU:
         if (value < 1 || value > 64) return set_err(c, COMD_ERR_ARG, "blocks per CU must be 1..64");
         c->force_blocks_per_cu = value;
-        return COMD_OK;
+        break;
     case COMD_TUNE_TABLES_IN_LDS:
         c->tab_lds = value ? 1 : 0;
-        return COMD_OK;
+        break;
     case COMD_TUNE_BALANCED_DRAIN:
-        c->pot.balanced = value ? 1 : 0;
-        return COMD_OK;
+        for (Replica &R : c->reps) R.pot.balanced = value ? 1 : 0;
+        return upload_descriptors(c);
+    case COMD_TUNE_GRAPH:
+        c->use_graph = value ? 1 : 0;
+        break;
     default:
         return set_err(c, COMD_ERR_ARG, "unknown tuning key %d", key);
     }
+    invalidate_graph(c);
+    return COMD_OK;
+}
+
+int comd_autotune(comd_ctx *c, int *waves, int *blocks_per_cu, int *balanced, double *best_ms)
+{
+    if (!c) return COMD_ERR_ARG;
+    if (!c->have_atoms) return set_err(c, COMD_ERR_STATE, "no atoms");
+    const int timing_saved = c->timing;
+    c->timing = 0;
+    hipEvent_t a, b;
+    HIPCHK(c, hipEventCreate(&a));
+    HIPCHK(c, hipEventCreate(&b));
+    const int wave_opts[] = {1, 2, 4};
+    const int bpc_opts[] = {4, 8, 12, 16};
+    const int nw = c->nrep > 1 ? 1 : 3;
+    double best = 1e300;
+    int bw = c->force_waves, bb = c->force_blocks_per_cu, bd = c->reps[0].pot.balanced;
+    int rc = COMD_OK;
+    for (int dr = 0; dr < 2 && !rc; dr++) {
+        for (Replica &R : c->reps) R.pot.balanced = dr;
+        rc = upload_descriptors(c);
+        for (int iw = 0; iw < nw && !rc; iw++)
+            for (int ib = 0; ib < 4 && !rc; ib++) {
+                c->force_waves = wave_opts[iw];
+                c->force_blocks_per_cu = bpc_opts[ib];
+                op_force(c);  // warm-up
+                float t = 0, tmin = 1e30f;
+                for (int k = 0; k < 3; k++) {
+                    hipEventRecord(a, c->stream);
+                    op_force(c);
+                    hipEventRecord(b, c->stream);
+                    if (hipEventSynchronize(b) != hipSuccess) {
+                        rc = set_err(c, COMD_ERR_HIP, "autotune timing failed");
+                        break;
+                    }
+                    hipEventElapsedTime(&t, a, b);
+                    tmin = std::min(tmin, t);
+                }
+                if (tmin < best) {
+                    best = tmin;
+                    bw = c->force_waves;
+                    bb = c->force_blocks_per_cu;
+                    bd = dr;
+                }
+            }
+    }
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    c->force_waves = bw;
+    c->force_blocks_per_cu = bb;
+    for (Replica &R : c->reps) R.pot.balanced = bd;
+    if (!rc) rc = upload_descriptors(c);
+    c->timing = timing_saved;
+    if (rc) return rc;
+    if (waves) *waves = bw;
+    if (blocks_per_cu) *blocks_per_cu = bb;
+    if (balanced) *balanced = bd;
+    if (best_ms) *best_ms = best;
+    return COMD_OK;
 }
 
 int comd_download_aux(comd_ctx *c, int *nnb, double *vir)
@@ -759,7 +1100,7 @@ int comd_pair_count(comd_ctx *c, long long *half_pairs)
     std::vector<int> nn(S), na(C);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipMemcpy(nn.data(), c->nnb, S * 4, hipMemcpyDeviceToHost));
-    HIPCHK(c, hipMemcpy(na.data(), c->natoms[c->cur], C * 4, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(na.data(), c->natoms, C * 4, hipMemcpyDeviceToHost));
     long long tot = 0;
     for (size_t b = 0; b < C; b++)
         for (int i = 0; i < na[b]; i++) tot += nn[b * CM_SLOTS + i];
@@ -807,38 +1148,27 @@ int comd_timing_read(comd_ctx *c, double *ms, int *launches)
     return COMD_OK;
 }
 
-
-// ---- Chebyshev coefficients and their adjoint
-
-// Derivative series of a Chebyshev series on [a, b]: the recurrence of
-// src-flat/cheby.c:219-270 (last coefficient 0, d[j-1] = d[j+1] + 2 j c[j],
-// scaled by 2/(b-a)).
-static void cheb_derivative(int n, double a, double b, const double *c, double *d)
-{
-    d[n - 1] = 0.0;
-    d[n - 2] = 2.0 * (n - 1) * c[n - 1];
-    for (int j = n - 2; j > 0; j--) d[j - 1] = d[j + 1] + 2 * (j)*c[j];
-    const double con = 2.0 / (b - a);
-    for (int j = 0; j < n; j++) d[j] = d[j] * con;
-}
+// ---- Chebyshev coefficients and their adjoint (single-replica contexts)
 
 int comd_set_cheb(comd_ctx *c, const double *coef, int n)
 {
     if (!c || !coef) return COMD_ERR_ARG;
-    if (c->prm.potential != COMD_POT_EAM || !c->pot.cheb)
+    if (c->nrep != 1) return set_err(c, COMD_ERR_ARG, "Chebyshev coefficients are set per single-replica context");
+    Replica &R = c->reps[0];
+    if (R.prm.potential != COMD_POT_EAM || !R.pot.cheb)
         return set_err(c, COMD_ERR_ARG, "the context has no Chebyshev series");
-    const int m = c->pot.cn;
+    const int m = R.pot.cn;
     if (n != m) return set_err(c, COMD_ERR_ARG, "expected %d coefficients per series, got %d", m, n);
     std::vector<double> all((size_t)6 * m);
     std::vector<float> all32((size_t)6 * m);
     for (int k = 0; k < 3; k++) {
         for (int j = 0; j < m; j++) all[(size_t)k * m + j] = coef[(size_t)k * m + j];
-        cheb_derivative(m, c->pot.ca[k], c->pot.cb[k], coef + (size_t)k * m, all.data() + (size_t)(3 + k) * m);
+        cheb_derivative(m, R.pot.ca[k], R.pot.cb[k], coef + (size_t)k * m, all.data() + (size_t)(3 + k) * m);
     }
     for (int i = 0; i < 6 * m; i++) all32[i] = (float)all[i];
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipMemcpy(c->d_cheb, all.data(), all.size() * 8, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->d_cheb32, all32.data(), all32.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(R.d_cheb, all.data(), all.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(R.d_cheb32, all32.data(), all32.size() * 4, hipMemcpyHostToDevice));
     c->cheb_host.assign(coef, coef + 3 * m);
     return COMD_OK;
 }
@@ -846,9 +1176,10 @@ int comd_set_cheb(comd_ctx *c, const double *coef, int n)
 int comd_get_cheb(comd_ctx *c, double *coef6)
 {
     if (!c || !coef6) return COMD_ERR_ARG;
-    if (!c->pot.cheb) return set_err(c, COMD_ERR_ARG, "the context has no Chebyshev series");
+    const Replica &R = c->reps[0];
+    if (!R.pot.cheb) return set_err(c, COMD_ERR_ARG, "the context has no Chebyshev series");
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipMemcpy(coef6, c->d_cheb, (size_t)6 * c->pot.cn * 8, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(coef6, R.d_cheb, (size_t)6 * R.pot.cn * 8, hipMemcpyDeviceToHost));
     return COMD_OK;
 }
 
@@ -862,6 +1193,7 @@ int comd_eam_cheb_grad(comd_ctx *c, const double *coef, int n, const double *f_t
                        cm_adjoint_max_coefs());
     int rc = comd_set_cheb(c, coef, n);
     if (rc) return rc;
+    Replica &R = c->reps[0];
     const int m = n;
     const int NM = cm_adjoint_moments();
     const size_t S = (size_t)c->nslots;
@@ -883,21 +1215,21 @@ int comd_eam_cheb_grad(comd_ctx *c, const double *coef, int n, const double *f_t
     }
     // F'' series: the derivative recurrence applied twice
     std::vector<double> d1(m), d2(m);
-    cheb_derivative(m, c->pot.ca[2], c->pot.cb[2], coef + 2 * m, d1.data());
-    cheb_derivative(m, c->pot.ca[2], c->pot.cb[2], d1.data(), d2.data());
+    cheb_derivative(m, R.pot.ca[2], R.pot.cb[2], coef + 2 * m, d1.data());
+    cheb_derivative(m, R.pot.ca[2], R.pot.cb[2], d1.data(), d2.data());
     HIPCHK(c, hipMemcpy(c->adj_f2, d2.data(), (size_t)m * 8, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->adj_ft, f_target, S * 3 * 8, hipMemcpyHostToDevice));
 
     // forward: forces and energy through the series
-    const int eval_saved = c->prm.eval;
-    c->prm.eval = COMD_EVAL_CHEB;
+    const int eval_saved = R.pot.eval;
+    R.pot.eval = COMD_EVAL_CHEB;
     op_force(c);
-    c->prm.eval = eval_saved;
+    R.pot.eval = eval_saved;
     double E = 0.0;
     rc = comd_energy(c, &E, nullptr, nullptr);
     if (rc) return rc;
 
-    const int mixed = c->prm.precision == COMD_PREC_MIXED;
+    const int mixed = c->precision == COMD_PREC_MIXED;
     const double N = (double)c->ntot;
     CmAdjoint J;
     J.gx = c->adj_g[0];
@@ -910,7 +1242,7 @@ int comd_eam_cheb_grad(comd_ctx *c, const double *coef, int n, const double *f_t
     J.part = c->adj_part;
     J.nblocks = c->adj_nblocks;
     J.out = c->adj_out;
-    PotDev P = c->pot;
+    PotDev P = R.pot;
     P.eval = COMD_EVAL_CHEB;
     DevState d = dev_state(c);
     cm_launch_adj_resid(d, J, c->adj_ft, 2.0 / (3.0 * N), c->stream);
@@ -930,7 +1262,7 @@ int comd_eam_cheb_grad(comd_ctx *c, const double *coef, int n, const double *f_t
         std::vector<double> e(m, 0.0);
         e[j] = 1.0;
         col.assign(m, 0.0);
-        cheb_derivative(m, c->pot.ca[fn], c->pot.cb[fn], e.data(), col.data());
+        cheb_derivative(m, R.pot.ca[fn], R.pot.cb[fn], e.data(), col.data());
     };
     std::vector<double> col;
     for (int j = 0; j < m; j++) {

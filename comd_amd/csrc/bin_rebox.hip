@@ -36,13 +36,14 @@ __device__ static inline void mark_touched(const DevState &d, int c)
 // timestep.c:76-104) and the per-atom destination of reBoxAll
 // (domains.c:356-377); departing atoms are recorded as movers.  One block of
 // 64 threads per cell.
-__global__ __launch_bounds__(64) void k_move(DevState d, double dtk, double dtd, double rmass, int kick, int drift,
-                                             int rebin)
+__global__ __launch_bounds__(64) void k_move(DevState d, double dtk, double dtd, int kick, int drift, int rebin)
 {
     const int c = blockIdx.x;
     const int t = threadIdx.x;
     const int n = d.natoms[c];
     if (t >= n) return;
+    const Grid g = grid_of(d, c);
+    const double rmass = g.rmass;
     const int s = c * CM_SLOTS + t;
     double x = d.rx[s], y = d.ry[s], z = d.rz[s];
     double px = d.px[s], py = d.py[s], pz = d.pz[s];
@@ -64,25 +65,25 @@ __global__ __launch_bounds__(64) void k_move(DevState d, double dtk, double dtd,
     }
     if (!rebin) return;
     int ix, iy, iz;
-    cell_xyz(d, c, ix, iy, iz);
+    cell_xyz(g, c, ix, iy, iz);
     int nx = ix, ny = iy, nz = iz;
     double ax = x, ay = y, az = z;
-    if (x < 0.0) { nx = ix - 1; ax = x + d.bs0; } else if (x >= d.bs0) { nx = ix + 1; ax = x - d.bs0; }
-    if (y < 0.0) { ny = iy - 1; ay = y + d.bs1; } else if (y >= d.bs1) { ny = iy + 1; ay = y - d.bs1; }
-    if (z < 0.0) { nz = iz - 1; az = z + d.bs2; } else if (z >= d.bs2) { nz = iz + 1; az = z - d.bs2; }
-    const int j = cell_wrap(d, nx, ny, nz);
+    if (x < 0.0) { nx = ix - 1; ax = x + g.bs0; } else if (x >= g.bs0) { nx = ix + 1; ax = x - g.bs0; }
+    if (y < 0.0) { ny = iy - 1; ay = y + g.bs1; } else if (y >= g.bs1) { ny = iy + 1; ay = y - g.bs1; }
+    if (z < 0.0) { nz = iz - 1; az = z + g.bs2; } else if (z >= g.bs2) { nz = iz + 1; az = z - g.bs2; }
+    const int j = cell_wrap(g, nx, ny, nz);
     const int dst = (j < 0 || j == c) ? -1 : j;
     d.dest[s] = dst;
     if (dst < 0) return;
     if (dst > c) {
         // the arrival is examined again when dst is scanned
         int jx, jy, jz;
-        cell_xyz(d, dst, jx, jy, jz);
+        cell_xyz(g, dst, jx, jy, jz);
         int mx = jx, my = jy, mz = jz;
-        if (ax < 0.0) mx = jx - 1; else if (ax >= d.bs0) mx = jx + 1;
-        if (ay < 0.0) my = jy - 1; else if (ay >= d.bs1) my = jy + 1;
-        if (az < 0.0) mz = jz - 1; else if (az >= d.bs2) mz = jz + 1;
-        const int j2 = cell_wrap(d, mx, my, mz);
+        if (ax < 0.0) mx = jx - 1; else if (ax >= g.bs0) mx = jx + 1;
+        if (ay < 0.0) my = jy - 1; else if (ay >= g.bs1) my = jy + 1;
+        if (az < 0.0) mz = jz - 1; else if (az >= g.bs2) mz = jz + 1;
+        const int j2 = cell_wrap(g, mx, my, mz);
         if (j2 >= 0 && j2 != dst) atomicOr(d.flags, (unsigned)CM_FLAG_MULTIHOP);
     }
     const int m = atomicAdd(&d.counts[0], 1);  // capacity: one record per slot
@@ -265,18 +266,19 @@ __global__ void k_rebox_serial(DevState d)
     if (!(*d.flags & CM_FLAG_MULTIHOP)) return;
     unsigned fl = 0;
     for (int c = 0; c < d.ncells; c++) {
+        const Grid g = grid_of(d, c);
         int ix, iy, iz;
-        cell_xyz(d, c, ix, iy, iz);
+        cell_xyz(g, c, ix, iy, iz);
         int i = 0;
         while (i < d.natoms[c]) {
             const int o = c * CM_SLOTS + i;
             const double x = d.rx[o], y = d.ry[o], z = d.rz[o];
             int nx = ix, ny = iy, nz = iz;
             double ax = x, ay = y, az = z;
-            if (x < 0.0) { nx = ix - 1; ax = x + d.bs0; } else if (x >= d.bs0) { nx = ix + 1; ax = x - d.bs0; }
-            if (y < 0.0) { ny = iy - 1; ay = y + d.bs1; } else if (y >= d.bs1) { ny = iy + 1; ay = y - d.bs1; }
-            if (z < 0.0) { nz = iz - 1; az = z + d.bs2; } else if (z >= d.bs2) { nz = iz + 1; az = z - d.bs2; }
-            const int j = cell_wrap(d, nx, ny, nz);
+            if (x < 0.0) { nx = ix - 1; ax = x + g.bs0; } else if (x >= g.bs0) { nx = ix + 1; ax = x - g.bs0; }
+            if (y < 0.0) { ny = iy - 1; ay = y + g.bs1; } else if (y >= g.bs1) { ny = iy + 1; ay = y - g.bs1; }
+            if (z < 0.0) { nz = iz - 1; az = z + g.bs2; } else if (z >= g.bs2) { nz = iz + 1; az = z - g.bs2; }
+            const int j = cell_wrap(g, nx, ny, nz);
             if (j < 0 || j == c) {
                 i++;
                 continue;
@@ -306,26 +308,26 @@ __global__ void k_rebox_serial(DevState d)
 // (src-flat/domains.c:237-291): wrap once into [0,L), cell =
 // floor(x/boxsize), append.  Stable counting sort: atomic tickets per cell,
 // then each cell's entries are ordered by input index.
-__device__ static inline bool wrap_bin(const DevState &d, double &x, double &y, double &z, int &cell)
+__device__ static inline bool wrap_bin(const Grid &g, double &x, double &y, double &z, int &cell)
 {
-    if (x < 0.0) x += d.L0; else if (x >= d.L0) x -= d.L0;
-    if (y < 0.0) y += d.L1; else if (y >= d.L1) y -= d.L1;
-    if (z < 0.0) z += d.L2; else if (z >= d.L2) z -= d.L2;
-    const int ix = (int)floor(x / d.bs0);
-    const int iy = (int)floor(y / d.bs1);
-    const int iz = (int)floor(z / d.bs2);
-    if (ix < 0 || ix >= d.nbx0 || iy < 0 || iy >= d.nbx1 || iz < 0 || iz >= d.nbx2) return false;
-    cell = ix + d.nbx0 * iy + d.nbx0 * d.nbx1 * iz;
+    if (x < 0.0) x += g.L0; else if (x >= g.L0) x -= g.L0;
+    if (y < 0.0) y += g.L1; else if (y >= g.L1) y -= g.L1;
+    if (z < 0.0) z += g.L2; else if (z >= g.L2) z -= g.L2;
+    const int ix = (int)floor(x / g.bs0);
+    const int iy = (int)floor(y / g.bs1);
+    const int iz = (int)floor(z / g.bs2);
+    if (ix < 0 || ix >= g.nbx0 || iy < 0 || iy >= g.nbx1 || iz < 0 || iz >= g.nbx2) return false;
+    cell = g.cell_base + ix + g.nbx0 * iy + g.nbx0 * g.nbx1 * iz;
     return true;
 }
 
-__global__ void k_bin_key(DevState d, int n, const double *xyz, int *cnt, int *tmp)
+__global__ void k_bin_key(DevState d, Grid g, int n, const double *xyz, int *cnt, int *tmp)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
     int cell;
-    if (!wrap_bin(d, x, y, z, cell)) {
+    if (!wrap_bin(g, x, y, z, cell)) {
         atomicOr(d.flags, (unsigned)CM_FLAG_OUTSIDE);
         return;
     }
@@ -334,11 +336,11 @@ __global__ void k_bin_key(DevState d, int n, const double *xyz, int *cnt, int *t
     else atomicOr(d.flags, (unsigned)CM_FLAG_OVERFLOW);
 }
 
-__global__ __launch_bounds__(64) void k_bin_place(DevState d, const double *xyz, const double *pxyz, int itype,
-                                                  const int *cnt, const int *tmp)
+__global__ __launch_bounds__(64) void k_bin_place(DevState d, Grid g, const double *xyz, const double *pxyz,
+                                                  int itype, const int *cnt, const int *tmp)
 {
     __shared__ int sidx[CM_SLOTS];
-    const int c = blockIdx.x;
+    const int c = g.cell_base + blockIdx.x;
     const int t = threadIdx.x;
     int n = cnt[c];
     if (n > CM_SLOTS) n = CM_SLOTS;
@@ -351,9 +353,9 @@ __global__ __launch_bounds__(64) void k_bin_place(DevState d, const double *xyz,
     for (int u = 0; u < n; u++) rank += (sidx[u] < me);
     double x = xyz[3 * me], y = xyz[3 * me + 1], z = xyz[3 * me + 2];
     int cell;
-    wrap_bin(d, x, y, z, cell);
+    wrap_bin(g, x, y, z, cell);
     double cx, cy, cz;
-    cell_corner(d, c, cx, cy, cz);
+    cell_corner(g, c, cx, cy, cz);
     const int s = c * CM_SLOTS + rank;
     d.id[s] = me;
     d.itype[s] = itype;
@@ -373,10 +375,9 @@ __global__ void k_fill_int(int *p, int v, long n)
 // ---------------------------------------------------------------------------
 // host launchers
 
-void cm_launch_move(const DevState &d, hipStream_t st, double dtk, double dtd, double rmass, int kick, int drift,
-                    int rebin)
+void cm_launch_move(const DevState &d, hipStream_t st, double dtk, double dtd, int kick, int drift, int rebin)
 {
-    hipLaunchKernelGGL(k_move, dim3(d.ncells), dim3(64), 0, st, d, dtk, dtd, rmass, kick, drift, rebin);
+    hipLaunchKernelGGL(k_move, dim3(d.ncells), dim3(64), 0, st, d, dtk, dtd, kick, drift, rebin);
 }
 
 void cm_launch_rebox(const DevState &d, hipStream_t st)
@@ -397,10 +398,10 @@ void cm_launch_fill_int(int *p, int v, long n, hipStream_t st)
     hipLaunchKernelGGL(k_fill_int, dim3((unsigned)blocks), dim3(256), 0, st, p, v, n);
 }
 
-void cm_launch_bin(const DevState &d, hipStream_t st, int n, const double *xyz, const double *pxyz, int itype,
-                   int *cnt, int *tmp)
+void cm_launch_bin(const DevState &d, const Grid &g, hipStream_t st, int n, const double *xyz, const double *pxyz,
+                   int itype, int *cnt, int *tmp)
 {
-    cm_launch_fill_int(cnt, 0, d.ncells, st);
-    if (n > 0) hipLaunchKernelGGL(k_bin_key, dim3((n + 255) / 256), dim3(256), 0, st, d, n, xyz, cnt, tmp);
-    hipLaunchKernelGGL(k_bin_place, dim3(d.ncells), dim3(64), 0, st, d, xyz, pxyz, itype, cnt, tmp);
+    cm_launch_fill_int(cnt + g.cell_base, 0, g.ncells, st);
+    if (n > 0) hipLaunchKernelGGL(k_bin_key, dim3((n + 255) / 256), dim3(256), 0, st, d, g, n, xyz, cnt, tmp);
+    hipLaunchKernelGGL(k_bin_place, dim3(g.ncells), dim3(64), 0, st, d, g, xyz, pxyz, itype, cnt, tmp);
 }

@@ -16,12 +16,23 @@ enum {
     CM_FLAG_MULTIHOP = 4,  // an arrival would move again in the same rebin pass
 };
 
+// Link-cell grid of one replica.  All replicas of a context share the slot
+// arrays; replica r owns the global cells [cell_base, cell_base + ncells)
+// and slot i of global cell c is c*64+i.
+struct Grid {
+    int nbx0, nbx1, nbx2, ncells, cell_base, periodic, pot, pad;
+    double bs0, bs1, bs2;  // boxsize
+    double L0, L1, L2;     // bounds
+    double rmass;          // mass in the reference's internal unit (1822.83 m_amu)
+};
+
 // Plain view of the device arrays passed to kernels by value.
 struct DevState {
-    int nbx0, nbx1, nbx2, ncells;
-    int periodic;
-    double bs0, bs1, bs2;      // boxsize
-    double L0, L1, L2;         // bounds
+    int ncells;              // cells of all replicas
+    int ngrids;              // replicas
+    Grid g0;                 // replica 0 (the only one of a single simulation)
+    const Grid *grids;       // device copy of all replicas' grids
+    const int *cell_grid;    // per global cell: its replica (used when ngrids > 1)
     // moving state (slot layout, updated in place by the rebin)
     int *natoms;
     int *id;
@@ -48,35 +59,43 @@ struct DevState {
     unsigned *flags;    // error flag word
 };
 
-__device__ __forceinline__ void cell_xyz(const DevState &d, int c, int &ix, int &iy, int &iz)
+// Grid of the replica that owns global cell c.
+__device__ __forceinline__ Grid grid_of(const DevState &d, int c)
 {
-    ix = c % d.nbx0;
-    iy = (c / d.nbx0) % d.nbx1;
-    iz = (c / d.nbx0 / d.nbx1) % d.nbx2;
+    return d.ngrids == 1 ? d.g0 : d.grids[d.cell_grid[c]];
 }
 
-// Periodic wrap of a cell triple, or -1 outside a non-periodic box
-// (src-flat/domains.c:297-325).
-__device__ __forceinline__ int cell_wrap(const DevState &d, int ix, int iy, int iz)
+__device__ __forceinline__ void cell_xyz(const Grid &g, int c, int &ix, int &iy, int &iz)
 {
-    if (!d.periodic) {
-        if (ix < 0 || ix >= d.nbx0 || iy < 0 || iy >= d.nbx1 || iz < 0 || iz >= d.nbx2) return -1;
+    const int l = c - g.cell_base;
+    ix = l % g.nbx0;
+    iy = (l / g.nbx0) % g.nbx1;
+    iz = (l / g.nbx0 / g.nbx1) % g.nbx2;
+}
+
+// Periodic wrap of a cell triple to its global index, or -1 outside a
+// non-periodic box (src-flat/domains.c:297-325).
+__device__ __forceinline__ int cell_wrap(const Grid &g, int ix, int iy, int iz)
+{
+    if (!g.periodic) {
+        if (ix < 0 || ix >= g.nbx0 || iy < 0 || iy >= g.nbx1 || iz < 0 || iz >= g.nbx2) return -1;
     }
-    if (ix < 0) ix += d.nbx0; else if (ix >= d.nbx0) ix -= d.nbx0;
-    if (iy < 0) iy += d.nbx1; else if (iy >= d.nbx1) iy -= d.nbx1;
-    if (iz < 0) iz += d.nbx2; else if (iz >= d.nbx2) iz -= d.nbx2;
-    return ix + iy * d.nbx0 + iz * d.nbx0 * d.nbx1;
+    if (ix < 0) ix += g.nbx0; else if (ix >= g.nbx0) ix -= g.nbx0;
+    if (iy < 0) iy += g.nbx1; else if (iy >= g.nbx1) iy -= g.nbx1;
+    if (iz < 0) iz += g.nbx2; else if (iz >= g.nbx2) iz -= g.nbx2;
+    return g.cell_base + ix + iy * g.nbx0 + iz * g.nbx0 * g.nbx1;
 }
 
 // Stencil entry e (0..26) of cell c in the reference's order: x outer, y,
-// z inner (src-flat/domains.c:419-426); -1 outside a non-periodic box.
-__device__ __forceinline__ int stencil_cell(const DevState &d, int c, int e)
+// z inner (src-flat/domains.c:419-426); -1 outside a non-periodic box.  A
+// one-cell grid sees only itself (domains.c:406-410).
+__device__ __forceinline__ int stencil_cell(const Grid &g, int c, int e)
 {
-    if (d.ncells == 1) return e == 0 ? 0 : -1;
+    if (g.ncells == 1) return e == 0 ? c : -1;
     int ix, iy, iz;
-    cell_xyz(d, c, ix, iy, iz);
-    int ox = e / 9 - 1, oy = (e / 3) % 3 - 1, oz = e % 3 - 1;
-    return cell_wrap(d, ix + ox, iy + oy, iz + oz);
+    cell_xyz(g, c, ix, iy, iz);
+    const int ox = e / 9 - 1, oy = (e / 3) % 3 - 1, oz = e % 3 - 1;
+    return cell_wrap(g, ix + ox, iy + oy, iz + oz);
 }
 
 // dcenter[c][k] = boxsize[k] * ix_k (src-flat/domains.c:191-197).  The
@@ -84,14 +103,14 @@ __device__ __forceinline__ int stencil_cell(const DevState &d, int c, int e)
 // with contraction: the shift of a cell against itself must be exactly 0
 // (fma(b, i, -b*i) is the rounding error of b*i, not 0), or an atom would
 // see itself as a neighbour at distance ~1e-16.
-__device__ __forceinline__ void cell_corner(const DevState &d, int c, double &x, double &y, double &z)
+__device__ __forceinline__ void cell_corner(const Grid &g, int c, double &x, double &y, double &z)
 {
 #pragma clang fp contract(off)
     int ix, iy, iz;
-    cell_xyz(d, c, ix, iy, iz);
-    x = d.bs0 * (double)ix;
-    y = d.bs1 * (double)iy;
-    z = d.bs2 * (double)iz;
+    cell_xyz(g, c, ix, iy, iz);
+    x = g.bs0 * (double)ix;
+    y = g.bs1 * (double)iy;
+    z = g.bs2 * (double)iz;
 }
 
 // Minimum-imaged corner difference dcenter[i]-dcenter[j]
@@ -107,13 +126,13 @@ __device__ __forceinline__ double image1(double dd, double L, int periodic)
 }
 
 // dcenter[c] - dcenter[b], minimum-imaged, uncontracted (0 for b == c).
-__device__ __forceinline__ void corner_shift(const DevState &d, int c, int b, double &sx, double &sy, double &sz)
+__device__ __forceinline__ void corner_shift(const Grid &g, int c, int b, double &sx, double &sy, double &sz)
 {
 #pragma clang fp contract(off)
     double cx, cy, cz, bx, by, bz;
-    cell_corner(d, c, cx, cy, cz);
-    cell_corner(d, b, bx, by, bz);
-    sx = image1(cx - bx, d.L0, d.periodic);
-    sy = image1(cy - by, d.L1, d.periodic);
-    sz = image1(cz - bz, d.L2, d.periodic);
+    cell_corner(g, c, cx, cy, cz);
+    cell_corner(g, b, bx, by, bz);
+    sx = image1(cx - bx, g.L0, g.periodic);
+    sy = image1(cy - by, g.L1, g.periodic);
+    sz = image1(cz - bz, g.L2, g.periodic);
 }

@@ -72,14 +72,18 @@ struct ForceCfg {
 };
 int cm_force_lds_bytes(const PotDev &P, int kind, int mixed, int waves, int tab_lds);
 
-void cm_launch_move(const DevState &d, hipStream_t st, double dtk, double dtd, double rmass, int kick, int drift,
-                    int rebin);
+void cm_launch_move(const DevState &d, hipStream_t st, double dtk, double dtd, int kick, int drift, int rebin);
 void cm_launch_rebox(const DevState &d, hipStream_t st);
-void cm_launch_bin(const DevState &d, hipStream_t st, int n, const double *xyz, const double *pxyz, int itype,
-                   int *cnt, int *tmp);
+void cm_launch_bin(const DevState &d, const Grid &g, hipStream_t st, int n, const double *xyz, const double *pxyz,
+                   int itype, int *cnt, int *tmp);
 void cm_launch_fill_int(int *p, int v, long n, hipStream_t st);
 void cm_launch_force(const DevState &d, const PotDev &pot, int kind, int mixed, const ForceCfg &cfg, hipStream_t st);
-void cm_launch_reduce(const DevState &d, const PotDev &pot, double *partials, int nparts, hipStream_t st);
+// All replicas of a batch in one launch (pass 1: EAM density or LJ; pass 3:
+// EAM embedding force); pots: device array indexed by Grid::pot.
+void cm_launch_force_batch(const DevState &d, const PotDev *pots, int pass, int mixed, const ForceCfg &cfg,
+                           hipStream_t st);
+// Energies of one replica's cells: per-block partial (PE, KE, virial).
+void cm_launch_reduce(const DevState &d, const Grid &g, double *partials, int nparts, hipStream_t st);
 void cm_launch_reduce_final(const double *partials, int nparts, double *out, hipStream_t st);
 
 // Device buffers and scalars of the Chebyshev-coefficient adjoint

@@ -245,6 +245,204 @@ __device__ __forceinline__ void pair_term(const PotDev &P, double dx, double dy,
 
 // ---------------------------------------------------------------------------
 
+// One home cell: stencil, halo staging, candidate tests, pair terms,
+// reduction of the lanes' partial sums and the epilogue.  The 64 lanes of a
+// wave call it together.
+template <int KIND, int EVAL, int MIXED>
+__device__ __forceinline__ void pair_cell(const DevState &d, const Grid &g, const PotDev &P, const WaveLds &w, int c,
+                                          int L)
+{
+    const double rc2 = P.rc2;
+    const int nh = d.natoms[c];
+    if (nh == 0) return;
+
+    // ---- stencil entries, counts and corner shifts
+    const int nent = g.ncells == 1 ? 1 : 27;
+    int ne = 0;
+    if (L < nent) {
+        const int b = stencil_cell(g, c, L);
+        w.cell[L] = b;
+        if (b >= 0) {
+            ne = d.natoms[b];
+            double sx, sy, sz;
+            corner_shift(g, c, b, sx, sy, sz);
+            w.drb[3 * L + 0] = sx;
+            w.drb[3 * L + 1] = sy;
+            w.drb[3 * L + 2] = sz;
+        }
+    }
+    int incl = ne;
+    for (int o = 1; o < 32; o <<= 1) {
+        const int t = __shfl_up(incl, o, 64);
+        if (L >= o) incl += t;
+    }
+    const int H = __shfl(incl, 31, 64);
+    const int nonempty = __popcll(__ballot(ne > 0));
+    if (L < nent) w.off[L] = incl - ne;
+    if (L == 0) w.off[nent] = H;
+
+    // ---- home atoms: lane (a, s)
+    const int S = 64 / nh;
+    const bool act = L < S * nh;
+    const int a = act ? L % nh : 0;
+    const int sg = act ? L / nh : 0;
+    const int hs = c * CM_SLOTS + a;
+    const double xa = d.rx[hs], ya = d.ry[hs], za = d.rz[hs];
+    const double fia = KIND == CM_KIND_EAM3 ? d.fi[hs] : 0.0;
+    Acc A = {0, 0, 0, 0, 0, 0};
+    int npair = 0;
+    wave_sync();
+
+    for (int h0 = 0; h0 < H; h0 += HMAX) {
+        const int hb = H - h0 < HMAX ? H - h0 : HMAX;
+        // stage the batch: lane L takes halo atoms L, L+64, ...; its
+        // stencil entry only moves forward
+        {
+            int e = 0;
+            for (int h = h0 + L; h < h0 + hb; h += 64) {
+                while (w.off[e + 1] <= h) e++;
+                const int sl = w.cell[e] * CM_SLOTS + (h - w.off[e]);
+                const double x = d.rx[sl], y = d.ry[sl], z = d.rz[sl];
+                w.hx[h - h0] = x - w.drb[3 * e + 0];
+                w.hy[h - h0] = y - w.drb[3 * e + 1];
+                w.hz[h - h0] = z - w.drb[3 * e + 2];
+                if (KIND == CM_KIND_EAM3) w.hf[h - h0] = d.fi[sl];
+            }
+        }
+        wave_sync();
+
+        int cnt = 0;
+        // drain all queues: atom a's entries (the concatenation of the
+        // queues of lanes a, a+nh, ...) are taken round-robin by its lanes
+        auto drain = [&]() {
+            if (!P.balanced) {
+                // each lane drains its own queue
+                for (int k = 0; __ballot(k < cnt) != 0; k++) {
+                    if (k < cnt) {
+                        const int j = w.q[k * 64 + L];
+                        const double dx = xa - w.hx[j], dy = ya - w.hy[j], dz = za - w.hz[j];
+                        const double r2 = dx * dx + dy * dy + dz * dz;
+                        pair_term<KIND, EVAL, MIXED>(P, dx, dy, dz, r2, fia,
+                                                     KIND == CM_KIND_EAM3 ? w.hf[j] : 0.0, A);
+                        npair++;
+                    }
+                }
+                cnt = 0;
+                return;
+            }
+            w.qn[L] = act ? cnt : 0;
+            wave_sync();
+            int tot = 0;
+            if (act)
+                for (int s = 0; s < S; s++) tot += w.qn[a + s * nh];
+            int v = sg, sp = 0, base = 0;
+            while (__ballot(act && v < tot) != 0) {
+                if (act && v < tot) {
+                    int cs = w.qn[a + sp * nh];
+                    while (v >= base + cs) {
+                        base += cs;
+                        sp++;
+                        cs = w.qn[a + sp * nh];
+                    }
+                    const int j = w.q[(v - base) * 64 + a + sp * nh];
+                    const double dx = xa - w.hx[j], dy = ya - w.hy[j], dz = za - w.hz[j];
+                    const double r2 = dx * dx + dy * dy + dz * dz;
+                    pair_term<KIND, EVAL, MIXED>(P, dx, dy, dz, r2, fia, KIND == CM_KIND_EAM3 ? w.hf[j] : 0.0, A);
+                    npair++;
+                    v += S;
+                }
+            }
+            cnt = 0;
+            wave_sync();
+        };
+        const int trips = (hb + S - 1) / S;
+        for (int it0 = 0; it0 < trips; it0 += UNROLL) {
+            bool acc[UNROLL];
+            int jj[UNROLL];
+#pragma unroll
+            for (int u = 0; u < UNROLL; u++) {
+                const int j = sg + (it0 + u) * S;
+                const bool valid = act && j < hb;
+                const int jc = valid ? j : 0;
+                const double dx = xa - w.hx[jc], dy = ya - w.hy[jc], dz = za - w.hz[jc];
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                const bool in = KIND == CM_KIND_EAM3 ? (r2 < rc2) : (r2 <= rc2);
+                acc[u] = valid && in && r2 > 0.0;
+                jj[u] = j;
+            }
+#pragma unroll
+            for (int u = 0; u < UNROLL; u++)
+                if (acc[u]) {
+                    w.q[cnt * 64 + L] = (uint16_t)jj[u];
+                    cnt++;
+                }
+            if (__ballot(cnt > QCAP - UNROLL) != 0) drain();
+        }
+        drain();
+    }
+
+    // ---- combine the S partial sums of each home atom, in s order
+    // (scratch in the halo area, free once the last batch is drained)
+    double *red = w.hx;
+    int *redn = (int *)(red + 6 * 64);
+    red[0 * 64 + L] = act ? A.x : 0.0;
+    red[1 * 64 + L] = act ? A.y : 0.0;
+    red[2 * 64 + L] = act ? A.z : 0.0;
+    red[3 * 64 + L] = act ? A.e : 0.0;
+    red[4 * 64 + L] = act ? A.rho : 0.0;
+    red[5 * 64 + L] = act ? A.vir : 0.0;
+    redn[L] = act ? npair : 0;
+    wave_sync();
+    if (L < nh) {
+        double sx = 0, sy = 0, sz = 0, se = 0, sr = 0, sv = 0;
+        int sn = 0;
+        for (int s = 0; s < S; s++) {
+            const int l = L + s * nh;
+            sx += red[0 * 64 + l];
+            sy += red[1 * 64 + l];
+            sz += red[2 * 64 + l];
+            se += red[3 * 64 + l];
+            sr += red[4 * 64 + l];
+            sv += red[5 * 64 + l];
+            sn += redn[l];
+        }
+        const int o = c * CM_SLOTS + L;
+        if (KIND == CM_KIND_EAM1) {
+            // pass 2, src-flat/eam.c:576-594
+            double fv, fp;
+            if (EVAL == COMD_EVAL_CHEB) {
+                fv = cheb64(P.cheb + 2 * P.cn, P.cn, P.ca[2], P.cb[2], sr);
+                fp = cheb64(P.cheb + 5 * P.cn, P.cn, P.ca[2], P.cb[2], sr);
+            } else {
+                interp64(P.tab[2], P.x0[2], P.xn[2], P.invdx[2], P.n[2], sr, fv, fp);
+            }
+            d.fx[o] = sx;
+            d.fy[o] = sy;
+            d.fz[o] = sz;
+            d.fe[o] = se + fv;
+            d.rho[o] = sr;
+            d.fi[o] = fp;
+            d.nnb[o] = sn;
+        } else if (KIND == CM_KIND_EAM3) {
+            d.fx[o] += sx;
+            d.fy[o] += sy;
+            d.fz[o] += sz;
+            d.vir[o] = sv;
+        } else {
+            // kinetic term of src-flat/ljForce.c:175, once per non-empty
+            // stencil cell, with the mass in amu
+            const double px = d.px[o];
+            d.fx[o] = sx;
+            d.fy[o] = sy;
+            d.fz[o] = sz;
+            d.fe[o] = se;
+            d.vir[o] = sv - (double)nonempty * (px * px / P.mass_amu);
+            d.nnb[o] = sn;
+        }
+    }
+    wave_sync();
+}
+
 template <int KIND, int EVAL, int MIXED>
 __global__ __launch_bounds__(256) void k_pair(DevState d, PotDev P)
 {
@@ -263,213 +461,59 @@ __global__ __launch_bounds__(256) void k_pair(DevState d, PotDev P)
     const int worker = (blockIdx.x / nxcd) * W + wv;
     const int c_begin = (int)((long)d.ncells * xcd / nxcd);
     const int c_end = (int)((long)d.ncells * (xcd + 1) / nxcd);
-    const double rc2 = P.rc2;
+    for (int c = c_begin + worker; c < c_end; c += workers) pair_cell<KIND, EVAL, MIXED>(d, d.g0, P, w, c, L);
+}
 
-    for (int c = c_begin + worker; c < c_end; c += workers) {
-        const int nh = d.natoms[c];
-        if (nh == 0) continue;
-
-        // ---- stencil entries, counts and corner shifts
-        const int nent = d.ncells == 1 ? 1 : 27;
-        int ne = 0;
-        if (L < nent) {
-            const int b = stencil_cell(d, c, L);
-            w.cell[L] = b;
-            if (b >= 0) {
-                ne = d.natoms[b];
-                double sx, sy, sz;
-                corner_shift(d, c, b, sx, sy, sz);
-                w.drb[3 * L + 0] = sx;
-                w.drb[3 * L + 1] = sy;
-                w.drb[3 * L + 2] = sz;
-            }
+// Many replicas in one launch: the cells of all replicas are dealt to the
+// waves with a global stride (replicas of different cost are spread evenly
+// over the waves); each cell uses its replica's grid and potential.  PASS 1
+// runs the EAM density pass or Lennard-Jones, PASS 3 the EAM embedding-force
+// pass (Lennard-Jones cells have none).
+template <int PASS, int MIXED>
+__global__ __launch_bounds__(64) void k_pair_batch(DevState d, const PotDev *pots)
+{
+    extern __shared__ __align__(16) char lds[];
+    const int L = threadIdx.x;
+    WaveLds w = carve(lds, PASS == 1 ? CM_KIND_EAM1 : CM_KIND_EAM3);
+    for (int c = blockIdx.x; c < d.ncells; c += gridDim.x) {
+        const Grid g = d.grids[d.cell_grid[c]];
+        const PotDev P = pots[g.pot];
+        if (PASS == 1) {
+            if (P.kind == COMD_POT_LJ) pair_cell<CM_KIND_LJ, COMD_EVAL_TABLE, MIXED>(d, g, P, w, c, L);
+            else if (P.eval == COMD_EVAL_CHEB) pair_cell<CM_KIND_EAM1, COMD_EVAL_CHEB, MIXED>(d, g, P, w, c, L);
+            else pair_cell<CM_KIND_EAM1, COMD_EVAL_TABLE, MIXED>(d, g, P, w, c, L);
+        } else if (P.kind != COMD_POT_LJ) {
+            if (P.eval == COMD_EVAL_CHEB) pair_cell<CM_KIND_EAM3, COMD_EVAL_CHEB, MIXED>(d, g, P, w, c, L);
+            else pair_cell<CM_KIND_EAM3, COMD_EVAL_TABLE, MIXED>(d, g, P, w, c, L);
         }
-        int incl = ne;
-        for (int o = 1; o < 32; o <<= 1) {
-            const int t = __shfl_up(incl, o, 64);
-            if (L >= o) incl += t;
-        }
-        const int H = __shfl(incl, 31, 64);
-        const int nonempty = __popcll(__ballot(ne > 0));
-        if (L < nent) w.off[L] = incl - ne;
-        if (L == 0) w.off[nent] = H;
-
-        // ---- home atoms: lane (a, s)
-        const int S = 64 / nh;
-        const bool act = L < S * nh;
-        const int a = act ? L % nh : 0;
-        const int sg = act ? L / nh : 0;
-        const int hs = c * CM_SLOTS + a;
-        const double xa = d.rx[hs], ya = d.ry[hs], za = d.rz[hs];
-        const double fia = KIND == CM_KIND_EAM3 ? d.fi[hs] : 0.0;
-        Acc A = {0, 0, 0, 0, 0, 0};
-        int npair = 0;
-        wave_sync();
-
-        for (int h0 = 0; h0 < H; h0 += HMAX) {
-            const int hb = H - h0 < HMAX ? H - h0 : HMAX;
-            // stage the batch: lane L takes halo atoms L, L+64, ...; its
-            // stencil entry only moves forward
-            {
-                int e = 0;
-                for (int h = h0 + L; h < h0 + hb; h += 64) {
-                    while (w.off[e + 1] <= h) e++;
-                    const int sl = w.cell[e] * CM_SLOTS + (h - w.off[e]);
-                    const double x = d.rx[sl], y = d.ry[sl], z = d.rz[sl];
-                    w.hx[h - h0] = x - w.drb[3 * e + 0];
-                    w.hy[h - h0] = y - w.drb[3 * e + 1];
-                    w.hz[h - h0] = z - w.drb[3 * e + 2];
-                    if (KIND == CM_KIND_EAM3) w.hf[h - h0] = d.fi[sl];
-                }
-            }
-            wave_sync();
-
-            int cnt = 0;
-            // drain all queues: atom a's entries (the concatenation of the
-            // queues of lanes a, a+nh, ...) are taken round-robin by its lanes
-            auto drain = [&]() {
-                if (!P.balanced) {
-                    // each lane drains its own queue
-                    for (int k = 0; __ballot(k < cnt) != 0; k++) {
-                        if (k < cnt) {
-                            const int j = w.q[k * 64 + L];
-                            const double dx = xa - w.hx[j], dy = ya - w.hy[j], dz = za - w.hz[j];
-                            const double r2 = dx * dx + dy * dy + dz * dz;
-                            pair_term<KIND, EVAL, MIXED>(P, dx, dy, dz, r2, fia,
-                                                         KIND == CM_KIND_EAM3 ? w.hf[j] : 0.0, A);
-                            npair++;
-                        }
-                    }
-                    cnt = 0;
-                    return;
-                }
-                w.qn[L] = act ? cnt : 0;
-                wave_sync();
-                int tot = 0;
-                if (act)
-                    for (int s = 0; s < S; s++) tot += w.qn[a + s * nh];
-                int v = sg, sp = 0, base = 0;
-                while (__ballot(act && v < tot) != 0) {
-                    if (act && v < tot) {
-                        int cs = w.qn[a + sp * nh];
-                        while (v >= base + cs) {
-                            base += cs;
-                            sp++;
-                            cs = w.qn[a + sp * nh];
-                        }
-                        const int j = w.q[(v - base) * 64 + a + sp * nh];
-                        const double dx = xa - w.hx[j], dy = ya - w.hy[j], dz = za - w.hz[j];
-                        const double r2 = dx * dx + dy * dy + dz * dz;
-                        pair_term<KIND, EVAL, MIXED>(P, dx, dy, dz, r2, fia, KIND == CM_KIND_EAM3 ? w.hf[j] : 0.0, A);
-                        npair++;
-                        v += S;
-                    }
-                }
-                cnt = 0;
-                wave_sync();
-            };
-            const int trips = (hb + S - 1) / S;
-            for (int it0 = 0; it0 < trips; it0 += UNROLL) {
-                bool acc[UNROLL];
-                int jj[UNROLL];
-#pragma unroll
-                for (int u = 0; u < UNROLL; u++) {
-                    const int j = sg + (it0 + u) * S;
-                    const bool valid = act && j < hb;
-                    const int jc = valid ? j : 0;
-                    const double dx = xa - w.hx[jc], dy = ya - w.hy[jc], dz = za - w.hz[jc];
-                    const double r2 = dx * dx + dy * dy + dz * dz;
-                    const bool in = KIND == CM_KIND_EAM3 ? (r2 < rc2) : (r2 <= rc2);
-                    acc[u] = valid && in && r2 > 0.0;
-                    jj[u] = j;
-                }
-#pragma unroll
-                for (int u = 0; u < UNROLL; u++)
-                    if (acc[u]) {
-                        w.q[cnt * 64 + L] = (uint16_t)jj[u];
-                        cnt++;
-                    }
-                if (__ballot(cnt > QCAP - UNROLL) != 0) drain();
-            }
-            drain();
-        }
-
-        // ---- combine the S partial sums of each home atom, in s order
-        // (scratch in the halo area, free once the last batch is drained)
-        double *red = w.hx;
-        int *redn = (int *)(red + 6 * 64);
-        red[0 * 64 + L] = act ? A.x : 0.0;
-        red[1 * 64 + L] = act ? A.y : 0.0;
-        red[2 * 64 + L] = act ? A.z : 0.0;
-        red[3 * 64 + L] = act ? A.e : 0.0;
-        red[4 * 64 + L] = act ? A.rho : 0.0;
-        red[5 * 64 + L] = act ? A.vir : 0.0;
-        redn[L] = act ? npair : 0;
-        wave_sync();
-        if (L < nh) {
-            double sx = 0, sy = 0, sz = 0, se = 0, sr = 0, sv = 0;
-            int sn = 0;
-            for (int s = 0; s < S; s++) {
-                const int l = L + s * nh;
-                sx += red[0 * 64 + l];
-                sy += red[1 * 64 + l];
-                sz += red[2 * 64 + l];
-                se += red[3 * 64 + l];
-                sr += red[4 * 64 + l];
-                sv += red[5 * 64 + l];
-                sn += redn[l];
-            }
-            const int o = c * CM_SLOTS + L;
-            if (KIND == CM_KIND_EAM1) {
-                // pass 2, src-flat/eam.c:576-594
-                double fv, fp;
-                if (EVAL == COMD_EVAL_CHEB) {
-                    fv = cheb64(P.cheb + 2 * P.cn, P.cn, P.ca[2], P.cb[2], sr);
-                    fp = cheb64(P.cheb + 5 * P.cn, P.cn, P.ca[2], P.cb[2], sr);
-                } else {
-                    interp64(P.tab[2], P.x0[2], P.xn[2], P.invdx[2], P.n[2], sr, fv, fp);
-                }
-                d.fx[o] = sx;
-                d.fy[o] = sy;
-                d.fz[o] = sz;
-                d.fe[o] = se + fv;
-                d.rho[o] = sr;
-                d.fi[o] = fp;
-                d.nnb[o] = sn;
-            } else if (KIND == CM_KIND_EAM3) {
-                d.fx[o] += sx;
-                d.fy[o] += sy;
-                d.fz[o] += sz;
-                d.vir[o] = sv;
-            } else {
-                // kinetic term of src-flat/ljForce.c:175, once per non-empty
-                // stencil cell, with the mass in amu
-                const double px = d.px[o];
-                d.fx[o] = sx;
-                d.fy[o] = sy;
-                d.fz[o] = sz;
-                d.fe[o] = se;
-                d.vir[o] = sv - (double)nonempty * (px * px / P.mass_amu);
-                d.nnb[o] = sn;
-            }
-        }
-        wave_sync();
     }
+}
+
+// Persistent grids never exceed the blocks that can be resident at once (or
+// the last ones would start only when others finish).  The occupancy query
+// is made once per kernel and block shape, so that launches made while a
+// step is captured into a graph issue nothing else.
+template <typename K>
+static int resident_blocks(K kern, int threads, size_t lds, int ncu, int *cache)
+{
+    if (*cache == 0) {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, lds) != hipSuccess || per_cu < 1)
+            per_cu = -1;
+        *cache = per_cu;
+    }
+    return *cache > 0 && ncu > 0 ? *cache * ncu : 1 << 30;
 }
 
 template <int KIND, int EVAL, int MIXED>
 static void launch3(const DevState &d, const PotDev &P, const ForceCfg &cfg, hipStream_t st)
 {
+    static int occ[5] = {0, 0, 0, 0, 0};
     const int W = cfg.waves;
     const size_t lds = (size_t)W * wave_lds_bytes(KIND);
-    // the grid is persistent: never launch more blocks than can be resident
-    // at once, or the last ones would start only when others finish
     auto kern = k_pair<KIND, EVAL, MIXED>;
-    int per_cu = 0;
-    int blocks = cfg.blocks;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * W, lds) == hipSuccess && per_cu > 0 &&
-        cfg.ncu > 0 && blocks > per_cu * cfg.ncu)
-        blocks = per_cu * cfg.ncu;
+    const int cap = resident_blocks(kern, 64 * W, lds, cfg.ncu, &occ[W < 5 ? W : 4]);
+    const int blocks = cfg.blocks < cap ? cfg.blocks : cap;
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * W), lds, st, d, P);
 }
 
@@ -486,6 +530,29 @@ int cm_force_lds_bytes(const PotDev &P, int kind, int mixed, int waves, int tab_
     (void)mixed;
     (void)tab_lds;
     return waves * wave_lds_bytes(kind);
+}
+
+template <int PASS, int MIXED>
+static void launch_batch(const DevState &d, const PotDev *pots, const ForceCfg &cfg, hipStream_t st)
+{
+    static int occ = 0;
+    const size_t lds = wave_lds_bytes(PASS == 1 ? CM_KIND_EAM1 : CM_KIND_EAM3);
+    auto kern = k_pair_batch<PASS, MIXED>;
+    const int cap = resident_blocks(kern, 64, lds, cfg.ncu, &occ);
+    const int blocks = cfg.blocks < cap ? cfg.blocks : cap;
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(64), lds, st, d, pots);
+}
+
+void cm_launch_force_batch(const DevState &d, const PotDev *pots, int pass, int mixed, const ForceCfg &cfg,
+                           hipStream_t st)
+{
+    if (pass == 1) {
+        if (mixed) launch_batch<1, 1>(d, pots, cfg, st);
+        else launch_batch<1, 0>(d, pots, cfg, st);
+    } else {
+        if (mixed) launch_batch<3, 1>(d, pots, cfg, st);
+        else launch_batch<3, 0>(d, pots, cfg, st);
+    }
 }
 
 void cm_launch_force(const DevState &d, const PotDev &P, int kind, int mixed, const ForceCfg &cfg, hipStream_t st)

@@ -3,12 +3,13 @@
 // block sums the partials in a fixed order.
 #include "comd_internal.h"
 
-__global__ __launch_bounds__(256) void k_reduce(DevState d, double rmass, double *part)
+__global__ __launch_bounds__(256) void k_reduce(DevState d, double rmass, int cell0, int ncells, double *part)
 {
     __shared__ double sh[3][256];
     double pe = 0, ke = 0, vi = 0;
-    const long ns = (long)d.ncells * CM_SLOTS;
-    for (long s = blockIdx.x * 256L + threadIdx.x; s < ns; s += (long)gridDim.x * 256L) {
+    const long s0 = (long)cell0 * CM_SLOTS;
+    const long ns = (long)ncells * CM_SLOTS;
+    for (long s = s0 + blockIdx.x * 256L + threadIdx.x; s < s0 + ns; s += (long)gridDim.x * 256L) {
         const int c = (int)(s / CM_SLOTS);
         const int i = (int)(s % CM_SLOTS);
         if (i < d.natoms[c]) {
@@ -48,10 +49,9 @@ __global__ __launch_bounds__(256) void k_reduce_final(const double *part, int np
         for (int k = 0; k < 3; k++) out[k] = sh[k][0];
 }
 
-void cm_launch_reduce(const DevState &d, const PotDev &P, double *partials, int nparts, hipStream_t st)
+void cm_launch_reduce(const DevState &d, const Grid &g, double *partials, int nparts, hipStream_t st)
 {
-    const double rmass = 1822.83 * P.mass_amu;
-    hipLaunchKernelGGL(k_reduce, dim3(nparts), dim3(256), 0, st, d, rmass, partials);
+    hipLaunchKernelGGL(k_reduce, dim3(nparts), dim3(256), 0, st, d, g.rmass, g.cell_base, g.ncells, partials);
 }
 
 void cm_launch_reduce_final(const double *partials, int nparts, double *out, hipStream_t st)

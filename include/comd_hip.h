@@ -152,8 +152,18 @@ enum comd_tune_key {
     COMD_TUNE_BLOCKS_PER_CU = 2,   /* 1..64, default 8 */
     COMD_TUNE_TABLES_IN_LDS = 3,   /* 0/1, default 1 */
     COMD_TUNE_BALANCED_DRAIN = 4,  /* 0/1: share an atom's accepted pairs among its lanes */
+    COMD_TUNE_GRAPH = 5,           /* 0/1: comd_steps/comd_nsteps replay one captured step
+                                      (default 1 for batches, 0 for single simulations) */
 };
 int comd_tune(comd_ctx *ctx, int key, int value);
+
+/* Pick the force kernels' launch shape by timing: cells (waves) per
+ * workgroup, persistent workgroups per CU, and how an atom's accepted pairs
+ * are split over its lanes (balanced or per lane).  Force calls only; the
+ * state does not change.  The choice is applied and returned; the drain
+ * split changes the summation order, so forces may differ in the last bits
+ * between the two splits (both within the parity bar). */
+int comd_autotune(comd_ctx *ctx, int *waves, int *blocks_per_cu, int *balanced, double *best_ms);
 
 /* Number of atom pairs inside the cutoff found by the last EAM density or
  * LJ pass (each unordered pair once: the reference's half-pair count). */
@@ -187,6 +197,31 @@ enum comd_kclass {
 };
 int comd_timing(comd_ctx *ctx, int on);
 int comd_timing_read(comd_ctx *ctx, double *ms /* COMD_K_NCLASS */, int *launches /* COMD_K_NCLASS */);
+
+/* ---- Batches: many independent replicas in one context ----
+ *
+ * No counterpart in the reference (a src-flat process holds one simulation;
+ * its globals make it non-reentrant).  Each replica has its own parameters
+ * (potential, grid from its own bounds, periodicity) and its own slot layout
+ * with the reference's semantics; their cells are numbered one after the
+ * other, so every kernel of a step (drift, rebin, force passes) covers all
+ * replicas in one launch, and comd_steps replays one captured step graph.
+ * All replicas share the precision.  The single-simulation calls above act
+ * on replica 0 (comd_upload, comd_download, comd_bin_atoms*) or on all
+ * replicas (comd_force, comd_steps, comd_nsteps, comd_rebox; comd_energy
+ * returns the sums over replicas). */
+int comd_batch_create(int n, const comd_params *prms /* n */, comd_ctx **out);
+/* Cells of replica r start at global cell *cell_base; *natoms atoms. */
+int comd_batch_info(comd_ctx *ctx, int r, int *cell_base, int *ncells, int *natoms);
+int comd_batch_bin_atoms(comd_ctx *ctx, int r, int n, const double *xyz, const double *pxyz, int itype);
+int comd_batch_upload(comd_ctx *ctx, int r, const int *natoms, const int *id, const int *itype, const double *rr,
+                      const double *p);
+/* As comd_download, for replica r's own cells and slots. */
+int comd_batch_download(comd_ctx *ctx, int r, int *natoms, int *id, int *itype, double *rr, double *p, double *f,
+                        double *rho, double *fi);
+int comd_batch_energy(comd_ctx *ctx, int r, double *pe, double *ke, double *stress);
+/* n steps of every replica (comd_steps). */
+int comd_batch_step(comd_ctx *ctx, int n, double dt_seconds);
 
 /* ---- Chebyshev coefficients and their adjoint ---- */
 

@@ -1,0 +1,166 @@
+"""GPU tests of batches: independent replicas of uneven size, x-stretch and
+potential (EAM table, EAM Chebyshev, Lennard-Jones) in one context, where
+every kernel of a step covers all replicas in one launch and steps replay a
+captured HIP graph.
+
+Each replica must evolve exactly as the same system run alone (itself
+parity-tested against the oracle): same cells and slot order, forces and
+trajectories equal to 1e-12.  Graph replays are checked chunk by chunk with
+eager work (energies, downloads) between them.
+"""
+import numpy as np
+import pytest
+
+import comd_amd as CA
+import oracle_py as O
+from helpers import POTS, compact, force_rel_error, seeded_atoms
+
+pytestmark = pytest.mark.gpu
+
+LJ = CA.copper_lj()
+
+
+def replica_specs(seed=0, count=6):
+    rng = np.random.default_rng(seed)
+    specs = []
+    for k in range(count):
+        kind = ("eam", "lj", "eam_cheby")[k % 3]
+        n = int(rng.integers(5, 9))
+        stretch = float(rng.uniform(1.0, 1.1))
+        specs.append(dict(kind=kind, n=n, stretch=stretch, seed=100 + k))
+    return specs
+
+
+def build(spec, cu500):
+    pot = LJ if spec["kind"] == "lj" else cu500
+    lat = LJ.lat if spec["kind"] == "lj" else 3.615
+    xyz, p, bounds = seeded_atoms(spec["n"], lat, pot.mass, temp=300.0, disp=0.05, seed=spec["seed"],
+                                  defgrad=spec["stretch"])
+    ev = "cheby" if spec["kind"] == "eam_cheby" else "table"
+    return pot, xyz, p, bounds, ev
+
+
+def make_batch(specs, cu500, precision="fp64"):
+    reps, atoms = [], []
+    for sp in specs:
+        pot, xyz, p, bounds, ev = build(sp, cu500)
+        reps.append(dict(pot=pot, bounds=bounds, defgrad=0.0, evaluation=ev))
+        atoms.append((xyz, p))
+    b = CA.Batch(reps, precision=precision)
+    for r, (xyz, p) in enumerate(atoms):
+        b.bin_atoms(r, xyz, p)
+    return b
+
+
+def make_singles(specs, cu500, precision="fp64"):
+    out = []
+    for sp in specs:
+        pot, xyz, p, bounds, ev = build(sp, cu500)
+        s = CA.Simulation(pot, bounds, defgrad=0.0, precision=precision, evaluation=ev)
+        s.bin_atoms(xyz, p)
+        out.append(s)
+    return out
+
+
+def compare(b, singles, what, tol=1e-12):
+    for r, s in enumerate(singles):
+        g, o = b.download(r), s.download()
+        assert np.array_equal(g["natoms"], o["natoms"]), f"{what} replica {r}: cell counts"
+        gc, oc = compact(g), compact(o)
+        assert np.array_equal(gc["id"], oc["id"]), f"{what} replica {r}: slot order"
+        ef = force_rel_error(gc["f"], oc["f"]).max()
+        dr = np.abs(gc["r"] - oc["r"]).max()
+        dp = np.abs(gc["p"] - oc["p"]).max() / max(np.abs(oc["p"]).max(), 1e-300)
+        assert ef <= tol and dr <= 1e-12 and dp <= tol, f"{what} replica {r}: f {ef:.2e} r {dr:.2e} p {dp:.2e}"
+        pe_b = b.energy_of(r)[0]
+        pe_s = s.energy()[0]
+        assert abs(pe_b - pe_s) <= 1e-12 * abs(pe_s), f"{what} replica {r}: energy"
+
+
+def test_batch_force_equals_single_runs(gpu, cu500):
+    specs = replica_specs(1)
+    b = make_batch(specs, cu500)
+    singles = make_singles(specs, cu500)
+    for r, s in enumerate(singles):
+        assert b.nbx_list[r] == s.nbx
+    b.rebox()
+    b.force()
+    for s in singles:
+        s.rebox()
+        s.force()
+    print(f"[batch] {len(specs)} replicas, {b.ncells} cells, grids {b.nbx_list}", flush=True)
+    compare(b, singles, "force")
+
+
+def test_batch_graph_steps_equal_single_runs(gpu, cu500):
+    specs = replica_specs(2)
+    b = make_batch(specs, cu500)
+    singles = make_singles(specs, cu500)
+    b.rebox()
+    b.force()
+    for s in singles:
+        s.rebox()
+        s.force()
+    # graph replays in chunks, with eager calls in between
+    for chunk in (3, 5, 7):
+        b.steps(chunk)
+        b.force()
+        b.sync()
+        for s in singles:
+            s.steps(chunk)
+            s.force()
+        compare(b, singles, f"after {chunk} graph steps")
+        b.energy()
+
+
+def test_batch_replica_against_oracle(gpu, cu500, ocu500):
+    """One EAM and one LJ replica of a batch, 10 steps, against the oracle."""
+    specs = [dict(kind="eam", n=6, stretch=1.04, seed=7), dict(kind="lj", n=5, stretch=1.08, seed=8)]
+    b = make_batch(specs, cu500)
+    b.rebox()
+    b.force()
+    pe = b.nsteps(10)
+    olj = O.OraclePot.lj(LJ.sigma, LJ.epsilon, LJ.cutoff, LJ.mass)
+    total = 0.0
+    for r, sp in enumerate(specs):
+        pot, xyz, p, bounds, _ = build(sp, cu500)
+        osim = O.OracleSim.from_atoms(ocu500 if sp["kind"] == "eam" else olj, bounds, xyz, p)
+        osim.rebox()
+        osim.force()
+        osim.nsteps(10)
+        g, o = compact(b.download(r)), O.compact(osim.state())
+        assert np.array_equal(g["id"], o["id"])
+        ef = force_rel_error(g["f"], o["f"]).max()
+        print(f"[batch vs oracle] replica {r} ({sp['kind']}): force err {ef:.2e}", flush=True)
+        assert ef <= 1e-10
+        total += osim.info()["e"]
+    assert abs(pe - total) <= 1e-10 * abs(total)
+
+
+def test_batch_autotune_and_mixed(gpu, cu500):
+    specs = replica_specs(3, count=9)
+    b = make_batch(specs, cu500, precision="mixed")
+    b.rebox()
+    b.force()
+    f0 = [b.download(r)["f"].copy() for r in range(b.nrep)]
+    w, bpc, bal, ms = b.autotune()
+    print(f"[batch autotune] waves {w} blocks/CU {bpc} balanced {bal}: {ms:.3f} ms per force", flush=True)
+    b.force()
+    for r in range(b.nrep):
+        f = b.download(r)["f"]
+        m = np.abs(f0[r][:, :3]).max()
+        assert np.abs(f[:, :3] - f0[r][:, :3]).max() <= 1e-12 * m
+    b64 = make_batch(specs, cu500, precision="fp64")
+    b64.rebox()
+    b64.force()
+    for r in range(b.nrep):
+        g, o = compact(b.download(r)), compact(b64.download(r))
+        ef = force_rel_error(g["f"], o["f"]).max()
+        assert ef <= 1e-4, f"mixed replica {r}: {ef:.2e}"
+    b.steps(20)
+    b64.steps(20)
+    b.sync()
+    b64.sync()
+    for r in range(b.nrep):
+        e32, e64 = b.energy_of(r)[0], b64.energy_of(r)[0]
+        assert abs(e32 - e64) <= 1e-5 * abs(e64)
