@@ -35,7 +35,7 @@ POT_EAM, POT_LJ = 1, 2
 EVAL_TABLE, EVAL_CHEB = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 K_REBIN, K_DENSITY, K_EMBED_FORCE, K_REDUCE, K_NCLASS = 0, 1, 2, 3, 4
-TUNE_WAVES, TUNE_BLOCKS_PER_CU, TUNE_TABLES_LDS, TUNE_BALANCED, TUNE_GRAPH = 1, 2, 3, 4, 5
+TUNE_WAVES, TUNE_BLOCKS_PER_CU, TUNE_TABLES_LDS, TUNE_BALANCED, TUNE_GRAPH, TUNE_PAIR_LISTS, TUNE_SKIN = 1, 2, 3, 4, 5, 6, 7
 
 # unit constants of src-flat/include/constants.h:61-65
 BOHR_PER_ATU_TO_A_PER_S = 0.529 / 2.418e-17
@@ -134,6 +134,7 @@ def _load():
     hip.comd_batch_energy.argtypes = [vp, C.c_int, _dp, _dp, _dp]
     hip.comd_batch_step.argtypes = [vp, C.c_int, C.c_double]
     hip.comd_autotune.argtypes = [vp, _ip, _ip, _ip, _dp]
+    hip.comd_list_info.argtypes = [vp, _ip, _ip, _ip, _ip]
     hip.comd_set_cheb.argtypes = [vp, _dp, C.c_int]
     hip.comd_get_cheb.argtypes = [vp, _dp]
     hip.comd_eam_cheb_grad.argtypes = [vp, _dp, C.c_int, _dp, C.c_double, C.c_double, _dp, _dp]
@@ -421,13 +422,22 @@ class Simulation:
         self._check(self._hip.comd_energy(self.ctx, C.byref(pe), C.byref(ke), C.byref(st)))
         return pe.value, ke.value, st.value
 
-    def tune(self, waves_per_block=None, blocks_per_cu=None, tables_in_lds=None, balanced=None, graph=None):
-        """Launch shape of the force kernels and step replay as a HIP graph
-        (comd_tune)."""
+    def tune(self, waves_per_block=None, blocks_per_cu=None, tables_in_lds=None, balanced=None, graph=None,
+             pair_lists=None, skin=None):
+        """Launch shape of the force kernels, step replay as a HIP graph,
+        pair-list replay in the EAM embedding-force pass and the Verlet skin
+        in A (0 turns the neighbour lists off) (comd_tune)."""
         for key, val in ((TUNE_WAVES, waves_per_block), (TUNE_BLOCKS_PER_CU, blocks_per_cu),
-                         (TUNE_TABLES_LDS, tables_in_lds), (TUNE_BALANCED, balanced), (TUNE_GRAPH, graph)):
+                         (TUNE_TABLES_LDS, tables_in_lds), (TUNE_BALANCED, balanced), (TUNE_GRAPH, graph),
+                         (TUNE_PAIR_LISTS, pair_lists), (TUNE_SKIN, None if skin is None else round(1000 * skin))):
             if val is not None:
                 self._check(self._hip.comd_tune(self.ctx, key, int(val)))
+
+    def list_info(self):
+        """Neighbour lists: dict(active, valid, builds, kmax) (comd_list_info)."""
+        a, v, b, k = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        self._check(self._hip.comd_list_info(self.ctx, C.byref(a), C.byref(v), C.byref(b), C.byref(k)))
+        return dict(active=a.value, valid=v.value, builds=b.value, kmax=k.value)
 
     def autotune(self):
         """Time the force passes over launch shapes and keep the fastest
@@ -480,11 +490,14 @@ class Simulation:
         """Force-matching loss sum_i |f_i - t_i|^2 / (3N) + w_e (E - E_t)^2 at
         the value coefficients `coef` (3 x n) and its gradient (3 x n), from
         the adjoint kernels (comd_eam_cheb_grad).  f_target: slot layout
-        (S, 3) or (S, 4) as download()['f']."""
+        (S, 3) or (S, 4) as download()['f'], or None to reuse the target of
+        the previous call (kept in device memory)."""
         c = np.ascontiguousarray(coef, dtype=np.float64).reshape(3, -1)
-        ft = np.ascontiguousarray(np.asarray(f_target, dtype=np.float64)[:, :3])
-        if ft.shape[0] != self.nslots:
-            raise ValueError(f"f_target has {ft.shape[0]} rows, the context {self.nslots} slots")
+        ft = None
+        if f_target is not None:
+            ft = np.ascontiguousarray(np.asarray(f_target, dtype=np.float64)[:, :3])
+            if ft.shape[0] != self.nslots:
+                raise ValueError(f"f_target has {ft.shape[0]} rows, the context {self.nslots} slots")
         loss = C.c_double()
         grad = np.zeros_like(c)
         self._check(self._hip.comd_eam_cheb_grad(self.ctx, _ptr(c), c.shape[1], _ptr(ft), float(e_target),

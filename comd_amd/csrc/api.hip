@@ -62,6 +62,11 @@ struct comd_ctx {
     double *mv_rx, *mv_ry, *mv_rz, *mv_px, *mv_py, *mv_pz;
     uint8_t *perm;
     unsigned *flags;
+    // pair lists of the EAM density pass (EAM contexts)
+    uint16_t *plist;
+    uint8_t *pcount;
+    int *pvalid;
+    int use_plist;
     double *partials;
     int nparts;
     double *red_out;       // 3 doubles, device
@@ -74,10 +79,17 @@ struct comd_ctx {
     // Chebyshev-coefficient adjoint (allocated on first use)
     double *adj_g[3], *adj_lam, *adj_mu, *adj_ft, *adj_f2, *adj_part, *adj_out;
     int adj_nblocks;
+    int adj_have_target;  // adj_ft holds a target (comd_eam_cheb_grad with f_target NULL reuses it)
     std::vector<double> cheb_host;  // current value series phi, rho, F (3 x cn) of replica 0
     // force-kernel launch shape (comd_tune / comd_autotune)
     int ncu, max_lds;
     int force_waves, force_blocks_per_cu, tab_lds;
+    // Verlet neighbour lists (single periodic simulations; comd_tune)
+    CmList nl;
+    int nl_skin_milli;  // skin in 1e-3 A, 0 = link-cell kernels only
+    int nl_active;      // lists allocated and usable for the atoms held
+    long nl_ids;        // id capacity of the atoms held
+    long nl_alloc_n, nl_alloc_k, nl_alloc_aux;
     // one DKD step captured as a HIP graph (comd_steps)
     int use_graph;
     hipGraphExec_t graph;
@@ -139,6 +151,18 @@ static DevState dev_state(const comd_ctx *c)
     d.perm = c->perm;
     d.meta = c->meta;
     d.flags = c->flags;
+    const bool lists = c->use_plist && c->plist;
+    d.plist = lists ? c->plist : nullptr;
+    d.pcount = lists ? c->pcount : nullptr;
+    d.pvalid = lists ? c->pvalid : nullptr;
+    const bool nl = c->nl_active != 0;
+    d.nl_pos = nl ? c->nl.pos : nullptr;
+    d.nl_ref = nl ? c->nl.ref : nullptr;
+    d.nl_slot = nl ? c->nl.slot : nullptr;
+    d.nl_flags = nl ? c->nl.flags : nullptr;
+    const double hs = 0.5e-3 * c->nl_skin_milli;
+    d.nl_half2 = hs * hs;
+    d.cell_gate = nl ? c->nl.flags : nullptr;
     return d;
 }
 
@@ -235,15 +259,27 @@ static void op_force(comd_ctx *c)
         return;
     }
     const PotDev &P = c->reps[0].pot;
+    // with neighbour lists: the (usually skipped) list build, then each pass
+    // over the lists; the link-cell kernels run only if the lists are invalid
+    const bool nl = c->nl_active != 0;
     if (P.kind == COMD_POT_LJ) {
         tbegin(c, COMD_K_DENSITY, &ev);
+        if (nl) {
+            cm_nl_build(d, c->nl, c->stream);
+            cm_nl_force(d, P, c->nl, CM_KIND_LJ, mixed, c->stream);
+        }
         cm_launch_force(d, P, CM_KIND_LJ, mixed, force_cfg(c, CM_KIND_LJ, mixed), c->stream);
         tend(c, COMD_K_DENSITY, ev);
     } else {
         tbegin(c, COMD_K_DENSITY, &ev);
+        if (nl) {
+            cm_nl_build(d, c->nl, c->stream);
+            cm_nl_force(d, P, c->nl, CM_KIND_EAM1, mixed, c->stream);
+        }
         cm_launch_force(d, P, CM_KIND_EAM1, mixed, force_cfg(c, CM_KIND_EAM1, mixed), c->stream);
         tend(c, COMD_K_DENSITY, ev);
         tbegin(c, COMD_K_EMBED_FORCE, &ev);
+        if (nl) cm_nl_force(d, P, c->nl, CM_KIND_EAM3, mixed, c->stream);
         cm_launch_force(d, P, CM_KIND_EAM3, mixed, force_cfg(c, CM_KIND_EAM3, mixed), c->stream);
         tend(c, COMD_K_EMBED_FORCE, ev);
     }
@@ -324,14 +360,27 @@ int comd_grid_for(comd_params *p, const double bounds[3], double defgrad, double
 
 }
 
+static void free_nl(comd_ctx *c)
+{
+    void *p[] = {c->nl.pos, c->nl.ref, c->nl.slot, c->nl.order, c->nl.nbl, c->nl.ncnt, c->nl.fi,
+                 c->nl.aux_cnt, c->nl.aux_start, c->nl.aux_mem, c->nl.nonempty, c->nl.count, c->nl.flags};
+    for (void *q : p)
+        if (q) hipFree(q);
+    memset(&c->nl, 0, sizeof c->nl);
+    c->nl_alloc_n = c->nl_alloc_k = c->nl_alloc_aux = 0;
+    c->nl_active = 0;
+}
+
 static void free_all(comd_ctx *c)
 {
     if (c->graph) hipGraphExecDestroy(c->graph);
+    free_nl(c);
     void *ptrs[] = {c->natoms, c->id, c->itype, c->rx, c->ry, c->rz, c->px, c->py, c->pz,
                     c->fx, c->fy, c->fz, c->fe, c->rho, c->fi, c->vir, c->nnb, c->dest, c->mvidx, c->meta,
                     c->counts, c->touched, c->tc, c->arr_cnt, c->arr, c->mv_slot, c->mv_rank, c->mv_id,
                     c->mv_itype, c->mv_rx, c->mv_ry, c->mv_rz, c->mv_px, c->mv_py, c->mv_pz,
-                    c->perm, c->flags, c->partials, c->red_out, c->d_grids, c->d_cell_grid, c->d_pots,
+                    c->perm, c->flags, c->plist, c->pcount, c->pvalid,
+                    c->partials, c->red_out, c->d_grids, c->d_cell_grid, c->d_pots,
                     c->adj_g[0], c->adj_g[1], c->adj_g[2], c->adj_lam, c->adj_mu,
                     c->adj_ft, c->adj_f2, c->adj_part, c->adj_out};
     for (void *p : ptrs)
@@ -552,6 +601,14 @@ static int ctx_create(int nrep, const comd_params *prms, comd_ctx **out)
     alloc((void **)&c->mv_pz, S * 8);
     alloc((void **)&c->perm, S);
     alloc((void **)&c->flags, 16);
+    bool eam = false;
+    for (int r = 0; r < nrep; r++) eam = eam || prms[r].potential == COMD_POT_EAM;
+    if (eam) {
+        alloc((void **)&c->plist, C * CM_PLIST * 64 * 2);
+        alloc((void **)&c->pcount, S);
+        alloc((void **)&c->pvalid, C * 4);
+    }
+    c->use_plist = 1;
     c->nparts = 1024;
     alloc((void **)&c->partials, (size_t)c->nparts * 3 * 8);
     alloc((void **)&c->red_out, 4 * 8);
@@ -613,6 +670,7 @@ static int ctx_create(int nrep, const comd_params *prms, comd_ctx **out)
     c->force_blocks_per_cu = 8;
     c->tab_lds = 1;
     c->use_graph = nrep > 1;
+    c->nl_skin_milli = nrep > 1 ? 0 : 400;
     if (hipStreamSynchronize(c->stream) != hipSuccess) {
         free_all(c);
         delete c;
@@ -625,6 +683,79 @@ static int ctx_create(int nrep, const comd_params *prms, comd_ctx **out)
 static int check_rep(comd_ctx *c, int r)
 {
     if (r < 0 || r >= c->nrep) return set_err(c, COMD_ERR_ARG, "replica %d out of range", r);
+    return COMD_OK;
+}
+
+// Neighbour lists for the atoms just binned or uploaded (ids below nids):
+// used when the context holds one periodic simulation whose box has at
+// least three link cells and three (cutoff + skin) cells per dimension;
+// otherwise the link-cell kernels do every pass.  Sets the id -> slot and
+// id -> position maps and requests a build.
+static int nl_setup(comd_ctx *c, long nids)
+{
+    c->nl_active = 0;
+    c->nl_ids = nids;
+    invalidate_graph(c);
+    if (c->nrep != 1 || c->nl_skin_milli <= 0 || nids <= 0) return COMD_OK;
+    const comd_params &p = c->reps[0].prm;
+    if (!p.periodic || p.nbx[0] < 3 || p.nbx[1] < 3 || p.nbx[2] < 3) return COMD_OK;
+    const double rl = p.cutoff + 1e-3 * c->nl_skin_milli;
+    int na[3];
+    for (int j = 0; j < 3; j++) {
+        na[j] = (int)floor(p.bounds[j] / rl);
+        if (na[j] < 3) return COMD_OK;
+    }
+    const long naux = (long)na[0] * na[1] * na[2];
+    const double vol = p.bounds[0] * p.bounds[1] * p.bounds[2];
+    const double expect = 4.0 / 3.0 * 3.141592653589793 * rl * rl * rl * (double)c->ntot / vol;
+    const long kmax = (((long)(1.5 * expect) + 16 + 15) / 16) * 16;
+    if (nids > c->nl_alloc_n || kmax > c->nl_alloc_k || naux > c->nl_alloc_aux) {
+        free_nl(c);
+        bool ok = true;
+        auto alloc = [&](void **ptr, size_t bytes) {
+            if (ok && hipMalloc(ptr, bytes < 16 ? 16 : bytes) != hipSuccess) ok = false;
+        };
+        const size_t n = (size_t)nids;
+        alloc((void **)&c->nl.pos, n * sizeof(double4));
+        alloc((void **)&c->nl.ref, n * sizeof(double4));
+        alloc((void **)&c->nl.slot, n * 4);
+        alloc((void **)&c->nl.order, n * 4);
+        alloc((void **)&c->nl.nbl, n * (size_t)kmax * 4);
+        alloc((void **)&c->nl.ncnt, n * 4);
+        alloc((void **)&c->nl.fi, n * 8);
+        alloc((void **)&c->nl.aux_cnt, (size_t)naux * 4);
+        alloc((void **)&c->nl.aux_start, (size_t)naux * 4);
+        alloc((void **)&c->nl.aux_mem, (size_t)naux * CM_NL_AUXCAP * 4);
+        alloc((void **)&c->nl.nonempty, (size_t)c->ncells * 4);
+        alloc((void **)&c->nl.count, 16);
+        alloc((void **)&c->nl.flags, 16);
+        if (!ok) {
+            free_nl(c);
+            return set_err(c, COMD_ERR_NOMEM, "neighbour-list buffers");
+        }
+        c->nl_alloc_n = nids;
+        c->nl_alloc_k = kmax;
+        c->nl_alloc_aux = naux;
+    }
+    CmList &l = c->nl;
+    l.n = (int)nids;
+    l.kmax = (int)kmax;
+    l.rl2 = rl * rl;
+    l.L0 = p.bounds[0];
+    l.L1 = p.bounds[1];
+    l.L2 = p.bounds[2];
+    l.na0 = na[0];
+    l.na1 = na[1];
+    l.na2 = na[2];
+    l.naux = (int)naux;
+    l.ab0 = p.bounds[0] / na[0];
+    l.ab1 = p.bounds[1] / na[1];
+    l.ab2 = p.bounds[2] / na[2];
+    HIPCHK(c, hipMemsetAsync(l.flags, 0, 16, c->stream));
+    HIPCHK(c, hipMemsetAsync(l.count, 0, 16, c->stream));
+    c->nl_active = 1;
+    cm_nl_init(dev_state(c), l, c->stream);
+    HIPCHK(c, hipGetLastError());
     return COMD_OK;
 }
 
@@ -660,7 +791,10 @@ static int upload_rep(comd_ctx *c, int r, const int *natoms, const int *id, cons
     c->ntot += tot - c->reps[r].ntot;
     c->reps[r].ntot = tot;
     c->have_atoms = 1;
-    return COMD_OK;
+    long nids = 0;
+    for (size_t b = 0; b < C; b++)
+        for (int i = 0; i < natoms[b]; i++) nids = std::max<long>(nids, (long)id[b * CM_SLOTS + i] + 1);
+    return nl_setup(c, nids);
 }
 
 static int download_rep(comd_ctx *c, int r, int *natoms, int *id, int *itype, double *rr, double *p, double *f,
@@ -716,7 +850,7 @@ static int bin_rep_device(comd_ctx *c, int r, int n, const double *d_xyz, const 
     c->ntot += n - c->reps[r].ntot;
     c->reps[r].ntot = n;
     c->have_atoms = 1;
-    return COMD_OK;
+    return nl_setup(c, n);
 }
 
 static int bin_rep(comd_ctx *c, int r, int n, const double *xyz, const double *pxyz, int itype)
@@ -1018,6 +1152,14 @@ int comd_tune(comd_ctx *c, int key, int value)
     case COMD_TUNE_GRAPH:
         c->use_graph = value ? 1 : 0;
         break;
+    case COMD_TUNE_PAIR_LISTS:
+        c->use_plist = value ? 1 : 0;
+        break;
+    case COMD_TUNE_SKIN_MILLI:
+        if (value < 0 || value > 3000) return set_err(c, COMD_ERR_ARG, "skin must be 0..3000 (1e-3 A)");
+        c->nl_skin_milli = value;
+        if (c->have_atoms) return nl_setup(c, c->nl_ids);
+        break;
     default:
         return set_err(c, COMD_ERR_ARG, "unknown tuning key %d", key);
     }
@@ -1030,7 +1172,9 @@ int comd_autotune(comd_ctx *c, int *waves, int *blocks_per_cu, int *balanced, do
     if (!c) return COMD_ERR_ARG;
     if (!c->have_atoms) return set_err(c, COMD_ERR_STATE, "no atoms");
     const int timing_saved = c->timing;
+    const int nl_saved = c->nl_active;
     c->timing = 0;
+    c->nl_active = 0;  // the shapes tuned are those of the link-cell kernels
     hipEvent_t a, b;
     HIPCHK(c, hipEventCreate(&a));
     HIPCHK(c, hipEventCreate(&b));
@@ -1075,11 +1219,32 @@ int comd_autotune(comd_ctx *c, int *waves, int *blocks_per_cu, int *balanced, do
     for (Replica &R : c->reps) R.pot.balanced = bd;
     if (!rc) rc = upload_descriptors(c);
     c->timing = timing_saved;
+    c->nl_active = nl_saved;
     if (rc) return rc;
     if (waves) *waves = bw;
     if (blocks_per_cu) *blocks_per_cu = bb;
     if (balanced) *balanced = bd;
     if (best_ms) *best_ms = best;
+    return COMD_OK;
+}
+
+int comd_list_info(comd_ctx *c, int *active, int *valid, int *builds, int *kmax)
+{
+    if (!c) return COMD_ERR_ARG;
+    int v = 0, nb = 0;
+    if (c->nl_active) {
+        unsigned f = 0;
+        int cnt[2] = {0, 0};
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipMemcpy(&f, c->nl.flags, 4, hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(cnt, c->nl.count, 8, hipMemcpyDeviceToHost));
+        v = (f & CM_NL_INVALID) ? 0 : 1;
+        nb = cnt[1];
+    }
+    if (active) *active = c->nl_active;
+    if (valid) *valid = v;
+    if (builds) *builds = nb;
+    if (kmax) *kmax = c->nl_active ? c->nl.kmax : 0;
     return COMD_OK;
 }
 
@@ -1186,8 +1351,9 @@ int comd_get_cheb(comd_ctx *c, double *coef6)
 int comd_eam_cheb_grad(comd_ctx *c, const double *coef, int n, const double *f_target, double e_target, double w_e,
                        double *loss, double *grad)
 {
-    if (!c || !coef || !f_target) return COMD_ERR_ARG;
+    if (!c || !coef) return COMD_ERR_ARG;
     if (!c->have_atoms || c->ntot < 1) return set_err(c, COMD_ERR_STATE, "no atoms");
+    if (!f_target && !c->adj_have_target) return set_err(c, COMD_ERR_STATE, "no force target");
     if (n > cm_adjoint_max_coefs())
         return set_err(c, COMD_ERR_ARG, "the adjoint supports at most %d coefficients per series",
                        cm_adjoint_max_coefs());
@@ -1218,7 +1384,10 @@ int comd_eam_cheb_grad(comd_ctx *c, const double *coef, int n, const double *f_t
     cheb_derivative(m, R.pot.ca[2], R.pot.cb[2], coef + 2 * m, d1.data());
     cheb_derivative(m, R.pot.ca[2], R.pot.cb[2], d1.data(), d2.data());
     HIPCHK(c, hipMemcpy(c->adj_f2, d2.data(), (size_t)m * 8, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->adj_ft, f_target, S * 3 * 8, hipMemcpyHostToDevice));
+    if (f_target) {
+        HIPCHK(c, hipMemcpy(c->adj_ft, f_target, S * 3 * 8, hipMemcpyHostToDevice));
+        c->adj_have_target = 1;
+    }
 
     // forward: forces and energy through the series
     const int eval_saved = R.pot.eval;

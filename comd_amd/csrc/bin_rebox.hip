@@ -62,6 +62,25 @@ __global__ __launch_bounds__(64) void k_move(DevState d, double dtk, double dtd,
         d.rx[s] = x;
         d.ry[s] = y;
         d.rz[s] = z;
+        if (d.nl_pos) {
+            // absolute position for the neighbour lists, and the rebuild
+            // request once the atom has moved half the skin (nearest image)
+            double cx, cy, cz;
+            cell_corner(g, c, cx, cy, cz);
+            const int id = d.id[s];
+            const double ax = cx + x, ay = cy + y, az = cz + z;
+            d.nl_pos[id] = make_double4(ax, ay, az, 0.0);
+            const double4 q = d.nl_ref[id];
+            double ex = ax - q.x, ey = ay - q.y, ez = az - q.z;
+            ex -= g.L0 * rint(ex / g.L0);
+            ey -= g.L1 * rint(ey / g.L1);
+            ez -= g.L2 * rint(ez / g.L2);
+            // one request per wave is enough
+            const bool far = ex * ex + ey * ey + ez * ez > d.nl_half2;
+            const unsigned long long who = __ballot(far);
+            if (who != 0 && (int)__ffsll((long long)who) - 1 == (int)(threadIdx.x & 63))
+                atomicOr(d.nl_flags, (unsigned)CM_NL_REBUILD);
+        }
     }
     if (!rebin) return;
     int ix, iy, iz;
@@ -224,6 +243,7 @@ __global__ __launch_bounds__(64) void k_rebox_gather(DevState d)
             __syncthreads();  // every lane has read its source before any slot is overwritten
             if (L < nout) {
                 const int o = c * CM_SLOTS + L;
+                if (d.nl_slot) d.nl_slot[gid] = o;
                 d.id[o] = gid;
                 d.itype[o] = git;
                 d.rx[o] = gx;
@@ -248,6 +268,7 @@ __global__ __launch_bounds__(64) void k_rebox_gather(DevState d)
 // pass counters.
 __device__ static void serial_copy(const DevState &d, int from, int to)
 {
+    if (d.nl_slot) d.nl_slot[d.id[from]] = to;
     d.id[to] = d.id[from];
     d.itype[to] = d.itype[from];
     d.rx[to] = d.rx[from];

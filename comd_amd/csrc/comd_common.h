@@ -8,6 +8,7 @@
 #include "../../include/comd_hip.h"
 
 #define CM_SLOTS 64  // slots per link cell (src-flat/include/pmdTypes.h:57)
+#define CM_PLIST 24  // pair-list entries kept per lane of a cell (the force kernels' queue depth)
 
 // Error bits raised by kernels into the context's device flag word.
 enum {
@@ -57,7 +58,28 @@ struct DevState {
     int *mv_slot, *mv_rank, *mv_id, *mv_itype;
     double *mv_rx, *mv_ry, *mv_rz, *mv_px, *mv_py, *mv_pz;
     unsigned *flags;    // error flag word
+    // pair lists of the EAM density pass, replayed by the embedding-force
+    // pass (null when not kept): per cell CM_PLIST x 64 halo indices
+    // ([entry][lane]), per slot-lane the entry count, per cell 1 when no
+    // lane's list overflowed
+    uint16_t *plist;
+    uint8_t *pcount;
+    int *pvalid;
+    // neighbour-list bookkeeping kept by the drift and rebin kernels (null
+    // when lists are off): per atom id its absolute position, its position
+    // at the last list build and its slot; a flag word whose bit
+    // CM_NL_REBUILD is raised when an atom has moved more than half the skin
+    double4 *nl_pos;
+    const double4 *nl_ref;
+    int *nl_slot;
+    unsigned *nl_flags;
+    double nl_half2;     // (skin / 2)^2
+    // when non-null, the link-cell force kernels run only if this word has
+    // CM_NL_INVALID set (the lists could not be built)
+    const unsigned *cell_gate;
 };
+
+enum { CM_NL_REBUILD = 1, CM_NL_INVALID = 2 };
 
 // Grid of the replica that owns global cell c.
 __device__ __forceinline__ Grid grid_of(const DevState &d, int c)

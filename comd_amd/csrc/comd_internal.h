@@ -103,3 +103,29 @@ int cm_adjoint_moments();
 void cm_launch_adj_resid(const DevState &d, const CmAdjoint &J, const double *ft, double gscale, hipStream_t st);
 void cm_launch_adj_stage(const DevState &d, const PotDev &P, const CmAdjoint &J, int stage, int mixed,
                          hipStream_t st);
+
+// Verlet neighbour lists of a single periodic simulation (nlist.hip): per
+// atom the ids of all atoms within cutoff + skin at the last build, rebuilt
+// on the device whenever an atom has moved more than half the skin.
+struct CmList {
+    int n;                   // id capacity (ids 0 .. n-1)
+    int kmax;                // list capacity per atom
+    double rl2;              // (cutoff + skin)^2
+    double L0, L1, L2;       // bounds
+    int na0, na1, na2, naux; // auxiliary grid used to build the lists
+    double ab0, ab1, ab2;    // its cell edges (>= cutoff + skin)
+    double4 *pos, *ref;      // per id
+    int *slot;               // per id: current slot, -1 when absent
+    int *order;              // atoms in build order
+    int *nbl;                // [k * n + t]: neighbour ids of the t-th atom in build order
+    int *ncnt;               // per t
+    double *fi;              // per id: F' of the last density pass
+    int *aux_cnt, *aux_start, *aux_mem;
+    int *nonempty;           // per link cell: non-empty stencil cells (Lennard-Jones virial)
+    int *count;              // [0]: atoms in build order, [1]: builds so far
+    unsigned *flags;         // CM_NL_REBUILD, CM_NL_INVALID
+};
+#define CM_NL_AUXCAP 64
+void cm_nl_init(const DevState &d, const CmList &l, hipStream_t st);
+void cm_nl_build(const DevState &d, const CmList &l, hipStream_t st);
+void cm_nl_force(const DevState &d, const PotDev &P, const CmList &l, int kind, int mixed, hipStream_t st);

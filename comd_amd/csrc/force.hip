@@ -24,9 +24,10 @@
 // The S partial sums of each home atom are added in a fixed order
 // (deterministic results).
 #include "comd_internal.h"
+#include "pair_terms.h"
 
 #define HMAX 288    // halo atoms staged per batch
-#define QCAP 24     // queue entries per lane
+#define QCAP CM_PLIST  // queue entries per lane (also the pair-list rows a cell keeps)
 #define UNROLL 4    // candidates tested per lane between queue checks
 
 // ---------------------------------------------------------------------------
@@ -36,13 +37,15 @@
 struct WaveLds {
     double *hx, *hy, *hz, *hf;
     uint16_t *q;
+    uint16_t *lst;  // EAM density pass: the pair list being recorded (CM_PLIST x 64)
     int *qn, *cell, *off;
     double *drb;
 };
 
 __host__ __device__ constexpr int wave_lds_bytes(int kind)
 {
-    return HMAX * 8 * (kind == CM_KIND_EAM3 ? 4 : 3) + 27 * 3 * 8 + QCAP * 64 * 2 + 64 * 4 + 27 * 4 + 28 * 4;
+    return HMAX * 8 * (kind == CM_KIND_EAM3 ? 4 : 3) + 27 * 3 * 8 + QCAP * 64 * 2 +
+           (kind == CM_KIND_EAM1 ? CM_PLIST * 64 * 2 : 0) + 64 * 4 + 27 * 4 + 28 * 4;
 }
 
 __device__ __forceinline__ WaveLds carve(char *base, int kind)
@@ -60,7 +63,8 @@ __device__ __forceinline__ WaveLds carve(char *base, int kind)
     }
     w.drb = p;
     w.q = (uint16_t *)(w.drb + 27 * 3);
-    w.qn = (int *)(w.q + QCAP * 64);
+    w.lst = w.q + QCAP * 64;
+    w.qn = (int *)(w.lst + (kind == CM_KIND_EAM1 ? CM_PLIST * 64 : 0));
     w.cell = w.qn + 64;
     w.off = w.cell + 27;
     return w;
@@ -74,176 +78,6 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
-
-// ---------------------------------------------------------------------------
-// potential evaluation
-
-// 3-point quadratic value and interpolated central-difference derivative,
-// the formula of src-flat/eam.c:401-434 in double precision (used once per
-// atom for F).  v points at values[-1].
-__device__ __forceinline__ void interp64(const double *v, double x0, double xn, double invdx, int n, double r,
-                                         double &val, double &der)
-{
-    r = fmin(fmax(r, x0), xn);
-    r = (r - x0) * invdx;
-    const double fl = floor(r);
-    int i = (int)fl;
-    r = r - fl;
-    // r <= xn keeps i <= n-1 for a consistent table header; the clamp only
-    // keeps a rounding or a bad header inside values[-1..n+1]
-    i = min(max(i, 0), n - 1);
-    const double *w = v + 1 + i;
-    const double vm = w[-1], v0 = w[0], v1 = w[1], v2 = w[2];
-    const double g0 = v1 - vm;
-    const double g1 = v2 - v0;
-    val = v0 + 0.5 * r * (r * (v1 + vm - 2.0 * v0) + g0);
-    der = (i <= 0) ? 0.0 : 0.5 * (g0 + r * (g1 - g0)) * invdx;
-}
-
-// Interval of the table formula that holds r (clamped to [x0, xn] as in
-// src-flat/eam.c:413-414) and the fraction t inside it.
-__device__ __forceinline__ int table_interval(double r, double x0, double xn, double invdx, int n, double &t)
-{
-    r = fmin(fmax(r, x0), xn);
-    const double u = (r - x0) * invdx;
-    const double fl = floor(u);
-    t = u - fl;
-    return min(max((int)fl, 0), n - 1);
-}
-
-// Table formula from per-interval coefficients (d0 d1 c0 c1 c2 0):
-// value c0 + t(c1 + t c2), derivative d0 + t d1.
-__device__ __forceinline__ double2 cf_der64(const double *cf, int i) { return ((const double2 *)(cf + 6 * i))[0]; }
-__device__ __forceinline__ double cf_val64(const double *cf, int i, double t)
-{
-    const double2 a = ((const double2 *)(cf + 6 * i))[1];
-    const double c2 = cf[6 * i + 4];
-    return fma(t, fma(t, c2, a.y), a.x);
-}
-__device__ __forceinline__ float4 cf_row32(const float *cf, int i) { return ((const float4 *)(cf + 8 * i))[0]; }
-
-// 1/sqrt(x) in double precision: hardware estimate and two Newton steps.
-__device__ __forceinline__ double rsqrt64(double x)
-{
-    double y = __builtin_amdgcn_rsq(x);
-    double e = fma(-x * y, y, 1.0);
-    y = fma(0.5 * y, e, y);
-    e = fma(-x * y, y, 1.0);
-    return fma(0.5 * y, e, y);
-}
-
-// 1/sqrt(x): single-precision estimate and one Newton step in double.
-__device__ __forceinline__ double rsqrt_mixed(double x)
-{
-    const double y = (double)__builtin_amdgcn_rsqf((float)x);
-    const double e = fma(-x * y, y, 1.0);
-    return fma(0.5 * y, e, y);
-}
-
-// ---------------------------------------------------------------------------
-// one accepted pair (home atom a, neighbour j): dx = r_a - r_j
-
-struct Acc {
-    double x, y, z, e, rho, vir;
-};
-
-template <int KIND, int EVAL, int MIXED>
-__device__ __forceinline__ void pair_term(const PotDev &P, double dx, double dy, double dz, double r2, double fia,
-                                          double fij, Acc &A)
-{
-    if (KIND == CM_KIND_LJ) {
-        // src-flat/ljForce.c:196-210 (f holds -grad U, energy without 4 eps s6)
-        const double s6 = P.lj_s6;
-        double r2i;
-        if (MIXED) {
-            const double y = rsqrt_mixed(r2);
-            r2i = y * y;
-        } else {
-            r2i = 1.0 / r2;
-        }
-        const double r6 = r2i * r2i * r2i;
-        A.e += 0.5 * r6 * (s6 * r6 - 1.0);
-        const double fr = 4.0 * P.lj_eps * s6 * r6 * r2i * (12.0 * s6 * r6 - 6.0);
-        A.x += dx * fr;
-        A.y += dy * fr;
-        A.z += dz * fr;
-        A.vir += fr * dx * dx;
-        return;
-    }
-    const double y = MIXED ? rsqrt_mixed(r2) : rsqrt64(r2);  // 1/r
-    const double r = r2 * y;
-    if (KIND == CM_KIND_EAM1) {
-        // src-flat/eam.c:521-545 (f holds +grad U)
-        double phiv, dphi, rhov;
-        if (EVAL == COMD_EVAL_CHEB) {
-            const int m = P.cn;
-            if (MIXED) {
-                const float rf = (float)r;
-                const float a0 = (float)P.ca[0], b0 = (float)P.cb[0], a1 = (float)P.ca[1], b1 = (float)P.cb[1];
-                phiv = cheb32(P.cheb32 + 0 * m, m, a0, b0, rf);
-                dphi = cheb32(P.cheb32 + 3 * m, m, a0, b0, rf);
-                rhov = cheb32(P.cheb32 + 1 * m, m, a1, b1, rf);
-            } else {
-                phiv = cheb64(P.cheb + 0 * m, m, P.ca[0], P.cb[0], r);
-                dphi = cheb64(P.cheb + 3 * m, m, P.ca[0], P.cb[0], r);
-                rhov = cheb64(P.cheb + 1 * m, m, P.ca[1], P.cb[1], r);
-            }
-        } else {
-            double t0, t1;
-            const int i0 = table_interval(r, P.x0[0], P.xn[0], P.invdx[0], P.n[0], t0);
-            int i1 = i0;
-            t1 = t0;
-            if (!P.same_grid) i1 = table_interval(r, P.x0[1], P.xn[1], P.invdx[1], P.n[1], t1);
-            if (MIXED) {
-                const float4 a = cf_row32(P.cf32[0], i0);
-                const float a2 = P.cf32[0][8 * i0 + 4];
-                const float4 b = cf_row32(P.cf32[1], i1);
-                const float b2 = P.cf32[1][8 * i1 + 4];
-                const float f0 = (float)t0, f1 = (float)t1;
-                phiv = fmaf(f0, fmaf(f0, a2, a.w), a.z);
-                dphi = fmaf(f0, a.y, a.x);
-                rhov = fmaf(f1, fmaf(f1, b2, b.w), b.z);
-            } else {
-                const double2 dd = cf_der64(P.cf64[0], i0);
-                phiv = cf_val64(P.cf64[0], i0, t0);
-                dphi = fma(t0, dd.y, dd.x);
-                rhov = cf_val64(P.cf64[1], i1, t1);
-            }
-        }
-        const double s = dphi * y;
-        A.x += s * dx;
-        A.y += s * dy;
-        A.z += s * dz;
-        A.e += 0.5 * phiv;
-        A.rho += rhov;
-    } else {
-        // src-flat/eam.c:633-649
-        double drho;
-        if (EVAL == COMD_EVAL_CHEB) {
-            if (MIXED)
-                drho = cheb32(P.cheb32 + 4 * P.cn, P.cn, (float)P.ca[1], (float)P.cb[1], (float)r);
-            else
-                drho = cheb64(P.cheb + 4 * P.cn, P.cn, P.ca[1], P.cb[1], r);
-        } else {
-            double t1;
-            const int i1 = table_interval(r, P.x0[1], P.xn[1], P.invdx[1], P.n[1], t1);
-            if (MIXED) {
-                const float2 dd = ((const float2 *)(P.cf32[1] + 8 * i1))[0];
-                drho = fmaf((float)t1, dd.y, dd.x);
-            } else {
-                const double2 dd = cf_der64(P.cf64[1], i1);
-                drho = fma(t1, dd.y, dd.x);
-            }
-        }
-        const double g = (fia + fij) * drho * y;
-        A.x += g * dx;
-        A.y += g * dy;
-        A.z += g * dz;
-        A.vir += g * dx * dx;
-    }
-}
-
-// ---------------------------------------------------------------------------
 
 // One home cell: stencil, halo staging, candidate tests, pair terms,
 // reduction of the lanes' partial sums and the epilogue.  The 64 lanes of a
@@ -291,22 +125,68 @@ __device__ __forceinline__ void pair_cell(const DevState &d, const Grid &g, cons
     const double fia = KIND == CM_KIND_EAM3 ? d.fi[hs] : 0.0;
     Acc A = {0, 0, 0, 0, 0, 0};
     int npair = 0;
+
+    // pair list of the density pass: the halo index of every pair this lane
+    // evaluates, in order (entries past CM_PLIST are counted, not kept)
+    const bool keep = KIND == CM_KIND_EAM1 && d.plist != nullptr;
+    int nrec = 0;
+    auto record = [&](int hj) {
+        if (keep) {
+            if (nrec < CM_PLIST) w.lst[nrec * 64 + L] = (uint16_t)hj;
+            nrec++;
+        }
+    };
+    // the embedding-force pass replays that list when every lane's fitted:
+    // the same pairs in the same order as its own candidate search would give
+    const bool replay = KIND == CM_KIND_EAM3 && d.plist != nullptr && d.pvalid[c] != 0;
+    int ncnt = 0, kread = 0;
+    if (replay) {
+        ncnt = act ? d.pcount[c * CM_SLOTS + L] : 0;
+        int kmax = ncnt;
+        for (int o = 32; o > 0; o >>= 1) kmax = max(kmax, __shfl_xor(kmax, o, 64));
+        for (int k = 0; k < kmax; k++) w.q[k * 64 + L] = d.plist[((size_t)c * CM_PLIST + k) * 64 + L];
+    }
     wave_sync();
 
     for (int h0 = 0; h0 < H; h0 += HMAX) {
         const int hb = H - h0 < HMAX ? H - h0 : HMAX;
-        // stage the batch: lane L takes halo atoms L, L+64, ...; its
-        // stencil entry only moves forward
+        // stage the batch: lane L takes halo atoms L, L+64, ... (at most
+        // SU); all of its global loads are issued before any LDS store so
+        // that their latencies overlap; its stencil entry only moves forward
         {
+            constexpr int SU = (HMAX + 63) / 64;
+            int sl[SU], ee[SU];
             int e = 0;
-            for (int h = h0 + L; h < h0 + hb; h += 64) {
-                while (w.off[e + 1] <= h) e++;
-                const int sl = w.cell[e] * CM_SLOTS + (h - w.off[e]);
-                const double x = d.rx[sl], y = d.ry[sl], z = d.rz[sl];
-                w.hx[h - h0] = x - w.drb[3 * e + 0];
-                w.hy[h - h0] = y - w.drb[3 * e + 1];
-                w.hz[h - h0] = z - w.drb[3 * e + 2];
-                if (KIND == CM_KIND_EAM3) w.hf[h - h0] = d.fi[sl];
+#pragma unroll
+            for (int u = 0; u < SU; u++) {
+                const int h = h0 + L + 64 * u;
+                sl[u] = -1;
+                ee[u] = 0;
+                if (h < h0 + hb) {
+                    while (w.off[e + 1] <= h) e++;
+                    sl[u] = w.cell[e] * CM_SLOTS + (h - w.off[e]);
+                    ee[u] = e;
+                }
+            }
+            double sx[SU], sy[SU], sz[SU], sf[SU];
+#pragma unroll
+            for (int u = 0; u < SU; u++) {
+                const int q = sl[u] >= 0 ? sl[u] : 0;
+                sx[u] = d.rx[q];
+                sy[u] = d.ry[q];
+                sz[u] = d.rz[q];
+                sf[u] = KIND == CM_KIND_EAM3 ? d.fi[q] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < SU; u++) {
+                if (sl[u] >= 0) {
+                    const int hh = L + 64 * u;
+                    const int eu = ee[u];
+                    w.hx[hh] = sx[u] - w.drb[3 * eu + 0];
+                    w.hy[hh] = sy[u] - w.drb[3 * eu + 1];
+                    w.hz[hh] = sz[u] - w.drb[3 * eu + 2];
+                    if (KIND == CM_KIND_EAM3) w.hf[hh] = sf[u];
+                }
             }
         }
         wave_sync();
@@ -325,6 +205,7 @@ __device__ __forceinline__ void pair_cell(const DevState &d, const Grid &g, cons
                         pair_term<KIND, EVAL, MIXED>(P, dx, dy, dz, r2, fia,
                                                      KIND == CM_KIND_EAM3 ? w.hf[j] : 0.0, A);
                         npair++;
+                        record(h0 + j);
                     }
                 }
                 cnt = 0;
@@ -349,12 +230,35 @@ __device__ __forceinline__ void pair_cell(const DevState &d, const Grid &g, cons
                     const double r2 = dx * dx + dy * dy + dz * dz;
                     pair_term<KIND, EVAL, MIXED>(P, dx, dy, dz, r2, fia, KIND == CM_KIND_EAM3 ? w.hf[j] : 0.0, A);
                     npair++;
+                    record(h0 + j);
                     v += S;
                 }
             }
             cnt = 0;
             wave_sync();
         };
+        if (replay) {
+            // this batch's entries of the lane's list (entries rise with
+            // the batch), re-tested with the pass's own predicate
+            while (true) {
+                const int hj = kread < ncnt ? (int)w.q[kread * 64 + L] : 0;
+                const bool more = kread < ncnt && hj < h0 + hb;
+                if (__ballot(more) == 0) break;
+                if (more) {
+                    const int j = hj - h0;
+                    const double dx = xa - w.hx[j], dy = ya - w.hy[j], dz = za - w.hz[j];
+                    const double r2 = dx * dx + dy * dy + dz * dz;
+                    if (r2 < rc2) {
+                        pair_term<KIND, EVAL, MIXED>(P, dx, dy, dz, r2, fia, KIND == CM_KIND_EAM3 ? w.hf[j] : 0.0,
+                                                     A);
+                        npair++;
+                    }
+                    kread++;
+                }
+            }
+            wave_sync();
+            continue;
+        }
         const int trips = (hb + S - 1) / S;
         for (int it0 = 0; it0 < trips; it0 += UNROLL) {
             bool acc[UNROLL];
@@ -379,6 +283,17 @@ __device__ __forceinline__ void pair_cell(const DevState &d, const Grid &g, cons
             if (__ballot(cnt > QCAP - UNROLL) != 0) drain();
         }
         drain();
+    }
+
+    if (keep) {
+        // the list leaves LDS as whole 128-byte rows (entries past a lane's
+        // count are never read)
+        d.pcount[c * CM_SLOTS + L] = (uint8_t)(nrec < 255 ? nrec : 255);
+        const bool over = __ballot(nrec > CM_PLIST) != 0;
+        if (L == 0) d.pvalid[c] = over ? 0 : 1;
+        int kmax = nrec < CM_PLIST ? nrec : CM_PLIST;
+        for (int o = 32; o > 0; o >>= 1) kmax = max(kmax, __shfl_xor(kmax, o, 64));
+        for (int k = 0; k < kmax; k++) d.plist[((size_t)c * CM_PLIST + k) * 64 + L] = w.lst[k * 64 + L];
     }
 
     // ---- combine the S partial sums of each home atom, in s order
@@ -446,6 +361,8 @@ __device__ __forceinline__ void pair_cell(const DevState &d, const Grid &g, cons
 template <int KIND, int EVAL, int MIXED>
 __global__ __launch_bounds__(256) void k_pair(DevState d, PotDev P)
 {
+    // the neighbour-list kernels did this pass unless their lists are invalid
+    if (d.cell_gate && !(*d.cell_gate & CM_NL_INVALID)) return;
     extern __shared__ __align__(16) char lds[];
     const int W = blockDim.x >> 6;
     const int wv = threadIdx.x >> 6;

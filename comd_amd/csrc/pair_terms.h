@@ -1,0 +1,178 @@
+// pair_terms.h -- the per-pair arithmetic of the force kernels (table and
+// Chebyshev evaluation, EAM density/pair and embedding-force terms,
+// Lennard-Jones), shared by the link-cell kernels (force.hip) and the
+// neighbour-list kernels (nlist.hip).  Internal to comd_amd/csrc.
+#pragma once
+
+#include "comd_internal.h"
+
+// ---------------------------------------------------------------------------
+// potential evaluation
+
+// 3-point quadratic value and interpolated central-difference derivative,
+// the formula of src-flat/eam.c:401-434 in double precision (used once per
+// atom for F).  v points at values[-1].
+__device__ __forceinline__ void interp64(const double *v, double x0, double xn, double invdx, int n, double r,
+                                         double &val, double &der)
+{
+    r = fmin(fmax(r, x0), xn);
+    r = (r - x0) * invdx;
+    const double fl = floor(r);
+    int i = (int)fl;
+    r = r - fl;
+    // r <= xn keeps i <= n-1 for a consistent table header; the clamp only
+    // keeps a rounding or a bad header inside values[-1..n+1]
+    i = min(max(i, 0), n - 1);
+    const double *w = v + 1 + i;
+    const double vm = w[-1], v0 = w[0], v1 = w[1], v2 = w[2];
+    const double g0 = v1 - vm;
+    const double g1 = v2 - v0;
+    val = v0 + 0.5 * r * (r * (v1 + vm - 2.0 * v0) + g0);
+    der = (i <= 0) ? 0.0 : 0.5 * (g0 + r * (g1 - g0)) * invdx;
+}
+
+// Interval of the table formula that holds r (clamped to [x0, xn] as in
+// src-flat/eam.c:413-414) and the fraction t inside it.
+__device__ __forceinline__ int table_interval(double r, double x0, double xn, double invdx, int n, double &t)
+{
+    r = fmin(fmax(r, x0), xn);
+    const double u = (r - x0) * invdx;
+    const double fl = floor(u);
+    t = u - fl;
+    return min(max((int)fl, 0), n - 1);
+}
+
+// Table formula from per-interval coefficients (d0 d1 c0 c1 c2 0):
+// value c0 + t(c1 + t c2), derivative d0 + t d1.
+__device__ __forceinline__ double2 cf_der64(const double *cf, int i) { return ((const double2 *)(cf + 6 * i))[0]; }
+__device__ __forceinline__ double cf_val64(const double *cf, int i, double t)
+{
+    const double2 a = ((const double2 *)(cf + 6 * i))[1];
+    const double c2 = cf[6 * i + 4];
+    return fma(t, fma(t, c2, a.y), a.x);
+}
+__device__ __forceinline__ float4 cf_row32(const float *cf, int i) { return ((const float4 *)(cf + 8 * i))[0]; }
+
+// 1/sqrt(x) in double precision: hardware estimate and two Newton steps.
+__device__ __forceinline__ double rsqrt64(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+    double e = fma(-x * y, y, 1.0);
+    y = fma(0.5 * y, e, y);
+    e = fma(-x * y, y, 1.0);
+    return fma(0.5 * y, e, y);
+}
+
+// 1/sqrt(x): single-precision estimate and one Newton step in double.
+__device__ __forceinline__ double rsqrt_mixed(double x)
+{
+    const double y = (double)__builtin_amdgcn_rsqf((float)x);
+    const double e = fma(-x * y, y, 1.0);
+    return fma(0.5 * y, e, y);
+}
+
+// ---------------------------------------------------------------------------
+// one accepted pair (home atom a, neighbour j): dx = r_a - r_j
+
+struct Acc {
+    double x, y, z, e, rho, vir;
+};
+
+template <int KIND, int EVAL, int MIXED>
+__device__ __forceinline__ void pair_term(const PotDev &P, double dx, double dy, double dz, double r2, double fia,
+                                          double fij, Acc &A)
+{
+    if (KIND == CM_KIND_LJ) {
+        // src-flat/ljForce.c:196-210 (f holds -grad U, energy without 4 eps s6)
+        const double s6 = P.lj_s6;
+        double r2i;
+        if (MIXED) {
+            const double y = rsqrt_mixed(r2);
+            r2i = y * y;
+        } else {
+            r2i = 1.0 / r2;
+        }
+        const double r6 = r2i * r2i * r2i;
+        A.e += 0.5 * r6 * (s6 * r6 - 1.0);
+        const double fr = 4.0 * P.lj_eps * s6 * r6 * r2i * (12.0 * s6 * r6 - 6.0);
+        A.x += dx * fr;
+        A.y += dy * fr;
+        A.z += dz * fr;
+        A.vir += fr * dx * dx;
+        return;
+    }
+    const double y = MIXED ? rsqrt_mixed(r2) : rsqrt64(r2);  // 1/r
+    const double r = r2 * y;
+    if (KIND == CM_KIND_EAM1) {
+        // src-flat/eam.c:521-545 (f holds +grad U)
+        double phiv, dphi, rhov;
+        if (EVAL == COMD_EVAL_CHEB) {
+            const int m = P.cn;
+            if (MIXED) {
+                const float rf = (float)r;
+                const float a0 = (float)P.ca[0], b0 = (float)P.cb[0], a1 = (float)P.ca[1], b1 = (float)P.cb[1];
+                phiv = cheb32(P.cheb32 + 0 * m, m, a0, b0, rf);
+                dphi = cheb32(P.cheb32 + 3 * m, m, a0, b0, rf);
+                rhov = cheb32(P.cheb32 + 1 * m, m, a1, b1, rf);
+            } else {
+                phiv = cheb64(P.cheb + 0 * m, m, P.ca[0], P.cb[0], r);
+                dphi = cheb64(P.cheb + 3 * m, m, P.ca[0], P.cb[0], r);
+                rhov = cheb64(P.cheb + 1 * m, m, P.ca[1], P.cb[1], r);
+            }
+        } else {
+            double t0, t1;
+            const int i0 = table_interval(r, P.x0[0], P.xn[0], P.invdx[0], P.n[0], t0);
+            int i1 = i0;
+            t1 = t0;
+            if (!P.same_grid) i1 = table_interval(r, P.x0[1], P.xn[1], P.invdx[1], P.n[1], t1);
+            if (MIXED) {
+                const float4 a = cf_row32(P.cf32[0], i0);
+                const float a2 = P.cf32[0][8 * i0 + 4];
+                const float4 b = cf_row32(P.cf32[1], i1);
+                const float b2 = P.cf32[1][8 * i1 + 4];
+                const float f0 = (float)t0, f1 = (float)t1;
+                phiv = fmaf(f0, fmaf(f0, a2, a.w), a.z);
+                dphi = fmaf(f0, a.y, a.x);
+                rhov = fmaf(f1, fmaf(f1, b2, b.w), b.z);
+            } else {
+                const double2 dd = cf_der64(P.cf64[0], i0);
+                phiv = cf_val64(P.cf64[0], i0, t0);
+                dphi = fma(t0, dd.y, dd.x);
+                rhov = cf_val64(P.cf64[1], i1, t1);
+            }
+        }
+        const double s = dphi * y;
+        A.x += s * dx;
+        A.y += s * dy;
+        A.z += s * dz;
+        A.e += 0.5 * phiv;
+        A.rho += rhov;
+    } else {
+        // src-flat/eam.c:633-649
+        double drho;
+        if (EVAL == COMD_EVAL_CHEB) {
+            if (MIXED)
+                drho = cheb32(P.cheb32 + 4 * P.cn, P.cn, (float)P.ca[1], (float)P.cb[1], (float)r);
+            else
+                drho = cheb64(P.cheb + 4 * P.cn, P.cn, P.ca[1], P.cb[1], r);
+        } else {
+            double t1;
+            const int i1 = table_interval(r, P.x0[1], P.xn[1], P.invdx[1], P.n[1], t1);
+            if (MIXED) {
+                const float2 dd = ((const float2 *)(P.cf32[1] + 8 * i1))[0];
+                drho = fmaf((float)t1, dd.y, dd.x);
+            } else {
+                const double2 dd = cf_der64(P.cf64[1], i1);
+                drho = fma(t1, dd.y, dd.x);
+            }
+        }
+        const double g = (fia + fij) * drho * y;
+        A.x += g * dx;
+        A.y += g * dy;
+        A.z += g * dz;
+        A.vir += g * dx * dx;
+    }
+}
+
+// ---------------------------------------------------------------------------
+

@@ -154,6 +154,13 @@ enum comd_tune_key {
     COMD_TUNE_BALANCED_DRAIN = 4,  /* 0/1: share an atom's accepted pairs among its lanes */
     COMD_TUNE_GRAPH = 5,           /* 0/1: comd_steps/comd_nsteps replay one captured step
                                       (default 1 for batches, 0 for single simulations) */
+    COMD_TUNE_PAIR_LISTS = 6,      /* 0/1, default 1: the EAM embedding-force pass replays the pairs
+                                      the density pass accepted instead of testing candidates again */
+    COMD_TUNE_SKIN_MILLI = 7,      /* Verlet skin in 1e-3 A (default 400; 0 = link-cell kernels only).
+                                      Single periodic simulations with >= 3 cells of cutoff + skin per
+                                      dimension keep per-atom neighbour lists, rebuilt on the device
+                                      when an atom has moved half the skin; same pairs and outputs,
+                                      per-atom sums in another order */
 };
 int comd_tune(comd_ctx *ctx, int key, int value);
 
@@ -164,6 +171,11 @@ int comd_tune(comd_ctx *ctx, int key, int value);
  * split changes the summation order, so forces may differ in the last bits
  * between the two splits (both within the parity bar). */
 int comd_autotune(comd_ctx *ctx, int *waves, int *blocks_per_cu, int *balanced, double *best_ms);
+
+/* State of the Verlet neighbour lists (COMD_TUNE_SKIN_MILLI): whether the
+ * context uses them, whether the last build fitted (otherwise the link-cell
+ * kernels run), how many builds have run, and the list capacity per atom. */
+int comd_list_info(comd_ctx *ctx, int *active, int *valid, int *builds, int *kmax);
 
 /* Number of atom pairs inside the cutoff found by the last EAM density or
  * LJ pass (each unordered pair once: the reference's half-pair count). */
@@ -243,8 +255,9 @@ int comd_get_cheb(comd_ctx *ctx, double *coef6);
  * E the potential energy) and grad[3n] = dloss/dcoef from the adjoint
  * kernels.  f_target holds 3 doubles per slot in the context's current slot
  * layout (as comd_download returns f without its 4th component); empty
- * slots are ignored.  No counterpart in the reference: it replaces fitting
- * by finite differences over the force call. */
+ * slots are ignored.  f_target NULL reuses the target of the previous call
+ * on this context (it stays in device memory).  No counterpart in the
+ * reference: it replaces fitting by finite differences over the force call. */
 int comd_eam_cheb_grad(comd_ctx *ctx, const double *coef, int n, const double *f_target, double e_target,
                        double w_e, double *loss, double *grad);
 
