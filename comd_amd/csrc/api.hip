@@ -644,6 +644,9 @@ static int ctx_create(int nrep, const comd_params *prms, comd_ctx **out)
         g.L0 = p->bounds[0];
         g.L1 = p->bounds[1];
         g.L2 = p->bounds[2];
+        g.iL0 = 1.0 / g.L0;
+        g.iL1 = 1.0 / g.L1;
+        g.iL2 = 1.0 / g.L2;
         g.rmass = kAmuToMe * p->mass;  // src-flat/timestep.c:90
         if (nrep > 1) cell_grid.insert(cell_grid.end(), (size_t)g.ncells, r);
         base += g.ncells;

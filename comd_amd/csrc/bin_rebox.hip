@@ -34,17 +34,12 @@ __device__ static inline void mark_touched(const DevState &d, int c)
 // ---------------------------------------------------------------------------
 // kick (p -= dtk f, src-flat/timestep.c:62-74), drift (r += (dtd p)/rmass,
 // timestep.c:76-104) and the per-atom destination of reBoxAll
-// (domains.c:356-377); departing atoms are recorded as movers.  One block of
-// 64 threads per cell.
-__global__ __launch_bounds__(64) void k_move(DevState d, double dtk, double dtd, int kick, int drift, int rebin)
+// (domains.c:356-377) of the atom in slot s of cell c = (ix, iy, iz);
+// departing atoms are recorded as movers.
+__device__ __forceinline__ void move_atom(const DevState &d, const Grid &g, int c, int ix, int iy, int iz, int s,
+                                          double dtk, double dtd, int kick, int drift, int rebin)
 {
-    const int c = blockIdx.x;
-    const int t = threadIdx.x;
-    const int n = d.natoms[c];
-    if (t >= n) return;
-    const Grid g = grid_of(d, c);
     const double rmass = g.rmass;
-    const int s = c * CM_SLOTS + t;
     double x = d.rx[s], y = d.ry[s], z = d.rz[s];
     double px = d.px[s], py = d.py[s], pz = d.pz[s];
     if (kick) {
@@ -66,15 +61,15 @@ __global__ __launch_bounds__(64) void k_move(DevState d, double dtk, double dtd,
             // absolute position for the neighbour lists, and the rebuild
             // request once the atom has moved half the skin (nearest image)
             double cx, cy, cz;
-            cell_corner(g, c, cx, cy, cz);
+            corner_of(g, ix, iy, iz, cx, cy, cz);
             const int id = d.id[s];
             const double ax = cx + x, ay = cy + y, az = cz + z;
             d.nl_pos[id] = make_double4(ax, ay, az, 0.0);
             const double4 q = d.nl_ref[id];
             double ex = ax - q.x, ey = ay - q.y, ez = az - q.z;
-            ex -= g.L0 * rint(ex / g.L0);
-            ey -= g.L1 * rint(ey / g.L1);
-            ez -= g.L2 * rint(ez / g.L2);
+            ex -= g.L0 * rint(ex * g.iL0);
+            ey -= g.L1 * rint(ey * g.iL1);
+            ez -= g.L2 * rint(ez * g.iL2);
             // one request per wave is enough
             const bool far = ex * ex + ey * ey + ez * ez > d.nl_half2;
             const unsigned long long who = __ballot(far);
@@ -83,8 +78,6 @@ __global__ __launch_bounds__(64) void k_move(DevState d, double dtk, double dtd,
         }
     }
     if (!rebin) return;
-    int ix, iy, iz;
-    cell_xyz(g, c, ix, iy, iz);
     int nx = ix, ny = iy, nz = iz;
     double ax = x, ay = y, az = z;
     if (x < 0.0) { nx = ix - 1; ax = x + g.bs0; } else if (x >= g.bs0) { nx = ix + 1; ax = x - g.bs0; }
@@ -120,6 +113,32 @@ __global__ __launch_bounds__(64) void k_move(DevState d, double dtk, double dtd,
     if (k < CM_ACAP) d.arr[dst * CM_ACAP + k] = m;
     mark_touched(d, c);
     mark_touched(d, dst);
+}
+
+// Any number of replicas: one block of 64 threads per cell.
+__global__ __launch_bounds__(64) void k_move(DevState d, double dtk, double dtd, int kick, int drift, int rebin)
+{
+    const int c = blockIdx.x;
+    const int t = threadIdx.x;
+    if (t >= d.natoms[c]) return;
+    const Grid g = grid_of(d, c);
+    int ix, iy, iz;
+    cell_xyz(g, c, ix, iy, iz);
+    move_atom(d, g, c, ix, iy, iz, c * CM_SLOTS + t, dtk, dtd, kick, drift, rebin);
+}
+
+// One replica: four cells per wave, 16 lanes each, cell coordinates from the
+// 3-D grid (x: groups of four cells).
+__global__ __launch_bounds__(64) void k_move4(DevState d, double dtk, double dtd, int kick, int drift, int rebin)
+{
+    const Grid &g = d.g0;
+    const int ix = blockIdx.x * 4 + (threadIdx.x >> 4);
+    const int iy = blockIdx.y, iz = blockIdx.z;
+    if (ix >= g.nbx0) return;
+    const int c = ix + g.nbx0 * (iy + g.nbx1 * iz);
+    const int n = d.natoms[c];
+    for (int t = threadIdx.x & 15; t < n; t += 16)
+        move_atom(d, g, c, ix, iy, iz, c * CM_SLOTS + t, dtk, dtd, kick, drift, rebin);
 }
 
 // Per touched cell (one thread each): count arrivals from lower/higher cells,
@@ -398,7 +417,12 @@ __global__ void k_fill_int(int *p, int v, long n)
 
 void cm_launch_move(const DevState &d, hipStream_t st, double dtk, double dtd, int kick, int drift, int rebin)
 {
-    hipLaunchKernelGGL(k_move, dim3(d.ncells), dim3(64), 0, st, d, dtk, dtd, kick, drift, rebin);
+    if (d.ngrids == 1 && d.g0.cell_base == 0) {
+        const dim3 grid((unsigned)((d.g0.nbx0 + 3) / 4), (unsigned)d.g0.nbx1, (unsigned)d.g0.nbx2);
+        hipLaunchKernelGGL(k_move4, grid, dim3(64), 0, st, d, dtk, dtd, kick, drift, rebin);
+    } else {
+        hipLaunchKernelGGL(k_move, dim3(d.ncells), dim3(64), 0, st, d, dtk, dtd, kick, drift, rebin);
+    }
 }
 
 void cm_launch_rebox(const DevState &d, hipStream_t st)

@@ -24,6 +24,7 @@ struct Grid {
     int nbx0, nbx1, nbx2, ncells, cell_base, periodic, pot, pad;
     double bs0, bs1, bs2;  // boxsize
     double L0, L1, L2;     // bounds
+    double iL0, iL1, iL2;  // 1 / bounds
     double rmass;          // mass in the reference's internal unit (1822.83 m_amu)
 };
 
@@ -125,14 +126,19 @@ __device__ __forceinline__ int stencil_cell(const Grid &g, int c, int e)
 // with contraction: the shift of a cell against itself must be exactly 0
 // (fma(b, i, -b*i) is the rounding error of b*i, not 0), or an atom would
 // see itself as a neighbour at distance ~1e-16.
-__device__ __forceinline__ void cell_corner(const Grid &g, int c, double &x, double &y, double &z)
+__device__ __forceinline__ void corner_of(const Grid &g, int ix, int iy, int iz, double &x, double &y, double &z)
 {
 #pragma clang fp contract(off)
-    int ix, iy, iz;
-    cell_xyz(g, c, ix, iy, iz);
     x = g.bs0 * (double)ix;
     y = g.bs1 * (double)iy;
     z = g.bs2 * (double)iz;
+}
+
+__device__ __forceinline__ void cell_corner(const Grid &g, int c, double &x, double &y, double &z)
+{
+    int ix, iy, iz;
+    cell_xyz(g, c, ix, iy, iz);
+    corner_of(g, ix, iy, iz, x, y, z);
 }
 
 // Minimum-imaged corner difference dcenter[i]-dcenter[j]

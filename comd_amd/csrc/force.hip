@@ -87,6 +87,7 @@ __device__ __forceinline__ void pair_cell(const DevState &d, const Grid &g, cons
                                           int L)
 {
     const double rc2 = P.rc2;
+    const Tabs T = global_tabs(P);
     const int nh = d.natoms[c];
     if (nh == 0) return;
 
@@ -202,7 +203,7 @@ __device__ __forceinline__ void pair_cell(const DevState &d, const Grid &g, cons
                         const int j = w.q[k * 64 + L];
                         const double dx = xa - w.hx[j], dy = ya - w.hy[j], dz = za - w.hz[j];
                         const double r2 = dx * dx + dy * dy + dz * dz;
-                        pair_term<KIND, EVAL, MIXED>(P, dx, dy, dz, r2, fia,
+                        pair_term<KIND, EVAL, MIXED>(P, T, dx, dy, dz, r2, fia,
                                                      KIND == CM_KIND_EAM3 ? w.hf[j] : 0.0, A);
                         npair++;
                         record(h0 + j);
@@ -228,7 +229,7 @@ __device__ __forceinline__ void pair_cell(const DevState &d, const Grid &g, cons
                     const int j = w.q[(v - base) * 64 + a + sp * nh];
                     const double dx = xa - w.hx[j], dy = ya - w.hy[j], dz = za - w.hz[j];
                     const double r2 = dx * dx + dy * dy + dz * dz;
-                    pair_term<KIND, EVAL, MIXED>(P, dx, dy, dz, r2, fia, KIND == CM_KIND_EAM3 ? w.hf[j] : 0.0, A);
+                    pair_term<KIND, EVAL, MIXED>(P, T, dx, dy, dz, r2, fia, KIND == CM_KIND_EAM3 ? w.hf[j] : 0.0, A);
                     npair++;
                     record(h0 + j);
                     v += S;
@@ -249,7 +250,7 @@ __device__ __forceinline__ void pair_cell(const DevState &d, const Grid &g, cons
                     const double dx = xa - w.hx[j], dy = ya - w.hy[j], dz = za - w.hz[j];
                     const double r2 = dx * dx + dy * dy + dz * dz;
                     if (r2 < rc2) {
-                        pair_term<KIND, EVAL, MIXED>(P, dx, dy, dz, r2, fia, KIND == CM_KIND_EAM3 ? w.hf[j] : 0.0,
+                        pair_term<KIND, EVAL, MIXED>(P, T, dx, dy, dz, r2, fia, KIND == CM_KIND_EAM3 ? w.hf[j] : 0.0,
                                                      A);
                         npair++;
                     }

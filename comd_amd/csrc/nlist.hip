@@ -250,10 +250,34 @@ __global__ void k_nl_nonempty(DevState d, CmList l)
     l.nonempty[c] = n;
 }
 
+// Rows of the phi and rho coefficient tables a pass reads (phi: density
+// pass only), staged per block in LDS when they fit: every pair reads a
+// different interval, and from LDS these reads do not load the vector L1.
+__host__ __device__ inline int nl_rows(const PotDev &P, int kind) { return (kind == CM_KIND_EAM1 ? P.n[0] : 0) + P.n[1]; }
+
 template <int KIND, int EVAL, int MIXED>
-__global__ __launch_bounds__(256) void k_nl_pair(DevState d, PotDev P, CmList l)
+__global__ __launch_bounds__(512) void k_nl_pair(DevState d, PotDev P, CmList l, int tabs_in_lds)
 {
+    extern __shared__ __align__(16) char nls[];
     if (l.flags[0] & CM_NL_INVALID) return;
+    Tabs T = global_tabs(P);
+    if (EVAL == COMD_EVAL_TABLE && KIND != CM_KIND_LJ && tabs_in_lds) {
+        const int n0 = KIND == CM_KIND_EAM1 ? P.n[0] : 0, n1 = P.n[1];
+        if (MIXED) {
+            float *s0 = (float *)nls, *s1 = s0 + 8 * n0;
+            for (int q = threadIdx.x; q < 8 * n0; q += blockDim.x) s0[q] = P.cf32[0][q];
+            for (int q = threadIdx.x; q < 8 * n1; q += blockDim.x) s1[q] = P.cf32[1][q];
+            T.c32[0] = s0;
+            T.c32[1] = s1;
+        } else {
+            double *s0 = (double *)nls, *s1 = s0 + 6 * n0;
+            for (int q = threadIdx.x; q < 6 * n0; q += blockDim.x) s0[q] = P.cf64[0][q];
+            for (int q = threadIdx.x; q < 6 * n1; q += blockDim.x) s1[q] = P.cf64[1][q];
+            T.c64[0] = s0;
+            T.c64[1] = s1;
+        }
+        __syncthreads();
+    }
     // blocks are dealt round-robin over the 8 XCDs; give each XCD a
     // contiguous range of the build order (a slab of space) so that the
     // neighbours it gathers stay in its own L2
@@ -297,7 +321,7 @@ __global__ __launch_bounds__(256) void k_nl_pair(DevState d, PotDev P, CmList l)
             // a padding entry (k >= cnt) is the atom itself: r2 = 0
             const bool in = r2 > 0.0 && (KIND == CM_KIND_EAM3 ? (r2 < rc2) : (r2 <= rc2));
             if (in) {
-                pair_term<KIND, EVAL, MIXED>(P, dx, dy, dz, r2, fia, fj[u], A);
+                pair_term<KIND, EVAL, MIXED>(P, T, dx, dy, dz, r2, fia, fj[u], A);
                 npair++;
             }
         }
@@ -367,7 +391,13 @@ void cm_nl_build(const DevState &d, const CmList &l, hipStream_t st)
 template <int KIND, int EVAL, int MIXED>
 static void nl_launch(const DevState &d, const PotDev &P, const CmList &l, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_nl_pair<KIND, EVAL, MIXED>), dim3(grid_for(l.n, 256)), dim3(256), 0, st, d, P, l);
+    size_t lds = 0;
+    if (EVAL == COMD_EVAL_TABLE && KIND != CM_KIND_LJ) {
+        lds = (size_t)nl_rows(P, KIND) * (MIXED ? 32 : 48);
+        if (lds > 65536) lds = 0;  // tables too large: read them from global memory
+    }
+    hipLaunchKernelGGL((k_nl_pair<KIND, EVAL, MIXED>), dim3(grid_for(l.n, 512)), dim3(512), lds, st, d, P, l,
+                       lds > 0 ? 1 : 0);
 }
 
 template <int KIND, int EVAL>

@@ -74,13 +74,30 @@ __device__ __forceinline__ double rsqrt_mixed(double x)
 // ---------------------------------------------------------------------------
 // one accepted pair (home atom a, neighbour j): dx = r_a - r_j
 
+// Per-interval coefficient rows of the phi and rho tables, in global memory
+// or staged in LDS by the caller.
+struct Tabs {
+    const double *c64[2];
+    const float *c32[2];
+};
+
+__device__ __forceinline__ Tabs global_tabs(const PotDev &P)
+{
+    Tabs t;
+    t.c64[0] = P.cf64[0];
+    t.c64[1] = P.cf64[1];
+    t.c32[0] = P.cf32[0];
+    t.c32[1] = P.cf32[1];
+    return t;
+}
+
 struct Acc {
     double x, y, z, e, rho, vir;
 };
 
 template <int KIND, int EVAL, int MIXED>
-__device__ __forceinline__ void pair_term(const PotDev &P, double dx, double dy, double dz, double r2, double fia,
-                                          double fij, Acc &A)
+__device__ __forceinline__ void pair_term(const PotDev &P, const Tabs &T, double dx, double dy, double dz, double r2,
+                                          double fia, double fij, Acc &A)
 {
     if (KIND == CM_KIND_LJ) {
         // src-flat/ljForce.c:196-210 (f holds -grad U, energy without 4 eps s6)
@@ -126,19 +143,19 @@ __device__ __forceinline__ void pair_term(const PotDev &P, double dx, double dy,
             t1 = t0;
             if (!P.same_grid) i1 = table_interval(r, P.x0[1], P.xn[1], P.invdx[1], P.n[1], t1);
             if (MIXED) {
-                const float4 a = cf_row32(P.cf32[0], i0);
-                const float a2 = P.cf32[0][8 * i0 + 4];
-                const float4 b = cf_row32(P.cf32[1], i1);
-                const float b2 = P.cf32[1][8 * i1 + 4];
+                const float4 a = cf_row32(T.c32[0], i0);
+                const float a2 = T.c32[0][8 * i0 + 4];
+                const float4 b = cf_row32(T.c32[1], i1);
+                const float b2 = T.c32[1][8 * i1 + 4];
                 const float f0 = (float)t0, f1 = (float)t1;
                 phiv = fmaf(f0, fmaf(f0, a2, a.w), a.z);
                 dphi = fmaf(f0, a.y, a.x);
                 rhov = fmaf(f1, fmaf(f1, b2, b.w), b.z);
             } else {
-                const double2 dd = cf_der64(P.cf64[0], i0);
-                phiv = cf_val64(P.cf64[0], i0, t0);
+                const double2 dd = cf_der64(T.c64[0], i0);
+                phiv = cf_val64(T.c64[0], i0, t0);
                 dphi = fma(t0, dd.y, dd.x);
-                rhov = cf_val64(P.cf64[1], i1, t1);
+                rhov = cf_val64(T.c64[1], i1, t1);
             }
         }
         const double s = dphi * y;
@@ -159,10 +176,10 @@ __device__ __forceinline__ void pair_term(const PotDev &P, double dx, double dy,
             double t1;
             const int i1 = table_interval(r, P.x0[1], P.xn[1], P.invdx[1], P.n[1], t1);
             if (MIXED) {
-                const float2 dd = ((const float2 *)(P.cf32[1] + 8 * i1))[0];
+                const float2 dd = ((const float2 *)(T.c32[1] + 8 * i1))[0];
                 drho = fmaf((float)t1, dd.y, dd.x);
             } else {
-                const double2 dd = cf_der64(P.cf64[1], i1);
+                const double2 dd = cf_der64(T.c64[1], i1);
                 drho = fma(t1, dd.y, dd.x);
             }
         }
