@@ -34,7 +34,7 @@ ERRORS = {
 POT_EAM, POT_LJ = 1, 2
 EVAL_TABLE, EVAL_CHEB = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
-K_REBIN, K_DENSITY, K_EMBED_FORCE, K_REDUCE, K_NCLASS = 0, 1, 2, 3, 4
+K_REBIN, K_DENSITY, K_EMBED_FORCE, K_REDUCE, K_LIST_BUILD, K_CELL_FALLBACK, K_NCLASS = 0, 1, 2, 3, 4, 5, 6
 TUNE_WAVES, TUNE_BLOCKS_PER_CU, TUNE_TABLES_LDS, TUNE_BALANCED, TUNE_GRAPH, TUNE_PAIR_LISTS, TUNE_SKIN = 1, 2, 3, 4, 5, 6, 7
 
 # unit constants of src-flat/include/constants.h:61-65

@@ -160,6 +160,10 @@ static DevState dev_state(const comd_ctx *c)
     d.nl_ref = nl ? c->nl.ref : nullptr;
     d.nl_slot = nl ? c->nl.slot : nullptr;
     d.nl_flags = nl ? c->nl.flags : nullptr;
+    d.nl_pos32 = nl && c->precision == COMD_PREC_MIXED ? c->nl.pos32 : nullptr;
+    d.nl_qs0 = c->nl.qs0;
+    d.nl_qs1 = c->nl.qs1;
+    d.nl_qs2 = c->nl.qs2;
     const double hs = 0.5e-3 * c->nl_skin_milli;
     d.nl_half2 = hs * hs;
     d.cell_gate = nl ? c->nl.flags : nullptr;
@@ -262,26 +266,29 @@ static void op_force(comd_ctx *c)
     // with neighbour lists: the (usually skipped) list build, then each pass
     // over the lists; the link-cell kernels run only if the lists are invalid
     const bool nl = c->nl_active != 0;
+    if (nl) {
+        tbegin(c, COMD_K_LIST_BUILD, &ev);
+        cm_nl_build(d, c->nl, c->stream);
+        tend(c, COMD_K_LIST_BUILD, ev);
+    }
+    // one pass: the list kernel, then the link-cell kernel (which does the
+    // work when there are no lists, and returns at once while they are valid)
+    auto pass = [&](int kind, int cls) {
+        if (nl) {
+            tbegin(c, cls, &ev);
+            cm_nl_force(d, P, c->nl, kind, mixed, c->stream);
+            tend(c, cls, ev);
+        }
+        const int ccls = nl ? COMD_K_CELL_FALLBACK : cls;
+        tbegin(c, ccls, &ev);
+        cm_launch_force(d, P, kind, mixed, force_cfg(c, kind, mixed), c->stream);
+        tend(c, ccls, ev);
+    };
     if (P.kind == COMD_POT_LJ) {
-        tbegin(c, COMD_K_DENSITY, &ev);
-        if (nl) {
-            cm_nl_build(d, c->nl, c->stream);
-            cm_nl_force(d, P, c->nl, CM_KIND_LJ, mixed, c->stream);
-        }
-        cm_launch_force(d, P, CM_KIND_LJ, mixed, force_cfg(c, CM_KIND_LJ, mixed), c->stream);
-        tend(c, COMD_K_DENSITY, ev);
+        pass(CM_KIND_LJ, COMD_K_DENSITY);
     } else {
-        tbegin(c, COMD_K_DENSITY, &ev);
-        if (nl) {
-            cm_nl_build(d, c->nl, c->stream);
-            cm_nl_force(d, P, c->nl, CM_KIND_EAM1, mixed, c->stream);
-        }
-        cm_launch_force(d, P, CM_KIND_EAM1, mixed, force_cfg(c, CM_KIND_EAM1, mixed), c->stream);
-        tend(c, COMD_K_DENSITY, ev);
-        tbegin(c, COMD_K_EMBED_FORCE, &ev);
-        if (nl) cm_nl_force(d, P, c->nl, CM_KIND_EAM3, mixed, c->stream);
-        cm_launch_force(d, P, CM_KIND_EAM3, mixed, force_cfg(c, CM_KIND_EAM3, mixed), c->stream);
-        tend(c, COMD_K_EMBED_FORCE, ev);
+        pass(CM_KIND_EAM1, COMD_K_DENSITY);
+        pass(CM_KIND_EAM3, COMD_K_EMBED_FORCE);
     }
 }
 
@@ -362,7 +369,7 @@ int comd_grid_for(comd_params *p, const double bounds[3], double defgrad, double
 
 static void free_nl(comd_ctx *c)
 {
-    void *p[] = {c->nl.pos, c->nl.ref, c->nl.slot, c->nl.order, c->nl.nbl, c->nl.ncnt, c->nl.fi,
+    void *p[] = {c->nl.pos, c->nl.pos32, c->nl.ref, c->nl.slot, c->nl.order, c->nl.nbl, c->nl.ncnt,
                  c->nl.aux_cnt, c->nl.aux_start, c->nl.aux_mem, c->nl.nonempty, c->nl.count, c->nl.flags};
     for (void *q : p)
         if (q) hipFree(q);
@@ -720,12 +727,12 @@ static int nl_setup(comd_ctx *c, long nids)
         };
         const size_t n = (size_t)nids;
         alloc((void **)&c->nl.pos, n * sizeof(double4));
+        alloc((void **)&c->nl.pos32, n * sizeof(uint4));
         alloc((void **)&c->nl.ref, n * sizeof(double4));
         alloc((void **)&c->nl.slot, n * 4);
         alloc((void **)&c->nl.order, n * 4);
         alloc((void **)&c->nl.nbl, n * (size_t)kmax * 4);
         alloc((void **)&c->nl.ncnt, n * 4);
-        alloc((void **)&c->nl.fi, n * 8);
         alloc((void **)&c->nl.aux_cnt, (size_t)naux * 4);
         alloc((void **)&c->nl.aux_start, (size_t)naux * 4);
         alloc((void **)&c->nl.aux_mem, (size_t)naux * CM_NL_AUXCAP * 4);
@@ -754,6 +761,13 @@ static int nl_setup(comd_ctx *c, long nids)
     l.ab0 = p.bounds[0] / na[0];
     l.ab1 = p.bounds[1] / na[1];
     l.ab2 = p.bounds[2] / na[2];
+    const double two32 = 4294967296.0;
+    l.qs0 = two32 / l.L0;
+    l.qs1 = two32 / l.L1;
+    l.qs2 = two32 / l.L2;
+    l.qi0 = l.L0 / two32;
+    l.qi1 = l.L1 / two32;
+    l.qi2 = l.L2 / two32;
     HIPCHK(c, hipMemsetAsync(l.flags, 0, 16, c->stream));
     HIPCHK(c, hipMemsetAsync(l.count, 0, 16, c->stream));
     c->nl_active = 1;
@@ -1101,6 +1115,11 @@ int comd_set_precision(comd_ctx *c, int precision)
 {
     if (!c || (precision != COMD_PREC_FP64 && precision != COMD_PREC_MIXED)) return COMD_ERR_ARG;
     if (precision != c->precision) invalidate_graph(c);
+    // FP64 steps keep only the double list positions current
+    if (precision == COMD_PREC_MIXED && c->precision != COMD_PREC_MIXED && c->nl_active) {
+        cm_nl_refresh32(c->nl, c->stream);
+        HIPCHK(c, hipGetLastError());
+    }
     c->precision = precision;
     for (Replica &R : c->reps) R.prm.precision = precision;
     return COMD_OK;

@@ -32,9 +32,16 @@ struct PotDev {
 
 enum { CM_KIND_EAM1 = 0, CM_KIND_EAM3 = 1, CM_KIND_LJ = 2 };
 
-// Clenshaw sum with the lower clamp only, src-flat/cheby.c:276-310.
-__device__ __forceinline__ double cheb64(const double *c, int m, double a, double b, double x)
+// Address-space qualifiers: loads through CM_GLOBAL / CM_LDS pointers are
+// global / LDS instructions rather than flat ones.
+#define CM_GLOBAL __attribute__((address_space(1)))
+#define CM_LDS __attribute__((address_space(3)))
+
+// Clenshaw sum with the lower clamp only, src-flat/cheby.c:276-310.  The
+// coefficients are in global memory.
+__device__ __forceinline__ double cheb64(const double *cg, int m, double a, double b, double x)
 {
+    const CM_GLOBAL double *c = (const CM_GLOBAL double *)cg;
     if (x < a) x = a;
     double d = 0.0, dd = 0.0;
     const double y = (2.0 * x - a - b) / (b - a);
@@ -47,8 +54,9 @@ __device__ __forceinline__ double cheb64(const double *c, int m, double a, doubl
     return y * d - dd + 0.5 * c[0];
 }
 
-__device__ __forceinline__ float cheb32(const float *c, int m, float a, float b, float x)
+__device__ __forceinline__ float cheb32(const float *cg, int m, float a, float b, float x)
 {
+    const CM_GLOBAL float *c = (const CM_GLOBAL float *)cg;
     x = fmaxf(x, a);
     float d = 0.0f, dd = 0.0f;
     const float y = (2.0f * x - a - b) / (b - a);
@@ -114,12 +122,15 @@ struct CmList {
     double L0, L1, L2;       // bounds
     int na0, na1, na2, naux; // auxiliary grid used to build the lists
     double ab0, ab1, ab2;    // its cell edges (>= cutoff + skin)
-    double4 *pos, *ref;      // per id
+    double4 *pos, *ref;      // per id (pos.w: F' of the last density pass)
+    uint4 *pos32;            // per id: pos in fixed32 (x, y, z) and, in .w, the float bits of
+                             // F' of the last mixed-precision density pass
+    double qs0, qs1, qs2;    // 2^32 / L (encode)
+    double qi0, qi1, qi2;    // L / 2^32 (decode)
     int *slot;               // per id: current slot, -1 when absent
     int *order;              // atoms in build order
     int *nbl;                // [k * n + t]: neighbour ids of the t-th atom in build order
     int *ncnt;               // per t
-    double *fi;              // per id: F' of the last density pass
     int *aux_cnt, *aux_start, *aux_mem;
     int *nonempty;           // per link cell: non-empty stencil cells (Lennard-Jones virial)
     int *count;              // [0]: atoms in build order, [1]: builds so far
@@ -128,4 +139,5 @@ struct CmList {
 #define CM_NL_AUXCAP 64
 void cm_nl_init(const DevState &d, const CmList &l, hipStream_t st);
 void cm_nl_build(const DevState &d, const CmList &l, hipStream_t st);
+void cm_nl_refresh32(const CmList &l, hipStream_t st);
 void cm_nl_force(const DevState &d, const PotDev &P, const CmList &l, int kind, int mixed, hipStream_t st);

@@ -87,7 +87,7 @@ __device__ __forceinline__ void pair_cell(const DevState &d, const Grid &g, cons
                                           int L)
 {
     const double rc2 = P.rc2;
-    const Tabs T = global_tabs(P);
+    const TabsGlobal T = global_tabs(P);
     const int nh = d.natoms[c];
     if (nh == 0) return;
 

@@ -51,7 +51,19 @@ __global__ __launch_bounds__(64) void k_nl_from_slots(DevState d, CmList l)
     double cx, cy, cz;
     cell_corner(d.g0, c, cx, cy, cz);
     l.slot[id] = s;
-    l.pos[id] = make_double4(cx + d.rx[s], cy + d.ry[s], cz + d.rz[s], 0.0);
+    const double x = cx + d.rx[s], y = cy + d.ry[s], z = cz + d.rz[s];
+    l.pos[id] = make_double4(x, y, z, 0.0);
+    l.pos32[id] = make_uint4(fixed32(x, l.qs0), fixed32(y, l.qs1), fixed32(z, l.qs2), 0u);
+}
+
+// fixed32 positions from the double ones (before the first mixed-precision
+// pass after FP64 steps, which keep only the double positions current)
+__global__ void k_nl_refresh32(CmList l)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= l.n || l.slot[i] < 0) return;
+    const double4 p = l.pos[i];
+    l.pos32[i] = make_uint4(fixed32(p.x, l.qs0), fixed32(p.y, l.qs1), fixed32(p.z, l.qs2), 0u);
 }
 
 // ---------------------------------------------------------------------------
@@ -255,41 +267,26 @@ __global__ void k_nl_nonempty(DevState d, CmList l)
 // different interval, and from LDS these reads do not load the vector L1.
 __host__ __device__ inline int nl_rows(const PotDev &P, int kind) { return (kind == CM_KIND_EAM1 ? P.n[0] : 0) + P.n[1]; }
 
-template <int KIND, int EVAL, int MIXED>
-__global__ __launch_bounds__(512) void k_nl_pair(DevState d, PotDev P, CmList l, int tabs_in_lds)
+// One atom (thread t of the build order) over its list.  TB: the table
+// pointers' type (global or LDS rows).
+template <int KIND, int EVAL, int MIXED, class TB>
+__device__ __forceinline__ void nl_atom(const DevState &d, const PotDev &P, const CmList &l, const TB &T, int t)
 {
-    extern __shared__ __align__(16) char nls[];
-    if (l.flags[0] & CM_NL_INVALID) return;
-    Tabs T = global_tabs(P);
-    if (EVAL == COMD_EVAL_TABLE && KIND != CM_KIND_LJ && tabs_in_lds) {
-        const int n0 = KIND == CM_KIND_EAM1 ? P.n[0] : 0, n1 = P.n[1];
-        if (MIXED) {
-            float *s0 = (float *)nls, *s1 = s0 + 8 * n0;
-            for (int q = threadIdx.x; q < 8 * n0; q += blockDim.x) s0[q] = P.cf32[0][q];
-            for (int q = threadIdx.x; q < 8 * n1; q += blockDim.x) s1[q] = P.cf32[1][q];
-            T.c32[0] = s0;
-            T.c32[1] = s1;
-        } else {
-            double *s0 = (double *)nls, *s1 = s0 + 6 * n0;
-            for (int q = threadIdx.x; q < 6 * n0; q += blockDim.x) s0[q] = P.cf64[0][q];
-            for (int q = threadIdx.x; q < 6 * n1; q += blockDim.x) s1[q] = P.cf64[1][q];
-            T.c64[0] = s0;
-            T.c64[1] = s1;
-        }
-        __syncthreads();
-    }
-    // blocks are dealt round-robin over the 8 XCDs; give each XCD a
-    // contiguous range of the build order (a slab of space) so that the
-    // neighbours it gathers stay in its own L2
-    const int nb = gridDim.x, q = nb / 8, rem = nb % 8;
-    const int xcd = blockIdx.x % 8, idx = blockIdx.x / 8;
-    const int blk = nb < 8 ? (int)blockIdx.x : xcd * q + min(xcd, rem) + idx;
-    const int t = blk * blockDim.x + threadIdx.x;
-    if (t >= l.count[0]) return;
     const int i = l.order[t];
     const int s = l.slot[i];
-    const double4 pi = l.pos[i];
-    const double fia = KIND == CM_KIND_EAM3 ? l.fi[i] : 0.0;
+    // clang vector types: loadable through address-space-qualified pointers
+    typedef double v4d __attribute__((ext_vector_type(4)));
+    typedef unsigned v4u __attribute__((ext_vector_type(4)));
+    const CM_GLOBAL v4d *pos = (const CM_GLOBAL v4d *)l.pos;
+    const CM_GLOBAL v4u *pos32 = (const CM_GLOBAL v4u *)l.pos32;
+    // positions: FP64 passes gather double4 (x, y, z, F'), nearest image by
+    // rint; mixed passes gather 16-byte fixed32 (x, y, z, float F'), nearest
+    // image by 32-bit wrap-around.  F' is there once the density pass ran.
+    v4d pi = {0, 0, 0, 0};
+    v4u qi = {0, 0, 0, 0};
+    if (MIXED) qi = pos32[i];
+    else pi = pos[i];
+    const double fia = KIND != CM_KIND_EAM3 ? 0.0 : MIXED ? (double)__uint_as_float(qi.w) : pi.w;
     const double rc2 = P.rc2;
     const double L0 = l.L0, L1 = l.L1, L2 = l.L2;
     const double iL0 = 1.0 / L0, iL1 = 1.0 / L1, iL2 = 1.0 / L2;
@@ -299,29 +296,40 @@ __global__ __launch_bounds__(512) void k_nl_pair(DevState d, PotDev P, CmList l,
     // NU neighbours per trip: their list entries, then their positions (and
     // F'), are loaded together so that the loads' latencies overlap
     constexpr int NU = 4;
-    const int *row = l.nbl + t;
+    const CM_GLOBAL int *row = (const CM_GLOBAL int *)l.nbl + t;
     for (int k0 = 0; k0 < cnt; k0 += NU) {
         int jj[NU];
 #pragma unroll
         for (int u = 0; u < NU; u++) jj[u] = k0 + u < cnt ? row[(size_t)(k0 + u) * l.n] : i;
-        double4 pj[NU];
-        double fj[NU];
+        v4d pj[NU];
+        v4u qj[NU];
 #pragma unroll
         for (int u = 0; u < NU; u++) {
-            pj[u] = l.pos[jj[u]];
-            fj[u] = KIND == CM_KIND_EAM3 ? l.fi[jj[u]] : 0.0;
+            if (MIXED) qj[u] = pos32[jj[u]];
+            else pj[u] = pos[jj[u]];
         }
 #pragma unroll
         for (int u = 0; u < NU; u++) {
-            double dx = pi.x - pj[u].x, dy = pi.y - pj[u].y, dz = pi.z - pj[u].z;
-            dx -= L0 * rint(dx * iL0);
-            dy -= L1 * rint(dy * iL1);
-            dz -= L2 * rint(dz * iL2);
+            double dx, dy, dz, fij;
+            if (MIXED) {
+                dx = (double)(int)(qi.x - qj[u].x) * l.qi0;
+                dy = (double)(int)(qi.y - qj[u].y) * l.qi1;
+                dz = (double)(int)(qi.z - qj[u].z) * l.qi2;
+                fij = (double)__uint_as_float(qj[u].w);
+            } else {
+                dx = pi.x - pj[u].x;
+                dy = pi.y - pj[u].y;
+                dz = pi.z - pj[u].z;
+                dx -= L0 * rint(dx * iL0);
+                dy -= L1 * rint(dy * iL1);
+                dz -= L2 * rint(dz * iL2);
+                fij = pj[u].w;
+            }
             const double r2 = dx * dx + dy * dy + dz * dz;
             // a padding entry (k >= cnt) is the atom itself: r2 = 0
             const bool in = r2 > 0.0 && (KIND == CM_KIND_EAM3 ? (r2 < rc2) : (r2 <= rc2));
             if (in) {
-                pair_term<KIND, EVAL, MIXED>(P, T, dx, dy, dz, r2, fia, fj[u], A);
+                pair_term<KIND, EVAL, MIXED>(P, T, dx, dy, dz, r2, fia, KIND == CM_KIND_EAM3 ? fij : 0.0, A);
                 npair++;
             }
         }
@@ -342,7 +350,8 @@ __global__ __launch_bounds__(512) void k_nl_pair(DevState d, PotDev P, CmList l,
         d.rho[s] = A.rho;
         d.fi[s] = fp;
         d.nnb[s] = npair;
-        l.fi[i] = fp;
+        if (MIXED) ((unsigned *)&l.pos32[i])[3] = __float_as_uint((float)fp);
+        else ((double *)&l.pos[i])[3] = fp;
     } else if (KIND == CM_KIND_EAM3) {
         d.fx[s] += A.x;
         d.fy[s] += A.y;
@@ -359,6 +368,46 @@ __global__ __launch_bounds__(512) void k_nl_pair(DevState d, PotDev P, CmList l,
     }
 }
 
+// TLDS: the table rows are staged in LDS (1) or read from global memory (0);
+// a compile-time choice, so the row reads are LDS or global loads.
+template <int KIND, int EVAL, int MIXED, int TLDS>
+__global__ __launch_bounds__(512) void k_nl_pair(DevState d, PotDev P, CmList l)
+{
+    extern __shared__ __align__(16) char nls[];
+    if (l.flags[0] & CM_NL_INVALID) return;
+    // blocks are dealt round-robin over the 8 XCDs; give each XCD a
+    // contiguous range of the build order (a slab of space) so that the
+    // neighbours it gathers stay in its own L2
+    const int nb = gridDim.x, q = nb / 8, rem = nb % 8;
+    const int xcd = blockIdx.x % 8, idx = blockIdx.x / 8;
+    const int blk = nb < 8 ? (int)blockIdx.x : xcd * q + min(xcd, rem) + idx;
+    const int t = blk * blockDim.x + threadIdx.x;
+    const bool live = t < l.count[0];
+    if (TLDS) {
+        const int n0 = KIND == CM_KIND_EAM1 ? P.n[0] : 0, n1 = P.n[1];
+        TabsLds T;
+        if (MIXED) {
+            CM_LDS float *s0 = (CM_LDS float *)nls, *s1 = s0 + 8 * n0;
+            for (int k = threadIdx.x; k < 8 * n0; k += blockDim.x) s0[k] = P.cf32[0][k];
+            for (int k = threadIdx.x; k < 8 * n1; k += blockDim.x) s1[k] = P.cf32[1][k];
+            T.c32[0] = s0;
+            T.c32[1] = s1;
+            T.c64[0] = T.c64[1] = nullptr;
+        } else {
+            CM_LDS double *s0 = (CM_LDS double *)nls, *s1 = s0 + 6 * n0;
+            for (int k = threadIdx.x; k < 6 * n0; k += blockDim.x) s0[k] = P.cf64[0][k];
+            for (int k = threadIdx.x; k < 6 * n1; k += blockDim.x) s1[k] = P.cf64[1][k];
+            T.c64[0] = s0;
+            T.c64[1] = s1;
+            T.c32[0] = T.c32[1] = nullptr;
+        }
+        __syncthreads();
+        if (live) nl_atom<KIND, EVAL, MIXED>(d, P, l, T, t);
+    } else {
+        if (live) nl_atom<KIND, EVAL, MIXED>(d, P, l, global_tabs(P), t);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // host launchers
 
@@ -372,6 +421,11 @@ void cm_nl_init(const DevState &d, const CmList &l, hipStream_t st)
 {
     hipLaunchKernelGGL(k_nl_clear_ids, dim3(grid_for(l.n, 256)), dim3(256), 0, st, l);
     hipLaunchKernelGGL(k_nl_from_slots, dim3(d.ncells), dim3(64), 0, st, d, l);
+}
+
+void cm_nl_refresh32(const CmList &l, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_nl_refresh32, dim3(grid_for(l.n, 256)), dim3(256), 0, st, l);
 }
 
 void cm_nl_build(const DevState &d, const CmList &l, hipStream_t st)
@@ -396,8 +450,11 @@ static void nl_launch(const DevState &d, const PotDev &P, const CmList &l, hipSt
         lds = (size_t)nl_rows(P, KIND) * (MIXED ? 32 : 48);
         if (lds > 65536) lds = 0;  // tables too large: read them from global memory
     }
-    hipLaunchKernelGGL((k_nl_pair<KIND, EVAL, MIXED>), dim3(grid_for(l.n, 512)), dim3(512), lds, st, d, P, l,
-                       lds > 0 ? 1 : 0);
+    const dim3 grid(grid_for(l.n, 512)), block(512);
+    if (EVAL == COMD_EVAL_TABLE && KIND != CM_KIND_LJ && lds > 0)
+        hipLaunchKernelGGL((k_nl_pair<KIND, EVAL, MIXED, 1>), grid, block, lds, st, d, P, l);
+    else
+        hipLaunchKernelGGL((k_nl_pair<KIND, EVAL, MIXED, 0>), grid, block, 0, st, d, P, l);
 }
 
 template <int KIND, int EVAL>

@@ -12,9 +12,10 @@
 // 3-point quadratic value and interpolated central-difference derivative,
 // the formula of src-flat/eam.c:401-434 in double precision (used once per
 // atom for F).  v points at values[-1].
-__device__ __forceinline__ void interp64(const double *v, double x0, double xn, double invdx, int n, double r,
+__device__ __forceinline__ void interp64(const double *vg, double x0, double xn, double invdx, int n, double r,
                                          double &val, double &der)
 {
+    const CM_GLOBAL double *v = (const CM_GLOBAL double *)vg;
     r = fmin(fmax(r, x0), xn);
     r = (r - x0) * invdx;
     const double fl = floor(r);
@@ -23,7 +24,7 @@ __device__ __forceinline__ void interp64(const double *v, double x0, double xn, 
     // r <= xn keeps i <= n-1 for a consistent table header; the clamp only
     // keeps a rounding or a bad header inside values[-1..n+1]
     i = min(max(i, 0), n - 1);
-    const double *w = v + 1 + i;
+    const CM_GLOBAL double *w = v + 1 + i;
     const double vm = w[-1], v0 = w[0], v1 = w[1], v2 = w[2];
     const double g0 = v1 - vm;
     const double g1 = v2 - v0;
@@ -43,15 +44,23 @@ __device__ __forceinline__ int table_interval(double r, double x0, double xn, do
 }
 
 // Table formula from per-interval coefficients (d0 d1 c0 c1 c2 0):
-// value c0 + t(c1 + t c2), derivative d0 + t d1.
-__device__ __forceinline__ double2 cf_der64(const double *cf, int i) { return ((const double2 *)(cf + 6 * i))[0]; }
-__device__ __forceinline__ double cf_val64(const double *cf, int i, double t)
+// value c0 + t(c1 + t c2), derivative d0 + t d1.  PD / PF are pointers to
+// double / float in any address space (generic, global or LDS), so that a
+// caller that knows where the rows live gets global or LDS loads rather
+// than flat ones (flat loads of LDS data still occupy the texture
+// addresser).
+template <class PD> __device__ __forceinline__ double2 cf_der64(PD cf, int i)
 {
-    const double2 a = ((const double2 *)(cf + 6 * i))[1];
-    const double c2 = cf[6 * i + 4];
-    return fma(t, fma(t, c2, a.y), a.x);
+    return make_double2(cf[6 * i], cf[6 * i + 1]);
 }
-__device__ __forceinline__ float4 cf_row32(const float *cf, int i) { return ((const float4 *)(cf + 8 * i))[0]; }
+template <class PD> __device__ __forceinline__ double cf_val64(PD cf, int i, double t)
+{
+    return fma(t, fma(t, cf[6 * i + 4], cf[6 * i + 3]), cf[6 * i + 2]);
+}
+template <class PF> __device__ __forceinline__ float4 cf_row32(PF cf, int i)
+{
+    return make_float4(cf[8 * i], cf[8 * i + 1], cf[8 * i + 2], cf[8 * i + 3]);
+}
 
 // 1/sqrt(x) in double precision: hardware estimate and two Newton steps.
 __device__ __forceinline__ double rsqrt64(double x)
@@ -75,19 +84,22 @@ __device__ __forceinline__ double rsqrt_mixed(double x)
 // one accepted pair (home atom a, neighbour j): dx = r_a - r_j
 
 // Per-interval coefficient rows of the phi and rho tables, in global memory
-// or staged in LDS by the caller.
-struct Tabs {
-    const double *c64[2];
-    const float *c32[2];
+// or staged in LDS by the caller, behind pointers of a given address space.
+template <class PD, class PF> struct TabsT {
+    PD c64[2];
+    PF c32[2];
 };
+using Tabs = TabsT<const double *, const float *>;
+using TabsGlobal = TabsT<const CM_GLOBAL double *, const CM_GLOBAL float *>;
+using TabsLds = TabsT<const CM_LDS double *, const CM_LDS float *>;
 
-__device__ __forceinline__ Tabs global_tabs(const PotDev &P)
+__device__ __forceinline__ TabsGlobal global_tabs(const PotDev &P)
 {
-    Tabs t;
-    t.c64[0] = P.cf64[0];
-    t.c64[1] = P.cf64[1];
-    t.c32[0] = P.cf32[0];
-    t.c32[1] = P.cf32[1];
+    TabsGlobal t;
+    t.c64[0] = (const CM_GLOBAL double *)P.cf64[0];
+    t.c64[1] = (const CM_GLOBAL double *)P.cf64[1];
+    t.c32[0] = (const CM_GLOBAL float *)P.cf32[0];
+    t.c32[1] = (const CM_GLOBAL float *)P.cf32[1];
     return t;
 }
 
@@ -95,8 +107,8 @@ struct Acc {
     double x, y, z, e, rho, vir;
 };
 
-template <int KIND, int EVAL, int MIXED>
-__device__ __forceinline__ void pair_term(const PotDev &P, const Tabs &T, double dx, double dy, double dz, double r2,
+template <int KIND, int EVAL, int MIXED, class TB>
+__device__ __forceinline__ void pair_term(const PotDev &P, const TB &T, double dx, double dy, double dz, double r2,
                                           double fia, double fij, Acc &A)
 {
     if (KIND == CM_KIND_LJ) {
@@ -176,8 +188,7 @@ __device__ __forceinline__ void pair_term(const PotDev &P, const Tabs &T, double
             double t1;
             const int i1 = table_interval(r, P.x0[1], P.xn[1], P.invdx[1], P.n[1], t1);
             if (MIXED) {
-                const float2 dd = ((const float2 *)(T.c32[1] + 8 * i1))[0];
-                drho = fmaf((float)t1, dd.y, dd.x);
+                drho = fmaf((float)t1, T.c32[1][8 * i1 + 1], T.c32[1][8 * i1]);
             } else {
                 const double2 dd = cf_der64(T.c64[1], i1);
                 drho = fma(t1, dd.y, dd.x);

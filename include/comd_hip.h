@@ -205,7 +205,11 @@ enum comd_kclass {
     COMD_K_DENSITY = 1, /* EAM pass 1 (+ embedding) or LJ */
     COMD_K_EMBED_FORCE = 2, /* EAM pass 3 */
     COMD_K_REDUCE = 3,
-    COMD_K_NCLASS = 4
+    COMD_K_LIST_BUILD = 4, /* neighbour-list rebuild check (+ rebuild) */
+    COMD_K_CELL_FALLBACK = 5, /* link-cell passes launched beside the list
+                                 passes; they return at once while the lists
+                                 are valid */
+    COMD_K_NCLASS = 6
 };
 int comd_timing(comd_ctx *ctx, int on);
 int comd_timing_read(comd_ctx *ctx, double *ms /* COMD_K_NCLASS */, int *launches /* COMD_K_NCLASS */);
