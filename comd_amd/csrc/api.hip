@@ -88,6 +88,7 @@ struct comd_ctx {
     CmList nl;
     int nl_skin_milli;  // skin in 1e-3 A, 0 = link-cell kernels only
     int nl_active;      // lists allocated and usable for the atoms held
+    int nl_stale;       // list positions not updated by the last drift
     long nl_ids;        // id capacity of the atoms held
     long nl_alloc_n, nl_alloc_k, nl_alloc_aux;
     // one DKD step captured as a HIP graph (comd_steps)
@@ -214,13 +215,21 @@ static void op_rebox(comd_ctx *c)
 }
 
 // rebin: record movers for the op_rebox that follows (0 for a lone kick)
-static void op_move(comd_ctx *c, double dtk, double dtd, int kick, int drift, int rebin = 1)
+// nl_update = 0: the drift leaves the list positions stale (the step's
+// second half drift: the next step's first one renews them, and a force
+// call in between renews them first)
+static void op_move(comd_ctx *c, double dtk, double dtd, int kick, int drift, int rebin = 1, int nl_update = 1)
 {
     hipEvent_t ev = nullptr;
     tbegin(c, COMD_K_REBIN, &ev);
     DevState d = dev_state(c);
+    if (!nl_update) {
+        d.nl_pos = nullptr;
+        d.nl_pos32 = nullptr;
+    }
     cm_launch_move(d, c->stream, dtk, dtd, kick, drift, rebin);
     tend(c, COMD_K_REBIN, ev);
+    if (drift && c->nl_active) c->nl_stale = nl_update ? 0 : 1;
 }
 
 static ForceCfg force_cfg(const comd_ctx *c, int kind, int mixed)
@@ -268,6 +277,10 @@ static void op_force(comd_ctx *c)
     const bool nl = c->nl_active != 0;
     if (nl) {
         tbegin(c, COMD_K_LIST_BUILD, &ev);
+        if (c->nl_stale) {
+            cm_nl_sync(d, c->nl, c->stream);
+            c->nl_stale = 0;
+        }
         cm_nl_build(d, c->nl, c->stream);
         tend(c, COMD_K_LIST_BUILD, ev);
     }
@@ -299,7 +312,7 @@ static void op_step(comd_ctx *c, double dt)
     op_move(c, 0.0, dt / 2.0, 0, 1);
     op_rebox(c);
     op_force(c);
-    op_move(c, dt, dt / 2.0, 1, 1);
+    op_move(c, dt, dt / 2.0, 1, 1, 1, 0);
     op_rebox(c);
 }
 
@@ -770,7 +783,9 @@ static int nl_setup(comd_ctx *c, long nids)
     l.qi2 = l.L2 / two32;
     HIPCHK(c, hipMemsetAsync(l.flags, 0, 16, c->stream));
     HIPCHK(c, hipMemsetAsync(l.count, 0, 16, c->stream));
+    HIPCHK(c, hipMemsetAsync(l.aux_cnt, 0, (size_t)naux * 4, c->stream));
     c->nl_active = 1;
+    c->nl_stale = 0;
     cm_nl_init(dev_state(c), l, c->stream);
     HIPCHK(c, hipGetLastError());
     return COMD_OK;
@@ -1116,10 +1131,7 @@ int comd_set_precision(comd_ctx *c, int precision)
     if (!c || (precision != COMD_PREC_FP64 && precision != COMD_PREC_MIXED)) return COMD_ERR_ARG;
     if (precision != c->precision) invalidate_graph(c);
     // FP64 steps keep only the double list positions current
-    if (precision == COMD_PREC_MIXED && c->precision != COMD_PREC_MIXED && c->nl_active) {
-        cm_nl_refresh32(c->nl, c->stream);
-        HIPCHK(c, hipGetLastError());
-    }
+    if (precision == COMD_PREC_MIXED && c->precision != COMD_PREC_MIXED && c->nl_active) c->nl_stale = 1;
     c->precision = precision;
     for (Replica &R : c->reps) R.prm.precision = precision;
     return COMD_OK;

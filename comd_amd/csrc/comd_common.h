@@ -84,7 +84,7 @@ struct DevState {
     const unsigned *cell_gate;
 };
 
-enum { CM_NL_REBUILD = 1, CM_NL_INVALID = 2 };
+enum { CM_NL_REBUILD = 1, CM_NL_INVALID = 2, CM_NL_OVERFLOW = 4 };
 
 // Grid of the replica that owns global cell c.
 __device__ __forceinline__ Grid grid_of(const DevState &d, int c)

@@ -133,11 +133,11 @@ struct CmList {
     int *ncnt;               // per t
     int *aux_cnt, *aux_start, *aux_mem;
     int *nonempty;           // per link cell: non-empty stencil cells (Lennard-Jones virial)
-    int *count;              // [0]: atoms in build order, [1]: builds so far
-    unsigned *flags;         // CM_NL_REBUILD, CM_NL_INVALID
+    int *count;              // [0]: atoms in build order, [1]: builds so far, [2]: build blocks done
+    unsigned *flags;         // CM_NL_REBUILD, CM_NL_INVALID, CM_NL_OVERFLOW (during a build)
 };
 #define CM_NL_AUXCAP 64
 void cm_nl_init(const DevState &d, const CmList &l, hipStream_t st);
 void cm_nl_build(const DevState &d, const CmList &l, hipStream_t st);
-void cm_nl_refresh32(const CmList &l, hipStream_t st);
+void cm_nl_sync(const DevState &d, const CmList &l, hipStream_t st);
 void cm_nl_force(const DevState &d, const PotDev &P, const CmList &l, int kind, int mixed, hipStream_t st);

@@ -23,8 +23,6 @@
 #include "comd_internal.h"
 #include "pair_terms.h"
 
-#define NL_BATCH 448  // halo atoms staged per batch in the build
-
 __device__ __forceinline__ bool nl_rebuild(const CmList &l) { return (l.flags[0] & CM_NL_REBUILD) != 0; }
 
 // ---------------------------------------------------------------------------
@@ -37,130 +35,182 @@ __global__ void k_nl_clear_ids(CmList l)
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(l.flags, (unsigned)CM_NL_REBUILD);
 }
 
-__global__ __launch_bounds__(64) void k_nl_from_slots(DevState d, CmList l)
+// check = 0: after binning or an upload (the ids are checked);
+// check = 1: positions after steps that did not keep them, with the
+// displacement test of the drift kernel
+__global__ __launch_bounds__(64) void k_nl_from_slots(DevState d, CmList l, int check)
 {
     const int c = blockIdx.x;
     const int t = threadIdx.x;
-    if (t >= d.natoms[c]) return;
-    const int s = c * CM_SLOTS + t;
-    const int id = d.id[s];
-    if (id < 0 || id >= l.n) {
-        atomicOr(l.flags, (unsigned)CM_NL_INVALID);
-        return;
+    bool far = false;
+    if (t < d.natoms[c]) {
+        const int s = c * CM_SLOTS + t;
+        const int id = d.id[s];
+        if (id < 0 || id >= l.n) {
+            atomicOr(l.flags, (unsigned)CM_NL_INVALID);
+        } else {
+            double cx, cy, cz;
+            cell_corner(d.g0, c, cx, cy, cz);
+            const double x = cx + d.rx[s], y = cy + d.ry[s], z = cz + d.rz[s];
+            l.slot[id] = s;
+            l.pos[id] = make_double4(x, y, z, 0.0);
+            l.pos32[id] = make_uint4(fixed32(x, l.qs0), fixed32(y, l.qs1), fixed32(z, l.qs2), 0u);
+            if (check) {
+                const double4 q = l.ref[id];
+                double ex = x - q.x, ey = y - q.y, ez = z - q.z;
+                ex -= l.L0 * rint(ex / l.L0);
+                ey -= l.L1 * rint(ey / l.L1);
+                ez -= l.L2 * rint(ez / l.L2);
+                far = ex * ex + ey * ey + ez * ez > d.nl_half2;
+            }
+        }
     }
-    double cx, cy, cz;
-    cell_corner(d.g0, c, cx, cy, cz);
-    l.slot[id] = s;
-    const double x = cx + d.rx[s], y = cy + d.ry[s], z = cz + d.rz[s];
-    l.pos[id] = make_double4(x, y, z, 0.0);
-    l.pos32[id] = make_uint4(fixed32(x, l.qs0), fixed32(y, l.qs1), fixed32(z, l.qs2), 0u);
-}
-
-// fixed32 positions from the double ones (before the first mixed-precision
-// pass after FP64 steps, which keep only the double positions current)
-__global__ void k_nl_refresh32(CmList l)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= l.n || l.slot[i] < 0) return;
-    const double4 p = l.pos[i];
-    l.pos32[i] = make_uint4(fixed32(p.x, l.qs0), fixed32(p.y, l.qs1), fixed32(p.z, l.qs2), 0u);
+    if (__ballot(far) != 0 && t == 0) atomicOr(l.flags, (unsigned)CM_NL_REBUILD);
 }
 
 // ---------------------------------------------------------------------------
-// build (each kernel returns at once unless a rebuild was requested)
-
-__global__ void k_nl_zero(CmList l)
-{
-    if (!nl_rebuild(l)) return;
-    for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < l.naux; a += gridDim.x * blockDim.x) l.aux_cnt[a] = 0;
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAnd(l.flags, ~(unsigned)CM_NL_INVALID);
-}
+// build (each kernel returns at once unless a rebuild was requested, so the
+// launches that find no request cost only their dispatch: their grids are
+// kept small, with strided loops)
 
 __device__ __forceinline__ double wrap_coord(double x, double L) { return x - L * floor(x / L); }
 
-__global__ void k_nl_key(CmList l)
+__device__ __forceinline__ void nl_wave_sync()
 {
-    if (!nl_rebuild(l)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= l.n || l.slot[i] < 0) return;
-    const double4 p = l.pos[i];
-    int ax = (int)(wrap_coord(p.x, l.L0) / l.ab0);
-    int ay = (int)(wrap_coord(p.y, l.L1) / l.ab1);
-    int az = (int)(wrap_coord(p.z, l.L2) / l.ab2);
-    ax = min(max(ax, 0), l.na0 - 1);
-    ay = min(max(ay, 0), l.na1 - 1);
-    az = min(max(az, 0), l.na2 - 1);
-    const int a = ax + l.na0 * (ay + l.na1 * az);
-    const int k = atomicAdd(&l.aux_cnt[a], 1);
-    if (k < CM_NL_AUXCAP) l.aux_mem[a * CM_NL_AUXCAP + k] = i;
-    else atomicOr(l.flags, (unsigned)CM_NL_INVALID);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Exclusive prefix sum of the auxiliary cells' counts, one block, in a
-// fixed order; count[0] = atoms listed.
+// Each atom into its auxiliary cell (counts start at zero: the previous
+// build's last block cleared them).
+__global__ __launch_bounds__(256) void k_nl_key(CmList l)
+{
+    if (!nl_rebuild(l)) return;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < l.n; i += gridDim.x * blockDim.x) {
+        if (l.slot[i] < 0) continue;
+        const double4 p = l.pos[i];
+        int ax = (int)(wrap_coord(p.x, l.L0) / l.ab0);
+        int ay = (int)(wrap_coord(p.y, l.L1) / l.ab1);
+        int az = (int)(wrap_coord(p.z, l.L2) / l.ab2);
+        ax = min(max(ax, 0), l.na0 - 1);
+        ay = min(max(ay, 0), l.na1 - 1);
+        az = min(max(az, 0), l.na2 - 1);
+        const int a = ax + l.na0 * (ay + l.na1 * az);
+        const int k = atomicAdd(&l.aux_cnt[a], 1);
+        if (k < CM_NL_AUXCAP) l.aux_mem[a * CM_NL_AUXCAP + k] = i;
+        else atomicOr(l.flags, (unsigned)CM_NL_OVERFLOW);
+    }
+}
+
+// Exclusive prefix sum of the auxiliary cells' counts, one block walking
+// tiles of 8 x 1024 cells; count[0] = atoms listed.
 __global__ __launch_bounds__(1024) void k_nl_scan(CmList l)
 {
     if (!nl_rebuild(l)) return;
-    __shared__ int part[1024];
-    const int t = threadIdx.x;
-    const int per = (l.naux + 1023) / 1024;
-    const int a0 = t * per, a1 = min(a0 + per, l.naux);
-    int s = 0;
-    for (int a = a0; a < a1; a++) s += min(l.aux_cnt[a], CM_NL_AUXCAP);
-    part[t] = s;
-    __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {
-        const int v = t >= o ? part[t - o] : 0;
+    __shared__ int wsum[16];
+    __shared__ int carry_s;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    int carry = 0;
+    for (int base = 0; base < l.naux; base += 8 * 1024) {
+        int v[8], s = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int a = base + 8 * t + k;
+            v[k] = a < l.naux ? min(l.aux_cnt[a], CM_NL_AUXCAP) : 0;
+            s += v[k];
+        }
+        // inclusive scan of the threads' sums: within the wave, then across
+        int incl = s;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int u = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += u;
+        }
+        if (lane == 63) wsum[w] = incl;
         __syncthreads();
-        part[t] += v;
+        if (t == 0) {
+            int run = 0;
+            for (int k = 0; k < 16; k++) {
+                const int x = wsum[k];
+                wsum[k] = run;
+                run += x;
+            }
+            carry_s = run;
+        }
+        __syncthreads();
+        int run = carry + wsum[w] + incl - s;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int a = base + 8 * t + k;
+            if (a < l.naux) l.aux_start[a] = run;
+            run += v[k];
+        }
+        carry += carry_s;
         __syncthreads();
     }
-    int run = part[t] - s;
-    for (int a = a0; a < a1; a++) {
-        l.aux_start[a] = run;
-        run += min(l.aux_cnt[a], CM_NL_AUXCAP);
-    }
-    if (t == 1023) l.count[0] = part[1023];
+    if (t == 0) l.count[0] = carry;
 }
 
 // Members of each auxiliary cell in id order (deterministic lists), and the
-// build order.
-__global__ __launch_bounds__(64) void k_nl_sort(CmList l)
+// build order.  One wave per cell, four waves per block.
+__global__ __launch_bounds__(256) void k_nl_sort(CmList l)
 {
     if (!nl_rebuild(l)) return;
-    __shared__ int mem[CM_NL_AUXCAP];
-    const int t = threadIdx.x;
-    for (int a = blockIdx.x; a < l.naux; a += gridDim.x) {
+    __shared__ int mem_s[4][CM_NL_AUXCAP];
+    const int L = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int *mem = mem_s[w];
+    for (int a = blockIdx.x * 4 + w; a < l.naux; a += gridDim.x * 4) {
         const int n = min(l.aux_cnt[a], CM_NL_AUXCAP);
-        const int me = t < n ? l.aux_mem[a * CM_NL_AUXCAP + t] : 0;
-        if (t < n) mem[t] = me;
-        __syncthreads();
+        if (n == 0) continue;
+        const int me = L < n ? l.aux_mem[a * CM_NL_AUXCAP + L] : 0;
+        if (L < n) mem[L] = me;
+        nl_wave_sync();
         int rank = 0;
         for (int u = 0; u < n; u++) rank += mem[u] < me;
-        __syncthreads();
-        if (t < n) {
+        nl_wave_sync();
+        if (L < n) {
             l.aux_mem[a * CM_NL_AUXCAP + rank] = me;
             l.order[l.aux_start[a] + rank] = me;
         }
-        __syncthreads();
     }
 }
 
 // One wave per auxiliary cell: its 27 neighbours' atoms are staged in LDS at
-// their nearest periodic image; lane i (a home atom) keeps every atom within
-// cutoff + skin.
-__global__ __launch_bounds__(64) void k_nl_build(CmList l)
+// their nearest periodic image.  The nh home atoms are split over the lanes
+// P = 64 / nh ways: lane (a, q) tests halo entries q, q + P, ...; a count
+// pass and a prefix over the P lanes of an atom place each lane's entries,
+// so an atom's list holds, per staged batch, partition 0's neighbours, then
+// partition 1's, ... (a fixed order for a given build).  The last block to
+// finish closes the build: overflow -> lists invalid, rebuild request
+// cleared, counts cleared for the next build.
+#define NL_BATCH 384  // halo atoms staged per batch and wave
+__global__ __launch_bounds__(256) void k_nl_build(CmList l)
 {
     if (!nl_rebuild(l)) return;
-    __shared__ double hx[NL_BATCH], hy[NL_BATCH], hz[NL_BATCH];
-    __shared__ int hid[NL_BATCH];
-    __shared__ int cell[27], off[28];
-    __shared__ double shift[81];
-    const int L = threadIdx.x;
-    for (int a = blockIdx.x; a < l.naux; a += gridDim.x) {
+    struct WaveBuf {
+        double hx[NL_BATCH], hy[NL_BATCH], hz[NL_BATCH];
+        int hid[NL_BATCH];
+        double shift[81];
+        int cell[27], off[28];
+    };
+    __shared__ WaveBuf buf[4];
+    __shared__ int last;
+    const int L = threadIdx.x & 63, w = threadIdx.x >> 6;
+    WaveBuf &B = buf[w];
+    // waves walk contiguous ranges of cells, and the blocks of one XCD
+    // (blocks are dealt round-robin over the 8 XCDs) neighbouring ranges
+    const int nb = gridDim.x, qb = nb / 8, rb = nb % 8;
+    const int xcd = blockIdx.x % 8, bi = blockIdx.x / 8;
+    const int blk = nb < 8 ? (int)blockIdx.x : xcd * qb + min(xcd, rb) + bi;
+    const int nw = nb * 4, gw = blk * 4 + w;
+    const int per = (l.naux + nw - 1) / nw;
+    const int a0 = gw * per, a1 = min(a0 + per, l.naux);
+    bool ovf = false;
+    for (int a = a0; a < a1; a++) {
         const int nh = min(l.aux_cnt[a], CM_NL_AUXCAP);
         if (nh == 0) continue;
         const int ax = a % l.na0, ay = (a / l.na0) % l.na1, az = a / (l.na0 * l.na1);
+        nl_wave_sync();
         int ne = 0;
         if (L < 27) {
             int jx = ax + L / 9 - 1, jy = ay + (L / 3) % 3 - 1, jz = az + L % 3 - 1;
@@ -169,11 +219,11 @@ __global__ __launch_bounds__(64) void k_nl_build(CmList l)
             if (jy < 0) { jy += l.na1; sy = -l.L1; } else if (jy >= l.na1) { jy -= l.na1; sy = l.L1; }
             if (jz < 0) { jz += l.na2; sz = -l.L2; } else if (jz >= l.na2) { jz -= l.na2; sz = l.L2; }
             const int b = jx + l.na0 * (jy + l.na1 * jz);
-            cell[L] = b;
+            B.cell[L] = b;
             ne = min(l.aux_cnt[b], CM_NL_AUXCAP);
-            shift[3 * L + 0] = sx;
-            shift[3 * L + 1] = sy;
-            shift[3 * L + 2] = sz;
+            B.shift[3 * L + 0] = sx;
+            B.shift[3 * L + 1] = sy;
+            B.shift[3 * L + 2] = sz;
         }
         int incl = ne;
         for (int o = 1; o < 32; o <<= 1) {
@@ -181,65 +231,85 @@ __global__ __launch_bounds__(64) void k_nl_build(CmList l)
             if (L >= o) incl += v;
         }
         const int H = __shfl(incl, 31, 64);
-        if (L < 27) off[L] = incl - ne;
-        if (L == 0) off[27] = H;
-        // home atom of this lane, wrapped into the box like its cell
-        const int me = L < nh ? l.aux_mem[a * CM_NL_AUXCAP + L] : -1;
+        if (L < 27) B.off[L] = incl - ne;
+        if (L == 0) B.off[27] = H;
+        // lane (ha, q): home atom ha, partition q of P
+        const int P = 64 / nh;
+        const bool act = L < P * nh;
+        const int ha = act ? L % nh : 0, q = act ? L / nh : 0;
+        const int me = act ? l.aux_mem[a * CM_NL_AUXCAP + ha] : -1;
         double xi = 0, yi = 0, zi = 0;
-        if (me >= 0) {
+        if (act) {
             const double4 p = l.pos[me];
             xi = wrap_coord(p.x, l.L0);
             yi = wrap_coord(p.y, l.L1);
             zi = wrap_coord(p.z, l.L2);
         }
-        const int t = me >= 0 ? l.aux_start[a] + L : 0;
-        int k = 0;
+        const int t = l.aux_start[a] + ha;
+        int base = 0;  // entries of atom ha so far (the same in its P lanes)
         for (int h0 = 0; h0 < H; h0 += NL_BATCH) {
             const int hb = min(H - h0, NL_BATCH);
-            __syncthreads();
+            nl_wave_sync();
             int e = 0;
             for (int h = h0 + L; h < h0 + hb; h += 64) {
-                while (off[e + 1] <= h) e++;
-                const int j = l.aux_mem[cell[e] * CM_NL_AUXCAP + (h - off[e])];
+                while (B.off[e + 1] <= h) e++;
+                const int j = l.aux_mem[B.cell[e] * CM_NL_AUXCAP + (h - B.off[e])];
                 const double4 p = l.pos[j];
-                hx[h - h0] = wrap_coord(p.x, l.L0) + shift[3 * e + 0];
-                hy[h - h0] = wrap_coord(p.y, l.L1) + shift[3 * e + 1];
-                hz[h - h0] = wrap_coord(p.z, l.L2) + shift[3 * e + 2];
-                hid[h - h0] = j;
+                B.hx[h - h0] = wrap_coord(p.x, l.L0) + B.shift[3 * e + 0];
+                B.hy[h - h0] = wrap_coord(p.y, l.L1) + B.shift[3 * e + 1];
+                B.hz[h - h0] = wrap_coord(p.z, l.L2) + B.shift[3 * e + 2];
+                B.hid[h - h0] = j;
             }
-            __syncthreads();
-            if (me >= 0) {
-                for (int j = 0; j < hb; j++) {
-                    const double dx = xi - hx[j], dy = yi - hy[j], dz = zi - hz[j];
-                    const double r2 = dx * dx + dy * dy + dz * dz;
-                    if (r2 <= l.rl2 && hid[j] != me) {
-                        if (k < l.kmax) l.nbl[(size_t)k * l.n + t] = hid[j];
+            nl_wave_sync();
+            int c = 0;
+            if (act) {
+                for (int j = q; j < hb; j += P) {
+                    const double dx = xi - B.hx[j], dy = yi - B.hy[j], dz = zi - B.hz[j];
+                    c += (dx * dx + dy * dy + dz * dz <= l.rl2) && B.hid[j] != me;
+                }
+            }
+            int pre = 0, tot = 0;
+            for (int qq = 0; qq < P; qq++) {
+                const int v = __shfl(c, qq * nh + ha, 64);
+                pre += qq < q ? v : 0;
+                tot += v;
+            }
+            if (act) {
+                int k = base + pre;
+                for (int j = q; j < hb; j += P) {
+                    const double dx = xi - B.hx[j], dy = yi - B.hy[j], dz = zi - B.hz[j];
+                    if ((dx * dx + dy * dy + dz * dz <= l.rl2) && B.hid[j] != me) {
+                        if (k < l.kmax) l.nbl[(size_t)k * l.n + t] = B.hid[j];
                         k++;
                     }
                 }
             }
+            base += tot;
         }
-        if (me >= 0) {
-            l.ncnt[t] = min(k, l.kmax);
-            if (k > l.kmax) atomicOr(l.flags, (unsigned)CM_NL_INVALID);
+        if (act && q == 0) {
+            l.ncnt[t] = min(base, l.kmax);
+            ovf |= base > l.kmax;
+            l.ref[me] = l.pos[me];
         }
-        __syncthreads();
     }
-}
-
-__global__ void k_nl_finish(CmList l)
-{
-    if (!nl_rebuild(l)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < l.n && l.slot[i] >= 0) l.ref[i] = l.pos[i];
-}
-
-__global__ void k_nl_done(CmList l)
-{
-    if (!nl_rebuild(l)) return;
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
+    if (ovf) atomicOr(l.flags, (unsigned)CM_NL_OVERFLOW);
+    // the last block to finish closes the build
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        last = atomicAdd(&l.count[2], 1) == (int)gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    for (int a = threadIdx.x; a < l.naux; a += blockDim.x) l.aux_cnt[a] = 0;
+    if (threadIdx.x == 0) {
+        l.count[2] = 0;
         l.count[1] += 1;  // builds so far
-        atomicAnd(l.flags, ~(unsigned)CM_NL_REBUILD);
+        const unsigned f = atomicOr(l.flags, 0u);
+        if (f & CM_NL_OVERFLOW) atomicOr(l.flags, (unsigned)CM_NL_INVALID);
+        else atomicAnd(l.flags, ~(unsigned)CM_NL_INVALID);
+        atomicAnd(l.flags, ~(unsigned)(CM_NL_OVERFLOW | CM_NL_REBUILD));
     }
 }
 
@@ -420,26 +490,21 @@ static unsigned grid_for(long n, int block)
 void cm_nl_init(const DevState &d, const CmList &l, hipStream_t st)
 {
     hipLaunchKernelGGL(k_nl_clear_ids, dim3(grid_for(l.n, 256)), dim3(256), 0, st, l);
-    hipLaunchKernelGGL(k_nl_from_slots, dim3(d.ncells), dim3(64), 0, st, d, l);
+    hipLaunchKernelGGL(k_nl_from_slots, dim3(d.ncells), dim3(64), 0, st, d, l, 0);
 }
 
-void cm_nl_refresh32(const CmList &l, hipStream_t st)
+void cm_nl_sync(const DevState &d, const CmList &l, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_nl_refresh32, dim3(grid_for(l.n, 256)), dim3(256), 0, st, l);
+    hipLaunchKernelGGL(k_nl_from_slots, dim3(d.ncells), dim3(64), 0, st, d, l, 1);
 }
 
 void cm_nl_build(const DevState &d, const CmList &l, hipStream_t st)
 {
     (void)d;
-    const unsigned ga = grid_for(l.naux, 256), gi = grid_for(l.n, 256);
-    const unsigned gc = (unsigned)(l.naux < 8192 ? l.naux : 8192);
-    hipLaunchKernelGGL(k_nl_zero, dim3(ga), dim3(256), 0, st, l);
-    hipLaunchKernelGGL(k_nl_key, dim3(gi), dim3(256), 0, st, l);
+    hipLaunchKernelGGL(k_nl_key, dim3(1024), dim3(256), 0, st, l);
     hipLaunchKernelGGL(k_nl_scan, dim3(1), dim3(1024), 0, st, l);
-    hipLaunchKernelGGL(k_nl_sort, dim3(gc), dim3(64), 0, st, l);
-    hipLaunchKernelGGL(k_nl_build, dim3(gc), dim3(64), 0, st, l);
-    hipLaunchKernelGGL(k_nl_finish, dim3(gi), dim3(256), 0, st, l);
-    hipLaunchKernelGGL(k_nl_done, dim3(1), dim3(64), 0, st, l);
+    hipLaunchKernelGGL(k_nl_sort, dim3(1024), dim3(256), 0, st, l);
+    hipLaunchKernelGGL(k_nl_build, dim3(1024), dim3(256), 0, st, l);
 }
 
 template <int KIND, int EVAL, int MIXED>
