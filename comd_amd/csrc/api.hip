@@ -774,6 +774,12 @@ static int nl_setup(comd_ctx *c, long nids)
     l.ab0 = p.bounds[0] / na[0];
     l.ab1 = p.bounds[1] / na[1];
     l.ab2 = p.bounds[2] / na[2];
+    l.iab0 = 1.0 / l.ab0;
+    l.iab1 = 1.0 / l.ab1;
+    l.iab2 = 1.0 / l.ab2;
+    l.iL0 = 1.0 / l.L0;
+    l.iL1 = 1.0 / l.L1;
+    l.iL2 = 1.0 / l.L2;
     const double two32 = 4294967296.0;
     l.qs0 = two32 / l.L0;
     l.qs1 = two32 / l.L1;

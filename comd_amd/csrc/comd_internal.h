@@ -122,6 +122,8 @@ struct CmList {
     double L0, L1, L2;       // bounds
     int na0, na1, na2, naux; // auxiliary grid used to build the lists
     double ab0, ab1, ab2;    // its cell edges (>= cutoff + skin)
+    double iab0, iab1, iab2; // and their inverses
+    double iL0, iL1, iL2;    // 1 / L
     double4 *pos, *ref;      // per id (pos.w: F' of the last density pass)
     uint4 *pos32;            // per id: pos in fixed32 (x, y, z) and, in .w, the float bits of
                              // F' of the last mixed-precision density pass

@@ -58,9 +58,9 @@ __global__ __launch_bounds__(64) void k_nl_from_slots(DevState d, CmList l, int 
             if (check) {
                 const double4 q = l.ref[id];
                 double ex = x - q.x, ey = y - q.y, ez = z - q.z;
-                ex -= l.L0 * rint(ex / l.L0);
-                ey -= l.L1 * rint(ey / l.L1);
-                ez -= l.L2 * rint(ez / l.L2);
+                ex -= l.L0 * rint(ex * l.iL0);
+                ey -= l.L1 * rint(ey * l.iL1);
+                ez -= l.L2 * rint(ez * l.iL2);
                 far = ex * ex + ey * ey + ez * ez > d.nl_half2;
             }
         }
@@ -73,7 +73,8 @@ __global__ __launch_bounds__(64) void k_nl_from_slots(DevState d, CmList l, int 
 // launches that find no request cost only their dispatch: their grids are
 // kept small, with strided loops)
 
-__device__ __forceinline__ double wrap_coord(double x, double L) { return x - L * floor(x / L); }
+// x wrapped into [0, L) (to rounding); iL = 1 / L
+__device__ __forceinline__ double wrap_coord(double x, double L, double iL) { return x - L * floor(x * iL); }
 
 __device__ __forceinline__ void nl_wave_sync()
 {
@@ -90,9 +91,9 @@ __global__ __launch_bounds__(256) void k_nl_key(CmList l)
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < l.n; i += gridDim.x * blockDim.x) {
         if (l.slot[i] < 0) continue;
         const double4 p = l.pos[i];
-        int ax = (int)(wrap_coord(p.x, l.L0) / l.ab0);
-        int ay = (int)(wrap_coord(p.y, l.L1) / l.ab1);
-        int az = (int)(wrap_coord(p.z, l.L2) / l.ab2);
+        int ax = (int)(wrap_coord(p.x, l.L0, l.iL0) * l.iab0);
+        int ay = (int)(wrap_coord(p.y, l.L1, l.iL1) * l.iab1);
+        int az = (int)(wrap_coord(p.z, l.L2, l.iL2) * l.iab2);
         ax = min(max(ax, 0), l.na0 - 1);
         ay = min(max(ay, 0), l.na1 - 1);
         az = min(max(az, 0), l.na2 - 1);
@@ -182,8 +183,9 @@ __global__ __launch_bounds__(256) void k_nl_sort(CmList l)
 // so an atom's list holds, per staged batch, partition 0's neighbours, then
 // partition 1's, ... (a fixed order for a given build).  The last block to
 // finish closes the build: overflow -> lists invalid, rebuild request
-// cleared, counts cleared for the next build.
-#define NL_BATCH 384  // halo atoms staged per batch and wave
+// cleared, counts cleared for the next build.  About 8 KB of LDS per wave,
+// so that enough waves are resident to hide the gathers' latency.
+#define NL_BATCH 256   // halo atoms staged per batch and wave
 __global__ __launch_bounds__(256) void k_nl_build(CmList l)
 {
     if (!nl_rebuild(l)) return;
@@ -241,9 +243,9 @@ __global__ __launch_bounds__(256) void k_nl_build(CmList l)
         double xi = 0, yi = 0, zi = 0;
         if (act) {
             const double4 p = l.pos[me];
-            xi = wrap_coord(p.x, l.L0);
-            yi = wrap_coord(p.y, l.L1);
-            zi = wrap_coord(p.z, l.L2);
+            xi = wrap_coord(p.x, l.L0, l.iL0);
+            yi = wrap_coord(p.y, l.L1, l.iL1);
+            zi = wrap_coord(p.z, l.L2, l.iL2);
         }
         const int t = l.aux_start[a] + ha;
         int base = 0;  // entries of atom ha so far (the same in its P lanes)
@@ -255,9 +257,9 @@ __global__ __launch_bounds__(256) void k_nl_build(CmList l)
                 while (B.off[e + 1] <= h) e++;
                 const int j = l.aux_mem[B.cell[e] * CM_NL_AUXCAP + (h - B.off[e])];
                 const double4 p = l.pos[j];
-                B.hx[h - h0] = wrap_coord(p.x, l.L0) + B.shift[3 * e + 0];
-                B.hy[h - h0] = wrap_coord(p.y, l.L1) + B.shift[3 * e + 1];
-                B.hz[h - h0] = wrap_coord(p.z, l.L2) + B.shift[3 * e + 2];
+                B.hx[h - h0] = wrap_coord(p.x, l.L0, l.iL0) + B.shift[3 * e + 0];
+                B.hy[h - h0] = wrap_coord(p.y, l.L1, l.iL1) + B.shift[3 * e + 1];
+                B.hz[h - h0] = wrap_coord(p.z, l.L2, l.iL2) + B.shift[3 * e + 2];
                 B.hid[h - h0] = j;
             }
             nl_wave_sync();
@@ -359,7 +361,7 @@ __device__ __forceinline__ void nl_atom(const DevState &d, const PotDev &P, cons
     const double fia = KIND != CM_KIND_EAM3 ? 0.0 : MIXED ? (double)__uint_as_float(qi.w) : pi.w;
     const double rc2 = P.rc2;
     const double L0 = l.L0, L1 = l.L1, L2 = l.L2;
-    const double iL0 = 1.0 / L0, iL1 = 1.0 / L1, iL2 = 1.0 / L2;
+    const double iL0 = l.iL0, iL1 = l.iL1, iL2 = l.iL2;
     const int cnt = l.ncnt[t];
     Acc A = {0, 0, 0, 0, 0, 0};
     int npair = 0;
