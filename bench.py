@@ -3,10 +3,12 @@
 Workload: EAM Cu (cu500 tables, 3-point table interpolation), FCC 80^3 =
 2,048,000 atoms, a = 3.615 A, periodic, Maxwell-Boltzmann velocities at
 300 K from a fixed seed, dt = 1 fs.  One step = half drift + rebin, force
-(density + embedding pass, embedding-force pass), kick, half drift + rebin,
-all on the GPU with the state resident in HBM.  The headline value is the
-mixed-precision path (FP32 pair arithmetic, FP64 sums and integration); the
-FP64 path is timed in the same run.
+(neighbour-list check and rebuild when an atom has moved half the 0.4 A
+skin, density + embedding pass, embedding-force pass), kick, half drift +
+rebin, all on the GPU with the state resident in HBM.  The headline value
+is the mixed-precision path (FP32 table coefficients, fixed-point list
+positions, FP64 sums and integration); the FP64 path is timed in the same
+run.
 
 Prints one JSON line (rank 0).  --gpus above 1 is refused: this build runs
 on one GPU.
@@ -27,9 +29,12 @@ sys.path.insert(0, ROOT)
 
 import comd_amd as CA  # noqa: E402
 
-# MI355X published peaks
+# MI355X published peaks.  The vector FP32 figure (157.3) counts packed
+# FP32 instructions, which this library does not use; unpacked FP32 and
+# FP64 vector instructions both peak at 78.6 TFLOP/s, the ceiling that
+# applies to every kernel here.
 PEAK_FP64_TFLOPS = 78.6
-PEAK_FP32_TFLOPS = 157.3
+PEAK_VALU_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
 # Algorithmic work per atom-step (EAM table; +,-,*,/,sqrt = 1 FLOP each,
@@ -43,12 +48,15 @@ FLOP_PAIR_EMBED_FORCE = 49
 FLOP_ATOM_EMBED = 20
 FLOP_ATOM_VERLET = 24
 BYTES_ATOM_STEP = 232
+# HIP-event timing classes of comd_timing_read
+KCLASS = ("move+rebin", "density", "embed_force", "reduce", "list_build", "cell_fallback")
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: 1000 for eam, 500 for batch, 60 for adjoint)")
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=80, help="FCC unit cells per edge")
     ap.add_argument("--seed", type=int, default=2048)
@@ -57,7 +65,14 @@ def parse():
     ap.add_argument("--workload", choices=("eam", "batch", "adjoint"), default="eam",
                     help="eam: EAM Cu 2.05M atoms, FP64 and mixed (the headline line); batch: 64 LJ/EAM replicas "
                          "in one context; adjoint: Chebyshev-coefficient gradients of 16 Cu 20^3 configurations")
-    return ap.parse_args()
+    args = ap.parse_args()
+    if args.steps is None:
+        args.steps = {"eam": 1000, "batch": 500, "adjoint": 60}[args.workload]
+    return args
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def run_precision(sim, precision, steps, warmup, n_atoms):
@@ -67,6 +82,7 @@ def run_precision(sim, precision, steps, warmup, n_atoms):
     sim.force()
     pe0, ke0, _ = sim.energy()
     p0 = sim.pair_count() / n_atoms
+    b0 = sim.list_info()["builds"]
     sim.timing(True)
     sim.sync()
     t0 = time.perf_counter()
@@ -75,6 +91,7 @@ def run_precision(sim, precision, steps, warmup, n_atoms):
     t1 = time.perf_counter()
     ms, launches = sim.timing_read()
     sim.timing(False)
+    info = sim.list_info()
     sim.force()
     pe1, ke1, _ = sim.energy()
     p1 = sim.pair_count() / n_atoms
@@ -88,21 +105,25 @@ def run_precision(sim, precision, steps, warmup, n_atoms):
         p_half=0.5 * (p0 + p1),
         e0=pe0 + ke0,
         e1=pe1 + ke1,
+        lists=bool(info["active"] and info["valid"]),
+        builds=info["builds"] - b0,
+        kmax=info["kmax"],
     )
 
 
 def measured_traffic(kname):
     """HBM bytes per launch of a kernel from the committed rocprofv3 counter
-    summary (FETCH_SIZE and WRITE_SIZE runs, scaled by a copy kernel of known
-    size); (None, reason) when it holds no line for the kernel."""
-    path = os.path.join(ROOT, "profiles", "k_pair_eam_cu_fcc80_hbm_traffic.txt")
-    try:
+    summaries (FETCH_SIZE and WRITE_SIZE runs, scaled by a copy kernel of
+    known size); (None, reason) when none holds a line for the kernel."""
+    pdir = os.path.join(ROOT, "profiles")
+    for name in sorted(os.listdir(pdir)) if os.path.isdir(pdir) else []:
+        if not name.endswith("hbm_traffic.txt"):
+            continue
+        path = os.path.join(pdir, name)
         for line in open(path):
             m = re.match(r"traffic_bytes_per_launch (.+?) = ([0-9]+)", line.strip())
             if m and m.group(1) == kname:
                 return float(m.group(2)), os.path.relpath(path, ROOT)
-    except OSError:
-        pass
     return None, "no counter summary for this kernel"
 
 
@@ -160,6 +181,25 @@ def main():
     sim.force()
     t_setup = time.perf_counter() - t_setup
 
+    # per-atom force error of the mixed path against FP64 on the same state
+    fe = {}
+    for prec in ("fp64", "mixed"):
+        sim.set_precision(prec)
+        sim.force()
+        st = sim.download()
+        m = (np.arange(CA.MAXATOMS)[None, :] < st["natoms"][:, None]).reshape(-1)
+        fe[prec] = (st["f"][m], st["rho"][m])
+        del st
+    d = np.linalg.norm(fe["mixed"][0][:, :3] - fe["fp64"][0][:, :3], axis=1)
+    mag = np.linalg.norm(fe["fp64"][0][:, :3], axis=1)
+    rel = d / np.maximum(mag, np.sqrt(np.mean(mag ** 2)))
+    erel = np.abs(fe["mixed"][0][:, 3] - fe["fp64"][0][:, 3]) / np.abs(fe["fp64"][0][:, 3])
+    force_err = dict(force_max=float(rel.max()), force_median=float(np.median(rel)),
+                     energy_per_atom_max=float(erel.max()),
+                     rho_max=float((np.abs(fe["mixed"][1] - fe["fp64"][1]) / np.abs(fe["fp64"][1])).max()))
+    del fe, d, mag, rel, erel
+    log(f"mixed vs fp64 at the start state: {force_err}")
+
     res = {}
     for prec in ("fp64", "mixed"):
         res[prec] = run_precision(sim, prec, args.steps, args.warmup, n_atoms)
@@ -171,11 +211,15 @@ def main():
     mixed = res["mixed"]
     fp64 = res["fp64"]
     # dominant kernel of the mixed path (the force pass with the larger
-    # time), timed with HIP events on the stream it runs on
+    # time), timed with HIP events on the stream it runs on: the list pass
+    # while the neighbour lists are in use, else the link-cell pass
+    lists = mixed["lists"]
+    kd = "k_nl_pair<0, 0, 1, 1>" if lists else "k_pair<0, 0, 1>"
+    ke = "k_nl_pair<1, 0, 1, 1>" if lists else "k_pair<1, 0, 1>"
     passes = {
-        CA.K_DENSITY: ("k_pair<0, 0, 1>", "density + embedding pass (src-flat/eam.c:484-594), table, mixed",
+        CA.K_DENSITY: (kd, "density + embedding pass (src-flat/eam.c:484-594), table, mixed",
                        lambda ph: (FLOP_PAIR_DENSITY * ph + FLOP_ATOM_EMBED) * n_atoms),
-        CA.K_EMBED_FORCE: ("k_pair<1, 0, 1>", "embedding-force pass (src-flat/eam.c:595-653), table, mixed",
+        CA.K_EMBED_FORCE: (ke, "embedding-force pass (src-flat/eam.c:595-653), table, mixed",
                            lambda ph: FLOP_PAIR_EMBED_FORCE * ph * n_atoms),
     }
     k = max(passes, key=lambda c: mixed["kernel_ms"][c])
@@ -188,7 +232,7 @@ def main():
     flop64 = kflop(fp64["p_half"])
 
     f_alg = (FLOP_PAIR_DENSITY + FLOP_PAIR_EMBED_FORCE) * mixed["p_half"] + FLOP_ATOM_EMBED + FLOP_ATOM_VERLET
-    ceil_mixed = min(PEAK_FP32_TFLOPS * 1e12 / f_alg, PEAK_HBM_GBS * 1e9 / BYTES_ATOM_STEP)
+    ceil_mixed = min(PEAK_VALU_TFLOPS * 1e12 / f_alg, PEAK_HBM_GBS * 1e9 / BYTES_ATOM_STEP)
     ceil_fp64 = min(PEAK_FP64_TFLOPS * 1e12 / f_alg, PEAK_HBM_GBS * 1e9 / BYTES_ATOM_STEP)
 
     cpu = None
@@ -209,7 +253,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 pair arithmetic, f64 accumulation and integration",
+        "dtype": "f64 geometry, accumulation and integration; f32 table coefficients; 32-bit fixed-point list positions",
         "data": "synthetic: FCC lattice, seeded Maxwell-Boltzmann velocities at 300 K",
         "config": {
             "workload": f"EAM Cu cu500 table, FCC {n}^3 = {n_atoms} atoms, a=3.615 A, 300 K, dt=1 fs, "
@@ -224,6 +268,7 @@ def main():
             "dominant_kernel_ms": k64 * 1e3,
             "dominant_kernel_tflops": flop64 / k64 / 1e12,
             "dominant_kernel_frac_of_fp64_peak": flop64 / k64 / 1e12 / PEAK_FP64_TFLOPS,
+            "kernel_ms_per_step": {name: fp64["kernel_ms"][i] / args.steps for i, name in enumerate(KCLASS)},
             "step_roofline_frac": fp64["atom_steps_per_s"] / ceil_fp64,
             "energy_drift_per_atom_ha": (fp64["e1"] - fp64["e0"]) / n_atoms,
         },
@@ -232,9 +277,9 @@ def main():
             "bound": "valu",
             "kernel": f"{kname}: {kdesc}",
             "achieved": achieved,
-            "peak": PEAK_FP32_TFLOPS,
+            "peak": PEAK_VALU_TFLOPS,
             "unit": "TFLOP/s",
-            "frac": achieved / PEAK_FP32_TFLOPS,
+            "frac": achieved / PEAK_VALU_TFLOPS,
             "traffic": traffic,
             "traffic_source": traffic_src,
             "hbm_frac": (traffic / avg_s / 1e9 / PEAK_HBM_GBS) if traffic else None,
@@ -248,11 +293,12 @@ def main():
             "ceiling_atom_steps_per_s": ceil_mixed,
             "frac": mixed["atom_steps_per_s"] / ceil_mixed,
         },
-        "kernel_ms_per_step": {
-            name: mixed["kernel_ms"][i] / args.steps
-            for i, name in enumerate(("move+rebin", "density", "embed_force", "reduce"))
-        },
+        "kernel_ms_per_step": {name: mixed["kernel_ms"][i] / args.steps for i, name in enumerate(KCLASS)},
+        "neighbour_lists": {"active": mixed["lists"], "skin_A": 0.4, "builds_in_timed_steps": mixed["builds"],
+                            "list_capacity": mixed["kmax"]},
         "energy_drift_per_atom_ha": (mixed["e1"] - mixed["e0"]) / n_atoms,
+        "mixed_error_vs_fp64": dict(force_err, note="per-atom |df| / max(|f|, rms |f|) at the seeded start state; "
+                                    "energy drift of both paths over the timed steps is energy_drift_per_atom_ha"),
         "device": devname,
         "setup_s": t_setup,
         "cpu_baseline": cpu,
@@ -277,17 +323,23 @@ def bench_batch(args, devname):
         xyz, p = CA.seed_state(xyz, pot.mass, 300.0, 0.0, args.seed * 1000 + r)
         reps.append(dict(pot=pot, bounds=(n * lat * stretch, n * lat, n * lat), defgrad=0.0))
         atoms.append((xyz, p, n, stretch, "lj" if r % 2 == 0 else "eam"))
+    log(f"batch: {len(reps)} replicas, {sum(a[0].shape[0] for a in atoms)} atoms generated")
     t0 = time.perf_counter()
     b = CA.Batch(reps, precision="mixed")
+    # Lennard-Jones replicas move down their potential: in src-flat's sign
+    # convention a 300 K lattice flies apart within about 50 steps
+    b.tune(lj_physical=1)
     for r, (xyz, p, *_rest) in enumerate(atoms):
         b.bin_atoms(r, xyz, p)
     b.rebox()
     b.force()
     b.sync()
     t_setup = time.perf_counter() - t0
+    log(f"batch: binned and first force in {t_setup:.2f} s")
     sizes = np.array([a[0].shape[0] for a in atoms])
     total = int(sizes.sum())
     tuned = b.autotune()
+    log(f"batch: autotune {tuned}")
     steps = args.steps
     b.steps(args.warmup)
     b.sync()
@@ -296,6 +348,7 @@ def bench_batch(args, devname):
     b.steps(steps)
     b.sync()
     wall = time.perf_counter() - t0
+    log(f"batch: {steps} graph steps in {wall:.3f} s")
     b.force()
     pe1, ke1, _ = b.energy()
     # the same steps launched one by one, and the kernel split, for reference
@@ -325,7 +378,7 @@ def bench_batch(args, devname):
         "data": "synthetic: FCC lattices, seeded Maxwell-Boltzmann velocities at 300 K",
         "config": {
             "workload": f"64 replicas, {total} atoms, {b.ncells} link cells, one launch per kernel phase, "
-                        f"HIP-graph step",
+                        f"HIP-graph step; Lennard-Jones replicas in the physical force convention",
             "replicas": [dict(kind=a[4], n=a[2], atoms=int(a[0].shape[0]), stretch=round(a[3], 4))
                          for a in atoms],
             "parallelism": "single GPU",
@@ -335,7 +388,7 @@ def bench_batch(args, devname):
         "eager_ms_per_step": 1e3 * wall_eager / ke,
         "graph_speedup": (wall_eager / ke) / (wall / steps),
         "kernel_ms_per_step_eager": {
-            name: ms[i] / ke for i, name in enumerate(("move+rebin", "density/LJ", "embed_force", "reduce"))},
+            name: ms[i] / ke for i, name in enumerate(KCLASS)},
         "autotune": {"waves": tuned[0], "blocks_per_cu": tuned[1], "balanced_drain": tuned[2],
                      "force_ms": tuned[3]},
         "energy_drift_per_atom_ha": ((pe1 + ke1) - (pe0 + ke0)) / total,

@@ -485,6 +485,7 @@ static bool setup_pot(const comd_params *p, Replica &R)
         const double sg = p->lj_sigma;
         P.lj_s6 = sg * sg * sg * sg * sg * sg;  // src-flat/ljForce.c:143
         P.lj_eps = p->lj_epsilon;
+        P.lj_fsign = 1.0;
         return true;
     }
     for (int k = 0; k < 3 && ok; k++) {
@@ -1195,6 +1196,11 @@ int comd_tune(comd_ctx *c, int key, int value)
     case COMD_TUNE_PAIR_LISTS:
         c->use_plist = value ? 1 : 0;
         break;
+    case COMD_TUNE_LJ_PHYSICAL:
+        for (Replica &R : c->reps)
+            if (R.prm.potential == COMD_POT_LJ) R.pot.lj_fsign = value ? -1.0 : 1.0;
+        invalidate_graph(c);
+        return upload_descriptors(c);
     case COMD_TUNE_SKIN_MILLI:
         if (value < 0 || value > 3000) return set_err(c, COMD_ERR_ARG, "skin must be 0..3000 (1e-3 A)");
         c->nl_skin_milli = value;

@@ -27,6 +27,10 @@ struct PotDev {
     double ca[3], cb[3];   // domains of phi, rho, F
     // Lennard-Jones
     double lj_s6, lj_eps;
+    // sign of the stored LJ force: +1 stores -grad U as src-flat/ljForce.c
+    // does (its integrator then climbs the potential, timestep.c:71), -1
+    // stores +grad U, the convention of the EAM force, for physical motion
+    double lj_fsign;
     double mass_amu;
 };
 

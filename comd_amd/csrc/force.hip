@@ -348,9 +348,9 @@ __device__ __forceinline__ void pair_cell(const DevState &d, const Grid &g, cons
             // kinetic term of src-flat/ljForce.c:175, once per non-empty
             // stencil cell, with the mass in amu
             const double px = d.px[o];
-            d.fx[o] = sx;
-            d.fy[o] = sy;
-            d.fz[o] = sz;
+            d.fx[o] = P.lj_fsign * sx;
+            d.fy[o] = P.lj_fsign * sy;
+            d.fz[o] = P.lj_fsign * sz;
             d.fe[o] = se;
             d.vir[o] = sv - (double)nonempty * (px * px / P.mass_amu);
             d.nnb[o] = sn;

@@ -431,9 +431,9 @@ __device__ __forceinline__ void nl_atom(const DevState &d, const PotDev &P, cons
         d.vir[s] = A.vir;
     } else {
         const double px = d.px[s];
-        d.fx[s] = A.x;
-        d.fy[s] = A.y;
-        d.fz[s] = A.z;
+        d.fx[s] = P.lj_fsign * A.x;
+        d.fy[s] = P.lj_fsign * A.y;
+        d.fz[s] = P.lj_fsign * A.z;
         d.fe[s] = A.e;
         d.vir[s] = A.vir - (double)l.nonempty[s / CM_SLOTS] * (px * px / P.mass_amu);
         d.nnb[s] = npair;

@@ -143,10 +143,12 @@ int comd_energy(comd_ctx *ctx, double *pe, double *ke, double *stress);
 int comd_set_precision(comd_ctx *ctx, int precision);
 int comd_set_eval(comd_ctx *ctx, int eval);
 
-/* Launch shape of the force kernels (no counterpart in the reference, whose
- * force loops are serial): waves (home cells in flight) per block, persistent
- * blocks per compute unit, and whether the potential tables are staged in
- * LDS.  Results do not depend on these settings. */
+/* Tuning of a context (no counterpart in the reference, whose force loops are
+ * serial): launch shape of the link-cell force kernels (waves = home cells in
+ * flight per block, persistent blocks per compute unit, tables in LDS, lane
+ * balance), step replay, pair-list replay and the Verlet skin, none of which
+ * changes results beyond the order of per-atom sums; and the Lennard-Jones
+ * force convention (key 8), which does. */
 enum comd_tune_key {
     COMD_TUNE_WAVES_PER_BLOCK = 1, /* 1..4, default 1 */
     COMD_TUNE_BLOCKS_PER_CU = 2,   /* 1..64, default 8 */
@@ -161,6 +163,13 @@ enum comd_tune_key {
                                       dimension keep per-atom neighbour lists, rebuilt on the device
                                       when an atom has moved half the skin; same pairs and outputs,
                                       per-atom sums in another order */
+    COMD_TUNE_LJ_PHYSICAL = 8,     /* 0/1, default 0: Lennard-Jones forces stored as +grad U, so the
+                                      integrator moves atoms down the potential (physical motion,
+                                      KE + PE conserved).  0 keeps src-flat's convention (ljForce.c
+                                      stores -grad U and timestep.c:71 subtracts it: atoms climb the
+                                      potential, KE - PE is conserved and a lattice at any temperature
+                                      flies apart within about 50 fs).  No counterpart in the
+                                      reference. */
 };
 int comd_tune(comd_ctx *ctx, int key, int value);
 

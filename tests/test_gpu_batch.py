@@ -164,3 +164,31 @@ def test_batch_autotune_and_mixed(gpu, cu500):
     for r in range(b.nrep):
         e32, e64 = b.energy_of(r)[0], b64.energy_of(r)[0]
         assert abs(e32 - e64) <= 1e-5 * abs(e64)
+
+
+def test_batch_lj_physical_steps_equal_single_runs(gpu, cu500):
+    """Physical-convention Lennard-Jones replicas beside EAM ones, 60 graph
+    steps (past the point where src-flat's convention flies apart): each
+    replica equals its single run."""
+    specs = replica_specs(4, count=4)
+    b = make_batch(specs, cu500)
+    b.tune(lj_physical=1)
+    singles = make_singles(specs, cu500)
+    for s in singles:
+        s.tune(lj_physical=1)
+    b.rebox()
+    b.force()
+    for s in singles:
+        s.rebox()
+        s.force()
+    b.steps(60)
+    b.force()
+    b.sync()
+    for s in singles:
+        s.steps(60)
+        s.force()
+    compare(b, singles, "physical LJ, 60 steps", tol=1e-11)
+    for r, sp in enumerate(specs):
+        if sp["kind"] == "lj":
+            ke = b.energy_of(r)[1]
+            assert 0.0 < ke < 1.0, f"replica {r}: kinetic energy {ke} Ha"

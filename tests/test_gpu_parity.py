@@ -343,3 +343,51 @@ def test_lj_mixed_precision_error(gpu):
     ef = force_rel_error(g["f"], o["f"])
     print(f"[lj mixed] force max {ef.max():.3e} median {np.median(ef):.3e}", flush=True)
     assert ef.max() < 1e-5
+
+
+# ---------------------------------------------------------------------------
+# Lennard-Jones force convention (COMD_TUNE_LJ_PHYSICAL)
+
+def test_lj_physical_sign_negates_forces_only(gpu):
+    """The physical convention stores +grad U: forces are the exact negation
+    of the reference convention's, energies and neighbour counts unchanged."""
+    pot = CA.copper_lj()
+    xyz, p, bounds = seeded_atoms(6, pot.lat, pot.mass, temp=300.0, disp=0.05, seed=21)
+    a = CA.Simulation(pot, bounds, defgrad=0.0)
+    b = CA.Simulation(pot, bounds, defgrad=0.0)
+    b.tune(lj_physical=1)
+    for s in (a, b):
+        s.bin_atoms(xyz, p)
+        s.rebox()
+        s.force()
+    ga, gb = compact(a.download()), compact(b.download())
+    assert np.array_equal(ga["id"], gb["id"])
+    assert np.array_equal(gb["f"][:, :3], -ga["f"][:, :3])
+    assert np.array_equal(gb["f"][:, 3], ga["f"][:, 3])
+    assert a.energy()[0] == b.energy()[0]
+
+
+@pytest.mark.parametrize("skin", [0.4, 0.0])
+def test_lj_physical_motion_conserves_energy(gpu, skin):
+    """300 K, 300 steps: in the physical convention the lattice stays bound
+    and KE + PE is conserved to the level the unshifted cutoff allows (pairs
+    crossing rc add small jumps); the reference convention flies apart within
+    about 50 steps (shown on the oracle by the companion CPU test)."""
+    pot = CA.copper_lj()
+    xyz, p, bounds = seeded_atoms(8, pot.lat, pot.mass, temp=300.0, disp=0.0, seed=22)
+    s = CA.Simulation(pot, bounds, defgrad=0.0)
+    s.tune(lj_physical=1, skin=skin)
+    s.bin_atoms(xyz, p)
+    s.rebox()
+    s.force()
+    pe0, ke0, _ = s.energy()
+    e = []
+    for _ in range(6):
+        s.nsteps(50)
+        pe, ke, _ = s.energy()
+        e.append(pe + ke)
+    drift = max(abs(x - (pe0 + ke0)) for x in e) / ke0
+    kmax = max(abs(x) for x in (ke,))
+    print(f"[lj physical skin={skin}] KE0 {ke0:.4e} KE {ke:.4e} max |dE|/KE0 {drift:.2e}", flush=True)
+    assert drift < 2e-2
+    assert 0.2 * ke0 < kmax < 5.0 * ke0

@@ -470,3 +470,25 @@ def test_rebox_moves_and_swaps_with_last(cu):
     lower = c - 1
     assert st2["natoms"][lower] == 1 and st2["id"][lower * 64] == st["id"][c * 64]
     assert st2["r"][lower * 64, 0] == -1e-12 + bs[0]
+
+
+def test_lj_reference_convention_is_unstable():
+    """src-flat's LJ stores -grad U and its integrator subtracts it
+    (ljForce.c, timestep.c:71), so atoms climb the potential: a 300 K FCC
+    lattice gains kinetic energy without bound within about 50 fs.  This is
+    why trajectories that must stay physical use COMD_TUNE_LJ_PHYSICAL."""
+    sigma, eps, mass = 2.315, 0.167, 63.55
+    olj = O.OraclePot.lj(sigma, eps, 2.5 * sigma, mass)
+    lat = math.sqrt(2.0) * 2.0 ** (1.0 / 6.0) * sigma
+    s = O.OracleSim.fcc(olj, 5, lat)
+    st = s.state()
+    rng = np.random.default_rng(2024)
+    rmass = 1822.83 * mass
+    p = rng.normal(0.0, math.sqrt(rmass * 3.166811563e-6 * 300.0), st["p"].shape)
+    occupied = np.arange(p.shape[0]) % 64 < np.repeat(st["natoms"], 64)  # slot c*64+i used
+    s.set_rp(st["r"], p * occupied[:, None])
+    s.force()
+    ke0 = s.kinetic()
+    assert ke0 > 0.0
+    s.nsteps(40)
+    assert s.info()["err"] != 0 or s.kinetic() > 3.0 * ke0
