@@ -83,14 +83,17 @@ struct comd_ctx {
     std::vector<double> cheb_host;  // current value series phi, rho, F (3 x cn) of replica 0
     // force-kernel launch shape (comd_tune / comd_autotune)
     int ncu, max_lds;
+    int lds_block;         // LDS bytes a block may use on this device
     int force_waves, force_blocks_per_cu, tab_lds;
     // Verlet neighbour lists (single periodic simulations; comd_tune)
     CmList nl;
     int nl_skin_milli;  // skin in 1e-3 A, 0 = link-cell kernels only
     int nl_active;      // lists allocated and usable for the atoms held
     int nl_stale;       // list positions not updated by the last drift
+    int nl_group_opt;   // COMD_TUNE_LIST_GROUPS (-1: by precision)
+    int nl_rows_lds;    // COMD_TUNE_LIST_ROWS_LDS (-1: whenever they fit)
     long nl_ids;        // id capacity of the atoms held
-    long nl_alloc_n, nl_alloc_k, nl_alloc_aux;
+    long nl_alloc_n, nl_alloc_k, nl_alloc_aux, nl_alloc_groups;
     // one DKD step captured as a HIP graph (comd_steps)
     int use_graph;
     hipGraphExec_t graph;
@@ -382,12 +385,14 @@ int comd_grid_for(comd_params *p, const double bounds[3], double defgrad, double
 
 static void free_nl(comd_ctx *c)
 {
-    void *p[] = {c->nl.pos, c->nl.pos32, c->nl.ref, c->nl.slot, c->nl.order, c->nl.nbl, c->nl.ncnt,
-                 c->nl.aux_cnt, c->nl.aux_start, c->nl.aux_mem, c->nl.nonempty, c->nl.count, c->nl.flags};
+    void *p[] = {c->nl.pos,      c->nl.pos32,    c->nl.ref,       c->nl.slot,    c->nl.order,
+                 c->nl.nbl,      c->nl.ncnt,     c->nl.aux_cnt,   c->nl.aux_n,   c->nl.aux_start,
+                 c->nl.aux_mem,  c->nl.gstart,   c->nl.nonempty,  c->nl.count,   c->nl.flags,
+                 (void *)c->nl.cell_order, (void *)c->nl.group_q0};
     for (void *q : p)
         if (q) hipFree(q);
     memset(&c->nl, 0, sizeof c->nl);
-    c->nl_alloc_n = c->nl_alloc_k = c->nl_alloc_aux = 0;
+    c->nl_alloc_n = c->nl_alloc_k = c->nl_alloc_aux = c->nl_alloc_groups = 0;
     c->nl_active = 0;
 }
 
@@ -513,6 +518,37 @@ static bool setup_pot(const comd_params *p, Replica &R)
     }
     P.same_grid = p->tab_n[0] == p->tab_n[1] && p->tab_x0[0] == p->tab_x0[1] && p->tab_xn[0] == p->tab_xn[1] &&
                   p->tab_invdx[0] == p->tab_invdx[1];
+    if (ok && P.same_grid) {
+        // the rows the list passes read, packed per interval
+        std::vector<double> f64, g64, d64, r64;
+        std::vector<float> f32, g32;
+        table_coefs(p->tab_n[0], p->tab_invdx[0], p->tab_values[0], f64, f32);
+        table_coefs(p->tab_n[1], p->tab_invdx[1], p->tab_values[1], g64, g32);
+        const int n = p->tab_n[0];
+        d64.resize((size_t)8 * n);
+        r64.resize((size_t)2 * n);
+        for (int i = 0; i < n; i++) {
+            const double row[8] = {f64[6 * i + 0], f64[6 * i + 1], f64[6 * i + 2], f64[6 * i + 3],
+                                   f64[6 * i + 4], g64[6 * i + 2], g64[6 * i + 3], g64[6 * i + 4]};
+            for (int q = 0; q < 8; q++) d64[8 * i + q] = row[q];
+            r64[2 * i + 0] = g64[6 * i + 0];
+            r64[2 * i + 1] = g64[6 * i + 1];
+        }
+        std::vector<float> d32(d64.size()), r32(r64.size());
+        for (size_t q = 0; q < d64.size(); q++) d32[q] = (float)d64[q];
+        for (size_t q = 0; q < r64.size(); q++) r32[q] = (float)r64[q];
+        double *a = (double *)alloc(d64.size() * 8), *b = (double *)alloc(r64.size() * 8);
+        float *e = (float *)alloc(d32.size() * 4), *f = (float *)alloc(r32.size() * 4);
+        if (ok)
+            ok = hipMemcpy(a, d64.data(), d64.size() * 8, hipMemcpyHostToDevice) == hipSuccess &&
+                 hipMemcpy(b, r64.data(), r64.size() * 8, hipMemcpyHostToDevice) == hipSuccess &&
+                 hipMemcpy(e, d32.data(), d32.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
+                 hipMemcpy(f, r32.data(), r32.size() * 4, hipMemcpyHostToDevice) == hipSuccess;
+        P.cd64 = a;
+        P.cr64 = b;
+        P.cd32 = e;
+        P.cr32 = f;
+    }
     if (ok && p->cheb_n >= 2 && p->cheb_coef) {
         const int m = p->cheb_n;
         R.d_cheb = (double *)alloc((size_t)6 * m * 8);
@@ -685,9 +721,11 @@ static int ctx_create(int nrep, const comd_params *prms, comd_ctx **out)
     hipDeviceProp_t prop;
     c->ncu = 256;
     c->max_lds = 65536;
+    c->lds_block = 65536;
     if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) {
         if (prop.multiProcessorCount > 0) c->ncu = prop.multiProcessorCount;
         if (prop.sharedMemPerBlock > 0 && prop.sharedMemPerBlock < (size_t)c->max_lds) c->max_lds = (int)prop.sharedMemPerBlock;
+        if (prop.sharedMemPerBlock > 0) c->lds_block = (int)std::min<size_t>(prop.sharedMemPerBlock, 160 << 10);
     }
     // measured on MI355X (2.05M-atom EAM): single-wave blocks, 8 per CU
     c->force_waves = 1;
@@ -695,6 +733,8 @@ static int ctx_create(int nrep, const comd_params *prms, comd_ctx **out)
     c->tab_lds = 1;
     c->use_graph = nrep > 1;
     c->nl_skin_milli = nrep > 1 ? 0 : 400;
+    c->nl_group_opt = -1;
+    c->nl_rows_lds = -1;
     if (hipStreamSynchronize(c->stream) != hipSuccess) {
         free_all(c);
         delete c;
@@ -733,7 +773,27 @@ static int nl_setup(comd_ctx *c, long nids)
     const double vol = p.bounds[0] * p.bounds[1] * p.bounds[2];
     const double expect = 4.0 / 3.0 * 3.141592653589793 * rl * rl * rl * (double)c->ntot / vol;
     const long kmax = (((long)(1.5 * expect) + 16 + 15) / 16) * 16;
-    if (nids > c->nl_alloc_n || kmax > c->nl_alloc_k || naux > c->nl_alloc_aux) {
+    // groups of auxiliary cells: the largest of the options whose staged
+    // halo (1.3x the mean occupancy, plus 64 atoms) fits the LDS a block may
+    // use in every pass (16 B per halo atom in mixed precision, up to 32 B in
+    // FP64) and in the build (24 B), beside the kernels' static LDS
+    const double occ = (double)c->ntot / (double)naux;
+    int gd[3] = {0, 0, 0};
+    long hcap = 0;
+    const int gopts[4][3] = {{4, 4, 2}, {4, 2, 2}, {2, 2, 2}, {2, 2, 1}};
+    const long lds_avail = (long)c->lds_block - CM_NL_STATIC_LDS;
+    const long hb = c->precision == COMD_PREC_MIXED ? 24 : 32;
+    for (int o = c->nl_group_opt >= 0 ? c->nl_group_opt : 0; o < 4; o++) {
+        for (int j = 0; j < 3; j++) gd[j] = std::min(gopts[o][j], na[j]);
+        const long slots = (long)(gd[0] + 2) * (gd[1] + 2) * (gd[2] + 2);
+        hcap = (((long)(1.3 * occ * (double)slots) + 64 + 63) / 64) * 64;
+        if (hcap * hb <= lds_avail) break;
+    }
+    if (hcap * hb > lds_avail || hcap > 65535) return COMD_OK;  // too dense for staged halos
+    int ng[3];
+    for (int j = 0; j < 3; j++) ng[j] = (na[j] + gd[j] - 1) / gd[j];
+    const long ngroups = (long)ng[0] * ng[1] * ng[2];
+    if (nids > c->nl_alloc_n || kmax > c->nl_alloc_k || naux > c->nl_alloc_aux || ngroups > c->nl_alloc_groups) {
         free_nl(c);
         bool ok = true;
         auto alloc = [&](void **ptr, size_t bytes) {
@@ -745,11 +805,15 @@ static int nl_setup(comd_ctx *c, long nids)
         alloc((void **)&c->nl.ref, n * sizeof(double4));
         alloc((void **)&c->nl.slot, n * 4);
         alloc((void **)&c->nl.order, n * 4);
-        alloc((void **)&c->nl.nbl, n * (size_t)kmax * 4);
+        alloc((void **)&c->nl.nbl, n * (size_t)kmax * 2);
         alloc((void **)&c->nl.ncnt, n * 4);
         alloc((void **)&c->nl.aux_cnt, (size_t)naux * 4);
+        alloc((void **)&c->nl.aux_n, (size_t)naux * 4);
         alloc((void **)&c->nl.aux_start, (size_t)naux * 4);
         alloc((void **)&c->nl.aux_mem, (size_t)naux * CM_NL_AUXCAP * 4);
+        alloc((void **)&c->nl.gstart, (size_t)(ngroups + 1) * 4);
+        alloc((void **)&c->nl.cell_order, (size_t)naux * 4);
+        alloc((void **)&c->nl.group_q0, (size_t)(ngroups + 1) * 4);
         alloc((void **)&c->nl.nonempty, (size_t)c->ncells * 4);
         alloc((void **)&c->nl.count, 16);
         alloc((void **)&c->nl.flags, 16);
@@ -760,6 +824,7 @@ static int nl_setup(comd_ctx *c, long nids)
         c->nl_alloc_n = nids;
         c->nl_alloc_k = kmax;
         c->nl_alloc_aux = naux;
+        c->nl_alloc_groups = ngroups;
     }
     CmList &l = c->nl;
     l.n = (int)nids;
@@ -781,6 +846,41 @@ static int nl_setup(comd_ctx *c, long nids)
     l.iL0 = 1.0 / l.L0;
     l.iL1 = 1.0 / l.L1;
     l.iL2 = 1.0 / l.L2;
+    l.gd0 = gd[0];
+    l.gd1 = gd[1];
+    l.gd2 = gd[2];
+    l.ng0 = ng[0];
+    l.ng1 = ng[1];
+    l.ng2 = ng[2];
+    l.ngroups = (int)ngroups;
+    l.hcap = (int)hcap;
+    l.lds_avail = (int)lds_avail;
+    l.rows_lds = c->nl_rows_lds;
+    l.build_blocks = (int)std::min<long>(ngroups, (long)c->ncu * 2);
+    l.pass_blocks = (int)ngroups;
+    // threads per pass block: a group's home atoms with 10 % to spare
+    l.pass_threads = (int)std::min<long>(512, std::max<long>(64, ((long)(1.1 * occ * gd[0] * gd[1] * gd[2]) + 63) / 64 * 64));
+    {
+        // auxiliary cells in build order: groups (x fastest), then the
+        // group's cells (x fastest)
+        std::vector<int> corder((size_t)naux), gq0((size_t)ngroups + 1);
+        long q = 0, g = 0;
+        for (int gz = 0; gz < ng[2]; gz++)
+            for (int gy = 0; gy < ng[1]; gy++)
+                for (int gx = 0; gx < ng[0]; gx++) {
+                    gq0[g++] = (int)q;
+                    for (int z = gz * gd[2]; z < std::min(na[2], (gz + 1) * gd[2]); z++)
+                        for (int y = gy * gd[1]; y < std::min(na[1], (gy + 1) * gd[1]); y++)
+                            for (int x = gx * gd[0]; x < std::min(na[0], (gx + 1) * gd[0]); x++)
+                                corder[q++] = x + na[0] * (y + na[1] * z);
+                }
+        gq0[g] = (int)q;
+        HIPCHK(c, hipMemcpyAsync((void *)l.cell_order, corder.data(), corder.size() * 4, hipMemcpyHostToDevice,
+                                 c->stream));
+        HIPCHK(c, hipMemcpyAsync((void *)l.group_q0, gq0.data(), gq0.size() * 4, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    HIPCHK(c, hipMemsetAsync(l.nbl, 0, (size_t)nids * (size_t)kmax * 2, c->stream));
     const double two32 = 4294967296.0;
     l.qs0 = two32 / l.L0;
     l.qs1 = two32 / l.L1;
@@ -1137,10 +1237,11 @@ int comd_set_precision(comd_ctx *c, int precision)
 {
     if (!c || (precision != COMD_PREC_FP64 && precision != COMD_PREC_MIXED)) return COMD_ERR_ARG;
     if (precision != c->precision) invalidate_graph(c);
-    // FP64 steps keep only the double list positions current
-    if (precision == COMD_PREC_MIXED && c->precision != COMD_PREC_MIXED && c->nl_active) c->nl_stale = 1;
+    const int old = c->precision;
     c->precision = precision;
     for (Replica &R : c->reps) R.prm.precision = precision;
+    // the list groups depend on the precision (halo atom width): rebuild
+    if (precision != old && c->nl_active) return nl_setup(c, c->nl_ids);
     return COMD_OK;
 }
 
@@ -1201,6 +1302,16 @@ int comd_tune(comd_ctx *c, int key, int value)
             if (R.prm.potential == COMD_POT_LJ) R.pot.lj_fsign = value ? -1.0 : 1.0;
         invalidate_graph(c);
         return upload_descriptors(c);
+    case COMD_TUNE_LIST_GROUPS:
+        if (value < -1 || value > 3) return set_err(c, COMD_ERR_ARG, "list group option must be -1..3");
+        c->nl_group_opt = value;
+        if (c->have_atoms) return nl_setup(c, c->nl_ids);
+        break;
+    case COMD_TUNE_LIST_ROWS_LDS:
+        if (value < -1 || value > 1) return set_err(c, COMD_ERR_ARG, "list table rows option must be -1..1");
+        c->nl_rows_lds = value;
+        c->nl.rows_lds = value;
+        break;
     case COMD_TUNE_SKIN_MILLI:
         if (value < 0 || value > 3000) return set_err(c, COMD_ERR_ARG, "skin must be 0..3000 (1e-3 A)");
         c->nl_skin_milli = value;

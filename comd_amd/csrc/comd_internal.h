@@ -19,6 +19,11 @@ struct PotDev {
     const double *cf64[2];
     const float *cf32[2];
     int same_grid;  // phi and rho tables share x0, xn, invDx and n
+    // compact rows for the neighbour-list passes (only when same_grid):
+    // density pass (phi d0 d1 c0 c1 c2, rho c0 c1 c2) per interval,
+    // embedding-force pass (rho d0 d1) per interval
+    const double *cd64, *cr64;
+    const float *cd32, *cr32;
     int balanced;   // drain an atom's queues round-robin over its lanes
     // Chebyshev: 6 series of cn coefficients (phi rho F dphi drho dF)
     const double *cheb;
@@ -117,33 +122,54 @@ void cm_launch_adj_stage(const DevState &d, const PotDev &P, const CmAdjoint &J,
                          hipStream_t st);
 
 // Verlet neighbour lists of a single periodic simulation (nlist.hip): per
-// atom the ids of all atoms within cutoff + skin at the last build, rebuilt
-// on the device whenever an atom has moved more than half the skin.
+// atom all atoms within cutoff + skin at the last build, rebuilt on the
+// device whenever an atom has moved more than half the skin.
 struct CmList {
     int n;                   // id capacity (ids 0 .. n-1)
-    int kmax;                // list capacity per atom
+    int kmax;                // list capacity per atom (a multiple of 16)
     double rl2;              // (cutoff + skin)^2
     double L0, L1, L2;       // bounds
     int na0, na1, na2, naux; // auxiliary grid used to build the lists
     double ab0, ab1, ab2;    // its cell edges (>= cutoff + skin)
     double iab0, iab1, iab2; // and their inverses
     double iL0, iL1, iL2;    // 1 / L
+    // groups of gd0 x gd1 x gd2 auxiliary cells (fewer at the upper edges),
+    // ng0 x ng1 x ng2 of them; the build order is group-major
+    int gd0, gd1, gd2, ng0, ng1, ng2, ngroups;
+    int hcap;                // staged halo capacity per group (atoms, <= 65535)
+    int lds_avail;           // dynamic LDS a list kernel's block may use
+    int rows_lds;            // table rows in LDS: 0 never, else whenever they fit
+    int build_blocks, pass_blocks, pass_threads;
+    const int *cell_order;   // [naux]: auxiliary cells in build order (group, then cell)
+    const int *group_q0;     // [ngroups + 1]: a group's first position in cell_order
     double4 *pos, *ref;      // per id (pos.w: F' of the last density pass)
     uint4 *pos32;            // per id: pos in fixed32 (x, y, z) and, in .w, the float bits of
                              // F' of the last mixed-precision density pass
     double qs0, qs1, qs2;    // 2^32 / L (encode)
     double qi0, qi1, qi2;    // L / 2^32 (decode)
     int *slot;               // per id: current slot, -1 when absent
-    int *order;              // atoms in build order
-    int *nbl;                // [k * n + t]: neighbour ids of the t-th atom in build order
+    int *order;              // atoms (ids) in build order
+    uint16_t *nbl;           // list of the t-th atom in build order: indices into its group's
+                             // staged halo, entry k at nl_entry(k, t, n) (groups of 8 entries
+                             // per atom, consecutive over t: one 16-byte load per lane fetches
+                             // 8 entries, and a wave's loads are contiguous)
     int *ncnt;               // per t
-    int *aux_cnt, *aux_start, *aux_mem;
+    int *aux_cnt;            // per auxiliary cell: atoms keyed (cleared after each build)
+    int *aux_n;              // the same, capped at CM_NL_AUXCAP, kept until the next build
+    int *aux_start;          // first atom of each auxiliary cell in build order
+    int *aux_mem;            // [cell][CM_NL_AUXCAP] member ids
+    int *gstart;             // [ngroups + 1]: first atom of each group in build order
     int *nonempty;           // per link cell: non-empty stencil cells (Lennard-Jones virial)
     int *count;              // [0]: atoms in build order, [1]: builds so far, [2]: build blocks done
     unsigned *flags;         // CM_NL_REBUILD, CM_NL_INVALID, CM_NL_OVERFLOW (during a build)
 };
 #define CM_NL_AUXCAP 64
+#define CM_NL_SLOTS 144                 // halo cells of a group, at most (4 + 2) x (4 + 2) x (2 + 2)
+#define CM_NL_STATIC_LDS (6 << 10)      // static LDS of the list kernels, rounded up
+__host__ __device__ inline size_t nl_entry(int k, int t, int n) { return ((size_t)(k >> 3) * n + t) * 8 + (k & 7); }
 void cm_nl_init(const DevState &d, const CmList &l, hipStream_t st);
 void cm_nl_build(const DevState &d, const CmList &l, hipStream_t st);
 void cm_nl_sync(const DevState &d, const CmList &l, hipStream_t st);
+size_t cm_nl_build_lds(const CmList &l);
+size_t cm_nl_pass_lds(const PotDev &P, const CmList &l, int kind, int mixed, int *tables_in_lds);
 void cm_nl_force(const DevState &d, const PotDev &P, const CmList &l, int kind, int mixed, hipStream_t st);

@@ -2,28 +2,55 @@
 // periodic simulation.
 //
 // The link-cell kernels test ~27 x 10.5 candidates per atom and pass in every
-// force call.  Here each atom keeps the ids of all atoms that were within
-// cutoff + skin when the lists were last built; the force passes test only
-// those (about 55 per atom for Cu with a 0.4 A skin).  Every pair inside the
-// cutoff is on the lists as long as no atom has moved more than half the skin
-// since the build; the drift kernel checks that per atom and raises a device
-// flag, and the build kernels, launched every step, return at once unless the
-// flag is up.  Nothing waits on the host, so a step stays capturable.
+// force call.  Here each atom keeps all atoms that were within cutoff + skin
+// when the lists were last built; the force passes test only those (about
+// 55 per atom for Cu with a 0.4 A skin).  Every pair inside the cutoff is on
+// the lists as long as no atom has moved more than half the skin since the
+// build; the drift kernel checks that per atom and raises a device flag, and
+// the build kernels, launched every step, return at once unless the flag is
+// up.  Nothing waits on the host, so a step stays capturable.
 //
-// Lists are keyed by atom id, so the rebinning (which moves atoms between
-// slots with the reference's swap-with-last order, src-flat/domains.c:
-// 347-389) does not invalidate them: the rebin kernels keep an id -> slot map
-// and the drift kernel an id -> absolute position map.  Outputs go to the
-// slot arrays exactly as the link-cell kernels write them, with the same
-// cutoff predicates (src-flat/eam.c:519,633, ljForce.c:188); only the order
-// of the per-atom sums differs.  Positions are absolute (cell corner + r),
-// pairs are imaged by the nearest periodic image, so the lists need at
-// least three auxiliary cells of edge cutoff + skin per dimension; the
-// host falls back to the link-cell kernels otherwise.
+// Atoms are binned into auxiliary cells of edge >= cutoff + skin, and the
+// cells into groups of up to 4 x 4 x 2.  The build order (t) is group-major,
+// then cell, then atom id.  A force pass stages a group's halo -- the atoms
+// of its cells and of the cells around them -- in LDS once, and each home
+// atom's list holds 16-bit indices into that halo: the neighbour positions
+// are LDS reads, not gathers through the vector L1, and the list stream is
+// half as wide as with atom ids.
+//
+// Lists refer to atoms through the build order and an id -> position map
+// kept by the drift kernel, so the rebinning (which moves atoms between slots
+// with the reference's swap-with-last order, src-flat/domains.c:347-389) does
+// not invalidate them; the rebin kernels keep an id -> slot map.  Outputs go
+// to the slot arrays exactly as the link-cell kernels write them, with the
+// same cutoff predicates (src-flat/eam.c:519,633, ljForce.c:188); only the
+// order of the per-atom sums differs.  Positions are absolute (cell corner +
+// r) and pairs are imaged by the nearest periodic image, so the lists need at
+// least three auxiliary cells per dimension; the host falls back to the
+// link-cell kernels otherwise.
 #include "comd_internal.h"
 #include "pair_terms.h"
 
 __device__ __forceinline__ bool nl_rebuild(const CmList &l) { return (l.flags[0] & CM_NL_REBUILD) != 0; }
+
+// x wrapped into [0, L) (to rounding); iL = 1 / L
+__device__ __forceinline__ double wrap_coord(double x, double L, double iL) { return x - L * floor(x * iL); }
+
+// Auxiliary cell of a position (the same arithmetic wherever it is needed,
+// so that the key and build kernels agree).
+__device__ __forceinline__ void aux_coords(const CmList &l, const double4 &p, int &ax, int &ay, int &az)
+{
+    ax = min(max((int)(wrap_coord(p.x, l.L0, l.iL0) * l.iab0), 0), l.na0 - 1);
+    ay = min(max((int)(wrap_coord(p.y, l.L1, l.iL1) * l.iab1), 0), l.na1 - 1);
+    az = min(max((int)(wrap_coord(p.z, l.L2, l.iL2) * l.iab2), 0), l.na2 - 1);
+}
+
+__device__ __forceinline__ void nl_wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // ---------------------------------------------------------------------------
 // id -> slot and id -> position from the slot arrays (after binning or an
@@ -69,19 +96,118 @@ __global__ __launch_bounds__(64) void k_nl_from_slots(DevState d, CmList l, int 
 }
 
 // ---------------------------------------------------------------------------
+// groups of auxiliary cells and their halos
+
+struct GroupGeo {
+    int x0, y0, z0;  // first cell
+    int lx, ly, lz;  // cells per dimension (fewer at the upper edges)
+};
+
+__device__ __forceinline__ GroupGeo group_geo(const CmList &l, int g)
+{
+    GroupGeo G;
+    const int gx = g % l.ng0, gy = (g / l.ng0) % l.ng1, gz = g / (l.ng0 * l.ng1);
+    G.x0 = gx * l.gd0;
+    G.y0 = gy * l.gd1;
+    G.z0 = gz * l.gd2;
+    G.lx = min(l.gd0, l.na0 - G.x0);
+    G.ly = min(l.gd1, l.na1 - G.y0);
+    G.lz = min(l.gd2, l.na2 - G.z0);
+    return G;
+}
+
+// Halo slot s (x fastest over (lx + 2) x (ly + 2) x (lz + 2)): its cell and
+// the image shift that brings the cell next to the group.
+__device__ __forceinline__ int halo_cell(const CmList &l, const GroupGeo &G, int s, double &sx, double &sy,
+                                         double &sz)
+{
+    const int hx = G.lx + 2, hy = G.ly + 2;
+    int jx = G.x0 - 1 + s % hx, jy = G.y0 - 1 + (s / hx) % hy, jz = G.z0 - 1 + s / (hx * hy);
+    sx = sy = sz = 0.0;
+    if (jx < 0) { jx += l.na0; sx = -l.L0; } else if (jx >= l.na0) { jx -= l.na0; sx = l.L0; }
+    if (jy < 0) { jy += l.na1; sy = -l.L1; } else if (jy >= l.na1) { jy -= l.na1; sy = l.L1; }
+    if (jz < 0) { jz += l.na2; sz = -l.L2; } else if (jz >= l.na2) { jz -= l.na2; sz = l.L2; }
+    return jx + l.na0 * (jy + l.na1 * jz);
+}
+
+// A group's halo table in LDS: per slot its first atom in build order;
+// off[] the slots' offsets in the staged halo (off[ns] = halo atoms).
+struct GroupLds {
+    int start[CM_NL_SLOTS], off[CM_NL_SLOTS + 1];
+    int ns, t0, t1;
+};
+
+// Fills S (and, when shift is given, the slots' image shifts) for group g
+// (all threads of the block; starts and ends with a barrier).
+__device__ __forceinline__ void group_setup(const CmList &l, int g, GroupLds &S, double (*shift)[3])
+{
+    __syncthreads();
+    const GroupGeo G = group_geo(l, g);
+    const int ns = (G.lx + 2) * (G.ly + 2) * (G.lz + 2);
+    for (int s = threadIdx.x; s < ns; s += blockDim.x) {
+        double sx, sy, sz;
+        const int c = halo_cell(l, G, s, sx, sy, sz);
+        S.start[s] = l.aux_start[c];
+        S.off[s + 1] = l.aux_n[c];  // counts, scanned below
+        if (shift) {
+            shift[s][0] = sx;
+            shift[s][1] = sy;
+            shift[s][2] = sz;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const int L = threadIdx.x;
+        int carry = 0;
+        for (int b = 0; b < ns; b += 64) {
+            const int v = b + L < ns ? S.off[b + L + 1] : 0;
+            int incl = v;
+            for (int o = 1; o < 64; o <<= 1) {
+                const int u = __shfl_up(incl, o, 64);
+                if (L >= o) incl += u;
+            }
+            if (b + L < ns) S.off[b + L + 1] = carry + incl;
+            carry += __shfl(incl, 63, 64);
+        }
+        if (L == 0) {
+            S.off[0] = 0;
+            S.ns = ns;
+            S.t0 = l.gstart[g];
+            S.t1 = l.gstart[g + 1];
+        }
+    }
+    __syncthreads();
+}
+
+// Slot of staged halo entry h (off[] ascending, 0 <= h < off[ns]).
+__device__ __forceinline__ int halo_slot(const GroupLds &S, int h)
+{
+    int lo = 0, hi = S.ns;  // off[lo] <= h < off[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (S.off[mid] <= h) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Groups of this block: blocks are dealt round-robin over the 8 XCDs; each
+// XCD takes a contiguous range of groups (a slab of space), so that the
+// halos it stages stay in its own L2.
+__device__ __forceinline__ void block_groups(int ngroups, int &g0, int &g1)
+{
+    const int nb = gridDim.x, q = nb / 8, rem = nb % 8;
+    const int xcd = blockIdx.x % 8, idx = blockIdx.x / 8;
+    const int blk = nb < 8 ? (int)blockIdx.x : xcd * q + min(xcd, rem) + idx;
+    const int per = (ngroups + nb - 1) / nb;
+    g0 = min(blk * per, ngroups);
+    g1 = min(g0 + per, ngroups);
+}
+
+// ---------------------------------------------------------------------------
 // build (each kernel returns at once unless a rebuild was requested, so the
 // launches that find no request cost only their dispatch: their grids are
 // kept small, with strided loops)
-
-// x wrapped into [0, L) (to rounding); iL = 1 / L
-__device__ __forceinline__ double wrap_coord(double x, double L, double iL) { return x - L * floor(x * iL); }
-
-__device__ __forceinline__ void nl_wave_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // Each atom into its auxiliary cell (counts start at zero: the previous
 // build's last block cleared them).
@@ -90,13 +216,8 @@ __global__ __launch_bounds__(256) void k_nl_key(CmList l)
     if (!nl_rebuild(l)) return;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < l.n; i += gridDim.x * blockDim.x) {
         if (l.slot[i] < 0) continue;
-        const double4 p = l.pos[i];
-        int ax = (int)(wrap_coord(p.x, l.L0, l.iL0) * l.iab0);
-        int ay = (int)(wrap_coord(p.y, l.L1, l.iL1) * l.iab1);
-        int az = (int)(wrap_coord(p.z, l.L2, l.iL2) * l.iab2);
-        ax = min(max(ax, 0), l.na0 - 1);
-        ay = min(max(ay, 0), l.na1 - 1);
-        az = min(max(az, 0), l.na2 - 1);
+        int ax, ay, az;
+        aux_coords(l, l.pos[i], ax, ay, az);
         const int a = ax + l.na0 * (ay + l.na1 * az);
         const int k = atomicAdd(&l.aux_cnt[a], 1);
         if (k < CM_NL_AUXCAP) l.aux_mem[a * CM_NL_AUXCAP + k] = i;
@@ -104,8 +225,9 @@ __global__ __launch_bounds__(256) void k_nl_key(CmList l)
     }
 }
 
-// Exclusive prefix sum of the auxiliary cells' counts, one block walking
-// tiles of 8 x 1024 cells; count[0] = atoms listed.
+// Exclusive prefix sum of the auxiliary cells' counts in group-major order
+// (cell_order), one block walking tiles of 8 x 1024 cells; the capped counts
+// (aux_n), each group's first atom (gstart) and count[0] = atoms listed.
 __global__ __launch_bounds__(1024) void k_nl_scan(CmList l)
 {
     if (!nl_rebuild(l)) return;
@@ -117,8 +239,8 @@ __global__ __launch_bounds__(1024) void k_nl_scan(CmList l)
         int v[8], s = 0;
 #pragma unroll
         for (int k = 0; k < 8; k++) {
-            const int a = base + 8 * t + k;
-            v[k] = a < l.naux ? min(l.aux_cnt[a], CM_NL_AUXCAP) : 0;
+            const int q = base + 8 * t + k;
+            v[k] = q < l.naux ? min(l.aux_cnt[l.cell_order[q]], CM_NL_AUXCAP) : 0;
             s += v[k];
         }
         // inclusive scan of the threads' sums: within the wave, then across
@@ -142,18 +264,26 @@ __global__ __launch_bounds__(1024) void k_nl_scan(CmList l)
         int run = carry + wsum[w] + incl - s;
 #pragma unroll
         for (int k = 0; k < 8; k++) {
-            const int a = base + 8 * t + k;
-            if (a < l.naux) l.aux_start[a] = run;
+            const int q = base + 8 * t + k;
+            if (q < l.naux) {
+                const int a = l.cell_order[q];
+                l.aux_start[a] = run;
+                l.aux_n[a] = v[k];
+            }
             run += v[k];
         }
         carry += carry_s;
         __syncthreads();
     }
-    if (t == 0) l.count[0] = carry;
+    for (int g = t; g < l.ngroups; g += blockDim.x) l.gstart[g] = l.aux_start[l.cell_order[l.group_q0[g]]];
+    if (t == 0) {
+        l.count[0] = carry;
+        l.gstart[l.ngroups] = carry;
+    }
 }
 
-// Members of each auxiliary cell in id order (deterministic lists), and the
-// build order.  One wave per cell, four waves per block.
+// Members of each auxiliary cell in id order (deterministic lists), placed in
+// the build order.  One wave per cell, four waves per block.
 __global__ __launch_bounds__(256) void k_nl_sort(CmList l)
 {
     if (!nl_rebuild(l)) return;
@@ -161,7 +291,7 @@ __global__ __launch_bounds__(256) void k_nl_sort(CmList l)
     const int L = threadIdx.x & 63, w = threadIdx.x >> 6;
     int *mem = mem_s[w];
     for (int a = blockIdx.x * 4 + w; a < l.naux; a += gridDim.x * 4) {
-        const int n = min(l.aux_cnt[a], CM_NL_AUXCAP);
+        const int n = l.aux_n[a];
         if (n == 0) continue;
         const int me = L < n ? l.aux_mem[a * CM_NL_AUXCAP + L] : 0;
         if (L < n) mem[L] = me;
@@ -169,129 +299,69 @@ __global__ __launch_bounds__(256) void k_nl_sort(CmList l)
         int rank = 0;
         for (int u = 0; u < n; u++) rank += mem[u] < me;
         nl_wave_sync();
-        if (L < n) {
-            l.aux_mem[a * CM_NL_AUXCAP + rank] = me;
-            l.order[l.aux_start[a] + rank] = me;
-        }
+        if (L < n) l.order[l.aux_start[a] + rank] = me;
     }
 }
 
-// One wave per auxiliary cell: its 27 neighbours' atoms are staged in LDS at
-// their nearest periodic image.  The nh home atoms are split over the lanes
-// P = 64 / nh ways: lane (a, q) tests halo entries q, q + P, ...; a count
-// pass and a prefix over the P lanes of an atom place each lane's entries,
-// so an atom's list holds, per staged batch, partition 0's neighbours, then
-// partition 1's, ... (a fixed order for a given build).  The last block to
-// finish closes the build: overflow -> lists invalid, rebuild request
-// cleared, counts cleared for the next build.  About 8 KB of LDS per wave,
-// so that enough waves are resident to hide the gathers' latency.
-#define NL_BATCH 256   // halo atoms staged per batch and wave
+// One block per group (persistent over groups): the halo is staged in LDS at
+// its image next to the group; each home atom (a thread) tests the atoms of
+// its own 27-cell stencil, in stencil order (x outer, as
+// src-flat/domains.c:419-426) and id order within a cell, and lists the
+// halo indices within cutoff + skin.  The last block to finish closes the
+// build: overflow (a list past kmax, a halo past hcap) -> lists invalid,
+// rebuild request cleared, counts cleared for the next build.
 __global__ __launch_bounds__(256) void k_nl_build(CmList l)
 {
     if (!nl_rebuild(l)) return;
-    struct WaveBuf {
-        double hx[NL_BATCH], hy[NL_BATCH], hz[NL_BATCH];
-        int hid[NL_BATCH];
-        double shift[81];
-        int cell[27], off[28];
-    };
-    __shared__ WaveBuf buf[4];
+    extern __shared__ double hl[];  // hx[hcap], hy[hcap], hz[hcap]
+    __shared__ GroupLds S;
+    __shared__ double shift[CM_NL_SLOTS][3];
     __shared__ int last;
-    const int L = threadIdx.x & 63, w = threadIdx.x >> 6;
-    WaveBuf &B = buf[w];
-    // waves walk contiguous ranges of cells, and the blocks of one XCD
-    // (blocks are dealt round-robin over the 8 XCDs) neighbouring ranges
-    const int nb = gridDim.x, qb = nb / 8, rb = nb % 8;
-    const int xcd = blockIdx.x % 8, bi = blockIdx.x / 8;
-    const int blk = nb < 8 ? (int)blockIdx.x : xcd * qb + min(xcd, rb) + bi;
-    const int nw = nb * 4, gw = blk * 4 + w;
-    const int per = (l.naux + nw - 1) / nw;
-    const int a0 = gw * per, a1 = min(a0 + per, l.naux);
+    double *hx = hl, *hy = hl + l.hcap, *hz = hl + 2 * l.hcap;
     bool ovf = false;
-    for (int a = a0; a < a1; a++) {
-        const int nh = min(l.aux_cnt[a], CM_NL_AUXCAP);
-        if (nh == 0) continue;
-        const int ax = a % l.na0, ay = (a / l.na0) % l.na1, az = a / (l.na0 * l.na1);
-        nl_wave_sync();
-        int ne = 0;
-        if (L < 27) {
-            int jx = ax + L / 9 - 1, jy = ay + (L / 3) % 3 - 1, jz = az + L % 3 - 1;
-            double sx = 0.0, sy = 0.0, sz = 0.0;
-            if (jx < 0) { jx += l.na0; sx = -l.L0; } else if (jx >= l.na0) { jx -= l.na0; sx = l.L0; }
-            if (jy < 0) { jy += l.na1; sy = -l.L1; } else if (jy >= l.na1) { jy -= l.na1; sy = l.L1; }
-            if (jz < 0) { jz += l.na2; sz = -l.L2; } else if (jz >= l.na2) { jz -= l.na2; sz = l.L2; }
-            const int b = jx + l.na0 * (jy + l.na1 * jz);
-            B.cell[L] = b;
-            ne = min(l.aux_cnt[b], CM_NL_AUXCAP);
-            B.shift[3 * L + 0] = sx;
-            B.shift[3 * L + 1] = sy;
-            B.shift[3 * L + 2] = sz;
+    int g0, g1;
+    block_groups(l.ngroups, g0, g1);
+    for (int g = g0; g < g1; g++) {
+        group_setup(l, g, S, shift);
+        const int H = S.off[S.ns];
+        if (H > l.hcap) {  // the same in every thread of the block
+            ovf = true;
+            continue;
         }
-        int incl = ne;
-        for (int o = 1; o < 32; o <<= 1) {
-            const int v = __shfl_up(incl, o, 64);
-            if (L >= o) incl += v;
+        for (int h = threadIdx.x; h < H; h += blockDim.x) {
+            const int s = halo_slot(S, h);
+            const double4 p = l.pos[l.order[S.start[s] + (h - S.off[s])]];
+            hx[h] = wrap_coord(p.x, l.L0, l.iL0) + shift[s][0];
+            hy[h] = wrap_coord(p.y, l.L1, l.iL1) + shift[s][1];
+            hz[h] = wrap_coord(p.z, l.L2, l.iL2) + shift[s][2];
         }
-        const int H = __shfl(incl, 31, 64);
-        if (L < 27) B.off[L] = incl - ne;
-        if (L == 0) B.off[27] = H;
-        // lane (ha, q): home atom ha, partition q of P
-        const int P = 64 / nh;
-        const bool act = L < P * nh;
-        const int ha = act ? L % nh : 0, q = act ? L / nh : 0;
-        const int me = act ? l.aux_mem[a * CM_NL_AUXCAP + ha] : -1;
-        double xi = 0, yi = 0, zi = 0;
-        if (act) {
-            const double4 p = l.pos[me];
-            xi = wrap_coord(p.x, l.L0, l.iL0);
-            yi = wrap_coord(p.y, l.L1, l.iL1);
-            zi = wrap_coord(p.z, l.L2, l.iL2);
-        }
-        const int t = l.aux_start[a] + ha;
-        int base = 0;  // entries of atom ha so far (the same in its P lanes)
-        for (int h0 = 0; h0 < H; h0 += NL_BATCH) {
-            const int hb = min(H - h0, NL_BATCH);
-            nl_wave_sync();
-            int e = 0;
-            for (int h = h0 + L; h < h0 + hb; h += 64) {
-                while (B.off[e + 1] <= h) e++;
-                const int j = l.aux_mem[B.cell[e] * CM_NL_AUXCAP + (h - B.off[e])];
-                const double4 p = l.pos[j];
-                B.hx[h - h0] = wrap_coord(p.x, l.L0, l.iL0) + B.shift[3 * e + 0];
-                B.hy[h - h0] = wrap_coord(p.y, l.L1, l.iL1) + B.shift[3 * e + 1];
-                B.hz[h - h0] = wrap_coord(p.z, l.L2, l.iL2) + B.shift[3 * e + 2];
-                B.hid[h - h0] = j;
-            }
-            nl_wave_sync();
-            int c = 0;
-            if (act) {
-                for (int j = q; j < hb; j += P) {
-                    const double dx = xi - B.hx[j], dy = yi - B.hy[j], dz = zi - B.hz[j];
-                    c += (dx * dx + dy * dy + dz * dz <= l.rl2) && B.hid[j] != me;
-                }
-            }
-            int pre = 0, tot = 0;
-            for (int qq = 0; qq < P; qq++) {
-                const int v = __shfl(c, qq * nh + ha, 64);
-                pre += qq < q ? v : 0;
-                tot += v;
-            }
-            if (act) {
-                int k = base + pre;
-                for (int j = q; j < hb; j += P) {
-                    const double dx = xi - B.hx[j], dy = yi - B.hy[j], dz = zi - B.hz[j];
-                    if ((dx * dx + dy * dy + dz * dz <= l.rl2) && B.hid[j] != me) {
-                        if (k < l.kmax) l.nbl[(size_t)k * l.n + t] = B.hid[j];
+        __syncthreads();
+        const GroupGeo G = group_geo(l, g);
+        const int shx = G.lx + 2, shy = G.ly + 2;
+        for (int t = S.t0 + threadIdx.x; t < S.t1; t += blockDim.x) {
+            const int id = l.order[t];
+            const double4 p = l.pos[id];
+            int ax, ay, az;
+            aux_coords(l, p, ax, ay, az);
+            // home slot: the atom's own cell, an interior slot of the halo
+            const int cx = ax - G.x0 + 1, cy = ay - G.y0 + 1, cz = az - G.z0 + 1;
+            const int hs = cx + shx * (cy + shy * cz);
+            const int self = S.off[hs] + (t - S.start[hs]);
+            const double xi = hx[self], yi = hy[self], zi = hz[self];
+            int k = 0;
+            for (int e = 0; e < 27; e++) {
+                const int s = (cx + e / 9 - 1) + shx * ((cy + (e / 3) % 3 - 1) + shy * (cz + e % 3 - 1));
+                for (int h = S.off[s]; h < S.off[s + 1]; h++) {
+                    const double dx = xi - hx[h], dy = yi - hy[h], dz = zi - hz[h];
+                    if (dx * dx + dy * dy + dz * dz <= l.rl2 && h != self) {
+                        if (k < l.kmax) l.nbl[nl_entry(k, t, l.n)] = (uint16_t)h;
                         k++;
                     }
                 }
             }
-            base += tot;
-        }
-        if (act && q == 0) {
-            l.ncnt[t] = min(base, l.kmax);
-            ovf |= base > l.kmax;
-            l.ref[me] = l.pos[me];
+            l.ncnt[t] = min(k, l.kmax);
+            ovf |= k > l.kmax;
+            l.ref[id] = p;
         }
     }
     if (ovf) atomicOr(l.flags, (unsigned)CM_NL_OVERFLOW);
@@ -334,30 +404,37 @@ __global__ void k_nl_nonempty(DevState d, CmList l)
     l.nonempty[c] = n;
 }
 
-// Rows of the phi and rho coefficient tables a pass reads (phi: density
-// pass only), staged per block in LDS when they fit: every pair reads a
-// different interval, and from LDS these reads do not load the vector L1.
-__host__ __device__ inline int nl_rows(const PotDev &P, int kind) { return (kind == CM_KIND_EAM1 ? P.n[0] : 0) + P.n[1]; }
+// Compact table rows a pass reads (density: phi and rho, 8 values per
+// interval; embedding force: rho', 2 values), staged per block in LDS: every
+// pair reads a different interval, and from LDS these reads do not load the
+// vector L1.
+static inline size_t nl_tab_bytes(const PotDev &P, int kind, int mixed)
+{
+    const size_t b = (size_t)P.n[0] * (kind == CM_KIND_EAM1 ? 8 : 2) * (mixed ? 4 : 8);
+    return (b + 15) / 16 * 16;  // the halo after the rows stays 16-byte aligned
+}
+// Staged halo bytes per atom: fixed32 x, y, z and float F' (mixed); double
+// x, y, z and, for the embedding-force pass, F' (FP64).
+static inline size_t nl_halo_bytes(int kind, int mixed) { return mixed ? 16 : (kind == CM_KIND_EAM3 ? 32 : 24); }
 
-// One atom (thread t of the build order) over its list.  TB: the table
-// pointers' type (global or LDS rows).
-template <int KIND, int EVAL, int MIXED, class TB>
-__device__ __forceinline__ void nl_atom(const DevState &d, const PotDev &P, const CmList &l, const TB &T, int t)
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef unsigned v4u __attribute__((ext_vector_type(4)));
+
+// One home atom t over its list.  TB: the table pointers (compact rows in
+// LDS when COMPACT, else the full tables in global memory); hq (mixed) or
+// hx, hy, hz, hf (FP64): the staged halo.
+template <int KIND, int EVAL, int MIXED, int COMPACT, class TB>
+__device__ __forceinline__ void nl_atom(const DevState &d, const PotDev &P, const CmList &l, const TB &T, int t,
+                                        const CM_LDS v4u *hq, const CM_LDS double *hx, const CM_LDS double *hy,
+                                        const CM_LDS double *hz, const CM_LDS double *hf)
 {
     const int i = l.order[t];
     const int s = l.slot[i];
-    // clang vector types: loadable through address-space-qualified pointers
-    typedef double v4d __attribute__((ext_vector_type(4)));
-    typedef unsigned v4u __attribute__((ext_vector_type(4)));
-    const CM_GLOBAL v4d *pos = (const CM_GLOBAL v4d *)l.pos;
-    const CM_GLOBAL v4u *pos32 = (const CM_GLOBAL v4u *)l.pos32;
-    // positions: FP64 passes gather double4 (x, y, z, F'), nearest image by
-    // rint; mixed passes gather 16-byte fixed32 (x, y, z, float F'), nearest
-    // image by 32-bit wrap-around.  F' is there once the density pass ran.
+    // the atom's own position and F', from the id maps
     v4d pi = {0, 0, 0, 0};
     v4u qi = {0, 0, 0, 0};
-    if (MIXED) qi = pos32[i];
-    else pi = pos[i];
+    if (MIXED) qi = ((const CM_GLOBAL v4u *)l.pos32)[i];
+    else pi = ((const CM_GLOBAL v4d *)l.pos)[i];
     const double fia = KIND != CM_KIND_EAM3 ? 0.0 : MIXED ? (double)__uint_as_float(qi.w) : pi.w;
     const double rc2 = P.rc2;
     const double L0 = l.L0, L1 = l.L1, L2 = l.L2;
@@ -365,43 +442,38 @@ __device__ __forceinline__ void nl_atom(const DevState &d, const PotDev &P, cons
     const int cnt = l.ncnt[t];
     Acc A = {0, 0, 0, 0, 0, 0};
     int npair = 0;
-    // NU neighbours per trip: their list entries, then their positions (and
-    // F'), are loaded together so that the loads' latencies overlap
-    constexpr int NU = 4;
-    const CM_GLOBAL int *row = (const CM_GLOBAL int *)l.nbl + t;
-    for (int k0 = 0; k0 < cnt; k0 += NU) {
-        int jj[NU];
+    // 8 entries per 16-byte load, then 8 halo reads from LDS
+    const CM_GLOBAL v4u *row = (const CM_GLOBAL v4u *)l.nbl + t;
+    for (int k0 = 0; k0 < cnt; k0 += 8) {
+        const v4u e = row[(size_t)(k0 >> 3) * l.n];
+        const unsigned ew[4] = {e.x, e.y, e.z, e.w};
 #pragma unroll
-        for (int u = 0; u < NU; u++) jj[u] = k0 + u < cnt ? row[(size_t)(k0 + u) * l.n] : i;
-        v4d pj[NU];
-        v4u qj[NU];
-#pragma unroll
-        for (int u = 0; u < NU; u++) {
-            if (MIXED) qj[u] = pos32[jj[u]];
-            else pj[u] = pos[jj[u]];
-        }
-#pragma unroll
-        for (int u = 0; u < NU; u++) {
+        for (int u = 0; u < 8; u++) {
+            // entries past cnt are not this build's: read entry 0 instead
+            // (valid entries are below the staged halo size, <= hcap)
+            const bool live = k0 + u < cnt;
+            const int hj = live ? (int)((ew[u >> 1] >> (16 * (u & 1))) & 0xffffu) : 0;
             double dx, dy, dz, fij;
             if (MIXED) {
-                dx = (double)(int)(qi.x - qj[u].x) * l.qi0;
-                dy = (double)(int)(qi.y - qj[u].y) * l.qi1;
-                dz = (double)(int)(qi.z - qj[u].z) * l.qi2;
-                fij = (double)__uint_as_float(qj[u].w);
+                const v4u qj = hq[hj];
+                dx = (double)(int)(qi.x - qj.x) * l.qi0;
+                dy = (double)(int)(qi.y - qj.y) * l.qi1;
+                dz = (double)(int)(qi.z - qj.z) * l.qi2;
+                fij = (double)__uint_as_float(qj.w);
             } else {
-                dx = pi.x - pj[u].x;
-                dy = pi.y - pj[u].y;
-                dz = pi.z - pj[u].z;
+                dx = pi.x - hx[hj];
+                dy = pi.y - hy[hj];
+                dz = pi.z - hz[hj];
                 dx -= L0 * rint(dx * iL0);
                 dy -= L1 * rint(dy * iL1);
                 dz -= L2 * rint(dz * iL2);
-                fij = pj[u].w;
+                fij = KIND == CM_KIND_EAM3 ? hf[hj] : 0.0;
             }
             const double r2 = dx * dx + dy * dy + dz * dz;
-            // a padding entry (k >= cnt) is the atom itself: r2 = 0
-            const bool in = r2 > 0.0 && (KIND == CM_KIND_EAM3 ? (r2 < rc2) : (r2 <= rc2));
+            const bool in = live && (KIND == CM_KIND_EAM3 ? (r2 < rc2) : (r2 <= rc2));
             if (in) {
-                pair_term<KIND, EVAL, MIXED>(P, T, dx, dy, dz, r2, fia, KIND == CM_KIND_EAM3 ? fij : 0.0, A);
+                if constexpr (COMPACT) pair_term_compact<KIND, MIXED>(P, T, dx, dy, dz, r2, fia, fij, A);
+                else pair_term<KIND, EVAL, MIXED>(P, T, dx, dy, dz, r2, fia, fij, A);
                 npair++;
             }
         }
@@ -440,43 +512,61 @@ __device__ __forceinline__ void nl_atom(const DevState &d, const PotDev &P, cons
     }
 }
 
-// TLDS: the table rows are staged in LDS (1) or read from global memory (0);
-// a compile-time choice, so the row reads are LDS or global loads.
+// A block per group: the halo is staged in LDS, then each thread takes home
+// atoms.  TLDS: the compact table rows are staged in LDS (1), or the full
+// tables are read from global memory (0); a compile-time choice, so the row
+// reads are LDS or global loads.  Dynamic LDS: [rows (TLDS)] [halo, hcap
+// atoms].
 template <int KIND, int EVAL, int MIXED, int TLDS>
-__global__ __launch_bounds__(512) void k_nl_pair(DevState d, PotDev P, CmList l)
+__global__ __launch_bounds__(512) void k_nl_pair(DevState d, PotDev P, CmList l, int tab_bytes)
 {
     extern __shared__ __align__(16) char nls[];
+    __shared__ GroupLds S;
     if (l.flags[0] & CM_NL_INVALID) return;
-    // blocks are dealt round-robin over the 8 XCDs; give each XCD a
-    // contiguous range of the build order (a slab of space) so that the
-    // neighbours it gathers stay in its own L2
-    const int nb = gridDim.x, q = nb / 8, rem = nb % 8;
-    const int xcd = blockIdx.x % 8, idx = blockIdx.x / 8;
-    const int blk = nb < 8 ? (int)blockIdx.x : xcd * q + min(xcd, rem) + idx;
-    const int t = blk * blockDim.x + threadIdx.x;
-    const bool live = t < l.count[0];
+    CM_LDS char *halo = (CM_LDS char *)nls + tab_bytes;
+    CM_LDS v4u *hq = (CM_LDS v4u *)halo;
+    CM_LDS double *hx = (CM_LDS double *)halo, *hy = hx + l.hcap, *hz = hy + l.hcap, *hf = hz + l.hcap;
+    CompactLds TL;
+    TL.d64 = TL.r64 = nullptr;
+    TL.d32 = TL.r32 = nullptr;
     if (TLDS) {
-        const int n0 = KIND == CM_KIND_EAM1 ? P.n[0] : 0, n1 = P.n[1];
-        TabsLds T;
+        const int nv = P.n[0] * (KIND == CM_KIND_EAM1 ? 8 : 2);
         if (MIXED) {
-            CM_LDS float *s0 = (CM_LDS float *)nls, *s1 = s0 + 8 * n0;
-            for (int k = threadIdx.x; k < 8 * n0; k += blockDim.x) s0[k] = P.cf32[0][k];
-            for (int k = threadIdx.x; k < 8 * n1; k += blockDim.x) s1[k] = P.cf32[1][k];
-            T.c32[0] = s0;
-            T.c32[1] = s1;
-            T.c64[0] = T.c64[1] = nullptr;
+            const float *src = KIND == CM_KIND_EAM1 ? P.cd32 : P.cr32;
+            CM_LDS float *dst = (CM_LDS float *)nls;
+            for (int k = threadIdx.x; k < nv; k += blockDim.x) dst[k] = src[k];
+            TL.d32 = TL.r32 = dst;
         } else {
-            CM_LDS double *s0 = (CM_LDS double *)nls, *s1 = s0 + 6 * n0;
-            for (int k = threadIdx.x; k < 6 * n0; k += blockDim.x) s0[k] = P.cf64[0][k];
-            for (int k = threadIdx.x; k < 6 * n1; k += blockDim.x) s1[k] = P.cf64[1][k];
-            T.c64[0] = s0;
-            T.c64[1] = s1;
-            T.c32[0] = T.c32[1] = nullptr;
+            const double *src = KIND == CM_KIND_EAM1 ? P.cd64 : P.cr64;
+            CM_LDS double *dst = (CM_LDS double *)nls;
+            for (int k = threadIdx.x; k < nv; k += blockDim.x) dst[k] = src[k];
+            TL.d64 = TL.r64 = dst;
+        }
+    }
+    const TabsGlobal TG = global_tabs(P);
+    // one group per block: the dispatcher balances the groups' uneven work
+    // over the CUs
+    for (int g = blockIdx.x; g < l.ngroups; g += gridDim.x) {
+        group_setup(l, g, S, nullptr);  // its first barrier also covers the table staging
+        const int H = min(S.off[S.ns], l.hcap);  // never above hcap while the lists are valid
+        for (int h = threadIdx.x; h < H; h += blockDim.x) {
+            const int s = halo_slot(S, h);
+            const int j = l.order[S.start[s] + (h - S.off[s])];
+            if (MIXED) {
+                hq[h] = ((const CM_GLOBAL v4u *)l.pos32)[j];
+            } else {
+                const v4d p = ((const CM_GLOBAL v4d *)l.pos)[j];
+                hx[h] = p.x;
+                hy[h] = p.y;
+                hz[h] = p.z;
+                if (KIND == CM_KIND_EAM3) hf[h] = p.w;
+            }
         }
         __syncthreads();
-        if (live) nl_atom<KIND, EVAL, MIXED>(d, P, l, T, t);
-    } else {
-        if (live) nl_atom<KIND, EVAL, MIXED>(d, P, l, global_tabs(P), t);
+        for (int t = S.t0 + threadIdx.x; t < S.t1; t += blockDim.x) {
+            if (TLDS) nl_atom<KIND, EVAL, MIXED, 1>(d, P, l, TL, t, hq, hx, hy, hz, hf);
+            else nl_atom<KIND, EVAL, MIXED, 0>(d, P, l, TG, t, hq, hx, hy, hz, hf);
+        }
     }
 }
 
@@ -500,28 +590,39 @@ void cm_nl_sync(const DevState &d, const CmList &l, hipStream_t st)
     hipLaunchKernelGGL(k_nl_from_slots, dim3(d.ncells), dim3(64), 0, st, d, l, 1);
 }
 
+size_t cm_nl_build_lds(const CmList &l) { return (size_t)l.hcap * 24; }
+
 void cm_nl_build(const DevState &d, const CmList &l, hipStream_t st)
 {
     (void)d;
     hipLaunchKernelGGL(k_nl_key, dim3(1024), dim3(256), 0, st, l);
     hipLaunchKernelGGL(k_nl_scan, dim3(1), dim3(1024), 0, st, l);
     hipLaunchKernelGGL(k_nl_sort, dim3(1024), dim3(256), 0, st, l);
-    hipLaunchKernelGGL(k_nl_build, dim3(1024), dim3(256), 0, st, l);
+    hipLaunchKernelGGL(k_nl_build, dim3(l.build_blocks), dim3(256), cm_nl_build_lds(l), st, l);
+}
+
+size_t cm_nl_pass_lds(const PotDev &P, const CmList &l, int kind, int mixed, int *tables_in_lds)
+{
+    const size_t halo = (size_t)l.hcap * nl_halo_bytes(kind, mixed);
+    size_t tab = 0;
+    if (P.eval == COMD_EVAL_TABLE && kind != CM_KIND_LJ && P.cd64) tab = nl_tab_bytes(P, kind, mixed);
+    // rows in LDS whenever they fit beside the halo (unless tuned off)
+    const int tl = tab > 0 && l.rows_lds != 0 && halo + tab <= (size_t)l.lds_avail;
+    if (tables_in_lds) *tables_in_lds = tl;
+    return halo + (tl ? tab : 0);
 }
 
 template <int KIND, int EVAL, int MIXED>
 static void nl_launch(const DevState &d, const PotDev &P, const CmList &l, hipStream_t st)
 {
-    size_t lds = 0;
-    if (EVAL == COMD_EVAL_TABLE && KIND != CM_KIND_LJ) {
-        lds = (size_t)nl_rows(P, KIND) * (MIXED ? 32 : 48);
-        if (lds > 65536) lds = 0;  // tables too large: read them from global memory
-    }
-    const dim3 grid(grid_for(l.n, 512)), block(512);
-    if (EVAL == COMD_EVAL_TABLE && KIND != CM_KIND_LJ && lds > 0)
-        hipLaunchKernelGGL((k_nl_pair<KIND, EVAL, MIXED, 1>), grid, block, lds, st, d, P, l);
+    int tl = 0;
+    const size_t lds = cm_nl_pass_lds(P, l, KIND, MIXED, &tl);
+    const int tab = tl ? (int)nl_tab_bytes(P, KIND, MIXED) : 0;
+    const dim3 grid(l.pass_blocks), block(l.pass_threads);
+    if (EVAL == COMD_EVAL_TABLE && KIND != CM_KIND_LJ && tl)
+        hipLaunchKernelGGL((k_nl_pair<KIND, EVAL, MIXED, 1>), grid, block, lds, st, d, P, l, tab);
     else
-        hipLaunchKernelGGL((k_nl_pair<KIND, EVAL, MIXED, 0>), grid, block, 0, st, d, P, l);
+        hipLaunchKernelGGL((k_nl_pair<KIND, EVAL, MIXED, 0>), grid, block, lds, st, d, P, l, 0);
 }
 
 template <int KIND, int EVAL>

@@ -80,6 +80,15 @@ __device__ __forceinline__ double rsqrt_mixed(double x)
     return fma(0.5 * y, e, y);
 }
 
+// Compact rows of the neighbour-list passes (PotDev::cd*, cr*), behind
+// pointers of a given address space.
+template <class PD, class PF> struct CompactT {
+    PD d64, r64;
+    PF d32, r32;
+};
+using CompactGlobal = CompactT<const CM_GLOBAL double *, const CM_GLOBAL float *>;
+using CompactLds = CompactT<const CM_LDS double *, const CM_LDS float *>;
+
 // ---------------------------------------------------------------------------
 // one accepted pair (home atom a, neighbour j): dx = r_a - r_j
 
@@ -203,4 +212,50 @@ __device__ __forceinline__ void pair_term(const PotDev &P, const TB &T, double d
 }
 
 // ---------------------------------------------------------------------------
+
+// The table branch of pair_term for EAM with phi and rho on one grid, from
+// the compact rows (the same formulas, src-flat/eam.c:401-434 and 521-545 /
+// 633-649).
+template <int KIND, int MIXED, class CT>
+__device__ __forceinline__ void pair_term_compact(const PotDev &P, const CT &T, double dx, double dy, double dz,
+                                                  double r2, double fia, double fij, Acc &A)
+{
+    const double y = MIXED ? rsqrt_mixed(r2) : rsqrt64(r2);  // 1/r
+    const double r = r2 * y;
+    double t;
+    const int i = table_interval(r, P.x0[0], P.xn[0], P.invdx[0], P.n[0], t);
+    if (KIND == CM_KIND_EAM1) {
+        double phiv, dphi, rhov;
+        if (MIXED) {
+            const float f = (float)t;
+            const float pd0 = T.d32[8 * i + 0], pd1 = T.d32[8 * i + 1], pc0 = T.d32[8 * i + 2],
+                        pc1 = T.d32[8 * i + 3], pc2 = T.d32[8 * i + 4], rc0 = T.d32[8 * i + 5],
+                        rc1 = T.d32[8 * i + 6], rc2 = T.d32[8 * i + 7];
+            phiv = fmaf(f, fmaf(f, pc2, pc1), pc0);
+            dphi = fmaf(f, pd1, pd0);
+            rhov = fmaf(f, fmaf(f, rc2, rc1), rc0);
+        } else {
+            const double pd0 = T.d64[8 * i + 0], pd1 = T.d64[8 * i + 1], pc0 = T.d64[8 * i + 2],
+                         pc1 = T.d64[8 * i + 3], pc2 = T.d64[8 * i + 4], rc0 = T.d64[8 * i + 5],
+                         rc1 = T.d64[8 * i + 6], rc2 = T.d64[8 * i + 7];
+            phiv = fma(t, fma(t, pc2, pc1), pc0);
+            dphi = fma(t, pd1, pd0);
+            rhov = fma(t, fma(t, rc2, rc1), rc0);
+        }
+        const double s = dphi * y;
+        A.x += s * dx;
+        A.y += s * dy;
+        A.z += s * dz;
+        A.e += 0.5 * phiv;
+        A.rho += rhov;
+    } else {
+        const double drho = MIXED ? (double)fmaf((float)t, T.r32[2 * i + 1], T.r32[2 * i])
+                                  : fma(t, T.r64[2 * i + 1], T.r64[2 * i]);
+        const double g = (fia + fij) * drho * y;
+        A.x += g * dx;
+        A.y += g * dy;
+        A.z += g * dz;
+        A.vir += g * dx * dx;
+    }
+}
 
