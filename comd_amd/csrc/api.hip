@@ -164,10 +164,6 @@ static DevState dev_state(const comd_ctx *c)
     d.nl_ref = nl ? c->nl.ref : nullptr;
     d.nl_slot = nl ? c->nl.slot : nullptr;
     d.nl_flags = nl ? c->nl.flags : nullptr;
-    d.nl_pos32 = nl && c->precision == COMD_PREC_MIXED ? c->nl.pos32 : nullptr;
-    d.nl_qs0 = c->nl.qs0;
-    d.nl_qs1 = c->nl.qs1;
-    d.nl_qs2 = c->nl.qs2;
     const double hs = 0.5e-3 * c->nl_skin_milli;
     d.nl_half2 = hs * hs;
     d.cell_gate = nl ? c->nl.flags : nullptr;
@@ -226,10 +222,7 @@ static void op_move(comd_ctx *c, double dtk, double dtd, int kick, int drift, in
     hipEvent_t ev = nullptr;
     tbegin(c, COMD_K_REBIN, &ev);
     DevState d = dev_state(c);
-    if (!nl_update) {
-        d.nl_pos = nullptr;
-        d.nl_pos32 = nullptr;
-    }
+    if (!nl_update) d.nl_pos = nullptr;
     cm_launch_move(d, c->stream, dtk, dtd, kick, drift, rebin);
     tend(c, COMD_K_REBIN, ev);
     if (drift && c->nl_active) c->nl_stale = nl_update ? 0 : 1;
@@ -385,7 +378,7 @@ int comd_grid_for(comd_params *p, const double bounds[3], double defgrad, double
 
 static void free_nl(comd_ctx *c)
 {
-    void *p[] = {c->nl.pos,      c->nl.pos32,    c->nl.ref,       c->nl.slot,    c->nl.order,
+    void *p[] = {c->nl.pos,      c->nl.selfh,    c->nl.ref,       c->nl.slot,    c->nl.order,
                  c->nl.nbl,      c->nl.ncnt,     c->nl.aux_cnt,   c->nl.aux_n,   c->nl.aux_start,
                  c->nl.aux_mem,  c->nl.gstart,   c->nl.nonempty,  c->nl.count,   c->nl.flags,
                  (void *)c->nl.cell_order, (void *)c->nl.group_q0};
@@ -534,20 +527,21 @@ static bool setup_pot(const comd_params *p, Replica &R)
             r64[2 * i + 0] = g64[6 * i + 0];
             r64[2 * i + 1] = g64[6 * i + 1];
         }
-        std::vector<float> d32(d64.size()), r32(r64.size());
-        for (size_t q = 0; q < d64.size(); q++) d32[q] = (float)d64[q];
-        for (size_t q = 0; q < r64.size(); q++) r32[q] = (float)r64[q];
+        // mixed density rows: phi d0 d1 (double), then 6 floats (40 bytes)
+        std::vector<float> dm((size_t)10 * n);
+        for (int i = 0; i < n; i++) {
+            memcpy(&dm[10 * i], &d64[8 * i], 2 * sizeof(double));
+            for (int q = 0; q < 6; q++) dm[10 * i + 4 + q] = (float)d64[8 * i + 2 + q];
+        }
         double *a = (double *)alloc(d64.size() * 8), *b = (double *)alloc(r64.size() * 8);
-        float *e = (float *)alloc(d32.size() * 4), *f = (float *)alloc(r32.size() * 4);
+        float *e = (float *)alloc(dm.size() * 4);
         if (ok)
             ok = hipMemcpy(a, d64.data(), d64.size() * 8, hipMemcpyHostToDevice) == hipSuccess &&
                  hipMemcpy(b, r64.data(), r64.size() * 8, hipMemcpyHostToDevice) == hipSuccess &&
-                 hipMemcpy(e, d32.data(), d32.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
-                 hipMemcpy(f, r32.data(), r32.size() * 4, hipMemcpyHostToDevice) == hipSuccess;
+                 hipMemcpy(e, dm.data(), dm.size() * 4, hipMemcpyHostToDevice) == hipSuccess;
         P.cd64 = a;
         P.cr64 = b;
-        P.cd32 = e;
-        P.cr32 = f;
+        P.cdm = e;
     }
     if (ok && p->cheb_n >= 2 && p->cheb_coef) {
         const int m = p->cheb_n;
@@ -801,7 +795,7 @@ static int nl_setup(comd_ctx *c, long nids)
         };
         const size_t n = (size_t)nids;
         alloc((void **)&c->nl.pos, n * sizeof(double4));
-        alloc((void **)&c->nl.pos32, n * sizeof(uint4));
+        alloc((void **)&c->nl.selfh, n * 2);
         alloc((void **)&c->nl.ref, n * sizeof(double4));
         alloc((void **)&c->nl.slot, n * 4);
         alloc((void **)&c->nl.order, n * 4);
@@ -854,6 +848,17 @@ static int nl_setup(comd_ctx *c, long nids)
     l.ng2 = ng[2];
     l.ngroups = (int)ngroups;
     l.hcap = (int)hcap;
+    {
+        // 32-bit fixed point of the mixed passes' group frames: a halo atom
+        // lies within (gd/2 + 1) cells of the group centre plus the half skin
+        // it may have moved since the build; 5 % to spare
+        double ext = 0.0;
+        const double ab[3] = {l.ab0, l.ab1, l.ab2};
+        for (int j = 0; j < 3; j++) ext = std::max(ext, (0.5 * gd[j] + 1.0) * ab[j]);
+        ext += 1e-3 * c->nl_skin_milli;
+        l.qscale = 2147483648.0 / (1.05 * ext);
+        l.qinv = 1.0 / l.qscale;
+    }
     l.lds_avail = (int)lds_avail;
     l.rows_lds = c->nl_rows_lds;
     l.build_blocks = (int)std::min<long>(ngroups, (long)c->ncu * 2);
@@ -881,13 +886,6 @@ static int nl_setup(comd_ctx *c, long nids)
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
     HIPCHK(c, hipMemsetAsync(l.nbl, 0, (size_t)nids * (size_t)kmax * 2, c->stream));
-    const double two32 = 4294967296.0;
-    l.qs0 = two32 / l.L0;
-    l.qs1 = two32 / l.L1;
-    l.qs2 = two32 / l.L2;
-    l.qi0 = l.L0 / two32;
-    l.qi1 = l.L1 / two32;
-    l.qi2 = l.L2 / two32;
     HIPCHK(c, hipMemsetAsync(l.flags, 0, 16, c->stream));
     HIPCHK(c, hipMemsetAsync(l.count, 0, 16, c->stream));
     HIPCHK(c, hipMemsetAsync(l.aux_cnt, 0, (size_t)naux * 4, c->stream));

@@ -65,8 +65,6 @@ __device__ __forceinline__ void move_atom(const DevState &d, const Grid &g, int 
             const int id = d.id[s];
             const double ax = cx + x, ay = cy + y, az = cz + z;
             d.nl_pos[id] = make_double4(ax, ay, az, 0.0);
-            if (d.nl_pos32)
-                d.nl_pos32[id] = make_uint4(fixed32(ax, d.nl_qs0), fixed32(ay, d.nl_qs1), fixed32(az, d.nl_qs2), 0u);
             const double4 q = d.nl_ref[id];
             double ex = ax - q.x, ey = ay - q.y, ez = az - q.z;
             ex -= g.L0 * rint(ex * g.iL0);

@@ -72,10 +72,6 @@ struct DevState {
     // CM_NL_REBUILD is raised when an atom has moved more than half the skin
     double4 *nl_pos;
     const double4 *nl_ref;
-    // the same position in 32-bit periodic fixed point (x L / 2^32 per
-    // unit) for the mixed-precision list passes; null unless they run
-    uint4 *nl_pos32;
-    double nl_qs0, nl_qs1, nl_qs2;  // 2^32 / L per dimension
     int *nl_slot;
     unsigned *nl_flags;
     double nl_half2;     // (skin / 2)^2
@@ -168,9 +164,3 @@ __device__ __forceinline__ void corner_shift(const Grid &g, int c, int b, double
     sy = image1(cy - by, g.L1, g.periodic);
     sz = image1(cz - bz, g.L2, g.periodic);
 }
-
-// Periodic 32-bit fixed point: x (within one box length of [0, L)) times
-// 2^32 / L, rounded, modulo 2^32.  The difference of two such coordinates,
-// read as a signed 32-bit integer, is the nearest-image separation in units
-// of L / 2^32 whenever it is under L / 2.
-__device__ __forceinline__ unsigned fixed32(double x, double qs) { return (unsigned)(long long)rint(x * qs); }

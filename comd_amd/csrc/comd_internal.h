@@ -19,11 +19,12 @@ struct PotDev {
     const double *cf64[2];
     const float *cf32[2];
     int same_grid;  // phi and rho tables share x0, xn, invDx and n
-    // compact rows for the neighbour-list passes (only when same_grid):
-    // density pass (phi d0 d1 c0 c1 c2, rho c0 c1 c2) per interval,
-    // embedding-force pass (rho d0 d1) per interval
+    // compact rows for the neighbour-list passes (only when same_grid), per
+    // interval: density pass, FP64 (phi d0 d1 c0 c1 c2, rho c0 c1 c2) and
+    // mixed (phi d0 d1 in double, then phi c0 c1 c2, rho c0 c1 c2 in float:
+    // 40 bytes); embedding-force pass (rho d0 d1) in double
     const double *cd64, *cr64;
-    const float *cd32, *cr32;
+    const float *cdm;
     int balanced;   // drain an atom's queues round-robin over its lanes
     // Chebyshev: 6 series of cn coefficients (phi rho F dphi drho dF)
     const double *cheb;
@@ -143,10 +144,8 @@ struct CmList {
     const int *cell_order;   // [naux]: auxiliary cells in build order (group, then cell)
     const int *group_q0;     // [ngroups + 1]: a group's first position in cell_order
     double4 *pos, *ref;      // per id (pos.w: F' of the last density pass)
-    uint4 *pos32;            // per id: pos in fixed32 (x, y, z) and, in .w, the float bits of
-                             // F' of the last mixed-precision density pass
-    double qs0, qs1, qs2;    // 2^32 / L (encode)
-    double qi0, qi1, qi2;    // L / 2^32 (decode)
+    double qscale, qinv;     // group-frame fixed point of the mixed passes: units per A and A per unit
+    uint16_t *selfh;         // per t: the atom's own index in its group's staged halo
     int *slot;               // per id: current slot, -1 when absent
     int *order;              // atoms (ids) in build order
     uint16_t *nbl;           // list of the t-th atom in build order: indices into its group's

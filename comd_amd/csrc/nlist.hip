@@ -81,7 +81,6 @@ __global__ __launch_bounds__(64) void k_nl_from_slots(DevState d, CmList l, int 
             const double x = cx + d.rx[s], y = cy + d.ry[s], z = cz + d.rz[s];
             l.slot[id] = s;
             l.pos[id] = make_double4(x, y, z, 0.0);
-            l.pos32[id] = make_uint4(fixed32(x, l.qs0), fixed32(y, l.qs1), fixed32(z, l.qs2), 0u);
             if (check) {
                 const double4 q = l.ref[id];
                 double ex = x - q.x, ey = y - q.y, ez = z - q.z;
@@ -116,13 +115,16 @@ __device__ __forceinline__ GroupGeo group_geo(const CmList &l, int g)
     return G;
 }
 
-// Halo slot s (x fastest over (lx + 2) x (ly + 2) x (lz + 2)): its cell and
-// the image shift that brings the cell next to the group.
+// Halo slot s (x fastest over (lx + 2) x (ly + 2) x (lz + 2)): its cell
+// (index, and coordinates jx, jy, jz) and the image shift that brings the
+// cell next to the group.
 __device__ __forceinline__ int halo_cell(const CmList &l, const GroupGeo &G, int s, double &sx, double &sy,
-                                         double &sz)
+                                         double &sz, int &jx, int &jy, int &jz)
 {
     const int hx = G.lx + 2, hy = G.ly + 2;
-    int jx = G.x0 - 1 + s % hx, jy = G.y0 - 1 + (s / hx) % hy, jz = G.z0 - 1 + s / (hx * hy);
+    jx = G.x0 - 1 + s % hx;
+    jy = G.y0 - 1 + (s / hx) % hy;
+    jz = G.z0 - 1 + s / (hx * hy);
     sx = sy = sz = 0.0;
     if (jx < 0) { jx += l.na0; sx = -l.L0; } else if (jx >= l.na0) { jx -= l.na0; sx = l.L0; }
     if (jy < 0) { jy += l.na1; sy = -l.L1; } else if (jy >= l.na1) { jy -= l.na1; sy = l.L1; }
@@ -146,7 +148,8 @@ __device__ __forceinline__ void group_setup(const CmList &l, int g, GroupLds &S,
     const int ns = (G.lx + 2) * (G.ly + 2) * (G.lz + 2);
     for (int s = threadIdx.x; s < ns; s += blockDim.x) {
         double sx, sy, sz;
-        const int c = halo_cell(l, G, s, sx, sy, sz);
+        int jx, jy, jz;
+        const int c = halo_cell(l, G, s, sx, sy, sz, jx, jy, jz);
         S.start[s] = l.aux_start[c];
         S.off[s + 1] = l.aux_n[c];  // counts, scanned below
         if (shift) {
@@ -347,6 +350,7 @@ __global__ __launch_bounds__(256) void k_nl_build(CmList l)
             const int cx = ax - G.x0 + 1, cy = ay - G.y0 + 1, cz = az - G.z0 + 1;
             const int hs = cx + shx * (cy + shy * cz);
             const int self = S.off[hs] + (t - S.start[hs]);
+            l.selfh[t] = (uint16_t)self;
             const double xi = hx[self], yi = hy[self], zi = hz[self];
             int k = 0;
             for (int e = 0; e < 27; e++) {
@@ -405,24 +409,47 @@ __global__ void k_nl_nonempty(DevState d, CmList l)
 }
 
 // Compact table rows a pass reads (density: phi and rho, 8 values per
-// interval; embedding force: rho', 2 values), staged per block in LDS: every
+// interval, 64 bytes or 40 in mixed precision; embedding force: rho', 2
+// doubles), staged per block in LDS: every
 // pair reads a different interval, and from LDS these reads do not load the
 // vector L1.
 static inline size_t nl_tab_bytes(const PotDev &P, int kind, int mixed)
 {
-    const size_t b = (size_t)P.n[0] * (kind == CM_KIND_EAM1 ? 8 : 2) * (mixed ? 4 : 8);
+    const size_t b = (size_t)P.n[0] * (kind == CM_KIND_EAM1 ? (mixed ? 40 : 64) : 16);
     return (b + 15) / 16 * 16;  // the halo after the rows stays 16-byte aligned
 }
-// Staged halo bytes per atom: fixed32 x, y, z and float F' (mixed); double
-// x, y, z and, for the embedding-force pass, F' (FP64).
+// Staged halo bytes per atom: group-frame fixed point x, y, z and float F'
+// (mixed); group-frame double x, y, z and, for the embedding-force pass, F'
+// (FP64).
 static inline size_t nl_halo_bytes(int kind, int mixed) { return mixed ? 16 : (kind == CM_KIND_EAM3 ? 32 : 24); }
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef unsigned v4u __attribute__((ext_vector_type(4)));
 
+// Position of halo entry h (slot s) in its group's frame: the nearest image
+// of the atom to its slot's cell, moved with the cell next to the group,
+// relative to the frame origin o.  Atoms that crossed the periodic boundary
+// since the build, and cells that appear twice in a small box's halo, are
+// placed right.
+__device__ __forceinline__ void frame_pos(const CmList &l, const GroupGeo &G, int s, const v4d &p, double ox,
+                                          double oy, double oz, double &x, double &y, double &z)
+{
+    double sx, sy, sz;
+    int jx, jy, jz;
+    halo_cell(l, G, s, sx, sy, sz, jx, jy, jz);
+    const double cx = (jx + 0.5) * l.ab0, cy = (jy + 0.5) * l.ab1, cz = (jz + 0.5) * l.ab2;
+    double ex = p.x - cx, ey = p.y - cy, ez = p.z - cz;
+    ex -= l.L0 * rint(ex * l.iL0);
+    ey -= l.L1 * rint(ey * l.iL1);
+    ez -= l.L2 * rint(ez * l.iL2);
+    x = (cx + sx - ox) + ex;
+    y = (cy + sy - oy) + ey;
+    z = (cz + sz - oz) + ez;
+}
+
 // One home atom t over its list.  TB: the table pointers (compact rows in
 // LDS when COMPACT, else the full tables in global memory); hq (mixed) or
-// hx, hy, hz, hf (FP64): the staged halo.
+// hx, hy, hz, hf (FP64): the staged halo, which holds the atom itself too.
 template <int KIND, int EVAL, int MIXED, int COMPACT, class TB>
 __device__ __forceinline__ void nl_atom(const DevState &d, const PotDev &P, const CmList &l, const TB &T, int t,
                                         const CM_LDS v4u *hq, const CM_LDS double *hx, const CM_LDS double *hy,
@@ -430,15 +457,21 @@ __device__ __forceinline__ void nl_atom(const DevState &d, const PotDev &P, cons
 {
     const int i = l.order[t];
     const int s = l.slot[i];
-    // the atom's own position and F', from the id maps
-    v4d pi = {0, 0, 0, 0};
+    const int self = l.selfh[t];
     v4u qi = {0, 0, 0, 0};
-    if (MIXED) qi = ((const CM_GLOBAL v4u *)l.pos32)[i];
-    else pi = ((const CM_GLOBAL v4d *)l.pos)[i];
-    const double fia = KIND != CM_KIND_EAM3 ? 0.0 : MIXED ? (double)__uint_as_float(qi.w) : pi.w;
+    double xi = 0.0, yi = 0.0, zi = 0.0, fia = 0.0;
+    if (MIXED) {
+        qi = hq[self];
+        fia = (double)__uint_as_float(qi.w);
+    } else {
+        xi = hx[self];
+        yi = hy[self];
+        zi = hz[self];
+        if (KIND == CM_KIND_EAM3) fia = hf[self];
+    }
+    if (KIND != CM_KIND_EAM3) fia = 0.0;
     const double rc2 = P.rc2;
-    const double L0 = l.L0, L1 = l.L1, L2 = l.L2;
-    const double iL0 = l.iL0, iL1 = l.iL1, iL2 = l.iL2;
+    const double qinv = l.qinv;
     const int cnt = l.ncnt[t];
     Acc A = {0, 0, 0, 0, 0, 0};
     int npair = 0;
@@ -455,18 +488,17 @@ __device__ __forceinline__ void nl_atom(const DevState &d, const PotDev &P, cons
             const int hj = live ? (int)((ew[u >> 1] >> (16 * (u & 1))) & 0xffffu) : 0;
             double dx, dy, dz, fij;
             if (MIXED) {
+                // fixed-point differences (modulo 2^32, exact for pairs on
+                // the lists, which are well within range); one rounding
                 const v4u qj = hq[hj];
-                dx = (double)(int)(qi.x - qj.x) * l.qi0;
-                dy = (double)(int)(qi.y - qj.y) * l.qi1;
-                dz = (double)(int)(qi.z - qj.z) * l.qi2;
+                dx = (double)(int)(qi.x - qj.x) * qinv;
+                dy = (double)(int)(qi.y - qj.y) * qinv;
+                dz = (double)(int)(qi.z - qj.z) * qinv;
                 fij = (double)__uint_as_float(qj.w);
             } else {
-                dx = pi.x - hx[hj];
-                dy = pi.y - hy[hj];
-                dz = pi.z - hz[hj];
-                dx -= L0 * rint(dx * iL0);
-                dy -= L1 * rint(dy * iL1);
-                dz -= L2 * rint(dz * iL2);
+                dx = xi - hx[hj];
+                dy = yi - hy[hj];
+                dz = zi - hz[hj];
                 fij = KIND == CM_KIND_EAM3 ? hf[hj] : 0.0;
             }
             const double r2 = dx * dx + dy * dy + dz * dz;
@@ -494,8 +526,7 @@ __device__ __forceinline__ void nl_atom(const DevState &d, const PotDev &P, cons
         d.rho[s] = A.rho;
         d.fi[s] = fp;
         d.nnb[s] = npair;
-        if (MIXED) ((unsigned *)&l.pos32[i])[3] = __float_as_uint((float)fp);
-        else ((double *)&l.pos[i])[3] = fp;
+        ((double *)&l.pos[i])[3] = fp;  // staged with the position by the embedding-force pass
     } else if (KIND == CM_KIND_EAM3) {
         d.fx[s] += A.x;
         d.fy[s] += A.y;
@@ -512,11 +543,11 @@ __device__ __forceinline__ void nl_atom(const DevState &d, const PotDev &P, cons
     }
 }
 
-// A block per group: the halo is staged in LDS, then each thread takes home
-// atoms.  TLDS: the compact table rows are staged in LDS (1), or the full
-// tables are read from global memory (0); a compile-time choice, so the row
-// reads are LDS or global loads.  Dynamic LDS: [rows (TLDS)] [halo, hcap
-// atoms].
+// A block per group: the halo is staged in LDS in the group's frame (origin
+// at the centre of the group's cells), then each thread takes home atoms.
+// TLDS: the compact table rows are staged in LDS (1), or the full tables are
+// read from global memory (0); a compile-time choice, so the row reads are
+// LDS or global loads.  Dynamic LDS: [rows (TLDS)] [halo, hcap atoms].
 template <int KIND, int EVAL, int MIXED, int TLDS>
 __global__ __launch_bounds__(512) void k_nl_pair(DevState d, PotDev P, CmList l, int tab_bytes)
 {
@@ -530,13 +561,14 @@ __global__ __launch_bounds__(512) void k_nl_pair(DevState d, PotDev P, CmList l,
     TL.d64 = TL.r64 = nullptr;
     TL.d32 = TL.r32 = nullptr;
     if (TLDS) {
-        const int nv = P.n[0] * (KIND == CM_KIND_EAM1 ? 8 : 2);
-        if (MIXED) {
-            const float *src = KIND == CM_KIND_EAM1 ? P.cd32 : P.cr32;
+        if (MIXED && KIND == CM_KIND_EAM1) {
+            const int nv = P.n[0] * 10;
             CM_LDS float *dst = (CM_LDS float *)nls;
-            for (int k = threadIdx.x; k < nv; k += blockDim.x) dst[k] = src[k];
-            TL.d32 = TL.r32 = dst;
+            for (int k = threadIdx.x; k < nv; k += blockDim.x) dst[k] = P.cdm[k];
+            TL.d32 = dst;
+            TL.d64 = (CM_LDS double *)dst;
         } else {
+            const int nv = P.n[0] * (KIND == CM_KIND_EAM1 ? 8 : 2);
             const double *src = KIND == CM_KIND_EAM1 ? P.cd64 : P.cr64;
             CM_LDS double *dst = (CM_LDS double *)nls;
             for (int k = threadIdx.x; k < nv; k += blockDim.x) dst[k] = src[k];
@@ -548,17 +580,28 @@ __global__ __launch_bounds__(512) void k_nl_pair(DevState d, PotDev P, CmList l,
     // over the CUs
     for (int g = blockIdx.x; g < l.ngroups; g += gridDim.x) {
         group_setup(l, g, S, nullptr);  // its first barrier also covers the table staging
+        const GroupGeo G = group_geo(l, g);
+        const double ox = (G.x0 + 0.5 * G.lx) * l.ab0, oy = (G.y0 + 0.5 * G.ly) * l.ab1,
+                     oz = (G.z0 + 0.5 * G.lz) * l.ab2;
         const int H = min(S.off[S.ns], l.hcap);  // never above hcap while the lists are valid
         for (int h = threadIdx.x; h < H; h += blockDim.x) {
             const int s = halo_slot(S, h);
             const int j = l.order[S.start[s] + (h - S.off[s])];
+            const v4d p = ((const CM_GLOBAL v4d *)l.pos)[j];
+            double x, y, z;
+            frame_pos(l, G, s, p, ox, oy, oz, x, y, z);
             if (MIXED) {
-                hq[h] = ((const CM_GLOBAL v4u *)l.pos32)[j];
+                const float fj = (float)p.w;
+                v4u q;
+                q.x = (unsigned)(int)rint(x * l.qscale);
+                q.y = (unsigned)(int)rint(y * l.qscale);
+                q.z = (unsigned)(int)rint(z * l.qscale);
+                q.w = __float_as_uint(fj);
+                hq[h] = q;
             } else {
-                const v4d p = ((const CM_GLOBAL v4d *)l.pos)[j];
-                hx[h] = p.x;
-                hy[h] = p.y;
-                hz[h] = p.z;
+                hx[h] = x;
+                hy[h] = y;
+                hz[h] = z;
                 if (KIND == CM_KIND_EAM3) hf[h] = p.w;
             }
         }

@@ -80,8 +80,9 @@ __device__ __forceinline__ double rsqrt_mixed(double x)
     return fma(0.5 * y, e, y);
 }
 
-// Compact rows of the neighbour-list passes (PotDev::cd*, cr*), behind
-// pointers of a given address space.
+// Compact rows of the neighbour-list passes (PotDev::cd64, cdm, cr64), behind
+// pointers of a given address space (d32 and d64 both point at the mixed
+// density rows when those are used).
 template <class PD, class PF> struct CompactT {
     PD d64, r64;
     PF d32, r32;
@@ -147,10 +148,12 @@ __device__ __forceinline__ void pair_term(const PotDev &P, const TB &T, double d
         if (EVAL == COMD_EVAL_CHEB) {
             const int m = P.cn;
             if (MIXED) {
+                // values in FP32; the derivative, whose pair terms cancel in
+                // the force sum, in FP64 (the same issue rate on this device)
                 const float rf = (float)r;
                 const float a0 = (float)P.ca[0], b0 = (float)P.cb[0], a1 = (float)P.ca[1], b1 = (float)P.cb[1];
                 phiv = cheb32(P.cheb32 + 0 * m, m, a0, b0, rf);
-                dphi = cheb32(P.cheb32 + 3 * m, m, a0, b0, rf);
+                dphi = cheb64(P.cheb + 3 * m, m, P.ca[0], P.cb[0], r);
                 rhov = cheb32(P.cheb32 + 1 * m, m, a1, b1, rf);
             } else {
                 phiv = cheb64(P.cheb + 0 * m, m, P.ca[0], P.cb[0], r);
@@ -164,13 +167,15 @@ __device__ __forceinline__ void pair_term(const PotDev &P, const TB &T, double d
             t1 = t0;
             if (!P.same_grid) i1 = table_interval(r, P.x0[1], P.xn[1], P.invdx[1], P.n[1], t1);
             if (MIXED) {
+                // values in FP32, the derivative in FP64 (see above)
                 const float4 a = cf_row32(T.c32[0], i0);
                 const float a2 = T.c32[0][8 * i0 + 4];
                 const float4 b = cf_row32(T.c32[1], i1);
                 const float b2 = T.c32[1][8 * i1 + 4];
                 const float f0 = (float)t0, f1 = (float)t1;
+                const double2 dd = cf_der64(T.c64[0], i0);
                 phiv = fmaf(f0, fmaf(f0, a2, a.w), a.z);
-                dphi = fmaf(f0, a.y, a.x);
+                dphi = fma(t0, dd.y, dd.x);
                 rhov = fmaf(f1, fmaf(f1, b2, b.w), b.z);
             } else {
                 const double2 dd = cf_der64(T.c64[0], i0);
@@ -189,19 +194,12 @@ __device__ __forceinline__ void pair_term(const PotDev &P, const TB &T, double d
         // src-flat/eam.c:633-649
         double drho;
         if (EVAL == COMD_EVAL_CHEB) {
-            if (MIXED)
-                drho = cheb32(P.cheb32 + 4 * P.cn, P.cn, (float)P.ca[1], (float)P.cb[1], (float)r);
-            else
-                drho = cheb64(P.cheb + 4 * P.cn, P.cn, P.ca[1], P.cb[1], r);
+            drho = cheb64(P.cheb + 4 * P.cn, P.cn, P.ca[1], P.cb[1], r);  // FP64 in both modes
         } else {
             double t1;
             const int i1 = table_interval(r, P.x0[1], P.xn[1], P.invdx[1], P.n[1], t1);
-            if (MIXED) {
-                drho = fmaf((float)t1, T.c32[1][8 * i1 + 1], T.c32[1][8 * i1]);
-            } else {
-                const double2 dd = cf_der64(T.c64[1], i1);
-                drho = fma(t1, dd.y, dd.x);
-            }
+            const double2 dd = cf_der64(T.c64[1], i1);  // FP64 in both modes
+            drho = fma(t1, dd.y, dd.x);
         }
         const double g = (fia + fij) * drho * y;
         A.x += g * dx;
@@ -227,12 +225,13 @@ __device__ __forceinline__ void pair_term_compact(const PotDev &P, const CT &T, 
     if (KIND == CM_KIND_EAM1) {
         double phiv, dphi, rhov;
         if (MIXED) {
+            // 40-byte rows: the derivative coefficients in FP64, the values' in FP32
             const float f = (float)t;
-            const float pd0 = T.d32[8 * i + 0], pd1 = T.d32[8 * i + 1], pc0 = T.d32[8 * i + 2],
-                        pc1 = T.d32[8 * i + 3], pc2 = T.d32[8 * i + 4], rc0 = T.d32[8 * i + 5],
-                        rc1 = T.d32[8 * i + 6], rc2 = T.d32[8 * i + 7];
+            const double pd0 = T.d64[5 * i + 0], pd1 = T.d64[5 * i + 1];
+            const float pc0 = T.d32[10 * i + 4], pc1 = T.d32[10 * i + 5], pc2 = T.d32[10 * i + 6],
+                        rc0 = T.d32[10 * i + 7], rc1 = T.d32[10 * i + 8], rc2 = T.d32[10 * i + 9];
             phiv = fmaf(f, fmaf(f, pc2, pc1), pc0);
-            dphi = fmaf(f, pd1, pd0);
+            dphi = fma(t, pd1, pd0);
             rhov = fmaf(f, fmaf(f, rc2, rc1), rc0);
         } else {
             const double pd0 = T.d64[8 * i + 0], pd1 = T.d64[8 * i + 1], pc0 = T.d64[8 * i + 2],
@@ -249,8 +248,7 @@ __device__ __forceinline__ void pair_term_compact(const PotDev &P, const CT &T, 
         A.e += 0.5 * phiv;
         A.rho += rhov;
     } else {
-        const double drho = MIXED ? (double)fmaf((float)t, T.r32[2 * i + 1], T.r32[2 * i])
-                                  : fma(t, T.r64[2 * i + 1], T.r64[2 * i]);
+        const double drho = fma(t, T.r64[2 * i + 1], T.r64[2 * i]);  // FP64 in both modes
         const double g = (fia + fij) * drho * y;
         A.x += g * dx;
         A.y += g * dy;

@@ -122,3 +122,25 @@ def test_upload_path_sets_up_lists(gpu, cu500):
     b.force()
     assert c.list_info()["active"] == 1
     assert_same(c, b, "uploaded state")
+
+
+@pytest.mark.parametrize("groups,rows", [(0, 1), (1, 1), (2, 0), (3, 1)])
+@pytest.mark.parametrize("n", [5, 7])
+def test_group_options_equal_cells(gpu, cu500, groups, rows, n):
+    """Every group size (4x4x2 .. 2x2x1 auxiliary cells), table rows in LDS
+    or not, and boxes of 3 auxiliary cells per dimension (n = 5: a group's
+    halo holds the same cell twice, at two images): the same pairs and
+    forces as the link-cell kernels."""
+    for prec in ("fp64", "mixed"):
+        a, b = pair_of_sims(cu500, n, 3.615, 900.0, 0.1, 30 + n, "table", prec)
+        a.tune(list_groups=groups, list_rows_lds=rows)
+        if prec == "mixed":
+            b.set_precision("fp64")
+        a.force()
+        b.force()
+        info = a.list_info()
+        assert info["active"] == 1 and info["valid"] == 1, info
+        assert_same(a, b, f"groups {groups} rows {rows} n {n} {prec}", tol=1e-12 if prec == "fp64" else 1e-6)
+        a.nsteps(20)
+        b.nsteps(20)
+        assert_same(a, b, f"groups {groups} rows {rows} n {n} {prec}, 20 steps", tol=1e-10 if prec == "fp64" else 1e-5)
