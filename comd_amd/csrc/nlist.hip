@@ -115,16 +115,15 @@ __device__ __forceinline__ GroupGeo group_geo(const CmList &l, int g)
     return G;
 }
 
-// Halo slot s (x fastest over (lx + 2) x (ly + 2) x (lz + 2)): its cell
-// (index, and coordinates jx, jy, jz) and the image shift that brings the
-// cell next to the group.
-__device__ __forceinline__ int halo_cell(const CmList &l, const GroupGeo &G, int s, double &sx, double &sy,
-                                         double &sz, int &jx, int &jy, int &jz)
+// Halo slot with coordinates (ux, uy, uz) in the group's (lx + 2) x (ly + 2)
+// x (lz + 2) slots (x fastest): its cell (index, and coordinates jx, jy, jz)
+// and the image shift that brings the cell next to the group.
+__device__ __forceinline__ int halo_cell(const CmList &l, const GroupGeo &G, int ux, int uy, int uz, double &sx,
+                                         double &sy, double &sz, int &jx, int &jy, int &jz)
 {
-    const int hx = G.lx + 2, hy = G.ly + 2;
-    jx = G.x0 - 1 + s % hx;
-    jy = G.y0 - 1 + (s / hx) % hy;
-    jz = G.z0 - 1 + s / (hx * hy);
+    jx = G.x0 - 1 + ux;
+    jy = G.y0 - 1 + uy;
+    jz = G.z0 - 1 + uz;
     sx = sy = sz = 0.0;
     if (jx < 0) { jx += l.na0; sx = -l.L0; } else if (jx >= l.na0) { jx -= l.na0; sx = l.L0; }
     if (jy < 0) { jy += l.na1; sy = -l.L1; } else if (jy >= l.na1) { jy -= l.na1; sy = l.L1; }
@@ -132,31 +131,30 @@ __device__ __forceinline__ int halo_cell(const CmList &l, const GroupGeo &G, int
     return jx + l.na0 * (jy + l.na1 * jz);
 }
 
-// A group's halo table in LDS: per slot its first atom in build order;
-// off[] the slots' offsets in the staged halo (off[ns] = halo atoms).
+// A group's halo table in LDS: per slot its first atom in build order and
+// its coordinates (ux | uy << 8 | uz << 16); off[] the slots' offsets in the
+// staged halo (off[ns] = halo atoms).
 struct GroupLds {
-    int start[CM_NL_SLOTS], off[CM_NL_SLOTS + 1];
+    int start[CM_NL_SLOTS], off[CM_NL_SLOTS + 1], uxyz[CM_NL_SLOTS];
     int ns, t0, t1;
 };
 
-// Fills S (and, when shift is given, the slots' image shifts) for group g
-// (all threads of the block; starts and ends with a barrier).
-__device__ __forceinline__ void group_setup(const CmList &l, int g, GroupLds &S, double (*shift)[3])
+// Fills S for group g (all threads of the block; starts and ends with a
+// barrier).
+__device__ __forceinline__ void group_setup(const CmList &l, int g, GroupLds &S)
 {
     __syncthreads();
     const GroupGeo G = group_geo(l, g);
     const int ns = (G.lx + 2) * (G.ly + 2) * (G.lz + 2);
+    const int hx = G.lx + 2, hy = G.ly + 2;
     for (int s = threadIdx.x; s < ns; s += blockDim.x) {
+        const int ux = s % hx, uy = (s / hx) % hy, uz = s / (hx * hy);
         double sx, sy, sz;
         int jx, jy, jz;
-        const int c = halo_cell(l, G, s, sx, sy, sz, jx, jy, jz);
+        const int c = halo_cell(l, G, ux, uy, uz, sx, sy, sz, jx, jy, jz);
         S.start[s] = l.aux_start[c];
+        S.uxyz[s] = ux | uy << 8 | uz << 16;
         S.off[s + 1] = l.aux_n[c];  // counts, scanned below
-        if (shift) {
-            shift[s][0] = sx;
-            shift[s][1] = sy;
-            shift[s][2] = sz;
-        }
     }
     __syncthreads();
     if (threadIdx.x < 64) {
@@ -205,6 +203,30 @@ __device__ __forceinline__ void block_groups(int ngroups, int &g0, int &g1)
     const int per = (ngroups + nb - 1) / nb;
     g0 = min(blk * per, ngroups);
     g1 = min(g0 + per, ngroups);
+}
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef unsigned v4u __attribute__((ext_vector_type(4)));
+
+// Position of halo entry h (slot s) in its group's frame: the nearest image
+// of the atom to its slot's cell, moved with the cell next to the group,
+// relative to the frame origin o.  Atoms that crossed the periodic boundary
+// since the build, and cells that appear twice in a small box's halo, are
+// placed right.
+__device__ __forceinline__ void frame_pos(const CmList &l, const GroupGeo &G, int uxyz, const v4d &p, double ox,
+                                          double oy, double oz, double &x, double &y, double &z)
+{
+    double sx, sy, sz;
+    int jx, jy, jz;
+    halo_cell(l, G, uxyz & 0xff, (uxyz >> 8) & 0xff, uxyz >> 16, sx, sy, sz, jx, jy, jz);
+    const double cx = (jx + 0.5) * l.ab0, cy = (jy + 0.5) * l.ab1, cz = (jz + 0.5) * l.ab2;
+    double ex = p.x - cx, ey = p.y - cy, ez = p.z - cz;
+    ex -= l.L0 * rint(ex * l.iL0);
+    ey -= l.L1 * rint(ey * l.iL1);
+    ez -= l.L2 * rint(ez * l.iL2);
+    x = (cx + sx - ox) + ex;
+    y = (cy + sy - oy) + ey;
+    z = (cz + sz - oz) + ez;
 }
 
 // ---------------------------------------------------------------------------
@@ -306,40 +328,45 @@ __global__ __launch_bounds__(256) void k_nl_sort(CmList l)
     }
 }
 
-// One block per group (persistent over groups): the halo is staged in LDS at
-// its image next to the group; each home atom (a thread) tests the atoms of
-// its own 27-cell stencil, in stencil order (x outer, as
-// src-flat/domains.c:419-426) and id order within a cell, and lists the
-// halo indices within cutoff + skin.  The last block to finish closes the
-// build: overflow (a list past kmax, a halo past hcap) -> lists invalid,
-// rebuild request cleared, counts cleared for the next build.
+// One block per group (persistent over groups): the halo is staged in LDS in
+// the group's frame, in single precision; each home atom (a thread) tests
+// the atoms of its own 27-cell stencil, in stencil order (x outer, as
+// src-flat/domains.c:419-426) and id order within a cell, and lists the halo
+// indices within cutoff + skin.  The single-precision test compares with
+// (cutoff + skin)^2 x (1 + 4e-6), more than its rounding (frame coordinates
+// within 20 A: at most 1.5e-6 of r^2), so no pair within cutoff + skin is
+// left out; the few extra pairs are rejected by the passes' cutoff test.
+// The last block to finish closes the build: overflow (a list past kmax, a
+// halo past hcap) -> lists invalid, rebuild request cleared, counts cleared
+// for the next build.
 __global__ __launch_bounds__(256) void k_nl_build(CmList l)
 {
     if (!nl_rebuild(l)) return;
-    extern __shared__ double hl[];  // hx[hcap], hy[hcap], hz[hcap]
+    extern __shared__ float4 hl4[];  // [hcap] frame x, y, z
     __shared__ GroupLds S;
-    __shared__ double shift[CM_NL_SLOTS][3];
     __shared__ int last;
-    double *hx = hl, *hy = hl + l.hcap, *hz = hl + 2 * l.hcap;
+    const float rl2 = (float)(l.rl2 * (1.0 + 4e-6));
     bool ovf = false;
     int g0, g1;
     block_groups(l.ngroups, g0, g1);
     for (int g = g0; g < g1; g++) {
-        group_setup(l, g, S, shift);
+        group_setup(l, g, S);
         const int H = S.off[S.ns];
         if (H > l.hcap) {  // the same in every thread of the block
             ovf = true;
             continue;
         }
+        const GroupGeo G = group_geo(l, g);
+        const double ox = (G.x0 + 0.5 * G.lx) * l.ab0, oy = (G.y0 + 0.5 * G.ly) * l.ab1,
+                     oz = (G.z0 + 0.5 * G.lz) * l.ab2;
         for (int h = threadIdx.x; h < H; h += blockDim.x) {
             const int s = halo_slot(S, h);
-            const double4 p = l.pos[l.order[S.start[s] + (h - S.off[s])]];
-            hx[h] = wrap_coord(p.x, l.L0, l.iL0) + shift[s][0];
-            hy[h] = wrap_coord(p.y, l.L1, l.iL1) + shift[s][1];
-            hz[h] = wrap_coord(p.z, l.L2, l.iL2) + shift[s][2];
+            const v4d p = ((const CM_GLOBAL v4d *)l.pos)[l.order[S.start[s] + (h - S.off[s])]];
+            double x, y, z;
+            frame_pos(l, G, S.uxyz[s], p, ox, oy, oz, x, y, z);
+            hl4[h] = make_float4((float)x, (float)y, (float)z, 0.0f);
         }
         __syncthreads();
-        const GroupGeo G = group_geo(l, g);
         const int shx = G.lx + 2, shy = G.ly + 2;
         for (int t = S.t0 + threadIdx.x; t < S.t1; t += blockDim.x) {
             const int id = l.order[t];
@@ -351,13 +378,14 @@ __global__ __launch_bounds__(256) void k_nl_build(CmList l)
             const int hs = cx + shx * (cy + shy * cz);
             const int self = S.off[hs] + (t - S.start[hs]);
             l.selfh[t] = (uint16_t)self;
-            const double xi = hx[self], yi = hy[self], zi = hz[self];
+            const float4 pi = hl4[self];
             int k = 0;
             for (int e = 0; e < 27; e++) {
                 const int s = (cx + e / 9 - 1) + shx * ((cy + (e / 3) % 3 - 1) + shy * (cz + e % 3 - 1));
                 for (int h = S.off[s]; h < S.off[s + 1]; h++) {
-                    const double dx = xi - hx[h], dy = yi - hy[h], dz = zi - hz[h];
-                    if (dx * dx + dy * dy + dz * dz <= l.rl2 && h != self) {
+                    const float4 q = hl4[h];
+                    const float dx = pi.x - q.x, dy = pi.y - q.y, dz = pi.z - q.z;
+                    if (fmaf(dx, dx, fmaf(dy, dy, dz * dz)) <= rl2 && h != self) {
                         if (k < l.kmax) l.nbl[nl_entry(k, t, l.n)] = (uint16_t)h;
                         k++;
                     }
@@ -422,30 +450,6 @@ static inline size_t nl_tab_bytes(const PotDev &P, int kind, int mixed)
 // (mixed); group-frame double x, y, z and, for the embedding-force pass, F'
 // (FP64).
 static inline size_t nl_halo_bytes(int kind, int mixed) { return mixed ? 16 : (kind == CM_KIND_EAM3 ? 32 : 24); }
-
-typedef double v4d __attribute__((ext_vector_type(4)));
-typedef unsigned v4u __attribute__((ext_vector_type(4)));
-
-// Position of halo entry h (slot s) in its group's frame: the nearest image
-// of the atom to its slot's cell, moved with the cell next to the group,
-// relative to the frame origin o.  Atoms that crossed the periodic boundary
-// since the build, and cells that appear twice in a small box's halo, are
-// placed right.
-__device__ __forceinline__ void frame_pos(const CmList &l, const GroupGeo &G, int s, const v4d &p, double ox,
-                                          double oy, double oz, double &x, double &y, double &z)
-{
-    double sx, sy, sz;
-    int jx, jy, jz;
-    halo_cell(l, G, s, sx, sy, sz, jx, jy, jz);
-    const double cx = (jx + 0.5) * l.ab0, cy = (jy + 0.5) * l.ab1, cz = (jz + 0.5) * l.ab2;
-    double ex = p.x - cx, ey = p.y - cy, ez = p.z - cz;
-    ex -= l.L0 * rint(ex * l.iL0);
-    ey -= l.L1 * rint(ey * l.iL1);
-    ez -= l.L2 * rint(ez * l.iL2);
-    x = (cx + sx - ox) + ex;
-    y = (cy + sy - oy) + ey;
-    z = (cz + sz - oz) + ez;
-}
 
 // One home atom t over its list.  TB: the table pointers (compact rows in
 // LDS when COMPACT, else the full tables in global memory); hq (mixed) or
@@ -559,13 +563,13 @@ __global__ __launch_bounds__(512) void k_nl_pair(DevState d, PotDev P, CmList l,
     CM_LDS double *hx = (CM_LDS double *)halo, *hy = hx + l.hcap, *hz = hy + l.hcap, *hf = hz + l.hcap;
     CompactLds TL;
     TL.d64 = TL.r64 = nullptr;
-    TL.d32 = TL.r32 = nullptr;
+    TL.dm = nullptr;
     if (TLDS) {
         if (MIXED && KIND == CM_KIND_EAM1) {
             const int nv = P.n[0] * 10;
             CM_LDS float *dst = (CM_LDS float *)nls;
             for (int k = threadIdx.x; k < nv; k += blockDim.x) dst[k] = P.cdm[k];
-            TL.d32 = dst;
+            TL.dm = dst;
             TL.d64 = (CM_LDS double *)dst;
         } else {
             const int nv = P.n[0] * (KIND == CM_KIND_EAM1 ? 8 : 2);
@@ -579,7 +583,7 @@ __global__ __launch_bounds__(512) void k_nl_pair(DevState d, PotDev P, CmList l,
     // one group per block: the dispatcher balances the groups' uneven work
     // over the CUs
     for (int g = blockIdx.x; g < l.ngroups; g += gridDim.x) {
-        group_setup(l, g, S, nullptr);  // its first barrier also covers the table staging
+        group_setup(l, g, S);  // its first barrier also covers the table staging
         const GroupGeo G = group_geo(l, g);
         const double ox = (G.x0 + 0.5 * G.lx) * l.ab0, oy = (G.y0 + 0.5 * G.ly) * l.ab1,
                      oz = (G.z0 + 0.5 * G.lz) * l.ab2;
@@ -589,7 +593,7 @@ __global__ __launch_bounds__(512) void k_nl_pair(DevState d, PotDev P, CmList l,
             const int j = l.order[S.start[s] + (h - S.off[s])];
             const v4d p = ((const CM_GLOBAL v4d *)l.pos)[j];
             double x, y, z;
-            frame_pos(l, G, s, p, ox, oy, oz, x, y, z);
+            frame_pos(l, G, S.uxyz[s], p, ox, oy, oz, x, y, z);
             if (MIXED) {
                 const float fj = (float)p.w;
                 v4u q;
@@ -633,7 +637,7 @@ void cm_nl_sync(const DevState &d, const CmList &l, hipStream_t st)
     hipLaunchKernelGGL(k_nl_from_slots, dim3(d.ncells), dim3(64), 0, st, d, l, 1);
 }
 
-size_t cm_nl_build_lds(const CmList &l) { return (size_t)l.hcap * 24; }
+size_t cm_nl_build_lds(const CmList &l) { return (size_t)l.hcap * 16; }
 
 void cm_nl_build(const DevState &d, const CmList &l, hipStream_t st)
 {

@@ -80,12 +80,11 @@ __device__ __forceinline__ double rsqrt_mixed(double x)
     return fma(0.5 * y, e, y);
 }
 
-// Compact rows of the neighbour-list passes (PotDev::cd64, cdm, cr64), behind
-// pointers of a given address space (d32 and d64 both point at the mixed
-// density rows when those are used).
+// Compact rows of the neighbour-list passes (PotDev::cd64, cr64), behind
+// pointers of a given address space.
 template <class PD, class PF> struct CompactT {
     PD d64, r64;
-    PF d32, r32;
+    PF dm;  // mixed density rows (40 bytes: phi d0 d1 in double, 6 floats)
 };
 using CompactGlobal = CompactT<const CM_GLOBAL double *, const CM_GLOBAL float *>;
 using CompactLds = CompactT<const CM_LDS double *, const CM_LDS float *>;
@@ -212,24 +211,30 @@ __device__ __forceinline__ void pair_term(const PotDev &P, const TB &T, double d
 // ---------------------------------------------------------------------------
 
 // The table branch of pair_term for EAM with phi and rho on one grid, from
-// the compact rows (the same formulas, src-flat/eam.c:401-434 and 521-545 /
-// 633-649).
+// the compact FP64 rows (the formulas of src-flat/eam.c:401-434 and 521-545 /
+// 633-649).  The interval and fraction: u = (r - x0) invDx as one fma,
+// clamped below at 0 (r <= rc = xn holds for every accepted pair; u = n, at
+// r = xn, is evaluated as the end of interval n - 1, the same value and
+// derivative as the reference's i = n, t = 0).  The derivatives, whose pair
+// terms cancel in the force sums, are FP64 in both modes; MIXED evaluates the
+// density pass's values (phi, rho) in FP32 from 40-byte rows, which halves
+// the rows' LDS traffic.
 template <int KIND, int MIXED, class CT>
 __device__ __forceinline__ void pair_term_compact(const PotDev &P, const CT &T, double dx, double dy, double dz,
                                                   double r2, double fia, double fij, Acc &A)
 {
     const double y = MIXED ? rsqrt_mixed(r2) : rsqrt64(r2);  // 1/r
     const double r = r2 * y;
-    double t;
-    const int i = table_interval(r, P.x0[0], P.xn[0], P.invdx[0], P.n[0], t);
+    const double u = fmax(fma(r, P.invdx[0], -P.x0[0] * P.invdx[0]), 0.0);
+    const int i = min((int)u, P.n[0] - 1);
+    const double t = u - (double)i;
     if (KIND == CM_KIND_EAM1) {
         double phiv, dphi, rhov;
         if (MIXED) {
-            // 40-byte rows: the derivative coefficients in FP64, the values' in FP32
-            const float f = (float)t;
             const double pd0 = T.d64[5 * i + 0], pd1 = T.d64[5 * i + 1];
-            const float pc0 = T.d32[10 * i + 4], pc1 = T.d32[10 * i + 5], pc2 = T.d32[10 * i + 6],
-                        rc0 = T.d32[10 * i + 7], rc1 = T.d32[10 * i + 8], rc2 = T.d32[10 * i + 9];
+            const float f = (float)t;
+            const float pc0 = T.dm[10 * i + 4], pc1 = T.dm[10 * i + 5], pc2 = T.dm[10 * i + 6],
+                        rc0 = T.dm[10 * i + 7], rc1 = T.dm[10 * i + 8], rc2 = T.dm[10 * i + 9];
             phiv = fmaf(f, fmaf(f, pc2, pc1), pc0);
             dphi = fma(t, pd1, pd0);
             rhov = fmaf(f, fmaf(f, rc2, rc1), rc0);
@@ -248,7 +253,7 @@ __device__ __forceinline__ void pair_term_compact(const PotDev &P, const CT &T, 
         A.e += 0.5 * phiv;
         A.rho += rhov;
     } else {
-        const double drho = fma(t, T.r64[2 * i + 1], T.r64[2 * i]);  // FP64 in both modes
+        const double drho = fma(t, T.r64[2 * i + 1], T.r64[2 * i]);
         const double g = (fia + fij) * drho * y;
         A.x += g * dx;
         A.y += g * dy;
@@ -256,4 +261,3 @@ __device__ __forceinline__ void pair_term_compact(const PotDev &P, const CT &T, 
         A.vir += g * dx * dx;
     }
 }
-
