@@ -90,6 +90,7 @@ struct comd_ctx {
     int nl_skin_milli;  // skin in 1e-3 A, 0 = link-cell kernels only
     int nl_active;      // lists allocated and usable for the atoms held
     int nl_stale;       // list positions not updated by the last drift
+    int nl_invalid_seen; // the lists were invalid when the host last looked
     int nl_group_opt;   // COMD_TUNE_LIST_GROUPS (-1: by precision)
     int nl_rows_lds;    // COMD_TUNE_LIST_ROWS_LDS (-1: whenever they fit)
     long nl_ids;        // id capacity of the atoms held
@@ -290,7 +291,12 @@ static void op_force(comd_ctx *c)
         }
         const int ccls = nl ? COMD_K_CELL_FALLBACK : cls;
         tbegin(c, ccls, &ev);
-        cm_launch_force(d, P, kind, mixed, force_cfg(c, kind, mixed), c->stream);
+        ForceCfg cfg = force_cfg(c, kind, mixed);
+        // beside valid lists the link-cell pass returns at once: a small grid
+        // keeps that cheap (the full grid once the host has seen the lists
+        // invalid)
+        if (nl && !c->nl_invalid_seen && cfg.blocks > c->ncu) cfg.blocks = c->ncu;
+        cm_launch_force(d, P, kind, mixed, cfg, c->stream);
         tend(c, ccls, ev);
     };
     if (P.kind == COMD_POT_LJ) {
@@ -342,8 +348,17 @@ static int run_steps(comd_ctx *c, int n, double dt)
 static int check_flags(comd_ctx *c)
 {
     HIPCHK(c, hipMemcpyAsync(c->host_flags, c->flags, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+    if (c->nl_active)
+        HIPCHK(c, hipMemcpyAsync(c->host_flags + 1, c->nl.flags, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     const unsigned f = *c->host_flags;
+    // lists found invalid: the link-cell passes beside the list passes get
+    // their full grid until the lists are valid again
+    if (c->nl_active) {
+        const int inv = (c->host_flags[1] & CM_NL_INVALID) != 0;
+        if (inv != c->nl_invalid_seen) invalidate_graph(c);
+        c->nl_invalid_seen = inv;
+    }
     if (f & CM_FLAG_OVERFLOW) return set_err(c, COMD_ERR_OVERFLOW, "a link cell reached %d atoms", CM_SLOTS);
     if (f & CM_FLAG_OUTSIDE) return set_err(c, COMD_ERR_OUTSIDE, "an atom fell outside the cell grid");
     return COMD_OK;
@@ -758,32 +773,39 @@ static int nl_setup(comd_ctx *c, long nids)
     const comd_params &p = c->reps[0].prm;
     if (!p.periodic || p.nbx[0] < 3 || p.nbx[1] < 3 || p.nbx[2] < 3) return COMD_OK;
     const double rl = p.cutoff + 1e-3 * c->nl_skin_milli;
-    int na[3];
+    int nmax[3];
     for (int j = 0; j < 3; j++) {
-        na[j] = (int)floor(p.bounds[j] / rl);
-        if (na[j] < 3) return COMD_OK;
+        nmax[j] = (int)floor(p.bounds[j] / rl);
+        if (nmax[j] < 3) return COMD_OK;
     }
-    const long naux = (long)na[0] * na[1] * na[2];
     const double vol = p.bounds[0] * p.bounds[1] * p.bounds[2];
     const double expect = 4.0 / 3.0 * 3.141592653589793 * rl * rl * rl * (double)c->ntot / vol;
     const long kmax = (((long)(1.5 * expect) + 16 + 15) / 16) * 16;
     // groups of auxiliary cells: the largest of the options whose staged
-    // halo (1.3x the mean occupancy, plus 64 atoms) fits the LDS a block may
+    // halo (1.2x the mean occupancy, plus 64 atoms) fits the LDS a block may
     // use in every pass (16 B per halo atom in mixed precision, up to 32 B in
-    // FP64) and in the build (24 B), beside the kernels' static LDS
-    const double occ = (double)c->ntot / (double)naux;
-    int gd[3] = {0, 0, 0};
-    long hcap = 0;
+    // FP64) and in the build, beside the kernels' static LDS.  The auxiliary
+    // grid is rounded down to whole groups per dimension (cells a little
+    // larger than cutoff + skin): a partial group at an edge would take a
+    // whole block for a sliver of atoms.
+    int na[3] = {0, 0, 0}, gd[3] = {0, 0, 0};
+    long hcap = 0, naux = 0;
     const int gopts[4][3] = {{4, 4, 2}, {4, 2, 2}, {2, 2, 2}, {2, 2, 1}};
     const long lds_avail = (long)c->lds_block - CM_NL_STATIC_LDS;
-    const long hb = c->precision == COMD_PREC_MIXED ? 24 : 32;
+    const long hb = c->precision == COMD_PREC_MIXED ? 16 : 32;
     for (int o = c->nl_group_opt >= 0 ? c->nl_group_opt : 0; o < 4; o++) {
-        for (int j = 0; j < 3; j++) gd[j] = std::min(gopts[o][j], na[j]);
+        for (int j = 0; j < 3; j++) {
+            gd[j] = std::min(gopts[o][j], nmax[j]);
+            na[j] = nmax[j] / gd[j] * gd[j];
+        }
+        naux = (long)na[0] * na[1] * na[2];
+        const double occ = (double)c->ntot / (double)naux;
         const long slots = (long)(gd[0] + 2) * (gd[1] + 2) * (gd[2] + 2);
-        hcap = (((long)(1.3 * occ * (double)slots) + 64 + 63) / 64) * 64;
+        hcap = (((long)(1.2 * occ * (double)slots) + 64 + 63) / 64) * 64;
         if (hcap * hb <= lds_avail) break;
     }
     if (hcap * hb > lds_avail || hcap > 65535) return COMD_OK;  // too dense for staged halos
+    const double occ = (double)c->ntot / (double)naux;
     int ng[3];
     for (int j = 0; j < 3; j++) ng[j] = (na[j] + gd[j] - 1) / gd[j];
     const long ngroups = (long)ng[0] * ng[1] * ng[2];
@@ -891,6 +913,7 @@ static int nl_setup(comd_ctx *c, long nids)
     HIPCHK(c, hipMemsetAsync(l.aux_cnt, 0, (size_t)naux * 4, c->stream));
     c->nl_active = 1;
     c->nl_stale = 0;
+    c->nl_invalid_seen = 0;
     cm_nl_init(dev_state(c), l, c->stream);
     HIPCHK(c, hipGetLastError());
     return COMD_OK;

@@ -144,3 +144,33 @@ def test_group_options_equal_cells(gpu, cu500, groups, rows, n):
         a.nsteps(20)
         b.nsteps(20)
         assert_same(a, b, f"groups {groups} rows {rows} n {n} {prec}, 20 steps", tol=1e-10 if prec == "fp64" else 1e-5)
+
+
+def test_overflow_falls_back_to_cells(gpu, cu500):
+    """A Cu 8^3 lattice whose slab x < 18 A is squeezed 2.25-fold: the lists
+    are set up for the mean density, but the fullest 7.2 A auxiliary cell
+    gets 72 atoms, more than the 64 one holds (the fullest 5.8 A link cell
+    56).  The build marks the lists invalid and the link-cell passes,
+    launched beside the list passes, give the forces -- first with the small
+    grid they get while the lists were last seen valid, then with the full
+    grid once the host has seen them invalid."""
+    xyz, p, bounds = seeded_atoms(8, 3.615, cu500.mass, temp=0.0, disp=0.02, seed=41)
+    slab = xyz[:, 0] < 18.0
+    xyz[slab, 0] /= 2.25
+    out = []
+    for skin in (0.4, 0.0):
+        s = CA.Simulation(cu500, bounds, defgrad=0.0)
+        s.tune(skin=skin)
+        s.bin_atoms(xyz, p)
+        s.rebox()
+        out.append(s)
+    a, b = out
+    a.force()
+    b.force()
+    info = a.list_info()
+    print(f"[overflow] {info}", flush=True)
+    assert info["active"] == 1 and info["valid"] == 0
+    assert_same(a, b, "overflow, first force")
+    a.nsteps(2)
+    b.nsteps(2)
+    assert_same(a, b, "overflow, after steps", tol=1e-10)
