@@ -580,9 +580,14 @@ __global__ __launch_bounds__(512) void k_nl_pair(DevState d, PotDev P, CmList l,
         }
     }
     const TabsGlobal TG = global_tabs(P);
-    // one group per block: the dispatcher balances the groups' uneven work
-    // over the CUs
-    for (int g = blockIdx.x; g < l.ngroups; g += gridDim.x) {
+    // one group per block (the dispatcher balances the groups' uneven work
+    // over the CUs); blocks are dealt round-robin over the 8 XCDs, and each
+    // XCD takes a contiguous range of groups, so that neighbouring groups,
+    // which share halo cells, stage them through the same L2
+    {
+        const int nb = gridDim.x, qb = nb / 8, rem = nb % 8;
+        const int xcd = blockIdx.x % 8, idx = blockIdx.x / 8;
+        const int g = nb < 8 ? (int)blockIdx.x : xcd * qb + min(xcd, rem) + idx;
         group_setup(l, g, S);  // its first barrier also covers the table staging
         const GroupGeo G = group_geo(l, g);
         const double ox = (G.x0 + 0.5 * G.lx) * l.ab0, oy = (G.y0 + 0.5 * G.ly) * l.ab1,
