@@ -265,7 +265,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 geometry, accumulation and integration; f32 table coefficients; 32-bit fixed-point list positions",
+        "dtype": "f32 potential values; f64 derivatives, accumulation and integration; 32-bit fixed-point staged positions",
         "data": "synthetic: FCC lattice, seeded Maxwell-Boltzmann velocities at 300 K",
         "config": {
             "workload": f"EAM Cu cu500 table, FCC {n}^3 = {n_atoms} atoms, a=3.615 A, 300 K, dt=1 fs, "

@@ -136,6 +136,7 @@ def _load():
     hip.comd_batch_step.argtypes = [vp, C.c_int, C.c_double]
     hip.comd_autotune.argtypes = [vp, _ip, _ip, _ip, _dp]
     hip.comd_list_info.argtypes = [vp, _ip, _ip, _ip, _ip]
+    hip.comd_list_occupancy.argtypes = [vp, _ip, _ip, _ip]
     hip.comd_set_cheb.argtypes = [vp, _dp, C.c_int]
     hip.comd_get_cheb.argtypes = [vp, _dp]
     hip.comd_eam_cheb_grad.argtypes = [vp, _dp, C.c_int, _dp, C.c_double, C.c_double, _dp, _dp]
@@ -442,6 +443,13 @@ class Simulation:
         a, v, b, k = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         self._check(self._hip.comd_list_info(self.ctx, C.byref(a), C.byref(v), C.byref(b), C.byref(k)))
         return dict(active=a.value, valid=v.value, builds=b.value, kmax=k.value)
+
+    def list_occupancy(self):
+        """Neighbour-list passes: dict(blocks_per_cu, lds_bytes, threads)
+        (comd_list_occupancy); lists of two (density or LJ, embedding force)."""
+        b, l, t = (C.c_int * 2)(), (C.c_int * 2)(), C.c_int()
+        self._check(self._hip.comd_list_occupancy(self.ctx, b, l, C.byref(t)))
+        return dict(blocks_per_cu=list(b), lds_bytes=list(l), threads=t.value)
 
     def autotune(self):
         """Time the force passes over launch shapes and keep the fastest

@@ -393,7 +393,7 @@ int comd_grid_for(comd_params *p, const double bounds[3], double defgrad, double
 
 static void free_nl(comd_ctx *c)
 {
-    void *p[] = {c->nl.pos,      c->nl.selfh,    c->nl.ref,       c->nl.slot,    c->nl.order,
+    void *p[] = {c->nl.pos,      c->nl.selfh,    c->nl.ref,       c->nl.slot,    c->nl.order,    c->nl.fq,
                  c->nl.nbl,      c->nl.ncnt,     c->nl.aux_cnt,   c->nl.aux_n,   c->nl.aux_start,
                  c->nl.aux_mem,  c->nl.gstart,   c->nl.nonempty,  c->nl.count,   c->nl.flags,
                  (void *)c->nl.cell_order, (void *)c->nl.group_q0};
@@ -802,9 +802,9 @@ static int nl_setup(comd_ctx *c, long nids)
         const double occ = (double)c->ntot / (double)naux;
         const long slots = (long)(gd[0] + 2) * (gd[1] + 2) * (gd[2] + 2);
         hcap = (((long)(1.2 * occ * (double)slots) + 64 + 63) / 64) * 64;
-        if (hcap * hb <= lds_avail) break;
+        if (hcap * (hb + 1) <= lds_avail) break;  // halo entries and their slot map
     }
-    if (hcap * hb > lds_avail || hcap > 65535) return COMD_OK;  // too dense for staged halos
+    if (hcap * (hb + 1) > lds_avail || hcap > 65535) return COMD_OK;  // too dense for staged halos
     const double occ = (double)c->ntot / (double)naux;
     int ng[3];
     for (int j = 0; j < 3; j++) ng[j] = (na[j] + gd[j] - 1) / gd[j];
@@ -821,6 +821,7 @@ static int nl_setup(comd_ctx *c, long nids)
         alloc((void **)&c->nl.ref, n * sizeof(double4));
         alloc((void **)&c->nl.slot, n * 4);
         alloc((void **)&c->nl.order, n * 4);
+        alloc((void **)&c->nl.fq, n * 8);
         alloc((void **)&c->nl.nbl, n * (size_t)kmax * 2);
         alloc((void **)&c->nl.ncnt, n * 4);
         alloc((void **)&c->nl.aux_cnt, (size_t)naux * 4);
@@ -873,12 +874,14 @@ static int nl_setup(comd_ctx *c, long nids)
     {
         // 32-bit fixed point of the mixed passes' group frames: a halo atom
         // lies within (gd/2 + 1) cells of the group centre plus the half skin
-        // it may have moved since the build; 5 % to spare
+        // it may have moved since the build; 5 % to spare.  A power of two,
+        // so that scaling by it or its inverse is exact: the passes then
+        // work in fixed-point units and fold the scale into constants
         double ext = 0.0;
         const double ab[3] = {l.ab0, l.ab1, l.ab2};
         for (int j = 0; j < 3; j++) ext = std::max(ext, (0.5 * gd[j] + 1.0) * ab[j]);
         ext += 1e-3 * c->nl_skin_milli;
-        l.qscale = 2147483648.0 / (1.05 * ext);
+        l.qscale = std::ldexp(1.0, (int)std::floor(std::log2(2147483648.0 / (1.05 * ext))));
         l.qinv = 1.0 / l.qscale;
     }
     l.lds_avail = (int)lds_avail;
@@ -1423,6 +1426,21 @@ int comd_list_info(comd_ctx *c, int *active, int *valid, int *builds, int *kmax)
     if (valid) *valid = v;
     if (builds) *builds = nb;
     if (kmax) *kmax = c->nl_active ? c->nl.kmax : 0;
+    return COMD_OK;
+}
+
+int comd_list_occupancy(comd_ctx *c, int *blocks_per_cu, int *lds_bytes, int *threads)
+{
+    if (!c || !blocks_per_cu || !lds_bytes || !threads) return COMD_ERR_ARG;
+    for (int k = 0; k < 2; k++) blocks_per_cu[k] = lds_bytes[k] = 0;
+    *threads = 0;
+    if (!c->nl_active) return COMD_OK;
+    const PotDev &P = c->reps[0].pot;
+    const int mixed = c->precision == COMD_PREC_MIXED;
+    const int kinds[2] = {P.kind == COMD_POT_LJ ? CM_KIND_LJ : CM_KIND_EAM1, CM_KIND_EAM3};
+    for (int k = 0; k < (P.kind == COMD_POT_LJ ? 1 : 2); k++)
+        blocks_per_cu[k] = cm_nl_occupancy(P, c->nl, kinds[k], mixed, &lds_bytes[k]);
+    *threads = c->nl.pass_threads;
     return COMD_OK;
 }
 

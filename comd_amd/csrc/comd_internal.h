@@ -143,7 +143,8 @@ struct CmList {
     int build_blocks, pass_blocks, pass_threads;
     const int *cell_order;   // [naux]: auxiliary cells in build order (group, then cell)
     const int *group_q0;     // [ngroups + 1]: a group's first position in cell_order
-    double4 *pos, *ref;      // per id (pos.w: F' of the last density pass)
+    double4 *pos, *ref;      // per id (w unused)
+    double *fq;              // per position in build order: F' of the last density pass
     double qscale, qinv;     // group-frame fixed point of the mixed passes: units per A and A per unit
     uint16_t *selfh;         // per t: the atom's own index in its group's staged halo
     int *slot;               // per id: current slot, -1 when absent
@@ -172,3 +173,6 @@ void cm_nl_sync(const DevState &d, const CmList &l, hipStream_t st);
 size_t cm_nl_build_lds(const CmList &l);
 size_t cm_nl_pass_lds(const PotDev &P, const CmList &l, int kind, int mixed, int *tables_in_lds);
 void cm_nl_force(const DevState &d, const PotDev &P, const CmList &l, int kind, int mixed, hipStream_t st);
+// Resident blocks per CU of a list pass (-1 if the runtime cannot tell) and
+// its dynamic LDS bytes.
+int cm_nl_occupancy(const PotDev &P, const CmList &l, int kind, int mixed, int *lds_bytes);

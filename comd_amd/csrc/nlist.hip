@@ -31,6 +31,8 @@
 #include "comd_internal.h"
 #include "pair_terms.h"
 
+#include <type_traits>
+
 __device__ __forceinline__ bool nl_rebuild(const CmList &l) { return (l.flags[0] & CM_NL_REBUILD) != 0; }
 
 // x wrapped into [0, L) (to rounding); iL = 1 / L
@@ -140,8 +142,9 @@ struct GroupLds {
 };
 
 // Fills S for group g (all threads of the block; starts and ends with a
-// barrier).
-__device__ __forceinline__ void group_setup(const CmList &l, int g, GroupLds &S)
+// barrier), and, when slot_of is given, the slot of each staged halo entry
+// (entries below hcap).
+__device__ __forceinline__ void group_setup(const CmList &l, int g, GroupLds &S, CM_LDS uint8_t *slot_of = nullptr)
 {
     __syncthreads();
     const GroupGeo G = group_geo(l, g);
@@ -178,6 +181,13 @@ __device__ __forceinline__ void group_setup(const CmList &l, int g, GroupLds &S)
         }
     }
     __syncthreads();
+    if (slot_of) {
+        for (int s = threadIdx.x; s < ns; s += blockDim.x) {
+            const int h1 = min(S.off[s + 1], l.hcap);
+            for (int h = S.off[s]; h < h1; h++) slot_of[h] = (uint8_t)s;
+        }
+        __syncthreads();
+    }
 }
 
 // Slot of staged halo entry h (off[] ascending, 0 <= h < off[ns]).
@@ -449,7 +459,9 @@ static inline size_t nl_tab_bytes(const PotDev &P, int kind, int mixed)
 // Staged halo bytes per atom: group-frame fixed point x, y, z and float F'
 // (mixed); group-frame double x, y, z and, for the embedding-force pass, F'
 // (FP64).
-static inline size_t nl_halo_bytes(int kind, int mixed) { return mixed ? 16 : (kind == CM_KIND_EAM3 ? 32 : 24); }
+__host__ __device__ constexpr int nl_halo_bytes(int kind, int mixed) { return mixed ? 16 : (kind == CM_KIND_EAM3 ? 32 : 24); }
+// Bytes of the staged halo entries' slot map (one byte per entry).
+__host__ __device__ constexpr int nl_slotmap_bytes(int hcap) { return (hcap + 15) / 16 * 16; }
 
 // One home atom t over its list.  TB: the table pointers (compact rows in
 // LDS when COMPACT, else the full tables in global memory); hq (mixed) or
@@ -476,44 +488,89 @@ __device__ __forceinline__ void nl_atom(const DevState &d, const PotDev &P, cons
     if (KIND != CM_KIND_EAM3) fia = 0.0;
     const double rc2 = P.rc2;
     const double qinv = l.qinv;
+    // mixed passes over compact rows work in fixed-point units (qinv is a
+    // power of two: the cutoff test and the force terms are the same as in
+    // angstrom; the table coordinate takes the unit scale)
+    constexpr bool UNITS = MIXED && COMPACT && KIND != CM_KIND_LJ;
+    const double rc2u = rc2 * (l.qscale * l.qscale);
+    const double iu = qinv * P.invdx[0], cu = -P.x0[0] * P.invdx[0];
     const int cnt = l.ncnt[t];
     Acc A = {0, 0, 0, 0, 0, 0};
     int npair = 0;
-    // 8 entries per 16-byte load, then 8 halo reads from LDS
+    // 8 entries per 16-byte load, then the 8 halo reads from LDS, all issued
+    // before the first pair is evaluated.  Whole chunks of 8 first (the next
+    // chunk's load issued before this one is processed), then the partial
+    // last chunk, whose entries past cnt are not this build's: entry 0 of
+    // the halo is read in their place and they count as out of range.
     const CM_GLOBAL v4u *row = (const CM_GLOBAL v4u *)l.nbl + t;
-    for (int k0 = 0; k0 < cnt; k0 += 8) {
-        const v4u e = row[(size_t)(k0 >> 3) * l.n];
+    auto chunk = [&](const v4u &e, int k0, auto tail) {
+        constexpr bool TAIL = decltype(tail)::value;
         const unsigned ew[4] = {e.x, e.y, e.z, e.w};
+        int hj[8];
 #pragma unroll
         for (int u = 0; u < 8; u++) {
-            // entries past cnt are not this build's: read entry 0 instead
-            // (valid entries are below the staged halo size, <= hcap)
-            const bool live = k0 + u < cnt;
-            const int hj = live ? (int)((ew[u >> 1] >> (16 * (u & 1))) & 0xffffu) : 0;
-            double dx, dy, dz, fij;
-            if (MIXED) {
+            hj[u] = (int)((ew[u >> 1] >> (16 * (u & 1))) & 0xffffu);
+            if (TAIL && k0 + u >= cnt) hj[u] = 0;
+        }
+        if constexpr (MIXED) {
+            v4u qj[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) qj[u] = hq[hj[u]];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
                 // fixed-point differences (modulo 2^32, exact for pairs on
-                // the lists, which are well within range); one rounding
-                const v4u qj = hq[hj];
-                dx = (double)(int)(qi.x - qj.x) * qinv;
-                dy = (double)(int)(qi.y - qj.y) * qinv;
-                dz = (double)(int)(qi.z - qj.z) * qinv;
-                fij = (double)__uint_as_float(qj.w);
-            } else {
-                dx = xi - hx[hj];
-                dy = yi - hy[hj];
-                dz = zi - hz[hj];
-                fij = KIND == CM_KIND_EAM3 ? hf[hj] : 0.0;
+                // the lists, which are well within range)
+                const double dxu = (double)(int)(qi.x - qj[u].x);
+                const double dyu = (double)(int)(qi.y - qj[u].y);
+                const double dzu = (double)(int)(qi.z - qj[u].z);
+                const double fij = (double)__uint_as_float(qj[u].w);
+                const double r2u = dxu * dxu + dyu * dyu + dzu * dzu;
+                const bool in = (!TAIL || k0 + u < cnt) && (KIND == CM_KIND_EAM3 ? (r2u < rc2u) : (r2u <= rc2u));
+                if (in) {
+                    if constexpr (UNITS) {
+                        pair_term_units<KIND>(P, T, dxu, dyu, dzu, r2u, iu, cu, fia, fij, A);
+                    } else {
+                        const double dx = dxu * qinv, dy = dyu * qinv, dz = dzu * qinv;
+                        const double r2 = r2u * (qinv * qinv);
+                        if constexpr (COMPACT) pair_term_compact<KIND, MIXED>(P, T, dx, dy, dz, r2, fia, fij, A);
+                        else pair_term<KIND, EVAL, MIXED>(P, T, dx, dy, dz, r2, fia, fij, A);
+                    }
+                    npair++;
+                }
             }
-            const double r2 = dx * dx + dy * dy + dz * dz;
-            const bool in = live && (KIND == CM_KIND_EAM3 ? (r2 < rc2) : (r2 <= rc2));
-            if (in) {
-                if constexpr (COMPACT) pair_term_compact<KIND, MIXED>(P, T, dx, dy, dz, r2, fia, fij, A);
-                else pair_term<KIND, EVAL, MIXED>(P, T, dx, dy, dz, r2, fia, fij, A);
-                npair++;
+        } else {
+            double xj[8], yj[8], zj[8], fj[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                xj[u] = hx[hj[u]];
+                yj[u] = hy[hj[u]];
+                zj[u] = hz[hj[u]];
+                fj[u] = KIND == CM_KIND_EAM3 ? hf[hj[u]] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const double dx = xi - xj[u], dy = yi - yj[u], dz = zi - zj[u];
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                const bool in = (!TAIL || k0 + u < cnt) && (KIND == CM_KIND_EAM3 ? (r2 < rc2) : (r2 <= rc2));
+                if (in) {
+                    if constexpr (COMPACT) pair_term_compact<KIND, MIXED>(P, T, dx, dy, dz, r2, fia, fj[u], A);
+                    else pair_term<KIND, EVAL, MIXED>(P, T, dx, dy, dz, r2, fia, fj[u], A);
+                    npair++;
+                }
             }
         }
+    };
+    // the next chunk's index clamped to the list capacity: always a valid load
+    const int clast = l.kmax / 8 - 1;
+    v4u e = row[0];
+    int k0 = 0;
+    for (; k0 + 8 <= cnt; k0 += 8) {
+        const v4u en = row[(size_t)min((k0 >> 3) + 1, clast) * l.n];
+        chunk(e, k0, std::false_type());
+        e = en;
     }
+    if (k0 < cnt) chunk(e, k0, std::true_type());
+    if (UNITS) A.vir *= qinv;
     if (KIND == CM_KIND_EAM1) {
         // pass 2, src-flat/eam.c:576-594
         double fv, fp;
@@ -530,7 +587,7 @@ __device__ __forceinline__ void nl_atom(const DevState &d, const PotDev &P, cons
         d.rho[s] = A.rho;
         d.fi[s] = fp;
         d.nnb[s] = npair;
-        ((double *)&l.pos[i])[3] = fp;  // staged with the position by the embedding-force pass
+        l.fq[t] = fp;  // staged with the position by the embedding-force pass (coalesced: t is the thread's)
     } else if (KIND == CM_KIND_EAM3) {
         d.fx[s] += A.x;
         d.fy[s] += A.y;
@@ -551,14 +608,17 @@ __device__ __forceinline__ void nl_atom(const DevState &d, const PotDev &P, cons
 // at the centre of the group's cells), then each thread takes home atoms.
 // TLDS: the compact table rows are staged in LDS (1), or the full tables are
 // read from global memory (0); a compile-time choice, so the row reads are
-// LDS or global loads.  Dynamic LDS: [rows (TLDS)] [halo, hcap atoms].
+// LDS or global loads.  Dynamic LDS: [rows (TLDS)] [halo, hcap atoms]
+// [slot of each halo entry, hcap bytes].
 template <int KIND, int EVAL, int MIXED, int TLDS>
-__global__ __launch_bounds__(512) void k_nl_pair(DevState d, PotDev P, CmList l, int tab_bytes)
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(MIXED && KIND == CM_KIND_EAM3 && TLDS ? 6 : 1)))
+void k_nl_pair(DevState d, PotDev P, CmList l, int tab_bytes)
 {
     extern __shared__ __align__(16) char nls[];
     __shared__ GroupLds S;
     if (l.flags[0] & CM_NL_INVALID) return;
     CM_LDS char *halo = (CM_LDS char *)nls + tab_bytes;
+    CM_LDS uint8_t *slot_of = (CM_LDS uint8_t *)(halo + (size_t)l.hcap * nl_halo_bytes(KIND, MIXED));
     CM_LDS v4u *hq = (CM_LDS v4u *)halo;
     CM_LDS double *hx = (CM_LDS double *)halo, *hy = hx + l.hcap, *hz = hy + l.hcap, *hf = hz + l.hcap;
     CompactLds TL;
@@ -588,30 +648,54 @@ __global__ __launch_bounds__(512) void k_nl_pair(DevState d, PotDev P, CmList l,
         const int nb = gridDim.x, qb = nb / 8, rem = nb % 8;
         const int xcd = blockIdx.x % 8, idx = blockIdx.x / 8;
         const int g = nb < 8 ? (int)blockIdx.x : xcd * qb + min(xcd, rem) + idx;
-        group_setup(l, g, S);  // its first barrier also covers the table staging
+        group_setup(l, g, S, slot_of);  // its first barrier also covers the table staging
         const GroupGeo G = group_geo(l, g);
         const double ox = (G.x0 + 0.5 * G.lx) * l.ab0, oy = (G.y0 + 0.5 * G.ly) * l.ab1,
                      oz = (G.z0 + 0.5 * G.lz) * l.ab2;
         const int H = min(S.off[S.ns], l.hcap);  // never above hcap while the lists are valid
-        for (int h = threadIdx.x; h < H; h += blockDim.x) {
-            const int s = halo_slot(S, h);
-            const int j = l.order[S.start[s] + (h - S.off[s])];
-            const v4d p = ((const CM_GLOBAL v4d *)l.pos)[j];
-            double x, y, z;
-            frame_pos(l, G, S.uxyz[s], p, ox, oy, oz, x, y, z);
-            if (MIXED) {
-                const float fj = (float)p.w;
-                v4u q;
-                q.x = (unsigned)(int)rint(x * l.qscale);
-                q.y = (unsigned)(int)rint(y * l.qscale);
-                q.z = (unsigned)(int)rint(z * l.qscale);
-                q.w = __float_as_uint(fj);
-                hq[h] = q;
-            } else {
-                hx[h] = x;
-                hy[h] = y;
-                hz[h] = z;
-                if (KIND == CM_KIND_EAM3) hf[h] = p.w;
+        // up to SB entries per thread in flight: their id and position loads
+        // are issued together (a thread stages about hcap / blockDim of
+        // them; one at a time in the mixed embedding-force pass, which keeps
+        // to 80 registers for six waves per SIMD)
+        constexpr int SB = MIXED && KIND == CM_KIND_EAM3 ? 1 : 6;
+        for (int h0 = threadIdx.x; h0 < H; h0 += SB * blockDim.x) {
+            // slots first, then the id loads, then the position loads
+            int sl[SB], q[SB], id[SB];
+#pragma unroll
+            for (int k = 0; k < SB; k++) {
+                const int h = min(h0 + k * (int)blockDim.x, H - 1);
+                sl[k] = slot_of[h];
+                q[k] = S.start[sl[k]] + (h - S.off[sl[k]]);
+            }
+#pragma unroll
+            for (int k = 0; k < SB; k++) id[k] = l.order[q[k]];
+            v4d p[SB];
+            double fp[SB];
+#pragma unroll
+            for (int k = 0; k < SB; k++) {
+                p[k] = ((const CM_GLOBAL v4d *)l.pos)[id[k]];
+                fp[k] = KIND == CM_KIND_EAM3 ? ((const CM_GLOBAL double *)l.fq)[q[k]] : 0.0;
+            }
+#pragma unroll
+            for (int k = 0; k < SB; k++) {
+                const int h = h0 + k * (int)blockDim.x;
+                if (h >= H) break;
+                double x, y, z;
+                frame_pos(l, G, S.uxyz[sl[k]], p[k], ox, oy, oz, x, y, z);
+                if (MIXED) {
+                    const float fj = (float)fp[k];
+                    v4u q;
+                    q.x = (unsigned)(int)rint(x * l.qscale);
+                    q.y = (unsigned)(int)rint(y * l.qscale);
+                    q.z = (unsigned)(int)rint(z * l.qscale);
+                    q.w = __float_as_uint(fj);
+                    hq[h] = q;
+                } else {
+                    hx[h] = x;
+                    hy[h] = y;
+                    hz[h] = z;
+                    if (KIND == CM_KIND_EAM3) hf[h] = fp[k];
+                }
             }
         }
         __syncthreads();
@@ -655,7 +739,7 @@ void cm_nl_build(const DevState &d, const CmList &l, hipStream_t st)
 
 size_t cm_nl_pass_lds(const PotDev &P, const CmList &l, int kind, int mixed, int *tables_in_lds)
 {
-    const size_t halo = (size_t)l.hcap * nl_halo_bytes(kind, mixed);
+    const size_t halo = (size_t)l.hcap * nl_halo_bytes(kind, mixed) + nl_slotmap_bytes(l.hcap);
     size_t tab = 0;
     if (P.eval == COMD_EVAL_TABLE && kind != CM_KIND_LJ && P.cd64) tab = nl_tab_bytes(P, kind, mixed);
     // rows in LDS whenever they fit beside the halo (unless tuned off)
@@ -675,6 +759,40 @@ static void nl_launch(const DevState &d, const PotDev &P, const CmList &l, hipSt
         hipLaunchKernelGGL((k_nl_pair<KIND, EVAL, MIXED, 1>), grid, block, lds, st, d, P, l, tab);
     else
         hipLaunchKernelGGL((k_nl_pair<KIND, EVAL, MIXED, 0>), grid, block, lds, st, d, P, l, 0);
+}
+
+// Resident blocks per CU of the pass nl_launch would start, and its dynamic LDS.
+template <int KIND, int EVAL, int MIXED>
+static int nl_occupancy(const PotDev &P, const CmList &l, int *lds_bytes)
+{
+    int tl = 0;
+    const size_t lds = cm_nl_pass_lds(P, l, KIND, MIXED, &tl);
+    const void *fn = (EVAL == COMD_EVAL_TABLE && KIND != CM_KIND_LJ && tl)
+                         ? (const void *)k_nl_pair<KIND, EVAL, MIXED, 1>
+                         : (const void *)k_nl_pair<KIND, EVAL, MIXED, 0>;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, l.pass_threads, lds) != hipSuccess) nb = -1;
+    if (lds_bytes) *lds_bytes = (int)lds;
+    return nb;
+}
+
+int cm_nl_occupancy(const PotDev &P, const CmList &l, int kind, int mixed, int *lds_bytes)
+{
+    if (kind == CM_KIND_LJ)
+        return mixed ? nl_occupancy<CM_KIND_LJ, COMD_EVAL_TABLE, 1>(P, l, lds_bytes)
+                     : nl_occupancy<CM_KIND_LJ, COMD_EVAL_TABLE, 0>(P, l, lds_bytes);
+    if (P.eval == COMD_EVAL_CHEB) {
+        if (kind == CM_KIND_EAM1)
+            return mixed ? nl_occupancy<CM_KIND_EAM1, COMD_EVAL_CHEB, 1>(P, l, lds_bytes)
+                         : nl_occupancy<CM_KIND_EAM1, COMD_EVAL_CHEB, 0>(P, l, lds_bytes);
+        return mixed ? nl_occupancy<CM_KIND_EAM3, COMD_EVAL_CHEB, 1>(P, l, lds_bytes)
+                     : nl_occupancy<CM_KIND_EAM3, COMD_EVAL_CHEB, 0>(P, l, lds_bytes);
+    }
+    if (kind == CM_KIND_EAM1)
+        return mixed ? nl_occupancy<CM_KIND_EAM1, COMD_EVAL_TABLE, 1>(P, l, lds_bytes)
+                     : nl_occupancy<CM_KIND_EAM1, COMD_EVAL_TABLE, 0>(P, l, lds_bytes);
+    return mixed ? nl_occupancy<CM_KIND_EAM3, COMD_EVAL_TABLE, 1>(P, l, lds_bytes)
+                 : nl_occupancy<CM_KIND_EAM3, COMD_EVAL_TABLE, 0>(P, l, lds_bytes);
 }
 
 template <int KIND, int EVAL>

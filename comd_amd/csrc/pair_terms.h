@@ -261,3 +261,39 @@ __device__ __forceinline__ void pair_term_compact(const PotDev &P, const CT &T, 
         A.vir += g * dx * dx;
     }
 }
+
+// The same for the mixed passes, with the pair's separation in fixed-point
+// units (dx = dxu * qinv, r2 = r2u * qinv^2, qinv a power of two): 1/r in
+// units yu = y * qinv, so the force terms dphi * y * dx equal dphi * yu * dxu
+// exactly and the unit scale only enters the table coordinate, through
+// iu = qinv * invDx.  The virial sum comes out in units of 1/qinv (the
+// caller scales it).
+template <int KIND, class CT>
+__device__ __forceinline__ void pair_term_units(const PotDev &P, const CT &T, double dxu, double dyu, double dzu,
+                                                double r2u, double iu, double cu, double fia, double fij, Acc &A)
+{
+    const double yu = rsqrt_mixed(r2u);
+    const double u = fmax(fma(r2u * yu, iu, cu), 0.0);
+    const int i = min((int)u, P.n[0] - 1);
+    const double t = u - (double)i;
+    if (KIND == CM_KIND_EAM1) {
+        // row offset in doubles by a 24-bit multiply (i < n < 2^24)
+        const int o = __mul24(i, 5);
+        const double pd0 = T.d64[o + 0], pd1 = T.d64[o + 1];
+        const float f = (float)t;
+        const float pc0 = T.dm[2 * o + 4], pc1 = T.dm[2 * o + 5], pc2 = T.dm[2 * o + 6],
+                    rc0 = T.dm[2 * o + 7], rc1 = T.dm[2 * o + 8], rc2 = T.dm[2 * o + 9];
+        const double s = fma(t, pd1, pd0) * yu;
+        A.x += s * dxu;
+        A.y += s * dyu;
+        A.z += s * dzu;
+        A.e += 0.5 * (double)fmaf(f, fmaf(f, pc2, pc1), pc0);
+        A.rho += (double)fmaf(f, fmaf(f, rc2, rc1), rc0);
+    } else {
+        const double g = (fia + fij) * fma(t, T.r64[2 * i + 1], T.r64[2 * i]) * yu;
+        A.x += g * dxu;
+        A.y += g * dyu;
+        A.z += g * dzu;
+        A.vir += g * dxu * dxu;
+    }
+}

@@ -191,6 +191,14 @@ int comd_autotune(comd_ctx *ctx, int *waves, int *blocks_per_cu, int *balanced, 
  * kernels run), how many builds have run, and the list capacity per atom. */
 int comd_list_info(comd_ctx *ctx, int *active, int *valid, int *builds, int *kmax);
 
+/* Launch shape of the neighbour-list passes at the current precision: for
+ * the density (or Lennard-Jones) pass and the embedding-force pass, the
+ * blocks resident per compute unit (as the HIP occupancy calculator gives
+ * it; -1 when it cannot) and the dynamic LDS bytes per block, and the
+ * threads per block.  All zero while the lists are off.  A diagnostic for
+ * tuning the group shape; no counterpart in the reference. */
+int comd_list_occupancy(comd_ctx *ctx, int blocks_per_cu[2], int lds_bytes[2], int *threads);
+
 /* Number of atom pairs inside the cutoff found by the last EAM density or
  * LJ pass (each unordered pair once: the reference's half-pair count). */
 int comd_pair_count(comd_ctx *ctx, long long *half_pairs);
