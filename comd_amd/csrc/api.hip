@@ -526,15 +526,19 @@ static bool setup_pot(const comd_params *p, Replica &R)
     }
     P.same_grid = p->tab_n[0] == p->tab_n[1] && p->tab_x0[0] == p->tab_x0[1] && p->tab_xn[0] == p->tab_xn[1] &&
                   p->tab_invdx[0] == p->tab_invdx[1];
-    if (ok && P.same_grid) {
-        // the rows the list passes read, packed per interval
+    // compact rows need every pair inside the cutoff on the tables
+    // (r <= cutoff <= xn), so that the interval index stays <= n
+    if (ok && P.same_grid && p->cutoff <= p->tab_xn[0]) {
+        // the rows the list passes read, packed per interval, and one more:
+        // the last interval's polynomials re-centred on its end (t -> 1 + t),
+        // read at r = xn (interval n, t = 0: the reference's value there)
         std::vector<double> f64, g64, d64, r64;
         std::vector<float> f32, g32;
         table_coefs(p->tab_n[0], p->tab_invdx[0], p->tab_values[0], f64, f32);
         table_coefs(p->tab_n[1], p->tab_invdx[1], p->tab_values[1], g64, g32);
         const int n = p->tab_n[0];
-        d64.resize((size_t)8 * n);
-        r64.resize((size_t)2 * n);
+        d64.resize((size_t)8 * (n + 1));
+        r64.resize((size_t)2 * (n + 1));
         for (int i = 0; i < n; i++) {
             const double row[8] = {f64[6 * i + 0], f64[6 * i + 1], f64[6 * i + 2], f64[6 * i + 3],
                                    f64[6 * i + 4], g64[6 * i + 2], g64[6 * i + 3], g64[6 * i + 4]};
@@ -542,9 +546,23 @@ static bool setup_pot(const comd_params *p, Replica &R)
             r64[2 * i + 0] = g64[6 * i + 0];
             r64[2 * i + 1] = g64[6 * i + 1];
         }
+        {
+            const double *a = &d64[8 * (n - 1)], *b = &r64[2 * (n - 1)];
+            double *e = &d64[8 * n], *f = &r64[2 * n];
+            e[0] = a[0] + a[1];  // phi': d0 + d1 t
+            e[1] = a[1];
+            e[2] = a[2] + a[3] + a[4];  // phi: c0 + t (c1 + t c2)
+            e[3] = a[3] + 2.0 * a[4];
+            e[4] = a[4];
+            e[5] = a[5] + a[6] + a[7];  // rho
+            e[6] = a[6] + 2.0 * a[7];
+            e[7] = a[7];
+            f[0] = b[0] + b[1];  // rho'
+            f[1] = b[1];
+        }
         // mixed density rows: phi d0 d1 (double), then 6 floats (40 bytes)
-        std::vector<float> dm((size_t)10 * n);
-        for (int i = 0; i < n; i++) {
+        std::vector<float> dm((size_t)10 * (n + 1));
+        for (int i = 0; i <= n; i++) {
             memcpy(&dm[10 * i], &d64[8 * i], 2 * sizeof(double));
             for (int q = 0; q < 6; q++) dm[10 * i + 4 + q] = (float)d64[8 * i + 2 + q];
         }
@@ -790,10 +808,10 @@ static int nl_setup(comd_ctx *c, long nids)
     // whole block for a sliver of atoms.
     int na[3] = {0, 0, 0}, gd[3] = {0, 0, 0};
     long hcap = 0, naux = 0;
-    const int gopts[4][3] = {{4, 4, 2}, {4, 2, 2}, {2, 2, 2}, {2, 2, 1}};
+    const int gopts[5][3] = {{4, 4, 3}, {4, 4, 2}, {4, 2, 2}, {2, 2, 2}, {2, 2, 1}};
     const long lds_avail = (long)c->lds_block - CM_NL_STATIC_LDS;
     const long hb = c->precision == COMD_PREC_MIXED ? 16 : 32;
-    for (int o = c->nl_group_opt >= 0 ? c->nl_group_opt : 0; o < 4; o++) {
+    for (int o = c->nl_group_opt >= 0 ? c->nl_group_opt : 0; o < 5; o++) {
         for (int j = 0; j < 3; j++) {
             gd[j] = std::min(gopts[o][j], nmax[j]);
             na[j] = nmax[j] / gd[j] * gd[j];
@@ -889,7 +907,7 @@ static int nl_setup(comd_ctx *c, long nids)
     l.build_blocks = (int)std::min<long>(ngroups, (long)c->ncu * 2);
     l.pass_blocks = (int)ngroups;
     // threads per pass block: a group's home atoms with 10 % to spare
-    l.pass_threads = (int)std::min<long>(512, std::max<long>(64, ((long)(1.1 * occ * gd[0] * gd[1] * gd[2]) + 63) / 64 * 64));
+    l.pass_threads = (int)std::min<long>(768, std::max<long>(64, ((long)(1.1 * occ * gd[0] * gd[1] * gd[2]) + 63) / 64 * 64));
     {
         // auxiliary cells in build order: groups (x fastest), then the
         // group's cells (x fastest)
@@ -1327,7 +1345,7 @@ int comd_tune(comd_ctx *c, int key, int value)
         invalidate_graph(c);
         return upload_descriptors(c);
     case COMD_TUNE_LIST_GROUPS:
-        if (value < -1 || value > 3) return set_err(c, COMD_ERR_ARG, "list group option must be -1..3");
+        if (value < -1 || value > 4) return set_err(c, COMD_ERR_ARG, "list group option must be -1..4");
         c->nl_group_opt = value;
         if (c->have_atoms) return nl_setup(c, c->nl_ids);
         break;

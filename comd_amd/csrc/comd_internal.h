@@ -19,10 +19,11 @@ struct PotDev {
     const double *cf64[2];
     const float *cf32[2];
     int same_grid;  // phi and rho tables share x0, xn, invDx and n
-    // compact rows for the neighbour-list passes (only when same_grid), per
-    // interval: density pass, FP64 (phi d0 d1 c0 c1 c2, rho c0 c1 c2) and
-    // mixed (phi d0 d1 in double, then phi c0 c1 c2, rho c0 c1 c2 in float:
-    // 40 bytes); embedding-force pass (rho d0 d1) in double
+    // compact rows for the neighbour-list passes (only when same_grid and
+    // cutoff <= xn), per interval and one more (the last interval re-centred
+    // on its end, read at r = xn): density pass, FP64 (phi d0 d1 c0 c1 c2,
+    // rho c0 c1 c2) and mixed (phi d0 d1 in double, then phi c0 c1 c2, rho
+    // c0 c1 c2 in float: 40 bytes); embedding-force pass (rho d0 d1) in double
     const double *cd64, *cr64;
     const float *cdm;
     int balanced;   // drain an atom's queues round-robin over its lanes
@@ -164,7 +165,7 @@ struct CmList {
     unsigned *flags;         // CM_NL_REBUILD, CM_NL_INVALID, CM_NL_OVERFLOW (during a build)
 };
 #define CM_NL_AUXCAP 64
-#define CM_NL_SLOTS 144                 // halo cells of a group, at most (4 + 2) x (4 + 2) x (2 + 2)
+#define CM_NL_SLOTS 180                 // halo cells of a group, at most (4 + 2) x (4 + 2) x (3 + 2)
 #define CM_NL_STATIC_LDS (6 << 10)      // static LDS of the list kernels, rounded up
 __host__ __device__ inline size_t nl_entry(int k, int t, int n) { return ((size_t)(k >> 3) * n + t) * 8 + (k & 7); }
 void cm_nl_init(const DevState &d, const CmList &l, hipStream_t st);

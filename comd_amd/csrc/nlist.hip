@@ -217,6 +217,7 @@ __device__ __forceinline__ void block_groups(int ngroups, int &g0, int &g1)
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef unsigned v4u __attribute__((ext_vector_type(4)));
+typedef unsigned v3u __attribute__((ext_vector_type(3)));
 
 // Position of halo entry h (slot s) in its group's frame: the nearest image
 // of the atom to its slot's cell, moved with the cell next to the group,
@@ -453,7 +454,7 @@ __global__ void k_nl_nonempty(DevState d, CmList l)
 // vector L1.
 static inline size_t nl_tab_bytes(const PotDev &P, int kind, int mixed)
 {
-    const size_t b = (size_t)P.n[0] * (kind == CM_KIND_EAM1 ? (mixed ? 40 : 64) : 16);
+    const size_t b = (size_t)(P.n[0] + 1) * (kind == CM_KIND_EAM1 ? (mixed ? 40 : 64) : 16);
     return (b + 15) / 16 * 16;  // the halo after the rows stays 16-byte aligned
 }
 // Staged halo bytes per atom: group-frame fixed point x, y, z and float F'
@@ -513,9 +514,18 @@ __device__ __forceinline__ void nl_atom(const DevState &d, const PotDev &P, cons
             if (TAIL && k0 + u >= cnt) hj[u] = 0;
         }
         if constexpr (MIXED) {
+            // x, y, z only (12-byte reads) unless F' is needed
             v4u qj[8];
 #pragma unroll
-            for (int u = 0; u < 8; u++) qj[u] = hq[hj[u]];
+            for (int u = 0; u < 8; u++) {
+                if constexpr (KIND == CM_KIND_EAM3) {
+                    qj[u] = hq[hj[u]];
+                } else {
+                    const v3u q3 = *(const CM_LDS v3u *)&hq[hj[u]];
+                    qj[u] = v4u{q3.x, q3.y, q3.z, 0u};
+                }
+            }
+            float ef = 0.0f, rf = 0.0f;  // the chunk's FP32 value sums (density pass)
 #pragma unroll
             for (int u = 0; u < 8; u++) {
                 // fixed-point differences (modulo 2^32, exact for pairs on
@@ -528,7 +538,7 @@ __device__ __forceinline__ void nl_atom(const DevState &d, const PotDev &P, cons
                 const bool in = (!TAIL || k0 + u < cnt) && (KIND == CM_KIND_EAM3 ? (r2u < rc2u) : (r2u <= rc2u));
                 if (in) {
                     if constexpr (UNITS) {
-                        pair_term_units<KIND>(P, T, dxu, dyu, dzu, r2u, iu, cu, fia, fij, A);
+                        pair_term_units<KIND>(P, T, dxu, dyu, dzu, r2u, iu, cu, fia, fij, A, ef, rf);
                     } else {
                         const double dx = dxu * qinv, dy = dyu * qinv, dz = dzu * qinv;
                         const double r2 = r2u * (qinv * qinv);
@@ -537,6 +547,10 @@ __device__ __forceinline__ void nl_atom(const DevState &d, const PotDev &P, cons
                     }
                     npair++;
                 }
+            }
+            if (UNITS && KIND == CM_KIND_EAM1) {
+                A.e += (double)ef;
+                A.rho += (double)rf;
             }
         } else {
             double xj[8], yj[8], zj[8], fj[8];
@@ -611,7 +625,7 @@ __device__ __forceinline__ void nl_atom(const DevState &d, const PotDev &P, cons
 // LDS or global loads.  Dynamic LDS: [rows (TLDS)] [halo, hcap atoms]
 // [slot of each halo entry, hcap bytes].
 template <int KIND, int EVAL, int MIXED, int TLDS>
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(MIXED && KIND == CM_KIND_EAM3 && TLDS ? 6 : 1)))
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(MIXED && KIND != CM_KIND_LJ && TLDS ? 6 : 1)))
 void k_nl_pair(DevState d, PotDev P, CmList l, int tab_bytes)
 {
     extern __shared__ __align__(16) char nls[];
@@ -626,13 +640,13 @@ void k_nl_pair(DevState d, PotDev P, CmList l, int tab_bytes)
     TL.dm = nullptr;
     if (TLDS) {
         if (MIXED && KIND == CM_KIND_EAM1) {
-            const int nv = P.n[0] * 10;
+            const int nv = (P.n[0] + 1) * 10;
             CM_LDS float *dst = (CM_LDS float *)nls;
             for (int k = threadIdx.x; k < nv; k += blockDim.x) dst[k] = P.cdm[k];
             TL.dm = dst;
             TL.d64 = (CM_LDS double *)dst;
         } else {
-            const int nv = P.n[0] * (KIND == CM_KIND_EAM1 ? 8 : 2);
+            const int nv = (P.n[0] + 1) * (KIND == CM_KIND_EAM1 ? 8 : 2);
             const double *src = KIND == CM_KIND_EAM1 ? P.cd64 : P.cr64;
             CM_LDS double *dst = (CM_LDS double *)nls;
             for (int k = threadIdx.x; k < nv; k += blockDim.x) dst[k] = src[k];

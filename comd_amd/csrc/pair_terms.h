@@ -18,12 +18,11 @@ __device__ __forceinline__ void interp64(const double *vg, double x0, double xn,
     const CM_GLOBAL double *v = (const CM_GLOBAL double *)vg;
     r = fmin(fmax(r, x0), xn);
     r = (r - x0) * invdx;
-    const double fl = floor(r);
-    int i = (int)fl;
-    r = r - fl;
-    // r <= xn keeps i <= n-1 for a consistent table header; the clamp only
-    // keeps a rounding or a bad header inside values[-1..n+1]
-    i = min(max(i, 0), n - 1);
+    // the reference reads interval i = n at r = xn (t = 0); the same value
+    // and derivative are the end (t = 1) of interval n - 1, whose stencil
+    // stays inside values[-1..n+1]
+    const int i = min(max((int)floor(r), 0), n - 1);
+    r = r - (double)i;
     const CM_GLOBAL double *w = v + 1 + i;
     const double vm = w[-1], v0 = w[0], v1 = w[1], v2 = w[2];
     const double g0 = v1 - vm;
@@ -38,9 +37,9 @@ __device__ __forceinline__ int table_interval(double r, double x0, double xn, do
 {
     r = fmin(fmax(r, x0), xn);
     const double u = (r - x0) * invdx;
-    const double fl = floor(u);
-    t = u - fl;
-    return min(max((int)fl, 0), n - 1);
+    const int i = min(max((int)floor(u), 0), n - 1);  // r = xn: the end of interval n - 1
+    t = u - (double)i;
+    return i;
 }
 
 // Table formula from per-interval coefficients (d0 d1 c0 c1 c2 0):
@@ -213,21 +212,27 @@ __device__ __forceinline__ void pair_term(const PotDev &P, const TB &T, double d
 // The table branch of pair_term for EAM with phi and rho on one grid, from
 // the compact FP64 rows (the formulas of src-flat/eam.c:401-434 and 521-545 /
 // 633-649).  The interval and fraction: u = (r - x0) invDx as one fma,
-// clamped below at 0 (r <= rc = xn holds for every accepted pair; u = n, at
-// r = xn, is evaluated as the end of interval n - 1, the same value and
-// derivative as the reference's i = n, t = 0).  The derivatives, whose pair
+// clamped below at 0; r <= rc <= xn holds for every accepted pair, so the
+// interval is at most n (r = xn: row n, the last interval re-centred on its
+// end, gives the reference's value at i = n, t = 0).  The derivatives, whose pair
 // terms cancel in the force sums, are FP64 in both modes; MIXED evaluates the
 // density pass's values (phi, rho) in FP32 from 40-byte rows, which halves
 // the rows' LDS traffic.
+__device__ __forceinline__ int compact_interval(double u, double &t)
+{
+    u = fmax(u, 0.0);
+    t = __builtin_amdgcn_fract(u);  // u - floor(u), exact
+    return (int)u;
+}
+
 template <int KIND, int MIXED, class CT>
 __device__ __forceinline__ void pair_term_compact(const PotDev &P, const CT &T, double dx, double dy, double dz,
                                                   double r2, double fia, double fij, Acc &A)
 {
     const double y = MIXED ? rsqrt_mixed(r2) : rsqrt64(r2);  // 1/r
     const double r = r2 * y;
-    const double u = fmax(fma(r, P.invdx[0], -P.x0[0] * P.invdx[0]), 0.0);
-    const int i = min((int)u, P.n[0] - 1);
-    const double t = u - (double)i;
+    double t;
+    const int i = compact_interval(fma(r, P.invdx[0], -P.x0[0] * P.invdx[0]), t);
     if (KIND == CM_KIND_EAM1) {
         double phiv, dphi, rhov;
         if (MIXED) {
@@ -267,17 +272,19 @@ __device__ __forceinline__ void pair_term_compact(const PotDev &P, const CT &T, 
 // units yu = y * qinv, so the force terms dphi * y * dx equal dphi * yu * dxu
 // exactly and the unit scale only enters the table coordinate, through
 // iu = qinv * invDx.  The virial sum comes out in units of 1/qinv (the
-// caller scales it).
+// caller scales it).  The density pass's FP32 values (pair energy, density)
+// go to the caller's FP32 sums ef, rf over one list chunk (8 pairs), which
+// the caller adds to the FP64 sums.
 template <int KIND, class CT>
 __device__ __forceinline__ void pair_term_units(const PotDev &P, const CT &T, double dxu, double dyu, double dzu,
-                                                double r2u, double iu, double cu, double fia, double fij, Acc &A)
+                                                double r2u, double iu, double cu, double fia, double fij, Acc &A,
+                                                float &ef, float &rf)
 {
     const double yu = rsqrt_mixed(r2u);
-    const double u = fmax(fma(r2u * yu, iu, cu), 0.0);
-    const int i = min((int)u, P.n[0] - 1);
-    const double t = u - (double)i;
+    double t;
+    const int i = compact_interval(fma(r2u * yu, iu, cu), t);
     if (KIND == CM_KIND_EAM1) {
-        // row offset in doubles by a 24-bit multiply (i < n < 2^24)
+        // row offset in doubles by a 24-bit multiply (i <= n < 2^24)
         const int o = __mul24(i, 5);
         const double pd0 = T.d64[o + 0], pd1 = T.d64[o + 1];
         const float f = (float)t;
@@ -287,8 +294,8 @@ __device__ __forceinline__ void pair_term_units(const PotDev &P, const CT &T, do
         A.x += s * dxu;
         A.y += s * dyu;
         A.z += s * dzu;
-        A.e += 0.5 * (double)fmaf(f, fmaf(f, pc2, pc1), pc0);
-        A.rho += (double)fmaf(f, fmaf(f, rc2, rc1), rc0);
+        ef = fmaf(0.5f, fmaf(f, fmaf(f, pc2, pc1), pc0), ef);
+        rf += fmaf(f, fmaf(f, rc2, rc1), rc0);
     } else {
         const double g = (fia + fij) * fma(t, T.r64[2 * i + 1], T.r64[2 * i]) * yu;
         A.x += g * dxu;

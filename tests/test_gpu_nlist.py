@@ -124,10 +124,10 @@ def test_upload_path_sets_up_lists(gpu, cu500):
     assert_same(c, b, "uploaded state")
 
 
-@pytest.mark.parametrize("groups,rows", [(0, 1), (1, 1), (2, 0), (3, 1)])
+@pytest.mark.parametrize("groups,rows", [(0, 1), (1, 1), (2, 0), (3, 1), (4, 1)])
 @pytest.mark.parametrize("n", [5, 7])
 def test_group_options_equal_cells(gpu, cu500, groups, rows, n):
-    """Every group size (4x4x2 .. 2x2x1 auxiliary cells), table rows in LDS
+    """Every group size (4x4x3 .. 2x2x1 auxiliary cells), table rows in LDS
     or not, and boxes of 3 auxiliary cells per dimension (n = 5: a group's
     halo holds the same cell twice, at two images): the same pairs and
     forces as the link-cell kernels."""
