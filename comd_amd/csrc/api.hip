@@ -904,7 +904,6 @@ static int nl_setup(comd_ctx *c, long nids)
     }
     l.lds_avail = (int)lds_avail;
     l.rows_lds = c->nl_rows_lds;
-    l.build_blocks = (int)std::min<long>(ngroups, (long)c->ncu * 2);
     l.pass_blocks = (int)ngroups;
     // threads per pass block: a group's home atoms with 10 % to spare
     l.pass_threads = (int)std::min<long>(768, std::max<long>(64, ((long)(1.1 * occ * gd[0] * gd[1] * gd[2]) + 63) / 64 * 64));

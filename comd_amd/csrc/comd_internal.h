@@ -141,7 +141,7 @@ struct CmList {
     int hcap;                // staged halo capacity per group (atoms, <= 65535)
     int lds_avail;           // dynamic LDS a list kernel's block may use
     int rows_lds;            // table rows in LDS: 0 never, else whenever they fit
-    int build_blocks, pass_blocks, pass_threads;
+    int pass_blocks, pass_threads;  // one block per group (passes and build)
     const int *cell_order;   // [naux]: auxiliary cells in build order (group, then cell)
     const int *group_q0;     // [ngroups + 1]: a group's first position in cell_order
     double4 *pos, *ref;      // per id (w unused)
@@ -161,7 +161,8 @@ struct CmList {
     int *aux_mem;            // [cell][CM_NL_AUXCAP] member ids
     int *gstart;             // [ngroups + 1]: first atom of each group in build order
     int *nonempty;           // per link cell: non-empty stencil cells (Lennard-Jones virial)
-    int *count;              // [0]: atoms in build order, [1]: builds so far, [2]: build blocks done
+    int *count;              // [0]: atoms in build order, [1]: builds so far, [2]: build blocks done,
+                             // [3]: group-sum blocks done
     unsigned *flags;         // CM_NL_REBUILD, CM_NL_INVALID, CM_NL_OVERFLOW (during a build)
 };
 #define CM_NL_AUXCAP 64

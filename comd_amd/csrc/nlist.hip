@@ -190,31 +190,6 @@ __device__ __forceinline__ void group_setup(const CmList &l, int g, GroupLds &S,
     }
 }
 
-// Slot of staged halo entry h (off[] ascending, 0 <= h < off[ns]).
-__device__ __forceinline__ int halo_slot(const GroupLds &S, int h)
-{
-    int lo = 0, hi = S.ns;  // off[lo] <= h < off[hi]
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (S.off[mid] <= h) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
-
-// Groups of this block: blocks are dealt round-robin over the 8 XCDs; each
-// XCD takes a contiguous range of groups (a slab of space), so that the
-// halos it stages stay in its own L2.
-__device__ __forceinline__ void block_groups(int ngroups, int &g0, int &g1)
-{
-    const int nb = gridDim.x, q = nb / 8, rem = nb % 8;
-    const int xcd = blockIdx.x % 8, idx = blockIdx.x / 8;
-    const int blk = nb < 8 ? (int)blockIdx.x : xcd * q + min(xcd, rem) + idx;
-    const int per = (ngroups + nb - 1) / nb;
-    g0 = min(blk * per, ngroups);
-    g1 = min(g0 + per, ngroups);
-}
-
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef unsigned v4u __attribute__((ext_vector_type(4)));
 typedef unsigned v3u __attribute__((ext_vector_type(3)));
@@ -261,35 +236,61 @@ __global__ __launch_bounds__(256) void k_nl_key(CmList l)
     }
 }
 
-// Exclusive prefix sum of the auxiliary cells' counts in group-major order
-// (cell_order), one block walking tiles of 8 x 1024 cells; the capped counts
-// (aux_n), each group's first atom (gstart) and count[0] = atoms listed.
-__global__ __launch_bounds__(1024) void k_nl_scan(CmList l)
+// Per group (one wave each; a group has at most 64 cells): its cells' atom
+// counts (capped) and their exclusive offsets within the group in build
+// order; the block that finishes last then turns the group totals into each
+// group's first atom (an exclusive prefix sum, gstart) and count[0].
+__global__ __launch_bounds__(256) void k_nl_gsum(CmList l)
 {
     if (!nl_rebuild(l)) return;
-    __shared__ int wsum[16];
-    __shared__ int carry_s;
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    __shared__ int wsum[4];
+    __shared__ int carry_s, last;
+    const int t = threadIdx.x, L = t & 63, w = t >> 6;
+    for (int g = blockIdx.x * 4 + w; g < l.ngroups; g += gridDim.x * 4) {
+        const int q0 = l.group_q0[g], nq = l.group_q0[g + 1] - q0;
+        int a = 0, v = 0;
+        if (L < nq) {
+            a = l.cell_order[q0 + L];
+            v = min(l.aux_cnt[a], CM_NL_AUXCAP);
+        }
+        int incl = v;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int u = __shfl_up(incl, o, 64);
+            if (L >= o) incl += u;
+        }
+        if (L < nq) {
+            l.aux_start[a] = incl - v;
+            l.aux_n[a] = v;
+        }
+        if (L == 63) l.gstart[g] = incl;  // the group's total, for now
+    }
+    __syncthreads();
+    if (t == 0) {
+        __threadfence();
+        last = atomicAdd(&l.count[3], 1) == (int)gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
     int carry = 0;
-    for (int base = 0; base < l.naux; base += 8 * 1024) {
-        int v[8], s = 0;
+    for (int base = 0; base < l.ngroups; base += 4 * 256) {
+        int v[4], s = 0;
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
-            const int q = base + 8 * t + k;
-            v[k] = q < l.naux ? min(l.aux_cnt[l.cell_order[q]], CM_NL_AUXCAP) : 0;
+        for (int k = 0; k < 4; k++) {
+            const int q = base + 4 * t + k;
+            v[k] = q < l.ngroups ? l.gstart[q] : 0;
             s += v[k];
         }
-        // inclusive scan of the threads' sums: within the wave, then across
         int incl = s;
         for (int o = 1; o < 64; o <<= 1) {
             const int u = __shfl_up(incl, o, 64);
-            if (lane >= o) incl += u;
+            if (L >= o) incl += u;
         }
-        if (lane == 63) wsum[w] = incl;
+        if (L == 63) wsum[w] = incl;
         __syncthreads();
         if (t == 0) {
             int run = 0;
-            for (int k = 0; k < 16; k++) {
+            for (int k = 0; k < 4; k++) {
                 const int x = wsum[k];
                 wsum[k] = run;
                 run += x;
@@ -299,27 +300,24 @@ __global__ __launch_bounds__(1024) void k_nl_scan(CmList l)
         __syncthreads();
         int run = carry + wsum[w] + incl - s;
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
-            const int q = base + 8 * t + k;
-            if (q < l.naux) {
-                const int a = l.cell_order[q];
-                l.aux_start[a] = run;
-                l.aux_n[a] = v[k];
-            }
+        for (int k = 0; k < 4; k++) {
+            const int q = base + 4 * t + k;
+            if (q < l.ngroups) l.gstart[q] = run;
             run += v[k];
         }
         carry += carry_s;
         __syncthreads();
     }
-    for (int g = t; g < l.ngroups; g += blockDim.x) l.gstart[g] = l.aux_start[l.cell_order[l.group_q0[g]]];
     if (t == 0) {
         l.count[0] = carry;
         l.gstart[l.ngroups] = carry;
+        l.count[3] = 0;
     }
 }
 
 // Members of each auxiliary cell in id order (deterministic lists), placed in
-// the build order.  One wave per cell, four waves per block.
+// the build order; each cell's first atom made global (its group's first
+// atom added).  One wave per cell, four waves per block.
 __global__ __launch_bounds__(256) void k_nl_sort(CmList l)
 {
     if (!nl_rebuild(l)) return;
@@ -327,6 +325,11 @@ __global__ __launch_bounds__(256) void k_nl_sort(CmList l)
     const int L = threadIdx.x & 63, w = threadIdx.x >> 6;
     int *mem = mem_s[w];
     for (int a = blockIdx.x * 4 + w; a < l.naux; a += gridDim.x * 4) {
+        const int ax = a % l.na0, ay = (a / l.na0) % l.na1, az = a / (l.na0 * l.na1);
+        const int g = ax / l.gd0 + l.ng0 * (ay / l.gd1 + l.ng1 * (az / l.gd2));
+        const int start = l.aux_start[a] + l.gstart[g];
+        nl_wave_sync();
+        if (L == 0) l.aux_start[a] = start;
         const int n = l.aux_n[a];
         if (n == 0) continue;
         const int me = L < n ? l.aux_mem[a * CM_NL_AUXCAP + L] : 0;
@@ -335,47 +338,65 @@ __global__ __launch_bounds__(256) void k_nl_sort(CmList l)
         int rank = 0;
         for (int u = 0; u < n; u++) rank += mem[u] < me;
         nl_wave_sync();
-        if (L < n) l.order[l.aux_start[a] + rank] = me;
+        if (L < n) l.order[start + rank] = me;
     }
 }
 
-// One block per group (persistent over groups): the halo is staged in LDS in
-// the group's frame, in single precision; each home atom (a thread) tests
-// the atoms of its own 27-cell stencil, in stencil order (x outer, as
-// src-flat/domains.c:419-426) and id order within a cell, and lists the halo
-// indices within cutoff + skin.  The single-precision test compares with
+// One block per group (the passes' launch shape): the halo is staged in LDS
+// in the group's frame, in single precision; each home atom (a thread) tests
+// the atoms of its own 27-cell stencil, as 9 runs of 3 x-adjacent cells
+// (contiguous in the halo; the order of the list entries only orders the
+// passes' sums), id order within a cell, and lists the halo indices within
+// cutoff + skin.  The single-precision test compares with
 // (cutoff + skin)^2 x (1 + 4e-6), more than its rounding (frame coordinates
 // within 20 A: at most 1.5e-6 of r^2), so no pair within cutoff + skin is
 // left out; the few extra pairs are rejected by the passes' cutoff test.
 // The last block to finish closes the build: overflow (a list past kmax, a
 // halo past hcap) -> lists invalid, rebuild request cleared, counts cleared
 // for the next build.
-__global__ __launch_bounds__(256) void k_nl_build(CmList l)
+__global__ __launch_bounds__(1024) void k_nl_build(CmList l)
 {
     if (!nl_rebuild(l)) return;
-    extern __shared__ float4 hl4[];  // [hcap] frame x, y, z
+    extern __shared__ float4 hl4[];  // [hcap] frame x, y, z; then the slot map
+    CM_LDS uint8_t *slot_of = (CM_LDS uint8_t *)((CM_LDS char *)hl4 + (size_t)l.hcap * 16);
     __shared__ GroupLds S;
     __shared__ int last;
     const float rl2 = (float)(l.rl2 * (1.0 + 4e-6));
     bool ovf = false;
-    int g0, g1;
-    block_groups(l.ngroups, g0, g1);
-    for (int g = g0; g < g1; g++) {
-        group_setup(l, g, S);
-        const int H = S.off[S.ns];
-        if (H > l.hcap) {  // the same in every thread of the block
-            ovf = true;
-            continue;
-        }
+    // one group per block, XCD-contiguous as in the passes
+    const int nb = gridDim.x, qb = nb / 8, rem = nb % 8;
+    const int xcd = blockIdx.x % 8, idx = blockIdx.x / 8;
+    const int g = nb < 8 ? (int)blockIdx.x : xcd * qb + min(xcd, rem) + idx;
+    group_setup(l, g, S, slot_of);
+    const int H = S.off[S.ns];
+    if (H > l.hcap) {  // the same in every thread of the block
+        ovf = true;
+    } else {
         const GroupGeo G = group_geo(l, g);
         const double ox = (G.x0 + 0.5 * G.lx) * l.ab0, oy = (G.y0 + 0.5 * G.ly) * l.ab1,
                      oz = (G.z0 + 0.5 * G.lz) * l.ab2;
-        for (int h = threadIdx.x; h < H; h += blockDim.x) {
-            const int s = halo_slot(S, h);
-            const v4d p = ((const CM_GLOBAL v4d *)l.pos)[l.order[S.start[s] + (h - S.off[s])]];
-            double x, y, z;
-            frame_pos(l, G, S.uxyz[s], p, ox, oy, oz, x, y, z);
-            hl4[h] = make_float4((float)x, (float)y, (float)z, 0.0f);
+        constexpr int SB = 4;  // staged entries per thread in flight
+        for (int h0 = threadIdx.x; h0 < H; h0 += SB * blockDim.x) {
+            int sl[SB], q[SB], id[SB];
+#pragma unroll
+            for (int k = 0; k < SB; k++) {
+                const int h = min(h0 + k * (int)blockDim.x, H - 1);
+                sl[k] = slot_of[h];
+                q[k] = S.start[sl[k]] + (h - S.off[sl[k]]);
+            }
+#pragma unroll
+            for (int k = 0; k < SB; k++) id[k] = l.order[q[k]];
+            v4d p[SB];
+#pragma unroll
+            for (int k = 0; k < SB; k++) p[k] = ((const CM_GLOBAL v4d *)l.pos)[id[k]];
+#pragma unroll
+            for (int k = 0; k < SB; k++) {
+                const int h = h0 + k * (int)blockDim.x;
+                if (h >= H) break;
+                double x, y, z;
+                frame_pos(l, G, S.uxyz[sl[k]], p[k], ox, oy, oz, x, y, z);
+                hl4[h] = make_float4((float)x, (float)y, (float)z, 0.0f);
+            }
         }
         __syncthreads();
         const int shx = G.lx + 2, shy = G.ly + 2;
@@ -390,17 +411,37 @@ __global__ __launch_bounds__(256) void k_nl_build(CmList l)
             const int self = S.off[hs] + (t - S.start[hs]);
             l.selfh[t] = (uint16_t)self;
             const float4 pi = hl4[self];
+            // the 27 stencil cells as 9 runs of three x-adjacent halo slots,
+            // whose atoms are contiguous in the halo; entries gathered 8 at a
+            // time and stored as one 16-byte group
             int k = 0;
-            for (int e = 0; e < 27; e++) {
-                const int s = (cx + e / 9 - 1) + shx * ((cy + (e / 3) % 3 - 1) + shy * (cz + e % 3 - 1));
-                for (int h = S.off[s]; h < S.off[s + 1]; h++) {
-                    const float4 q = hl4[h];
-                    const float dx = pi.x - q.x, dy = pi.y - q.y, dz = pi.z - q.z;
+            unsigned long long lo = 0, hi = 0;
+            for (int e = 0; e < 9; e++) {
+                const int s0 = (cx - 1) + shx * ((cy + e / 3 - 1) + shy * (cz + e % 3 - 1));
+                const int h1 = S.off[s0 + 3];
+                for (int h = S.off[s0]; h < h1; h++) {
+                    const float4 qh = hl4[h];
+                    const float dx = pi.x - qh.x, dy = pi.y - qh.y, dz = pi.z - qh.z;
                     if (fmaf(dx, dx, fmaf(dy, dy, dz * dz)) <= rl2 && h != self) {
-                        if (k < l.kmax) l.nbl[nl_entry(k, t, l.n)] = (uint16_t)h;
+                        const int c8 = k & 7;
+                        if (c8 < 4) lo |= (unsigned long long)h << (16 * c8);
+                        else hi |= (unsigned long long)h << (16 * (c8 - 4));
+                        if (c8 == 7 && k < l.kmax) {
+                            ulonglong2 w;
+                            w.x = lo;
+                            w.y = hi;
+                            *(ulonglong2 *)&l.nbl[nl_entry(k - 7, t, l.n)] = w;
+                            lo = hi = 0;
+                        }
                         k++;
                     }
                 }
+            }
+            if ((k & 7) != 0 && k <= l.kmax) {
+                ulonglong2 w;
+                w.x = lo;
+                w.y = hi;
+                *(ulonglong2 *)&l.nbl[nl_entry(k & ~7, t, l.n)] = w;
             }
             l.ncnt[t] = min(k, l.kmax);
             ovf |= k > l.kmax;
@@ -740,15 +781,15 @@ void cm_nl_sync(const DevState &d, const CmList &l, hipStream_t st)
     hipLaunchKernelGGL(k_nl_from_slots, dim3(d.ncells), dim3(64), 0, st, d, l, 1);
 }
 
-size_t cm_nl_build_lds(const CmList &l) { return (size_t)l.hcap * 16; }
+size_t cm_nl_build_lds(const CmList &l) { return (size_t)l.hcap * 16 + nl_slotmap_bytes(l.hcap); }
 
 void cm_nl_build(const DevState &d, const CmList &l, hipStream_t st)
 {
     (void)d;
     hipLaunchKernelGGL(k_nl_key, dim3(1024), dim3(256), 0, st, l);
-    hipLaunchKernelGGL(k_nl_scan, dim3(1), dim3(1024), 0, st, l);
+    hipLaunchKernelGGL(k_nl_gsum, dim3((unsigned)std::min(4096, (l.ngroups + 3) / 4)), dim3(256), 0, st, l);
     hipLaunchKernelGGL(k_nl_sort, dim3(1024), dim3(256), 0, st, l);
-    hipLaunchKernelGGL(k_nl_build, dim3(l.build_blocks), dim3(256), cm_nl_build_lds(l), st, l);
+    hipLaunchKernelGGL(k_nl_build, dim3(l.pass_blocks), dim3(l.pass_threads), cm_nl_build_lds(l), st, l);
 }
 
 size_t cm_nl_pass_lds(const PotDev &P, const CmList &l, int kind, int mixed, int *tables_in_lds)
