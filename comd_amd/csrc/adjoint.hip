@@ -291,13 +291,28 @@ __global__ __launch_bounds__(64) void k_adj_resid(DevState d, const double *ft, 
     part[(size_t)blockIdx.x * ANMOM + L] = L == 0 ? t : 0.0;
 }
 
-// Sums the blocks' partial moments in block order: out[k] = sum_b part[b][k].
-__global__ __launch_bounds__(64) void k_adj_final(const double *part, int nblocks, double *out)
+// Sums the blocks' partial moments in a fixed order: out[k] = sum_b
+// part[b][k].  Sixteen waves take interleaved runs of 8 blocks (their loads
+// issued together), and their sums are added in wave order.
+__global__ __launch_bounds__(1024) void k_adj_final(const double *part, int nblocks, double *out)
 {
-    const int k = threadIdx.x;
+    __shared__ double acc[16][ANMOM];
+    const int k = threadIdx.x & 63, j = threadIdx.x >> 6;
     double s = 0.0;
-    for (int b = 0; b < nblocks; b++) s += part[(size_t)b * ANMOM + k];
-    out[k] = s;
+    for (int b0 = j * 8; b0 < nblocks; b0 += 16 * 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = b0 + u < nblocks ? part[(size_t)(b0 + u) * ANMOM + k] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 8; u++) s += v[u];
+    }
+    acc[j][k] = s;
+    __syncthreads();
+    if (j == 0) {
+        double t = 0.0;
+        for (int q = 0; q < 16; q++) t += acc[q][k];
+        out[k] = t;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -326,13 +341,13 @@ static void launch_all(const DevState &d, const PotDev &P, const CmAdjoint &J, i
     case 3: hipLaunchKernelGGL((k_adj_pair<3, MIXED>), g, b, 0, st, d, P, A); break;
     default: return;
     }
-    hipLaunchKernelGGL(k_adj_final, dim3(1), dim3(64), 0, st, J.part, J.nblocks, o);
+    hipLaunchKernelGGL(k_adj_final, dim3(1), dim3(1024), 0, st, J.part, J.nblocks, o);
 }
 
 void cm_launch_adj_resid(const DevState &d, const CmAdjoint &J, const double *ft, double gscale, hipStream_t st)
 {
     hipLaunchKernelGGL(k_adj_resid, dim3(J.nblocks), dim3(64), 0, st, d, ft, gscale, J.gx, J.gy, J.gz, J.part);
-    hipLaunchKernelGGL(k_adj_final, dim3(1), dim3(64), 0, st, J.part, J.nblocks, J.out + (size_t)4 * ANMOM);
+    hipLaunchKernelGGL(k_adj_final, dim3(1), dim3(1024), 0, st, J.part, J.nblocks, J.out + (size_t)4 * ANMOM);
 }
 
 void cm_launch_adj_stage(const DevState &d, const PotDev &P, const CmAdjoint &J, int stage, int mixed, hipStream_t st)

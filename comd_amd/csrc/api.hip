@@ -799,10 +799,12 @@ static int nl_setup(comd_ctx *c, long nids)
     const double vol = p.bounds[0] * p.bounds[1] * p.bounds[2];
     const double expect = 4.0 / 3.0 * 3.141592653589793 * rl * rl * rl * (double)c->ntot / vol;
     const long kmax = (((long)(1.5 * expect) + 16 + 15) / 16) * 16;
-    // groups of auxiliary cells: the largest of the options whose staged
-    // halo (1.2x the mean occupancy, plus 64 atoms) fits the LDS a block may
-    // use in every pass (16 B per halo atom in mixed precision, up to 32 B in
-    // FP64) and in the build, beside the kernels' static LDS.  The auxiliary
+    // groups of auxiliary cells: of the options whose staged halo (1.2x the
+    // mean occupancy, plus 64 atoms) fits the LDS a block may use in every
+    // pass (16 B per halo atom in mixed precision, up to 32 B in FP64, and a
+    // byte of slot map) beside the kernels' static LDS, the largest that
+    // still gives two groups (blocks) per CU, else the one with the most
+    // groups; a tuned option is taken as the largest allowed.  The auxiliary
     // grid is rounded down to whole groups per dimension (cells a little
     // larger than cutoff + skin): a partial group at an edge would take a
     // whole block for a sliver of atoms.
@@ -811,7 +813,7 @@ static int nl_setup(comd_ctx *c, long nids)
     const int gopts[5][3] = {{4, 4, 3}, {4, 4, 2}, {4, 2, 2}, {2, 2, 2}, {2, 2, 1}};
     const long lds_avail = (long)c->lds_block - CM_NL_STATIC_LDS;
     const long hb = c->precision == COMD_PREC_MIXED ? 16 : 32;
-    for (int o = c->nl_group_opt >= 0 ? c->nl_group_opt : 0; o < 5; o++) {
+    auto shape = [&](int o) {  // fills gd, na, naux, hcap; returns the group count
         for (int j = 0; j < 3; j++) {
             gd[j] = std::min(gopts[o][j], nmax[j]);
             na[j] = nmax[j] / gd[j] * gd[j];
@@ -820,9 +822,24 @@ static int nl_setup(comd_ctx *c, long nids)
         const double occ = (double)c->ntot / (double)naux;
         const long slots = (long)(gd[0] + 2) * (gd[1] + 2) * (gd[2] + 2);
         hcap = (((long)(1.2 * occ * (double)slots) + 64 + 63) / 64) * 64;
-        if (hcap * (hb + 1) <= lds_avail) break;  // halo entries and their slot map
+        return (long)((na[0] + gd[0] - 1) / gd[0]) * ((na[1] + gd[1] - 1) / gd[1]) * ((na[2] + gd[2] - 1) / gd[2]);
+    };
+    int best = -1;
+    long best_ng = -1;
+    for (int o = c->nl_group_opt >= 0 ? c->nl_group_opt : 0; o < 5; o++) {
+        const long ngo = shape(o);
+        if (hcap * (hb + 1) > lds_avail || hcap > 65535) continue;
+        if (c->nl_group_opt >= 0 || ngo >= 2L * c->ncu) {
+            best = o;
+            break;
+        }
+        if (ngo > best_ng) {
+            best = o;
+            best_ng = ngo;
+        }
     }
-    if (hcap * (hb + 1) > lds_avail || hcap > 65535) return COMD_OK;  // too dense for staged halos
+    if (best < 0) return COMD_OK;  // too dense for staged halos
+    shape(best);
     const double occ = (double)c->ntot / (double)naux;
     int ng[3];
     for (int j = 0; j < 3; j++) ng[j] = (na[j] + gd[j] - 1) / gd[j];

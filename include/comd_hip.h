@@ -170,10 +170,11 @@ enum comd_tune_key {
                                       potential, KE - PE is conserved and a lattice at any temperature
                                       flies apart within about 50 fs).  No counterpart in the
                                       reference. */
-    COMD_TUNE_LIST_GROUPS = 9,     /* -1 (default, = 0), 0..4: auxiliary cells per group of the
+    COMD_TUNE_LIST_GROUPS = 9,     /* -1 (default), 0..4: auxiliary cells per group of the
                                       neighbour-list passes, 4x4x3, 4x4x2, 4x2x2, 2x2x2, 2x2x1 (from the
                                       option given, the first whose staged halo fits the LDS of a
-                                      block) */
+                                      block; by default the largest fitting shape that still gives two
+                                      groups per compute unit, else the one with the most groups) */
     COMD_TUNE_LIST_ROWS_LDS = 10,  /* -1 (default, = 1), 0, 1: table rows of the list passes staged in
                                       LDS when they fit beside the halo */
 };
