@@ -132,8 +132,9 @@ __global__ __launch_bounds__(64) void k_adj_pair(DevState d, PotDev P, AdjDev A)
     const int dom = PASS == 1 ? 0 : 1;
     const double ca = P.ca[dom], cb = P.cb[dom];
     const double rc2 = P.rc2;
-    const double *dser = P.cheb + 4 * m;      // rho' series
-    const float *dser32 = P.cheb32 + 4 * m;
+    const int m8 = P.cstride;
+    const double *dser = P.cheb + 4 * m8;      // rho' series
+    const float *dser32 = P.cheb32 + 4 * m8;
 
     double m0[AMAX], m1[AMAX];
 #pragma unroll
@@ -215,8 +216,8 @@ __global__ __launch_bounds__(64) void k_adj_pair(DevState d, PotDev P, AdjDev A)
                             add_basis<MIXED, 1>(y, m, 0.5 * w, 0.5, m0, m1);
                         } else {
                             add_basis<MIXED, 0>(y, m, 0.5 * (fia + hf[j]) * w, 0.0, m0, m1);
-                            const double drho = MIXED ? (double)cheb32(dser32, m, (float)ca, (float)cb, (float)r)
-                                                      : cheb64(dser, m, ca, cb, r);
+                            const double drho = MIXED ? (double)cheb32(dser32, m8, (float)ca, (float)cb, (float)r)
+                                                      : cheb64(dser, m8, ca, cb, r);
                             lam += drho * w;
                         }
                     }
@@ -255,7 +256,7 @@ __global__ __launch_bounds__(64) void k_adj_atom(DevState d, PotDev P, AdjDev A)
             const int s = c * CM_SLOTS + L;
             const double rho = d.rho[s];
             const double lam = A.lam[s];
-            const double f2 = cheb64(A.f2, m, ca, cb, rho);
+            const double f2 = cheb64(A.f2, P.cstride, ca, cb, rho);  // zero-padded as the series
             A.mu[s] = lam * f2 + A.dlde * d.fi[s];
             add_basis<MIXED, 1>(cheb_arg(rho, ca, cb), m, lam, 1.0, m0, m1);
         }

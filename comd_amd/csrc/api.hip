@@ -36,7 +36,7 @@ struct Replica {
     PotDev pot;
     int ntot;
     std::vector<void *> owned;  // device buffers of the potential
-    double *d_cheb;             // 6 x cn series (FP64), or null
+    double *d_cheb;             // 6 series of cn at a stride of pot.cstride (FP64), or null
     float *d_cheb32;
 };
 
@@ -577,17 +577,24 @@ static bool setup_pot(const comd_params *p, Replica &R)
         P.cdm = e;
     }
     if (ok && p->cheb_n >= 2 && p->cheb_coef) {
-        const int m = p->cheb_n;
-        R.d_cheb = (double *)alloc((size_t)6 * m * 8);
-        R.d_cheb32 = (float *)alloc((size_t)6 * m * 4);
+        const int m = p->cheb_n, m8 = (m + 7) / 8 * 8;
+        R.d_cheb = (double *)alloc((size_t)6 * m8 * 8);
+        R.d_cheb32 = (float *)alloc((size_t)6 * m8 * 4);
         if (ok) {
-            std::vector<float> c32(6 * m);
-            for (int i = 0; i < 6 * m; i++) c32[i] = (float)p->cheb_coef[i];
-            ok = hipMemcpy(R.d_cheb, p->cheb_coef, (size_t)6 * m * 8, hipMemcpyHostToDevice) == hipSuccess &&
-                 hipMemcpy(R.d_cheb32, c32.data(), (size_t)6 * m * 4, hipMemcpyHostToDevice) == hipSuccess;
+            // series at a stride of m8, zero-padded
+            std::vector<double> c64((size_t)6 * m8, 0.0);
+            std::vector<float> c32((size_t)6 * m8, 0.0f);
+            for (int k = 0; k < 6; k++)
+                for (int j = 0; j < m; j++) {
+                    c64[(size_t)k * m8 + j] = p->cheb_coef[(size_t)k * m + j];
+                    c32[(size_t)k * m8 + j] = (float)p->cheb_coef[(size_t)k * m + j];
+                }
+            ok = hipMemcpy(R.d_cheb, c64.data(), c64.size() * 8, hipMemcpyHostToDevice) == hipSuccess &&
+                 hipMemcpy(R.d_cheb32, c32.data(), c32.size() * 4, hipMemcpyHostToDevice) == hipSuccess;
             P.cheb = R.d_cheb;
             P.cheb32 = R.d_cheb32;
             P.cn = m;
+            P.cstride = m8;
             for (int k = 0; k < 3; k++) {
                 P.ca[k] = p->cheb_x0[k];
                 P.cb[k] = p->cheb_xn[k];
@@ -1552,15 +1559,15 @@ int comd_set_cheb(comd_ctx *c, const double *coef, int n)
     Replica &R = c->reps[0];
     if (R.prm.potential != COMD_POT_EAM || !R.pot.cheb)
         return set_err(c, COMD_ERR_ARG, "the context has no Chebyshev series");
-    const int m = R.pot.cn;
+    const int m = R.pot.cn, m8 = R.pot.cstride;
     if (n != m) return set_err(c, COMD_ERR_ARG, "expected %d coefficients per series, got %d", m, n);
-    std::vector<double> all((size_t)6 * m);
-    std::vector<float> all32((size_t)6 * m);
+    std::vector<double> all((size_t)6 * m8, 0.0);
+    std::vector<float> all32((size_t)6 * m8, 0.0f);
     for (int k = 0; k < 3; k++) {
-        for (int j = 0; j < m; j++) all[(size_t)k * m + j] = coef[(size_t)k * m + j];
-        cheb_derivative(m, R.pot.ca[k], R.pot.cb[k], coef + (size_t)k * m, all.data() + (size_t)(3 + k) * m);
+        for (int j = 0; j < m; j++) all[(size_t)k * m8 + j] = coef[(size_t)k * m + j];
+        cheb_derivative(m, R.pot.ca[k], R.pot.cb[k], coef + (size_t)k * m, all.data() + (size_t)(3 + k) * m8);
     }
-    for (int i = 0; i < 6 * m; i++) all32[i] = (float)all[i];
+    for (size_t i = 0; i < all.size(); i++) all32[i] = (float)all[i];
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipMemcpy(R.d_cheb, all.data(), all.size() * 8, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(R.d_cheb32, all32.data(), all32.size() * 4, hipMemcpyHostToDevice));
@@ -1574,7 +1581,11 @@ int comd_get_cheb(comd_ctx *c, double *coef6)
     const Replica &R = c->reps[0];
     if (!R.pot.cheb) return set_err(c, COMD_ERR_ARG, "the context has no Chebyshev series");
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipMemcpy(coef6, R.d_cheb, (size_t)6 * R.pot.cn * 8, hipMemcpyDeviceToHost));
+    const int m = R.pot.cn, m8 = R.pot.cstride;
+    std::vector<double> all((size_t)6 * m8);
+    HIPCHK(c, hipMemcpy(all.data(), R.d_cheb, all.size() * 8, hipMemcpyDeviceToHost));
+    for (int k = 0; k < 6; k++)
+        for (int j = 0; j < m; j++) coef6[(size_t)k * m + j] = all[(size_t)k * m8 + j];
     return COMD_OK;
 }
 
@@ -1613,7 +1624,8 @@ int comd_eam_cheb_grad(comd_ctx *c, const double *coef, int n, const double *f_t
     std::vector<double> d1(m), d2(m);
     cheb_derivative(m, R.pot.ca[2], R.pot.cb[2], coef + 2 * m, d1.data());
     cheb_derivative(m, R.pot.ca[2], R.pot.cb[2], d1.data(), d2.data());
-    HIPCHK(c, hipMemcpy(c->adj_f2, d2.data(), (size_t)m * 8, hipMemcpyHostToDevice));
+    d2.resize((size_t)R.pot.cstride, 0.0);  // zero-padded as the series
+    HIPCHK(c, hipMemcpy(c->adj_f2, d2.data(), d2.size() * 8, hipMemcpyHostToDevice));
     if (f_target) {
         HIPCHK(c, hipMemcpy(c->adj_ft, f_target, S * 3 * 8, hipMemcpyHostToDevice));
         c->adj_have_target = 1;

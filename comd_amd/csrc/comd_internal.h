@@ -27,10 +27,11 @@ struct PotDev {
     const double *cd64, *cr64;
     const float *cdm;
     int balanced;   // drain an atom's queues round-robin over its lanes
-    // Chebyshev: 6 series of cn coefficients (phi rho F dphi drho dF)
+    // Chebyshev: 6 series of cn coefficients (phi rho F dphi drho dF), each
+    // at a stride of cstride (cn rounded up to a multiple of 8, zero-padded)
     const double *cheb;
     const float *cheb32;
-    int cn;
+    int cn, cstride;
     double ca[3], cb[3];   // domains of phi, rho, F
     // Lennard-Jones
     double lj_s6, lj_eps;
@@ -48,34 +49,57 @@ enum { CM_KIND_EAM1 = 0, CM_KIND_EAM3 = 1, CM_KIND_LJ = 2 };
 #define CM_GLOBAL __attribute__((address_space(1)))
 #define CM_LDS __attribute__((address_space(3)))
 
-// Clenshaw sum with the lower clamp only, src-flat/cheby.c:276-310.  The
-// coefficients are in global memory.
-__device__ __forceinline__ double cheb64(const double *cg, int m, double a, double b, double x)
+// Read-only data read through the scalar unit when the address is the same
+// in every lane (series coefficients).
+#define CM_CONST __attribute__((address_space(4)))
+
+// Clenshaw sum with the lower clamp only, src-flat/cheby.c:276-310, over a
+// series zero-padded to m8 coefficients (a multiple of 8; leading zeros
+// leave the sum exactly as it is).  The coefficients, in global memory at an
+// address common to the wave, are loaded 8 at a time ahead of their
+// iterations.
+__device__ __forceinline__ double cheb64(const double *cg, int m8, double a, double b, double x)
 {
-    const CM_GLOBAL double *c = (const CM_GLOBAL double *)cg;
+    const CM_CONST double *c = (const CM_CONST double *)cg;
     if (x < a) x = a;
     double d = 0.0, dd = 0.0;
     const double y = (2.0 * x - a - b) / (b - a);
     const double y2 = 2.0 * y;
-    for (int j = m - 1; j > 0; j--) {
-        const double sv = d;
-        d = y2 * d - dd + c[j];
-        dd = sv;
+    for (int jb = m8 - 8; jb >= 0; jb -= 8) {
+        double cc[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) cc[u] = c[jb + u];
+#pragma unroll
+        for (int u = 7; u >= 0; u--) {
+            if (jb + u > 0) {
+                const double sv = d;
+                d = y2 * d - dd + cc[u];
+                dd = sv;
+            }
+        }
     }
     return y * d - dd + 0.5 * c[0];
 }
 
-__device__ __forceinline__ float cheb32(const float *cg, int m, float a, float b, float x)
+__device__ __forceinline__ float cheb32(const float *cg, int m8, float a, float b, float x)
 {
-    const CM_GLOBAL float *c = (const CM_GLOBAL float *)cg;
+    const CM_CONST float *c = (const CM_CONST float *)cg;
     x = fmaxf(x, a);
     float d = 0.0f, dd = 0.0f;
     const float y = (2.0f * x - a - b) / (b - a);
     const float y2 = 2.0f * y;
-    for (int j = m - 1; j > 0; j--) {
-        const float sv = d;
-        d = fmaf(y2, d, c[j] - dd);
-        dd = sv;
+    for (int jb = m8 - 8; jb >= 0; jb -= 8) {
+        float cc[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) cc[u] = c[jb + u];
+#pragma unroll
+        for (int u = 7; u >= 0; u--) {
+            if (jb + u > 0) {
+                const float sv = d;
+                d = fmaf(y2, d, cc[u] - dd);
+                dd = sv;
+            }
+        }
     }
     return fmaf(y, d, 0.5f * c[0] - dd);
 }

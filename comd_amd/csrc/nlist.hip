@@ -630,8 +630,8 @@ __device__ __forceinline__ void nl_atom(const DevState &d, const PotDev &P, cons
         // pass 2, src-flat/eam.c:576-594
         double fv, fp;
         if (EVAL == COMD_EVAL_CHEB) {
-            fv = cheb64(P.cheb + 2 * P.cn, P.cn, P.ca[2], P.cb[2], A.rho);
-            fp = cheb64(P.cheb + 5 * P.cn, P.cn, P.ca[2], P.cb[2], A.rho);
+            fv = cheb64(P.cheb + 2 * P.cstride, P.cstride, P.ca[2], P.cb[2], A.rho);
+            fp = cheb64(P.cheb + 5 * P.cstride, P.cstride, P.ca[2], P.cb[2], A.rho);
         } else {
             interp64(P.tab[2], P.x0[2], P.xn[2], P.invdx[2], P.n[2], A.rho, fv, fp);
         }

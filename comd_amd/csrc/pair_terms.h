@@ -144,7 +144,7 @@ __device__ __forceinline__ void pair_term(const PotDev &P, const TB &T, double d
         // src-flat/eam.c:521-545 (f holds +grad U)
         double phiv, dphi, rhov;
         if (EVAL == COMD_EVAL_CHEB) {
-            const int m = P.cn;
+            const int m = P.cstride;
             if (MIXED) {
                 // values in FP32; the derivative, whose pair terms cancel in
                 // the force sum, in FP64 (the same issue rate on this device)
@@ -192,7 +192,7 @@ __device__ __forceinline__ void pair_term(const PotDev &P, const TB &T, double d
         // src-flat/eam.c:633-649
         double drho;
         if (EVAL == COMD_EVAL_CHEB) {
-            drho = cheb64(P.cheb + 4 * P.cn, P.cn, P.ca[1], P.cb[1], r);  // FP64 in both modes
+            drho = cheb64(P.cheb + 4 * P.cstride, P.cstride, P.ca[1], P.cb[1], r);  // FP64 in both modes
         } else {
             double t1;
             const int i1 = table_interval(r, P.x0[1], P.xn[1], P.invdx[1], P.n[1], t1);
