@@ -431,53 +431,48 @@ def bench_adjoint(args, devname):
     """Gradient of the force-matching loss with respect to the 96 Chebyshev
     coefficients, over 16 configurations of Cu FCC 20^3 (32 000 atoms each,
     displacements sigma 0.1 A), targets from the FP64 forward at 1 %-perturbed
-    coefficients; FP64 and mixed."""
+    coefficients; FP64 and mixed.  The 16 configurations are one batch
+    (comd_batch_cheb_grad: one launch per adjoint stage for all of them)."""
     cu = CA.EAMPotential("cu500")
     c0 = cu.cheb[:3].copy()
     rng = np.random.default_rng(args.seed)
     ct = c0 * (1.0 + 0.01 * rng.standard_normal(c0.shape))
-    n = 20
-    sims, e_t = [], []
-    for k in range(16):
+    n, nconf = 20, 16
+    bounds = (n * 3.615,) * 3
+    b = CA.Batch([dict(pot=cu, bounds=bounds, defgrad=0.0, evaluation="cheby") for _ in range(nconf)],
+                 precision="fp64")
+    for k in range(nconf):
         xyz = CA.fcc_positions(n, n, n, 3.615)
         xyz, p = CA.seed_state(xyz, cu.mass, 0.0, 0.1, args.seed * 100 + k)
-        s = CA.Simulation(cu, (n * 3.615,) * 3, defgrad=0.0, evaluation="cheby")
-        s.bin_atoms(xyz, p)
-        s.rebox()
-        s.set_cheb(ct)
-        s.force()
-        ft = s.download()["f"][:, :3].copy()
-        e_t.append(s.energy()[0])
-        s.cheb_grad(c0, ft, e_t[-1], 1.0 / (4 * n ** 3) ** 2)  # uploads the target once
-        sims.append(s)
-    natoms = 16 * 4 * n ** 3
+        b.bin_atoms(k, xyz, p)
+    b.rebox()
+    b.set_cheb(ct)
+    b.force()
+    fts = [b.download(k)["f"][:, :3].copy() for k in range(nconf)]
+    e_t = [b.energy_of(k)[0] for k in range(nconf)]
+    w_e = 1.0 / (4 * n ** 3) ** 2
+    b.cheb_grad(c0, fts, e_t, w_e)  # uploads the targets once
+    del fts
+    natoms = nconf * 4 * n ** 3
     res = {}
     reps = max(3, args.steps // 20)
     for prec in ("fp64", "mixed"):
-        for s in sims:
-            s.set_precision(prec)
-        grad = np.zeros_like(c0)
-        loss = 0.0
-        for s, e in zip(sims, e_t):  # warm-up
-            s.cheb_grad(c0, None, e, 1.0 / (4 * n ** 3) ** 2)
+        b.set_precision(prec)
+        b.cheb_grad(c0, None, e_t, w_e)  # warm-up
         t0 = time.perf_counter()
         for _ in range(reps):
-            grad[:] = 0.0
-            loss = 0.0
-            for s, e in zip(sims, e_t):
-                l, g = s.cheb_grad(c0, None, e, 1.0 / (4 * n ** 3) ** 2)
-                loss += l
-                grad += g
+            loss, grad = b.cheb_grad(c0, None, e_t, w_e)
         wall = (time.perf_counter() - t0) / reps
+        b.set_cheb(c0)
+        b.sync()
         t0 = time.perf_counter()
         for _ in range(reps):
-            for s in sims:
-                s.force()
-            for s in sims:
-                s.sync()
+            b.force()
+        b.sync()
         fwd = (time.perf_counter() - t0) / reps
         res[prec] = dict(seconds_per_gradient=wall, atom_gradients_per_s=natoms / wall, loss=loss,
                          forward_seconds=fwd, gradient_over_forward=wall / fwd, grad=grad)
+    b.close()
     rel = np.abs(res["mixed"]["grad"] - res["fp64"]["grad"]).max(axis=1) / np.abs(res["fp64"]["grad"]).max(axis=1)
     out = {
         "metric": "Chebyshev-coefficient gradients: atom-gradients/s over 16 x Cu FCC 20^3, 1 GPU",
@@ -492,7 +487,8 @@ def bench_adjoint(args, devname):
         "vs_baseline": None,
         "dtype": "f32 basis and potential evaluation, f64 geometry and accumulation",
         "data": "synthetic: FCC 20^3 Cu with seeded displacements sigma 0.1 A, targets at 1 %-perturbed coefficients",
-        "config": {"workload": "16 configurations x 32 000 atoms, 96 coefficients", "parallelism": "single GPU"},
+        "config": {"workload": "16 configurations x 32 000 atoms in one batch, 96 coefficients",
+                   "parallelism": "single GPU"},
         "fp64": {k: v for k, v in res["fp64"].items() if k != "grad"},
         "mixed": {k: v for k, v in res["mixed"].items() if k != "grad"},
         "mixed_gradient_rel_error_per_function": rel.tolist(),

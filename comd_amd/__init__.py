@@ -140,6 +140,7 @@ def _load():
     hip.comd_set_cheb.argtypes = [vp, _dp, C.c_int]
     hip.comd_get_cheb.argtypes = [vp, _dp]
     hip.comd_eam_cheb_grad.argtypes = [vp, _dp, C.c_int, _dp, C.c_double, C.c_double, _dp, _dp]
+    hip.comd_batch_cheb_grad.argtypes = [vp, _dp, C.c_int, _dp, _dp, C.c_double, _dp, _dp]
     host.setEamPot.argtypes = [C.c_char_p, C.c_char_p]
     host.setEamPot.restype = C.POINTER(EamPotT)
     host.setChebPot.argtypes = [C.POINTER(EamPotT), C.c_int]
@@ -580,6 +581,29 @@ class Batch(Simulation):
         pe, ke, st = C.c_double(), C.c_double(), C.c_double()
         self._check(self._hip.comd_batch_energy(self.ctx, r, C.byref(pe), C.byref(ke), C.byref(st)))
         return pe.value, ke.value, st.value
+
+    def cheb_grad(self, coef, f_targets, e_targets, w_e=1.0):
+        """Loss sum_r [sum_{i in r} |f_i - t_i|^2 / (3 N_r) + w_e (E_r -
+        E_t,r)^2] over the replicas and its gradient (3 x n) at the value
+        coefficients `coef`, one launch per adjoint stage
+        (comd_batch_cheb_grad).  f_targets: per replica its slot-layout
+        target (as download(r)['f']), or None to reuse the previous one;
+        e_targets: per replica."""
+        c = np.ascontiguousarray(coef, dtype=np.float64).reshape(3, -1)
+        ft = None
+        if f_targets is not None:
+            ft = np.zeros((self.ncells * MAXATOMS, 3))
+            for r, t in enumerate(f_targets):
+                b, nc, _ = self.info(r)
+                ft[b * MAXATOMS:(b + nc) * MAXATOMS] = np.asarray(t, dtype=np.float64)[:, :3]
+        et = np.ascontiguousarray(e_targets, dtype=np.float64)
+        if et.shape != (self.nrep,):
+            raise ValueError(f"e_targets needs {self.nrep} values")
+        loss = C.c_double()
+        grad = np.zeros_like(c)
+        self._check(self._hip.comd_batch_cheb_grad(self.ctx, _ptr(c), c.shape[1], _ptr(ft), _ptr(et), float(w_e),
+                                                   C.byref(loss), _ptr(grad)))
+        return loss.value, grad
 
 
 def fcc_simulation(pot, n, lat=None, temp=0.0, disp=0.0, seed=1, precision="fp64", evaluation="table",

@@ -41,10 +41,13 @@ struct AdjDev {
     const double *gx, *gy, *gz;  // per slot: dL/df
     double *lam;                 // per slot: lambda (pair pass 2)
     double *mu;                  // per slot: mu (atom pass)
-    const double *f2;            // F'' series (cn coefficients, FP64)
-    double dlde;                 // dL/dE
+    const double *f2;            // F'' series (cstride coefficients, zero-padded, FP64)
+    const double *dlde;          // per replica: dL/dE of its energy term
     double *part;                // per block: ANMOM partial moments
 };
+
+// Replica of global cell c.
+__device__ __forceinline__ int rep_of(const DevState &d, int c) { return d.ngrids == 1 ? 0 : d.cell_grid[c]; }
 
 // T_0..T_{m-1} at y, each times wa added to a[] (and times wb to b[]).
 template <int MIXED, int TWO>
@@ -143,15 +146,17 @@ __global__ __launch_bounds__(64) void k_adj_pair(DevState d, PotDev P, AdjDev A)
     for (int c = blockIdx.x; c < d.ncells; c += gridDim.x) {
         const int nh = d.natoms[c];
         if (nh == 0) continue;
-        const int nent = d.g0.ncells == 1 ? 1 : 27;
+        const Grid g = grid_of(d, c);
+        const double de = PASS == 1 ? A.dlde[rep_of(d, c)] : 0.0;  // weight of the energy moments
+        const int nent = g.ncells == 1 ? 1 : 27;
         int ne = 0;
         if (L < nent) {
-            const int b = stencil_cell(d.g0, c, L);
+            const int b = stencil_cell(g, c, L);
             cell[L] = b;
             if (b >= 0) {
                 ne = d.natoms[b];
                 double sx, sy, sz;
-                corner_shift(d.g0, c, b, sx, sy, sz);
+                corner_shift(g, c, b, sx, sy, sz);
                 drb[3 * L + 0] = sx;
                 drb[3 * L + 1] = sy;
                 drb[3 * L + 2] = sz;
@@ -213,7 +218,7 @@ __global__ __launch_bounds__(64) void k_adj_pair(DevState d, PotDev P, AdjDev A)
                     } else {
                         const double w = ((gax - hgx[j]) * dx + (gay - hgy[j]) * dy + (gaz - hgz[j]) * dz) / r;
                         if (PASS == 1) {
-                            add_basis<MIXED, 1>(y, m, 0.5 * w, 0.5, m0, m1);
+                            add_basis<MIXED, 1>(y, m, 0.5 * w, 0.5 * de, m0, m1);
                         } else {
                             add_basis<MIXED, 0>(y, m, 0.5 * (fia + hf[j]) * w, 0.0, m0, m1);
                             const double drho = MIXED ? (double)cheb32(dser32, m8, (float)ca, (float)cb, (float)r)
@@ -254,20 +259,22 @@ __global__ __launch_bounds__(64) void k_adj_atom(DevState d, PotDev P, AdjDev A)
     for (int c = blockIdx.x; c < d.ncells; c += gridDim.x) {
         if (L < d.natoms[c]) {
             const int s = c * CM_SLOTS + L;
+            const double de = A.dlde[rep_of(d, c)];
             const double rho = d.rho[s];
             const double lam = A.lam[s];
             const double f2 = cheb64(A.f2, P.cstride, ca, cb, rho);  // zero-padded as the series
-            A.mu[s] = lam * f2 + A.dlde * d.fi[s];
-            add_basis<MIXED, 1>(cheb_arg(rho, ca, cb), m, lam, 1.0, m0, m1);
+            A.mu[s] = lam * f2 + de * d.fi[s];
+            add_basis<MIXED, 1>(cheb_arg(rho, ca, cb), m, lam, de, m0, m1);
         }
     }
     write_moments<1>(m0, m1, A.part + (size_t)blockIdx.x * ANMOM);
 }
 
 // ---------------------------------------------------------------------------
-// residual: g_i = gscale (f_i - t_i), and the partial sums of |f_i - t_i|^2
+// residual: with w the atom's replica weight (1 / (3 N) of its replica),
+// g_i = 2 w (f_i - t_i), and the partial sums of w |f_i - t_i|^2
 
-__global__ __launch_bounds__(64) void k_adj_resid(DevState d, const double *ft, double gscale, double *gx,
+__global__ __launch_bounds__(64) void k_adj_resid(DevState d, const double *ft, const double *rw, double *gx,
                                                   double *gy, double *gz, double *part)
 {
     const int L = threadIdx.x;
@@ -275,13 +282,14 @@ __global__ __launch_bounds__(64) void k_adj_resid(DevState d, const double *ft, 
     for (int c = blockIdx.x; c < d.ncells; c += gridDim.x) {
         const int s = c * CM_SLOTS + L;
         if (L < d.natoms[c]) {
+            const double w = rw[rep_of(d, c)];
             const double ex = d.fx[s] - ft[3L * s + 0];
             const double ey = d.fy[s] - ft[3L * s + 1];
             const double ez = d.fz[s] - ft[3L * s + 2];
-            gx[s] = gscale * ex;
-            gy[s] = gscale * ey;
-            gz[s] = gscale * ez;
-            ss += ex * ex + ey * ey + ez * ez;
+            gx[s] = 2.0 * w * ex;
+            gy[s] = 2.0 * w * ey;
+            gz[s] = 2.0 * w * ez;
+            ss += w * (ex * ex + ey * ey + ez * ez);
         } else {
             gx[s] = 0.0;
             gy[s] = 0.0;
@@ -345,9 +353,9 @@ static void launch_all(const DevState &d, const PotDev &P, const CmAdjoint &J, i
     hipLaunchKernelGGL(k_adj_final, dim3(1), dim3(1024), 0, st, J.part, J.nblocks, o);
 }
 
-void cm_launch_adj_resid(const DevState &d, const CmAdjoint &J, const double *ft, double gscale, hipStream_t st)
+void cm_launch_adj_resid(const DevState &d, const CmAdjoint &J, const double *ft, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_adj_resid, dim3(J.nblocks), dim3(64), 0, st, d, ft, gscale, J.gx, J.gy, J.gz, J.part);
+    hipLaunchKernelGGL(k_adj_resid, dim3(J.nblocks), dim3(64), 0, st, d, ft, J.rw, J.gx, J.gy, J.gz, J.part);
     hipLaunchKernelGGL(k_adj_final, dim3(1), dim3(1024), 0, st, J.part, J.nblocks, J.out + (size_t)4 * ANMOM);
 }
 

@@ -78,6 +78,7 @@ struct comd_ctx {
     PotDev *d_pots;
     // Chebyshev-coefficient adjoint (allocated on first use)
     double *adj_g[3], *adj_lam, *adj_mu, *adj_ft, *adj_f2, *adj_part, *adj_out;
+    double *adj_rep;      // per replica: dL/dE, then the residual weight 1 / (3 N)
     int adj_nblocks;
     int adj_have_target;  // adj_ft holds a target (comd_eam_cheb_grad with f_target NULL reuses it)
     std::vector<double> cheb_host;  // current value series phi, rho, F (3 x cn) of replica 0
@@ -415,7 +416,7 @@ static void free_all(comd_ctx *c)
                     c->perm, c->flags, c->plist, c->pcount, c->pvalid,
                     c->partials, c->red_out, c->d_grids, c->d_cell_grid, c->d_pots,
                     c->adj_g[0], c->adj_g[1], c->adj_g[2], c->adj_lam, c->adj_mu,
-                    c->adj_ft, c->adj_f2, c->adj_part, c->adj_out};
+                    c->adj_ft, c->adj_f2, c->adj_part, c->adj_out, c->adj_rep};
     for (void *p : ptrs)
         if (p) hipFree(p);
     for (Replica &r : c->reps)
@@ -1552,27 +1553,36 @@ int comd_timing_read(comd_ctx *c, double *ms, int *launches)
 
 // ---- Chebyshev coefficients and their adjoint (single-replica contexts)
 
+// The same value series (3 x n coefficients) for every EAM replica, with the
+// derivative series (src-flat/cheby.c:219-270), zero-padded to the stride.
+static int set_cheb_all(comd_ctx *c, const double *coef, int n)
+{
+    for (Replica &R : c->reps) {
+        if (R.prm.potential != COMD_POT_EAM || !R.pot.cheb)
+            return set_err(c, COMD_ERR_ARG, "a replica has no Chebyshev series");
+        if (n != R.pot.cn) return set_err(c, COMD_ERR_ARG, "expected %d coefficients per series, got %d", R.pot.cn, n);
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (Replica &R : c->reps) {
+        const int m = R.pot.cn, m8 = R.pot.cstride;
+        std::vector<double> all((size_t)6 * m8, 0.0);
+        std::vector<float> all32((size_t)6 * m8, 0.0f);
+        for (int k = 0; k < 3; k++) {
+            for (int j = 0; j < m; j++) all[(size_t)k * m8 + j] = coef[(size_t)k * m + j];
+            cheb_derivative(m, R.pot.ca[k], R.pot.cb[k], coef + (size_t)k * m, all.data() + (size_t)(3 + k) * m8);
+        }
+        for (size_t i = 0; i < all.size(); i++) all32[i] = (float)all[i];
+        HIPCHK(c, hipMemcpy(R.d_cheb, all.data(), all.size() * 8, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(R.d_cheb32, all32.data(), all32.size() * 4, hipMemcpyHostToDevice));
+    }
+    c->cheb_host.assign(coef, coef + 3 * n);
+    return COMD_OK;
+}
+
 int comd_set_cheb(comd_ctx *c, const double *coef, int n)
 {
     if (!c || !coef) return COMD_ERR_ARG;
-    if (c->nrep != 1) return set_err(c, COMD_ERR_ARG, "Chebyshev coefficients are set per single-replica context");
-    Replica &R = c->reps[0];
-    if (R.prm.potential != COMD_POT_EAM || !R.pot.cheb)
-        return set_err(c, COMD_ERR_ARG, "the context has no Chebyshev series");
-    const int m = R.pot.cn, m8 = R.pot.cstride;
-    if (n != m) return set_err(c, COMD_ERR_ARG, "expected %d coefficients per series, got %d", m, n);
-    std::vector<double> all((size_t)6 * m8, 0.0);
-    std::vector<float> all32((size_t)6 * m8, 0.0f);
-    for (int k = 0; k < 3; k++) {
-        for (int j = 0; j < m; j++) all[(size_t)k * m8 + j] = coef[(size_t)k * m + j];
-        cheb_derivative(m, R.pot.ca[k], R.pot.cb[k], coef + (size_t)k * m, all.data() + (size_t)(3 + k) * m8);
-    }
-    for (size_t i = 0; i < all.size(); i++) all32[i] = (float)all[i];
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipMemcpy(R.d_cheb, all.data(), all.size() * 8, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(R.d_cheb32, all32.data(), all32.size() * 4, hipMemcpyHostToDevice));
-    c->cheb_host.assign(coef, coef + 3 * m);
-    return COMD_OK;
+    return set_cheb_all(c, coef, n);
 }
 
 int comd_get_cheb(comd_ctx *c, double *coef6)
@@ -1589,18 +1599,25 @@ int comd_get_cheb(comd_ctx *c, double *coef6)
     return COMD_OK;
 }
 
-int comd_eam_cheb_grad(comd_ctx *c, const double *coef, int n, const double *f_target, double e_target, double w_e,
-                       double *loss, double *grad)
+// Loss and gradient over every replica of the context (one replica for
+// comd_eam_cheb_grad): L = sum_r [ sum_{i in r} |f_i - t_i|^2 / (3 N_r)
+// + w_e (E_r - e_target[r])^2 ], the replicas' moments summed by the same
+// kernels in one launch per stage.
+static int cheb_grad(comd_ctx *c, const double *coef, int n, const double *f_target, const double *e_target,
+                     double w_e, double *loss, double *grad)
 {
-    if (!c || !coef) return COMD_ERR_ARG;
     if (!c->have_atoms || c->ntot < 1) return set_err(c, COMD_ERR_STATE, "no atoms");
     if (!f_target && !c->adj_have_target) return set_err(c, COMD_ERR_STATE, "no force target");
     if (n > cm_adjoint_max_coefs())
         return set_err(c, COMD_ERR_ARG, "the adjoint supports at most %d coefficients per series",
                        cm_adjoint_max_coefs());
-    int rc = comd_set_cheb(c, coef, n);
+    const Replica &R0 = c->reps[0];
+    for (const Replica &R : c->reps)
+        for (int k = 0; k < 3; k++)
+            if (R.pot.ca[k] != R0.pot.ca[k] || R.pot.cb[k] != R0.pot.cb[k] || R.pot.rc2 != R0.pot.rc2)
+                return set_err(c, COMD_ERR_ARG, "the replicas' series domains or cutoffs differ");
+    int rc = set_cheb_all(c, coef, n);
     if (rc) return rc;
-    Replica &R = c->reps[0];
     const int m = n;
     const int NM = cm_adjoint_moments();
     const size_t S = (size_t)c->nslots;
@@ -1618,30 +1635,48 @@ int comd_eam_cheb_grad(comd_ctx *c, const double *coef, int n, const double *f_t
         alloc((void **)&c->adj_f2, 64 * 8);
         alloc((void **)&c->adj_part, (size_t)c->adj_nblocks * NM * 8);
         alloc((void **)&c->adj_out, (size_t)5 * NM * 8);
+        alloc((void **)&c->adj_rep, (size_t)2 * c->nrep * 8);
         if (!ok) return set_err(c, COMD_ERR_NOMEM, "adjoint buffers");
     }
-    // F'' series: the derivative recurrence applied twice
+    // F'' series: the derivative recurrence applied twice, zero-padded
     std::vector<double> d1(m), d2(m);
-    cheb_derivative(m, R.pot.ca[2], R.pot.cb[2], coef + 2 * m, d1.data());
-    cheb_derivative(m, R.pot.ca[2], R.pot.cb[2], d1.data(), d2.data());
-    d2.resize((size_t)R.pot.cstride, 0.0);  // zero-padded as the series
+    cheb_derivative(m, R0.pot.ca[2], R0.pot.cb[2], coef + 2 * m, d1.data());
+    cheb_derivative(m, R0.pot.ca[2], R0.pot.cb[2], d1.data(), d2.data());
+    d2.resize((size_t)R0.pot.cstride, 0.0);
     HIPCHK(c, hipMemcpy(c->adj_f2, d2.data(), d2.size() * 8, hipMemcpyHostToDevice));
     if (f_target) {
         HIPCHK(c, hipMemcpy(c->adj_ft, f_target, S * 3 * 8, hipMemcpyHostToDevice));
         c->adj_have_target = 1;
     }
 
-    // forward: forces and energy through the series
-    const int eval_saved = R.pot.eval;
-    R.pot.eval = COMD_EVAL_CHEB;
+    // forward: forces and energies through the series
+    std::vector<int> eval_saved(c->nrep);
+    for (int r = 0; r < c->nrep; r++) {
+        eval_saved[r] = c->reps[r].pot.eval;
+        c->reps[r].pot.eval = COMD_EVAL_CHEB;
+    }
+    if (c->nrep > 1) {
+        rc = upload_descriptors(c);
+        if (rc) return rc;
+    }
     op_force(c);
-    R.pot.eval = eval_saved;
-    double E = 0.0;
-    rc = comd_energy(c, &E, nullptr, nullptr);
-    if (rc) return rc;
+    for (int r = 0; r < c->nrep; r++) c->reps[r].pot.eval = eval_saved[r];
+    if (c->nrep > 1) {
+        rc = upload_descriptors(c);
+        if (rc) return rc;
+    }
+    std::vector<double> E(c->nrep), rep((size_t)2 * c->nrep);
+    double lossE = 0.0;
+    for (int r = 0; r < c->nrep; r++) {
+        rc = energy_rep(c, r, &E[r], nullptr, nullptr);
+        if (rc) return rc;
+        rep[r] = 2.0 * w_e * (E[r] - e_target[r]);
+        rep[c->nrep + r] = c->reps[r].ntot > 0 ? 1.0 / (3.0 * (double)c->reps[r].ntot) : 0.0;
+        lossE += w_e * (E[r] - e_target[r]) * (E[r] - e_target[r]);
+    }
+    HIPCHK(c, hipMemcpy(c->adj_rep, rep.data(), rep.size() * 8, hipMemcpyHostToDevice));
 
     const int mixed = c->precision == COMD_PREC_MIXED;
-    const double N = (double)c->ntot;
     CmAdjoint J;
     J.gx = c->adj_g[0];
     J.gy = c->adj_g[1];
@@ -1649,35 +1684,36 @@ int comd_eam_cheb_grad(comd_ctx *c, const double *coef, int n, const double *f_t
     J.lam = c->adj_lam;
     J.mu = c->adj_mu;
     J.f2 = c->adj_f2;
-    J.dlde = 2.0 * w_e * (E - e_target);
+    J.dlde = c->adj_rep;
+    J.rw = c->adj_rep + c->nrep;
     J.part = c->adj_part;
     J.nblocks = c->adj_nblocks;
     J.out = c->adj_out;
-    PotDev P = R.pot;
+    PotDev P = R0.pot;  // series, domains and cutoff common to the replicas
     P.eval = COMD_EVAL_CHEB;
     DevState d = dev_state(c);
-    cm_launch_adj_resid(d, J, c->adj_ft, 2.0 / (3.0 * N), c->stream);
+    cm_launch_adj_resid(d, J, c->adj_ft, c->stream);
     for (int stage = 0; stage < 4; stage++) cm_launch_adj_stage(d, P, J, stage, mixed, c->stream);
     HIPCHK(c, hipGetLastError());
     std::vector<double> out((size_t)5 * NM);
     HIPCHK(c, hipMemcpyAsync(out.data(), c->adj_out, out.size() * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
 
-    const double L = out[(size_t)4 * NM] / (3.0 * N) + w_e * (E - e_target) * (E - e_target);
-    if (loss) *loss = L;
+    if (loss) *loss = out[(size_t)4 * NM] + lossE;
     if (!grad) return COMD_OK;
-    // B_0 = T_0 / 2 in the series convention of src-flat/cheby.c:309
+    // B_0 = T_0 / 2 in the series convention of src-flat/cheby.c:309; the
+    // energy moments (set 1 of stages 0 and 2) carry each replica's dL/dE
     auto mom = [&](int stage, int set, int k) { return out[(size_t)stage * NM + set * 32 + k] * (k == 0 ? 0.5 : 1.0); };
     // columns of the derivative operator: D e_j
     auto dcol = [&](int fn, int j, std::vector<double> &col) {
         std::vector<double> e(m, 0.0);
         e[j] = 1.0;
         col.assign(m, 0.0);
-        cheb_derivative(m, R.pot.ca[fn], R.pot.cb[fn], e.data(), col.data());
+        cheb_derivative(m, R0.pot.ca[fn], R0.pot.cb[fn], e.data(), col.data());
     };
     std::vector<double> col;
     for (int j = 0; j < m; j++) {
-        double gphi = J.dlde * mom(0, 1, j), grho = mom(3, 0, j), gF = J.dlde * mom(2, 1, j);
+        double gphi = mom(0, 1, j), grho = mom(3, 0, j), gF = mom(2, 1, j);
         dcol(0, j, col);
         for (int k = 0; k < m; k++) gphi += col[k] * mom(0, 0, k);
         dcol(1, j, col);
@@ -1689,6 +1725,21 @@ int comd_eam_cheb_grad(comd_ctx *c, const double *coef, int n, const double *f_t
         grad[2 * m + j] = gF;
     }
     return COMD_OK;
+}
+
+int comd_eam_cheb_grad(comd_ctx *c, const double *coef, int n, const double *f_target, double e_target, double w_e,
+                       double *loss, double *grad)
+{
+    if (!c || !coef) return COMD_ERR_ARG;
+    if (c->nrep != 1) return set_err(c, COMD_ERR_ARG, "a batch takes comd_batch_cheb_grad");
+    return cheb_grad(c, coef, n, f_target, &e_target, w_e, loss, grad);
+}
+
+int comd_batch_cheb_grad(comd_ctx *c, const double *coef, int n, const double *f_target, const double *e_target,
+                         double w_e, double *loss, double *grad)
+{
+    if (!c || !coef || !e_target) return COMD_ERR_ARG;
+    return cheb_grad(c, coef, n, f_target, e_target, w_e, loss, grad);
 }
 
 }  // extern "C"

@@ -131,19 +131,20 @@ void cm_launch_reduce_final(const double *partials, int nparts, double *out, hip
 
 // Device buffers and scalars of the Chebyshev-coefficient adjoint
 // (adjoint.hip).  out holds 5 x cm_adjoint_moments() doubles: the moments of
-// stages 0-3 and, at out[4*64], the sum of squared force residuals.
+// stages 0-3 and, at out[4*64], the weighted sum of squared force residuals.
 struct CmAdjoint {
     double *gx, *gy, *gz;  // per slot: dL/df
     double *lam, *mu;      // per slot
     const double *f2;      // F'' series
-    double dlde;
+    const double *dlde;    // per replica: dL/dE of its energy term
+    const double *rw;      // per replica: weight of its force residuals, 1 / (3 N)
     double *part;          // nblocks x 64 partial moments
     int nblocks;
     double *out;
 };
 int cm_adjoint_max_coefs();
 int cm_adjoint_moments();
-void cm_launch_adj_resid(const DevState &d, const CmAdjoint &J, const double *ft, double gscale, hipStream_t st);
+void cm_launch_adj_resid(const DevState &d, const CmAdjoint &J, const double *ft, hipStream_t st);
 void cm_launch_adj_stage(const DevState &d, const PotDev &P, const CmAdjoint &J, int stage, int mixed,
                          hipStream_t st);
 

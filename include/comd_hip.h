@@ -269,10 +269,12 @@ int comd_batch_step(comd_ctx *ctx, int n, double dt_seconds);
  * order, on the domains the context was created with) and rebuild the three
  * derivative series with the recurrence of src-flat/cheby.c:219-270 -- the
  * runtime form of setChebPot (src-flat/cheby.c:99-121).  n must equal the
- * context's cheb_n. */
+ * context's cheb_n.  In a batch, every replica (all EAM with series) takes
+ * the same coefficients. */
 int comd_set_cheb(comd_ctx *ctx, const double *coef, int n);
 
-/* The six series in use (phi rho F dphi drho dF, 6*cheb_n doubles). */
+/* The six series in use (phi rho F dphi drho dF, 6*cheb_n doubles; of
+ * replica 0 in a batch). */
 int comd_get_cheb(comd_ctx *ctx, double *coef6);
 
 /* Force-matching loss and its gradient with respect to the 3*n value
@@ -288,6 +290,16 @@ int comd_get_cheb(comd_ctx *ctx, double *coef6);
  * reference: it replaces fitting by finite differences over the force call. */
 int comd_eam_cheb_grad(comd_ctx *ctx, const double *coef, int n, const double *f_target, double e_target,
                        double w_e, double *loss, double *grad);
+
+/* The same over every replica of a batch of EAM replicas with the same
+ * series domains and cutoff, in one launch per adjoint stage:
+ *   loss = sum_r [ sum_{i in r} |f_i - f_target_i|^2 / (3 N_r)
+ *                  + w_e (E_r - e_target[r])^2 ],
+ * e_target[nrep]; f_target (or NULL, as above) covers the batch's slots.
+ * The coefficients are set in every replica.  No counterpart in the
+ * reference. */
+int comd_batch_cheb_grad(comd_ctx *ctx, const double *coef, int n, const double *f_target, const double *e_target,
+                         double w_e, double *loss, double *grad);
 
 #ifdef __cplusplus
 }

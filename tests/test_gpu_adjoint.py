@@ -170,3 +170,43 @@ def test_autograd_function(gpu, cu500):
     sub = coef.detach()[idx].clone().requires_grad_(True)
     assert torch.autograd.gradcheck(lambda s: force_matching_loss(
         coef.detach().index_put((idx,), s), sims, targets, 1e-4), (sub,), eps=1e-7, atol=1e-7, rtol=1e-3)
+
+
+def test_batch_gradient_equals_sum_of_single_contexts(gpu, cu500):
+    """comd_batch_cheb_grad over three configurations of different size in
+    one batch (link-cell forward, one launch per adjoint stage) against the
+    sum of comd_eam_cheb_grad over the same configurations in their own
+    contexts: the loss is the sum of the losses (each with its own 1/(3N)
+    and energy term) and the gradient the sum of the gradients."""
+    c0 = cu500.cheb[:3].copy()
+    w_e = 1e-4
+    reps, fts, ets = [], [], []
+    loss_sum, grad_sum = 0.0, np.zeros_like(c0)
+    for k, (n, seed) in enumerate(((5, 31), (6, 32), (7, 33))):
+        sim, xyz, bounds = config(cu500, n, seed)
+        ft, et = target_of(sim, perturbed(c0, 40 + k))
+        loss, g = sim.cheb_grad(c0, ft, et, w_e)
+        loss_sum += loss
+        grad_sum += g
+        _, p, _ = seeded_atoms(n, 3.615, 63.55, temp=0.0, disp=0.1, seed=seed)
+        reps.append((xyz, p, bounds))
+        fts.append(ft)
+        ets.append(et)
+        sim.close()
+    b = CA.Batch([dict(pot=cu500, bounds=bd, defgrad=0.0, evaluation="cheby") for _, _, bd in reps],
+                 precision="fp64")
+    for r, (xyz, p, _) in enumerate(reps):
+        b.bin_atoms(r, xyz, p)
+    b.rebox()
+    # the same slot layout as the single contexts (the targets are per slot)
+    for r in range(3):
+        assert b.info(r)[1] * CA.MAXATOMS == fts[r].shape[0]
+    loss_b, grad_b = b.cheb_grad(c0, fts, ets, w_e)
+    print(f"[batch adjoint] loss {loss_b:.12e} sum of singles {loss_sum:.12e}", flush=True)
+    assert abs(loss_b - loss_sum) <= 1e-10 * abs(loss_sum)
+    compare(grad_b, grad_sum, "batch vs sum of singles", 1e-9)
+    # reusing the batch's target
+    loss_c, grad_c = b.cheb_grad(c0, None, ets, w_e)
+    assert loss_c == loss_b
+    np.testing.assert_array_equal(grad_c, grad_b)
+    b.close()
