@@ -12,6 +12,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
 #include <string>
 #include <vector>
 
@@ -35,6 +36,8 @@ struct Replica {
     Grid g;
     PotDev pot;
     int ntot;
+    long nl_ids;                // id capacity of its atoms (max id + 1)
+    uint64_t pot_hash;          // of the potential's parameters, tables and series
     std::vector<void *> owned;  // device buffers of the potential
     double *d_cheb;             // 6 series of cn at a stride of pot.cstride (FP64), or null
     float *d_cheb32;
@@ -94,8 +97,9 @@ struct comd_ctx {
     int nl_invalid_seen; // the lists were invalid when the host last looked
     int nl_group_opt;   // COMD_TUNE_LIST_GROUPS (-1: by precision)
     int nl_rows_lds;    // COMD_TUNE_LIST_ROWS_LDS (-1: whenever they fit)
-    long nl_ids;        // id capacity of the atoms held
-    long nl_alloc_n, nl_alloc_k, nl_alloc_aux, nl_alloc_groups;
+    long nl_alloc_n, nl_alloc_k, nl_alloc_aux, nl_alloc_groups, nl_alloc_rep;
+    int nl_eam_groups, nl_lj_groups;  // the list groups of the EAM replicas (first), then of the LJ ones
+    int nl_eam_rep, nl_lj_rep;        // a replica of each kind (their potentials stand for the kind's)
     // one DKD step captured as a HIP graph (comd_steps)
     int use_graph;
     hipGraphExec_t graph;
@@ -258,18 +262,6 @@ static void op_force(comd_ctx *c)
     DevState d = dev_state(c);
     const int mixed = c->precision == COMD_PREC_MIXED;
     hipEvent_t ev = nullptr;
-    if (c->nrep > 1) {
-        tbegin(c, COMD_K_DENSITY, &ev);
-        cm_launch_force_batch(d, c->d_pots, 1, mixed, force_cfg(c, CM_KIND_EAM1, mixed), c->stream);
-        tend(c, COMD_K_DENSITY, ev);
-        if (any_eam(c)) {
-            tbegin(c, COMD_K_EMBED_FORCE, &ev);
-            cm_launch_force_batch(d, c->d_pots, 3, mixed, force_cfg(c, CM_KIND_EAM3, mixed), c->stream);
-            tend(c, COMD_K_EMBED_FORCE, ev);
-        }
-        return;
-    }
-    const PotDev &P = c->reps[0].pot;
     // with neighbour lists: the (usually skipped) list build, then each pass
     // over the lists; the link-cell kernels run only if the lists are invalid
     const bool nl = c->nl_active != 0;
@@ -282,12 +274,45 @@ static void op_force(comd_ctx *c)
         cm_nl_build(d, c->nl, c->stream);
         tend(c, COMD_K_LIST_BUILD, ev);
     }
+    if (c->nrep > 1) {
+        // lists: the EAM replicas' groups (density, then embedding force)
+        // and the LJ replicas' groups, each kind with its shared potential;
+        // then the link-cell passes over all replicas, which do the work
+        // when there are no lists and return at once beside valid ones
+        if (nl) {
+            tbegin(c, COMD_K_DENSITY, &ev);
+            if (c->nl_eam_groups > 0)
+                cm_nl_force(d, c->reps[c->nl_eam_rep].pot, c->nl, CM_KIND_EAM1, mixed, c->stream, 0,
+                            c->nl_eam_groups);
+            if (c->nl_lj_groups > 0)
+                cm_nl_force(d, c->reps[c->nl_lj_rep].pot, c->nl, CM_KIND_LJ, mixed, c->stream, c->nl_eam_groups,
+                            c->nl_lj_groups);
+            tend(c, COMD_K_DENSITY, ev);
+            if (c->nl_eam_groups > 0) {
+                tbegin(c, COMD_K_EMBED_FORCE, &ev);
+                cm_nl_force(d, c->reps[c->nl_eam_rep].pot, c->nl, CM_KIND_EAM3, mixed, c->stream, 0,
+                            c->nl_eam_groups);
+                tend(c, COMD_K_EMBED_FORCE, ev);
+            }
+        }
+        auto cells = [&](int pass, int kind, int cls) {
+            tbegin(c, nl ? COMD_K_CELL_FALLBACK : cls, &ev);
+            ForceCfg cfg = force_cfg(c, kind, mixed);
+            if (nl && !c->nl_invalid_seen && cfg.blocks > c->ncu) cfg.blocks = c->ncu;
+            cm_launch_force_batch(d, c->d_pots, pass, mixed, cfg, c->stream);
+            tend(c, nl ? COMD_K_CELL_FALLBACK : cls, ev);
+        };
+        cells(1, CM_KIND_EAM1, COMD_K_DENSITY);
+        if (any_eam(c)) cells(3, CM_KIND_EAM3, COMD_K_EMBED_FORCE);
+        return;
+    }
+    const PotDev &P = c->reps[0].pot;
     // one pass: the list kernel, then the link-cell kernel (which does the
     // work when there are no lists, and returns at once while they are valid)
     auto pass = [&](int kind, int cls) {
         if (nl) {
             tbegin(c, cls, &ev);
-            cm_nl_force(d, P, c->nl, kind, mixed, c->stream);
+            cm_nl_force(d, P, c->nl, kind, mixed, c->stream, 0, c->nl.ngroups);
             tend(c, cls, ev);
         }
         const int ccls = nl ? COMD_K_CELL_FALLBACK : cls;
@@ -397,11 +422,12 @@ static void free_nl(comd_ctx *c)
     void *p[] = {c->nl.pos,      c->nl.selfh,    c->nl.ref,       c->nl.slot,    c->nl.order,    c->nl.fq,
                  c->nl.nbl,      c->nl.ncnt,     c->nl.aux_cnt,   c->nl.aux_n,   c->nl.aux_start,
                  c->nl.aux_mem,  c->nl.gstart,   c->nl.nonempty,  c->nl.count,   c->nl.flags,
-                 (void *)c->nl.cell_order, (void *)c->nl.group_q0};
+                 (void *)c->nl.cell_order, (void *)c->nl.group_q0, (void *)c->nl.group_rep,
+                 (void *)c->nl.aux_group, (void *)c->nl.rep};
     for (void *q : p)
         if (q) hipFree(q);
     memset(&c->nl, 0, sizeof c->nl);
-    c->nl_alloc_n = c->nl_alloc_k = c->nl_alloc_aux = c->nl_alloc_groups = 0;
+    c->nl_alloc_n = c->nl_alloc_k = c->nl_alloc_aux = c->nl_alloc_groups = c->nl_alloc_rep = 0;
     c->nl_active = 0;
 }
 
@@ -475,8 +501,42 @@ static int check_params(const comd_params *p)
 
 // Device copy of one replica's potential (tables, per-interval coefficients,
 // Chebyshev series) and its PotDev.
+// FNV-1a over bytes, to tell replicas that share a potential.
+static uint64_t fnv1a(uint64_t h, const void *data, size_t n)
+{
+    const unsigned char *b = (const unsigned char *)data;
+    for (size_t i = 0; i < n; i++) {
+        h ^= b[i];
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+static uint64_t pot_hash(const comd_params *p)
+{
+    uint64_t h = 1469598103934665603ull;
+    const double sc[5] = {p->cutoff, p->mass, p->lj_sigma, p->lj_epsilon, (double)p->eval};
+    h = fnv1a(h, &p->potential, sizeof p->potential);
+    h = fnv1a(h, sc, sizeof sc);
+    if (p->potential == COMD_POT_EAM) {
+        for (int k = 0; k < 3; k++) {
+            const double hd[3] = {p->tab_x0[k], p->tab_xn[k], p->tab_invdx[k]};
+            h = fnv1a(h, hd, sizeof hd);
+            h = fnv1a(h, &p->tab_n[k], sizeof p->tab_n[k]);
+            if (p->tab_values[k]) h = fnv1a(h, p->tab_values[k], (size_t)(p->tab_n[k] + 3) * 8);
+        }
+        if (p->cheb_n >= 2 && p->cheb_coef) {
+            h = fnv1a(h, p->cheb_coef, (size_t)6 * p->cheb_n * 8);
+            h = fnv1a(h, p->cheb_x0, sizeof p->cheb_x0);
+            h = fnv1a(h, p->cheb_xn, sizeof p->cheb_xn);
+        }
+    }
+    return h;
+}
+
 static bool setup_pot(const comd_params *p, Replica &R)
 {
+    R.pot_hash = pot_hash(p);
     PotDev &P = R.pot;
     memset(&P, 0, sizeof P);
     P.kind = p->potential;
@@ -767,7 +827,7 @@ static int ctx_create(int nrep, const comd_params *prms, comd_ctx **out)
     c->force_blocks_per_cu = 8;
     c->tab_lds = 1;
     c->use_graph = nrep > 1;
-    c->nl_skin_milli = nrep > 1 ? 0 : 400;
+    c->nl_skin_milli = 400;
     c->nl_group_opt = -1;
     c->nl_rows_lds = -1;
     if (hipStreamSynchronize(c->stream) != hipSuccess) {
@@ -790,69 +850,96 @@ static int check_rep(comd_ctx *c, int r)
 // least three link cells and three (cutoff + skin) cells per dimension;
 // otherwise the link-cell kernels do every pass.  Sets the id -> slot and
 // id -> position maps and requests a build.
-static int nl_setup(comd_ctx *c, long nids)
+static int nl_setup(comd_ctx *c)
 {
     c->nl_active = 0;
-    c->nl_ids = nids;
     invalidate_graph(c);
-    if (c->nrep != 1 || c->nl_skin_milli <= 0 || nids <= 0) return COMD_OK;
-    const comd_params &p = c->reps[0].prm;
-    if (!p.periodic || p.nbx[0] < 3 || p.nbx[1] < 3 || p.nbx[2] < 3) return COMD_OK;
-    const double rl = p.cutoff + 1e-3 * c->nl_skin_milli;
-    int nmax[3];
-    for (int j = 0; j < 3; j++) {
-        nmax[j] = (int)floor(p.bounds[j] / rl);
-        if (nmax[j] < 3) return COMD_OK;
+    const int nrep = c->nrep;
+    if (c->nl_skin_milli <= 0) return COMD_OK;
+    const double skin = 1e-3 * c->nl_skin_milli;
+    // every replica periodic, with atoms and at least 3 auxiliary cells per
+    // dimension; the replicas of a kind share one potential
+    int eam_rep = -1, lj_rep = -1;
+    std::vector<std::array<int, 3>> nmax(nrep);
+    long kmax = 16;
+    for (int r = 0; r < nrep; r++) {
+        const Replica &R = c->reps[r];
+        const comd_params &p = R.prm;
+        if (R.ntot <= 0 || R.nl_ids <= 0) return COMD_OK;
+        if (!p.periodic || p.nbx[0] < 3 || p.nbx[1] < 3 || p.nbx[2] < 3) return COMD_OK;
+        int &kr = p.potential == COMD_POT_LJ ? lj_rep : eam_rep;
+        if (kr < 0) kr = r;
+        else if (c->reps[kr].pot_hash != R.pot_hash || c->reps[kr].pot.lj_fsign != R.pot.lj_fsign ||
+                 c->reps[kr].pot.eval != R.pot.eval)
+            return COMD_OK;
+        const double rl = p.cutoff + skin;
+        for (int j = 0; j < 3; j++) {
+            nmax[r][j] = (int)floor(p.bounds[j] / rl);
+            if (nmax[r][j] < 3) return COMD_OK;
+        }
+        const double vol = p.bounds[0] * p.bounds[1] * p.bounds[2];
+        const double expect = 4.0 / 3.0 * 3.141592653589793 * rl * rl * rl * (double)R.ntot / vol;
+        kmax = std::max<long>(kmax, (((long)(1.5 * expect) + 16 + 15) / 16) * 16);
     }
-    const double vol = p.bounds[0] * p.bounds[1] * p.bounds[2];
-    const double expect = 4.0 / 3.0 * 3.141592653589793 * rl * rl * rl * (double)c->ntot / vol;
-    const long kmax = (((long)(1.5 * expect) + 16 + 15) / 16) * 16;
     // groups of auxiliary cells: of the options whose staged halo (1.2x the
-    // mean occupancy, plus 64 atoms) fits the LDS a block may use in every
-    // pass (16 B per halo atom in mixed precision, up to 32 B in FP64, and a
-    // byte of slot map) beside the kernels' static LDS, the largest that
-    // still gives two groups (blocks) per CU, else the one with the most
-    // groups; a tuned option is taken as the largest allowed.  The auxiliary
-    // grid is rounded down to whole groups per dimension (cells a little
-    // larger than cutoff + skin): a partial group at an edge would take a
-    // whole block for a sliver of atoms.
-    int na[3] = {0, 0, 0}, gd[3] = {0, 0, 0};
-    long hcap = 0, naux = 0;
+    // mean occupancy, plus 64 atoms, in the densest replica) fits the LDS a
+    // block may use in every pass (16 B per halo atom in mixed precision, up
+    // to 32 B in FP64, and a byte of slot map) beside the kernels' static
+    // LDS, the largest that still gives two groups (blocks) per CU, else the
+    // one with the most groups; a tuned option is taken as the largest
+    // allowed.  Each replica's auxiliary grid is rounded down to whole groups
+    // per dimension (cells a little larger than cutoff + skin): a partial
+    // group at an edge would take a whole block for a sliver of atoms.
     const int gopts[5][3] = {{4, 4, 3}, {4, 4, 2}, {4, 2, 2}, {2, 2, 2}, {2, 2, 1}};
     const long lds_avail = (long)c->lds_block - CM_NL_STATIC_LDS;
     const long hb = c->precision == COMD_PREC_MIXED ? 16 : 32;
-    auto shape = [&](int o) {  // fills gd, na, naux, hcap; returns the group count
-        for (int j = 0; j < 3; j++) {
-            gd[j] = std::min(gopts[o][j], nmax[j]);
-            na[j] = nmax[j] / gd[j] * gd[j];
+    std::vector<std::array<int, 3>> gd(nrep), na(nrep), ng(nrep);
+    long hcap = 0, naux = 0, ngroups = 0, threads = 64;
+    auto shape = [&](int o) {  // fills gd, na, ng, hcap, naux, ngroups, threads
+        hcap = naux = ngroups = 0;
+        threads = 64;
+        for (int r = 0; r < nrep; r++) {
+            long nr = 1;
+            for (int j = 0; j < 3; j++) {
+                gd[r][j] = std::min(gopts[o][j], nmax[r][j]);
+                na[r][j] = nmax[r][j] / gd[r][j] * gd[r][j];
+                ng[r][j] = na[r][j] / gd[r][j];
+                nr *= na[r][j];
+            }
+            const double occ = (double)c->reps[r].ntot / (double)nr;
+            const long cells = (long)gd[r][0] * gd[r][1] * gd[r][2];
+            const long slots = (long)(gd[r][0] + 2) * (gd[r][1] + 2) * (gd[r][2] + 2);
+            hcap = std::max<long>(hcap, (((long)(1.2 * occ * (double)slots) + 64 + 63) / 64) * 64);
+            threads = std::max<long>(threads, ((long)(1.1 * occ * (double)cells) + 63) / 64 * 64);
+            naux += nr;
+            ngroups += (long)ng[r][0] * ng[r][1] * ng[r][2];
         }
-        naux = (long)na[0] * na[1] * na[2];
-        const double occ = (double)c->ntot / (double)naux;
-        const long slots = (long)(gd[0] + 2) * (gd[1] + 2) * (gd[2] + 2);
-        hcap = (((long)(1.2 * occ * (double)slots) + 64 + 63) / 64) * 64;
-        return (long)((na[0] + gd[0] - 1) / gd[0]) * ((na[1] + gd[1] - 1) / gd[1]) * ((na[2] + gd[2] - 1) / gd[2]);
     };
     int best = -1;
     long best_ng = -1;
     for (int o = c->nl_group_opt >= 0 ? c->nl_group_opt : 0; o < 5; o++) {
-        const long ngo = shape(o);
+        shape(o);
         if (hcap * (hb + 1) > lds_avail || hcap > 65535) continue;
-        if (c->nl_group_opt >= 0 || ngo >= 2L * c->ncu) {
+        if (c->nl_group_opt >= 0 || ngroups >= 2L * c->ncu) {
             best = o;
             break;
         }
-        if (ngo > best_ng) {
+        if (ngroups > best_ng) {
             best = o;
-            best_ng = ngo;
+            best_ng = ngroups;
         }
     }
     if (best < 0) return COMD_OK;  // too dense for staged halos
     shape(best);
-    const double occ = (double)c->ntot / (double)naux;
-    int ng[3];
-    for (int j = 0; j < 3; j++) ng[j] = (na[j] + gd[j] - 1) / gd[j];
-    const long ngroups = (long)ng[0] * ng[1] * ng[2];
-    if (nids > c->nl_alloc_n || kmax > c->nl_alloc_k || naux > c->nl_alloc_aux || ngroups > c->nl_alloc_groups) {
+    // replicas in group order: the EAM ones, then the LJ ones
+    std::vector<int> gorder;
+    for (int pass = 0; pass < 2; pass++)
+        for (int r = 0; r < nrep; r++)
+            if ((c->reps[r].prm.potential == COMD_POT_LJ) == (pass == 1)) gorder.push_back(r);
+    long nids = 0;
+    for (int r = 0; r < nrep; r++) nids += c->reps[r].nl_ids;
+    if (nids > c->nl_alloc_n || kmax > c->nl_alloc_k || naux > c->nl_alloc_aux || ngroups > c->nl_alloc_groups ||
+        nrep > c->nl_alloc_rep) {
         free_nl(c);
         bool ok = true;
         auto alloc = [&](void **ptr, size_t bytes) {
@@ -874,6 +961,9 @@ static int nl_setup(comd_ctx *c, long nids)
         alloc((void **)&c->nl.gstart, (size_t)(ngroups + 1) * 4);
         alloc((void **)&c->nl.cell_order, (size_t)naux * 4);
         alloc((void **)&c->nl.group_q0, (size_t)(ngroups + 1) * 4);
+        alloc((void **)&c->nl.group_rep, (size_t)ngroups * 4);
+        alloc((void **)&c->nl.aux_group, (size_t)naux * 4);
+        alloc((void **)&c->nl.rep, (size_t)nrep * sizeof(NlRep));
         alloc((void **)&c->nl.nonempty, (size_t)c->ncells * 4);
         alloc((void **)&c->nl.count, 16);
         alloc((void **)&c->nl.flags, 16);
@@ -885,73 +975,103 @@ static int nl_setup(comd_ctx *c, long nids)
         c->nl_alloc_k = kmax;
         c->nl_alloc_aux = naux;
         c->nl_alloc_groups = ngroups;
+        c->nl_alloc_rep = nrep;
     }
     CmList &l = c->nl;
     l.n = (int)nids;
     l.kmax = (int)kmax;
-    l.rl2 = rl * rl;
-    l.L0 = p.bounds[0];
-    l.L1 = p.bounds[1];
-    l.L2 = p.bounds[2];
-    l.na0 = na[0];
-    l.na1 = na[1];
-    l.na2 = na[2];
+    l.nrep = nrep;
     l.naux = (int)naux;
-    l.ab0 = p.bounds[0] / na[0];
-    l.ab1 = p.bounds[1] / na[1];
-    l.ab2 = p.bounds[2] / na[2];
-    l.iab0 = 1.0 / l.ab0;
-    l.iab1 = 1.0 / l.ab1;
-    l.iab2 = 1.0 / l.ab2;
-    l.iL0 = 1.0 / l.L0;
-    l.iL1 = 1.0 / l.L1;
-    l.iL2 = 1.0 / l.L2;
-    l.gd0 = gd[0];
-    l.gd1 = gd[1];
-    l.gd2 = gd[2];
-    l.ng0 = ng[0];
-    l.ng1 = ng[1];
-    l.ng2 = ng[2];
     l.ngroups = (int)ngroups;
     l.hcap = (int)hcap;
-    {
-        // 32-bit fixed point of the mixed passes' group frames: a halo atom
-        // lies within (gd/2 + 1) cells of the group centre plus the half skin
-        // it may have moved since the build; 5 % to spare.  A power of two,
-        // so that scaling by it or its inverse is exact: the passes then
-        // work in fixed-point units and fold the scale into constants
-        double ext = 0.0;
-        const double ab[3] = {l.ab0, l.ab1, l.ab2};
-        for (int j = 0; j < 3; j++) ext = std::max(ext, (0.5 * gd[j] + 1.0) * ab[j]);
-        ext += 1e-3 * c->nl_skin_milli;
-        l.qscale = std::ldexp(1.0, (int)std::floor(std::log2(2147483648.0 / (1.05 * ext))));
-        l.qinv = 1.0 / l.qscale;
-    }
     l.lds_avail = (int)lds_avail;
     l.rows_lds = c->nl_rows_lds;
     l.pass_blocks = (int)ngroups;
     // threads per pass block: a group's home atoms with 10 % to spare
-    l.pass_threads = (int)std::min<long>(768, std::max<long>(64, ((long)(1.1 * occ * gd[0] * gd[1] * gd[2]) + 63) / 64 * 64));
-    {
-        // auxiliary cells in build order: groups (x fastest), then the
+    l.pass_threads = (int)std::min<long>(768, threads);
+    std::vector<NlRep> reps(nrep);
+    std::vector<int> corder((size_t)naux), gq0((size_t)ngroups + 1), grep((size_t)ngroups), agroup((size_t)naux);
+    long q = 0, g = 0, abase = 0, ibase = 0;
+    c->nl_eam_groups = c->nl_lj_groups = 0;
+    for (int r : gorder) {
+        Replica &Rp = c->reps[r];
+        const comd_params &p = Rp.prm;
+        NlRep &R = reps[r];
+        R.L0 = p.bounds[0];
+        R.L1 = p.bounds[1];
+        R.L2 = p.bounds[2];
+        R.iL0 = 1.0 / R.L0;
+        R.iL1 = 1.0 / R.L1;
+        R.iL2 = 1.0 / R.L2;
+        R.na0 = na[r][0];
+        R.na1 = na[r][1];
+        R.na2 = na[r][2];
+        R.ab0 = R.L0 / R.na0;
+        R.ab1 = R.L1 / R.na1;
+        R.ab2 = R.L2 / R.na2;
+        R.iab0 = 1.0 / R.ab0;
+        R.iab1 = 1.0 / R.ab1;
+        R.iab2 = 1.0 / R.ab2;
+        R.gd0 = gd[r][0];
+        R.gd1 = gd[r][1];
+        R.gd2 = gd[r][2];
+        R.ng0 = ng[r][0];
+        R.ng1 = ng[r][1];
+        R.ng2 = ng[r][2];
+        const double rl = p.cutoff + skin;
+        R.rl2 = rl * rl;
+        {
+            // 32-bit fixed point of the mixed passes' group frames: a halo
+            // atom lies within (gd/2 + 1) cells of the group centre plus the
+            // half skin it may have moved since the build; 5 % to spare.  A
+            // power of two, so that scaling by it or its inverse is exact:
+            // the passes then work in fixed-point units and fold the scale
+            // into constants
+            const double ab[3] = {R.ab0, R.ab1, R.ab2};
+            double ext = 0.0;
+            for (int j = 0; j < 3; j++) ext = std::max(ext, (0.5 * gd[r][j] + 1.0) * ab[j]);
+            ext += skin;
+            R.qscale = std::ldexp(1.0, (int)std::floor(std::log2(2147483648.0 / (1.05 * ext))));
+            R.qinv = 1.0 / R.qscale;
+        }
+        R.aux_base = (int)abase;
+        R.group_base = (int)g;
+        R.id_base = (int)ibase;
+        R.nid = (int)Rp.nl_ids;
+        Rp.g.nl_id_base = R.id_base;
+        // its auxiliary cells in build order: groups (x fastest), then the
         // group's cells (x fastest)
-        std::vector<int> corder((size_t)naux), gq0((size_t)ngroups + 1);
-        long q = 0, g = 0;
-        for (int gz = 0; gz < ng[2]; gz++)
-            for (int gy = 0; gy < ng[1]; gy++)
-                for (int gx = 0; gx < ng[0]; gx++) {
-                    gq0[g++] = (int)q;
-                    for (int z = gz * gd[2]; z < std::min(na[2], (gz + 1) * gd[2]); z++)
-                        for (int y = gy * gd[1]; y < std::min(na[1], (gy + 1) * gd[1]); y++)
-                            for (int x = gx * gd[0]; x < std::min(na[0], (gx + 1) * gd[0]); x++)
-                                corder[q++] = x + na[0] * (y + na[1] * z);
+        for (int gz = 0; gz < R.ng2; gz++)
+            for (int gy = 0; gy < R.ng1; gy++)
+                for (int gx = 0; gx < R.ng0; gx++) {
+                    gq0[g] = (int)q;
+                    grep[g] = r;
+                    for (int z = gz * R.gd2; z < (gz + 1) * R.gd2; z++)
+                        for (int y = gy * R.gd1; y < (gy + 1) * R.gd1; y++)
+                            for (int x = gx * R.gd0; x < (gx + 1) * R.gd0; x++) {
+                                const long a = abase + x + R.na0 * (y + (long)R.na1 * z);
+                                corder[q++] = (int)a;
+                                agroup[a] = (int)g;
+                            }
+                    g++;
                 }
-        gq0[g] = (int)q;
-        HIPCHK(c, hipMemcpyAsync((void *)l.cell_order, corder.data(), corder.size() * 4, hipMemcpyHostToDevice,
-                                 c->stream));
-        HIPCHK(c, hipMemcpyAsync((void *)l.group_q0, gq0.data(), gq0.size() * 4, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
+        const int rg = R.ng0 * R.ng1 * R.ng2;
+        if (p.potential == COMD_POT_LJ) c->nl_lj_groups += rg;
+        else c->nl_eam_groups += rg;
+        abase += (long)R.na0 * R.na1 * R.na2;
+        ibase += Rp.nl_ids;
     }
+    gq0[g] = (int)q;
+    c->nl_eam_rep = eam_rep;
+    c->nl_lj_rep = lj_rep;
+    HIPCHK(c, hipMemcpyAsync((void *)l.cell_order, corder.data(), corder.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync((void *)l.group_q0, gq0.data(), gq0.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync((void *)l.group_rep, grep.data(), grep.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync((void *)l.aux_group, agroup.data(), agroup.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync((void *)l.rep, reps.data(), reps.size() * sizeof(NlRep), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int rc = upload_descriptors(c);  // the grids' list id bases
+    if (rc) return rc;
     HIPCHK(c, hipMemsetAsync(l.nbl, 0, (size_t)nids * (size_t)kmax * 2, c->stream));
     HIPCHK(c, hipMemsetAsync(l.flags, 0, 16, c->stream));
     HIPCHK(c, hipMemsetAsync(l.count, 0, 16, c->stream));
@@ -999,7 +1119,8 @@ static int upload_rep(comd_ctx *c, int r, const int *natoms, const int *id, cons
     long nids = 0;
     for (size_t b = 0; b < C; b++)
         for (int i = 0; i < natoms[b]; i++) nids = std::max<long>(nids, (long)id[b * CM_SLOTS + i] + 1);
-    return nl_setup(c, nids);
+    c->reps[r].nl_ids = nids;
+    return nl_setup(c);
 }
 
 static int download_rep(comd_ctx *c, int r, int *natoms, int *id, int *itype, double *rr, double *p, double *f,
@@ -1055,7 +1176,8 @@ static int bin_rep_device(comd_ctx *c, int r, int n, const double *d_xyz, const 
     c->ntot += n - c->reps[r].ntot;
     c->reps[r].ntot = n;
     c->have_atoms = 1;
-    return nl_setup(c, n);
+    c->reps[r].nl_ids = n;
+    return nl_setup(c);
 }
 
 static int bin_rep(comd_ctx *c, int r, int n, const double *xyz, const double *pxyz, int itype)
@@ -1307,7 +1429,7 @@ int comd_set_precision(comd_ctx *c, int precision)
     c->precision = precision;
     for (Replica &R : c->reps) R.prm.precision = precision;
     // the list groups depend on the precision (halo atom width): rebuild
-    if (precision != old && c->nl_active) return nl_setup(c, c->nl_ids);
+    if (precision != old && c->nl_active) return nl_setup(c);
     return COMD_OK;
 }
 
@@ -1371,7 +1493,7 @@ int comd_tune(comd_ctx *c, int key, int value)
     case COMD_TUNE_LIST_GROUPS:
         if (value < -1 || value > 4) return set_err(c, COMD_ERR_ARG, "list group option must be -1..4");
         c->nl_group_opt = value;
-        if (c->have_atoms) return nl_setup(c, c->nl_ids);
+        if (c->have_atoms) return nl_setup(c);
         break;
     case COMD_TUNE_LIST_ROWS_LDS:
         if (value < -1 || value > 1) return set_err(c, COMD_ERR_ARG, "list table rows option must be -1..1");
@@ -1381,7 +1503,7 @@ int comd_tune(comd_ctx *c, int key, int value)
     case COMD_TUNE_SKIN_MILLI:
         if (value < 0 || value > 3000) return set_err(c, COMD_ERR_ARG, "skin must be 0..3000 (1e-3 A)");
         c->nl_skin_milli = value;
-        if (c->have_atoms) return nl_setup(c, c->nl_ids);
+        if (c->have_atoms) return nl_setup(c);
         break;
     default:
         return set_err(c, COMD_ERR_ARG, "unknown tuning key %d", key);

@@ -62,7 +62,7 @@ __device__ __forceinline__ void move_atom(const DevState &d, const Grid &g, int 
             // request once the atom has moved half the skin (nearest image)
             double cx, cy, cz;
             corner_of(g, ix, iy, iz, cx, cy, cz);
-            const int id = d.id[s];
+            const int id = g.nl_id_base + d.id[s];
             const double ax = cx + x, ay = cy + y, az = cz + z;
             d.nl_pos[id] = make_double4(ax, ay, az, 0.0);
             const double4 q = d.nl_ref[id];
@@ -262,7 +262,7 @@ __global__ __launch_bounds__(64) void k_rebox_gather(DevState d)
             __syncthreads();  // every lane has read its source before any slot is overwritten
             if (L < nout) {
                 const int o = c * CM_SLOTS + L;
-                if (d.nl_slot) d.nl_slot[gid] = o;
+                if (d.nl_slot) d.nl_slot[grid_of(d, c).nl_id_base + gid] = o;
                 d.id[o] = gid;
                 d.itype[o] = git;
                 d.rx[o] = gx;
@@ -287,7 +287,7 @@ __global__ __launch_bounds__(64) void k_rebox_gather(DevState d)
 // pass counters.
 __device__ static void serial_copy(const DevState &d, int from, int to)
 {
-    if (d.nl_slot) d.nl_slot[d.id[from]] = to;
+    if (d.nl_slot) d.nl_slot[grid_of(d, to / CM_SLOTS).nl_id_base + d.id[from]] = to;
     d.id[to] = d.id[from];
     d.itype[to] = d.itype[from];
     d.rx[to] = d.rx[from];

@@ -21,7 +21,8 @@ enum {
 // arrays; replica r owns the global cells [cell_base, cell_base + ncells)
 // and slot i of global cell c is c*64+i.
 struct Grid {
-    int nbx0, nbx1, nbx2, ncells, cell_base, periodic, pot, pad;
+    int nbx0, nbx1, nbx2, ncells, cell_base, periodic, pot;
+    int nl_id_base;        // its atom ids i are neighbour-list ids nl_id_base + i
     double bs0, bs1, bs2;  // boxsize
     double L0, L1, L2;     // bounds
     double iL0, iL1, iL2;  // 1 / bounds

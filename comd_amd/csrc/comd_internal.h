@@ -148,21 +148,34 @@ void cm_launch_adj_resid(const DevState &d, const CmAdjoint &J, const double *ft
 void cm_launch_adj_stage(const DevState &d, const PotDev &P, const CmAdjoint &J, int stage, int mixed,
                          hipStream_t st);
 
-// Verlet neighbour lists of a single periodic simulation (nlist.hip): per
-// atom all atoms within cutoff + skin at the last build, rebuilt on the
-// device whenever an atom has moved more than half the skin.
+// One replica's auxiliary grid (cells of edge >= cutoff + skin) and its
+// groups of auxiliary cells, for the neighbour lists.
+struct NlRep {
+    double L0, L1, L2, iL0, iL1, iL2;        // bounds and their inverses
+    double ab0, ab1, ab2, iab0, iab1, iab2;  // auxiliary cell edges and their inverses
+    double qscale, qinv;                     // group-frame fixed point of the mixed passes: units per A, A per unit
+    double rl2;                              // (cutoff + skin)^2
+    int na0, na1, na2;                       // auxiliary cells per dimension
+    int gd0, gd1, gd2;                       // cells per group
+    int ng0, ng1, ng2;                       // groups per dimension
+    int aux_base, group_base;                // its first auxiliary cell and group in the global numbering
+    int id_base, nid;                        // its atom ids i map to list ids id_base + i, 0 <= i < nid
+};
+
+// Verlet neighbour lists of periodic simulations (a single one, or every
+// replica of a batch) (nlist.hip): per atom all atoms within cutoff + skin
+// at the last build, rebuilt on the device whenever an atom has moved more
+// than half the skin.  Atoms, auxiliary cells and groups are numbered over
+// all replicas; a group's replica is group_rep[g].
 struct CmList {
-    int n;                   // id capacity (ids 0 .. n-1)
+    int n;                   // list id capacity (ids 0 .. n-1, over all replicas)
     int kmax;                // list capacity per atom (a multiple of 16)
-    double rl2;              // (cutoff + skin)^2
-    double L0, L1, L2;       // bounds
-    int na0, na1, na2, naux; // auxiliary grid used to build the lists
-    double ab0, ab1, ab2;    // its cell edges (>= cutoff + skin)
-    double iab0, iab1, iab2; // and their inverses
-    double iL0, iL1, iL2;    // 1 / L
-    // groups of gd0 x gd1 x gd2 auxiliary cells (fewer at the upper edges),
-    // ng0 x ng1 x ng2 of them; the build order is group-major
-    int gd0, gd1, gd2, ng0, ng1, ng2, ngroups;
+    const NlRep *rep;        // [nrep]
+    int nrep;
+    int naux;                // auxiliary cells, all replicas
+    const int *group_rep;    // [ngroups]: replica of each group
+    const int *aux_group;    // [naux]: group of each auxiliary cell
+    int ngroups;             // groups, all replicas; the build order is group-major
     int hcap;                // staged halo capacity per group (atoms, <= 65535)
     int lds_avail;           // dynamic LDS a list kernel's block may use
     int rows_lds;            // table rows in LDS: 0 never, else whenever they fit
@@ -171,7 +184,6 @@ struct CmList {
     const int *group_q0;     // [ngroups + 1]: a group's first position in cell_order
     double4 *pos, *ref;      // per id (w unused)
     double *fq;              // per position in build order: F' of the last density pass
-    double qscale, qinv;     // group-frame fixed point of the mixed passes: units per A and A per unit
     uint16_t *selfh;         // per t: the atom's own index in its group's staged halo
     int *slot;               // per id: current slot, -1 when absent
     int *order;              // atoms (ids) in build order
@@ -199,7 +211,11 @@ void cm_nl_build(const DevState &d, const CmList &l, hipStream_t st);
 void cm_nl_sync(const DevState &d, const CmList &l, hipStream_t st);
 size_t cm_nl_build_lds(const CmList &l);
 size_t cm_nl_pass_lds(const PotDev &P, const CmList &l, int kind, int mixed, int *tables_in_lds);
-void cm_nl_force(const DevState &d, const PotDev &P, const CmList &l, int kind, int mixed, hipStream_t st);
+// The list pass of one kind over groups [g0, g0 + ng) (a single
+// simulation: all of them; a batch: the groups of its EAM or of its LJ
+// replicas, which share one potential).
+void cm_nl_force(const DevState &d, const PotDev &P, const CmList &l, int kind, int mixed, hipStream_t st, int g0,
+                 int ng);
 // Resident blocks per CU of a list pass (-1 if the runtime cannot tell) and
 // its dynamic LDS bytes.
 int cm_nl_occupancy(const PotDev &P, const CmList &l, int kind, int mixed, int *lds_bytes);

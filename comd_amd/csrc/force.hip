@@ -390,6 +390,8 @@ __global__ __launch_bounds__(256) void k_pair(DevState d, PotDev P)
 template <int PASS, int MIXED>
 __global__ __launch_bounds__(64) void k_pair_batch(DevState d, const PotDev *pots)
 {
+    // beside valid neighbour lists there is nothing to do
+    if (d.cell_gate && !(*d.cell_gate & CM_NL_INVALID)) return;
     extern __shared__ __align__(16) char lds[];
     const int L = threadIdx.x;
     WaveLds w = carve(lds, PASS == 1 ? CM_KIND_EAM1 : CM_KIND_EAM3);

@@ -38,13 +38,19 @@ __device__ __forceinline__ bool nl_rebuild(const CmList &l) { return (l.flags[0]
 // x wrapped into [0, L) (to rounding); iL = 1 / L
 __device__ __forceinline__ double wrap_coord(double x, double L, double iL) { return x - L * floor(x * iL); }
 
-// Auxiliary cell of a position (the same arithmetic wherever it is needed,
-// so that the key and build kernels agree).
-__device__ __forceinline__ void aux_coords(const CmList &l, const double4 &p, int &ax, int &ay, int &az)
+// Replica r's list geometry.
+__device__ __forceinline__ NlRep nl_rep(const CmList &l, int r) { return l.rep[r]; }
+
+// Replica of global cell c.
+__device__ __forceinline__ int cell_rep(const DevState &d, int c) { return d.ngrids == 1 ? 0 : d.cell_grid[c]; }
+
+// Auxiliary cell of a position within its replica (the same arithmetic
+// wherever it is needed, so that the key and build kernels agree).
+__device__ __forceinline__ void aux_coords(const NlRep &R, const double4 &p, int &ax, int &ay, int &az)
 {
-    ax = min(max((int)(wrap_coord(p.x, l.L0, l.iL0) * l.iab0), 0), l.na0 - 1);
-    ay = min(max((int)(wrap_coord(p.y, l.L1, l.iL1) * l.iab1), 0), l.na1 - 1);
-    az = min(max((int)(wrap_coord(p.z, l.L2, l.iL2) * l.iab2), 0), l.na2 - 1);
+    ax = min(max((int)(wrap_coord(p.x, R.L0, R.iL0) * R.iab0), 0), R.na0 - 1);
+    ay = min(max((int)(wrap_coord(p.y, R.L1, R.iL1) * R.iab1), 0), R.na1 - 1);
+    az = min(max((int)(wrap_coord(p.z, R.L2, R.iL2) * R.iab2), 0), R.na2 - 1);
 }
 
 __device__ __forceinline__ void nl_wave_sync()
@@ -74,21 +80,23 @@ __global__ __launch_bounds__(64) void k_nl_from_slots(DevState d, CmList l, int 
     bool far = false;
     if (t < d.natoms[c]) {
         const int s = c * CM_SLOTS + t;
-        const int id = d.id[s];
-        if (id < 0 || id >= l.n) {
+        const NlRep R = nl_rep(l, cell_rep(d, c));
+        const int i = d.id[s];
+        if (i < 0 || i >= R.nid) {
             atomicOr(l.flags, (unsigned)CM_NL_INVALID);
         } else {
+            const int id = R.id_base + i;
             double cx, cy, cz;
-            cell_corner(d.g0, c, cx, cy, cz);
+            cell_corner(grid_of(d, c), c, cx, cy, cz);
             const double x = cx + d.rx[s], y = cy + d.ry[s], z = cz + d.rz[s];
             l.slot[id] = s;
             l.pos[id] = make_double4(x, y, z, 0.0);
             if (check) {
                 const double4 q = l.ref[id];
                 double ex = x - q.x, ey = y - q.y, ez = z - q.z;
-                ex -= l.L0 * rint(ex * l.iL0);
-                ey -= l.L1 * rint(ey * l.iL1);
-                ez -= l.L2 * rint(ez * l.iL2);
+                ex -= R.L0 * rint(ex * R.iL0);
+                ey -= R.L1 * rint(ey * R.iL1);
+                ez -= R.L2 * rint(ez * R.iL2);
                 far = ex * ex + ey * ey + ez * ez > d.nl_half2;
             }
         }
@@ -104,33 +112,34 @@ struct GroupGeo {
     int lx, ly, lz;  // cells per dimension (fewer at the upper edges)
 };
 
-__device__ __forceinline__ GroupGeo group_geo(const CmList &l, int g)
+// Group gl of replica R (gl counted within the replica).
+__device__ __forceinline__ GroupGeo group_geo(const NlRep &R, int gl)
 {
     GroupGeo G;
-    const int gx = g % l.ng0, gy = (g / l.ng0) % l.ng1, gz = g / (l.ng0 * l.ng1);
-    G.x0 = gx * l.gd0;
-    G.y0 = gy * l.gd1;
-    G.z0 = gz * l.gd2;
-    G.lx = min(l.gd0, l.na0 - G.x0);
-    G.ly = min(l.gd1, l.na1 - G.y0);
-    G.lz = min(l.gd2, l.na2 - G.z0);
+    const int gx = gl % R.ng0, gy = (gl / R.ng0) % R.ng1, gz = gl / (R.ng0 * R.ng1);
+    G.x0 = gx * R.gd0;
+    G.y0 = gy * R.gd1;
+    G.z0 = gz * R.gd2;
+    G.lx = min(R.gd0, R.na0 - G.x0);
+    G.ly = min(R.gd1, R.na1 - G.y0);
+    G.lz = min(R.gd2, R.na2 - G.z0);
     return G;
 }
 
 // Halo slot with coordinates (ux, uy, uz) in the group's (lx + 2) x (ly + 2)
 // x (lz + 2) slots (x fastest): its cell (index, and coordinates jx, jy, jz)
 // and the image shift that brings the cell next to the group.
-__device__ __forceinline__ int halo_cell(const CmList &l, const GroupGeo &G, int ux, int uy, int uz, double &sx,
+__device__ __forceinline__ int halo_cell(const NlRep &R, const GroupGeo &G, int ux, int uy, int uz, double &sx,
                                          double &sy, double &sz, int &jx, int &jy, int &jz)
 {
     jx = G.x0 - 1 + ux;
     jy = G.y0 - 1 + uy;
     jz = G.z0 - 1 + uz;
     sx = sy = sz = 0.0;
-    if (jx < 0) { jx += l.na0; sx = -l.L0; } else if (jx >= l.na0) { jx -= l.na0; sx = l.L0; }
-    if (jy < 0) { jy += l.na1; sy = -l.L1; } else if (jy >= l.na1) { jy -= l.na1; sy = l.L1; }
-    if (jz < 0) { jz += l.na2; sz = -l.L2; } else if (jz >= l.na2) { jz -= l.na2; sz = l.L2; }
-    return jx + l.na0 * (jy + l.na1 * jz);
+    if (jx < 0) { jx += R.na0; sx = -R.L0; } else if (jx >= R.na0) { jx -= R.na0; sx = R.L0; }
+    if (jy < 0) { jy += R.na1; sy = -R.L1; } else if (jy >= R.na1) { jy -= R.na1; sy = R.L1; }
+    if (jz < 0) { jz += R.na2; sz = -R.L2; } else if (jz >= R.na2) { jz -= R.na2; sz = R.L2; }
+    return R.aux_base + jx + R.na0 * (jy + R.na1 * jz);  // global auxiliary cell
 }
 
 // A group's halo table in LDS: per slot its first atom in build order and
@@ -141,20 +150,20 @@ struct GroupLds {
     int ns, t0, t1;
 };
 
-// Fills S for group g (all threads of the block; starts and ends with a
-// barrier), and, when slot_of is given, the slot of each staged halo entry
-// (entries below hcap).
-__device__ __forceinline__ void group_setup(const CmList &l, int g, GroupLds &S, CM_LDS uint8_t *slot_of = nullptr)
+// Fills S for group g (global index; G its geometry in replica R) (all
+// threads of the block; starts and ends with a barrier), and, when slot_of
+// is given, the slot of each staged halo entry (entries below hcap).
+__device__ __forceinline__ void group_setup(const CmList &l, const NlRep &R, const GroupGeo &G, int g, GroupLds &S,
+                                            CM_LDS uint8_t *slot_of = nullptr)
 {
     __syncthreads();
-    const GroupGeo G = group_geo(l, g);
     const int ns = (G.lx + 2) * (G.ly + 2) * (G.lz + 2);
     const int hx = G.lx + 2, hy = G.ly + 2;
     for (int s = threadIdx.x; s < ns; s += blockDim.x) {
         const int ux = s % hx, uy = (s / hx) % hy, uz = s / (hx * hy);
         double sx, sy, sz;
         int jx, jy, jz;
-        const int c = halo_cell(l, G, ux, uy, uz, sx, sy, sz, jx, jy, jz);
+        const int c = halo_cell(R, G, ux, uy, uz, sx, sy, sz, jx, jy, jz);
         S.start[s] = l.aux_start[c];
         S.uxyz[s] = ux | uy << 8 | uz << 16;
         S.off[s + 1] = l.aux_n[c];  // counts, scanned below
@@ -199,17 +208,17 @@ typedef unsigned v3u __attribute__((ext_vector_type(3)));
 // relative to the frame origin o.  Atoms that crossed the periodic boundary
 // since the build, and cells that appear twice in a small box's halo, are
 // placed right.
-__device__ __forceinline__ void frame_pos(const CmList &l, const GroupGeo &G, int uxyz, const v4d &p, double ox,
+__device__ __forceinline__ void frame_pos(const NlRep &R, const GroupGeo &G, int uxyz, const v4d &p, double ox,
                                           double oy, double oz, double &x, double &y, double &z)
 {
     double sx, sy, sz;
     int jx, jy, jz;
-    halo_cell(l, G, uxyz & 0xff, (uxyz >> 8) & 0xff, uxyz >> 16, sx, sy, sz, jx, jy, jz);
-    const double cx = (jx + 0.5) * l.ab0, cy = (jy + 0.5) * l.ab1, cz = (jz + 0.5) * l.ab2;
+    halo_cell(R, G, uxyz & 0xff, (uxyz >> 8) & 0xff, uxyz >> 16, sx, sy, sz, jx, jy, jz);
+    const double cx = (jx + 0.5) * R.ab0, cy = (jy + 0.5) * R.ab1, cz = (jz + 0.5) * R.ab2;
     double ex = p.x - cx, ey = p.y - cy, ez = p.z - cz;
-    ex -= l.L0 * rint(ex * l.iL0);
-    ey -= l.L1 * rint(ey * l.iL1);
-    ez -= l.L2 * rint(ez * l.iL2);
+    ex -= R.L0 * rint(ex * R.iL0);
+    ey -= R.L1 * rint(ey * R.iL1);
+    ez -= R.L2 * rint(ez * R.iL2);
     x = (cx + sx - ox) + ex;
     y = (cy + sy - oy) + ey;
     z = (cz + sz - oz) + ez;
@@ -222,14 +231,16 @@ __device__ __forceinline__ void frame_pos(const CmList &l, const GroupGeo &G, in
 
 // Each atom into its auxiliary cell (counts start at zero: the previous
 // build's last block cleared them).
-__global__ __launch_bounds__(256) void k_nl_key(CmList l)
+__global__ __launch_bounds__(256) void k_nl_key(DevState d, CmList l)
 {
     if (!nl_rebuild(l)) return;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < l.n; i += gridDim.x * blockDim.x) {
-        if (l.slot[i] < 0) continue;
+        const int s = l.slot[i];
+        if (s < 0) continue;
+        const NlRep R = nl_rep(l, cell_rep(d, s / CM_SLOTS));
         int ax, ay, az;
-        aux_coords(l, l.pos[i], ax, ay, az);
-        const int a = ax + l.na0 * (ay + l.na1 * az);
+        aux_coords(R, l.pos[i], ax, ay, az);
+        const int a = R.aux_base + ax + R.na0 * (ay + R.na1 * az);
         const int k = atomicAdd(&l.aux_cnt[a], 1);
         if (k < CM_NL_AUXCAP) l.aux_mem[a * CM_NL_AUXCAP + k] = i;
         else atomicOr(l.flags, (unsigned)CM_NL_OVERFLOW);
@@ -325,8 +336,7 @@ __global__ __launch_bounds__(256) void k_nl_sort(CmList l)
     const int L = threadIdx.x & 63, w = threadIdx.x >> 6;
     int *mem = mem_s[w];
     for (int a = blockIdx.x * 4 + w; a < l.naux; a += gridDim.x * 4) {
-        const int ax = a % l.na0, ay = (a / l.na0) % l.na1, az = a / (l.na0 * l.na1);
-        const int g = ax / l.gd0 + l.ng0 * (ay / l.gd1 + l.ng1 * (az / l.gd2));
+        const int g = l.aux_group[a];
         const int start = l.aux_start[a] + l.gstart[g];
         nl_wave_sync();
         if (L == 0) l.aux_start[a] = start;
@@ -361,20 +371,21 @@ __global__ __launch_bounds__(1024) void k_nl_build(CmList l)
     CM_LDS uint8_t *slot_of = (CM_LDS uint8_t *)((CM_LDS char *)hl4 + (size_t)l.hcap * 16);
     __shared__ GroupLds S;
     __shared__ int last;
-    const float rl2 = (float)(l.rl2 * (1.0 + 4e-6));
     bool ovf = false;
     // one group per block, XCD-contiguous as in the passes
     const int nb = gridDim.x, qb = nb / 8, rem = nb % 8;
     const int xcd = blockIdx.x % 8, idx = blockIdx.x / 8;
     const int g = nb < 8 ? (int)blockIdx.x : xcd * qb + min(xcd, rem) + idx;
-    group_setup(l, g, S, slot_of);
+    const NlRep R = nl_rep(l, l.group_rep[g]);
+    const GroupGeo G = group_geo(R, g - R.group_base);
+    const float rl2 = (float)(R.rl2 * (1.0 + 4e-6));
+    group_setup(l, R, G, g, S, slot_of);
     const int H = S.off[S.ns];
     if (H > l.hcap) {  // the same in every thread of the block
         ovf = true;
     } else {
-        const GroupGeo G = group_geo(l, g);
-        const double ox = (G.x0 + 0.5 * G.lx) * l.ab0, oy = (G.y0 + 0.5 * G.ly) * l.ab1,
-                     oz = (G.z0 + 0.5 * G.lz) * l.ab2;
+        const double ox = (G.x0 + 0.5 * G.lx) * R.ab0, oy = (G.y0 + 0.5 * G.ly) * R.ab1,
+                     oz = (G.z0 + 0.5 * G.lz) * R.ab2;
         constexpr int SB = 4;  // staged entries per thread in flight
         for (int h0 = threadIdx.x; h0 < H; h0 += SB * blockDim.x) {
             int sl[SB], q[SB], id[SB];
@@ -394,7 +405,7 @@ __global__ __launch_bounds__(1024) void k_nl_build(CmList l)
                 const int h = h0 + k * (int)blockDim.x;
                 if (h >= H) break;
                 double x, y, z;
-                frame_pos(l, G, S.uxyz[sl[k]], p[k], ox, oy, oz, x, y, z);
+                frame_pos(R, G, S.uxyz[sl[k]], p[k], ox, oy, oz, x, y, z);
                 hl4[h] = make_float4((float)x, (float)y, (float)z, 0.0f);
             }
         }
@@ -404,7 +415,7 @@ __global__ __launch_bounds__(1024) void k_nl_build(CmList l)
             const int id = l.order[t];
             const double4 p = l.pos[id];
             int ax, ay, az;
-            aux_coords(l, p, ax, ay, az);
+            aux_coords(R, p, ax, ay, az);
             // home slot: the atom's own cell, an interior slot of the halo
             const int cx = ax - G.x0 + 1, cy = ay - G.y0 + 1, cz = az - G.z0 + 1;
             const int hs = cx + shx * (cy + shy * cz);
@@ -480,9 +491,10 @@ __global__ void k_nl_nonempty(DevState d, CmList l)
     if (l.flags[0] & CM_NL_INVALID) return;
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= d.ncells) return;
+    const Grid g = grid_of(d, c);
     int n = 0;
     for (int e = 0; e < 27; e++) {
-        const int b = stencil_cell(d.g0, c, e);
+        const int b = stencil_cell(g, c, e);
         n += b >= 0 && d.natoms[b] > 0;
     }
     l.nonempty[c] = n;
@@ -511,7 +523,7 @@ __host__ __device__ constexpr int nl_slotmap_bytes(int hcap) { return (hcap + 15
 template <int KIND, int EVAL, int MIXED, int COMPACT, class TB>
 __device__ __forceinline__ void nl_atom(const DevState &d, const PotDev &P, const CmList &l, const TB &T, int t,
                                         const CM_LDS v4u *hq, const CM_LDS double *hx, const CM_LDS double *hy,
-                                        const CM_LDS double *hz, const CM_LDS double *hf)
+                                        const CM_LDS double *hz, const CM_LDS double *hf, double qscale, double qinv)
 {
     const int i = l.order[t];
     const int s = l.slot[i];
@@ -529,12 +541,11 @@ __device__ __forceinline__ void nl_atom(const DevState &d, const PotDev &P, cons
     }
     if (KIND != CM_KIND_EAM3) fia = 0.0;
     const double rc2 = P.rc2;
-    const double qinv = l.qinv;
     // mixed passes over compact rows work in fixed-point units (qinv is a
     // power of two: the cutoff test and the force terms are the same as in
     // angstrom; the table coordinate takes the unit scale)
     constexpr bool UNITS = MIXED && COMPACT && KIND != CM_KIND_LJ;
-    const double rc2u = rc2 * (l.qscale * l.qscale);
+    const double rc2u = rc2 * (qscale * qscale);
     const double iu = qinv * P.invdx[0], cu = -P.x0[0] * P.invdx[0];
     const int cnt = l.ncnt[t];
     Acc A = {0, 0, 0, 0, 0, 0};
@@ -667,7 +678,7 @@ __device__ __forceinline__ void nl_atom(const DevState &d, const PotDev &P, cons
 // [slot of each halo entry, hcap bytes].
 template <int KIND, int EVAL, int MIXED, int TLDS>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(MIXED && KIND != CM_KIND_LJ && TLDS ? 6 : 1)))
-void k_nl_pair(DevState d, PotDev P, CmList l, int tab_bytes)
+void k_nl_pair(DevState d, PotDev P, CmList l, int tab_bytes, int gbase)
 {
     extern __shared__ __align__(16) char nls[];
     __shared__ GroupLds S;
@@ -702,11 +713,12 @@ void k_nl_pair(DevState d, PotDev P, CmList l, int tab_bytes)
     {
         const int nb = gridDim.x, qb = nb / 8, rem = nb % 8;
         const int xcd = blockIdx.x % 8, idx = blockIdx.x / 8;
-        const int g = nb < 8 ? (int)blockIdx.x : xcd * qb + min(xcd, rem) + idx;
-        group_setup(l, g, S, slot_of);  // its first barrier also covers the table staging
-        const GroupGeo G = group_geo(l, g);
-        const double ox = (G.x0 + 0.5 * G.lx) * l.ab0, oy = (G.y0 + 0.5 * G.ly) * l.ab1,
-                     oz = (G.z0 + 0.5 * G.lz) * l.ab2;
+        const int g = gbase + (nb < 8 ? (int)blockIdx.x : xcd * qb + min(xcd, rem) + idx);
+        const NlRep R = nl_rep(l, l.group_rep[g]);
+        const GroupGeo G = group_geo(R, g - R.group_base);
+        group_setup(l, R, G, g, S, slot_of);  // its first barrier also covers the table staging
+        const double ox = (G.x0 + 0.5 * G.lx) * R.ab0, oy = (G.y0 + 0.5 * G.ly) * R.ab1,
+                     oz = (G.z0 + 0.5 * G.lz) * R.ab2;
         const int H = min(S.off[S.ns], l.hcap);  // never above hcap while the lists are valid
         // up to SB entries per thread in flight: their id and position loads
         // are issued together (a thread stages about hcap / blockDim of
@@ -736,13 +748,13 @@ void k_nl_pair(DevState d, PotDev P, CmList l, int tab_bytes)
                 const int h = h0 + k * (int)blockDim.x;
                 if (h >= H) break;
                 double x, y, z;
-                frame_pos(l, G, S.uxyz[sl[k]], p[k], ox, oy, oz, x, y, z);
+                frame_pos(R, G, S.uxyz[sl[k]], p[k], ox, oy, oz, x, y, z);
                 if (MIXED) {
                     const float fj = (float)fp[k];
                     v4u q;
-                    q.x = (unsigned)(int)rint(x * l.qscale);
-                    q.y = (unsigned)(int)rint(y * l.qscale);
-                    q.z = (unsigned)(int)rint(z * l.qscale);
+                    q.x = (unsigned)(int)rint(x * R.qscale);
+                    q.y = (unsigned)(int)rint(y * R.qscale);
+                    q.z = (unsigned)(int)rint(z * R.qscale);
                     q.w = __float_as_uint(fj);
                     hq[h] = q;
                 } else {
@@ -755,8 +767,8 @@ void k_nl_pair(DevState d, PotDev P, CmList l, int tab_bytes)
         }
         __syncthreads();
         for (int t = S.t0 + threadIdx.x; t < S.t1; t += blockDim.x) {
-            if (TLDS) nl_atom<KIND, EVAL, MIXED, 1>(d, P, l, TL, t, hq, hx, hy, hz, hf);
-            else nl_atom<KIND, EVAL, MIXED, 0>(d, P, l, TG, t, hq, hx, hy, hz, hf);
+            if (TLDS) nl_atom<KIND, EVAL, MIXED, 1>(d, P, l, TL, t, hq, hx, hy, hz, hf, R.qscale, R.qinv);
+            else nl_atom<KIND, EVAL, MIXED, 0>(d, P, l, TG, t, hq, hx, hy, hz, hf, R.qscale, R.qinv);
         }
     }
 }
@@ -785,8 +797,7 @@ size_t cm_nl_build_lds(const CmList &l) { return (size_t)l.hcap * 16 + nl_slotma
 
 void cm_nl_build(const DevState &d, const CmList &l, hipStream_t st)
 {
-    (void)d;
-    hipLaunchKernelGGL(k_nl_key, dim3(1024), dim3(256), 0, st, l);
+    hipLaunchKernelGGL(k_nl_key, dim3(1024), dim3(256), 0, st, d, l);
     hipLaunchKernelGGL(k_nl_gsum, dim3((unsigned)std::min(4096, (l.ngroups + 3) / 4)), dim3(256), 0, st, l);
     hipLaunchKernelGGL(k_nl_sort, dim3(1024), dim3(256), 0, st, l);
     hipLaunchKernelGGL(k_nl_build, dim3(l.pass_blocks), dim3(l.pass_threads), cm_nl_build_lds(l), st, l);
@@ -804,16 +815,16 @@ size_t cm_nl_pass_lds(const PotDev &P, const CmList &l, int kind, int mixed, int
 }
 
 template <int KIND, int EVAL, int MIXED>
-static void nl_launch(const DevState &d, const PotDev &P, const CmList &l, hipStream_t st)
+static void nl_launch(const DevState &d, const PotDev &P, const CmList &l, hipStream_t st, int g0, int ng)
 {
     int tl = 0;
     const size_t lds = cm_nl_pass_lds(P, l, KIND, MIXED, &tl);
     const int tab = tl ? (int)nl_tab_bytes(P, KIND, MIXED) : 0;
-    const dim3 grid(l.pass_blocks), block(l.pass_threads);
+    const dim3 grid(ng), block(l.pass_threads);
     if (EVAL == COMD_EVAL_TABLE && KIND != CM_KIND_LJ && tl)
-        hipLaunchKernelGGL((k_nl_pair<KIND, EVAL, MIXED, 1>), grid, block, lds, st, d, P, l, tab);
+        hipLaunchKernelGGL((k_nl_pair<KIND, EVAL, MIXED, 1>), grid, block, lds, st, d, P, l, tab, g0);
     else
-        hipLaunchKernelGGL((k_nl_pair<KIND, EVAL, MIXED, 0>), grid, block, lds, st, d, P, l, 0);
+        hipLaunchKernelGGL((k_nl_pair<KIND, EVAL, MIXED, 0>), grid, block, lds, st, d, P, l, 0, g0);
 }
 
 // Resident blocks per CU of the pass nl_launch would start, and its dynamic LDS.
@@ -851,22 +862,24 @@ int cm_nl_occupancy(const PotDev &P, const CmList &l, int kind, int mixed, int *
 }
 
 template <int KIND, int EVAL>
-static void nl_launch2(const DevState &d, const PotDev &P, const CmList &l, int mixed, hipStream_t st)
+static void nl_launch2(const DevState &d, const PotDev &P, const CmList &l, int mixed, hipStream_t st, int g0, int ng)
 {
-    if (mixed) nl_launch<KIND, EVAL, 1>(d, P, l, st);
-    else nl_launch<KIND, EVAL, 0>(d, P, l, st);
+    if (mixed) nl_launch<KIND, EVAL, 1>(d, P, l, st, g0, ng);
+    else nl_launch<KIND, EVAL, 0>(d, P, l, st, g0, ng);
 }
 
-void cm_nl_force(const DevState &d, const PotDev &P, const CmList &l, int kind, int mixed, hipStream_t st)
+void cm_nl_force(const DevState &d, const PotDev &P, const CmList &l, int kind, int mixed, hipStream_t st, int g0,
+                 int ng)
 {
+    if (ng <= 0) return;
     if (kind == CM_KIND_LJ) {
         hipLaunchKernelGGL(k_nl_nonempty, dim3(grid_for(d.ncells, 256)), dim3(256), 0, st, d, l);
-        nl_launch2<CM_KIND_LJ, COMD_EVAL_TABLE>(d, P, l, mixed, st);
+        nl_launch2<CM_KIND_LJ, COMD_EVAL_TABLE>(d, P, l, mixed, st, g0, ng);
     } else if (P.eval == COMD_EVAL_CHEB) {
-        if (kind == CM_KIND_EAM1) nl_launch2<CM_KIND_EAM1, COMD_EVAL_CHEB>(d, P, l, mixed, st);
-        else nl_launch2<CM_KIND_EAM3, COMD_EVAL_CHEB>(d, P, l, mixed, st);
+        if (kind == CM_KIND_EAM1) nl_launch2<CM_KIND_EAM1, COMD_EVAL_CHEB>(d, P, l, mixed, st, g0, ng);
+        else nl_launch2<CM_KIND_EAM3, COMD_EVAL_CHEB>(d, P, l, mixed, st, g0, ng);
     } else {
-        if (kind == CM_KIND_EAM1) nl_launch2<CM_KIND_EAM1, COMD_EVAL_TABLE>(d, P, l, mixed, st);
-        else nl_launch2<CM_KIND_EAM3, COMD_EVAL_TABLE>(d, P, l, mixed, st);
+        if (kind == CM_KIND_EAM1) nl_launch2<CM_KIND_EAM1, COMD_EVAL_TABLE>(d, P, l, mixed, st, g0, ng);
+        else nl_launch2<CM_KIND_EAM3, COMD_EVAL_TABLE>(d, P, l, mixed, st, g0, ng);
     }
 }
