@@ -159,10 +159,11 @@ enum comd_tune_key {
     COMD_TUNE_PAIR_LISTS = 6,      /* 0/1, default 1: the EAM embedding-force pass replays the pairs
                                       the density pass accepted instead of testing candidates again */
     COMD_TUNE_SKIN_MILLI = 7,      /* Verlet skin in 1e-3 A (default 400; 0 = link-cell kernels only).
-                                      Single periodic simulations with >= 3 cells of cutoff + skin per
-                                      dimension keep per-atom neighbour lists, rebuilt on the device
-                                      when an atom has moved half the skin; same pairs and outputs,
-                                      per-atom sums in another order */
+                                      Periodic simulations, and batches whose replicas all are, with
+                                      >= 3 cells of cutoff + skin per dimension (and one potential per
+                                      kind in a batch) keep per-atom neighbour lists, rebuilt on the
+                                      device when an atom has moved half the skin; same pairs and
+                                      outputs, per-atom sums in another order */
     COMD_TUNE_LJ_PHYSICAL = 8,     /* 0/1, default 0: Lennard-Jones forces stored as +grad U, so the
                                       integrator moves atoms down the potential (physical motion,
                                       KE + PE conserved).  0 keeps src-flat's convention (ljForce.c
