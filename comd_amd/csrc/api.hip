@@ -368,6 +368,9 @@ static int run_steps(comd_ctx *c, int n, double dt)
         c->graph_dt = dt;
     }
     for (int i = 0; i < n; i++) HIPCHK(c, hipGraphLaunch(c->graph, c->stream));
+    // each step ends with a drift that leaves the list positions stale (the
+    // capture recorded that on the host once; replays must too)
+    if (c->nl_active) c->nl_stale = 1;
     return COMD_OK;
 }
 
@@ -895,18 +898,25 @@ static int nl_setup(comd_ctx *c)
     const long hb = c->precision == COMD_PREC_MIXED ? 16 : 32;
     std::vector<std::array<int, 3>> gd(nrep), na(nrep), ng(nrep);
     long hcap = 0, naux = 0, ngroups = 0, threads = 64;
-    auto shape = [&](int o) {  // fills gd, na, ng, hcap, naux, ngroups, threads
+    double occ_max = 0.0;
+    auto shape = [&](int o) {  // fills gd, na, ng, hcap, naux, ngroups, threads, occ_max
         hcap = naux = ngroups = 0;
         threads = 64;
+        occ_max = 0.0;
         for (int r = 0; r < nrep; r++) {
             long nr = 1;
             for (int j = 0; j < 3; j++) {
                 gd[r][j] = std::min(gopts[o][j], nmax[r][j]);
                 na[r][j] = nmax[r][j] / gd[r][j] * gd[r][j];
+                if (na[r][j] < 3) {  // keep >= 3 cells (one group across the dimension)
+                    gd[r][j] = nmax[r][j];
+                    na[r][j] = nmax[r][j];
+                }
                 ng[r][j] = na[r][j] / gd[r][j];
                 nr *= na[r][j];
             }
             const double occ = (double)c->reps[r].ntot / (double)nr;
+            occ_max = std::max(occ_max, occ);
             const long cells = (long)gd[r][0] * gd[r][1] * gd[r][2];
             const long slots = (long)(gd[r][0] + 2) * (gd[r][1] + 2) * (gd[r][2] + 2);
             hcap = std::max<long>(hcap, (((long)(1.2 * occ * (double)slots) + 64 + 63) / 64) * 64);
@@ -919,7 +929,7 @@ static int nl_setup(comd_ctx *c)
     long best_ng = -1;
     for (int o = c->nl_group_opt >= 0 ? c->nl_group_opt : 0; o < 5; o++) {
         shape(o);
-        if (hcap * (hb + 1) > lds_avail || hcap > 65535) continue;
+        if (hcap * (hb + 1) > lds_avail || hcap > 65535 || 1.5 * occ_max > CM_NL_AUXCAP) continue;
         if (c->nl_group_opt >= 0 || ngroups >= 2L * c->ncu) {
             best = o;
             break;

@@ -174,3 +174,18 @@ def test_overflow_falls_back_to_cells(gpu, cu500):
     a.nsteps(2)
     b.nsteps(2)
     assert_same(a, b, "overflow, after steps", tol=1e-10)
+
+
+def test_force_after_replayed_step_graph(gpu, cu500):
+    """A force call after steps replayed from an already captured step graph
+    (the second steps() call reuses the graph of the first) sees the atoms'
+    current positions: each step ends with a drift that leaves the list
+    positions stale, whether the step ran eagerly, was captured, or was
+    replayed."""
+    a, b = pair_of_sims(cu500, 8, 3.615, 600.0, 0.05, 77)
+    for chunk in (4, 9, 9):
+        for s in (a, b):
+            s.steps(chunk)
+            s.force()
+        assert a.list_info()["active"] == 1
+        assert_same(a, b, f"after a {chunk}-step graph chunk", tol=1e-10)
