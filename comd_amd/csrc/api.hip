@@ -898,11 +898,9 @@ static int nl_setup(comd_ctx *c)
     const long hb = c->precision == COMD_PREC_MIXED ? 16 : 32;
     std::vector<std::array<int, 3>> gd(nrep), na(nrep), ng(nrep);
     long hcap = 0, naux = 0, ngroups = 0, threads = 64;
-    double occ_max = 0.0;
-    auto shape = [&](int o) {  // fills gd, na, ng, hcap, naux, ngroups, threads, occ_max
+    auto shape = [&](int o) {  // fills gd, na, ng, hcap, naux, ngroups, threads
         hcap = naux = ngroups = 0;
         threads = 64;
-        occ_max = 0.0;
         for (int r = 0; r < nrep; r++) {
             long nr = 1;
             for (int j = 0; j < 3; j++) {
@@ -916,7 +914,6 @@ static int nl_setup(comd_ctx *c)
                 nr *= na[r][j];
             }
             const double occ = (double)c->reps[r].ntot / (double)nr;
-            occ_max = std::max(occ_max, occ);
             const long cells = (long)gd[r][0] * gd[r][1] * gd[r][2];
             const long slots = (long)(gd[r][0] + 2) * (gd[r][1] + 2) * (gd[r][2] + 2);
             hcap = std::max<long>(hcap, (((long)(1.2 * occ * (double)slots) + 64 + 63) / 64) * 64);
@@ -929,7 +926,7 @@ static int nl_setup(comd_ctx *c)
     long best_ng = -1;
     for (int o = c->nl_group_opt >= 0 ? c->nl_group_opt : 0; o < 5; o++) {
         shape(o);
-        if (hcap * (hb + 1) > lds_avail || hcap > 65535 || 1.5 * occ_max > CM_NL_AUXCAP) continue;
+        if (hcap * (hb + 1) > lds_avail || hcap > 65535) continue;
         if (c->nl_group_opt >= 0 || ngroups >= 2L * c->ncu) {
             best = o;
             break;
