@@ -115,16 +115,18 @@ __device__ __forceinline__ void move_atom(const DevState &d, const Grid &g, int 
     mark_touched(d, dst);
 }
 
-// Any number of replicas: one block of 64 threads per cell.
+// Any number of replicas: four consecutive global cells per wave, 16 lanes
+// each (a cell's atoms beyond 16 taken in further rounds).
 __global__ __launch_bounds__(64) void k_move(DevState d, double dtk, double dtd, int kick, int drift, int rebin)
 {
-    const int c = blockIdx.x;
-    const int t = threadIdx.x;
-    if (t >= d.natoms[c]) return;
+    const int c = blockIdx.x * 4 + (threadIdx.x >> 4);
+    if (c >= d.ncells) return;
+    const int n = d.natoms[c];
+    if ((threadIdx.x & 15) >= n) return;
     const Grid g = grid_of(d, c);
     int ix, iy, iz;
     cell_xyz(g, c, ix, iy, iz);
-    move_atom(d, g, c, ix, iy, iz, c * CM_SLOTS + t, dtk, dtd, kick, drift, rebin);
+    for (int t = threadIdx.x & 15; t < n; t += 16) move_atom(d, g, c, ix, iy, iz, c * CM_SLOTS + t, dtk, dtd, kick, drift, rebin);
 }
 
 // One replica: four cells per wave, 16 lanes each, cell coordinates from the
@@ -421,7 +423,7 @@ void cm_launch_move(const DevState &d, hipStream_t st, double dtk, double dtd, i
         const dim3 grid((unsigned)((d.g0.nbx0 + 3) / 4), (unsigned)d.g0.nbx1, (unsigned)d.g0.nbx2);
         hipLaunchKernelGGL(k_move4, grid, dim3(64), 0, st, d, dtk, dtd, kick, drift, rebin);
     } else {
-        hipLaunchKernelGGL(k_move, dim3(d.ncells), dim3(64), 0, st, d, dtk, dtd, kick, drift, rebin);
+        hipLaunchKernelGGL(k_move, dim3((d.ncells + 3) / 4), dim3(64), 0, st, d, dtk, dtd, kick, drift, rebin);
     }
 }
 
