@@ -898,20 +898,30 @@ static int nl_setup(comd_ctx *c)
     const long hb = c->precision == COMD_PREC_MIXED ? 16 : 32;
     std::vector<std::array<int, 3>> gd(nrep), na(nrep), ng(nrep);
     long hcap = 0, naux = 0, ngroups = 0, threads = 64;
+    // o >= 0: that option for every replica; o < 0 (batches): per replica
+    // the largest option whose groups' atoms (10 % spare) fit 768 threads,
+    // so that replicas of different density and cell rounding get groups of
+    // similar size (one block shape serves them all)
     auto shape = [&](int o) {  // fills gd, na, ng, hcap, naux, ngroups, threads
         hcap = naux = ngroups = 0;
         threads = 64;
         for (int r = 0; r < nrep; r++) {
             long nr = 1;
-            for (int j = 0; j < 3; j++) {
-                gd[r][j] = std::min(gopts[o][j], nmax[r][j]);
-                na[r][j] = nmax[r][j] / gd[r][j] * gd[r][j];
-                if (na[r][j] < 3) {  // keep >= 3 cells (one group across the dimension)
-                    gd[r][j] = nmax[r][j];
-                    na[r][j] = nmax[r][j];
+            for (int oo = o >= 0 ? o : 0; oo < 5; oo++) {
+                nr = 1;
+                for (int j = 0; j < 3; j++) {
+                    gd[r][j] = std::min(gopts[oo][j], nmax[r][j]);
+                    na[r][j] = nmax[r][j] / gd[r][j] * gd[r][j];
+                    if (na[r][j] < 3) {  // keep >= 3 cells (one group across the dimension)
+                        gd[r][j] = nmax[r][j];
+                        na[r][j] = nmax[r][j];
+                    }
+                    ng[r][j] = na[r][j] / gd[r][j];
+                    nr *= na[r][j];
                 }
-                ng[r][j] = na[r][j] / gd[r][j];
-                nr *= na[r][j];
+                if (o >= 0) break;
+                const double apg = (double)c->reps[r].ntot / (double)nr * gd[r][0] * gd[r][1] * gd[r][2];
+                if (1.1 * apg <= 768.0) break;
             }
             const double occ = (double)c->reps[r].ntot / (double)nr;
             const long cells = (long)gd[r][0] * gd[r][1] * gd[r][2];
@@ -924,20 +934,27 @@ static int nl_setup(comd_ctx *c)
     };
     int best = -1;
     long best_ng = -1;
-    for (int o = c->nl_group_opt >= 0 ? c->nl_group_opt : 0; o < 5; o++) {
-        shape(o);
-        if (hcap * (hb + 1) > lds_avail || hcap > 65535) continue;
-        if (c->nl_group_opt >= 0 || ngroups >= 2L * c->ncu) {
-            best = o;
-            break;
-        }
-        if (ngroups > best_ng) {
-            best = o;
-            best_ng = ngroups;
-        }
+    if (nrep > 1 && c->nl_group_opt < 0) {
+        shape(-1);
+        if (hcap * (hb + 1) <= lds_avail && hcap <= 65535) best = -1, best_ng = 0;
+        else best_ng = -2;  // fall back to one option for all
     }
-    if (best < 0) return COMD_OK;  // too dense for staged halos
-    shape(best);
+    if (best_ng != 0) {
+        for (int o = c->nl_group_opt >= 0 ? c->nl_group_opt : 0; o < 5; o++) {
+            shape(o);
+            if (hcap * (hb + 1) > lds_avail || hcap > 65535) continue;
+            if (c->nl_group_opt >= 0 || ngroups >= 2L * c->ncu) {
+                best = o;
+                break;
+            }
+            if (ngroups > best_ng) {
+                best = o;
+                best_ng = ngroups;
+            }
+        }
+        if (best < 0) return COMD_OK;  // too dense for staged halos
+        shape(best);
+    }
     // replicas in group order: the EAM ones, then the LJ ones
     std::vector<int> gorder;
     for (int pass = 0; pass < 2; pass++)

@@ -169,12 +169,16 @@ __global__ __launch_bounds__(64) void k_rebox_scan(DevState d)
         int len = n0 + nlow;
         if (len > 127) len = 127;
         for (int k = 0; k < len; k++) perm[k] = (uint8_t)(k < n0 ? k : CM_SLOTS + (k - n0));
+        // which of the cell's own atoms leave, read up front (independent
+        // loads) rather than one dependent load per step of the scan
+        unsigned long long leaves = 0;
+        for (int k = 0; k < n0 && k < CM_SLOTS; k++)
+            if (d.dest[c * CM_SLOTS + k] >= 0) leaves |= 1ull << k;
         int p = 0, rk = 0;
         while (p < len) {
             const int e = perm[p];
-            const int o = c * CM_SLOTS + e;
-            if (e < CM_SLOTS && d.dest[o] >= 0) {
-                d.mv_rank[d.mvidx[o]] = rk++;
+            if (e < CM_SLOTS && ((leaves >> e) & 1ull)) {
+                d.mv_rank[d.mvidx[c * CM_SLOTS + e]] = rk++;
                 len--;
                 perm[p] = perm[len];
             } else {
