@@ -60,7 +60,8 @@ struct comd_ctx {
     double *fx, *fy, *fz, *fe, *rho, *fi, *vir;
     int *nnb;
     // rebin scratch
-    int *dest, *mvidx, *meta, *counts, *touched, *tc, *arr_cnt, *arr;
+    unsigned long long *leave;
+    int *mvidx, *meta, *counts, *touched, *tc, *arr_cnt, *arr;
     int *mv_slot, *mv_rank, *mv_id, *mv_itype;
     double *mv_rx, *mv_ry, *mv_rz, *mv_px, *mv_py, *mv_pz;
     uint8_t *perm;
@@ -148,7 +149,7 @@ static DevState dev_state(const comd_ctx *c)
     d.fx = c->fx; d.fy = c->fy; d.fz = c->fz; d.fe = c->fe;
     d.rho = c->rho; d.fi = c->fi; d.vir = c->vir;
     d.nnb = c->nnb;
-    d.dest = c->dest;
+    d.leave = c->leave;
     d.mvidx = c->mvidx;
     d.counts = c->counts;
     d.touched = c->touched;
@@ -170,8 +171,13 @@ static DevState dev_state(const comd_ctx *c)
     d.nl_ref = nl ? c->nl.ref : nullptr;
     d.nl_slot = nl ? c->nl.slot : nullptr;
     d.nl_flags = nl ? c->nl.flags : nullptr;
-    const double hs = 0.5e-3 * c->nl_skin_milli;
-    d.nl_half2 = hs * hs;
+    // the displacement test reads the build's positions in FP32: half the
+    // skin less their rounding (2^-23 of a coordinate, which stays within
+    // about one box length; three coordinates)
+    double lmax = 0.0;
+    for (const Replica &r : c->reps) lmax = fmax(lmax, fmax(r.g.L0, fmax(r.g.L1, r.g.L2)));
+    const double hs = 0.5e-3 * c->nl_skin_milli - 2.0 * 0x1p-23 * lmax;
+    d.nl_half2 = hs > 0.0 ? hs * hs : 0.0;
     d.cell_gate = nl ? c->nl.flags : nullptr;
     return d;
 }
@@ -439,7 +445,7 @@ static void free_all(comd_ctx *c)
     if (c->graph) hipGraphExecDestroy(c->graph);
     free_nl(c);
     void *ptrs[] = {c->natoms, c->id, c->itype, c->rx, c->ry, c->rz, c->px, c->py, c->pz,
-                    c->fx, c->fy, c->fz, c->fe, c->rho, c->fi, c->vir, c->nnb, c->dest, c->mvidx, c->meta,
+                    c->fx, c->fy, c->fz, c->fe, c->rho, c->fi, c->vir, c->nnb, c->leave, c->mvidx, c->meta,
                     c->counts, c->touched, c->tc, c->arr_cnt, c->arr, c->mv_slot, c->mv_rank, c->mv_id,
                     c->mv_itype, c->mv_rx, c->mv_ry, c->mv_rz, c->mv_px, c->mv_py, c->mv_pz,
                     c->perm, c->flags, c->plist, c->pcount, c->pvalid,
@@ -735,7 +741,7 @@ static int ctx_create(int nrep, const comd_params *prms, comd_ctx **out)
     alloc((void **)&c->fi, S * 8);
     alloc((void **)&c->vir, S * 8);
     alloc((void **)&c->nnb, S * 4);
-    alloc((void **)&c->dest, S * 4);
+    alloc((void **)&c->leave, C * 8);
     alloc((void **)&c->mvidx, S * 4);
     alloc((void **)&c->meta, C * 16);
     alloc((void **)&c->counts, 16);
@@ -970,9 +976,9 @@ static int nl_setup(comd_ctx *c)
             if (ok && hipMalloc(ptr, bytes < 16 ? 16 : bytes) != hipSuccess) ok = false;
         };
         const size_t n = (size_t)nids;
-        alloc((void **)&c->nl.pos, n * sizeof(double4));
+        alloc((void **)&c->nl.pos, n * sizeof(CmPos));
         alloc((void **)&c->nl.selfh, n * 2);
-        alloc((void **)&c->nl.ref, n * sizeof(double4));
+        alloc((void **)&c->nl.ref, n * sizeof(float4));
         alloc((void **)&c->nl.slot, n * 4);
         alloc((void **)&c->nl.order, n * 4);
         alloc((void **)&c->nl.fq, n * 8);
@@ -1438,8 +1444,8 @@ int comd_advance_velocity(comd_ctx *c, double dt_internal)
 {
     if (!c) return COMD_ERR_ARG;
     if (!c->have_atoms) return set_err(c, COMD_ERR_STATE, "no atoms");
-    // kick only: k_move with drift off leaves r untouched; its destination
-    // scratch is rewritten before the next rebin
+    // kick only: k_move with drift and rebin off leaves r and the rebin
+    // state untouched
     op_move(c, dt_internal, 0.0, 1, 0, 0);
     HIPCHK(c, hipGetLastError());
     return COMD_OK;

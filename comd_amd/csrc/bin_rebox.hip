@@ -64,8 +64,8 @@ __device__ __forceinline__ void move_atom(const DevState &d, const Grid &g, int 
             corner_of(g, ix, iy, iz, cx, cy, cz);
             const int id = g.nl_id_base + d.id[s];
             const double ax = cx + x, ay = cy + y, az = cz + z;
-            d.nl_pos[id] = make_double4(ax, ay, az, 0.0);
-            const double4 q = d.nl_ref[id];
+            d.nl_pos[id] = CmPos{ax, ay, az};
+            const float4 q = d.nl_ref[id];
             double ex = ax - q.x, ey = ay - q.y, ez = az - q.z;
             ex -= g.L0 * rint(ex * g.iL0);
             ey -= g.L1 * rint(ey * g.iL1);
@@ -84,9 +84,8 @@ __device__ __forceinline__ void move_atom(const DevState &d, const Grid &g, int 
     if (y < 0.0) { ny = iy - 1; ay = y + g.bs1; } else if (y >= g.bs1) { ny = iy + 1; ay = y - g.bs1; }
     if (z < 0.0) { nz = iz - 1; az = z + g.bs2; } else if (z >= g.bs2) { nz = iz + 1; az = z - g.bs2; }
     const int j = cell_wrap(g, nx, ny, nz);
-    const int dst = (j < 0 || j == c) ? -1 : j;
-    d.dest[s] = dst;
-    if (dst < 0) return;
+    if (j < 0 || j == c) return;
+    const int dst = j;
     if (dst > c) {
         // the arrival is examined again when dst is scanned
         int jx, jy, jz;
@@ -109,6 +108,7 @@ __device__ __forceinline__ void move_atom(const DevState &d, const Grid &g, int 
     d.mv_py[m] = py;
     d.mv_pz[m] = pz;
     d.mvidx[s] = m;
+    atomicOr(&d.leave[c], 1ull << (s - c * CM_SLOTS));
     const int k = atomicAdd(&d.arr_cnt[dst], 1);
     if (k < CM_ACAP) d.arr[dst * CM_ACAP + k] = m;
     mark_touched(d, c);
@@ -169,11 +169,8 @@ __global__ __launch_bounds__(64) void k_rebox_scan(DevState d)
         int len = n0 + nlow;
         if (len > 127) len = 127;
         for (int k = 0; k < len; k++) perm[k] = (uint8_t)(k < n0 ? k : CM_SLOTS + (k - n0));
-        // which of the cell's own atoms leave, read up front (independent
-        // loads) rather than one dependent load per step of the scan
-        unsigned long long leaves = 0;
-        for (int k = 0; k < n0 && k < CM_SLOTS; k++)
-            if (d.dest[c * CM_SLOTS + k] >= 0) leaves |= 1ull << k;
+        // which of the cell's own atoms leave (set by k_move per departure)
+        const unsigned long long leaves = d.leave[c];
         int p = 0, rk = 0;
         while (p < len) {
             const int e = perm[p];
@@ -283,6 +280,7 @@ __global__ __launch_bounds__(64) void k_rebox_gather(DevState d)
         if (L == 0) {
             d.arr_cnt[c] = 0;
             d.touched[c] = 0;
+            d.leave[c] = 0;
         }
         __syncthreads();
     }

@@ -30,6 +30,11 @@ struct Grid {
 };
 
 // Plain view of the device arrays passed to kernels by value.
+// Absolute position of an atom for the neighbour lists (angstrom).
+struct CmPos {
+    double x, y, z;
+};
+
 struct DevState {
     int ncells;              // cells of all replicas
     int ngrids;              // replicas
@@ -46,7 +51,7 @@ struct DevState {
     double *fx, *fy, *fz, *fe, *rho, *fi, *vir;
     int *nnb;           // per slot: neighbours inside the cutoff (last density/LJ pass)
     // rebin scratch
-    int *dest;          // per slot: destination cell, or -1 to stay
+    unsigned long long *leave;  // per cell: bit k set when slot k departs (cleared with touched)
     int *mvidx;         // per slot: index of its mover record (departing atoms)
     uint8_t *perm;      // per cell: arrangement after the cell's own scan
     int *meta;          // per cell: (len, nlow, nhigh, n0)
@@ -69,13 +74,13 @@ struct DevState {
     int *pvalid;
     // neighbour-list bookkeeping kept by the drift and rebin kernels (null
     // when lists are off): per atom id its absolute position, its position
-    // at the last list build and its slot; a flag word whose bit
+    // at the last list build (FP32) and its slot; a flag word whose bit
     // CM_NL_REBUILD is raised when an atom has moved more than half the skin
-    double4 *nl_pos;
-    const double4 *nl_ref;
+    CmPos *nl_pos;
+    const float4 *nl_ref;
     int *nl_slot;
     unsigned *nl_flags;
-    double nl_half2;     // (skin / 2)^2
+    double nl_half2;     // (skin / 2 - FP32 rounding of the reference positions)^2
     // when non-null, the link-cell force kernels run only if this word has
     // CM_NL_INVALID set (the lists could not be built)
     const unsigned *cell_gate;

@@ -38,6 +38,13 @@ __device__ __forceinline__ bool nl_rebuild(const CmList &l) { return (l.flags[0]
 // x wrapped into [0, L) (to rounding); iL = 1 / L
 __device__ __forceinline__ double wrap_coord(double x, double L, double iL) { return x - L * floor(x * iL); }
 
+// List position of atom id, read through the global address space.
+__device__ __forceinline__ CmPos load_pos(const CmPos *pos, int id)
+{
+    const CM_GLOBAL double *q = (const CM_GLOBAL double *)pos + 3 * id;
+    return CmPos{q[0], q[1], q[2]};
+}
+
 // Replica r's list geometry.
 __device__ __forceinline__ NlRep nl_rep(const CmList &l, int r) { return l.rep[r]; }
 
@@ -46,7 +53,7 @@ __device__ __forceinline__ int cell_rep(const DevState &d, int c) { return d.ngr
 
 // Auxiliary cell of a position within its replica (the same arithmetic
 // wherever it is needed, so that the key and build kernels agree).
-__device__ __forceinline__ void aux_coords(const NlRep &R, const double4 &p, int &ax, int &ay, int &az)
+__device__ __forceinline__ void aux_coords(const NlRep &R, const CmPos &p, int &ax, int &ay, int &az)
 {
     ax = min(max((int)(wrap_coord(p.x, R.L0, R.iL0) * R.iab0), 0), R.na0 - 1);
     ay = min(max((int)(wrap_coord(p.y, R.L1, R.iL1) * R.iab1), 0), R.na1 - 1);
@@ -90,9 +97,9 @@ __global__ __launch_bounds__(64) void k_nl_from_slots(DevState d, CmList l, int 
             cell_corner(grid_of(d, c), c, cx, cy, cz);
             const double x = cx + d.rx[s], y = cy + d.ry[s], z = cz + d.rz[s];
             l.slot[id] = s;
-            l.pos[id] = make_double4(x, y, z, 0.0);
+            l.pos[id] = CmPos{x, y, z};
             if (check) {
-                const double4 q = l.ref[id];
+                const float4 q = l.ref[id];
                 double ex = x - q.x, ey = y - q.y, ez = z - q.z;
                 ex -= R.L0 * rint(ex * R.iL0);
                 ey -= R.L1 * rint(ey * R.iL1);
@@ -199,7 +206,6 @@ __device__ __forceinline__ void group_setup(const CmList &l, const NlRep &R, con
     }
 }
 
-typedef double v4d __attribute__((ext_vector_type(4)));
 typedef unsigned v4u __attribute__((ext_vector_type(4)));
 typedef unsigned v3u __attribute__((ext_vector_type(3)));
 
@@ -208,7 +214,7 @@ typedef unsigned v3u __attribute__((ext_vector_type(3)));
 // relative to the frame origin o.  Atoms that crossed the periodic boundary
 // since the build, and cells that appear twice in a small box's halo, are
 // placed right.
-__device__ __forceinline__ void frame_pos(const NlRep &R, const GroupGeo &G, int uxyz, const v4d &p, double ox,
+__device__ __forceinline__ void frame_pos(const NlRep &R, const GroupGeo &G, int uxyz, const CmPos &p, double ox,
                                           double oy, double oz, double &x, double &y, double &z)
 {
     double sx, sy, sz;
@@ -397,9 +403,9 @@ __global__ __launch_bounds__(1024) void k_nl_build(CmList l)
             }
 #pragma unroll
             for (int k = 0; k < SB; k++) id[k] = l.order[q[k]];
-            v4d p[SB];
+            CmPos p[SB];
 #pragma unroll
-            for (int k = 0; k < SB; k++) p[k] = ((const CM_GLOBAL v4d *)l.pos)[id[k]];
+            for (int k = 0; k < SB; k++) p[k] = load_pos(l.pos, id[k]);
 #pragma unroll
             for (int k = 0; k < SB; k++) {
                 const int h = h0 + k * (int)blockDim.x;
@@ -413,7 +419,7 @@ __global__ __launch_bounds__(1024) void k_nl_build(CmList l)
         const int shx = G.lx + 2, shy = G.ly + 2;
         for (int t = S.t0 + threadIdx.x; t < S.t1; t += blockDim.x) {
             const int id = l.order[t];
-            const double4 p = l.pos[id];
+            const CmPos p = l.pos[id];
             int ax, ay, az;
             aux_coords(R, p, ax, ay, az);
             // home slot: the atom's own cell, an interior slot of the halo
@@ -456,7 +462,7 @@ __global__ __launch_bounds__(1024) void k_nl_build(CmList l)
             }
             l.ncnt[t] = min(k, l.kmax);
             ovf |= k > l.kmax;
-            l.ref[id] = p;
+            l.ref[id] = make_float4((float)p.x, (float)p.y, (float)p.z, 0.0f);
         }
     }
     if (ovf) atomicOr(l.flags, (unsigned)CM_NL_OVERFLOW);
@@ -736,11 +742,11 @@ void k_nl_pair(DevState d, PotDev P, CmList l, int tab_bytes, int gbase)
             }
 #pragma unroll
             for (int k = 0; k < SB; k++) id[k] = l.order[q[k]];
-            v4d p[SB];
+            CmPos p[SB];
             double fp[SB];
 #pragma unroll
             for (int k = 0; k < SB; k++) {
-                p[k] = ((const CM_GLOBAL v4d *)l.pos)[id[k]];
+                p[k] = load_pos(l.pos, id[k]);
                 fp[k] = KIND == CM_KIND_EAM3 ? ((const CM_GLOBAL double *)l.fq)[q[k]] : 0.0;
             }
 #pragma unroll
