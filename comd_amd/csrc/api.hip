@@ -905,34 +905,59 @@ static int nl_setup(comd_ctx *c)
     std::vector<std::array<int, 3>> gd(nrep), na(nrep), ng(nrep);
     long hcap = 0, naux = 0, ngroups = 0, threads = 64;
     // o >= 0: that option for every replica; o < 0 (batches): per replica
-    // the largest option whose groups' atoms (10 % spare) fit 768 threads,
-    // so that replicas of different density and cell rounding get groups of
-    // similar size (one block shape serves them all)
+    // the largest option whose groups' atoms (10 % spare) fit 768 threads
+    // and whose staged halo keeps two blocks per CU in the mixed passes, so
+    // that replicas of different density and cell rounding get groups of
+    // similar size (one block shape and one halo capacity serve them all).
+    // Per option the grid rounded down to whole groups is tried first; then
+    // the grid of the largest cells >= cutoff + skin, split into groups
+    // whose extents differ by at most one (rounding down a small replica's
+    // grid can make its cells, and so its halos, much larger)
+    long htarget = 1L << 30;
+    if (c->precision == COMD_PREC_MIXED) {
+        long tab = 0;
+        for (const Replica &R : c->reps) tab = std::max<long>(tab, (long)cm_nl_tab_bytes(R.pot, CM_KIND_EAM1, 1));
+        htarget = ((long)c->lds_block / 2 - CM_NL_STATIC_LDS - tab) / (hb + 1) / 64 * 64;
+    }
+    auto hcap_of = [](double occ, long slots) { return (((long)(1.2 * occ * (double)slots) + 64 + 63) / 64) * 64; };
     auto shape = [&](int o) {  // fills gd, na, ng, hcap, naux, ngroups, threads
         hcap = naux = ngroups = 0;
         threads = 64;
         for (int r = 0; r < nrep; r++) {
             long nr = 1;
-            for (int oo = o >= 0 ? o : 0; oo < 5; oo++) {
-                nr = 1;
-                for (int j = 0; j < 3; j++) {
-                    gd[r][j] = std::min(gopts[oo][j], nmax[r][j]);
-                    na[r][j] = nmax[r][j] / gd[r][j] * gd[r][j];
-                    if (na[r][j] < 3) {  // keep >= 3 cells (one group across the dimension)
-                        gd[r][j] = nmax[r][j];
-                        na[r][j] = nmax[r][j];
+            bool fits = false;
+            for (int oo = o >= 0 ? o : 0; oo < 5 && !fits; oo++) {
+                for (int bal = 0; bal < (o >= 0 ? 1 : 2) && !fits; bal++) {
+                    nr = 1;
+                    for (int j = 0; j < 3; j++) {
+                        gd[r][j] = std::min(gopts[oo][j], nmax[r][j]);
+                        if (bal) {
+                            na[r][j] = nmax[r][j];
+                            ng[r][j] = (na[r][j] + gd[r][j] - 1) / gd[r][j];
+                            gd[r][j] = (na[r][j] + ng[r][j] - 1) / ng[r][j];
+                        } else {
+                            na[r][j] = nmax[r][j] / gd[r][j] * gd[r][j];
+                            if (na[r][j] < 3) {  // keep >= 3 cells (one group across the dimension)
+                                gd[r][j] = nmax[r][j];
+                                na[r][j] = nmax[r][j];
+                            }
+                            ng[r][j] = na[r][j] / gd[r][j];
+                        }
+                        nr *= na[r][j];
                     }
-                    ng[r][j] = na[r][j] / gd[r][j];
-                    nr *= na[r][j];
+                    if (o >= 0) {
+                        fits = true;
+                        break;
+                    }
+                    const double occ = (double)c->reps[r].ntot / (double)nr;
+                    const long slots = (long)(gd[r][0] + 2) * (gd[r][1] + 2) * (gd[r][2] + 2);
+                    fits = 1.1 * occ * gd[r][0] * gd[r][1] * gd[r][2] <= 768.0 && hcap_of(occ, slots) <= htarget;
                 }
-                if (o >= 0) break;
-                const double apg = (double)c->reps[r].ntot / (double)nr * gd[r][0] * gd[r][1] * gd[r][2];
-                if (1.1 * apg <= 768.0) break;
             }
             const double occ = (double)c->reps[r].ntot / (double)nr;
             const long cells = (long)gd[r][0] * gd[r][1] * gd[r][2];
             const long slots = (long)(gd[r][0] + 2) * (gd[r][1] + 2) * (gd[r][2] + 2);
-            hcap = std::max<long>(hcap, (((long)(1.2 * occ * (double)slots) + 64 + 63) / 64) * 64);
+            hcap = std::max<long>(hcap, hcap_of(occ, slots));
             threads = std::max<long>(threads, ((long)(1.1 * occ * (double)cells) + 63) / 64 * 64);
             naux += nr;
             ngroups += (long)ng[r][0] * ng[r][1] * ng[r][2];
@@ -1076,9 +1101,13 @@ static int nl_setup(comd_ctx *c)
                 for (int gx = 0; gx < R.ng0; gx++) {
                     gq0[g] = (int)q;
                     grep[g] = r;
-                    for (int z = gz * R.gd2; z < (gz + 1) * R.gd2; z++)
-                        for (int y = gy * R.gd1; y < (gy + 1) * R.gd1; y++)
-                            for (int x = gx * R.gd0; x < (gx + 1) * R.gd0; x++) {
+                    int x0, lx, y0, ly, z0, lz;
+                    group_span(gx, R.na0, R.ng0, x0, lx);
+                    group_span(gy, R.na1, R.ng1, y0, ly);
+                    group_span(gz, R.na2, R.ng2, z0, lz);
+                    for (int z = z0; z < z0 + lz; z++)
+                        for (int y = y0; y < y0 + ly; y++)
+                            for (int x = x0; x < x0 + lx; x++) {
                                 const long a = abase + x + R.na0 * (y + (long)R.na1 * z);
                                 corder[q++] = (int)a;
                                 agroup[a] = (int)g;

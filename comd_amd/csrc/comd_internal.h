@@ -148,6 +148,15 @@ void cm_launch_adj_resid(const DevState &d, const CmAdjoint &J, const double *ft
 void cm_launch_adj_stage(const DevState &d, const PotDev &P, const CmAdjoint &J, int stage, int mixed,
                          hipStream_t st);
 
+// Group gi of ng along a dimension of na auxiliary cells: its first cell x0
+// and extent lx (extents differ by at most one; na / ng each when ng divides
+// na).
+__host__ __device__ inline void group_span(int gi, int na, int ng, int &x0, int &lx)
+{
+    x0 = gi * na / ng;
+    lx = (gi + 1) * na / ng - x0;
+}
+
 // One replica's auxiliary grid (cells of edge >= cutoff + skin) and its
 // groups of auxiliary cells, for the neighbour lists.
 struct NlRep {
@@ -156,7 +165,7 @@ struct NlRep {
     double qscale, qinv;                     // group-frame fixed point of the mixed passes: units per A, A per unit
     double rl2;                              // (cutoff + skin)^2
     int na0, na1, na2;                       // auxiliary cells per dimension
-    int gd0, gd1, gd2;                       // cells per group
+    int gd0, gd1, gd2;                       // cells per group (the largest extent)
     int ng0, ng1, ng2;                       // groups per dimension
     int aux_base, group_base;                // its first auxiliary cell and group in the global numbering
     int id_base, nid;                        // its atom ids i map to list ids id_base + i, 0 <= i < nid
@@ -212,6 +221,7 @@ void cm_nl_build(const DevState &d, const CmList &l, hipStream_t st);
 void cm_nl_sync(const DevState &d, const CmList &l, hipStream_t st);
 size_t cm_nl_build_lds(const CmList &l);
 size_t cm_nl_pass_lds(const PotDev &P, const CmList &l, int kind, int mixed, int *tables_in_lds);
+size_t cm_nl_tab_bytes(const PotDev &P, int kind, int mixed);
 // The list pass of one kind over groups [g0, g0 + ng) (a single
 // simulation: all of them; a batch: the groups of its EAM or of its LJ
 // replicas, which share one potential).

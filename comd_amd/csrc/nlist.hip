@@ -124,12 +124,9 @@ __device__ __forceinline__ GroupGeo group_geo(const NlRep &R, int gl)
 {
     GroupGeo G;
     const int gx = gl % R.ng0, gy = (gl / R.ng0) % R.ng1, gz = gl / (R.ng0 * R.ng1);
-    G.x0 = gx * R.gd0;
-    G.y0 = gy * R.gd1;
-    G.z0 = gz * R.gd2;
-    G.lx = min(R.gd0, R.na0 - G.x0);
-    G.ly = min(R.gd1, R.na1 - G.y0);
-    G.lz = min(R.gd2, R.na2 - G.z0);
+    group_span(gx, R.na0, R.ng0, G.x0, G.lx);
+    group_span(gy, R.na1, R.ng1, G.y0, G.ly);
+    group_span(gz, R.na2, R.ng2, G.z0, G.lz);
     return G;
 }
 
@@ -807,6 +804,11 @@ void cm_nl_build(const DevState &d, const CmList &l, hipStream_t st)
     hipLaunchKernelGGL(k_nl_gsum, dim3((unsigned)std::min(4096, (l.ngroups + 3) / 4)), dim3(256), 0, st, l);
     hipLaunchKernelGGL(k_nl_sort, dim3(1024), dim3(256), 0, st, l);
     hipLaunchKernelGGL(k_nl_build, dim3(l.pass_blocks), dim3(l.pass_threads), cm_nl_build_lds(l), st, l);
+}
+
+size_t cm_nl_tab_bytes(const PotDev &P, int kind, int mixed)
+{
+    return P.eval == COMD_EVAL_TABLE && kind != CM_KIND_LJ && P.cd64 ? nl_tab_bytes(P, kind, mixed) : 0;
 }
 
 size_t cm_nl_pass_lds(const PotDev &P, const CmList &l, int kind, int mixed, int *tables_in_lds)
