@@ -900,19 +900,24 @@ static int nl_setup(comd_ctx *c)
     // per dimension (cells a little larger than cutoff + skin): a partial
     // group at an edge would take a whole block for a sliver of atoms.
     const int gopts[5][3] = {{4, 4, 3}, {4, 4, 2}, {4, 2, 2}, {2, 2, 2}, {2, 2, 1}};
+    // the shapes of the per-replica search of batches (the halo of each,
+    // (x + 2)(y + 2)(z + 2) slots, within CM_NL_SLOTS)
+    const int bopts[10][3] = {{4, 4, 3}, {4, 3, 3}, {4, 4, 2}, {3, 3, 3}, {4, 3, 2},
+                              {3, 3, 2}, {4, 2, 2}, {3, 2, 2}, {2, 2, 2}, {2, 2, 1}};
     const long lds_avail = (long)c->lds_block - CM_NL_STATIC_LDS;
     const long hb = c->precision == COMD_PREC_MIXED ? 16 : 32;
     std::vector<std::array<int, 3>> gd(nrep), na(nrep), ng(nrep);
     long hcap = 0, naux = 0, ngroups = 0, threads = 64;
     // o >= 0: that option for every replica; o < 0 (batches): per replica
-    // the largest option whose groups' atoms (10 % spare) fit 768 threads
-    // and whose staged halo keeps two blocks per CU in the mixed passes, so
-    // that replicas of different density and cell rounding get groups of
-    // similar size (one block shape and one halo capacity serve them all).
-    // Per option the grid rounded down to whole groups is tried first; then
-    // the grid of the largest cells >= cutoff + skin, split into groups
-    // whose extents differ by at most one (rounding down a small replica's
-    // grid can make its cells, and so its halos, much larger)
+    // the shape with the fewest groups (then the fewest staged halo atoms)
+    // among those whose groups' atoms (10 % spare) fit 768 threads and whose
+    // staged halo keeps two blocks per CU in the mixed passes, so that
+    // replicas of different density and cell rounding get groups of similar
+    // size (one block shape and one halo capacity serve them all).  Each
+    // option is tried on the grid rounded down to whole groups and on the
+    // grid of the smallest cells >= cutoff + skin split into groups whose
+    // extents differ by at most one (rounding a small replica's grid down
+    // can make its cells, and so its halos, much larger)
     long htarget = 1L << 30;
     if (c->precision == COMD_PREC_MIXED) {
         long tab = 0;
@@ -920,39 +925,51 @@ static int nl_setup(comd_ctx *c)
         htarget = ((long)c->lds_block / 2 - CM_NL_STATIC_LDS - tab) / (hb + 1) / 64 * 64;
     }
     auto hcap_of = [](double occ, long slots) { return (((long)(1.2 * occ * (double)slots) + 64 + 63) / 64) * 64; };
+    // replica r's grid and groups for option (opts, oo), rounded or balanced
+    auto layout = [&](int r, const int (*opts)[3], int oo, int bal) {
+        long nr = 1;
+        for (int j = 0; j < 3; j++) {
+            gd[r][j] = std::min(opts[oo][j], nmax[r][j]);
+            if (bal) {
+                na[r][j] = nmax[r][j];
+                ng[r][j] = (na[r][j] + gd[r][j] - 1) / gd[r][j];
+                gd[r][j] = (na[r][j] + ng[r][j] - 1) / ng[r][j];
+            } else {
+                na[r][j] = nmax[r][j] / gd[r][j] * gd[r][j];
+                if (na[r][j] < 3) {  // keep >= 3 cells (one group across the dimension)
+                    gd[r][j] = nmax[r][j];
+                    na[r][j] = nmax[r][j];
+                }
+                ng[r][j] = na[r][j] / gd[r][j];
+            }
+            nr *= na[r][j];
+        }
+        return nr;
+    };
     auto shape = [&](int o) {  // fills gd, na, ng, hcap, naux, ngroups, threads
         hcap = naux = ngroups = 0;
         threads = 64;
         for (int r = 0; r < nrep; r++) {
-            long nr = 1;
-            bool fits = false;
-            for (int oo = o >= 0 ? o : 0; oo < 5 && !fits; oo++) {
-                for (int bal = 0; bal < (o >= 0 ? 1 : 2) && !fits; bal++) {
-                    nr = 1;
-                    for (int j = 0; j < 3; j++) {
-                        gd[r][j] = std::min(gopts[oo][j], nmax[r][j]);
-                        if (bal) {
-                            na[r][j] = nmax[r][j];
-                            ng[r][j] = (na[r][j] + gd[r][j] - 1) / gd[r][j];
-                            gd[r][j] = (na[r][j] + ng[r][j] - 1) / ng[r][j];
-                        } else {
-                            na[r][j] = nmax[r][j] / gd[r][j] * gd[r][j];
-                            if (na[r][j] < 3) {  // keep >= 3 cells (one group across the dimension)
-                                gd[r][j] = nmax[r][j];
-                                na[r][j] = nmax[r][j];
-                            }
-                            ng[r][j] = na[r][j] / gd[r][j];
-                        }
-                        nr *= na[r][j];
+            long nr;
+            if (o >= 0) {
+                nr = layout(r, gopts, o, 0);
+            } else {
+                int bo = 9, bb = 1;  // none fits: the smallest shape
+                long bg = -1;
+                double bh = 0.0;
+                for (int oo = 0; oo < 10; oo++)
+                    for (int bal = 0; bal < 2; bal++) {
+                        const long n = layout(r, bopts, oo, bal);
+                        const double occ = (double)c->reps[r].ntot / (double)n;
+                        const long slots = (long)(gd[r][0] + 2) * (gd[r][1] + 2) * (gd[r][2] + 2);
+                        if (1.1 * occ * gd[r][0] * gd[r][1] * gd[r][2] > 768.0 || hcap_of(occ, slots) > htarget)
+                            continue;
+                        const long g = (long)ng[r][0] * ng[r][1] * ng[r][2];
+                        double h = occ;  // staged halo atoms: prod_j (na_j + 2 ng_j) cells
+                        for (int j = 0; j < 3; j++) h *= (double)(na[r][j] + 2 * ng[r][j]);
+                        if (bg < 0 || g < bg || (g == bg && h < bh)) bo = oo, bb = bal, bg = g, bh = h;
                     }
-                    if (o >= 0) {
-                        fits = true;
-                        break;
-                    }
-                    const double occ = (double)c->reps[r].ntot / (double)nr;
-                    const long slots = (long)(gd[r][0] + 2) * (gd[r][1] + 2) * (gd[r][2] + 2);
-                    fits = 1.1 * occ * gd[r][0] * gd[r][1] * gd[r][2] <= 768.0 && hcap_of(occ, slots) <= htarget;
-                }
+                nr = layout(r, bopts, bo, bb);
             }
             const double occ = (double)c->reps[r].ntot / (double)nr;
             const long cells = (long)gd[r][0] * gd[r][1] * gd[r][2];

@@ -489,16 +489,27 @@ __global__ __launch_bounds__(1024) void k_nl_build(CmList l)
 
 // Non-empty stencil cells of each link cell: the reference adds the kinetic
 // virial term once per non-empty neighbour cell (src-flat/ljForce.c:175).
-__global__ void k_nl_nonempty(DevState d, CmList l)
+// Cells of the replicas whose groups start at g0 or later (the
+// Lennard-Jones ones); the 27 entries of stencil_cell, duplicates included.
+__global__ void k_nl_nonempty(DevState d, CmList l, int g0)
 {
     if (l.flags[0] & CM_NL_INVALID) return;
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= d.ncells) return;
+    if (g0 > 0 && l.rep[cell_rep(d, c)].group_base < g0) return;
     const Grid g = grid_of(d, c);
     int n = 0;
-    for (int e = 0; e < 27; e++) {
-        const int b = stencil_cell(g, c, e);
-        n += b >= 0 && d.natoms[b] > 0;
+    if (g.ncells == 1) {
+        n = d.natoms[c] > 0;
+    } else {
+        int ix, iy, iz;
+        cell_xyz(g, c, ix, iy, iz);
+        for (int ox = -1; ox <= 1; ox++)
+            for (int oy = -1; oy <= 1; oy++)
+                for (int oz = -1; oz <= 1; oz++) {
+                    const int b = cell_wrap(g, ix + ox, iy + oy, iz + oz);
+                    n += b >= 0 && d.natoms[b] > 0;
+                }
     }
     l.nonempty[c] = n;
 }
@@ -881,7 +892,7 @@ void cm_nl_force(const DevState &d, const PotDev &P, const CmList &l, int kind, 
 {
     if (ng <= 0) return;
     if (kind == CM_KIND_LJ) {
-        hipLaunchKernelGGL(k_nl_nonempty, dim3(grid_for(d.ncells, 256)), dim3(256), 0, st, d, l);
+        hipLaunchKernelGGL(k_nl_nonempty, dim3(grid_for(d.ncells, 256)), dim3(256), 0, st, d, l, g0);
         nl_launch2<CM_KIND_LJ, COMD_EVAL_TABLE>(d, P, l, mixed, st, g0, ng);
     } else if (P.eval == COMD_EVAL_CHEB) {
         if (kind == CM_KIND_EAM1) nl_launch2<CM_KIND_EAM1, COMD_EVAL_CHEB>(d, P, l, mixed, st, g0, ng);
