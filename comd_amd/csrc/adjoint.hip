@@ -27,8 +27,9 @@
 //            dL/dc_F   = D^T M_F + dL/dE M_FE.
 //
 // Every pass is full-shell over the 27-cell stencil with the halo staged in
-// LDS, one wave per cell on a persistent grid; each lane keeps its moments
-// in registers and the waves' partials are summed in a fixed order, so the
+// LDS, one wave per cell on a persistent grid; the pairs a wave finds are
+// queued and evaluated 64 at a time; each lane keeps its moments in
+// registers and the waves' partials are summed in a fixed order, so the
 // gradient is deterministic.  MIXED evaluates the Chebyshev basis and rho'
 // in single precision and accumulates in double.
 #include "comd_internal.h"
@@ -119,6 +120,18 @@ __device__ __forceinline__ void write_moments(const double *a, const double *b, 
 
 // ---------------------------------------------------------------------------
 // pair passes 1-3
+//
+// One wave per cell: the wave tests 64 candidates of one home atom at a time
+// (home atoms in order) and queues the pairs inside the cutoff in LDS, in
+// that order; every 64 queued pairs are evaluated one per lane.  The basis
+// sums, the costly part, so run on full waves (tested lane by lane, about
+// one candidate in eight is a pair, and a wave would run them for whichever
+// few lanes found one).  Pass 2's lambda_i is summed per batch by a
+// segmented scan over the lanes (the queue is in home-atom order, and is
+// emptied after each staged halo batch so that it stays so), so every sum
+// keeps a fixed order.
+
+#define AQ 128  // queued pairs: a batch of 64 and up to 63 more
 
 template <int PASS, int MIXED>
 __global__ __launch_bounds__(64) void k_adj_pair(DevState d, PotDev P, AdjDev A)
@@ -127,8 +140,18 @@ __global__ __launch_bounds__(64) void k_adj_pair(DevState d, PotDev P, AdjDev A)
     __shared__ double hgx[PASS == 3 ? 1 : AHMAX], hgy[PASS == 3 ? 1 : AHMAX], hgz[PASS == 3 ? 1 : AHMAX];
     __shared__ double hf[PASS == 2 ? AHMAX : 1];
     __shared__ double drb[81];
-    __shared__ double red[64];
     __shared__ int cell[27], off[28];
+    // the cell's home atoms: position, dL/df (passes 1, 2), F' (pass 2) or
+    // mu (pass 3), and lambda (pass 2)
+    __shared__ double ox[CM_SLOTS], oy[CM_SLOTS], oz[CM_SLOTS];
+    __shared__ double ogx[PASS == 3 ? 1 : CM_SLOTS], ogy[PASS == 3 ? 1 : CM_SLOTS], ogz[PASS == 3 ? 1 : CM_SLOTS];
+    __shared__ double ow[PASS == 1 ? 1 : CM_SLOTS];
+    __shared__ double lamacc[PASS == 2 ? CM_SLOTS : 1];
+    // queued pairs: r^2; (g_i - g_j) . (r_i - r_j) (passes 1, 2) or mu_i
+    // (pass 3); F'_i + F'_j and the home atom (pass 2)
+    __shared__ double qr2[AQ], qv[AQ];
+    __shared__ double qf[PASS == 2 ? AQ : 1];
+    __shared__ int qh[PASS == 2 ? AQ : 1];
 
     const int L = threadIdx.x;
     const int m = P.cn;
@@ -142,6 +165,43 @@ __global__ __launch_bounds__(64) void k_adj_pair(DevState d, PotDev P, AdjDev A)
     double m0[AMAX], m1[AMAX];
 #pragma unroll
     for (int k = 0; k < AMAX; k++) m0[k] = m1[k] = 0.0;
+
+    // evaluates queue entries [0, n), n <= 64, one per lane
+    auto flush = [&](int n, double de) {
+        __syncthreads();
+        double v = 0.0;
+        int key = 1 << 30;
+        if (L < n) {
+            const double r = sqrt(qr2[L]);
+            const double y = cheb_arg(r, ca, cb);
+            if (PASS == 3) {
+                add_basis<MIXED, 0>(y, m, qv[L], 0.0, m0, m1);
+            } else {
+                const double w = qv[L] / r;
+                if (PASS == 1) {
+                    add_basis<MIXED, 1>(y, m, 0.5 * w, 0.5 * de, m0, m1);
+                } else {
+                    add_basis<MIXED, 0>(y, m, 0.5 * qf[L] * w, 0.0, m0, m1);
+                    const double drho = MIXED ? (double)cheb32(dser32, m8, (float)ca, (float)cb, (float)r)
+                                              : cheb64(dser, m8, ca, cb, r);
+                    v = drho * w;
+                    key = qh[L];
+                }
+            }
+        }
+        if (PASS == 2) {
+            // lambda_a += the batch's rho'(r) w of home atom a
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const double vu = __shfl_up(v, o, 64);
+                const int ku = __shfl_up(key, o, 64);
+                if (L >= o && ku == key) v += vu;
+            }
+            const int kn = __shfl_down(key, 1, 64);
+            if (L < n && (L == 63 || kn != key)) lamacc[key] += v;
+        }
+        __syncthreads();
+    };
 
     for (int c = blockIdx.x; c < d.ncells; c += gridDim.x) {
         const int nh = d.natoms[c];
@@ -170,19 +230,23 @@ __global__ __launch_bounds__(64) void k_adj_pair(DevState d, PotDev P, AdjDev A)
         const int H = __shfl(incl, 31, 64);
         if (L < nent) off[L] = incl - ne;
         if (L == 0) off[nent] = H;
-
-        const int S = 64 / nh;
-        const bool act = L < S * nh;
-        const int a = act ? L % nh : 0;
-        const int sg = act ? L / nh : 0;
-        const int hs = c * CM_SLOTS + a;
-        const double xa = d.rx[hs], ya = d.ry[hs], za = d.rz[hs];
-        const double gax = PASS == 3 ? 0.0 : A.gx[hs];
-        const double gay = PASS == 3 ? 0.0 : A.gy[hs];
-        const double gaz = PASS == 3 ? 0.0 : A.gz[hs];
-        const double fia = PASS == 2 ? d.fi[hs] : 0.0;
-        const double mua = PASS == 3 ? A.mu[hs] : 0.0;
-        double lam = 0.0;
+        if (L < nh) {
+            const int hs = c * CM_SLOTS + L;
+            ox[L] = d.rx[hs];
+            oy[L] = d.ry[hs];
+            oz[L] = d.rz[hs];
+            if (PASS != 3) {
+                ogx[L] = A.gx[hs];
+                ogy[L] = A.gy[hs];
+                ogz[L] = A.gz[hs];
+            }
+            if (PASS == 2) {
+                ow[L] = d.fi[hs];
+                lamacc[L] = 0.0;
+            }
+            if (PASS == 3) ow[L] = A.mu[hs];
+        }
+        int qn = 0;
 
         for (int h0 = 0; h0 < H; h0 += AHMAX) {
             const int hb = H - h0 < AHMAX ? H - h0 : AHMAX;
@@ -204,41 +268,58 @@ __global__ __launch_bounds__(64) void k_adj_pair(DevState d, PotDev P, AdjDev A)
                 }
             }
             __syncthreads();
-            if (act) {
-                for (int j = sg; j < hb; j += S) {
-                    const double dx = xa - hx[j], dy = ya - hy[j], dz = za - hz[j];
-                    const double r2 = dx * dx + dy * dy + dz * dz;
-                    // pair predicates of src-flat/eam.c:519 (phi, rho) and 633 (rho')
-                    const bool in = PASS == 2 ? (r2 < rc2) : (r2 <= rc2);
-                    if (!in || !(r2 > 0.0)) continue;
-                    const double r = sqrt(r2);
-                    const double y = cheb_arg(r, ca, cb);
-                    if (PASS == 3) {
-                        add_basis<MIXED, 0>(y, m, mua, 0.0, m0, m1);
-                    } else {
-                        const double w = ((gax - hgx[j]) * dx + (gay - hgy[j]) * dy + (gaz - hgz[j]) * dz) / r;
-                        if (PASS == 1) {
-                            add_basis<MIXED, 1>(y, m, 0.5 * w, 0.5 * de, m0, m1);
+            for (int a = 0; a < nh; a++) {
+                const double xa = ox[a], ya = oy[a], za = oz[a];
+                for (int j0 = 0; j0 < hb; j0 += 64) {
+                    const int j = j0 + L;
+                    double dx = 0.0, dy = 0.0, dz = 0.0, r2 = 0.0;
+                    bool acc = false;
+                    if (j < hb) {
+                        dx = xa - hx[j];
+                        dy = ya - hy[j];
+                        dz = za - hz[j];
+                        r2 = dx * dx + dy * dy + dz * dz;
+                        // pair predicates of src-flat/eam.c:519 (phi, rho) and 633 (rho')
+                        const bool in = PASS == 2 ? (r2 < rc2) : (r2 <= rc2);
+                        acc = in && r2 > 0.0;
+                    }
+                    const unsigned long long mk = __ballot(acc);
+                    if (acc) {
+                        const int p = qn + __popcll(mk & ((1ull << L) - 1ull));
+                        qr2[p] = r2;
+                        if (PASS == 3) {
+                            qv[p] = ow[a];
                         } else {
-                            add_basis<MIXED, 0>(y, m, 0.5 * (fia + hf[j]) * w, 0.0, m0, m1);
-                            const double drho = MIXED ? (double)cheb32(dser32, m8, (float)ca, (float)cb, (float)r)
-                                                      : cheb64(dser, m8, ca, cb, r);
-                            lam += drho * w;
+                            qv[p] = (ogx[a] - hgx[j]) * dx + (ogy[a] - hgy[j]) * dy + (ogz[a] - hgz[j]) * dz;
+                            if (PASS == 2) {
+                                qf[p] = ow[a] + hf[j];
+                                qh[p] = a;
+                            }
                         }
+                    }
+                    qn += __popcll(mk);
+                    if (qn >= 64) {
+                        flush(64, de);
+                        // the rest (fewer than 64) to the front
+                        if (L < qn - 64) {
+                            qr2[L] = qr2[64 + L];
+                            qv[L] = qv[64 + L];
+                            if (PASS == 2) {
+                                qf[L] = qf[64 + L];
+                                qh[L] = qh[64 + L];
+                            }
+                        }
+                        qn -= 64;
                     }
                 }
             }
+            // at the end of each staged halo batch: a batch holds each home
+            // atom's pairs in one run (the scan of lambda relies on it)
+            if (qn > 0) flush(qn, de);
+            qn = 0;
         }
-        if (PASS == 2) {
-            // lambda of each home atom: its S partial sums in s order
-            red[L] = act ? lam : 0.0;
-            __syncthreads();
-            if (L < nh) {
-                double s = 0.0;
-                for (int q = 0; q < S; q++) s += red[L + q * nh];
-                A.lam[c * CM_SLOTS + L] = s;
-            }
-        }
+        __syncthreads();
+        if (PASS == 2 && L < nh) A.lam[c * CM_SLOTS + L] = lamacc[L];
         __syncthreads();
     }
     write_moments<PASS == 1>(m0, m1, A.part + (size_t)blockIdx.x * ANMOM);
