@@ -79,22 +79,25 @@ __global__ void k_nl_clear_ids(CmList l)
 
 // check = 0: after binning or an upload (the ids are checked);
 // check = 1: positions after steps that did not keep them, with the
-// displacement test of the drift kernel
+// displacement test of the drift kernel.  Four consecutive cells per wave,
+// 16 lanes each.
 __global__ __launch_bounds__(64) void k_nl_from_slots(DevState d, CmList l, int check)
 {
-    const int c = blockIdx.x;
-    const int t = threadIdx.x;
+    const int c = blockIdx.x * 4 + (threadIdx.x >> 4);
     bool far = false;
-    if (t < d.natoms[c]) {
-        const int s = c * CM_SLOTS + t;
+    if (c < d.ncells) {
+        const int n = d.natoms[c];
         const NlRep R = nl_rep(l, cell_rep(d, c));
-        const int i = d.id[s];
-        if (i < 0 || i >= R.nid) {
-            atomicOr(l.flags, (unsigned)CM_NL_INVALID);
-        } else {
+        double cx, cy, cz;
+        cell_corner(grid_of(d, c), c, cx, cy, cz);
+        for (int t = threadIdx.x & 15; t < n; t += 16) {
+            const int s = c * CM_SLOTS + t;
+            const int i = d.id[s];
+            if (i < 0 || i >= R.nid) {
+                atomicOr(l.flags, (unsigned)CM_NL_INVALID);
+                continue;
+            }
             const int id = R.id_base + i;
-            double cx, cy, cz;
-            cell_corner(grid_of(d, c), c, cx, cy, cz);
             const double x = cx + d.rx[s], y = cy + d.ry[s], z = cz + d.rz[s];
             l.slot[id] = s;
             l.pos[id] = CmPos{x, y, z};
@@ -104,11 +107,11 @@ __global__ __launch_bounds__(64) void k_nl_from_slots(DevState d, CmList l, int 
                 ex -= R.L0 * rint(ex * R.iL0);
                 ey -= R.L1 * rint(ey * R.iL1);
                 ez -= R.L2 * rint(ez * R.iL2);
-                far = ex * ex + ey * ey + ez * ez > d.nl_half2;
+                far = far || ex * ex + ey * ey + ez * ez > d.nl_half2;
             }
         }
     }
-    if (__ballot(far) != 0 && t == 0) atomicOr(l.flags, (unsigned)CM_NL_REBUILD);
+    if (__ballot(far) != 0 && threadIdx.x == 0) atomicOr(l.flags, (unsigned)CM_NL_REBUILD);
 }
 
 // ---------------------------------------------------------------------------
@@ -799,12 +802,12 @@ static unsigned grid_for(long n, int block)
 void cm_nl_init(const DevState &d, const CmList &l, hipStream_t st)
 {
     hipLaunchKernelGGL(k_nl_clear_ids, dim3(grid_for(l.n, 256)), dim3(256), 0, st, l);
-    hipLaunchKernelGGL(k_nl_from_slots, dim3(d.ncells), dim3(64), 0, st, d, l, 0);
+    hipLaunchKernelGGL(k_nl_from_slots, dim3(grid_for(d.ncells, 4)), dim3(64), 0, st, d, l, 0);
 }
 
 void cm_nl_sync(const DevState &d, const CmList &l, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_nl_from_slots, dim3(d.ncells), dim3(64), 0, st, d, l, 1);
+    hipLaunchKernelGGL(k_nl_from_slots, dim3(grid_for(d.ncells, 4)), dim3(64), 0, st, d, l, 1);
 }
 
 size_t cm_nl_build_lds(const CmList &l) { return (size_t)l.hcap * 16 + nl_slotmap_bytes(l.hcap); }
