@@ -143,53 +143,67 @@ __global__ __launch_bounds__(64) void k_move4(DevState d, double dtk, double dtd
         move_atom(d, g, c, ix, iy, iz, c * CM_SLOTS + t, dtk, dtd, kick, drift, rebin);
 }
 
-// Per touched cell (one thread each): count arrivals from lower/higher cells,
+// Per touched cell (one wave each): count arrivals from lower/higher cells,
 // then run the cell's own swap-with-last scan with n_low never-departing
 // placeholders appended, recording each departure's rank and the resulting
-// arrangement.
+// arrangement.  The lanes load the arrivals and the departing atoms' mover
+// records together; lane 0 runs the scan in LDS.
 __global__ __launch_bounds__(64) void k_rebox_scan(DevState d)
 {
-    __shared__ uint8_t sperm[64][128];
+    __shared__ uint8_t perm[128];
+    __shared__ int rank_of[CM_SLOTS];
     if (*d.flags & CM_FLAG_MULTIHOP) return;
-    uint8_t *perm = sperm[threadIdx.x];
+    const int L = threadIdx.x;
     const int ntc = d.counts[1];
-    for (int i = blockIdx.x * 64 + threadIdx.x; i < ntc; i += gridDim.x * 64) {
+    for (int i = blockIdx.x; i < ntc; i += gridDim.x) {
         const int c = d.tc[i];
         const int n0 = d.natoms[c];
         int na = d.arr_cnt[c];
+        // which of the cell's own atoms leave (set by k_move per departure)
+        const unsigned long long leaves = d.leave[c];
         unsigned fl = 0;
         if (na > CM_ACAP) {
             fl |= CM_FLAG_OVERFLOW;
             na = CM_ACAP;
         }
-        int nlow = 0;
-        for (int k = 0; k < na; k++) nlow += (d.mv_slot[d.arr[c * CM_ACAP + k]] / CM_SLOTS) < c;
+        bool low = false;
+        if (L < na) low = (d.mv_slot[d.arr[c * CM_ACAP + L]] / CM_SLOTS) < c;
+        const bool leaving = L < n0 && ((leaves >> L) & 1ull);
+        const int mv = leaving ? d.mvidx[c * CM_SLOTS + L] : -1;
+        const int nlow = __popcll(__ballot(low));
         const int nhigh = na - nlow;
         if (nlow > 0 && n0 + nlow >= CM_SLOTS) fl |= CM_FLAG_OVERFLOW;
         int len = n0 + nlow;
         if (len > 127) len = 127;
-        for (int k = 0; k < len; k++) perm[k] = (uint8_t)(k < n0 ? k : CM_SLOTS + (k - n0));
-        // which of the cell's own atoms leave (set by k_move per departure)
-        const unsigned long long leaves = d.leave[c];
-        int p = 0, rk = 0;
-        while (p < len) {
-            const int e = perm[p];
-            if (e < CM_SLOTS && ((leaves >> e) & 1ull)) {
-                d.mv_rank[d.mvidx[c * CM_SLOTS + e]] = rk++;
-                len--;
-                perm[p] = perm[len];
-            } else {
-                p++;
+        for (int k = L; k < len; k += 64) perm[k] = (uint8_t)(k < n0 ? k : CM_SLOTS + (k - n0));
+        __syncthreads();
+        if (L == 0) {
+            int p = 0, rk = 0;
+            while (p < len) {
+                const int e = perm[p];
+                if (e < CM_SLOTS && ((leaves >> e) & 1ull)) {
+                    rank_of[e] = rk++;
+                    len--;
+                    perm[p] = perm[len];
+                } else {
+                    p++;
+                }
             }
         }
+        len = __shfl(len, 0, 64);
+        __syncthreads();
+        if (leaving) d.mv_rank[mv] = rank_of[L];
         if (nhigh > 0 && len + nhigh >= CM_SLOTS) fl |= CM_FLAG_OVERFLOW;
         const int keep = len < CM_SLOTS ? len : CM_SLOTS;
-        for (int k = 0; k < keep; k++) d.perm[c * CM_SLOTS + k] = perm[k];
-        d.meta[4 * c + 0] = len;
-        d.meta[4 * c + 1] = nlow;
-        d.meta[4 * c + 2] = nhigh;
-        d.meta[4 * c + 3] = n0;
-        if (fl) atomicOr(d.flags, fl);
+        if (L < keep) d.perm[c * CM_SLOTS + L] = perm[L];
+        if (L == 0) {
+            d.meta[4 * c + 0] = len;
+            d.meta[4 * c + 1] = nlow;
+            d.meta[4 * c + 2] = nhigh;
+            d.meta[4 * c + 3] = n0;
+            if (fl) atomicOr(d.flags, fl);
+        }
+        __syncthreads();
     }
 }
 
@@ -431,9 +445,8 @@ void cm_launch_move(const DevState &d, hipStream_t st, double dtk, double dtd, i
 
 void cm_launch_rebox(const DevState &d, hipStream_t st)
 {
-    int nb = (d.ncells + 63) / 64;
-    nb = nb < 1024 ? nb : 1024;
-    hipLaunchKernelGGL(k_rebox_scan, dim3(nb), dim3(64), 0, st, d);
+    const int ns = d.ncells < 8192 ? d.ncells : 8192;
+    hipLaunchKernelGGL(k_rebox_scan, dim3(ns), dim3(64), 0, st, d);
     const int ng = d.ncells < 4096 ? d.ncells : 4096;
     hipLaunchKernelGGL(k_rebox_gather, dim3(ng), dim3(64), 0, st, d);
     hipLaunchKernelGGL(k_rebox_serial, dim3(1), dim3(64), 0, st, d);
