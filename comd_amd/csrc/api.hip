@@ -62,7 +62,7 @@ struct comd_ctx {
     // rebin scratch
     unsigned long long *leave;
     int *mvidx, *meta, *counts, *touched, *tc, *arr_cnt, *arr;
-    int *mv_slot, *mv_rank, *mv_id, *mv_itype;
+    int *mv_slot, *mv_rank, *mv_id, *mv_itype, *mv_nlq, *nlq;
     double *mv_rx, *mv_ry, *mv_rz, *mv_px, *mv_py, *mv_pz;
     uint8_t *perm;
     unsigned *flags;
@@ -157,6 +157,8 @@ static DevState dev_state(const comd_ctx *c)
     d.arr_cnt = c->arr_cnt;
     d.arr = c->arr;
     d.mv_slot = c->mv_slot; d.mv_rank = c->mv_rank; d.mv_id = c->mv_id; d.mv_itype = c->mv_itype;
+    d.mv_nlq = c->mv_nlq;
+    d.nlq = c->nlq;
     d.mv_rx = c->mv_rx; d.mv_ry = c->mv_ry; d.mv_rz = c->mv_rz;
     d.mv_px = c->mv_px; d.mv_py = c->mv_py; d.mv_pz = c->mv_pz;
     d.perm = c->perm;
@@ -446,7 +448,7 @@ static void free_all(comd_ctx *c)
     free_nl(c);
     void *ptrs[] = {c->natoms, c->id, c->itype, c->rx, c->ry, c->rz, c->px, c->py, c->pz,
                     c->fx, c->fy, c->fz, c->fe, c->rho, c->fi, c->vir, c->nnb, c->leave, c->mvidx, c->meta,
-                    c->counts, c->touched, c->tc, c->arr_cnt, c->arr, c->mv_slot, c->mv_rank, c->mv_id,
+                    c->counts, c->touched, c->tc, c->arr_cnt, c->arr, c->mv_slot, c->mv_rank, c->mv_id, c->mv_nlq, c->nlq,
                     c->mv_itype, c->mv_rx, c->mv_ry, c->mv_rz, c->mv_px, c->mv_py, c->mv_pz,
                     c->perm, c->flags, c->plist, c->pcount, c->pvalid,
                     c->partials, c->red_out, c->d_grids, c->d_cell_grid, c->d_pots,
@@ -753,6 +755,8 @@ static int ctx_create(int nrep, const comd_params *prms, comd_ctx **out)
     alloc((void **)&c->mv_slot, S * 4);
     alloc((void **)&c->mv_rank, S * 4);
     alloc((void **)&c->mv_id, S * 4);
+    alloc((void **)&c->mv_nlq, S * 4);
+    alloc((void **)&c->nlq, S * 4);
     alloc((void **)&c->mv_itype, S * 4);
     alloc((void **)&c->mv_rx, S * 8);
     alloc((void **)&c->mv_ry, S * 8);

@@ -62,10 +62,10 @@ __device__ __forceinline__ void move_atom(const DevState &d, const Grid &g, int 
             // request once the atom has moved half the skin (nearest image)
             double cx, cy, cz;
             corner_of(g, ix, iy, iz, cx, cy, cz);
-            const int id = g.nl_id_base + d.id[s];
+            const int q0 = d.nlq[s];
             const double ax = cx + x, ay = cy + y, az = cz + z;
-            d.nl_pos[id] = CmPos{ax, ay, az};
-            const float4 q = d.nl_ref[id];
+            d.nl_pos[q0] = CmPos{ax, ay, az};
+            const float4 q = d.nl_ref[q0];
             double ex = ax - q.x, ey = ay - q.y, ez = az - q.z;
             ex -= g.L0 * rint(ex * g.iL0);
             ey -= g.L1 * rint(ey * g.iL1);
@@ -100,6 +100,7 @@ __device__ __forceinline__ void move_atom(const DevState &d, const Grid &g, int 
     const int m = atomicAdd(&d.counts[0], 1);  // capacity: one record per slot
     d.mv_slot[m] = s;
     d.mv_id[m] = d.id[s];
+    d.mv_nlq[m] = d.nlq[s];
     d.mv_itype[m] = d.itype[s];
     d.mv_rx[m] = ax;
     d.mv_ry[m] = ay;
@@ -245,7 +246,7 @@ __global__ __launch_bounds__(64) void k_rebox_gather(DevState d)
             const int tot = len + nhigh;
             const int nout = tot < CM_SLOTS ? tot : CM_SLOTS;
             const int keep = len < CM_SLOTS ? len : CM_SLOTS;
-            int gid = 0, git = 0;
+            int gid = 0, git = 0, gq = 0;
             double gx = 0, gy = 0, gz = 0, gpx = 0, gpy = 0, gpz = 0;
             if (L < nout) {
                 int mv = -1, own = -1;
@@ -258,6 +259,7 @@ __global__ __launch_bounds__(64) void k_rebox_gather(DevState d)
                 }
                 if (own >= 0) {
                     gid = d.id[own];
+                    gq = d.nlq[own];
                     git = d.itype[own];
                     gx = d.rx[own];
                     gy = d.ry[own];
@@ -267,6 +269,7 @@ __global__ __launch_bounds__(64) void k_rebox_gather(DevState d)
                     gpz = d.pz[own];
                 } else if (mv >= 0) {
                     gid = d.mv_id[mv];
+                    gq = d.mv_nlq[mv];
                     git = d.mv_itype[mv];
                     gx = d.mv_rx[mv];
                     gy = d.mv_ry[mv];
@@ -281,6 +284,7 @@ __global__ __launch_bounds__(64) void k_rebox_gather(DevState d)
                 const int o = c * CM_SLOTS + L;
                 if (d.nl_slot) d.nl_slot[grid_of(d, c).nl_id_base + gid] = o;
                 d.id[o] = gid;
+                d.nlq[o] = gq;
                 d.itype[o] = git;
                 d.rx[o] = gx;
                 d.ry[o] = gy;
@@ -307,6 +311,7 @@ __device__ static void serial_copy(const DevState &d, int from, int to)
 {
     if (d.nl_slot) d.nl_slot[grid_of(d, to / CM_SLOTS).nl_id_base + d.id[from]] = to;
     d.id[to] = d.id[from];
+    d.nlq[to] = d.nlq[from];
     d.itype[to] = d.itype[from];
     d.rx[to] = d.rx[from];
     d.ry[to] = d.ry[from];

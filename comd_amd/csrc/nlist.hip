@@ -38,11 +38,21 @@ __device__ __forceinline__ bool nl_rebuild(const CmList &l) { return (l.flags[0]
 // x wrapped into [0, L) (to rounding); iL = 1 / L
 __device__ __forceinline__ double wrap_coord(double x, double L, double iL) { return x - L * floor(x * iL); }
 
-// List position of atom id, read through the global address space.
-__device__ __forceinline__ CmPos load_pos(const CmPos *pos, int id)
+// List position q (build order), read through the global address space.
+__device__ __forceinline__ CmPos load_pos(const CmPos *pos, int q)
 {
-    const CM_GLOBAL double *q = (const CM_GLOBAL double *)pos + 3 * id;
-    return CmPos{q[0], q[1], q[2]};
+    const CM_GLOBAL double *p = (const CM_GLOBAL double *)pos + 3 * q;
+    return CmPos{p[0], p[1], p[2]};
+}
+
+// Absolute position of the atom in slot s of a cell of grid g: the cell's
+// corner plus r, as the drift kernel forms it.
+__device__ __forceinline__ CmPos slot_pos(const Grid &g, const DevState &d, int s)
+{
+    const int c = s / CM_SLOTS;
+    double cx, cy, cz;
+    cell_corner(g, c, cx, cy, cz);
+    return CmPos{cx + d.rx[s], cy + d.ry[s], cz + d.rz[s]};
 }
 
 // Replica r's list geometry.
@@ -77,10 +87,10 @@ __global__ void k_nl_clear_ids(CmList l)
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(l.flags, (unsigned)CM_NL_REBUILD);
 }
 
-// check = 0: after binning or an upload (the ids are checked);
-// check = 1: positions after steps that did not keep them, with the
-// displacement test of the drift kernel.  Four consecutive cells per wave,
-// 16 lanes each.
+// check = 0: after binning or an upload (the ids are checked; a build
+// follows, which takes the positions from the slots); check = 1: positions
+// after steps that did not keep them, with the displacement test of the
+// drift kernel.  Four consecutive cells per wave, 16 lanes each.
 __global__ __launch_bounds__(64) void k_nl_from_slots(DevState d, CmList l, int check)
 {
     const int c = blockIdx.x * 4 + (threadIdx.x >> 4);
@@ -98,11 +108,13 @@ __global__ __launch_bounds__(64) void k_nl_from_slots(DevState d, CmList l, int 
                 continue;
             }
             const int id = R.id_base + i;
-            const double x = cx + d.rx[s], y = cy + d.ry[s], z = cz + d.rz[s];
             l.slot[id] = s;
-            l.pos[id] = CmPos{x, y, z};
+            if (!check) d.nlq[s] = id;  // any position in range until the first build
             if (check) {
-                const float4 q = l.ref[id];
+                const double x = cx + d.rx[s], y = cy + d.ry[s], z = cz + d.rz[s];
+                const int q0 = d.nlq[s];
+                l.pos[q0] = CmPos{x, y, z};
+                const float4 q = l.ref[q0];
                 double ex = x - q.x, ey = y - q.y, ez = z - q.z;
                 ex -= R.L0 * rint(ex * R.iL0);
                 ey -= R.L1 * rint(ey * R.iL1);
@@ -245,7 +257,7 @@ __global__ __launch_bounds__(256) void k_nl_key(DevState d, CmList l)
         if (s < 0) continue;
         const NlRep R = nl_rep(l, cell_rep(d, s / CM_SLOTS));
         int ax, ay, az;
-        aux_coords(R, l.pos[i], ax, ay, az);
+        aux_coords(R, slot_pos(grid_of(d, s / CM_SLOTS), d, s), ax, ay, az);
         const int a = R.aux_base + ax + R.na0 * (ay + R.na1 * az);
         const int k = atomicAdd(&l.aux_cnt[a], 1);
         if (k < CM_NL_AUXCAP) l.aux_mem[a * CM_NL_AUXCAP + k] = i;
@@ -370,7 +382,7 @@ __global__ __launch_bounds__(256) void k_nl_sort(CmList l)
 // The last block to finish closes the build: overflow (a list past kmax, a
 // halo past hcap) -> lists invalid, rebuild request cleared, counts cleared
 // for the next build.
-__global__ __launch_bounds__(1024) void k_nl_build(CmList l)
+__global__ __launch_bounds__(1024) void k_nl_build(DevState d, CmList l)
 {
     if (!nl_rebuild(l)) return;
     extern __shared__ float4 hl4[];  // [hcap] frame x, y, z; then the slot map
@@ -385,6 +397,8 @@ __global__ __launch_bounds__(1024) void k_nl_build(CmList l)
     const NlRep R = nl_rep(l, l.group_rep[g]);
     const GroupGeo G = group_geo(R, g - R.group_base);
     const float rl2 = (float)(R.rl2 * (1.0 + 4e-6));
+    // the group's replica: the grid of every atom it stages
+    const Grid gr = d.ngrids == 1 ? d.g0 : d.grids[l.group_rep[g]];
     group_setup(l, R, G, g, S, slot_of);
     const int H = S.off[S.ns];
     if (H > l.hcap) {  // the same in every thread of the block
@@ -392,7 +406,7 @@ __global__ __launch_bounds__(1024) void k_nl_build(CmList l)
     } else {
         const double ox = (G.x0 + 0.5 * G.lx) * R.ab0, oy = (G.y0 + 0.5 * G.ly) * R.ab1,
                      oz = (G.z0 + 0.5 * G.lz) * R.ab2;
-        constexpr int SB = 4;  // staged entries per thread in flight
+        constexpr int SB = 2;  // staged entries per thread in flight (each: slot, r, its cell's corner)
         for (int h0 = threadIdx.x; h0 < H; h0 += SB * blockDim.x) {
             int sl[SB], q[SB], id[SB];
 #pragma unroll
@@ -402,10 +416,10 @@ __global__ __launch_bounds__(1024) void k_nl_build(CmList l)
                 q[k] = S.start[sl[k]] + (h - S.off[sl[k]]);
             }
 #pragma unroll
-            for (int k = 0; k < SB; k++) id[k] = l.order[q[k]];
+            for (int k = 0; k < SB; k++) id[k] = l.slot[l.order[q[k]]];
             CmPos p[SB];
 #pragma unroll
-            for (int k = 0; k < SB; k++) p[k] = load_pos(l.pos, id[k]);
+            for (int k = 0; k < SB; k++) p[k] = slot_pos(gr, d, id[k]);
 #pragma unroll
             for (int k = 0; k < SB; k++) {
                 const int h = h0 + k * (int)blockDim.x;
@@ -418,8 +432,12 @@ __global__ __launch_bounds__(1024) void k_nl_build(CmList l)
         __syncthreads();
         const int shx = G.lx + 2, shy = G.ly + 2;
         for (int t = S.t0 + threadIdx.x; t < S.t1; t += blockDim.x) {
-            const int id = l.order[t];
-            const CmPos p = l.pos[id];
+            // the positions from the slots (as k_nl_key took them), kept in
+            // build order for the passes and the drift
+            const int ts = l.slot[l.order[t]];
+            const CmPos p = slot_pos(gr, d, ts);
+            l.pos[t] = p;
+            d.nlq[ts] = t;
             int ax, ay, az;
             aux_coords(R, p, ax, ay, az);
             // home slot: the atom's own cell, an interior slot of the halo
@@ -462,7 +480,7 @@ __global__ __launch_bounds__(1024) void k_nl_build(CmList l)
             }
             l.ncnt[t] = min(k, l.kmax);
             ovf |= k > l.kmax;
-            l.ref[id] = make_float4((float)p.x, (float)p.y, (float)p.z, 0.0f);
+            l.ref[t] = make_float4((float)p.x, (float)p.y, (float)p.z, 0.0f);
         }
     }
     if (ovf) atomicOr(l.flags, (unsigned)CM_NL_OVERFLOW);
@@ -743,21 +761,20 @@ void k_nl_pair(DevState d, PotDev P, CmList l, int tab_bytes, int gbase)
         // to 80 registers for six waves per SIMD)
         constexpr int SB = MIXED && KIND == CM_KIND_EAM3 ? 1 : 6;
         for (int h0 = threadIdx.x; h0 < H; h0 += SB * blockDim.x) {
-            // slots first, then the id loads, then the position loads
-            int sl[SB], q[SB], id[SB];
+            // slots first, then the position loads (build order: a
+            // halo slot's atoms are consecutive)
+            int sl[SB], q[SB];
 #pragma unroll
             for (int k = 0; k < SB; k++) {
                 const int h = min(h0 + k * (int)blockDim.x, H - 1);
                 sl[k] = slot_of[h];
                 q[k] = S.start[sl[k]] + (h - S.off[sl[k]]);
             }
-#pragma unroll
-            for (int k = 0; k < SB; k++) id[k] = l.order[q[k]];
             CmPos p[SB];
             double fp[SB];
 #pragma unroll
             for (int k = 0; k < SB; k++) {
-                p[k] = load_pos(l.pos, id[k]);
+                p[k] = load_pos(l.pos, q[k]);
                 fp[k] = KIND == CM_KIND_EAM3 ? ((const CM_GLOBAL double *)l.fq)[q[k]] : 0.0;
             }
 #pragma unroll
@@ -817,7 +834,7 @@ void cm_nl_build(const DevState &d, const CmList &l, hipStream_t st)
     hipLaunchKernelGGL(k_nl_key, dim3(1024), dim3(256), 0, st, d, l);
     hipLaunchKernelGGL(k_nl_gsum, dim3((unsigned)std::min(4096, (l.ngroups + 3) / 4)), dim3(256), 0, st, l);
     hipLaunchKernelGGL(k_nl_sort, dim3(1024), dim3(256), 0, st, l);
-    hipLaunchKernelGGL(k_nl_build, dim3(l.pass_blocks), dim3(l.pass_threads), cm_nl_build_lds(l), st, l);
+    hipLaunchKernelGGL(k_nl_build, dim3(l.pass_blocks), dim3(l.pass_threads), cm_nl_build_lds(l), st, d, l);
 }
 
 size_t cm_nl_tab_bytes(const PotDev &P, int kind, int mixed)
