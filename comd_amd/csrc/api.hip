@@ -62,7 +62,8 @@ struct comd_ctx {
     // rebin scratch
     unsigned long long *leave;
     int *mvidx, *meta, *counts, *touched, *tc, *arr_cnt, *arr;
-    int *mv_slot, *mv_rank, *mv_id, *mv_itype, *mv_nlq, *nlq;
+    int *mv_slot, *mv_rank, *mv_id, *mv_itype;
+    float4 *mv_nlr, *nlr;
     double *mv_rx, *mv_ry, *mv_rz, *mv_px, *mv_py, *mv_pz;
     uint8_t *perm;
     unsigned *flags;
@@ -157,8 +158,8 @@ static DevState dev_state(const comd_ctx *c)
     d.arr_cnt = c->arr_cnt;
     d.arr = c->arr;
     d.mv_slot = c->mv_slot; d.mv_rank = c->mv_rank; d.mv_id = c->mv_id; d.mv_itype = c->mv_itype;
-    d.mv_nlq = c->mv_nlq;
-    d.nlq = c->nlq;
+    d.mv_nlr = c->mv_nlr;
+    d.nlr = c->nlr;
     d.mv_rx = c->mv_rx; d.mv_ry = c->mv_ry; d.mv_rz = c->mv_rz;
     d.mv_px = c->mv_px; d.mv_py = c->mv_py; d.mv_pz = c->mv_pz;
     d.perm = c->perm;
@@ -170,7 +171,6 @@ static DevState dev_state(const comd_ctx *c)
     d.pvalid = lists ? c->pvalid : nullptr;
     const bool nl = c->nl_active != 0;
     d.nl_pos = nl ? c->nl.pos : nullptr;
-    d.nl_ref = nl ? c->nl.ref : nullptr;
     d.nl_slot = nl ? c->nl.slot : nullptr;
     d.nl_flags = nl ? c->nl.flags : nullptr;
     // the displacement test reads the build's positions in FP32: half the
@@ -430,7 +430,7 @@ int comd_grid_for(comd_params *p, const double bounds[3], double defgrad, double
 
 static void free_nl(comd_ctx *c)
 {
-    void *p[] = {c->nl.pos,      c->nl.selfh,    c->nl.ref,       c->nl.slot,    c->nl.order,    c->nl.fq,
+    void *p[] = {c->nl.pos,      c->nl.selfh,           c->nl.slot,    c->nl.order,    c->nl.fq,
                  c->nl.nbl,      c->nl.ncnt,     c->nl.aux_cnt,   c->nl.aux_n,   c->nl.aux_start,
                  c->nl.aux_mem,  c->nl.gstart,   c->nl.nonempty,  c->nl.count,   c->nl.flags,
                  (void *)c->nl.cell_order, (void *)c->nl.group_q0, (void *)c->nl.group_rep,
@@ -448,7 +448,7 @@ static void free_all(comd_ctx *c)
     free_nl(c);
     void *ptrs[] = {c->natoms, c->id, c->itype, c->rx, c->ry, c->rz, c->px, c->py, c->pz,
                     c->fx, c->fy, c->fz, c->fe, c->rho, c->fi, c->vir, c->nnb, c->leave, c->mvidx, c->meta,
-                    c->counts, c->touched, c->tc, c->arr_cnt, c->arr, c->mv_slot, c->mv_rank, c->mv_id, c->mv_nlq, c->nlq,
+                    c->counts, c->touched, c->tc, c->arr_cnt, c->arr, c->mv_slot, c->mv_rank, c->mv_id, c->mv_nlr, c->nlr,
                     c->mv_itype, c->mv_rx, c->mv_ry, c->mv_rz, c->mv_px, c->mv_py, c->mv_pz,
                     c->perm, c->flags, c->plist, c->pcount, c->pvalid,
                     c->partials, c->red_out, c->d_grids, c->d_cell_grid, c->d_pots,
@@ -755,8 +755,8 @@ static int ctx_create(int nrep, const comd_params *prms, comd_ctx **out)
     alloc((void **)&c->mv_slot, S * 4);
     alloc((void **)&c->mv_rank, S * 4);
     alloc((void **)&c->mv_id, S * 4);
-    alloc((void **)&c->mv_nlq, S * 4);
-    alloc((void **)&c->nlq, S * 4);
+    alloc((void **)&c->mv_nlr, S * 16);
+    alloc((void **)&c->nlr, S * 16);
     alloc((void **)&c->mv_itype, S * 4);
     alloc((void **)&c->mv_rx, S * 8);
     alloc((void **)&c->mv_ry, S * 8);
@@ -1024,7 +1024,6 @@ static int nl_setup(comd_ctx *c)
         const size_t n = (size_t)nids;
         alloc((void **)&c->nl.pos, n * sizeof(CmPos));
         alloc((void **)&c->nl.selfh, n * 2);
-        alloc((void **)&c->nl.ref, n * sizeof(float4));
         alloc((void **)&c->nl.slot, n * 4);
         alloc((void **)&c->nl.order, n * 4);
         alloc((void **)&c->nl.fq, n * 8);

@@ -62,10 +62,9 @@ __device__ __forceinline__ void move_atom(const DevState &d, const Grid &g, int 
             // request once the atom has moved half the skin (nearest image)
             double cx, cy, cz;
             corner_of(g, ix, iy, iz, cx, cy, cz);
-            const int q0 = d.nlq[s];
+            const float4 q = d.nlr[s];  // position at the last build, build position
             const double ax = cx + x, ay = cy + y, az = cz + z;
-            d.nl_pos[q0] = CmPos{ax, ay, az};
-            const float4 q = d.nl_ref[q0];
+            d.nl_pos[__float_as_int(q.w)] = CmPos{ax, ay, az};
             double ex = ax - q.x, ey = ay - q.y, ez = az - q.z;
             ex -= g.L0 * rint(ex * g.iL0);
             ey -= g.L1 * rint(ey * g.iL1);
@@ -100,7 +99,7 @@ __device__ __forceinline__ void move_atom(const DevState &d, const Grid &g, int 
     const int m = atomicAdd(&d.counts[0], 1);  // capacity: one record per slot
     d.mv_slot[m] = s;
     d.mv_id[m] = d.id[s];
-    d.mv_nlq[m] = d.nlq[s];
+    d.mv_nlr[m] = d.nlr[s];
     d.mv_itype[m] = d.itype[s];
     d.mv_rx[m] = ax;
     d.mv_ry[m] = ay;
@@ -246,7 +245,8 @@ __global__ __launch_bounds__(64) void k_rebox_gather(DevState d)
             const int tot = len + nhigh;
             const int nout = tot < CM_SLOTS ? tot : CM_SLOTS;
             const int keep = len < CM_SLOTS ? len : CM_SLOTS;
-            int gid = 0, git = 0, gq = 0;
+            int gid = 0, git = 0;
+            float4 gq = {0.0f, 0.0f, 0.0f, 0.0f};
             double gx = 0, gy = 0, gz = 0, gpx = 0, gpy = 0, gpz = 0;
             if (L < nout) {
                 int mv = -1, own = -1;
@@ -259,7 +259,7 @@ __global__ __launch_bounds__(64) void k_rebox_gather(DevState d)
                 }
                 if (own >= 0) {
                     gid = d.id[own];
-                    gq = d.nlq[own];
+                    gq = d.nlr[own];
                     git = d.itype[own];
                     gx = d.rx[own];
                     gy = d.ry[own];
@@ -269,7 +269,7 @@ __global__ __launch_bounds__(64) void k_rebox_gather(DevState d)
                     gpz = d.pz[own];
                 } else if (mv >= 0) {
                     gid = d.mv_id[mv];
-                    gq = d.mv_nlq[mv];
+                    gq = d.mv_nlr[mv];
                     git = d.mv_itype[mv];
                     gx = d.mv_rx[mv];
                     gy = d.mv_ry[mv];
@@ -284,7 +284,7 @@ __global__ __launch_bounds__(64) void k_rebox_gather(DevState d)
                 const int o = c * CM_SLOTS + L;
                 if (d.nl_slot) d.nl_slot[grid_of(d, c).nl_id_base + gid] = o;
                 d.id[o] = gid;
-                d.nlq[o] = gq;
+                d.nlr[o] = gq;
                 d.itype[o] = git;
                 d.rx[o] = gx;
                 d.ry[o] = gy;
@@ -311,7 +311,7 @@ __device__ static void serial_copy(const DevState &d, int from, int to)
 {
     if (d.nl_slot) d.nl_slot[grid_of(d, to / CM_SLOTS).nl_id_base + d.id[from]] = to;
     d.id[to] = d.id[from];
-    d.nlq[to] = d.nlq[from];
+    d.nlr[to] = d.nlr[from];
     d.itype[to] = d.itype[from];
     d.rx[to] = d.rx[from];
     d.ry[to] = d.ry[from];

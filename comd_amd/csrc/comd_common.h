@@ -50,7 +50,8 @@ struct DevState {
     // force outputs (slot layout)
     double *fx, *fy, *fz, *fe, *rho, *fi, *vir;
     int *nnb;           // per slot: neighbours inside the cutoff (last density/LJ pass)
-    int *nlq;           // per slot: the atom's list build position (moved with the atom)
+    float4 *nlr;        // per slot: the atom's FP32 position at the last list build (x, y, z)
+                        // and its build position (w, int bits); moved with the atom
     // rebin scratch
     unsigned long long *leave;  // per cell: bit k set when slot k departs (cleared with touched)
     int *mvidx;         // per slot: index of its mover record (departing atoms)
@@ -63,7 +64,8 @@ struct DevState {
     int *arr;           // per cell: up to 64 arriving mover indices
     // mover records: source slot, departure rank, and the atom with its
     // shifted coordinates
-    int *mv_slot, *mv_rank, *mv_id, *mv_itype, *mv_nlq;
+    int *mv_slot, *mv_rank, *mv_id, *mv_itype;
+    float4 *mv_nlr;
     double *mv_rx, *mv_ry, *mv_rz, *mv_px, *mv_py, *mv_pz;
     unsigned *flags;    // error flag word
     // pair lists of the EAM density pass, replayed by the embedding-force
@@ -74,12 +76,10 @@ struct DevState {
     uint8_t *pcount;
     int *pvalid;
     // neighbour-list bookkeeping kept by the drift and rebin kernels (null
-    // when lists are off): per build position the atom's absolute position
-    // and its position at the last list build (FP32); per atom id its slot;
-    // a flag word whose bit CM_NL_REBUILD is raised when an atom has moved
-    // more than half the skin
+    // when lists are off): per build position the atom's absolute position;
+    // per atom id its slot; a flag word whose bit CM_NL_REBUILD is raised
+    // when an atom has moved more than half the skin
     CmPos *nl_pos;
-    const float4 *nl_ref;
     int *nl_slot;
     unsigned *nl_flags;
     double nl_half2;     // (skin / 2 - FP32 rounding of the reference positions)^2

@@ -192,7 +192,6 @@ struct CmList {
     const int *cell_order;   // [naux]: auxiliary cells in build order (group, then cell)
     const int *group_q0;     // [ngroups + 1]: a group's first position in cell_order
     CmPos *pos;              // per build position: absolute position (kept by the drift)
-    float4 *ref;             // per build position: position at the last build (w unused)
     double *fq;              // per position in build order: F' of the last density pass
     uint16_t *selfh;         // per t: the atom's own index in its group's staged halo
     int *slot;               // per id: current slot, -1 when absent

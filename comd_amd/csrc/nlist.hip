@@ -109,12 +109,12 @@ __global__ __launch_bounds__(64) void k_nl_from_slots(DevState d, CmList l, int 
             }
             const int id = R.id_base + i;
             l.slot[id] = s;
-            if (!check) d.nlq[s] = id;  // any position in range until the first build
+            // any build position in range until the first build
+            if (!check) d.nlr[s] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(id));
             if (check) {
                 const double x = cx + d.rx[s], y = cy + d.ry[s], z = cz + d.rz[s];
-                const int q0 = d.nlq[s];
-                l.pos[q0] = CmPos{x, y, z};
-                const float4 q = l.ref[q0];
+                const float4 q = d.nlr[s];
+                l.pos[__float_as_int(q.w)] = CmPos{x, y, z};
                 double ex = x - q.x, ey = y - q.y, ez = z - q.z;
                 ex -= R.L0 * rint(ex * R.iL0);
                 ey -= R.L1 * rint(ey * R.iL1);
@@ -437,7 +437,6 @@ __global__ __launch_bounds__(1024) void k_nl_build(DevState d, CmList l)
             const int ts = l.slot[l.order[t]];
             const CmPos p = slot_pos(gr, d, ts);
             l.pos[t] = p;
-            d.nlq[ts] = t;
             int ax, ay, az;
             aux_coords(R, p, ax, ay, az);
             // home slot: the atom's own cell, an interior slot of the halo
@@ -480,7 +479,7 @@ __global__ __launch_bounds__(1024) void k_nl_build(DevState d, CmList l)
             }
             l.ncnt[t] = min(k, l.kmax);
             ovf |= k > l.kmax;
-            l.ref[t] = make_float4((float)p.x, (float)p.y, (float)p.z, 0.0f);
+            d.nlr[ts] = make_float4((float)p.x, (float)p.y, (float)p.z, __int_as_float(t));
         }
     }
     if (ovf) atomicOr(l.flags, (unsigned)CM_NL_OVERFLOW);
