@@ -85,17 +85,50 @@ class ChebT(C.Structure):
         (n, C.POINTER(PotArray)) for n in ("phi", "rho", "f", "dphi", "drho", "df")] + [("lat", C.c_double)]
 
 
+class Observables(C.Structure):
+    """struct comd_observables of include/comd_hip.h"""
+    _fields_ = [("pe", C.c_double), ("ke", C.c_double), ("etot", C.c_double), ("temperature", C.c_double),
+                ("virial", C.c_double * 9), ("virial_pair", C.c_double * 9), ("pressure", C.c_double),
+                ("stress_ref", C.c_double), ("volume", C.c_double), ("natoms", C.c_int)]
+
+    def as_dict(self):
+        out = {k: getattr(self, k) for k in ("pe", "ke", "etot", "temperature", "pressure", "stress_ref", "volume",
+                                             "natoms")}
+        out["virial"] = np.array(self.virial[:]).reshape(3, 3)
+        out["virial_pair"] = np.array(self.virial_pair[:]).reshape(3, 3)
+        return out
+
+
+class SimflatT(C.Structure):
+    """simflat_t (src-flat/include/pmdTypes.h:104-141), double precision"""
+    _fields_ = [("stateflag", C.c_int), ("nbx", C.c_int * 3), ("nboxes", C.c_int), ("ntot", C.c_int),
+                ("bounds", C.c_double * 3), ("boxsize", C.c_double * 3), ("dcenter", _dp), ("natoms", _ip),
+                ("id", _ip), ("iType", _ip), ("mass", _dp), ("r", _dp), ("p", _dp), ("f", _dp), ("rho", _dp),
+                ("fi", _dp), ("e", C.c_double), ("nAtoms", C.c_int), ("pot", C.c_void_p), ("comment", C.c_char_p),
+                ("stress", C.c_double), ("defgrad", C.c_double), ("bf", C.c_double), ("ch_pot", C.POINTER(ChebT))]
+
+
+class CommandT(C.Structure):
+    """command_t (src-flat/include/mycommand.h:60-75)"""
+    _fields_ = [("filename", C.c_char * 1024), ("potdir", C.c_char * 1024), ("potname", C.c_char * 1024),
+                ("debug", C.c_int), ("doeam", C.c_int), ("periodic", C.c_int), ("usegpu", C.c_int),
+                ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("bf", C.c_double), ("lat", C.c_double),
+                ("defgrad", C.c_double), ("temp", C.c_double)]
+
+
 _HIP = None
 _HOST = None
+_DROPIN = None
 
 
 def _load():
-    global _HIP, _HOST
+    global _HIP, _HOST, _DROPIN
     if _HIP is not None:
         return _HIP, _HOST
     hip_path = os.path.join(PKG_DIR, "libcomd_hip.so")
     host_path = os.path.join(PKG_DIR, "libcomd_host.so")
-    for p in (hip_path, host_path):
+    dropin_path = os.path.join(PKG_DIR, "libcomd_dropin.so")
+    for p in (hip_path, dropin_path, host_path):
         if not os.path.exists(p):
             raise ImportError(f"{p} is not built; run __graft_entry__.build() or make -C comd_amd")
     hip = C.CDLL(hip_path, mode=C.RTLD_GLOBAL)
@@ -141,6 +174,9 @@ def _load():
     hip.comd_get_cheb.argtypes = [vp, _dp]
     hip.comd_eam_cheb_grad.argtypes = [vp, _dp, C.c_int, _dp, C.c_double, C.c_double, _dp, _dp]
     hip.comd_batch_cheb_grad.argtypes = [vp, _dp, C.c_int, _dp, _dp, C.c_double, _dp, _dp]
+    hip.comd_observe.argtypes = [vp, C.POINTER(Observables)]
+    hip.comd_batch_observe.argtypes = [vp, C.c_int, C.POINTER(Observables)]
+    hip.comd_run_monitored.argtypes = [vp, C.c_int, C.c_int, C.c_double, _dp]
     host.setEamPot.argtypes = [C.c_char_p, C.c_char_p]
     host.setEamPot.restype = C.POINTER(EamPotT)
     host.setChebPot.argtypes = [C.POINTER(EamPotT), C.c_int]
@@ -150,13 +186,36 @@ def _load():
     host.comd_write_gz.argtypes = [C.c_char_p, C.c_int, C.c_double * 3, _dp, _dp]
     host.comd_read_gz.argtypes = [C.c_char_p, _ip, C.c_double * 3, C.POINTER(_dp), C.POINTER(_dp)]
     host.comd_host_error.restype = C.c_char_p
-    _HIP, _HOST = hip, host
+    # the reference's entry points and the drop-in adapters on simflat_t
+    sp, pp = C.POINTER(SimflatT), C.c_void_p
+    host.create_fcc_lattice.argtypes = [CommandT, pp]
+    host.create_fcc_lattice.restype = sp
+    host.fromFileGzip.argtypes = [CommandT, pp]
+    host.fromFileGzip.restype = sp
+    host.getLJPot.restype = pp
+    host.destroySimulation.argtypes = [C.POINTER(sp)]
+    host.parseCommandLine.argtypes = [C.POINTER(CommandT), C.c_int, C.POINTER(C.c_char_p)]
+    host.parseCommandLine.restype = None
+    dropin = C.CDLL(dropin_path, mode=C.RTLD_GLOBAL)
+    dropin.comd_hip_eam_force.argtypes = [sp]
+    dropin.comd_hip_lj_force.argtypes = [sp]
+    dropin.comd_hip_nTimeSteps.argtypes = [C.c_int, sp, C.c_double]
+    dropin.comd_hip_nTimeSteps.restype = C.c_double
+    dropin.comd_hip_reBoxAll.argtypes = [sp]
+    dropin.comd_host_error.restype = C.c_char_p
+    _HIP, _HOST, _DROPIN = hip, host, dropin
     return hip, host
 
 
 def libraries():
     """(hip, host) ctypes handles; raises ImportError when not built."""
     return _load()
+
+
+def dropin():
+    """ctypes handle of libcomd_dropin.so (include/comd_dropin.h)."""
+    _load()
+    return _DROPIN
 
 
 def device_info():
@@ -372,12 +431,25 @@ class Simulation:
         pp = np.ascontiguousarray(p, dtype=np.float64) if p is not None else None
         self._check(self._hip.comd_bin_atoms(self.ctx, x.shape[0], _ptr(x), _ptr(pp), int(itype)))
 
-    def upload(self, natoms, ids, itype, r, p):
+    @staticmethod
+    def _slot_arrays(ncells, natoms, ids, itype, r, p):
+        """Contiguous copies of a slot-layout state, with the shapes the C
+        side reads (it reads ncells counts, ncells*64 ids and types and
+        ncells*64 x 3 positions and momenta)."""
+        S = ncells * MAXATOMS
         na = np.ascontiguousarray(natoms, dtype=np.int32)
         i = np.ascontiguousarray(ids, dtype=np.int32)
         t = np.ascontiguousarray(itype, dtype=np.int32)
         rr = np.ascontiguousarray(r, dtype=np.float64)
         pp = np.ascontiguousarray(p, dtype=np.float64)
+        for name, a, shape in (("natoms", na, (ncells,)), ("ids", i, (S,)), ("itype", t, (S,)), ("r", rr, (S, 3)),
+                               ("p", pp, (S, 3))):
+            if a.shape != shape:
+                raise ValueError(f"{name} has shape {a.shape}, expected {shape}")
+        return na, i, t, rr, pp
+
+    def upload(self, natoms, ids, itype, r, p):
+        na, i, t, rr, pp = self._slot_arrays(self.ncells, natoms, ids, itype, r, p)
         self._check(self._hip.comd_upload(self.ctx, _ptr(na, C.c_int), _ptr(i, C.c_int), _ptr(t, C.c_int),
                                           _ptr(rr), _ptr(pp)))
 
@@ -486,6 +558,22 @@ class Simulation:
 
     # ---- Chebyshev coefficients and their adjoint
 
+    def observe(self):
+        """Force at the current positions, then PE, KE, temperature, the full
+        virial tensor (pair + embedding + kinetic) and its pair part,
+        pressure and the reference's scalar stress (comd_observe)."""
+        o = Observables()
+        self._check(self._hip.comd_observe(self.ctx, C.byref(o)))
+        return o.as_dict()
+
+    def run_monitored(self, nblocks, block_steps, dt=1.0e-15):
+        """PE and KE after a force at the start and after each of nblocks
+        nTimeSteps(block_steps) calls, recorded on the device
+        (comd_run_monitored); returns two arrays of nblocks + 1."""
+        out = np.zeros(2 * (nblocks + 1))
+        self._check(self._hip.comd_run_monitored(self.ctx, int(nblocks), int(block_steps), float(dt), _ptr(out)))
+        return out[0::2].copy(), out[1::2].copy()
+
     def set_cheb(self, coef):
         """Replace the phi, rho, F value series (3 x n) and rebuild the
         derivative series (comd_set_cheb)."""
@@ -566,6 +654,13 @@ class Batch(Simulation):
         pp = np.ascontiguousarray(p, dtype=np.float64) if p is not None else None
         self._check(self._hip.comd_batch_bin_atoms(self.ctx, r, x.shape[0], _ptr(x), _ptr(pp), int(itype)))
 
+    def upload(self, r, natoms, ids, itype, rr, p):
+        """Slot-layout state of replica r (comd_batch_upload)."""
+        _, Cn, _ = self.info(r)
+        na, i, t, x, pp = self._slot_arrays(Cn, natoms, ids, itype, rr, p)
+        self._check(self._hip.comd_batch_upload(self.ctx, r, _ptr(na, C.c_int), _ptr(i, C.c_int), _ptr(t, C.c_int),
+                                                _ptr(x), _ptr(pp)))
+
     def download(self, r):
         """Slot-layout state of replica r (as Simulation.download)."""
         _, Cn, _ = self.info(r)
@@ -576,6 +671,12 @@ class Batch(Simulation):
             self.ctx, r, _ptr(out["natoms"], C.c_int), _ptr(out["id"], C.c_int), _ptr(out["itype"], C.c_int),
             _ptr(out["r"]), _ptr(out["p"]), _ptr(out["f"]), _ptr(out["rho"]), _ptr(out["fi"])))
         return out
+
+    def observe_of(self, r):
+        """comd_batch_observe of replica r (as Simulation.observe)."""
+        o = Observables()
+        self._check(self._hip.comd_batch_observe(self.ctx, r, C.byref(o)))
+        return o.as_dict()
 
     def energy_of(self, r):
         pe, ke, st = C.c_double(), C.c_double(), C.c_double()

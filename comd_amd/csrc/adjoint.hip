@@ -47,8 +47,6 @@ struct AdjDev {
     double *part;                // per block: ANMOM partial moments
 };
 
-// Replica of global cell c.
-__device__ __forceinline__ int rep_of(const DevState &d, int c) { return d.ngrids == 1 ? 0 : d.cell_grid[c]; }
 
 // T_0..T_{m-1} at y, each times wa added to a[] (and times wb to b[]).
 template <int MIXED, int TWO>

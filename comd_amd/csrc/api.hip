@@ -67,6 +67,7 @@ struct comd_ctx {
     double *mv_rx, *mv_ry, *mv_rz, *mv_px, *mv_py, *mv_pz;
     uint8_t *perm;
     unsigned *flags;
+    int *mh;  // per replica: multi-hop flag of the current rebin pass
     // pair lists of the EAM density pass (EAM contexts)
     uint16_t *plist;
     uint8_t *pcount;
@@ -74,6 +75,10 @@ struct comd_ctx {
     int use_plist;
     double *partials;
     int nparts;
+    // observables and the energy monitor (allocated on first use)
+    double *obs_part, *mon;
+    comd_observables *obs_out;
+    long mon_cap;
     double *red_out;       // 3 doubles, device
     double *host_out;      // pinned 4 doubles
     unsigned *host_flags;  // pinned
@@ -165,6 +170,7 @@ static DevState dev_state(const comd_ctx *c)
     d.perm = c->perm;
     d.meta = c->meta;
     d.flags = c->flags;
+    d.mh = c->mh;
     const bool lists = c->use_plist && c->plist;
     d.plist = lists ? c->plist : nullptr;
     d.pcount = lists ? c->pcount : nullptr;
@@ -450,10 +456,10 @@ static void free_all(comd_ctx *c)
                     c->fx, c->fy, c->fz, c->fe, c->rho, c->fi, c->vir, c->nnb, c->leave, c->mvidx, c->meta,
                     c->counts, c->touched, c->tc, c->arr_cnt, c->arr, c->mv_slot, c->mv_rank, c->mv_id, c->mv_nlr, c->nlr,
                     c->mv_itype, c->mv_rx, c->mv_ry, c->mv_rz, c->mv_px, c->mv_py, c->mv_pz,
-                    c->perm, c->flags, c->plist, c->pcount, c->pvalid,
+                    c->perm, c->flags, c->mh, c->plist, c->pcount, c->pvalid,
                     c->partials, c->red_out, c->d_grids, c->d_cell_grid, c->d_pots,
                     c->adj_g[0], c->adj_g[1], c->adj_g[2], c->adj_lam, c->adj_mu,
-                    c->adj_ft, c->adj_f2, c->adj_part, c->adj_out, c->adj_rep};
+                    c->adj_ft, c->adj_f2, c->adj_part, c->adj_out, c->adj_rep, c->obs_part, c->obs_out, c->mon};
     for (void *p : ptrs)
         if (p) hipFree(p);
     for (Replica &r : c->reps)
@@ -766,6 +772,7 @@ static int ctx_create(int nrep, const comd_params *prms, comd_ctx **out)
     alloc((void **)&c->mv_pz, S * 8);
     alloc((void **)&c->perm, S);
     alloc((void **)&c->flags, 16);
+    alloc((void **)&c->mh, (size_t)nrep * 4);
     bool eam = false;
     for (int r = 0; r < nrep; r++) eam = eam || prms[r].potential == COMD_POT_EAM;
     if (eam) {
@@ -1173,9 +1180,26 @@ static int upload_rep(comd_ctx *c, int r, const int *natoms, const int *id, cons
     const size_t C = (size_t)g.ncells, S = C * CM_SLOTS;
     const size_t c0 = (size_t)g.cell_base, s0 = c0 * CM_SLOTS;
     int tot = 0;
+    long nids = 0;
     for (size_t b = 0; b < C; b++) {
         if (natoms[b] < 0 || natoms[b] > CM_SLOTS) return set_err(c, COMD_ERR_ARG, "cell %zu holds %d atoms", b, natoms[b]);
         tot += natoms[b];
+        for (int i = 0; i < natoms[b]; i++) {
+            const int a = id[b * CM_SLOTS + i];
+            if (a < 0 || a >= (1 << 30)) return set_err(c, COMD_ERR_ARG, "atom id %d in cell %zu out of range", a, b);
+            nids = std::max<long>(nids, (long)a + 1);
+        }
+    }
+    // the neighbour lists and the id -> slot map index atoms by id: each id
+    // must appear once
+    {
+        std::vector<uint8_t> seen((size_t)nids, 0);
+        for (size_t b = 0; b < C; b++)
+            for (int i = 0; i < natoms[b]; i++) {
+                const int a = id[b * CM_SLOTS + i];
+                if (seen[a]) return set_err(c, COMD_ERR_ARG, "atom id %d appears twice", a);
+                seen[a] = 1;
+            }
     }
     std::vector<double> buf(S);
     std::vector<int> it(S, 0);
@@ -1195,9 +1219,6 @@ static int upload_rep(comd_ctx *c, int r, const int *natoms, const int *id, cons
     c->ntot += tot - c->reps[r].ntot;
     c->reps[r].ntot = tot;
     c->have_atoms = 1;
-    long nids = 0;
-    for (size_t b = 0; b < C; b++)
-        for (int i = 0; i < natoms[b]; i++) nids = std::max<long>(nids, (long)id[b * CM_SLOTS + i] + 1);
     c->reps[r].nl_ids = nids;
     return nl_setup(c);
 }
@@ -1751,6 +1772,93 @@ int comd_timing_read(comd_ctx *c, double *ms, int *launches)
     }
     return COMD_OK;
 }
+
+// ---- observables and the energy monitor
+
+// Force at the current positions, then the observables of replica r into
+// out_dev (device memory) with part as the block partials.
+static int observe_rep(comd_ctx *c, int r, double *part, comd_observables *out_dev)
+{
+    const Replica &R = c->reps[r];
+    if (R.g.nbx0 < 3 || R.g.nbx1 < 3 || R.g.nbx2 < 3)
+        return set_err(c, COMD_ERR_GRID, "observables need at least 3 link cells per dimension");
+    op_force(c);
+    const double pe_scale = R.prm.potential == COMD_POT_LJ ? 4.0 * R.prm.lj_epsilon * R.pot.lj_s6 : 1.0;
+    cm_launch_observe(dev_state(c), R.pot, R.g, pe_scale, part, out_dev, c->stream);
+    HIPCHK(c, hipGetLastError());
+    return COMD_OK;
+}
+
+static int observables_rep(comd_ctx *c, int r, comd_observables *out)
+{
+    if (!out) return set_err(c, COMD_ERR_ARG, "null argument");
+    if (!c->have_atoms) return set_err(c, COMD_ERR_STATE, "no atoms");
+    if (!c->obs_part) {
+        if (hipMalloc((void **)&c->obs_part, (size_t)CM_OBS_MAXBLOCKS * CM_OBS_N * 8) != hipSuccess ||
+            hipMalloc((void **)&c->obs_out, sizeof(comd_observables)) != hipSuccess)
+            return set_err(c, COMD_ERR_NOMEM, "observables buffers");
+    }
+    int rc = observe_rep(c, r, c->obs_part, c->obs_out);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(out, c->obs_out, sizeof *out, hipMemcpyDeviceToHost, c->stream));
+    return check_flags(c);  // synchronises
+}
+
+extern "C" {
+
+int comd_observe(comd_ctx *c, comd_observables *out)
+{
+    if (!c) return COMD_ERR_ARG;
+    return observables_rep(c, 0, out);
+}
+
+int comd_batch_observe(comd_ctx *c, int r, comd_observables *out)
+{
+    if (!c) return COMD_ERR_ARG;
+    int rc = check_rep(c, r);
+    return rc ? rc : observables_rep(c, r, out);
+}
+
+int comd_run_monitored(comd_ctx *c, int nblocks, int block_steps, double dt_seconds, double *pe_ke)
+{
+    if (!c || nblocks < 0 || block_steps < 0 || !pe_ke) return COMD_ERR_ARG;
+    if (c->nrep != 1) return set_err(c, COMD_ERR_ARG, "the monitor runs single simulations");
+    if (!c->have_atoms) return set_err(c, COMD_ERR_STATE, "no atoms");
+    const long need = 3L * (nblocks + 1);
+    if (need > c->mon_cap) {
+        if (c->mon) hipFree(c->mon);
+        c->mon = nullptr;
+        c->mon_cap = 0;
+        if (hipMalloc((void **)&c->mon, (size_t)need * 8) != hipSuccess) return set_err(c, COMD_ERR_NOMEM, "monitor");
+        c->mon_cap = need;
+    }
+    int rc = clear_flags(c);
+    if (rc) return rc;
+    const Replica &R = c->reps[0];
+    const double dt = dt_seconds * kBohrPerAtuToAPerS;  // timestep.c:126
+    for (int b = 0; b <= nblocks; b++) {
+        if (b > 0) {
+            rc = run_steps(c, block_steps, dt);
+            if (rc) return rc;
+        }
+        op_force(c);  // the trailing force of nTimeSteps (timestep.c:136)
+        cm_launch_reduce(dev_state(c), R.g, c->partials, c->nparts, c->stream);
+        cm_launch_reduce_final(c->partials, c->nparts, c->mon + 3 * b, c->stream);
+    }
+    HIPCHK(c, hipGetLastError());
+    std::vector<double> m((size_t)need);
+    HIPCHK(c, hipMemcpyAsync(m.data(), c->mon, (size_t)need * 8, hipMemcpyDeviceToHost, c->stream));
+    rc = check_flags(c);  // synchronises
+    if (rc) return rc;
+    const double scale = R.prm.potential == COMD_POT_LJ ? 4.0 * R.prm.lj_epsilon * R.pot.lj_s6 : 1.0;
+    for (int b = 0; b <= nblocks; b++) {
+        pe_ke[2 * b] = m[3 * b] * scale;
+        pe_ke[2 * b + 1] = m[3 * b + 1];
+    }
+    return COMD_OK;
+}
+
+}  // extern "C"
 
 // ---- Chebyshev coefficients and their adjoint (single-replica contexts)
 

@@ -17,8 +17,9 @@
 // movers and of touched cells (a few per thousand atoms per half step) and
 // updates the slot arrays in place.  The argument holds while no arrival
 // into a higher-indexed cell would move again in the same pass; k_move flags
-// that case and a serial kernel then redoes the pass exactly as the reference
-// does.
+// that case for the atom's replica, whose cells the parallel kernels then
+// leave alone, and a serial kernel (one block per flagged replica) redoes
+// that replica's pass exactly as the reference does.
 #include "comd_internal.h"
 
 #define CM_ACAP 64  // arrivals recorded per cell and pass
@@ -94,7 +95,7 @@ __device__ __forceinline__ void move_atom(const DevState &d, const Grid &g, int 
         if (ay < 0.0) my = jy - 1; else if (ay >= g.bs1) my = jy + 1;
         if (az < 0.0) mz = jz - 1; else if (az >= g.bs2) mz = jz + 1;
         const int j2 = cell_wrap(g, mx, my, mz);
-        if (j2 >= 0 && j2 != dst) atomicOr(d.flags, (unsigned)CM_FLAG_MULTIHOP);
+        if (j2 >= 0 && j2 != dst) atomicOr(&d.mh[g.pot], 1);
     }
     const int m = atomicAdd(&d.counts[0], 1);  // capacity: one record per slot
     d.mv_slot[m] = s;
@@ -152,11 +153,11 @@ __global__ __launch_bounds__(64) void k_rebox_scan(DevState d)
 {
     __shared__ uint8_t perm[128];
     __shared__ int rank_of[CM_SLOTS];
-    if (*d.flags & CM_FLAG_MULTIHOP) return;
     const int L = threadIdx.x;
     const int ntc = d.counts[1];
     for (int i = blockIdx.x; i < ntc; i += gridDim.x) {
         const int c = d.tc[i];
+        if (d.mh[rep_of(d, c)]) continue;  // the same in every lane: replayed serially
         const int n0 = d.natoms[c];
         int na = d.arr_cnt[c];
         // which of the cell's own atoms leave (set by k_move per departure)
@@ -215,11 +216,12 @@ __global__ __launch_bounds__(64) void k_rebox_gather(DevState d)
 {
     __shared__ int sm[2][CM_SLOTS];  // movers by low / high arrival order
     const int L = threadIdx.x;
-    const bool skip = (*d.flags & CM_FLAG_MULTIHOP) != 0;
     const int ntc = d.counts[1];
     for (int i = blockIdx.x; i < ntc; i += gridDim.x) {
         const int c = d.tc[i];
-        if (!skip) {
+        // cells of a replica whose pass is replayed serially are left as the
+        // move kernel left them (the same in every lane)
+        if (!d.mh[rep_of(d, c)]) {
             const int len = d.meta[4 * c + 0];
             const int nhigh = d.meta[4 * c + 2];
             int na = d.arr_cnt[c];
@@ -304,9 +306,10 @@ __global__ __launch_bounds__(64) void k_rebox_gather(DevState d)
     }
 }
 
-// Serial pass, run only when k_move found an arrival that would move again:
-// reBoxAll exactly as the reference runs it, in place.  Always clears the
-// pass counters.
+// Serial pass of the replicas in which k_move found an arrival that would
+// move again (one block each, its first thread walking the replica's cells):
+// reBoxAll exactly as the reference runs it, in place.  Block 0 also clears
+// the pass counters.
 __device__ static void serial_copy(const DevState &d, int from, int to)
 {
     if (d.nl_slot) d.nl_slot[grid_of(d, to / CM_SLOTS).nl_id_base + d.id[from]] = to;
@@ -323,13 +326,16 @@ __device__ static void serial_copy(const DevState &d, int from, int to)
 
 __global__ void k_rebox_serial(DevState d)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    d.counts[0] = 0;
-    d.counts[1] = 0;
-    if (!(*d.flags & CM_FLAG_MULTIHOP)) return;
+    if (threadIdx.x != 0) return;
+    const int r = blockIdx.x;
+    if (r == 0) {
+        d.counts[0] = 0;
+        d.counts[1] = 0;
+    }
+    if (!d.mh[r]) return;
+    const Grid g = d.ngrids == 1 ? d.g0 : d.grids[r];
     unsigned fl = 0;
-    for (int c = 0; c < d.ncells; c++) {
-        const Grid g = grid_of(d, c);
+    for (int c = g.cell_base; c < g.cell_base + g.ncells; c++) {
         int ix, iy, iz;
         cell_xyz(g, c, ix, iy, iz);
         int i = 0;
@@ -362,7 +368,7 @@ __global__ void k_rebox_serial(DevState d)
             if (ni) serial_copy(d, c * CM_SLOTS + ni, o);
         }
     }
-    atomicAnd(d.flags, ~(unsigned)CM_FLAG_MULTIHOP);
+    d.mh[r] = 0;
     if (fl) atomicOr(d.flags, fl);
 }
 
@@ -454,7 +460,7 @@ void cm_launch_rebox(const DevState &d, hipStream_t st)
     hipLaunchKernelGGL(k_rebox_scan, dim3(ns), dim3(64), 0, st, d);
     const int ng = d.ncells < 4096 ? d.ncells : 4096;
     hipLaunchKernelGGL(k_rebox_gather, dim3(ng), dim3(64), 0, st, d);
-    hipLaunchKernelGGL(k_rebox_serial, dim3(1), dim3(64), 0, st, d);
+    hipLaunchKernelGGL(k_rebox_serial, dim3((unsigned)d.ngrids), dim3(64), 0, st, d);
 }
 
 void cm_launch_fill_int(int *p, int v, long n, hipStream_t st)

@@ -14,7 +14,6 @@
 enum {
     CM_FLAG_OVERFLOW = 1,  // a cell would reach 64 atoms (src-flat/domains.c:229)
     CM_FLAG_OUTSIDE = 2,   // atom outside the grid when binned
-    CM_FLAG_MULTIHOP = 4,  // an arrival would move again in the same rebin pass
 };
 
 // Link-cell grid of one replica.  All replicas of a context share the slot
@@ -68,6 +67,8 @@ struct DevState {
     float4 *mv_nlr;
     double *mv_rx, *mv_ry, *mv_rz, *mv_px, *mv_py, *mv_pz;
     unsigned *flags;    // error flag word
+    int *mh;            // per replica: an arrival of the current rebin pass would move again (the
+                        // replica's pass is then replayed serially)
     // pair lists of the EAM density pass, replayed by the embedding-force
     // pass (null when not kept): per cell CM_PLIST x 64 halo indices
     // ([entry][lane]), per slot-lane the entry count, per cell 1 when no
@@ -95,6 +96,9 @@ __device__ __forceinline__ Grid grid_of(const DevState &d, int c)
 {
     return d.ngrids == 1 ? d.g0 : d.grids[d.cell_grid[c]];
 }
+
+// Replica that owns global cell c.
+__device__ __forceinline__ int rep_of(const DevState &d, int c) { return d.ngrids == 1 ? 0 : d.cell_grid[c]; }
 
 __device__ __forceinline__ void cell_xyz(const Grid &g, int c, int &ix, int &iy, int &iz)
 {

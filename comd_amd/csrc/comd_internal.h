@@ -148,6 +148,18 @@ void cm_launch_adj_resid(const DevState &d, const CmAdjoint &J, const double *ft
 void cm_launch_adj_stage(const DevState &d, const PotDev &P, const CmAdjoint &J, int stage, int mixed,
                          hipStream_t st);
 
+// Observables (observe.hip): per-block partial sums of CM_OBS_N doubles
+// over one replica's cells (at most CM_OBS_MAXBLOCKS blocks, in part), then
+// their sum in block order and the comd_observables formed from it, written
+// to out (device memory).  pe_scale multiplies the per-atom energy sum (4
+// eps sigma^6 for Lennard-Jones, 1 for EAM).
+#define CM_OBS_N 16
+#define CM_OBS_MAXBLOCKS 2048
+#define CM_KB_HARTREE 3.166811563e-6  // Boltzmann constant, hartree per kelvin
+int cm_observe_blocks(const Grid &g);
+void cm_launch_observe(const DevState &d, const PotDev &P, const Grid &g, double pe_scale, double *part,
+                       comd_observables *out, hipStream_t st);
+
 // Group gi of ng along a dimension of na auxiliary cells: its first cell x0
 // and extent lx (extents differ by at most one; na / ng each when ng divides
 // na).

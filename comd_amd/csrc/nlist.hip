@@ -58,8 +58,6 @@ __device__ __forceinline__ CmPos slot_pos(const Grid &g, const DevState &d, int 
 // Replica r's list geometry.
 __device__ __forceinline__ NlRep nl_rep(const CmList &l, int r) { return l.rep[r]; }
 
-// Replica of global cell c.
-__device__ __forceinline__ int cell_rep(const DevState &d, int c) { return d.ngrids == 1 ? 0 : d.cell_grid[c]; }
 
 // Auxiliary cell of a position within its replica (the same arithmetic
 // wherever it is needed, so that the key and build kernels agree).
@@ -97,7 +95,7 @@ __global__ __launch_bounds__(64) void k_nl_from_slots(DevState d, CmList l, int 
     bool far = false;
     if (c < d.ncells) {
         const int n = d.natoms[c];
-        const NlRep R = nl_rep(l, cell_rep(d, c));
+        const NlRep R = nl_rep(l, rep_of(d, c));
         double cx, cy, cz;
         cell_corner(grid_of(d, c), c, cx, cy, cz);
         for (int t = threadIdx.x & 15; t < n; t += 16) {
@@ -255,7 +253,7 @@ __global__ __launch_bounds__(256) void k_nl_key(DevState d, CmList l)
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < l.n; i += gridDim.x * blockDim.x) {
         const int s = l.slot[i];
         if (s < 0) continue;
-        const NlRep R = nl_rep(l, cell_rep(d, s / CM_SLOTS));
+        const NlRep R = nl_rep(l, rep_of(d, s / CM_SLOTS));
         int ax, ay, az;
         aux_coords(R, slot_pos(grid_of(d, s / CM_SLOTS), d, s), ax, ay, az);
         const int a = R.aux_base + ax + R.na0 * (ay + R.na1 * az);
@@ -516,7 +514,7 @@ __global__ void k_nl_nonempty(DevState d, CmList l, int g0)
     if (l.flags[0] & CM_NL_INVALID) return;
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= d.ncells) return;
-    if (g0 > 0 && l.rep[cell_rep(d, c)].group_base < g0) return;
+    if (g0 > 0 && l.rep[rep_of(d, c)].group_base < g0) return;
     const Grid g = grid_of(d, c);
     int n = 0;
     if (g.ncells == 1) {

@@ -1,10 +1,10 @@
 /*
  * comd_host.c -- the reference's hot-path entry points (force slots,
  * computeForce, nTimeSteps, reBoxAll) and driver (parseCommandLine,
- * initSimFromCmdLine, do_compute_work) on top of the GPU C-ABI.
+ * initSimFromCmdLine, do_compute_work) on top of the drop-in adapters
+ * (comd_dropin.c) and the GPU C-ABI.
  */
 #include <getopt.h>
-#include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -13,249 +13,29 @@
 #include "comd_host.h"
 #include "comd_host_internal.h"
 
-static int g_status = 0;
-static char g_error[512] = "";
+int eamForce(void *s) { return comd_hip_eam_force(s); }
+int LJ(void *s) { return comd_hip_lj_force(s); }
 
-int comd_host_fail(int code, const char *fmt, ...)
-{
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(g_error, sizeof g_error, fmt, ap);
-    va_end(ap);
-    g_status = code;
-    return code;
-}
-
-int comd_host_status(void) { return g_status; }
-const char *comd_host_error(void) { return g_error; }
-
-/* ---------------------------------------------------------------------
- * simulation <-> device context bindings.  The reference is single-threaded
- * and non-reentrant; a small table is enough. */
-
-#define NBIND 64
-typedef struct binding {
-    const void *sim;
-    comd_ctx *ctx;
-    int kind;       /* COMD_POT_* */
-    int precision;
-    int eval;
-    int resident;   /* device holds the authoritative state */
-} binding;
-static binding g_bind[NBIND];
-
-static binding *find_binding(const void *s)
-{
-    for (int i = 0; i < NBIND; i++)
-        if (g_bind[i].sim == s) return &g_bind[i];
-    return NULL;
-}
-
-void comd_host_forget(simflat_t *s)
-{
-    binding *b = find_binding(s);
-    if (!b) return;
-    comd_destroy(b->ctx);
-    memset(b, 0, sizeof *b);
-}
-
-static int pot_kind(const simflat_t *s)
-{
-    if (s->pot->force == LJ || s->pot->force == comd_hip_lj_force) return COMD_POT_LJ;
-    return COMD_POT_EAM;
-}
-
-int comd_host_params(const simflat_t *s, comd_params *p)
-{
-    memset(p, 0, sizeof *p);
-    p->potential = pot_kind(s);
-    p->periodic = PERIODIC;
-    p->cutoff = s->pot->cutoff;
-    p->mass = s->pot->mass;
-    for (int j = 0; j < 3; j++) {
-        p->nbx[j] = s->nbx[j];
-        p->bounds[j] = s->bounds[j];
-        p->boxsize[j] = s->boxsize[j];
-    }
-    if (p->potential == COMD_POT_LJ) {
-        const ljpotential_t *lj = (const ljpotential_t *)s->pot;
-        p->lj_sigma = lj->sigma;
-        p->lj_epsilon = lj->epsilon;
-        return COMD_OK;
-    }
-    const eampotential_t *e = (const eampotential_t *)s->pot;
-    const potentialarray_t *t[3] = {e->phi, e->rho, e->f};
-    for (int k = 0; k < 3; k++) {
-        if (!t[k]) return comd_host_fail(COMD_ERR_ARG, "EAM table missing");
-        p->tab_n[k] = t[k]->n;
-        p->tab_x0[k] = t[k]->x0;
-        p->tab_xn[k] = t[k]->xn;
-        p->tab_invdx[k] = t[k]->invDx;
-        p->tab_values[k] = t[k]->values - 1;
-    }
-    return COMD_OK;
-}
-
-static double g_chebbuf[6 * 64];
-
-static binding *bind_sim(simflat_t *s, int kind_hint)
-{
-    binding *b = find_binding(s);
-    if (b) return b;
-    b = find_binding(NULL);
-    if (!b) {
-        comd_host_fail(COMD_ERR_ARG, "too many simulations");
-        return NULL;
-    }
-    comd_params p;
-    if (comd_host_params(s, &p)) return NULL;
-    if (kind_hint) p.potential = kind_hint;
-    if (p.potential == COMD_POT_EAM && s->ch_pot && s->ch_pot->phi && s->ch_pot->phi->n <= 64) {
-        const eam_cheby_t *ch = s->ch_pot;
-        const potentialarray_t *c[6] = {ch->phi, ch->rho, ch->f, ch->dphi, ch->drho, ch->df};
-        const int m = ch->phi->n;
-        for (int k = 0; k < 6; k++) memcpy(g_chebbuf + k * m, c[k]->values, (size_t)m * sizeof(double));
-        p.cheb_n = m;
-        p.cheb_coef = g_chebbuf;
-        p.cheb_x0[0] = ch->phi->x0; p.cheb_xn[0] = ch->phi->xn;
-        p.cheb_x0[1] = ch->rho->x0; p.cheb_xn[1] = ch->rho->xn;
-        p.cheb_x0[2] = ch->f->x0;   p.cheb_xn[2] = ch->f->xn;
-    }
-    comd_ctx *ctx = NULL;
-    int rc = comd_create(&p, &ctx);
-    if (rc) {
-        comd_host_fail(rc, "%s", comd_last_error(NULL));
-        return NULL;
-    }
-    b->sim = s;
-    b->ctx = ctx;
-    b->kind = p.potential;
-    b->precision = COMD_PREC_FP64;
-    b->eval = COMD_EVAL_TABLE;
-    b->resident = 0;
-    return b;
-}
-
-comd_ctx *comd_host_ctx(simflat_t *s)
-{
-    binding *b = bind_sim(s, 0);
-    return b ? b->ctx : NULL;
-}
-
-int comd_host_set_precision(simflat_t *s, int precision)
-{
-    binding *b = bind_sim(s, 0);
-    if (!b) return g_status;
-    b->precision = precision;
-    return comd_set_precision(b->ctx, precision);
-}
-
-int comd_host_set_eval(simflat_t *s, int cheby)
-{
-    binding *b = bind_sim(s, 0);
-    if (!b) return g_status;
-    b->eval = cheby ? COMD_EVAL_CHEB : COMD_EVAL_TABLE;
-    int rc = comd_set_eval(b->ctx, b->eval);
-    return rc ? comd_host_fail(rc, "%s", comd_last_error(b->ctx)) : COMD_OK;
-}
-
-static int upload_b(binding *b, simflat_t *s)
-{
-    int rc = comd_upload(b->ctx, s->natoms, s->id, s->iType, &s->r[0][0], &s->p[0][0]);
-    if (rc) return comd_host_fail(rc, "%s", comd_last_error(b->ctx));
-    return COMD_OK;
-}
-
-static int download_b(binding *b, simflat_t *s)
-{
-    int rc = comd_download(b->ctx, s->natoms, s->id, s->iType, &s->r[0][0], &s->p[0][0], &s->f[0][0], s->rho, s->fi);
-    if (rc) return comd_host_fail(rc, "%s", comd_last_error(b->ctx));
-    int tot = 0;
-    for (int c = 0; c < s->nboxes; c++) {
-        tot += s->natoms[c];
-        for (int i = 0; i < s->natoms[c]; i++) s->mass[c * MAXATOMS + i] = s->pot->mass;
-    }
-    s->ntot = tot;
-    return COMD_OK;
-}
-
-static int totals_b(binding *b, simflat_t *s)
-{
-    double pe = 0, st = 0;
-    int rc = comd_energy(b->ctx, &pe, NULL, &st);
-    if (rc) return comd_host_fail(rc, "%s", comd_last_error(b->ctx));
-    s->e = pe;
-    s->stress = st;
-    return COMD_OK;
-}
-
-int comd_host_upload(simflat_t *s)
-{
-    binding *b = bind_sim(s, 0);
-    return b ? upload_b(b, s) : g_status;
-}
-
-int comd_host_download(simflat_t *s)
-{
-    binding *b = bind_sim(s, 0);
-    if (!b) return g_status;
-    int rc = download_b(b, s);
-    return rc ? rc : totals_b(b, s);
-}
-
-/* force slot: mirror the host arrays, compute on the device, write back */
-static int force_adapter(void *sv, int kind)
-{
-    simflat_t *s = (simflat_t *)sv;
-    binding *b = bind_sim(s, kind);
-    if (!b) return g_status;
-    int rc;
-    if (!b->resident && (rc = upload_b(b, s))) return rc;
-    if ((rc = comd_force(b->ctx))) return comd_host_fail(rc, "%s", comd_last_error(b->ctx));
-    if ((rc = totals_b(b, s))) return rc;
-    if (!b->resident) return download_b(b, s);
-    return COMD_OK;
-}
-
-int comd_hip_eam_force(void *s) { return force_adapter(s, COMD_POT_EAM); }
-int comd_hip_lj_force(void *s) { return force_adapter(s, COMD_POT_LJ); }
-int eamForce(void *s) { return force_adapter(s, COMD_POT_EAM); }
-int LJ(void *s) { return force_adapter(s, COMD_POT_LJ); }
+/* the drop-in library tells LJ simulations by their force slot */
+__attribute__((constructor)) static void register_lj_slot(void) { comd_dropin_register_lj(LJ); }
 
 int computeForce(simflat_t *s) { return s->pot->force(s); }
 
-double comd_hip_nTimeSteps(int n, void *sv, double dt)
+static int kind_of(const simflat_t *s)
 {
-    simflat_t *s = (simflat_t *)sv;
-    binding *b = bind_sim(s, 0);
-    if (!b) return 0.0;
-    if (!b->resident && upload_b(b, s)) return 0.0;
-    double pe = 0.0;
-    int rc = comd_nsteps(b->ctx, n, dt, &pe);
-    if (rc) {
-        comd_host_fail(rc, "%s", comd_last_error(b->ctx));
-        return 0.0;
-    }
-    totals_b(b, s);
-    if (!b->resident) download_b(b, s);
-    return s->e;
+    return (s->pot->force == LJ || s->pot->force == comd_hip_lj_force) ? COMD_POT_LJ : COMD_POT_EAM;
 }
 
-double nTimeSteps(int n, simflat_t *s, real_t dt) { return comd_hip_nTimeSteps(n, s, dt); }
-
-int comd_hip_reBoxAll(void *sv)
+double nTimeSteps(int n, simflat_t *s, real_t dt)
 {
-    simflat_t *s = (simflat_t *)sv;
-    binding *b = bind_sim(s, 0);
-    if (!b) return g_status;
-    int rc;
-    if (!b->resident && (rc = upload_b(b, s))) return rc;
-    if ((rc = comd_rebox(b->ctx))) return comd_host_fail(rc, "%s", comd_last_error(b->ctx));
-    if (!b->resident) return download_b(b, s);
-    return COMD_OK;
+    if (comd_dropin_bind(s, kind_of(s))) return 0.0;
+    return comd_hip_nTimeSteps(n, s, dt);
 }
 
-void reBoxAll(simflat_t *s) { comd_hip_reBoxAll(s); }
+void reBoxAll(simflat_t *s)
+{
+    if (comd_dropin_bind(s, kind_of(s)) == COMD_OK) comd_hip_reBoxAll(s);
+}
 
 /* ---------------------------------------------------------------------
  * driver */
@@ -281,8 +61,22 @@ static void usage(const char *exe)
            exe);
 }
 
-int parseCommandLine(command_t *cmd, int argc, char **argv)
+static comd_run_options g_opts = {COMD_PREC_FP64, 0, 1, 0.0, 20, 10, 0.0, 0.0, 0.0, 0.0};
+
+const comd_run_options *comd_host_run_options(void) { return &g_opts; }
+void comd_host_set_run_options(const comd_run_options *o)
 {
+    if (o) g_opts = *o;
+}
+
+/* src-flat/mycommand.c:108-161: the reference's short options (-f -d -p -e
+ * -o -g -x -y -z -b -s -t, defaults potdir "pots", potname "ag", periodic,
+ * 20^3 cells, bf 1, defgrad 1) into cmd, and this library's long options
+ * into the driver settings.  Errors go to comd_host_status() (the
+ * reference's parser ignores unknown options; here they are reported). */
+void parseCommandLine(command_t *cmd, int argc, char **argv)
+{
+    comd_host_fail(COMD_OK, "");
     memset(cmd, 0, sizeof *cmd);
     strcpy(cmd->potdir, "pots");
     strcpy(cmd->potname, "ag");
@@ -290,9 +84,7 @@ int parseCommandLine(command_t *cmd, int argc, char **argv)
     cmd->nx = cmd->ny = cmd->nz = 20;
     cmd->bf = 1.0;
     cmd->defgrad = 1.0;
-    cmd->niter = 20;
-    cmd->nsteps = 10;
-    cmd->seed = 1;
+    comd_run_options o = {COMD_PREC_FP64, 0, 1, 0.0, 20, 10, 0.0, 0.0, 0.0, 0.0};
     static struct option opts[] = {
         {"help", no_argument, 0, 'h'},      {"infile", required_argument, 0, 'f'},
         {"potdir", required_argument, 0, 'd'}, {"potname", required_argument, 0, 'p'},
@@ -306,10 +98,11 @@ int parseCommandLine(command_t *cmd, int argc, char **argv)
         {"nsteps", required_argument, 0, 6}, {"lj", required_argument, 0, 7},
         {0, 0, 0, 0}};
     optind = 1;
+    opterr = 0;
     int c;
     while ((c = getopt_long(argc, argv, "hf:d:p:eogx:y:z:b:s:t:", opts, NULL)) != -1) {
         switch (c) {
-        case 'h': usage(argv[0]); return 2;
+        case 'h': usage(argv[0]); comd_host_fail(COMD_ERR_ARG, "help requested"); break;
         case 'f': snprintf(cmd->filename, sizeof cmd->filename, "%s", optarg); break;
         case 'd': snprintf(cmd->potdir, sizeof cmd->potdir, "%s", optarg); break;
         case 'p': snprintf(cmd->potname, sizeof cmd->potname, "%s", optarg); break;
@@ -322,27 +115,30 @@ int parseCommandLine(command_t *cmd, int argc, char **argv)
         case 'b': cmd->bf = atof(optarg); break;
         case 's': cmd->defgrad = atof(optarg); break;
         case 't': cmd->temp = atof(optarg); break;
-        case 1: cmd->precision = strcmp(optarg, "mixed") == 0 ? COMD_PREC_MIXED : COMD_PREC_FP64; break;
-        case 2: cmd->cheby = strcmp(optarg, "cheby") == 0; break;
-        case 3: cmd->seed = strtoull(optarg, NULL, 10); break;
-        case 4: cmd->disp = atof(optarg); break;
-        case 5: cmd->niter = atoi(optarg); break;
-        case 6: cmd->nsteps = atoi(optarg); break;
+        case 1: o.precision = strcmp(optarg, "mixed") == 0 ? COMD_PREC_MIXED : COMD_PREC_FP64; break;
+        case 2: o.cheby = strcmp(optarg, "cheby") == 0; break;
+        case 3: o.seed = strtoull(optarg, NULL, 10); break;
+        case 4: o.disp = atof(optarg); break;
+        case 5: o.niter = atoi(optarg); break;
+        case 6: o.nsteps = atoi(optarg); break;
         case 7:
-            if (sscanf(optarg, "%lf,%lf,%lf,%lf", &cmd->lj_sigma, &cmd->lj_epsilon, &cmd->lj_cutoff, &cmd->lj_mass) != 4)
-                return comd_host_fail(COMD_ERR_ARG, "--lj needs SIGMA,EPS,CUTOFF,MASS");
+            if (sscanf(optarg, "%lf,%lf,%lf,%lf", &o.lj_sigma, &o.lj_epsilon, &o.lj_cutoff, &o.lj_mass) != 4)
+                comd_host_fail(COMD_ERR_ARG, "--lj needs SIGMA,EPS,CUTOFF,MASS");
             break;
-        default: usage(argv[0]); return COMD_ERR_ARG;
+        default: usage(argv[0]); comd_host_fail(COMD_ERR_ARG, "unknown option"); break;
         }
     }
+    g_opts = o;
     PERIODIC = cmd->periodic;
-    return COMD_OK;
 }
 
 simflat_t *initSimFromCmdLine(int argc, char **argv)
 {
     command_t cmd;
-    if (parseCommandLine(&cmd, argc, argv)) return NULL;
+    comd_host_fail(COMD_OK, "");
+    parseCommandLine(&cmd, argc, argv);
+    if (comd_host_status()) return NULL;
+    const comd_run_options o = g_opts;
     pmd_base_potential_t *pot;
     if (cmd.doeam) {
         pot = setEamPot(cmd.potdir, cmd.potname);
@@ -350,24 +146,27 @@ simflat_t *initSimFromCmdLine(int argc, char **argv)
         cmd.lat = ((eampotential_t *)pot)->lat;  /* src-flat/docompute.c:101 */
     } else {
         ljpotential_t *lj = getLJPot();
-        if (cmd.lj_sigma > 0) {
-            lj->sigma = cmd.lj_sigma;
-            lj->epsilon = cmd.lj_epsilon;
-            lj->cutoff = cmd.lj_cutoff;
-            lj->mass = cmd.lj_mass;
+        if (o.lj_sigma > 0) {
+            lj->sigma = o.lj_sigma;
+            lj->epsilon = o.lj_epsilon;
+            lj->cutoff = o.lj_cutoff;
+            lj->mass = o.lj_mass;
         }
         pot = (pmd_base_potential_t *)lj;
         cmd.lat = 1.122462048 * lj->cutoff;  /* src-flat/docompute.c:117 */
-        if (cmd.lj_sigma > 0) cmd.lat = 1.4142135623730951 * 1.122462048309373 * lj->sigma;
+        if (o.lj_sigma > 0) cmd.lat = 1.4142135623730951 * 1.122462048309373 * lj->sigma;
     }
-    simflat_t *sim = cmd.filename[0] ? fromFileGzip(cmd, pot) : create_fcc_lattice(cmd, pot);
+    /* the reference's lattice is at rest whatever -t says; --disp or -t
+     * here give the seeded lattice */
+    simflat_t *sim;
+    if (cmd.filename[0]) sim = fromFileGzip(cmd, pot);
+    else if (cmd.temp > 0.0 || o.disp > 0.0) sim = comd_host_seeded_lattice(cmd, pot, o.seed, o.disp);
+    else sim = create_fcc_lattice(cmd, pot);
     if (!sim) return NULL;
     if (cmd.doeam) sim->ch_pot = setChebPot((eampotential_t *)pot, 32);  /* docompute.c:133 */
-    binding *b = bind_sim(sim, 0);
-    if (!b) return NULL;
-    if (comd_host_set_precision(sim, cmd.precision) || (cmd.cheby && comd_host_set_eval(sim, 1))) return NULL;
-    if (upload_b(b, sim)) return NULL;
-    b->resident = 1;
+    if (comd_dropin_bind(sim, kind_of(sim))) return NULL;
+    if (comd_host_set_precision(sim, o.precision) || (o.cheby && comd_host_set_eval(sim, 1))) return NULL;
+    if (comd_dropin_set_resident(sim, 1)) return NULL;
     reBoxAll(sim);
     printf("    total atoms is: %d\n", sim->ntot);
     return sim;
@@ -388,15 +187,15 @@ void *do_compute_work(SimThreadData *data)
     const int niter = data->niter > 0 ? data->niter : 20;
     const int nsteps = data->nsteps > 0 ? data->nsteps : 10;
     const double dt = 1.0e-15;
-    binding *b = bind_sim(s, 0);
+    comd_ctx *ctx = comd_dropin_bind(s, kind_of(s)) ? NULL : comd_host_ctx(s);
     double ts = wall();
     computeForce(s);
-    if (b) comd_sync(b->ctx);
+    if (ctx) comd_sync(ctx);
     double te = wall();
     double ke = 0;
     for (int iter = 0; iter < niter; iter++) {
         const double ns = iter == 0 ? 1.0 : (double)(1 + nsteps);
-        if (b) comd_energy(b->ctx, NULL, &ke, NULL);
+        if (ctx) comd_energy(ctx, NULL, &ke, NULL);
         printf(" %3d %30.20g computed in %.3fs (%8.4f us/atom for %d atoms)  ke %.12g\n", iter * nsteps, s->e, te - ts,
                1.0e6 * (te - ts) / ns / (double)s->ntot, s->ntot, ke);
         printf("Virial stress = %g\n", s->stress);
@@ -406,6 +205,6 @@ void *do_compute_work(SimThreadData *data)
     }
     printf(" %3d %30.20f computed in %.3fs (%8.4f us/atom for %d atoms)\n", niter * nsteps, s->e, te - ts,
            1.0e6 * (te - ts) / (double)(nsteps + 1) / (double)s->ntot, s->ntot);
-    if (b && b->resident) comd_host_download(s);
+    if (ctx && comd_dropin_resident(s)) comd_host_download(s);
     return NULL;
 }

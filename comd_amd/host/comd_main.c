@@ -13,13 +13,12 @@ int main(int argc, char **argv)
         fprintf(stderr, "initialisation failed (%d): %s\n", comd_host_status(), comd_host_error());
         return comd_host_status() ? comd_host_status() : 1;
     }
-    command_t cmd;
-    parseCommandLine(&cmd, argc, argv);
+    const comd_run_options *o = comd_host_run_options();
     SimThreadData data = {0};
     data.sim = sim;
-    data.niter = cmd.niter;
-    data.nsteps = cmd.nsteps;
-    data.eam_flag = cmd.doeam;
+    data.niter = o->niter;
+    data.nsteps = o->nsteps;
+    data.eam_flag = sim->pot->force == eamForce;
     do_compute_work(&data);
     int rc = comd_host_status();
     if (rc) fprintf(stderr, "error (%d): %s\n", rc, comd_host_error());

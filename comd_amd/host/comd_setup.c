@@ -13,7 +13,6 @@
 #include "comd_host.h"
 #include "comd_host_internal.h"
 
-int PERIODIC = 1;
 
 /* ---------------------------------------------------------------------
  * EAM tables: Fortran-record binary (byte-swapped when the first record
@@ -436,7 +435,10 @@ int comd_fcc_positions(int nx, int ny, int nz, double lat, double defgrad, doubl
     return n;
 }
 
-simflat_t *create_fcc_lattice(command_t cmd, pmd_base_potential_t *pot)
+/* Lattice of create_fcc_lattice, optionally with seeded displacements and
+ * momenta, binned in id order through putAtomInBox (ic_fcc.c:125). */
+static simflat_t *fcc_lattice(command_t cmd, pmd_base_potential_t *pot, int seeded, unsigned long long seed,
+                              double disp)
 {
     simflat_t *s = blankSimulation(pot);
     if (!s) return NULL;
@@ -454,7 +456,7 @@ simflat_t *create_fcc_lattice(command_t cmd, pmd_base_potential_t *pot)
     double *xyz = (double *)malloc((size_t)(n > 0 ? n : 1) * 3 * sizeof(double));
     double *p = (double *)calloc((size_t)(n > 0 ? n : 1) * 3, sizeof(double));
     comd_fcc_positions(cmd.nx, cmd.ny, cmd.nz, cmd.lat, cmd.defgrad, xyz, n);
-    if (cmd.temp > 0.0 || cmd.disp > 0.0) comd_seed_state(cmd.seed, n, pot->mass, cmd.temp, cmd.disp, xyz, p);
+    if (seeded && (cmd.temp > 0.0 || disp > 0.0)) comd_seed_state(seed, n, pot->mass, cmd.temp, disp, xyz, p);
     for (int a = 0; a < n; a++)
         if (putAtomInBox(s, a, 1, 1, pot->mass, xyz[3 * a], xyz[3 * a + 1], xyz[3 * a + 2], p[3 * a], p[3 * a + 1],
                          p[3 * a + 2], 0, 0, 0)) {
@@ -466,6 +468,15 @@ simflat_t *create_fcc_lattice(command_t cmd, pmd_base_potential_t *pot)
     free(xyz);
     free(p);
     return s;
+}
+
+/* src-flat/ic_fcc.c:63-141: a perfect lattice at rest (the reference does
+ * not use cmd.temp), type 1, ids in creation order. */
+simflat_t *create_fcc_lattice(command_t cmd, pmd_base_potential_t *pot) { return fcc_lattice(cmd, pot, 0, 0, 0.0); }
+
+simflat_t *comd_host_seeded_lattice(command_t cmd, pmd_base_potential_t *pot, unsigned long long seed, double disp)
+{
+    return fcc_lattice(cmd, pot, 1, seed, disp);
 }
 
 /* SplitMix64 (Steele, Lea, Flood 2014) */

@@ -239,6 +239,47 @@ enum comd_kclass {
 int comd_timing(comd_ctx *ctx, int on);
 int comd_timing_read(comd_ctx *ctx, double *ms /* COMD_K_NCLASS */, int *launches /* COMD_K_NCLASS */);
 
+/* ---- Observables beyond the reference's ----
+ *
+ * The reference reports only the potential energy and a partial scalar
+ * "stress" (the xx component of the EAM embedding virial,
+ * src-flat/eam.c:649-657; for LJ the pair xx term with a kinetic term
+ * counted once per non-empty neighbour cell, ljForce.c:175-221).  These
+ * add the kinetic energy and temperature and the full virial tensor
+ *   W_ab = sum_i p_ia p_ib / m + sum_{i<j} r_ij,a f_ij,b
+ * (f_ij the physical pair force on i from j, -grad_i U, whatever the sign
+ * convention of the force array), from which the pressure tr(W) / (3V) and
+ * the stress tensor -W / V follow.  No counterpart in the reference. */
+typedef struct comd_observables {
+    double pe;              /* potential energy, hartree (the reference's s->e) */
+    double ke;              /* sum |p|^2 / (2 m), hartree */
+    double etot;            /* pe + ke */
+    double temperature;     /* 2 ke / (3 N k_B), kelvin, k_B = 3.166811563e-6 hartree/K */
+    double virial[9];       /* W, hartree, row-major */
+    double virial_pair[9];  /* its pair and embedding part */
+    double pressure;        /* tr(W) / (3 V), hartree / A^3 (x 4359.74 for GPa) */
+    double stress_ref;      /* the reference's scalar "stress" of the same force call */
+    double volume;          /* A^3 */
+    int natoms;
+} comd_observables;
+
+/* Runs one force evaluation at the current positions (so that F' and the
+ * energies are current), then the observables of the simulation (of
+ * replica r for comd_batch_observe).  The pair sums are FP64 over the
+ * link-cell stencil with the force passes' cutoff predicates; F' is that
+ * of the force call, in the context's precision.  Needs at least 3 link
+ * cells per dimension (COMD_ERR_GRID otherwise). */
+int comd_observe(comd_ctx *ctx, comd_observables *out);
+int comd_batch_observe(comd_ctx *ctx, int r, comd_observables *out);
+
+/* Energy-conservation monitor: the potential and kinetic energy (hartree)
+ * after a force at the starting positions, then after each of nblocks
+ * calls of nTimeSteps(block_steps, dt) -- src-flat/docompute.c:196-207's
+ * loop, whose energies the reference prints -- recorded on the device
+ * without synchronising with the host between blocks.  pe_ke receives
+ * 2 (nblocks + 1) doubles (pe, ke) per record.  Single simulations only. */
+int comd_run_monitored(comd_ctx *ctx, int nblocks, int block_steps, double dt_seconds, double *pe_ke);
+
 /* ---- Batches: many independent replicas in one context ----
  *
  * No counterpart in the reference (a src-flat process holds one simulation;

@@ -11,7 +11,7 @@
  *   eampotential_t        src-flat/include/eamTypes.h:90-99
  *   eam_cheby_t           src-flat/include/eamTypes.h:102-112
  *   ljpotential_t         src-flat/include/ljTypes.h:67-76
- *   command_t             src-flat/include/mycommand.h:52-67 (+ new fields at the end)
+ *   command_t             src-flat/include/mycommand.h:60-75
  * so that comd_hip_eam_force / comd_hip_lj_force can be stored in a reference
  * potential's force slot and comd_hip_nTimeSteps can replace nTimeSteps.
  *
@@ -24,6 +24,7 @@
 #define COMD_HOST_H
 
 #include "comd_hip.h"
+#include "comd_dropin.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -113,14 +114,22 @@ typedef struct command_t {
     double lat;
     double defgrad;
     double temp;
-    /* additions of this library */
+} command_t;
+
+/* Settings of this library's driver beyond the reference's command_t (kept
+ * out of it, so that command_t, passed by value to create_fcc_lattice and
+ * fromFileGzip, has the reference's layout).  No counterpart in the
+ * reference.  parseCommandLine fills the process-wide copy from the options
+ * --precision, --eval, --seed, --disp, --niter, --nsteps and --lj;
+ * initSimFromCmdLine and the driver read it. */
+typedef struct comd_run_options {
     int precision;         /* COMD_PREC_FP64 / COMD_PREC_MIXED */
     int cheby;             /* evaluate EAM through the Chebyshev fit */
     unsigned long long seed;
     double disp;           /* Gaussian displacement sigma, A */
     int niter, nsteps;     /* do_compute_work loop shape (20 x 10 in the reference) */
     double lj_sigma, lj_epsilon, lj_cutoff, lj_mass; /* 0: the reference's defaults */
-} command_t;
+} comd_run_options;
 
 typedef struct SimThreadData {
     void *viz;
@@ -149,6 +158,8 @@ int putAtomInBox(simflat_t *s, int id, char flagMove, int iType, real_t mass, do
 void destroySimulation(simflat_t **ps);
 
 /* ---- initial conditions (src-flat/ic_fcc.c:63-141, src-flat/read.c:155-241) ---- */
+/* The reference's 0 K lattice: atoms at rest, type 1, ids in creation
+ * order (cmd.temp is not used, as in ic_fcc.c). */
 simflat_t *create_fcc_lattice(command_t cmd, pmd_base_potential_t *pot);
 simflat_t *fromFileGzip(command_t cmd, pmd_base_potential_t *pot);
 
@@ -159,23 +170,31 @@ int computeForce(simflat_t *s);                      /* src-flat/timestep.c:110 
 double nTimeSteps(int n, simflat_t *s, real_t dt);   /* src-flat/timestep.c:114 */
 void reBoxAll(simflat_t *s);                         /* src-flat/domains.c:347 */
 
-/* Drop-in adapters for a reference build (same signatures as the force
- * slot and nTimeSteps); they mirror the simflat_t arrays to the device,
- * run there and write the results back into the same slots. */
-int comd_hip_eam_force(void *simflat);
-int comd_hip_lj_force(void *simflat);
-double comd_hip_nTimeSteps(int n, void *simflat, double dt);
-int comd_hip_reBoxAll(void *simflat);
+/* The drop-in adapters comd_hip_eam_force, comd_hip_lj_force,
+ * comd_hip_nTimeSteps and comd_hip_reBoxAll are declared in comd_dropin.h
+ * (libcomd_dropin.so, a dependency of this library). */
 
 /* ---- driver (src-flat/docompute.c:68-218, src-flat/mycommand.c:108-161) ---- */
-int parseCommandLine(command_t *cmd, int argc, char **argv);
+/* As the reference's (mycommand.h:78, void); an unknown option or a bad
+ * value is reported through comd_host_status() instead of exit(). */
+void parseCommandLine(command_t *cmd, int argc, char **argv);
 simflat_t *initSimFromCmdLine(int argc, char **argv);
 void *do_compute_work(SimThreadData *data);
 
 /* ---- additions ---- */
-/* Status of the last failed call of this library (0 if none). */
-int comd_host_status(void);
-const char *comd_host_error(void);
+/* The driver settings parseCommandLine parsed last (defaults before any
+ * call: FP64, tables, seed 1, 20 x 10 steps), and a setter for callers that
+ * do not parse a command line. */
+const comd_run_options *comd_host_run_options(void);
+void comd_host_set_run_options(const comd_run_options *opts);
+/* The FCC lattice of create_fcc_lattice with seeded Gaussian displacements
+ * (sigma disp, A) and Maxwell-Boltzmann momenta at temp kelvin
+ * (comd_seed_state), binned in id order as putAtomInBox would be called for
+ * the displaced positions.  No counterpart in the reference (its lattice is
+ * always at rest, and -t is parsed but unused). */
+simflat_t *comd_host_seeded_lattice(command_t cmd, pmd_base_potential_t *pot, unsigned long long seed, double disp);
+/* comd_host_status / comd_host_error (comd_dropin.h): status of the last
+ * failed call of this library or of the adapters (0 if none). */
 /* Device context bound to a simulation (created on first use). */
 comd_ctx *comd_host_ctx(simflat_t *s);
 int comd_host_set_precision(simflat_t *s, int precision);

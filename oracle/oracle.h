@@ -65,6 +65,11 @@ typedef struct orc_pot {
     int eval;  /* ORC_EVAL_TABLE or ORC_EVAL_CHEB */
     /* LJ parameters (src-flat/ljForce.c:75-86) */
     double sigma, epsilon;
+    /* sign of the stored LJ force: +1 keeps src-flat's -grad U
+     * (ljForce.c:204-208); -1 stores +grad U, the EAM convention, under
+     * which the integrator's subtraction (timestep.c:71) gives physical
+     * motion.  A test-only variant, set by orc_pot_set_lj_sign. */
+    double lj_sign;
 } orc_pot;
 
 /* The simulation in the reference's slot layout
@@ -106,6 +111,7 @@ enum {
 /* ---- potentials ---- */
 int  orc_pot_read_eam(orc_pot *pot, const char *dir, const char *name);
 void orc_pot_set_lj(orc_pot *pot, double sigma, double epsilon, double cutoff, double mass);
+void orc_pot_set_lj_sign(orc_pot *pot, double sign);
 void orc_pot_free(orc_pot *pot);
 void orc_pot_set_eval(orc_pot *pot, int eval);
 double orc_pot_get(const orc_pot *pot, int which); /* 0 cutoff, 1 mass, 2 lat */
@@ -126,6 +132,7 @@ int  orc_neighbor_boxes(const orc_sim *s, int ibox, int *nbrs /* 27 */);
 /* ---- forces ---- */
 int  orc_eam_force(orc_sim *s);
 int  orc_lj_force(orc_sim *s);
+void orc_virial_tensor(const orc_sim *s, double *out);
 int  orc_compute_force(orc_sim *s);
 
 /* ---- timestep ---- */

@@ -39,6 +39,7 @@ def lib():
         L.orc_pot_delete.argtypes = [vp]
         L.orc_pot_read_eam.argtypes = [vp, C.c_char_p, C.c_char_p]
         L.orc_pot_set_lj.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double]
+        L.orc_pot_set_lj_sign.argtypes = [vp, C.c_double]
         L.orc_pot_fit_cheb.argtypes = [vp, C.c_int]
         L.orc_pot_set_eval.argtypes = [vp, C.c_int]
         L.orc_pot_get.argtypes = [vp, C.c_int]
@@ -73,6 +74,7 @@ def lib():
         L.orc_lj_force.argtypes = [vp]
         L.orc_pot_eval.argtypes = [vp, C.c_int, C.c_double, _dp, _dp]
         L.orc_pot_set_cheb3.argtypes = [vp, _dp, C.c_int]
+        L.orc_virial_tensor.argtypes = [vp, _dp]
         L.orc_neighbor_boxes.restype = C.c_int
         _LIB = L
     return _LIB
@@ -106,6 +108,11 @@ class OraclePot:
 
     def set_eval(self, cheby):
         lib().orc_pot_set_eval(self.h, 1 if cheby else 0)
+
+    def set_lj_physical(self, on):
+        """LJ forces stored as +grad U (the EAM convention) instead of the
+        reference's -grad U; the integrator then gives physical motion."""
+        lib().orc_pot_set_lj_sign(self.h, -1.0 if on else 1.0)
 
     @property
     def cutoff(self):
@@ -226,6 +233,13 @@ class OracleSim:
 
     def kinetic(self):
         return lib().orc_kinetic_energy(self.h)
+
+    def virial(self):
+        """(W, W_pair): the full virial tensor and its pair part (3x3,
+        hartree) of the current positions, momenta and last F'."""
+        out = np.zeros(18)
+        lib().orc_virial_tensor(self.h, _p(out))
+        return out[:9].reshape(3, 3), out[9:].reshape(3, 3)
 
     def __del__(self):
         try:

@@ -124,16 +124,23 @@ static void copy_slot(orc_sim *s, int from, int to)
 }
 
 /* Append to the destination, then fill the hole with the source's last
- * atom, src-flat/domains.c:224-236 (abort when the destination reaches 64). */
-static void move_atom(orc_sim *s, int i, int ibox, int jbox)
+ * atom, src-flat/domains.c:224-236.  The reference aborts the process when
+ * the destination reaches 64 atoms (domains.c:229); here the error is
+ * recorded and the caller stops at once, so no slot past a cell is ever
+ * written. */
+static int move_atom(orc_sim *s, int i, int ibox, int jbox)
 {
     int nj = s->natoms[jbox];
     copy_slot(s, ibox * ORC_MAXATOMS + i, jbox * ORC_MAXATOMS + nj);
     s->natoms[jbox]++;
-    if (s->natoms[jbox] >= ORC_MAXATOMS && !s->err) s->err = ORC_ERR_OVERFLOW;
+    if (s->natoms[jbox] >= ORC_MAXATOMS) {
+        s->err = ORC_ERR_OVERFLOW;
+        return 1;
+    }
     s->natoms[ibox]--;
     int ni = s->natoms[ibox];
     if (ni) copy_slot(s, ibox * ORC_MAXATOMS + ni, ibox * ORC_MAXATOMS + i);
+    return 0;
 }
 
 /* Ascending scan of cells; an atom outside its cell moves by one cell per
@@ -141,6 +148,7 @@ static void move_atom(orc_sim *s, int i, int ibox, int jbox)
  * slot is re-examined after the swap, src-flat/domains.c:347-389. */
 void orc_rebox_all(orc_sim *s)
 {
+    if (s->err) return;
     for (int ibox = 0; ibox < s->nboxes; ibox++) {
         int old[3] = {ibox % s->nbx[0], (ibox / s->nbx[0]) % s->nbx[1], (ibox / s->nbx[0] / s->nbx[1]) % s->nbx[2]};
         int i = 0;
@@ -166,7 +174,7 @@ void orc_rebox_all(orc_sim *s)
                 off++;
             } else {
                 memcpy(&s->r[3 * off], rn, sizeof rn);
-                move_atom(s, i, ibox, jbox);
+                if (move_atom(s, i, ibox, jbox)) return;
             }
         }
     }

@@ -201,6 +201,9 @@ int orc_lj_force(orc_sim *s)
     etot = etot * 4.0 * epsilon * s6;
     s->e = etot;
     s->stress = s->stress / (s->bounds[0] * s->bounds[1] * s->bounds[2]);
+    if (pot->lj_sign < 0.0)
+        for (size_t o = 0; o < ns; o++)
+            for (int m = 0; m < 3; m++) s->f[4 * o + m] = -s->f[4 * o + m];
     return 0;
 }
 
@@ -208,4 +211,69 @@ int orc_lj_force(orc_sim *s)
 int orc_compute_force(orc_sim *s)
 {
     return s->pot->kind == ORC_POT_EAM ? orc_eam_force(s) : orc_lj_force(s);
+}
+
+/* Virial tensor of the current state, W_ab = sum_i p_ia p_ib / m +
+ * sum_{i<j} r_ij,a f_ij,b with the physical pair force on i from j (EAM:
+ * -(phi'(r) + (F'_i + F'_j) rho'(r)) r_ij / r with F' of the last force
+ * call; LJ: -U'(r) r_ij / r), over the half-shell pairs of eamForce's loops
+ * (src-flat/eam.c:484-519, 595-633: jBox >= iBox, same box jj > ii) with
+ * the cutoff predicates of passes 1 (phi) and 3 (rho'), or of LJ
+ * (ljForce.c:188).  The reference computes only its scalar "stress"; this
+ * is the tensor it approximates.  out: W row-major (9), then the pair part
+ * alone (9).  Needs at least 3 cells per dimension (no repeated stencil
+ * cells). */
+void orc_virial_tensor(const orc_sim *s, double *out)
+{
+    const orc_pot *pot = s->pot;
+    const double rc2 = pot->cutoff * pot->cutoff;
+    const double rmass = 1822.83 * pot->mass;  /* src-flat/timestep.c:90 */
+    double w[3][3] = {{0}}, k[3][3] = {{0}};
+    int nbr[27];
+    for (int ib = 0; ib < s->nboxes; ib++) {
+        int ni = s->natoms[ib];
+        for (int ii = 0; ii < ni; ii++) {
+            int io = ib * ORC_MAXATOMS + ii;
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++) k[a][b] += s->p[3 * io + a] * s->p[3 * io + b] / rmass;
+        }
+        int nn = orc_neighbor_boxes(s, ib, nbr);
+        for (int t = 0; t < nn; t++) {
+            int jb = nbr[t];
+            if (jb < ib) continue;
+            double drb[3];
+            cell_shift(s, ib, jb, drb);
+            int nj = s->natoms[jb];
+            for (int ii = 0; ii < ni; ii++) {
+                int io = ib * ORC_MAXATOMS + ii;
+                for (int jj = (jb == ib ? ii + 1 : 0); jj < nj; jj++) {
+                    int jo = jb * ORC_MAXATOMS + jj;
+                    double dr[3], r2 = 0.0;
+                    for (int m = 0; m < 3; m++) {
+                        dr[m] = drb[m] + s->r[3 * io + m] - s->r[3 * jo + m];
+                        r2 += dr[m] * dr[m];
+                    }
+                    if (r2 > rc2) continue;
+                    double kk;
+                    if (pot->kind == ORC_POT_LJ) {
+                        double s6 = pot->sigma * pot->sigma * pot->sigma * pot->sigma * pot->sigma * pot->sigma;
+                        double r2i = 1.0 / r2, r6 = r2i * r2i * r2i;
+                        kk = 4.0 * pot->epsilon * s6 * r6 * r2i * (12.0 * s6 * r6 - 6.0);
+                    } else {
+                        double r = sqrt(r2), v, dphi, drho = 0.0;
+                        orc_pot_eval(pot, 0, r, &v, &dphi);
+                        if (r2 < rc2) orc_pot_eval(pot, 1, r, &v, &drho);
+                        kk = -(dphi + (s->fi[io] + s->fi[jo]) * drho) / r;
+                    }
+                    for (int a = 0; a < 3; a++)
+                        for (int b = 0; b < 3; b++) w[a][b] += kk * dr[a] * dr[b];
+                }
+            }
+        }
+    }
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+            out[3 * a + b] = w[a][b] + k[a][b];
+            out[9 + 3 * a + b] = w[a][b];
+        }
 }

@@ -145,7 +145,10 @@ void orc_pot_set_lj(orc_pot *pot, double sigma, double epsilon, double cutoff, d
     pot->epsilon = epsilon;
     pot->cutoff = cutoff;
     pot->mass = mass;
+    pot->lj_sign = 1.0;
 }
+
+void orc_pot_set_lj_sign(orc_pot *pot, double sign) { pot->lj_sign = sign < 0.0 ? -1.0 : 1.0; }
 
 void orc_pot_free(orc_pot *pot)
 {

@@ -28,6 +28,7 @@ void orc_advance_velocity(orc_sim *s, double dt)
 /* r += (dt p)/rmass then rebin, src-flat/timestep.c:76-108 */
 void orc_advance_positions(orc_sim *s, double dt)
 {
+    if (s->err) return;
     double rmass = (double)(kAmuToMe * (double)s->pot->mass);
     for (int b = 0; b < s->nboxes; b++)
         for (int i = 0; i < s->natoms[b]; i++) {
@@ -37,17 +38,18 @@ void orc_advance_positions(orc_sim *s, double dt)
     orc_rebox_all(s);
 }
 
-/* src-flat/timestep.c:114-140: n DKD steps, then one more force call. */
+/* src-flat/timestep.c:114-140: n DKD steps, then one more force call.
+ * Stops after a rebin overflow (the reference exits there). */
 double orc_n_time_steps(orc_sim *s, int n, double dt)
 {
     dt = dt * kBohrPerAtuToAPerS;
-    for (int i = 0; i < n; i++) {
+    for (int i = 0; i < n && !s->err; i++) {
         orc_advance_positions(s, dt / 2.0);
         orc_compute_force(s);
         orc_advance_velocity(s, dt);
         orc_advance_positions(s, dt / 2.0);
     }
-    orc_compute_force(s);
+    if (!s->err) orc_compute_force(s);
     return s->e;
 }
 

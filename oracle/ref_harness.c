@@ -20,6 +20,15 @@
  * writes PREFIX.0.bin after the initial force and PREFIX.m.bin after each
  * further nTimeSteps(K) call; with --eam also PREFIX.pot.bin (tables and
  * Chebyshev coefficients).  --time prints the wall time of the step calls.
+ *
+ * Built with -DCOMD_DROPIN (oracle/_ref/ref_dropin, linked with
+ * comd_amd/libcomd_dropin.so) it also takes
+ *   --hip-force   store comd_hip_eam_force / comd_hip_lj_force in the
+ *                 potential's force slot (the edit of eam.c:394 /
+ *                 ljForce.c:79), so the reference's own computeForce and
+ *                 nTimeSteps call the GPU force;
+ *   --hip-steps   call comd_hip_nTimeSteps instead of nTimeSteps (the edit
+ *                 of docompute.c:207): the whole step runs on the GPU.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -31,6 +40,10 @@
 #include "ic_fcc.h"
 #include "read.h"
 #include "eam.h"
+
+#ifdef COMD_DROPIN
+#include "comd_dropin.h"
+#endif
 
 extern int PERIODIC;
 
@@ -102,7 +115,7 @@ int main(int argc, char **argv)
     memset(&cmd, 0, sizeof cmd);
     cmd.bf = 1.0;
     cmd.defgrad = 1.0;
-    int doeam = 0, dolj = 0, dofcc = 0, steps = 10, dumps = 0, dotime = 0;
+    int doeam = 0, dolj = 0, dofcc = 0, steps = 10, dumps = 0, dotime = 0, hipforce = 0, hipsteps = 0;
     double ljs = 0, lje = 0, ljc = 0, ljm = 0;
     const char *prefix = NULL;
     char potdir[1024] = "", potname[1024] = "";
@@ -133,6 +146,12 @@ int main(int argc, char **argv)
             dotime = 1;
         } else if (!strcmp(argv[a], "--out") && a + 1 < argc) {
             prefix = argv[++a];
+#ifdef COMD_DROPIN
+        } else if (!strcmp(argv[a], "--hip-force")) {
+            hipforce = 1;
+        } else if (!strcmp(argv[a], "--hip-steps")) {
+            hipforce = hipsteps = 1;
+#endif
         } else {
             fprintf(stderr, "bad argument %s\n", argv[a]);
             return 2;
@@ -155,6 +174,9 @@ int main(int argc, char **argv)
         lj->mass = ljm;
         pot = (struct pmd_base_potential_t *)lj;
     }
+#ifdef COMD_DROPIN
+    if (hipforce) pot->force = doeam ? comd_hip_eam_force : comd_hip_lj_force;
+#endif
     simflat_t *sim = cmd.filename[0] ? fromFileGzip(cmd, pot) : create_fcc_lattice(cmd, pot);
     if (!sim) { fprintf(stderr, "no simulation\n"); return 3; }
     if (doeam) {
@@ -168,9 +190,19 @@ int main(int argc, char **argv)
     if (prefix) dump_state(sim, prefix, 0);
     double t0 = now();
     for (int m = 1; m <= dumps; m++) {
-        nTimeSteps(steps, sim, 1.0e-15);
+#ifdef COMD_DROPIN
+        if (hipsteps) comd_hip_nTimeSteps(steps, sim, 1.0e-15);
+        else
+#endif
+            nTimeSteps(steps, sim, 1.0e-15);
         if (prefix) dump_state(sim, prefix, m);
     }
+#ifdef COMD_DROPIN
+    if (comd_host_status()) {
+        fprintf(stderr, "drop-in error %d: %s\n", comd_host_status(), comd_host_error());
+        return 4;
+    }
+#endif
     double t1 = now();
     /* atom-steps/s excludes the extra force of each nTimeSteps call
      * (src-flat/timestep.c:136), estimated by the timed initial force */

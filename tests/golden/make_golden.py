@@ -14,9 +14,12 @@ nTimeSteps(K) call as a compressed .npz:
     nbx, bounds, boxsize, steps, dumps, cmd
 
 plus cu500_pot.npz with the table headers, guard slots and the 6 x 32
-Chebyshev coefficients exactly as the reference computed them.  The
-fixtures are data; the tests compare the oracle (and through it the HIP
-path) with them without the reference being present.
+Chebyshev coefficients exactly as the reference computed them, and
+ref_layout.json with the size and field offsets of the structs that cross
+the drop-in boundary, as the reference's headers lay them out
+(oracle/ref_layout.c).  The fixtures are data; the tests compare the oracle
+(and through it the HIP path) with them without the reference being
+present.  The reference's dumps are written to a temporary directory.
 
 Usage: python tests/golden/make_golden.py
 """
@@ -25,6 +28,7 @@ import shlex
 import struct
 import subprocess
 import sys
+import tempfile
 
 import numpy as np
 
@@ -38,7 +42,7 @@ import oracle_py as O  # noqa: E402  (dump reader)
 
 DATA = os.path.join(HERE, "data")
 POTS = os.path.join(ROOT, "comd_amd", "pots")
-WORK = os.path.join(ROOT, "_work", "golden_tmp")
+WORK = None  # temporary directory for the reference's dumps (set by main)
 
 LJ_SIGMA, LJ_EPS, LJ_CUT, LJ_MASS = 2.315, 0.167 / 27.211386, 2.5 * 2.315, 63.55
 LJ_LAT = 2.0 ** 0.5 * 2.0 ** (1.0 / 6.0) * LJ_SIGMA
@@ -103,10 +107,23 @@ def read_pot_bin(path):
 
 
 def main():
+    global WORK
     if not os.path.isdir("/root/reference/src-flat"):
         sys.exit("the reference tree is not present; the committed fixtures stay as they are")
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "comd_amd"), "all"], check=True)
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
-    os.makedirs(WORK, exist_ok=True)
+    with tempfile.TemporaryDirectory(prefix="comd_golden_") as tmp:
+        WORK = tmp
+        write_fixtures()
+
+
+def write_fixtures():
+    # struct layouts of the reference's headers
+    lay = subprocess.run([os.path.join(ROOT, "oracle", "_ref", "ref_layout")], check=True, capture_output=True,
+                         text=True).stdout
+    with open(os.path.join(HERE, "ref_layout.json"), "w") as fp:
+        fp.write(lay)
+    print("wrote ref_layout.json")
     tab, cheb = O.ref_binary(False), O.ref_binary(True)
     cu = ["--eam", POTS, "cu500"]
     lj = ["--lj", repr(LJ_SIGMA), repr(LJ_EPS), repr(LJ_CUT), repr(LJ_MASS)]

@@ -113,6 +113,39 @@ def test_batch_graph_steps_equal_single_runs(gpu, cu500):
         b.energy()
 
 
+def test_batch_multi_hop_in_one_replica(gpu, cu500, ocu500):
+    """A rebin pass in which an atom of one replica must move twice: that
+    replica's pass is replayed serially, the others' take the parallel
+    path; every replica ends in the oracle's arrangement."""
+    specs = [dict(kind="eam", n=6, stretch=1.0, seed=31), dict(kind="eam", n=7, stretch=1.05, seed=32),
+             dict(kind="eam", n=6, stretch=1.02, seed=33)]
+    b = make_batch(specs, cu500)
+    b.rebox()
+    oracles = []
+    for r, sp in enumerate(specs):
+        _, xyz, p, bounds, _ = build(sp, cu500)
+        o = O.OracleSim.from_atoms(ocu500, bounds, xyz, p)
+        o.rebox()
+        oracles.append(o)
+    # replica 1: one atom two cells up in x, rebinned in the same pass as
+    # the drift of every replica's atoms
+    st = b.download(1)
+    bs = oracles[1].info()["boxsize"]
+    st["r"][3 * 64, 0] += 2.2 * bs[0]
+    b.upload(1, st["natoms"], st["id"], st["itype"], st["r"], st["p"])
+    oracles[1].set_state(st)
+    dt = 30 * 1.0e-15 * CA.BOHR_PER_ATU_TO_A_PER_S
+    b.advance_positions(dt)
+    for o in oracles:
+        o.advance_positions(dt)
+    for r, o in enumerate(oracles):
+        g, oc = b.download(r), o.state()
+        assert np.array_equal(g["natoms"], oc["natoms"]), f"replica {r}: cell counts"
+        gc, occ = compact(g), compact(oc)
+        assert np.array_equal(gc["id"], occ["id"]), f"replica {r}: slot order"
+        assert np.array_equal(gc["r"], occ["r"]), f"replica {r}: positions"
+
+
 def test_batch_replica_against_oracle(gpu, cu500, ocu500):
     """One EAM and one LJ replica of a batch, 10 steps, against the oracle."""
     specs = [dict(kind="eam", n=6, stretch=1.04, seed=7), dict(kind="lj", n=5, stretch=1.08, seed=8)]

@@ -94,9 +94,15 @@ def test_drift_rebox_bit_exact(gpu, cu500, ocu500):
 
 def test_rebox_multi_hop_serial_pass(gpu, cu500, ocu500):
     """An atom that must move twice in one pass (arrives in a higher cell
-    still outside it) takes the serial path, which must match reBoxAll."""
-    xyz, p, bounds = seeded_atoms(8, 3.615, 63.55, temp=0.0, disp=0.0, seed=1)
+    still outside it) takes the serial path, which must match reBoxAll; the
+    neighbour lists it keeps up to date (list words and the id -> slot map
+    moved with the atoms) then give the oracle's forces, and a few steps
+    stay with the oracle."""
+    xyz, p, bounds = seeded_atoms(8, 3.615, 63.55, temp=300.0, disp=0.05, seed=1)
     sim, osim = make_pair(cu500, ocu500, xyz, p, bounds)
+    sim.rebox()
+    osim.rebox()
+    sim.force()  # lists built before the multi-hop pass
     st = osim.state()
     bs = osim.info()["boxsize"]
     c = 5
@@ -108,6 +114,15 @@ def test_rebox_multi_hop_serial_pass(gpu, cu500, ocu500):
     osim.rebox()
     gc, oc = assert_same_layout(sim.download(), osim.state(), "multi-hop")
     assert np.array_equal(gc["r"], oc["r"])
+    assert sim.list_info()[0] == 1  # the list path is the one checked below
+    sim.force()
+    osim.force()
+    check_forces(sim.download(), osim.state(), FP64_TOL, "multi-hop force")
+    for step in range(3):
+        sim.nsteps(1)
+        osim.nsteps(1)
+        check_forces(sim.download(), osim.state(), FP64_TOL, f"multi-hop step {step}")
+        assert abs(sim.energy()[0] - osim.info()["e"]) <= 1e-10 * abs(osim.info()["e"])
 
 
 def test_overflow_reported(gpu, cu500):

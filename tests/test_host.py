@@ -26,11 +26,11 @@ def declared_functions(header):
     return sorted(names)
 
 
-@pytest.mark.parametrize("header,lib", [("comd_hip.h", 0), ("comd_host.h", 1)])
+@pytest.mark.parametrize("header,lib", [("comd_hip.h", 0), ("comd_host.h", 1), ("comd_dropin.h", 2)])
 def test_libraries_export_every_declared_function(header, lib):
     names = declared_functions(header)
-    assert len(names) > 10
-    handle = CA.libraries()[lib]
+    assert len(names) > (10 if lib < 2 else 4)
+    handle = (CA.libraries() + (CA.dropin(),))[lib]
     missing = [n for n in names if not hasattr(handle, n)]
     assert not missing, f"{header}: not exported by the library: {missing}"
 
@@ -142,3 +142,99 @@ def test_grid_rejects_box_smaller_than_cutoff(cu500):
     prm = CA.Params()
     cu500.fill(prm)
     assert hip.comd_grid_for(C.byref(prm), (C.c_double * 3)(3.0, 20.0, 20.0), 1.0, 1.0) == 6
+
+
+def test_struct_layouts_match_reference():
+    """The structs that cross the drop-in boundary have the reference's
+    size and field offsets (fixture: oracle/ref_layout.c compiled against
+    the reference's headers; here against include/comd_host.h)."""
+    import json
+    import subprocess
+    import tempfile
+
+    ref = json.load(open(os.path.join(GOLDEN, "ref_layout.json")))
+    with tempfile.TemporaryDirectory() as tmp:
+        exe = os.path.join(tmp, "layout")
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", exe,
+                        os.path.join(ROOT, "oracle", "ref_layout.c")], check=True)
+        ours = json.loads(subprocess.run([exe], check=True, capture_output=True, text=True).stdout)
+    assert ours == ref
+
+
+def test_parse_command_line_keeps_reference_command_t():
+    """parseCommandLine fills the reference's command_t (mycommand.h:60-75)
+    and returns nothing, as the reference's does; this library's extra
+    options go to the driver settings, not into command_t."""
+    _, host = CA.libraries()
+    host.comd_host_run_options.restype = C.c_void_p
+    args = [b"comd", b"-e", b"-p", b"cu500", b"-x", b"5", b"-y", b"6", b"-z", b"7", b"-t", b"300", b"-s", b"1.05",
+            b"--precision", b"mixed", b"--nsteps", b"4", b"--seed", b"9"]
+    argv = (C.c_char_p * len(args))(*args)
+    cmd = CA.CommandT()
+    assert C.sizeof(cmd) == json_size("command_t")
+    assert host.parseCommandLine(C.byref(cmd), len(args), argv) is None
+    assert host.comd_host_status() == 0
+    assert (cmd.doeam, cmd.nx, cmd.ny, cmd.nz, cmd.periodic) == (1, 5, 6, 7, 1)
+    assert (cmd.temp, cmd.defgrad, cmd.bf) == (300.0, 1.05, 1.0)
+    assert cmd.potname == b"cu500" and cmd.potdir == b"pots"
+    # driver settings: precision, cheby, seed, disp, niter, nsteps, ...
+    opts = (C.c_int * 2).from_address(host.comd_host_run_options())
+    assert opts[0] == CA.PREC_MIXED
+    args = [b"comd", b"--bogus"]
+    host.parseCommandLine(C.byref(cmd), 2, (C.c_char_p * 2)(*args))
+    assert host.comd_host_status() != 0
+
+
+def json_size(name):
+    import json
+
+    return json.load(open(os.path.join(GOLDEN, "ref_layout.json")))[name]["size"]
+
+
+def test_create_fcc_lattice_reference_semantics():
+    """create_fcc_lattice takes a reference-layout command_t by value and
+    builds the reference's 0 K lattice (ic_fcc.c:63-141: type 1, ids in
+    creation order, atoms at rest; cmd.temp unused), the oracle's state."""
+    _, host = CA.libraries()
+    pot = host.setEamPot(POTS.encode(), b"cu500")
+    cmd = CA.CommandT()
+    cmd.nx, cmd.ny, cmd.nz = 5, 5, 5
+    cmd.lat, cmd.defgrad, cmd.bf, cmd.temp = 3.615, 1.07, 1.0, 600.0
+    cmd.periodic = 1
+    sp = host.create_fcc_lattice(cmd, C.cast(pot, C.c_void_p))
+    assert sp
+    s = sp.contents
+    nb = s.nboxes
+    natoms = np.ctypeslib.as_array(s.natoms, shape=(nb,)).copy()
+    ids = np.ctypeslib.as_array(s.id, shape=(nb * 64,)).copy()
+    typ = np.ctypeslib.as_array(s.iType, shape=(nb * 64,)).copy()
+    r = np.ctypeslib.as_array(s.r, shape=(nb * 64, 3)).copy()
+    p = np.ctypeslib.as_array(s.p, shape=(nb * 64, 3)).copy()
+    o = O.OracleSim.fcc(O.OraclePot.eam(POTS, "cu500"), 5, 3.615, defgrad=1.07)
+    so = o.state()
+    m = (np.arange(64)[None, :] < natoms[:, None]).reshape(-1)
+    np.testing.assert_array_equal(natoms, so["natoms"])
+    np.testing.assert_array_equal(ids[m], so["id"][m])
+    np.testing.assert_array_equal(r[m], so["r"][m])
+    assert (typ[m] == 1).all()
+    assert (p[m] == 0.0).all()
+    assert s.ntot == 500 and tuple(s.bounds) == tuple(o.info()["bounds"])
+    host.destroySimulation(C.byref(sp))
+
+
+def test_dropin_library_exports_and_links_no_reference_names():
+    """libcomd_dropin.so exports what include/comd_dropin.h declares and
+    defines none of the reference's function names (so a reference build
+    can link it beside its own objects)."""
+    import subprocess
+
+    names = declared_functions("comd_dropin.h")
+    assert {"comd_hip_eam_force", "comd_hip_lj_force", "comd_hip_nTimeSteps", "comd_hip_reBoxAll"} <= set(names)
+    d = CA.dropin()
+    assert all(hasattr(d, n) for n in names)
+    syms = subprocess.run(["nm", "-D", "--defined-only", os.path.join(ROOT, "comd_amd", "libcomd_dropin.so")],
+                          check=True, capture_output=True, text=True).stdout.split()
+    reference_names = {"eamForce", "LJ", "computeForce", "nTimeSteps", "reBoxAll", "setEamPot", "getLJPot",
+                       "allocDomains", "putAtomInBox", "create_fcc_lattice", "fromFileGzip", "parseCommandLine",
+                       "initSimFromCmdLine", "do_compute_work", "setChebPot", "blankSimulation"}
+    assert not reference_names & set(syms)

@@ -20,6 +20,7 @@ import os
 import numpy as np
 import pytest
 
+import comd_amd as CA
 import oracle_py as O
 from helpers import GOLDEN, POTS, DT_INTERNAL
 
@@ -492,3 +493,115 @@ def test_lj_reference_convention_is_unstable():
     assert ke0 > 0.0
     s.nsteps(40)
     assert s.info()["err"] != 0 or s.kinetic() > 3.0 * ke0
+
+
+# ---------------------------------------------------------------------------
+# virial tensor (orc_virial_tensor): pinned by the strain derivative of the
+# oracle's own energy.  Under the homogeneous deformation x -> (I + e) x,
+# dU/de_ab = sum_i (dU/dx_i,a) x_i,b = -sum_{i<j} f_ij,a r_ij,b, so the pair
+# part of the virial is W_pair,ab = -dU/de_ba; the kinetic part has trace
+# 2 KE.  A non-periodic cluster takes any strain (shear included); a
+# periodic box takes the diagonal ones.  EAM is checked in Chebyshev form:
+# its derivative series are the exact derivatives of its value series
+# (src-flat/cheby.c:219-270), whereas the table form interpolates the
+# derivative separately from the value (src-flat/eam.c:418-433), so that
+# its forces differ from the gradient of its energy by ~1e-3 relative
+# (test_table_forces_are_not_the_energy_gradient) and so would a strain
+# derivative from the virial.
+
+def _strained_energy(opot, xyz, bounds, eps, periodic, centre):
+    x = centre + (xyz - centre) @ (np.eye(3) + eps).T
+    b = tuple(bounds[k] * (1.0 + eps[k, k]) for k in range(3)) if periodic else bounds
+    s = O.OracleSim.from_atoms(opot, b, x, None, periodic=periodic)
+    assert s.info()["err"] == 0
+    s.rebox()
+    s.force()
+    return s.info()["e"]
+
+
+@pytest.mark.parametrize("kind,periodic", [("eam", False), ("eam", True), ("lj", False), ("lj", True)])
+def test_oracle_virial_is_strain_derivative(kind, periodic):
+    if kind == "eam":
+        opot = O.OraclePot.eam(POTS, "cu500")
+        opot.set_eval(True)
+        lat, mass = 3.615, 63.55
+    else:
+        pot = CA.copper_lj()
+        opot = O.OraclePot.lj(pot.sigma, pot.epsilon, pot.cutoff, pot.mass)
+        lat, mass = pot.lat, pot.mass
+    n = 4 if not periodic else 5
+    xyz, p = CA.seed_state(CA.fcc_positions(n, n, n, lat), mass, 300.0, 0.1, 5)
+    if periodic:
+        bounds = (n * lat,) * 3
+        centre = np.zeros(3)
+    else:
+        bounds = (50.0, 50.0, 50.0)
+        xyz = xyz + (25.0 - 0.5 * n * lat)
+        centre = np.full(3, 25.0)
+    s = O.OracleSim.from_atoms(opot, bounds, xyz, p, periodic=periodic)
+    assert min(s.info()["nbx"]) >= 3
+    s.rebox()
+    s.force()
+    w, wpair = s.virial()
+    st = s.state()
+    m = (np.arange(64)[None, :] < st["natoms"][:, None]).reshape(-1)
+    pp = st["p"][m]
+    kin = pp.T @ pp / (CA.AMU_TO_ME * mass)
+    np.testing.assert_allclose(w - wpair, kin, rtol=1e-12, atol=1e-12 * np.abs(kin).max())
+    assert abs(np.trace(kin) - 2.0 * s.kinetic()) <= 1e-12 * s.kinetic()
+    np.testing.assert_allclose(wpair, wpair.T, rtol=0, atol=1e-12 * np.abs(wpair).max())
+    # two small steps: neither potential vanishes at its cutoff (the fitted
+    # series and the unshifted LJ), so a pair that crosses it within a step
+    # spoils that difference; one of the two must match
+    comps = [(a, b) for a in range(3) for b in range(3)] if not periodic else [(0, 0), (1, 1), (2, 2)]
+    for a, b in comps:
+        fds = []
+        for h in (1e-7, 3e-8):
+            e = np.zeros((3, 3))
+            e[a, b] = h
+            up = _strained_energy(opot, xyz, bounds, e, periodic, centre)
+            dn = _strained_energy(opot, xyz, bounds, -e, periodic, centre)
+            fds.append(-(up - dn) / (2 * h))
+        err = min(abs(fd - wpair[b, a]) for fd in fds)
+        assert err <= 1e-6 * np.abs(wpair).max(), (kind, periodic, a, b, fds, wpair[b, a])
+
+
+def test_table_forces_are_not_the_energy_gradient():
+    """The reference's table interpolation (src-flat/eam.c:401-434) takes
+    the derivative from a linear interpolation of central differences, not
+    from the quadratic it uses for the value: its EAM forces differ from
+    the gradient of its energy by about 1e-3 of the force (in Chebyshev form
+    they agree to the finite difference's accuracy).  Both halves are
+    pinned here; parity with the reference keeps its forces."""
+    n, lat = 4, 3.615
+    xyz, _ = CA.seed_state(CA.fcc_positions(n, n, n, lat), 63.55, 0.0, 0.1, 5)
+    bounds = (50.0, 50.0, 50.0)
+    xyz = xyz + (25.0 - 0.5 * n * lat)
+    rel = {}
+    for cheby in (False, True):
+        opot = O.OraclePot.eam(POTS, "cu500")
+        opot.set_eval(cheby)
+
+        def energy(x):
+            t = O.OracleSim.from_atoms(opot, bounds, x, None, periodic=False)
+            t.rebox()
+            t.force()
+            return t.info()["e"]
+
+        s = O.OracleSim.from_atoms(opot, bounds, xyz, None, periodic=False)
+        s.rebox()
+        s.force()
+        st = s.state()
+        m = (np.arange(64)[None, :] < st["natoms"][:, None]).reshape(-1)
+        ids, f = st["id"][m], st["f"][m]
+        errs = []
+        for a in range(0, len(xyz), 37):
+            k = np.flatnonzero(ids == a)[0]
+            for c in range(3):
+                xp, xm = xyz.copy(), xyz.copy()
+                xp[a, c] += 1e-6
+                xm[a, c] -= 1e-6
+                errs.append(abs((energy(xp) - energy(xm)) / 2e-6 - f[k, c]) / np.abs(f[:, :3]).max())
+        rel[cheby] = max(errs)
+    assert rel[True] < 1e-6
+    assert 1e-5 < rel[False] < 1e-2
