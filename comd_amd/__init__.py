@@ -177,6 +177,8 @@ def _load():
     hip.comd_observe.argtypes = [vp, C.POINTER(Observables)]
     hip.comd_batch_observe.argtypes = [vp, C.c_int, C.POINTER(Observables)]
     hip.comd_run_monitored.argtypes = [vp, C.c_int, C.c_int, C.c_double, _dp]
+    hip.comd_snapshot_save.argtypes = [vp, C.c_char_p]
+    hip.comd_snapshot_load.argtypes = [vp, C.c_char_p]
     host.setEamPot.argtypes = [C.c_char_p, C.c_char_p]
     host.setEamPot.restype = C.POINTER(EamPotT)
     host.setChebPot.argtypes = [C.POINTER(EamPotT), C.c_int]
@@ -573,6 +575,15 @@ class Simulation:
         out = np.zeros(2 * (nblocks + 1))
         self._check(self._hip.comd_run_monitored(self.ctx, int(nblocks), int(block_steps), float(dt), _ptr(out)))
         return out[0::2].copy(), out[1::2].copy()
+
+    def snapshot_save(self, path):
+        """The whole device state, lists included, to a file (comd_snapshot_save)."""
+        self._check(self._hip.comd_snapshot_save(self.ctx, str(path).encode()))
+
+    def snapshot_load(self, path):
+        """Restore a snapshot into this context (same parameters); the run
+        then continues bit for bit (comd_snapshot_load)."""
+        self._check(self._hip.comd_snapshot_load(self.ctx, str(path).encode()))
 
     def set_cheb(self, coef):
         """Replace the phi, rho, F value series (3 x n) and rebuild the

@@ -24,6 +24,10 @@
 extern "C" {
 #endif
 
+/* The library is built with hidden visibility; these entry points are its
+ * only exported symbols. */
+#pragma GCC visibility push(default)
+
 #define COMD_MAXATOMS 64 /* slots per link cell, src-flat/include/pmdTypes.h:57 */
 
 enum comd_status {
@@ -280,6 +284,19 @@ int comd_batch_observe(comd_ctx *ctx, int r, comd_observables *out);
  * 2 (nblocks + 1) doubles (pe, ke) per record.  Single simulations only. */
 int comd_run_monitored(comd_ctx *ctx, int nblocks, int block_steps, double dt_seconds, double *pe_ke);
 
+/* ---- Snapshots ----
+ *
+ * The whole device state of a single simulation -- slot arrays (natoms,
+ * id, iType, r, p, the last forces, rho, F', virial terms) and the
+ * neighbour lists with their build state -- written to a file and read
+ * back into a context created with the same parameters, so that the run
+ * continues bit for bit as if it had not stopped.  Load applies the
+ * snapshot's precision, evaluation, skin and LJ sign to the context.  The
+ * reference has no restart (its clsman and gz writers are commented out of
+ * its driver, src-flat/pmd.c:117,123, and the gz text is %.8lf). */
+int comd_snapshot_save(comd_ctx *ctx, const char *path);
+int comd_snapshot_load(comd_ctx *ctx, const char *path);
+
 /* ---- Batches: many independent replicas in one context ----
  *
  * No counterpart in the reference (a src-flat process holds one simulation;
@@ -342,6 +359,8 @@ int comd_eam_cheb_grad(comd_ctx *ctx, const double *coef, int n, const double *f
  * reference. */
 int comd_batch_cheb_grad(comd_ctx *ctx, const double *coef, int n, const double *f_target, const double *e_target,
                          double w_e, double *loss, double *grad);
+
+#pragma GCC visibility pop
 
 #ifdef __cplusplus
 }

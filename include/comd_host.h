@@ -163,6 +163,15 @@ void destroySimulation(simflat_t **ps);
 simflat_t *create_fcc_lattice(command_t cmd, pmd_base_potential_t *pot);
 simflat_t *fromFileGzip(command_t cmd, pmd_base_potential_t *pot);
 
+/* ---- output files (src-flat/write.c:112-220) ---- */
+/* The reference's writers: clsman binary (Fortran records: natoms, an empty
+ * comment, the bounds, then x y z type px py pz per atom in id order) and
+ * the gz text read by fromFileGzip (%.8lf).  Positions are absolute (cell
+ * corner + r), types are written as 1.  Ids must be 0 .. ntot-1; errors go
+ * to comd_host_status() (the reference writes to stderr or exits). */
+void writeClsman(simflat_t *s, char *fname);
+void gzASCIIWrite(simflat_t *s, char *fname);
+
 /* ---- hot path on the GPU ---- */
 int eamForce(void *s);   /* force slot for EAM potentials, src-flat/eam.c:438 */
 int LJ(void *s);         /* force slot for LJ, src-flat/ljForce.c:89 */
@@ -220,6 +229,10 @@ int comd_write_gz(const char *path, int n, const double bounds[3], const double 
 /* Read the gz input format into arrays (n atoms, bounds); caller frees
  * *xyz and *pxyz with free(). */
 int comd_read_gz(const char *path, int *n, double bounds[3], double **xyz, double **pxyz);
+/* Read a clsman file (writeClsman's format) into arrays, as comd_read_gz.
+ * No counterpart in the reference, which writes the format but has no
+ * reader for it. */
+int comd_read_clsman(const char *path, int *n, double bounds[3], double **xyz, double **pxyz);
 
 #ifdef __cplusplus
 }

@@ -133,6 +133,8 @@ int  orc_neighbor_boxes(const orc_sim *s, int ibox, int *nbrs /* 27 */);
 int  orc_eam_force(orc_sim *s);
 int  orc_lj_force(orc_sim *s);
 void orc_virial_tensor(const orc_sim *s, double *out);
+/* Centrosymmetry parameter per slot (src-viz/AtomVisualize.c:399-673). */
+void orc_centrosymmetry(const orc_sim *s, double *out);
 int  orc_compute_force(orc_sim *s);
 
 /* ---- timestep ---- */

@@ -75,6 +75,7 @@ def lib():
         L.orc_pot_eval.argtypes = [vp, C.c_int, C.c_double, _dp, _dp]
         L.orc_pot_set_cheb3.argtypes = [vp, _dp, C.c_int]
         L.orc_virial_tensor.argtypes = [vp, _dp]
+        L.orc_centrosymmetry.argtypes = [vp, _dp]
         L.orc_neighbor_boxes.restype = C.c_int
         _LIB = L
     return _LIB
@@ -233,6 +234,13 @@ class OracleSim:
 
     def kinetic(self):
         return lib().orc_kinetic_energy(self.h)
+
+    def centrosymmetry(self):
+        """Centrosymmetry parameter per slot (A^2; 1000 for atoms with fewer
+        than 12 neighbours found)."""
+        out = np.zeros(self.info()["nboxes"] * MAXATOMS)
+        lib().orc_centrosymmetry(self.h, _p(out))
+        return out
 
     def virial(self):
         """(W, W_pair): the full virial tensor and its pair part (3x3,

@@ -16,10 +16,12 @@
  *   ref_harness (--eam DIR NAME | --lj SIGMA EPS CUTOFF MASS)
  *               (--fcc NX NY NZ LAT DEFGRAD | --input FILE)
  *               [--bf BF] [--noperiodic] [--steps K] [--dumps M]
- *               [--time] --out PREFIX
+ *               [--time] [--clsman FILE] [--gzwrite FILE] --out PREFIX
  * writes PREFIX.0.bin after the initial force and PREFIX.m.bin after each
  * further nTimeSteps(K) call; with --eam also PREFIX.pot.bin (tables and
  * Chebyshev coefficients).  --time prints the wall time of the step calls.
+ * --clsman / --gzwrite write the final state with the reference's own
+ * writeClsman / gzASCIIWrite (src-flat/write.c:112-220).
  *
  * Built with -DCOMD_DROPIN (oracle/_ref/ref_dropin, linked with
  * comd_amd/libcomd_dropin.so) it also takes
@@ -46,6 +48,7 @@
 #endif
 
 extern int PERIODIC;
+extern void gzASCIIWrite(simflat_t *s, char *fname); /* src-flat/write.c:175 (no header declares it) */
 
 static double now(void)
 {
@@ -118,6 +121,7 @@ int main(int argc, char **argv)
     int doeam = 0, dolj = 0, dofcc = 0, steps = 10, dumps = 0, dotime = 0, hipforce = 0, hipsteps = 0;
     double ljs = 0, lje = 0, ljc = 0, ljm = 0;
     const char *prefix = NULL;
+    char *clsman = NULL, *gzwrite = NULL;
     char potdir[1024] = "", potname[1024] = "";
     PERIODIC = 1;
     for (int a = 1; a < argc; a++) {
@@ -146,6 +150,10 @@ int main(int argc, char **argv)
             dotime = 1;
         } else if (!strcmp(argv[a], "--out") && a + 1 < argc) {
             prefix = argv[++a];
+        } else if (!strcmp(argv[a], "--clsman") && a + 1 < argc) {
+            clsman = argv[++a];
+        } else if (!strcmp(argv[a], "--gzwrite") && a + 1 < argc) {
+            gzwrite = argv[++a];
 #ifdef COMD_DROPIN
         } else if (!strcmp(argv[a], "--hip-force")) {
             hipforce = 1;
@@ -204,6 +212,8 @@ int main(int argc, char **argv)
     }
 #endif
     double t1 = now();
+    if (clsman) writeClsman(sim, clsman);
+    if (gzwrite) gzASCIIWrite(sim, gzwrite);
     /* atom-steps/s excludes the extra force of each nTimeSteps call
      * (src-flat/timestep.c:136), estimated by the timed initial force */
     if (dotime)

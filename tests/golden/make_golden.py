@@ -149,12 +149,21 @@ def write_fixtures():
         cmd, _ = run(exe, cu + ["--input", two, "--steps", "10", "--dumps", "2"], p)
         collect(p, 2, f"two_atom_{tag}.npz", cmd, 10)
 
-    # seeded Cu 6^3 (864 atoms, 4^3 cells), sigma 0.1 A, 300 K
+    # seeded Cu 6^3 (864 atoms, 4^3 cells), sigma 0.1 A, 300 K; the table
+    # run also writes its final state with the reference's writeClsman and
+    # gzASCIIWrite (the bytes of the one, the text of the other)
     cu6 = seeded_input("cu6.inp.gz", 6, 3.615, 63.55, 300.0, 0.1, 7)
     for exe, tag in ((tab, "table"), (cheb, "cheby")):
         p = os.path.join(WORK, f"cu6_{tag}")
-        cmd, _ = run(exe, cu + ["--input", cu6, "--steps", "5", "--dumps", "2"], p)
+        extra = []
+        if tag == "table":
+            extra = ["--clsman", os.path.join(HERE, "cu6_final.clsman"), "--gzwrite", os.path.join(WORK, "cu6_final.gz")]
+        cmd, _ = run(exe, cu + ["--input", cu6, "--steps", "5", "--dumps", "2"] + extra, p)
         collect(p, 2, f"cu6_{tag}.npz", cmd, 5)
+    import gzip
+    with gzip.open(os.path.join(WORK, "cu6_final.gz"), "rb") as fi, open(os.path.join(HERE, "cu6_final.txt"), "wb") as fo:
+        fo.write(fi.read())
+    print("wrote cu6_final.clsman, cu6_final.txt")
 
     # the same, stretched 1.07 in x (a file input: the grid follows the
     # stretched bounds)

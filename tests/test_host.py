@@ -238,3 +238,68 @@ def test_dropin_library_exports_and_links_no_reference_names():
                        "allocDomains", "putAtomInBox", "create_fcc_lattice", "fromFileGzip", "parseCommandLine",
                        "initSimFromCmdLine", "do_compute_work", "setChebPot", "blankSimulation"}
     assert not reference_names & set(syms)
+
+
+def _simflat_from_fixture(host, fx, m):
+    """A host simflat_t on the grid of data/cu6.inp.gz (fromFileGzip) holding
+    state m of a reference fixture."""
+    pot = host.setEamPot(POTS.encode(), b"cu500")
+    cmd = CA.CommandT()
+    cmd.filename = os.path.join(GOLDEN, "data", "cu6.inp.gz").encode()
+    cmd.bf, cmd.defgrad, cmd.periodic = 1.0, 1.0, 1
+    sp = host.fromFileGzip(cmd, C.cast(pot, C.c_void_p))
+    s = sp.contents
+    nb = s.nboxes
+    natoms = np.ctypeslib.as_array(s.natoms, shape=(nb,))
+    ids = np.ctypeslib.as_array(s.id, shape=(nb * 64,))
+    r = np.ctypeslib.as_array(s.r, shape=(nb * 64, 3))
+    p = np.ctypeslib.as_array(s.p, shape=(nb * 64, 3))
+    na = fx[f"s{m}_natoms"]
+    natoms[:] = na
+    mask = (np.arange(64)[None, :] < na[:, None]).reshape(-1)
+    ids[mask] = fx[f"s{m}_id"]
+    r[mask] = fx[f"s{m}_r"]
+    p[mask] = fx[f"s{m}_p"]
+    return sp
+
+
+def test_writers_match_reference_output(tmp_path):
+    """writeClsman and gzASCIIWrite (src-flat/write.c:112-220) on the state
+    the reference reached after 10 steps of data/cu6.inp.gz: the clsman file
+    is byte-identical to the reference's, the gz text identical to its
+    text; comd_read_clsman reads the atoms back exactly."""
+    import gzip
+
+    _, host = CA.libraries()
+    host.writeClsman.argtypes = [C.POINTER(CA.SimflatT), C.c_char_p]
+    host.writeClsman.restype = None
+    host.gzASCIIWrite.argtypes = [C.POINTER(CA.SimflatT), C.c_char_p]
+    host.gzASCIIWrite.restype = None
+    host.comd_read_clsman.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.c_double * 3,
+                                      C.POINTER(C.POINTER(C.c_double)), C.POINTER(C.POINTER(C.c_double))]
+    fx = np.load(os.path.join(GOLDEN, "cu6_table.npz"))
+    sp = _simflat_from_fixture(host, fx, 2)
+    out_b = tmp_path / "final.clsman"
+    out_t = tmp_path / "final.gz"
+    host.writeClsman(sp, str(out_b).encode())
+    host.gzASCIIWrite(sp, str(out_t).encode())
+    assert host.comd_host_status() == 0
+    assert out_b.read_bytes() == open(os.path.join(GOLDEN, "cu6_final.clsman"), "rb").read()
+    assert gzip.open(out_t).read() == open(os.path.join(GOLDEN, "cu6_final.txt"), "rb").read()
+    # read back: absolute positions in id order, as fromFileGzip would place them
+    n = C.c_int()
+    b = (C.c_double * 3)()
+    xp, pp = C.POINTER(C.c_double)(), C.POINTER(C.c_double)()
+    assert host.comd_read_clsman(str(out_b).encode(), C.byref(n), b, C.byref(xp), C.byref(pp)) == 0
+    xyz = np.ctypeslib.as_array(xp, shape=(n.value, 3)).copy()
+    pm = np.ctypeslib.as_array(pp, shape=(n.value, 3)).copy()
+    s = sp.contents
+    bs = np.array(s.boxsize[:])
+    nbx = np.array(s.nbx[:])
+    na = fx["s2_natoms"]
+    cells = np.repeat(np.arange(len(na)), na)
+    corner = np.stack([cells % nbx[0], (cells // nbx[0]) % nbx[1], cells // (nbx[0] * nbx[1])], 1) * bs
+    order = np.argsort(fx["s2_id"])
+    np.testing.assert_array_equal(xyz, (corner + fx["s2_r"])[order])
+    np.testing.assert_array_equal(pm, fx["s2_p"][order])
+    host.destroySimulation(C.byref(sp))

@@ -605,3 +605,87 @@ def test_table_forces_are_not_the_energy_gradient():
         rel[cheby] = max(errs)
     assert rel[True] < 1e-6
     assert 1e-5 < rel[False] < 1e-2
+
+
+# ---------------------------------------------------------------------------
+# centrosymmetry (orc_centrosymmetry, src-viz/AtomVisualize.c:399-673)
+
+def _matchings(items):
+    """All perfect matchings of an even list, as lists of pairs (lowest first)."""
+    if not items:
+        return [[]]
+    a, rest = items[0], items[1:]
+    out = []
+    for i, b in enumerate(rest):
+        for m in _matchings(rest[:i] + rest[i + 1:]):
+            out.append([(a, b)] + m)
+    return out
+
+
+_PAIRINGS = np.array(_matchings(list(range(12))))  # (10395, 6, 2)
+
+
+def _brute_centro(xyz, bounds):
+    """Minimum-image 12 nearest neighbours by a full sort, then the smallest
+    sum over all pairings of |r_j + r_k|^2, in double."""
+    L = np.asarray(bounds)
+    out = np.empty(len(xyz))
+    for i in range(len(xyz)):
+        d = xyz - xyz[i]
+        d -= L * np.round(d / L)
+        r2 = (d ** 2).sum(1)
+        r2[i] = np.inf
+        nn = d[np.argsort(r2, kind="stable")[:12]]
+        s = ((nn[:, None, :] + nn[None, :, :]) ** 2).sum(-1)
+        out[i] = s[_PAIRINGS[:, :, 0], _PAIRINGS[:, :, 1]].sum(1).min()
+    return out
+
+
+def _centro_by_id(sim):
+    st = sim.state()
+    m = (np.arange(64)[None, :] < st["natoms"][:, None]).reshape(-1)
+    c = np.empty(m.sum())
+    c[st["id"][m]] = sim.centrosymmetry()[m]
+    return c
+
+
+def test_centrosymmetry_perfect_lattice_and_vacancy():
+    """Perfect FCC: 0 for every atom.  A vacancy in a lattice displaced by
+    0.02 A (so that the 12th neighbour of the vacancy's neighbours, one of
+    six second-shell atoms at equal distance in the perfect lattice, is not
+    a tie decided by scan order): its 12 neighbours stand out (about a^2/2
+    and more), the others stay small, all as the brute-force search finds."""
+    assert len(_PAIRINGS) == 10395
+    pot = O.OraclePot.eam(POTS, "cu500")
+    n, a = 5, 3.615
+    xyz = CA.fcc_positions(n, n, n, a)
+    bounds = (n * a,) * 3
+    s = O.OracleSim.from_atoms(pot, bounds, xyz)
+    assert np.abs(_centro_by_id(s)).max() < 1e-10
+    gone = 137
+    xd, _ = CA.seed_state(xyz, 63.55, 0.0, 0.02, 3)
+    xv = np.delete(xd, gone, axis=0)
+    s = O.OracleSim.from_atoms(pot, bounds, xv)
+    c = _centro_by_id(s)
+    ref = _brute_centro(xv, bounds)
+    d = np.delete(xyz, gone, axis=0) - xyz[gone]
+    d -= np.asarray(bounds) * np.round(d / np.asarray(bounds))
+    near = (d ** 2).sum(1) < (0.75 * a) ** 2
+    assert near.sum() == 12
+    assert c[near].min() > 20 * c[~near].max()
+    np.testing.assert_allclose(c, ref, rtol=1e-5, atol=2e-6)
+
+
+def test_centrosymmetry_matches_brute_force_thermal():
+    pot = O.OraclePot.eam(POTS, "cu500")
+    n, a = 5, 3.615
+    xyz, _ = CA.seed_state(CA.fcc_positions(n, n, n, a), 63.55, 0.0, 0.15, 9)
+    bounds = (n * a,) * 3
+    s = O.OracleSim.from_atoms(pot, bounds, xyz)
+    c = _centro_by_id(s)
+    ref = _brute_centro(xyz, bounds)
+    # the oracle sums each pairing in float and rounds image offsets to
+    # float, as the reference does: about 1e-6 A per coordinate across the
+    # box faces, 2 |r_j + r_k| 1e-6 in a parameter
+    np.testing.assert_allclose(c, ref, rtol=1e-5, atol=2e-6)
+    assert c.max() > 0.05
