@@ -177,6 +177,8 @@ def _load():
     hip.comd_observe.argtypes = [vp, C.POINTER(Observables)]
     hip.comd_batch_observe.argtypes = [vp, C.c_int, C.POINTER(Observables)]
     hip.comd_run_monitored.argtypes = [vp, C.c_int, C.c_int, C.c_double, _dp]
+    hip.comd_centrosymmetry.argtypes = [vp, _dp]
+    hip.comd_batch_centrosymmetry.argtypes = [vp, C.c_int, _dp]
     hip.comd_snapshot_save.argtypes = [vp, C.c_char_p]
     hip.comd_snapshot_load.argtypes = [vp, C.c_char_p]
     host.setEamPot.argtypes = [C.c_char_p, C.c_char_p]
@@ -576,6 +578,14 @@ class Simulation:
         self._check(self._hip.comd_run_monitored(self.ctx, int(nblocks), int(block_steps), float(dt), _ptr(out)))
         return out[0::2].copy(), out[1::2].copy()
 
+    def centrosymmetry(self):
+        """Centrosymmetry parameter per slot (A^2; 1000 for atoms with fewer
+        than 12 neighbours in reach), src-viz's defect measure
+        (comd_centrosymmetry)."""
+        out = np.zeros(self.nslots)
+        self._check(self._hip.comd_centrosymmetry(self.ctx, _ptr(out)))
+        return out
+
     def snapshot_save(self, path):
         """The whole device state, lists included, to a file (comd_snapshot_save)."""
         self._check(self._hip.comd_snapshot_save(self.ctx, str(path).encode()))
@@ -688,6 +698,13 @@ class Batch(Simulation):
         o = Observables()
         self._check(self._hip.comd_batch_observe(self.ctx, r, C.byref(o)))
         return o.as_dict()
+
+    def centrosymmetry_of(self, r):
+        """comd_batch_centrosymmetry of replica r (as Simulation.centrosymmetry)."""
+        _, Cn, _ = self.info(r)
+        out = np.zeros(Cn * MAXATOMS)
+        self._check(self._hip.comd_batch_centrosymmetry(self.ctx, r, _ptr(out)))
+        return out
 
     def energy_of(self, r):
         pe, ke, st = C.c_double(), C.c_double(), C.c_double()

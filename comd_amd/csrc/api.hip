@@ -24,7 +24,6 @@ static const double kAmuToMe = (double)1822.83;
 
 static char g_create_err[512] = "";
 
-
 struct TimedEvent {
     int cls;
     hipEvent_t a, b;
@@ -105,7 +104,7 @@ struct comd_ctx {
     int nl_invalid_seen; // the lists were invalid when the host last looked
     int nl_group_opt;   // COMD_TUNE_LIST_GROUPS (-1: by precision)
     int nl_rows_lds;    // COMD_TUNE_LIST_ROWS_LDS (-1: whenever they fit)
-    long nl_alloc_n, nl_alloc_k, nl_alloc_kout, nl_alloc_aux, nl_alloc_groups, nl_alloc_rep;
+    long nl_alloc_n, nl_alloc_k, nl_alloc_aux, nl_alloc_groups, nl_alloc_rep;
     int nl_eam_groups, nl_lj_groups;  // the list groups of the EAM replicas (first), then of the LJ ones
     int nl_eam_rep, nl_lj_rep;        // a replica of each kind (their potentials stand for the kind's)
     // one DKD step captured as a HIP graph (comd_steps)
@@ -438,14 +437,14 @@ int comd_grid_for(comd_params *p, const double bounds[3], double defgrad, double
 static void free_nl(comd_ctx *c)
 {
     void *p[] = {c->nl.pos,      c->nl.selfh,           c->nl.slot,    c->nl.order,    c->nl.fq,
-                 c->nl.nbl,      c->nl.ncnt,     c->nl.nbo,       c->nl.ocnt,    c->nl.aux_cnt,   c->nl.aux_n,   c->nl.aux_start,
+                 c->nl.nbl,      c->nl.ncnt,     c->nl.aux_cnt,   c->nl.aux_n,   c->nl.aux_start,
                  c->nl.aux_mem,  c->nl.gstart,   c->nl.nonempty,  c->nl.count,   c->nl.flags,
                  (void *)c->nl.cell_order, (void *)c->nl.group_q0, (void *)c->nl.group_rep,
                  (void *)c->nl.aux_group, (void *)c->nl.rep};
     for (void *q : p)
         if (q) hipFree(q);
     memset(&c->nl, 0, sizeof c->nl);
-    c->nl_alloc_n = c->nl_alloc_k = c->nl_alloc_kout = c->nl_alloc_aux = c->nl_alloc_groups = c->nl_alloc_rep = 0;
+    c->nl_alloc_n = c->nl_alloc_k = c->nl_alloc_aux = c->nl_alloc_groups = c->nl_alloc_rep = 0;
     c->nl_active = 0;
 }
 
@@ -882,7 +881,7 @@ static int nl_setup(comd_ctx *c)
     // dimension; the replicas of a kind share one potential
     int eam_rep = -1, lj_rep = -1;
     std::vector<std::array<int, 3>> nmax(nrep);
-    long kmax = 16, kout = 8;
+    long kmax = 16;
     for (int r = 0; r < nrep; r++) {
         const Replica &R = c->reps[r];
         const comd_params &p = R.prm;
@@ -899,11 +898,8 @@ static int nl_setup(comd_ctx *c)
             if (nmax[r][j] < 3) return COMD_OK;
         }
         const double vol = p.bounds[0] * p.bounds[1] * p.bounds[2];
-        const double rin = std::max(p.cutoff - skin, 0.0);
-        const double dens = 4.0 / 3.0 * 3.141592653589793 * (double)R.ntot / vol;
-        const double e_in = dens * rin * rin * rin, e_out = dens * (rl * rl * rl - rin * rin * rin);
-        kmax = std::max<long>(kmax, (((long)(1.5 * e_in) + 16 + 15) / 16) * 16);
-        kout = std::max<long>(kout, std::min<long>(64, (((long)(1.5 * e_out) + 16 + 7) / 8) * 8));
+        const double expect = 4.0 / 3.0 * 3.141592653589793 * rl * rl * rl * (double)R.ntot / vol;
+        kmax = std::max<long>(kmax, (((long)(1.5 * expect) + 16 + 15) / 16) * 16);
     }
     // groups of auxiliary cells: of the options whose staged halo (1.2x the
     // mean occupancy, plus 64 atoms, in the densest replica) fits the LDS a
@@ -1025,8 +1021,8 @@ static int nl_setup(comd_ctx *c)
             if ((c->reps[r].prm.potential == COMD_POT_LJ) == (pass == 1)) gorder.push_back(r);
     long nids = 0;
     for (int r = 0; r < nrep; r++) nids += c->reps[r].nl_ids;
-    if (nids > c->nl_alloc_n || kmax > c->nl_alloc_k || kout > c->nl_alloc_kout || naux > c->nl_alloc_aux ||
-        ngroups > c->nl_alloc_groups || nrep > c->nl_alloc_rep) {
+    if (nids > c->nl_alloc_n || kmax > c->nl_alloc_k || naux > c->nl_alloc_aux || ngroups > c->nl_alloc_groups ||
+        nrep > c->nl_alloc_rep) {
         free_nl(c);
         bool ok = true;
         auto alloc = [&](void **ptr, size_t bytes) {
@@ -1040,8 +1036,6 @@ static int nl_setup(comd_ctx *c)
         alloc((void **)&c->nl.fq, n * 8);
         alloc((void **)&c->nl.nbl, n * (size_t)kmax * 2);
         alloc((void **)&c->nl.ncnt, n * 4);
-        alloc((void **)&c->nl.nbo, n * (size_t)kout * 2);
-        alloc((void **)&c->nl.ocnt, n * 4);
         alloc((void **)&c->nl.aux_cnt, (size_t)naux * 4);
         alloc((void **)&c->nl.aux_n, (size_t)naux * 4);
         alloc((void **)&c->nl.aux_start, (size_t)naux * 4);
@@ -1061,7 +1055,6 @@ static int nl_setup(comd_ctx *c)
         }
         c->nl_alloc_n = nids;
         c->nl_alloc_k = kmax;
-        c->nl_alloc_kout = kout;
         c->nl_alloc_aux = naux;
         c->nl_alloc_groups = ngroups;
         c->nl_alloc_rep = nrep;
@@ -1069,7 +1062,6 @@ static int nl_setup(comd_ctx *c)
     CmList &l = c->nl;
     l.n = (int)nids;
     l.kmax = (int)kmax;
-    l.kout = (int)kout;
     l.nrep = nrep;
     l.naux = (int)naux;
     l.ngroups = (int)ngroups;
@@ -1110,8 +1102,6 @@ static int nl_setup(comd_ctx *c)
         R.ng2 = ng[r][2];
         const double rl = p.cutoff + skin;
         R.rl2 = rl * rl;
-        const double rin = std::max(p.cutoff - skin, 0.0);
-        R.rin2 = rin * rin;
         {
             // 32-bit fixed point of the mixed passes' group frames: a halo
             // atom lies within (gd/2 + 1) cells of the group centre plus the
@@ -1169,7 +1159,6 @@ static int nl_setup(comd_ctx *c)
     int rc = upload_descriptors(c);  // the grids' list id bases
     if (rc) return rc;
     HIPCHK(c, hipMemsetAsync(l.nbl, 0, (size_t)nids * (size_t)kmax * 2, c->stream));
-    HIPCHK(c, hipMemsetAsync(l.nbo, 0, (size_t)nids * (size_t)kout * 2, c->stream));
     HIPCHK(c, hipMemsetAsync(l.flags, 0, 16, c->stream));
     HIPCHK(c, hipMemsetAsync(l.count, 0, 16, c->stream));
     HIPCHK(c, hipMemsetAsync(l.aux_cnt, 0, (size_t)naux * 4, c->stream));
@@ -1787,7 +1776,7 @@ int comd_timing_read(comd_ctx *c, double *ms, int *launches)
 // ---- snapshots
 
 // Header of a snapshot file (little-endian ints after the 8-byte magic).
-enum { SNAP_VERSION = 1, SNAP_NHDR = 16 };
+enum { SNAP_VERSION = 2, SNAP_NHDR = 16 };
 static const char kSnapMagic[8] = {'C', 'O', 'M', 'D', 'S', 'N', 'A', 'P'};
 
 // One device array to or from a file, through a host buffer.
@@ -1810,7 +1799,7 @@ static std::vector<std::pair<void *, size_t>> snap_list_buffers(comd_ctx *c)
     const size_t n = (size_t)l.n;
     return {{l.pos, n * sizeof(CmPos)}, {l.selfh, n * 2},        {l.slot, n * 4},
             {l.order, n * 4},           {l.fq, n * 8},           {l.nbl, n * (size_t)l.kmax * 2},
-            {l.ncnt, n * 4},            {l.nbo, n * (size_t)l.kout * 2}, {l.ocnt, n * 4},
+            {l.ncnt, n * 4},
             {l.aux_n, (size_t)l.naux * 4}, {l.aux_start, (size_t)l.naux * 4}, {l.gstart, (size_t)(l.ngroups + 1) * 4},
             {l.nonempty, (size_t)c->ncells * 4}, {l.count, 16}, {l.flags, 16}};
 }
@@ -1861,6 +1850,40 @@ int comd_batch_observe(comd_ctx *c, int r, comd_observables *out)
     return rc ? rc : observables_rep(c, r, out);
 }
 
+static int centro_rep(comd_ctx *c, int r, double *out)
+{
+    if (!out) return set_err(c, COMD_ERR_ARG, "null argument");
+    if (!c->have_atoms) return set_err(c, COMD_ERR_STATE, "no atoms");
+    const Grid &g = c->reps[r].g;
+    const size_t n = (size_t)g.ncells * CM_SLOTS;
+    double *dev = nullptr;
+    if (hipMalloc((void **)&dev, n * 8) != hipSuccess) return set_err(c, COMD_ERR_NOMEM, "centrosymmetry buffer");
+    int rc = COMD_OK;
+    if (hipMemsetAsync(dev, 0, n * 8, c->stream) != hipSuccess) rc = set_err(c, COMD_ERR_HIP, "memset failed");
+    if (!rc) {
+        cm_launch_centro(dev_state(c), g, dev, c->stream);
+        if (hipGetLastError() != hipSuccess) rc = set_err(c, COMD_ERR_HIP, "centrosymmetry launch failed");
+    }
+    if (!rc && (hipMemcpyAsync(out, dev, n * 8, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+                hipStreamSynchronize(c->stream) != hipSuccess))
+        rc = set_err(c, COMD_ERR_HIP, "centrosymmetry copy failed");
+    hipFree(dev);
+    return rc;
+}
+
+int comd_centrosymmetry(comd_ctx *c, double *out)
+{
+    if (!c) return COMD_ERR_ARG;
+    return centro_rep(c, 0, out);
+}
+
+int comd_batch_centrosymmetry(comd_ctx *c, int r, double *out)
+{
+    if (!c) return COMD_ERR_ARG;
+    int rc = check_rep(c, r);
+    return rc ? rc : centro_rep(c, r, out);
+}
+
 int comd_snapshot_save(comd_ctx *c, const char *path)
 {
     if (!c || !path) return COMD_ERR_ARG;
@@ -1877,7 +1900,7 @@ int comd_snapshot_save(comd_ctx *c, const char *path)
     const Replica &R = c->reps[0];
     const CmList &l = c->nl;
     int hdr[SNAP_NHDR] = {SNAP_VERSION, c->ncells, c->precision, R.pot.eval, c->nl_active, c->nl_skin_milli,
-                          l.n, l.kmax, l.kout, l.naux, l.ngroups, l.hcap, R.ntot, (int)R.nl_ids,
+                          l.n, l.kmax, 0, l.naux, l.ngroups, l.hcap, R.ntot, (int)R.nl_ids,
                           R.pot.lj_fsign < 0.0, 0};
     const size_t S = (size_t)c->nslots, C = (size_t)c->ncells;
     bool ok = fwrite(kSnapMagic, 1, 8, fp) == 8 && fwrite(hdr, sizeof(int), SNAP_NHDR, fp) == SNAP_NHDR;
@@ -1941,7 +1964,7 @@ int comd_snapshot_load(comd_ctx *c, const char *path)
     // ... then the forces, the list words and the lists as they were
     const CmList &l = c->nl;
     if (hdr[4] != c->nl_active ||
-        (c->nl_active && (hdr[6] != l.n || hdr[7] != l.kmax || hdr[8] != l.kout || hdr[9] != l.naux ||
+        (c->nl_active && (hdr[6] != l.n || hdr[7] != l.kmax || hdr[9] != l.naux ||
                           hdr[10] != l.ngroups || hdr[11] != l.hcap))) {
         fclose(fp);
         return set_err(c, COMD_ERR_STATE, "the snapshot's neighbour lists do not fit this context");

@@ -364,8 +364,11 @@ __global__ void k_rebox_serial(DevState d)
             serial_copy(d, o, j * CM_SLOTS + nj);
             d.natoms[j] = nj + 1;
             if (nj + 1 >= CM_SLOTS) fl |= CM_FLAG_OVERFLOW;
+            // the cell's last atom fills the hole; none to move when the
+            // departed atom was the last (a self-copy would point its id ->
+            // slot entry back at the slot it left)
             const int ni = --d.natoms[c];
-            if (ni) serial_copy(d, c * CM_SLOTS + ni, o);
+            if (ni > i) serial_copy(d, c * CM_SLOTS + ni, o);
         }
     }
     d.mh[r] = 0;

@@ -160,6 +160,10 @@ int cm_observe_blocks(const Grid &g);
 void cm_launch_observe(const DevState &d, const PotDev &P, const Grid &g, double pe_scale, double *part,
                        comd_observables *out, hipStream_t st);
 
+// Centrosymmetry parameter (centro.hip) of one replica's atoms into out
+// (device memory, one double per slot of the replica's cells).
+void cm_launch_centro(const DevState &d, const Grid &g, double *out, hipStream_t st);
+
 // Group gi of ng along a dimension of na auxiliary cells: its first cell x0
 // and extent lx (extents differ by at most one; na / ng each when ng divides
 // na).
@@ -176,7 +180,6 @@ struct NlRep {
     double ab0, ab1, ab2, iab0, iab1, iab2;  // auxiliary cell edges and their inverses
     double qscale, qinv;                     // group-frame fixed point of the mixed passes: units per A, A per unit
     double rl2;                              // (cutoff + skin)^2
-    double rin2;                             // (cutoff - skin)^2 (0 when the skin is not below the cutoff)
     int na0, na1, na2;                       // auxiliary cells per dimension
     int gd0, gd1, gd2;                       // cells per group (the largest extent)
     int ng0, ng1, ng2;                       // groups per dimension
@@ -191,8 +194,7 @@ struct NlRep {
 // all replicas; a group's replica is group_rep[g].
 struct CmList {
     int n;                   // list id capacity (ids 0 .. n-1, over all replicas)
-    int kmax;                // inner list capacity per atom (a multiple of 16)
-    int kout;                // outer list capacity per atom (a multiple of 8, at most 64)
+    int kmax;                // list capacity per atom (a multiple of 16)
     const NlRep *rep;        // [nrep]
     int nrep;
     int naux;                // auxiliary cells, all replicas
@@ -210,18 +212,11 @@ struct CmList {
     uint16_t *selfh;         // per t: the atom's own index in its group's staged halo
     int *slot;               // per id: current slot, -1 when absent
     int *order;              // atoms (ids) in build order
-    // The list of the t-th atom in build order, in two parts: the inner list
-    // holds the atoms that were closer than cutoff - skin at the build, which
-    // stay inside the cutoff until the next build (no atom moves more than
-    // half the skin in between), the outer list the others within
-    // cutoff + skin, whose cutoff test the passes make.  Entries are indices
-    // into the group's staged halo, entry k at nl_entry(k, t, n) (groups of 8
-    // entries per atom, consecutive over t: one 16-byte load per lane
-    // fetches 8 entries, and a wave's loads are contiguous).
-    uint16_t *nbl;           // inner entries
-    int *ncnt;               // per t: inner entries
-    uint16_t *nbo;           // outer entries
-    int *ocnt;               // per t: outer entries
+    uint16_t *nbl;           // list of the t-th atom in build order: indices into its group's
+                             // staged halo, entry k at nl_entry(k, t, n) (groups of 8 entries
+                             // per atom, consecutive over t: one 16-byte load per lane fetches
+                             // 8 entries, and a wave's loads are contiguous)
+    int *ncnt;               // per t
     int *aux_cnt;            // per auxiliary cell: atoms keyed (cleared after each build)
     int *aux_n;              // the same, capped at CM_NL_AUXCAP, kept until the next build
     int *aux_start;          // first atom of each auxiliary cell in build order

@@ -395,9 +395,6 @@ __global__ __launch_bounds__(1024) void k_nl_build(DevState d, CmList l)
     const NlRep R = nl_rep(l, l.group_rep[g]);
     const GroupGeo G = group_geo(R, g - R.group_base);
     const float rl2 = (float)(R.rl2 * (1.0 + 4e-6));
-    // inner list: below (cutoff - skin)^2 less the same rounding bound, so
-    // that every inner pair truly was closer than cutoff - skin
-    const float rin2 = (float)(R.rin2 * (1.0 - 4e-6));
     // the group's replica: the grid of every atom it stages
     const Grid gr = d.ngrids == 1 ? d.g0 : d.grids[l.group_rep[g]];
     group_setup(l, R, G, g, S, slot_of);
@@ -448,42 +445,27 @@ __global__ __launch_bounds__(1024) void k_nl_build(DevState d, CmList l)
             const float4 pi = hl4[self];
             // the 27 stencil cells as 9 runs of three x-adjacent halo slots,
             // whose atoms are contiguous in the halo; entries gathered 8 at a
-            // time per list (inner, outer) and stored as one 16-byte group
-            int k = 0, ko = 0;
-            unsigned long long lo = 0, hi = 0, olo = 0, ohi = 0;
+            // time and stored as one 16-byte group
+            int k = 0;
+            unsigned long long lo = 0, hi = 0;
             for (int e = 0; e < 9; e++) {
                 const int s0 = (cx - 1) + shx * ((cy + e / 3 - 1) + shy * (cz + e % 3 - 1));
                 const int h1 = S.off[s0 + 3];
                 for (int h = S.off[s0]; h < h1; h++) {
                     const float4 qh = hl4[h];
                     const float dx = pi.x - qh.x, dy = pi.y - qh.y, dz = pi.z - qh.z;
-                    const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                    if (d2 <= rl2 && h != self) {
-                        if (d2 < rin2) {
-                            const int c8 = k & 7;
-                            if (c8 < 4) lo |= (unsigned long long)h << (16 * c8);
-                            else hi |= (unsigned long long)h << (16 * (c8 - 4));
-                            if (c8 == 7 && k < l.kmax) {
-                                ulonglong2 w;
-                                w.x = lo;
-                                w.y = hi;
-                                *(ulonglong2 *)&l.nbl[nl_entry(k - 7, t, l.n)] = w;
-                                lo = hi = 0;
-                            }
-                            k++;
-                        } else {
-                            const int c8 = ko & 7;
-                            if (c8 < 4) olo |= (unsigned long long)h << (16 * c8);
-                            else ohi |= (unsigned long long)h << (16 * (c8 - 4));
-                            if (c8 == 7 && ko < l.kout) {
-                                ulonglong2 w;
-                                w.x = olo;
-                                w.y = ohi;
-                                *(ulonglong2 *)&l.nbo[nl_entry(ko - 7, t, l.n)] = w;
-                                olo = ohi = 0;
-                            }
-                            ko++;
+                    if (fmaf(dx, dx, fmaf(dy, dy, dz * dz)) <= rl2 && h != self) {
+                        const int c8 = k & 7;
+                        if (c8 < 4) lo |= (unsigned long long)h << (16 * c8);
+                        else hi |= (unsigned long long)h << (16 * (c8 - 4));
+                        if (c8 == 7 && k < l.kmax) {
+                            ulonglong2 w;
+                            w.x = lo;
+                            w.y = hi;
+                            *(ulonglong2 *)&l.nbl[nl_entry(k - 7, t, l.n)] = w;
+                            lo = hi = 0;
                         }
+                        k++;
                     }
                 }
             }
@@ -493,15 +475,8 @@ __global__ __launch_bounds__(1024) void k_nl_build(DevState d, CmList l)
                 w.y = hi;
                 *(ulonglong2 *)&l.nbl[nl_entry(k & ~7, t, l.n)] = w;
             }
-            if ((ko & 7) != 0 && ko <= l.kout) {
-                ulonglong2 w;
-                w.x = olo;
-                w.y = ohi;
-                *(ulonglong2 *)&l.nbo[nl_entry(ko & ~7, t, l.n)] = w;
-            }
             l.ncnt[t] = min(k, l.kmax);
-            l.ocnt[t] = min(ko, l.kout);
-            ovf |= k > l.kmax || ko > l.kout;
+            ovf |= k > l.kmax;
             d.nlr[ts] = make_float4((float)p.x, (float)p.y, (float)p.z, __int_as_float(t));
         }
     }
@@ -574,19 +549,9 @@ __host__ __device__ constexpr int nl_halo_bytes(int kind, int mixed) { return mi
 // Bytes of the staged halo entries' slot map (one byte per entry).
 __host__ __device__ constexpr int nl_slotmap_bytes(int hcap) { return (hcap + 15) / 16 * 16; }
 
-// One home atom t over its lists.  TB: the table pointers (compact rows in
+// One home atom t over its list.  TB: the table pointers (compact rows in
 // LDS when COMPACT, else the full tables in global memory); hq (mixed) or
 // hx, hy, hz, hf (FP64): the staged halo, which holds the atom itself too.
-//
-// Inner entries are inside the cutoff by construction: they are evaluated
-// in whole chunks of 8 with no cutoff test and no branch (one 16-byte entry
-// load, the next chunk's load issued first, then the 8 halo reads, then the
-// 8 pair terms), the partial last chunk under a range mask.  Outer entries
-// are tested first, all of them; each lane keeps the halo indices of up to
-// four accepted ones in one register (the rest as a bit mask, read back
-// from the list), and the pair terms then run for accepted pairs only, so
-// that a wave spends them on the few outer pairs inside the cutoff rather
-// than on every outer entry slot any of its lanes accepts.
 template <int KIND, int EVAL, int MIXED, int COMPACT, class TB>
 __device__ __forceinline__ void nl_atom(const DevState &d, const PotDev &P, const CmList &l, const TB &T, int t,
                                         const CM_LDS v4u *hq, const CM_LDS double *hx, const CM_LDS double *hy,
@@ -614,165 +579,95 @@ __device__ __forceinline__ void nl_atom(const DevState &d, const PotDev &P, cons
     constexpr bool UNITS = MIXED && COMPACT && KIND != CM_KIND_LJ;
     const double rc2u = rc2 * (qscale * qscale);
     const double iu = qinv * P.invdx[0], cu = -P.x0[0] * P.invdx[0];
-    const int cin = l.ncnt[t], cout = l.ocnt[t];
+    const int cnt = l.ncnt[t];
     Acc A = {0, 0, 0, 0, 0, 0};
-    float ef = 0.0f, rf = 0.0f;  // FP32 value sums of a chunk (mixed density pass)
     int npair = 0;
-
-    // a halo entry's position (and F' when the pass needs it) as the pair
-    // arithmetic takes it
-    struct Nb {
-        v4u q;
-        double x, y, z, f;
-    };
-    auto fetch = [&](int h) {
-        Nb n;
-        if constexpr (MIXED) {
-            if constexpr (KIND == CM_KIND_EAM3) {
-                n.q = hq[h];
-            } else {  // x, y, z only (12-byte reads)
-                const v3u q3 = *(const CM_LDS v3u *)&hq[h];
-                n.q = v4u{q3.x, q3.y, q3.z, 0u};
-            }
-        } else {
-            n.x = hx[h];
-            n.y = hy[h];
-            n.z = hz[h];
-            n.f = KIND == CM_KIND_EAM3 ? hf[h] : 0.0;
-        }
-        return n;
-    };
-    // squared distance of the pair (fixed-point units in mixed passes)
-    auto dist2 = [&](const Nb &n) {
-        if constexpr (MIXED) {
-            // fixed-point differences (modulo 2^32, exact for pairs on the
-            // lists, which are well within range)
-            const double dxu = (double)(int)(qi.x - n.q.x);
-            const double dyu = (double)(int)(qi.y - n.q.y);
-            const double dzu = (double)(int)(qi.z - n.q.z);
-            return dxu * dxu + dyu * dyu + dzu * dzu;
-        } else {
-            const double dx = xi - n.x, dy = yi - n.y, dz = zi - n.z;
-            return dx * dx + dy * dy + dz * dz;
-        }
-    };
-    auto inside = [&](double r2) { return KIND == CM_KIND_EAM3 ? (r2 < (MIXED ? rc2u : rc2)) : (r2 <= (MIXED ? rc2u : rc2)); };
-    // the pair terms of an accepted pair
-    auto pair = [&](const Nb &n) {
-        if constexpr (MIXED) {
-            const double dxu = (double)(int)(qi.x - n.q.x);
-            const double dyu = (double)(int)(qi.y - n.q.y);
-            const double dzu = (double)(int)(qi.z - n.q.z);
-            const double fij = (double)__uint_as_float(n.q.w);
-            const double r2u = dxu * dxu + dyu * dyu + dzu * dzu;
-            if constexpr (UNITS) {
-                pair_term_units<KIND>(P, T, dxu, dyu, dzu, r2u, iu, cu, fia, fij, A, ef, rf);
-            } else {
-                const double dx = dxu * qinv, dy = dyu * qinv, dz = dzu * qinv;
-                const double r2 = r2u * (qinv * qinv);
-                if constexpr (COMPACT) pair_term_compact<KIND, MIXED>(P, T, dx, dy, dz, r2, fia, fij, A);
-                else pair_term<KIND, EVAL, MIXED>(P, T, dx, dy, dz, r2, fia, fij, A);
-            }
-        } else {
-            const double dx = xi - n.x, dy = yi - n.y, dz = zi - n.z;
-            const double r2 = dx * dx + dy * dy + dz * dz;
-            if constexpr (COMPACT) pair_term_compact<KIND, MIXED>(P, T, dx, dy, dz, r2, fia, n.f, A);
-            else pair_term<KIND, EVAL, MIXED>(P, T, dx, dy, dz, r2, fia, n.f, A);
-        }
-    };
-    auto flush = [&]() {
-        if (UNITS && KIND == CM_KIND_EAM1) {
-            A.e += (double)ef;
-            A.rho += (double)rf;
-            ef = rf = 0.0f;
-        }
-    };
-    auto unpack = [](v4u e, int hj[8]) {
+    // 8 entries per 16-byte load, then the 8 halo reads from LDS, all issued
+    // before the first pair is evaluated.  Whole chunks of 8 first (the next
+    // chunk's load issued before this one is processed), then the partial
+    // last chunk, whose entries past cnt are not this build's: entry 0 of
+    // the halo is read in their place and they count as out of range.
+    const CM_GLOBAL v4u *row = (const CM_GLOBAL v4u *)l.nbl + t;
+    auto chunk = [&](const v4u &e, int k0, auto tail) {
+        constexpr bool TAIL = decltype(tail)::value;
         const unsigned ew[4] = {e.x, e.y, e.z, e.w};
+        int hj[8];
 #pragma unroll
-        for (int u = 0; u < 8; u++) hj[u] = (int)((ew[u >> 1] >> (16 * (u & 1))) & 0xffffu);
-    };
-
-    // ---- inner list: inside the cutoff by construction
-    {
-        const CM_GLOBAL v4u *row = (const CM_GLOBAL v4u *)l.nbl + t;
-        // the next chunk's index clamped to the list capacity: always a
-        // valid load
-        const int clast = l.kmax / 8 - 1;
-        v4u e = row[0];
-        int k0 = 0;
-        for (; k0 + 8 <= cin; k0 += 8) {
-            const v4u en = row[(size_t)min((k0 >> 3) + 1, clast) * l.n];
-            int hj[8];
-            unpack(e, hj);
-            // two halves of four: their halo reads issued together, within
-            // the register budget of the mixed passes
-#pragma unroll
-            for (int hb = 0; hb < 8; hb += 4) {
-                Nb nb[4];
-#pragma unroll
-                for (int u = 0; u < 4; u++) nb[u] = fetch(hj[hb + u]);
-#pragma unroll
-                for (int u = 0; u < 4; u++) pair(nb[u]);
-            }
-            flush();
-            e = en;
+        for (int u = 0; u < 8; u++) {
+            hj[u] = (int)((ew[u >> 1] >> (16 * (u & 1))) & 0xffffu);
+            if (TAIL && k0 + u >= cnt) hj[u] = 0;
         }
-        if (k0 < cin) {
-            // the partial last chunk: entries past cin are not this build's
-            // (entry 0 of the halo is read in their place and skipped)
-            int hj[8];
-            unpack(e, hj);
-#pragma unroll
-            for (int u = 0; u < 8; u++)
-                if (k0 + u >= cin) hj[u] = 0;
-            Nb nb[8];
-#pragma unroll
-            for (int u = 0; u < 8; u++) nb[u] = fetch(hj[u]);
-#pragma unroll
-            for (int u = 0; u < 8; u++)
-                if (k0 + u < cin) pair(nb[u]);
-            flush();
-        }
-        npair += cin;
-    }
-
-    // ---- outer list: test every entry, then evaluate the accepted ones
-    {
-        const CM_GLOBAL v4u *row = (const CM_GLOBAL v4u *)l.nbo + t;
-        unsigned long long keep = 0;  // up to four accepted halo indices, 16 bits each
-        int nkeep = 0;
-        unsigned long long more = 0;  // accepted entries beyond those four, by entry index
-        for (int k0 = 0; k0 < cout; k0 += 8) {
-            int hj[8];
-            unpack(row[(size_t)(k0 >> 3) * l.n], hj);
-#pragma unroll
-            for (int u = 0; u < 8; u++)
-                if (k0 + u >= cout) hj[u] = 0;
-            Nb nb[8];
-#pragma unroll
-            for (int u = 0; u < 8; u++) nb[u] = fetch(hj[u]);
+        if constexpr (MIXED) {
+            // x, y, z only (12-byte reads) unless F' is needed
+            v4u qj[8];
 #pragma unroll
             for (int u = 0; u < 8; u++) {
-                if (k0 + u < cout && inside(dist2(nb[u]))) {
-                    if (nkeep < 4) keep |= (unsigned long long)hj[u] << (16 * nkeep++);
-                    else more |= 1ull << (k0 + u);
+                if constexpr (KIND == CM_KIND_EAM3) {
+                    qj[u] = hq[hj[u]];
+                } else {
+                    const v3u q3 = *(const CM_LDS v3u *)&hq[hj[u]];
+                    qj[u] = v4u{q3.x, q3.y, q3.z, 0u};
+                }
+            }
+            float ef = 0.0f, rf = 0.0f;  // the chunk's FP32 value sums (density pass)
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                // fixed-point differences (modulo 2^32, exact for pairs on
+                // the lists, which are well within range)
+                const double dxu = (double)(int)(qi.x - qj[u].x);
+                const double dyu = (double)(int)(qi.y - qj[u].y);
+                const double dzu = (double)(int)(qi.z - qj[u].z);
+                const double fij = (double)__uint_as_float(qj[u].w);
+                const double r2u = dxu * dxu + dyu * dyu + dzu * dzu;
+                const bool in = (!TAIL || k0 + u < cnt) && (KIND == CM_KIND_EAM3 ? (r2u < rc2u) : (r2u <= rc2u));
+                if (in) {
+                    if constexpr (UNITS) {
+                        pair_term_units<KIND>(P, T, dxu, dyu, dzu, r2u, iu, cu, fia, fij, A, ef, rf);
+                    } else {
+                        const double dx = dxu * qinv, dy = dyu * qinv, dz = dzu * qinv;
+                        const double r2 = r2u * (qinv * qinv);
+                        if constexpr (COMPACT) pair_term_compact<KIND, MIXED>(P, T, dx, dy, dz, r2, fia, fij, A);
+                        else pair_term<KIND, EVAL, MIXED>(P, T, dx, dy, dz, r2, fia, fij, A);
+                    }
+                    npair++;
+                }
+            }
+            if (UNITS && KIND == CM_KIND_EAM1) {
+                A.e += (double)ef;
+                A.rho += (double)rf;
+            }
+        } else {
+            double xj[8], yj[8], zj[8], fj[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                xj[u] = hx[hj[u]];
+                yj[u] = hy[hj[u]];
+                zj[u] = hz[hj[u]];
+                fj[u] = KIND == CM_KIND_EAM3 ? hf[hj[u]] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const double dx = xi - xj[u], dy = yi - yj[u], dz = zi - zj[u];
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                const bool in = (!TAIL || k0 + u < cnt) && (KIND == CM_KIND_EAM3 ? (r2 < rc2) : (r2 <= rc2));
+                if (in) {
+                    if constexpr (COMPACT) pair_term_compact<KIND, MIXED>(P, T, dx, dy, dz, r2, fia, fj[u], A);
+                    else pair_term<KIND, EVAL, MIXED>(P, T, dx, dy, dz, r2, fia, fj[u], A);
+                    npair++;
                 }
             }
         }
-        npair += nkeep + __popcll(more);
-        for (; nkeep > 0; nkeep--) {
-            pair(fetch((int)(keep & 0xffffu)));
-            keep >>= 16;
-        }
-        while (more) {
-            const int k = __ffsll((long long)more) - 1;
-            more &= more - 1;
-            pair(fetch((int)((const CM_GLOBAL uint16_t *)l.nbo)[nl_entry(k, t, l.n)]));
-        }
-        flush();
+    };
+    // the next chunk's index clamped to the list capacity: always a valid load
+    const int clast = l.kmax / 8 - 1;
+    v4u e = row[0];
+    int k0 = 0;
+    for (; k0 + 8 <= cnt; k0 += 8) {
+        const v4u en = row[(size_t)min((k0 >> 3) + 1, clast) * l.n];
+        chunk(e, k0, std::false_type());
+        e = en;
     }
-
+    if (k0 < cnt) chunk(e, k0, std::true_type());
     if (UNITS) A.vir *= qinv;
     if (KIND == CM_KIND_EAM1) {
         // pass 2, src-flat/eam.c:576-594

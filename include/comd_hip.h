@@ -172,8 +172,8 @@ enum comd_tune_key {
                                       integrator moves atoms down the potential (physical motion,
                                       KE + PE conserved).  0 keeps src-flat's convention (ljForce.c
                                       stores -grad U and timestep.c:71 subtracts it: atoms climb the
-                                      potential, KE - PE is conserved and a lattice at any temperature
-                                      flies apart within about 50 fs).  No counterpart in the
+                                      potential, KE - PE is conserved, and a 300 K copper lattice
+                                      (epsilon 0.167 eV) flies apart within about 50 fs).  No counterpart in the
                                       reference. */
     COMD_TUNE_LIST_GROUPS = 9,     /* -1 (default), 0..4: auxiliary cells per group of the
                                       neighbour-list passes, 4x4x3, 4x4x2, 4x2x2, 2x2x2, 2x2x1 (from the
@@ -283,6 +283,18 @@ int comd_batch_observe(comd_ctx *ctx, int r, comd_observables *out);
  * without synchronising with the host between blocks.  pe_ke receives
  * 2 (nblocks + 1) doubles (pe, ke) per record.  Single simulations only. */
 int comd_run_monitored(comd_ctx *ctx, int nblocks, int block_steps, double dt_seconds, double *pe_ke);
+
+/* Centrosymmetry parameter per atom, the defect measure of src-viz's
+ * viewer (computeCentrosymmetry, src-viz/AtomVisualize.c:399-673): the
+ * smallest, over the ways of pairing an atom's 12 nearest neighbours, of
+ * the sum of |r_j + r_k|^2 (0 in a perfect FCC crystal, A^2), 1000 for
+ * atoms with fewer than 12 neighbours in reach; the reference's neighbour
+ * search and float sums.  out: one double per slot of the replica's cells
+ * (slot layout, as comd_download), empty slots 0.  The reference's link
+ * cells are at least the cutoff wide, which must exceed the 12th-neighbour
+ * distance. */
+int comd_centrosymmetry(comd_ctx *ctx, double *out);
+int comd_batch_centrosymmetry(comd_ctx *ctx, int r, double *out);
 
 /* ---- Snapshots ----
  *

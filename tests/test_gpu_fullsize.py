@@ -59,7 +59,8 @@ def errors(gc, oc, lj=False):
 def report(what, natoms, err, sim=None):
     extra = ""
     if sim is not None:
-        active, valid, builds, kmax = sim.list_info()
+        li = sim.list_info()
+        active, valid, builds, kmax = li["active"], li["valid"], li["builds"], li["kmax"]
         extra = f" lists active={active} valid={valid} builds={builds} kmax={kmax}"
     print(f"[{what}] atoms={natoms} " + " ".join(f"{k} {v:.3e}" for k, v in err.items()) + extra, flush=True)
 
@@ -78,7 +79,7 @@ def test_cu40_table_fp64_force_and_steps(gpu, cu500, ocu500):
     gc, oc = layout(sim.download(), osim.state(), "cu40 force")
     err = errors(gc, oc)
     report("cu40 table force", len(gc["id"]), err, sim)
-    assert sim.list_info()[0] == 1 and sim.list_info()[1] == 1  # the list passes ran
+    assert sim.list_info()["active"] == 1 and sim.list_info()["valid"] == 1  # the list passes ran
     assert max(err.values()) <= FP64_TOL
     pe, ke, stress = sim.energy()
     oi = osim.info()
@@ -141,7 +142,7 @@ def test_cu80_force_fp64_and_mixed(gpu, cu500, ocu500):
     err = errors(gc, oc)
     report("cu80 FP64 force", len(gc["id"]), err, sim)
     print(f"[cu80] list passes {sim.list_occupancy()}", flush=True)
-    assert sim.list_info()[:2] == (1, 1)
+    assert (sim.list_info()["active"], sim.list_info()["valid"]) == (1, 1)
     assert max(err.values()) <= FP64_TOL
     pe, ke, stress = sim.energy()
     assert abs(pe - oi["e"]) <= 1e-12 * abs(oi["e"])
@@ -155,7 +156,7 @@ def test_cu80_force_fp64_and_mixed(gpu, cu500, ocu500):
     assert np.array_equal(gm["id"], oc["id"])
     errm = errors(gm, oc)
     report("cu80 mixed force", len(gm["id"]), errm, sim)
-    assert sim.list_info()[:2] == (1, 1)
+    assert (sim.list_info()["active"], sim.list_info()["valid"]) == (1, 1)
     assert errm["force"] <= MIXED_TOL
     assert errm["rho"] <= MIXED_TOL
     assert errm["energy"] <= MIXED_TOL
@@ -177,7 +178,7 @@ def energy_series(precision, cu500, nblocks=20, block=50):
         pe = sim.nsteps(block)
         _, ke, _ = sim.energy()
         e.append(pe + ke)
-    builds = sim.list_info()[2]
+    builds = sim.list_info()["builds"]
     sim.close()
     return np.array(e) / xyz.shape[0], builds
 
@@ -232,8 +233,11 @@ def lj_per_step(physical, nsteps, skin):
         assert osim.info()["err"] == 0, f"oracle stopped at step {step}"
         g, o = layout(sim.download(), osim.state(), f"lj step {step}")
         pscale = max(np.abs(o["p"]).max(), 1e-300)
-        np.testing.assert_allclose(g["r"], o["r"], rtol=0, atol=1e-10)
-        np.testing.assert_allclose(g["p"], o["p"], rtol=0, atol=1e-10 * pscale)
+        # positions to 1e-10 A, or to 1e-14 relative for atoms that the
+        # reference convention has flung far out (displacements of 1e8 A a
+        # step, whose last bits follow the force sum's order)
+        np.testing.assert_allclose(g["r"], o["r"], rtol=1e-14, atol=1e-10, err_msg=f"step {step}")
+        np.testing.assert_allclose(g["p"], o["p"], rtol=0, atol=1e-10 * pscale, err_msg=f"step {step}")
         err = errors(g, o, lj=True)
         for k in worst:
             worst[k] = max(worst[k], err[k])

@@ -92,36 +92,48 @@ def test_drift_rebox_bit_exact(gpu, cu500, ocu500):
     assert moved > 100  # the passes really reordered cells
 
 
-def test_rebox_multi_hop_serial_pass(gpu, cu500, ocu500):
+@pytest.mark.parametrize("case", ["in_cell", "outside"])
+def test_rebox_multi_hop_serial_pass(gpu, cu500, ocu500, case):
     """An atom that must move twice in one pass (arrives in a higher cell
-    still outside it) takes the serial path, which must match reBoxAll; the
-    neighbour lists it keeps up to date (list words and the id -> slot map
-    moved with the atoms) then give the oracle's forces, and a few steps
-    stay with the oracle."""
+    still outside it) takes the serial path, which must match reBoxAll.
+
+    in_cell: every atom ends inside its cell; the neighbour lists, whose
+    id -> slot map and list words the serial pass moves with the atoms, then
+    give the oracle's forces, and a few steps stay with the oracle.
+    outside: a second atom moves down into an already scanned cell and stays
+    outside it, as reBoxAll leaves it; the reference then finds that atom's
+    pairs by its cell, not by distance, which the link-cell kernels follow
+    (lists off) and the lists, by design, do not."""
+    print(f"[case] multi-hop {case}", flush=True)
     xyz, p, bounds = seeded_atoms(8, 3.615, 63.55, temp=300.0, disp=0.05, seed=1)
     sim, osim = make_pair(cu500, ocu500, xyz, p, bounds)
+    if case == "outside":
+        sim.tune(skin=0.0)
     sim.rebox()
     osim.rebox()
     sim.force()  # lists built before the multi-hop pass
     st = osim.state()
     bs = osim.info()["boxsize"]
     c = 5
-    st["r"][c * 64 + 0, 0] += 2.2 * bs[0]   # two cells up in x
-    st["r"][c * 64 + 1, 1] -= 1.5 * bs[1]   # down in y: arrives in a lower cell
+    st["r"][c * 64 + 0, 0] += 2.2 * bs[0]  # two or three cells up in x
+    # down in y by one cell: inside it (in_cell), or still below it (outside)
+    st["r"][c * 64 + 1, 1] -= (1.5 if case == "outside" else 0.6) * bs[1]
     sim.upload(st["natoms"], st["id"], st["itype"], st["r"], st["p"])
     osim.set_state(st)
     sim.rebox()
     osim.rebox()
     gc, oc = assert_same_layout(sim.download(), osim.state(), "multi-hop")
     assert np.array_equal(gc["r"], oc["r"])
-    assert sim.list_info()[0] == 1  # the list path is the one checked below
+    outside = np.any((oc["r"] < 0.0) | (oc["r"] >= np.array(bs)), axis=1)
+    assert outside.sum() == (1 if case == "outside" else 0)
+    assert sim.list_info()["active"] == (1 if case == "in_cell" else 0)
     sim.force()
     osim.force()
-    check_forces(sim.download(), osim.state(), FP64_TOL, "multi-hop force")
+    check_forces(sim.download(), osim.state(), FP64_TOL, f"multi-hop {case} force")
     for step in range(3):
         sim.nsteps(1)
         osim.nsteps(1)
-        check_forces(sim.download(), osim.state(), FP64_TOL, f"multi-hop step {step}")
+        check_forces(sim.download(), osim.state(), FP64_TOL, f"multi-hop {case} step {step}")
         assert abs(sim.energy()[0] - osim.info()["e"]) <= 1e-10 * abs(osim.info()["e"])
 
 

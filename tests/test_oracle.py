@@ -476,9 +476,10 @@ def test_rebox_moves_and_swaps_with_last(cu):
 def test_lj_reference_convention_is_unstable():
     """src-flat's LJ stores -grad U and its integrator subtracts it
     (ljForce.c, timestep.c:71), so atoms climb the potential: a 300 K FCC
-    lattice gains kinetic energy without bound within about 50 fs.  This is
-    why trajectories that must stay physical use COMD_TUNE_LJ_PHYSICAL."""
-    sigma, eps, mass = 2.315, 0.167, 63.55
+    lattice gains kinetic energy without bound within about 50 fs (with
+    copper's epsilon of 0.167 eV: x7 by step 40, x1e31 by step 50).  This
+    is why trajectories that must stay physical use COMD_TUNE_LJ_PHYSICAL."""
+    sigma, eps, mass = 2.315, 0.167 / 27.211386, 63.55  # epsilon in hartree
     olj = O.OraclePot.lj(sigma, eps, 2.5 * sigma, mass)
     lat = math.sqrt(2.0) * 2.0 ** (1.0 / 6.0) * sigma
     s = O.OracleSim.fcc(olj, 5, lat)
@@ -493,6 +494,8 @@ def test_lj_reference_convention_is_unstable():
     assert ke0 > 0.0
     s.nsteps(40)
     assert s.info()["err"] != 0 or s.kinetic() > 3.0 * ke0
+    s.nsteps(10)
+    assert s.info()["err"] != 0 or s.kinetic() > 1e20 * ke0
 
 
 # ---------------------------------------------------------------------------

@@ -1,5 +1,7 @@
 """CPU model of the GPU's parallel reBoxAll emulation (comd_amd/csrc/bin_rebox.hip),
-checked against the oracle's serial reBoxAll (src-flat/domains.c:347-389).
+checked against the oracle's serial reBoxAll (src-flat/domains.c:347-389),
+and of the serial kernel (k_rebox_serial) with the id -> slot map it keeps
+for the neighbour lists.
 
 The model computes what the kernels compute -- per-atom destinations, each
 touched cell's own swap-with-last scan with n_low placeholders, arrivals
@@ -146,3 +148,70 @@ def test_model_flags_second_hop():
     st = sim.state()
     st["r"][0, 0] += 2.3 * info["boxsize"][0]
     assert model_rebox(st, info["nbx"], info["boxsize"]) is None
+
+
+def model_serial(st, nbx, bs, fill_last_onto_itself=False, periodic=True):
+    """The serial kernel's in-place replay of reBoxAll, with the id -> slot
+    map of the neighbour lists updated by every copy (serial_copy).  The
+    cell's last atom fills the hole of a departed one unless the departed
+    atom was the last; fill_last_onto_itself=True makes that copy anyway
+    (a self-copy), which points the departed atom's map entry back at the
+    slot it left."""
+    natoms = st["natoms"].copy()
+    ids, r = st["id"].copy(), st["r"].copy()
+    slot_of = {int(ids[c * M + i]): c * M + i for c in range(natoms.size) for i in range(natoms[c])}
+
+    def copy(a, b):
+        slot_of[int(ids[a])] = b
+        ids[b] = ids[a]
+        r[b] = r[a]
+
+    for c in range(natoms.size):
+        i = 0
+        while i < natoms[c]:
+            o = c * M + i
+            j, rn = destination(r[o], c, nbx, bs, periodic)
+            if j < 0:
+                i += 1
+                continue
+            r[o] = rn
+            copy(o, j * M + natoms[j])
+            natoms[j] += 1
+            natoms[c] -= 1
+            ni = natoms[c]
+            if ni > i or (fill_last_onto_itself and ni > 0):
+                copy(c * M + ni, o)
+    return natoms, ids, r, slot_of
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_serial_model_keeps_id_slot_map(seed):
+    """Multi-hop passes on random states, with the last atom of some cells
+    among the movers: the serial model's layout equals the oracle's reBoxAll
+    and its map sends every id to the slot that holds it; the self-copy rule
+    breaks the map in the same passes."""
+    rng = np.random.default_rng(100 + seed)
+    pot = O.OraclePot.eam(POTS, "cu500")
+    n = 5
+    L = n * 3.615
+    xyz = rng.uniform(0, L, size=(4 * n ** 3, 3))
+    sim = O.OracleSim.from_atoms(pot, (L, L, L), xyz)
+    info = sim.info()
+    nbx, bs = info["nbx"], info["boxsize"]
+    st = sim.state()
+    last = [c * M + st["natoms"][c] - 1 for c in range(st["natoms"].size) if st["natoms"][c] > 0]
+    pick = rng.choice(last, size=len(last) // 3, replace=False)
+    st["r"][pick, 0] += 2.3 * bs[0]  # two cells up: a multi-hop pass
+    sim.set_state(st)
+    assert model_rebox(st, nbx, bs) is None
+    natoms, ids, r, slot_of = model_serial(st, nbx, bs)
+    sim.rebox()
+    got = sim.state()
+    assert np.array_equal(got["natoms"], natoms)
+    mask = (np.arange(M)[None, :] < natoms[:, None]).reshape(-1)
+    assert np.array_equal(got["id"][mask], ids[mask])
+    assert np.array_equal(got["r"][mask], r[mask])
+    assert all(slot_of[int(ids[s])] == s for s in np.flatnonzero(mask))
+    _, ids2, _, slot2 = model_serial(st, nbx, bs, fill_last_onto_itself=True)
+    assert np.array_equal(ids2[mask], ids[mask])
+    assert any(slot2[int(ids2[s])] != s for s in np.flatnonzero(mask))
