@@ -61,7 +61,9 @@ def parse():
     ap.add_argument("--n", type=int, default=80, help="FCC unit cells per edge")
     ap.add_argument("--seed", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU reference timing")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--cpu-n", type=int, default=40, help="FCC cells per edge of the CPU reference's sample")
+    ap.add_argument("--no-extra", action="store_true", help="skip the other configurations' timings")
     ap.add_argument("--workload", choices=("eam", "batch", "adjoint"), default="eam",
                     help="eam: EAM Cu 2.05M atoms, FP64 and mixed (the headline line); batch: 64 LJ/EAM replicas "
                          "in one context; adjoint: Chebyshev-coefficient gradients of 16 Cu 20^3 configurations")
@@ -161,26 +163,52 @@ def measured_traffic(kname):
     return None, "no counter summary for this kernel"
 
 
-def cpu_baseline(n, steps):
-    """Compiled reference (oracle/_ref/ref_table, built from the reference's
-    own sources) on the same FCC lattice, one thread; falls back to the
-    oracle restatement when the compiled reference is absent."""
+def host_cpu():
+    """Model name of the host CPU and the number of CPUs this process may use."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return model, ncpu
+
+
+def cpu_baseline(n, steps, seed):
+    """The compiled reference (oracle/_ref/ref_table, built from the
+    reference's own sources), one thread pinned to one core, on a bounded
+    sample of the benchmark's workload: the same seeded 300 K Cu lattice at
+    FCC n^3, read from a gz input file written at %.17g; the reference's
+    extra end-of-call force is excluded.  Falls back to the oracle
+    restatement when the compiled reference is absent."""
+    import tempfile
+
+    model, ncpu = host_cpu()
     exe = os.path.join(ROOT, "oracle", "_ref", "ref_table")
-    cores = 1
-    sample = (f"EAM cu500 table, FCC {n}^3 ({4 * n ** 3} atoms) perfect lattice, {steps} steps after one force, "
-              f"extra end-of-call force excluded, 1 thread")
+    natoms = 4 * n ** 3
+    sample = (f"EAM cu500 table, the seeded 300 K FCC {n}^3 lattice ({natoms} atoms) of the benchmark at a "
+              f"smaller size, {steps} steps after one force, extra end-of-call force excluded, 1 thread")
+    xyz = CA.fcc_positions(n, n, n, 3.615)
+    xyz, p = CA.seed_state(xyz, 63.55, 300.0, 0.0, seed)
+    bounds = (n * 3.615,) * 3
+    extra = dict(cpu_model=model, nproc=ncpu)
     if os.path.exists(exe):
-        pin = ["taskset", "-c", "0"] if shutil.which("taskset") else []
-        out = subprocess.run(pin + [exe, "--eam", CA.POT_DIR, "cu500", "--fcc", str(n), str(n),
-                              str(n), "3.615", "1.0", "--steps", str(steps), "--dumps", "1", "--time"],
-                             capture_output=True, text=True, check=True).stdout
+        with tempfile.TemporaryDirectory() as tmp:
+            inp = os.path.join(tmp, "seeded.inp.gz")
+            CA.write_gz(inp, bounds, xyz, p)
+            pin = ["taskset", "-c", str(min(os.sched_getaffinity(0)))] if shutil.which("taskset") else []
+            out = subprocess.run(pin + [exe, "--eam", CA.POT_DIR, "cu500", "--input", inp, "--steps", str(steps),
+                                        "--dumps", "1", "--time"], capture_output=True, text=True, check=True).stdout
         m = re.search(r"atom_steps_per_s=([0-9.eE+-]+)", out)
-        return dict(value=float(m.group(1)), unit="atom-steps/s", cores=cores, kind="reference", sample=sample)
+        return dict(value=float(m.group(1)), unit="atom-steps/s", cores=1, kind="reference", sample=sample, **extra)
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_py as O
 
     pot = O.OraclePot.eam(CA.POT_DIR, "cu500")
-    s = O.OracleSim.fcc(pot, n, 3.615)
+    s = O.OracleSim.from_atoms(pot, bounds, xyz, p)
     s.rebox()
     t0 = time.perf_counter()
     s.force()
@@ -188,8 +216,52 @@ def cpu_baseline(n, steps):
     t0 = time.perf_counter()
     s.nsteps(steps)
     t = time.perf_counter() - t0 - tf
-    return dict(value=4 * n ** 3 * steps / t, unit="atom-steps/s", cores=cores, kind="port", sample=sample)
+    return dict(value=natoms * steps / t, unit="atom-steps/s", cores=1, kind="port", sample=sample, **extra)
 
+
+def timed_steps(sim, steps, warmup, natoms):
+    """atom-steps/s of steps replayed as the library runs them, after warmup."""
+    sim.steps(warmup)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.steps(steps)
+    sim.sync()
+    return natoms * steps / (time.perf_counter() - t0)
+
+
+def other_configs(seed):
+    """The other benchmark configurations, timed in the same run: plain
+    Lennard-Jones copper (FCC 20^3, 600 K, 100 steps, FP64, physical force
+    convention: in src-flat's the lattice flies apart within about 50
+    steps), EAM Cu FCC 40^3 (256 000 atoms, 0.05 A displacements, 300 K, 100
+    steps) by table and by 32-term Chebyshev series in FP64 and mixed
+    precision, the 64-replica batch and the Chebyshev-coefficient gradient."""
+    out = {}
+    lj = CA.copper_lj()
+    xyz = CA.fcc_positions(20, 20, 20, lj.lat)
+    xyz, p = CA.seed_state(xyz, lj.mass, 600.0, 0.0, seed)
+    s = CA.Simulation(lj, (20 * lj.lat,) * 3, defgrad=0.0)
+    s.tune(lj_physical=1)
+    s.bin_atoms(xyz, p)
+    s.rebox()
+    s.force()
+    out["lj_cu_fcc20_600K_fp64"] = timed_steps(s, 100, 5, len(xyz))
+    s.close()
+    cu = CA.EAMPotential("cu500")
+    xyz = CA.fcc_positions(40, 40, 40, 3.615)
+    xyz, p = CA.seed_state(xyz, cu.mass, 300.0, 0.05, seed)
+    for ev in ("table", "cheby"):
+        s = CA.Simulation(cu, (40 * 3.615,) * 3, defgrad=0.0, evaluation=ev)
+        s.bin_atoms(xyz, p)
+        s.rebox()
+        s.force()
+        for prec in ("fp64", "mixed"):
+            s.set_precision(prec)
+            s.force()
+            out[f"eam_cu_fcc40_{ev}_{prec}"] = timed_steps(s, 100, 5, len(xyz))
+        s.close()
+    log(f"other configs: {out}")
+    return {k: {"value": v, "unit": "atom-steps/s"} for k, v in out.items()}
 
 def main():
     args = parse()
@@ -262,9 +334,17 @@ def main():
     cpu = None
     if not args.no_cpu:
         try:
-            cpu = cpu_baseline(n, args.cpu_steps)
+            cpu = cpu_baseline(args.cpu_n, args.cpu_steps, args.seed)
         except Exception as exc:  # reported, not fatal
             print(f"[bench] cpu baseline failed: {exc}", file=sys.stderr)
+    extra = None
+    if not args.no_extra:
+        try:
+            extra = other_configs(args.seed)
+            extra["batch_64_replicas_mixed"] = batch_throughput(args, 200)
+            extra["adjoint_16x32000_cheb96"] = adjoint_throughput(args, 3)
+        except Exception as exc:  # reported, not fatal
+            print(f"[bench] other configurations failed: {exc}", file=sys.stderr)
 
     out = {
         "metric": "atom-steps/s, EAM Cu 2.05M atoms: mixed-prec vs FP64 on 1 GPU; % roofline",
@@ -330,12 +410,13 @@ def main():
                                     "seeded state, their energy drifts are energy_drift_per_atom_ha"),
         "device": devname,
         "setup_s": t_setup,
+        "other_configs": extra,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
 
 
-def bench_batch(args, devname):
+def run_batch(args, steps):
     """64 independent replicas, FCC n^3 with n uniform in 8..24, x-stretch
     uniform in [1.00, 1.10] (file-input grid rule), even index Lennard-Jones
     copper, odd index EAM Cu (cu500 tables), 300 K, mixed precision, in one
@@ -369,7 +450,6 @@ def bench_batch(args, devname):
     total = int(sizes.sum())
     tuned = b.autotune()
     log(f"batch: autotune {tuned}")
-    steps = args.steps
     b.steps(args.warmup)
     b.sync()
     pe0, ke0, _ = b.energy()
@@ -391,43 +471,62 @@ def bench_batch(args, devname):
     ms, launches = b.timing_read()
     b.timing(False)
     per_rep = sizes * steps / wall
+    res = dict(value=total * steps / wall, unit="atom-steps/s", steps=steps, ms_per_step=1e3 * wall / steps,
+               atoms=total, cells=b.ncells, atoms_list=atoms,
+               per_replica_atom_steps_per_s={"min": float(per_rep.min()), "median": float(np.median(per_rep)),
+                                             "max": float(per_rep.max())},
+               eager_ms_per_step=1e3 * wall_eager / ke, graph_speedup=(wall_eager / ke) / (wall / steps),
+               kernel_ms_per_step_eager={name: ms[i] / ke for i, name in enumerate(KCLASS)},
+               autotune={"waves": tuned[0], "blocks_per_cu": tuned[1], "balanced_drain": tuned[2],
+                         "force_ms": tuned[3]},
+               energy_drift_per_atom_ha=((pe1 + ke1) - (pe0 + ke0)) / total, setup_s=t_setup)
+    b.close()
+    return res
+
+
+def batch_throughput(args, steps):
+    res = run_batch(args, steps)
+    return {k: v for k, v in res.items() if k in ("value", "unit", "steps", "ms_per_step", "atoms", "graph_speedup",
+                                                  "energy_drift_per_atom_ha")}
+
+
+def bench_batch(args, devname):
+    res = run_batch(args, args.steps)
+    atoms = res.pop("atoms_list")
     out = {
         "metric": "atom-steps/s, 64-replica batch (FCC 8-24^3, x-stretch 1.00-1.10, half LJ half EAM Cu), "
                   "mixed precision, 1 GPU",
-        "value": total * steps / wall,
+        "value": res["value"],
         "unit": "atom-steps/s",
         "n_gpus": 1,
-        "steps": steps,
+        "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * wall / steps,
+        "ms_per_step": res["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 potential evaluation, f64 geometry, accumulation and integration",
         "data": "synthetic: FCC lattices, seeded Maxwell-Boltzmann velocities at 300 K",
         "config": {
-            "workload": f"64 replicas, {total} atoms, {b.ncells} link cells, one launch per kernel phase, "
+            "workload": f"64 replicas, {res['atoms']} atoms, {res['cells']} link cells, one launch per kernel phase, "
                         f"HIP-graph step; Lennard-Jones replicas in the physical force convention",
             "replicas": [dict(kind=a[4], n=a[2], atoms=int(a[0].shape[0]), stretch=round(a[3], 4))
                          for a in atoms],
             "parallelism": "single GPU",
         },
-        "per_replica_atom_steps_per_s": {"min": float(per_rep.min()), "median": float(np.median(per_rep)),
-                                         "max": float(per_rep.max())},
-        "eager_ms_per_step": 1e3 * wall_eager / ke,
-        "graph_speedup": (wall_eager / ke) / (wall / steps),
-        "kernel_ms_per_step_eager": {
-            name: ms[i] / ke for i, name in enumerate(KCLASS)},
-        "autotune": {"waves": tuned[0], "blocks_per_cu": tuned[1], "balanced_drain": tuned[2],
-                     "force_ms": tuned[3]},
-        "energy_drift_per_atom_ha": ((pe1 + ke1) - (pe0 + ke0)) / total,
+        "per_replica_atom_steps_per_s": res["per_replica_atom_steps_per_s"],
+        "eager_ms_per_step": res["eager_ms_per_step"],
+        "graph_speedup": res["graph_speedup"],
+        "kernel_ms_per_step_eager": res["kernel_ms_per_step_eager"],
+        "autotune": res["autotune"],
+        "energy_drift_per_atom_ha": res["energy_drift_per_atom_ha"],
         "device": devname,
-        "setup_s": t_setup,
+        "setup_s": res["setup_s"],
     }
     print(json.dumps(out))
 
 
-def bench_adjoint(args, devname):
+def run_adjoint(args, reps):
     """Gradient of the force-matching loss with respect to the 96 Chebyshev
     coefficients, over 16 configurations of Cu FCC 20^3 (32 000 atoms each,
     displacements sigma 0.1 A), targets from the FP64 forward at 1 %-perturbed
@@ -455,7 +554,6 @@ def bench_adjoint(args, devname):
     del fts
     natoms = nconf * 4 * n ** 3
     res = {}
-    reps = max(3, args.steps // 20)
     for prec in ("fp64", "mixed"):
         b.set_precision(prec)
         b.cheb_grad(c0, None, e_t, w_e)  # warm-up
@@ -473,6 +571,19 @@ def bench_adjoint(args, devname):
         res[prec] = dict(seconds_per_gradient=wall, atom_gradients_per_s=natoms / wall, loss=loss,
                          forward_seconds=fwd, gradient_over_forward=wall / fwd, grad=grad)
     b.close()
+    return res, natoms
+
+
+def adjoint_throughput(args, reps):
+    res, natoms = run_adjoint(args, reps)
+    return {"value": res["mixed"]["atom_gradients_per_s"], "unit": "atom-gradients/s",
+            "seconds_per_gradient_mixed": res["mixed"]["seconds_per_gradient"],
+            "seconds_per_gradient_fp64": res["fp64"]["seconds_per_gradient"], "atoms": natoms}
+
+
+def bench_adjoint(args, devname):
+    reps = max(3, args.steps // 20)
+    res, natoms = run_adjoint(args, reps)
     rel = np.abs(res["mixed"]["grad"] - res["fp64"]["grad"]).max(axis=1) / np.abs(res["fp64"]["grad"]).max(axis=1)
     out = {
         "metric": "Chebyshev-coefficient gradients: atom-gradients/s over 16 x Cu FCC 20^3, 1 GPU",
