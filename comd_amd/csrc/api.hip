@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <array>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -23,6 +24,37 @@ static const double kBohrPerAtuToAPerS = (double)0.529 / (double)2.418e-17;
 static const double kAmuToMe = (double)1822.83;
 
 static char g_create_err[512] = "";
+
+// Scratch of comd_observe_noalloc: the block partials of the observables
+// reduction (CM_OBS_MAXBLOCKS x CM_OBS_N doubles, 256 KiB), one copy per
+// device, held until the process ends.  Its name is a host-side stand-in:
+// host code uses only the device address that hipGetSymbolAddress gives
+// for the current device (scratch_of).
+static __device__ __align__(256) double scratch[CM_OBS_MAXBLOCKS * CM_OBS_N];
+
+#define CM_MAX_DEVICES 64
+static double *g_scratch_ptr[CM_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+// The address of scratch on device dev, looked up once per device with that
+// device current (at context creation); null when the lookup failed.
+static double *scratch_lookup(int dev)
+{
+    if (dev < 0 || dev >= CM_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) == hipSuccess) g_scratch_ptr[dev] = (double *)p;
+    }
+    return g_scratch_ptr[dev];
+}
+
+static double *scratch_of(int dev)
+{
+    if (dev < 0 || dev >= CM_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_ptr[dev];
+}
 
 struct TimedEvent {
     int cls;
@@ -721,6 +753,7 @@ static int ctx_create(int nrep, const comd_params *prms, comd_ctx **out)
     c->ncells = (int)ncl;
     c->nslots = ncl * CM_SLOTS;
     hipGetDevice(&c->device);
+    scratch_lookup(c->device);  // with the context's device current
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return set_err(nullptr, COMD_ERR_HIP, "stream creation failed");
@@ -1848,6 +1881,21 @@ int comd_batch_observe(comd_ctx *c, int r, comd_observables *out)
     if (!c) return COMD_ERR_ARG;
     int rc = check_rep(c, r);
     return rc ? rc : observables_rep(c, r, out);
+}
+
+int comd_observe_noalloc(comd_ctx *c, comd_observables *d_out)
+{
+    if (!c || !d_out) return COMD_ERR_ARG;
+    if (!c->have_atoms) return set_err(c, COMD_ERR_STATE, "no atoms");
+    double *ptr = scratch_of(c->device);
+    if (!ptr) return set_err(c, COMD_ERR_HIP, "no scratch address for device %d", c->device);
+    if (cm_observe_blocks(c->reps[0].g) > CM_OBS_MAXBLOCKS)
+        return set_err(c, COMD_ERR_ARG, "more than %d reduction blocks", CM_OBS_MAXBLOCKS);
+    const int timing = c->timing;  // timing events would be created on demand
+    c->timing = 0;
+    const int rc = observe_rep(c, 0, ptr, d_out);
+    c->timing = timing;
+    return rc;
 }
 
 static int centro_rep(comd_ctx *c, int r, double *out)

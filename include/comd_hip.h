@@ -276,6 +276,20 @@ typedef struct comd_observables {
 int comd_observe(comd_ctx *ctx, comd_observables *out);
 int comd_batch_observe(comd_ctx *ctx, int r, comd_observables *out);
 
+/* comd_observe that allocates nothing, for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses
+ * that forbid it).  No counterpart in the reference.  The same force call
+ * and reduction as comd_observe, stream-ordered on the context's stream
+ * with no host synchronisation; the result is written to d_out, a
+ * comd_observables in the caller's device memory.  The reduction's block
+ * partials live in a static device array of 256 KiB (2048 blocks x 16
+ * doubles; every simulation size fits, the reduction's grid being capped
+ * at 2048 blocks), one per device, kept until the process ends: calls that
+ * overlap on one device (contexts on different streams) race on it.
+ * Errors of the force kernels surface at the next synchronising call
+ * (comd_sync).  Single simulations (replica 0 of a batch). */
+int comd_observe_noalloc(comd_ctx *ctx, comd_observables *d_out);
+
 /* Energy-conservation monitor: the potential and kinetic energy (hartree)
  * after a force at the starting positions, then after each of nblocks
  * calls of nTimeSteps(block_steps, dt) -- src-flat/docompute.c:196-207's
