@@ -84,6 +84,8 @@ def run_precision(sim, precision, steps, warmup, n_atoms, timed_steps):
     events break the graph into separate launches, so these steps are not
     the throughput figure)."""
     sim.set_precision(precision)
+    tuned = sim.autotune_lists() if sim.list_info()["active"] else None  # setup, not timed
+    log(f"{precision}: list passes autotuned: {tuned}")
     sim.steps(warmup)
     sim.sync()
     sim.force()
@@ -122,6 +124,7 @@ def run_precision(sim, precision, steps, warmup, n_atoms, timed_steps):
         lists=bool(info["active"] and info["valid"]),
         builds=info["builds"] - b0,
         kmax=info["kmax"],
+        autotune=tuned,
     )
 
 
@@ -310,8 +313,9 @@ def main():
     # time), timed with HIP events on the stream it runs on: the list pass
     # while the neighbour lists are in use, else the link-cell pass
     lists = mixed["lists"]
-    kd = "k_nl_pair<0, 0, 1, 1>" if lists else "k_pair<0, 0, 1>"
-    ke = "k_nl_pair<1, 0, 1, 1>" if lists else "k_pair<1, 0, 1>"
+    rows = int(mixed["autotune"] is None or mixed["autotune"]["rows_lds"] != 0)  # table rows in LDS
+    kd = f"k_nl_pair<0, 0, 1, {rows}>" if lists else "k_pair<0, 0, 1>"
+    ke = f"k_nl_pair<1, 0, 1, {rows}>" if lists else "k_pair<1, 0, 1>"
     passes = {
         CA.K_DENSITY: (kd, "density + embedding pass (src-flat/eam.c:484-594), table, mixed",
                        lambda ph: (FLOP_PAIR_DENSITY * ph + FLOP_ATOM_EMBED) * n_atoms),
@@ -403,7 +407,8 @@ def main():
                           "note": "HIP events around each kernel class on the simulation's stream, over steps run "
                                   "after the timed ones (the events split the step graph into separate launches)"},
         "neighbour_lists": {"active": mixed["lists"], "skin_A": 0.4, "builds_in_timed_steps": mixed["builds"],
-                            "list_capacity": mixed["kmax"]},
+                            "list_capacity": mixed["kmax"],
+                            "autotuned_shape": {"mixed": mixed["autotune"], "fp64": fp64["autotune"]}},
         "energy_drift_per_atom_ha": (mixed["e1"] - mixed["e0"]) / n_atoms,
         "mixed_error_vs_fp64": dict(force_err, note="per-atom |df| / max(|f|, rms |f|) on the thermalised state "
                                     "at the end of the FP64 run; both paths run the timed steps from the same "
@@ -448,7 +453,8 @@ def run_batch(args, steps):
     log(f"batch: binned and first force in {t_setup:.2f} s")
     sizes = np.array([a[0].shape[0] for a in atoms])
     total = int(sizes.sum())
-    tuned = b.autotune()
+    tuned = b.autotune_lists()  # the list passes' group shape, rows in LDS, atoms per thread
+    tuned["occupancy"] = b.list_occupancy()
     log(f"batch: autotune {tuned}")
     b.steps(args.warmup)
     b.sync()
@@ -477,8 +483,7 @@ def run_batch(args, steps):
                                              "max": float(per_rep.max())},
                eager_ms_per_step=1e3 * wall_eager / ke, graph_speedup=(wall_eager / ke) / (wall / steps),
                kernel_ms_per_step_eager={name: ms[i] / ke for i, name in enumerate(KCLASS)},
-               autotune={"waves": tuned[0], "blocks_per_cu": tuned[1], "balanced_drain": tuned[2],
-                         "force_ms": tuned[3]},
+               autotune=tuned,
                energy_drift_per_atom_ha=((pe1 + ke1) - (pe0 + ke0)) / total, setup_s=t_setup)
     b.close()
     return res

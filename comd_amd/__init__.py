@@ -36,7 +36,7 @@ EVAL_TABLE, EVAL_CHEB = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 K_REBIN, K_DENSITY, K_EMBED_FORCE, K_REDUCE, K_LIST_BUILD, K_CELL_FALLBACK, K_NCLASS = 0, 1, 2, 3, 4, 5, 6
 TUNE_WAVES, TUNE_BLOCKS_PER_CU, TUNE_TABLES_LDS, TUNE_BALANCED, TUNE_GRAPH, TUNE_PAIR_LISTS, TUNE_SKIN = 1, 2, 3, 4, 5, 6, 7
-TUNE_LJ_PHYSICAL, TUNE_LIST_GROUPS, TUNE_LIST_ROWS_LDS = 8, 9, 10
+TUNE_LJ_PHYSICAL, TUNE_LIST_GROUPS, TUNE_LIST_ROWS_LDS, TUNE_LIST_THREADS_DIV = 8, 9, 10, 11
 
 # unit constants of src-flat/include/constants.h:61-65
 BOHR_PER_ATU_TO_A_PER_S = 0.529 / 2.418e-17
@@ -168,12 +168,15 @@ def _load():
     hip.comd_batch_energy.argtypes = [vp, C.c_int, _dp, _dp, _dp]
     hip.comd_batch_step.argtypes = [vp, C.c_int, C.c_double]
     hip.comd_autotune.argtypes = [vp, _ip, _ip, _ip, _dp]
+    hip.comd_autotune_lists.argtypes = [vp, _ip, _ip, _ip, _dp]
     hip.comd_list_info.argtypes = [vp, _ip, _ip, _ip, _ip]
     hip.comd_list_occupancy.argtypes = [vp, _ip, _ip, _ip]
     hip.comd_set_cheb.argtypes = [vp, _dp, C.c_int]
     hip.comd_get_cheb.argtypes = [vp, _dp]
     hip.comd_eam_cheb_grad.argtypes = [vp, _dp, C.c_int, _dp, C.c_double, C.c_double, _dp, _dp]
     hip.comd_batch_cheb_grad.argtypes = [vp, _dp, C.c_int, _dp, _dp, C.c_double, _dp, _dp]
+    hip.comd_eam_cheb_energy_grad.argtypes = [vp, _dp, C.c_int, _dp, _dp]
+    hip.comd_batch_cheb_energy_grad.argtypes = [vp, _dp, C.c_int, _dp, _dp, _dp]
     hip.comd_observe.argtypes = [vp, C.POINTER(Observables)]
     hip.comd_batch_observe.argtypes = [vp, C.c_int, C.POINTER(Observables)]
     hip.comd_run_monitored.argtypes = [vp, C.c_int, C.c_int, C.c_double, _dp]
@@ -502,7 +505,8 @@ class Simulation:
         return pe.value, ke.value, st.value
 
     def tune(self, waves_per_block=None, blocks_per_cu=None, tables_in_lds=None, balanced=None, graph=None,
-             pair_lists=None, skin=None, lj_physical=None, list_groups=None, list_rows_lds=None):
+             pair_lists=None, skin=None, lj_physical=None, list_groups=None, list_rows_lds=None,
+             list_threads_div=None):
         """Launch shape of the force kernels, step replay as a HIP graph,
         pair-list replay in the EAM embedding-force pass, the Verlet skin in A
         (0 turns the neighbour lists off) and the Lennard-Jones force sign
@@ -511,7 +515,7 @@ class Simulation:
                          (TUNE_TABLES_LDS, tables_in_lds), (TUNE_BALANCED, balanced), (TUNE_GRAPH, graph),
                          (TUNE_PAIR_LISTS, pair_lists), (TUNE_SKIN, None if skin is None else round(1000 * skin)),
                          (TUNE_LJ_PHYSICAL, lj_physical), (TUNE_LIST_GROUPS, list_groups),
-                         (TUNE_LIST_ROWS_LDS, list_rows_lds)):
+                         (TUNE_LIST_ROWS_LDS, list_rows_lds), (TUNE_LIST_THREADS_DIV, list_threads_div)):
             if val is not None:
                 self._check(self._hip.comd_tune(self.ctx, key, int(val)))
 
@@ -528,9 +532,19 @@ class Simulation:
         self._check(self._hip.comd_list_occupancy(self.ctx, b, l, C.byref(t)))
         return dict(blocks_per_cu=list(b), lds_bytes=list(l), threads=t.value)
 
+    def autotune_lists(self):
+        """Time the list passes over group shapes, table rows in LDS or not
+        and home atoms per thread, and keep the fastest (comd_autotune_lists);
+        returns dict(groups, rows_lds, threads_div, ms)."""
+        g, r, d = C.c_int(), C.c_int(), C.c_int()
+        ms = C.c_double()
+        self._check(self._hip.comd_autotune_lists(self.ctx, C.byref(g), C.byref(r), C.byref(d), C.byref(ms)))
+        return dict(groups=g.value, rows_lds=r.value, threads_div=d.value, ms=ms.value)
+
     def autotune(self):
-        """Time the force passes over launch shapes and keep the fastest
-        (comd_autotune); returns (waves, blocks_per_cu, balanced, ms)."""
+        """Time the force passes that run over launch shapes and keep the
+        fastest (comd_autotune: the list passes' shape when the context has
+        lists); returns (waves, blocks_per_cu, balanced, ms)."""
         w, b, d = C.c_int(), C.c_int(), C.c_int()
         ms = C.c_double()
         self._check(self._hip.comd_autotune(self.ctx, C.byref(w), C.byref(b), C.byref(d), C.byref(ms)))
@@ -625,6 +639,15 @@ class Simulation:
         self._check(self._hip.comd_eam_cheb_grad(self.ctx, _ptr(c), c.shape[1], _ptr(ft), float(e_target),
                                                  float(w_e), C.byref(loss), _ptr(grad)))
         return loss.value, grad
+
+    def cheb_energy_grad(self, coef):
+        """Potential energy E and its gradient dE/dc (3 x n) at the value
+        coefficients `coef` (comd_eam_cheb_energy_grad)."""
+        c = np.ascontiguousarray(coef, dtype=np.float64).reshape(3, -1)
+        e = C.c_double()
+        grad = np.zeros_like(c)
+        self._check(self._hip.comd_eam_cheb_energy_grad(self.ctx, _ptr(c), c.shape[1], C.byref(e), _ptr(grad)))
+        return e.value, grad
 
 
 class Batch(Simulation):
@@ -733,6 +756,20 @@ class Batch(Simulation):
         self._check(self._hip.comd_batch_cheb_grad(self.ctx, _ptr(c), c.shape[1], _ptr(ft), _ptr(et), float(w_e),
                                                    C.byref(loss), _ptr(grad)))
         return loss.value, grad
+
+    def cheb_energy_grad(self, coef, weights=None):
+        """Energies E_r of the replicas and the gradient of sum_r w_r E_r
+        (weights None: all 1) with respect to the value coefficients `coef`
+        (comd_batch_cheb_energy_grad)."""
+        c = np.ascontiguousarray(coef, dtype=np.float64).reshape(3, -1)
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+        if w is not None and w.shape != (self.nrep,):
+            raise ValueError(f"weights needs {self.nrep} values")
+        e = np.zeros(self.nrep)
+        grad = np.zeros_like(c)
+        self._check(self._hip.comd_batch_cheb_energy_grad(self.ctx, _ptr(c), c.shape[1], _ptr(w), _ptr(e),
+                                                          _ptr(grad)))
+        return e, grad
 
 
 def fcc_simulation(pot, n, lat=None, temp=0.0, disp=0.0, seed=1, precision="fp64", evaluation="table",

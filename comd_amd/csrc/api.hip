@@ -136,6 +136,7 @@ struct comd_ctx {
     int nl_invalid_seen; // the lists were invalid when the host last looked
     int nl_group_opt;   // COMD_TUNE_LIST_GROUPS (-1: by precision)
     int nl_rows_lds;    // COMD_TUNE_LIST_ROWS_LDS (-1: whenever they fit)
+    int nl_threads_div; // COMD_TUNE_LIST_THREADS_DIV: home atoms per thread of the list passes
     long nl_alloc_n, nl_alloc_k, nl_alloc_aux, nl_alloc_groups, nl_alloc_rep;
     int nl_eam_groups, nl_lj_groups;  // the list groups of the EAM replicas (first), then of the LJ ones
     int nl_eam_rep, nl_lj_rep;        // a replica of each kind (their potentials stand for the kind's)
@@ -883,6 +884,7 @@ static int ctx_create(int nrep, const comd_params *prms, comd_ctx **out)
     c->nl_skin_milli = 400;
     c->nl_group_opt = -1;
     c->nl_rows_lds = -1;
+    c->nl_threads_div = 1;
     if (hipStreamSynchronize(c->stream) != hipSuccess) {
         free_all(c);
         delete c;
@@ -1019,7 +1021,7 @@ static int nl_setup(comd_ctx *c)
             const long cells = (long)gd[r][0] * gd[r][1] * gd[r][2];
             const long slots = (long)(gd[r][0] + 2) * (gd[r][1] + 2) * (gd[r][2] + 2);
             hcap = std::max<long>(hcap, hcap_of(occ, slots));
-            threads = std::max<long>(threads, ((long)(1.1 * occ * (double)cells) + 63) / 64 * 64);
+            threads = std::max<long>(threads, ((long)(1.1 * occ * (double)cells / c->nl_threads_div) + 63) / 64 * 64);
             naux += nr;
             ngroups += (long)ng[r][0] * ng[r][1] * ng[r][2];
         }
@@ -1633,6 +1635,11 @@ int comd_tune(comd_ctx *c, int key, int value)
         c->nl_rows_lds = value;
         c->nl.rows_lds = value;
         break;
+    case COMD_TUNE_LIST_THREADS_DIV:
+        if (value < 1 || value > 4) return set_err(c, COMD_ERR_ARG, "list threads divisor must be 1..4");
+        c->nl_threads_div = value;
+        if (c->have_atoms) return nl_setup(c);
+        break;
     case COMD_TUNE_SKIN_MILLI:
         if (value < 0 || value > 3000) return set_err(c, COMD_ERR_ARG, "skin must be 0..3000 (1e-3 A)");
         c->nl_skin_milli = value;
@@ -1645,10 +1652,108 @@ int comd_tune(comd_ctx *c, int key, int value)
     return COMD_OK;
 }
 
+// Time of the fastest of 3 force calls after one untimed call (the list
+// build, when one is due, happens in the untimed call).
+static int time_force(comd_ctx *c, hipEvent_t a, hipEvent_t b, float *ms)
+{
+    op_force(c);
+    float t = 0, tmin = 1e30f;
+    for (int k = 0; k < 3; k++) {
+        hipEventRecord(a, c->stream);
+        op_force(c);
+        hipEventRecord(b, c->stream);
+        if (hipEventSynchronize(b) != hipSuccess) return set_err(c, COMD_ERR_HIP, "autotune timing failed");
+        hipEventElapsedTime(&t, a, b);
+        tmin = std::min(tmin, t);
+    }
+    *ms = tmin;
+    return COMD_OK;
+}
+
+// The list passes' launch shape by timing: the group of auxiliary cells
+// (the per-replica search of batches, then the five fixed options), the
+// table rows in LDS or read from global memory, and one or two home atoms
+// per thread; each option rebuilds the lists (its groups, halo capacity
+// and threads per block follow from the shape).  Options whose lists do
+// not fit (no list passes, or a build that overflowed) are skipped.
+static int autotune_lists(comd_ctx *c, int *group_opt, int *rows_lds, int *threads_div, double *best_ms)
+{
+    const int g0 = c->nl_group_opt, r0 = c->nl_rows_lds, d0 = c->nl_threads_div;
+    hipEvent_t a, b;
+    HIPCHK(c, hipEventCreate(&a));
+    HIPCHK(c, hipEventCreate(&b));
+    bool rows_matter = false;
+    for (const Replica &R : c->reps)
+        rows_matter = rows_matter || cm_nl_tab_bytes(R.pot, CM_KIND_EAM1, c->precision == COMD_PREC_MIXED) > 0;
+    double best = 1e300;
+    int bg = g0, br = r0, bd = d0;
+    int rc = COMD_OK;
+    for (int g = -1; g <= 4 && !rc; g++)
+        for (int r = rows_matter ? 0 : -1; r >= -1 && !rc; r--)
+            for (int dv = 1; dv <= 2 && !rc; dv++) {
+                c->nl_group_opt = g;
+                c->nl_rows_lds = r;
+                c->nl_threads_div = dv;
+                rc = nl_setup(c);
+                if (rc || !c->nl_active) continue;
+                float ms = 0;
+                rc = time_force(c, a, b, &ms);
+                if (rc) break;
+                unsigned f = 0;
+                if (hipMemcpy(&f, c->nl.flags, 4, hipMemcpyDeviceToHost) != hipSuccess) {
+                    rc = set_err(c, COMD_ERR_HIP, "autotune flag read failed");
+                    break;
+                }
+                if (f & CM_NL_INVALID) continue;
+                if (ms < best) {
+                    best = ms;
+                    bg = g;
+                    br = r;
+                    bd = dv;
+                }
+            }
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    c->nl_group_opt = bg;
+    c->nl_rows_lds = br;
+    c->nl_threads_div = bd;
+    const int rc2 = nl_setup(c);
+    if (rc) return rc;
+    if (rc2) return rc2;
+    op_force(c);  // the lists built again with the choice, and the forces of the state
+    HIPCHK(c, hipGetLastError());
+    if (group_opt) *group_opt = bg;
+    if (rows_lds) *rows_lds = br;
+    if (threads_div) *threads_div = bd;
+    if (best_ms) *best_ms = best < 1e299 ? best : -1.0;
+    return COMD_OK;
+}
+
+int comd_autotune_lists(comd_ctx *c, int *group_opt, int *rows_lds, int *threads_div, double *best_ms)
+{
+    if (!c) return COMD_ERR_ARG;
+    if (!c->have_atoms) return set_err(c, COMD_ERR_STATE, "no atoms");
+    if (!c->nl_active) return set_err(c, COMD_ERR_STATE, "no neighbour lists in this context");
+    const int timing_saved = c->timing;
+    c->timing = 0;
+    const int rc = autotune_lists(c, group_opt, rows_lds, threads_div, best_ms);
+    c->timing = timing_saved;
+    return rc;
+}
+
 int comd_autotune(comd_ctx *c, int *waves, int *blocks_per_cu, int *balanced, double *best_ms)
 {
     if (!c) return COMD_ERR_ARG;
     if (!c->have_atoms) return set_err(c, COMD_ERR_STATE, "no atoms");
+    if (c->nl_active) {
+        // the list passes are what runs: their shape is tuned
+        const int rc = comd_autotune_lists(c, nullptr, nullptr, nullptr, best_ms);
+        if (rc) return rc;
+        if (waves) *waves = c->force_waves;
+        if (blocks_per_cu) *blocks_per_cu = c->force_blocks_per_cu;
+        if (balanced) *balanced = c->reps[0].pot.balanced;
+        return COMD_OK;
+    }
     const int timing_saved = c->timing;
     const int nl_saved = c->nl_active;
     c->timing = 0;
@@ -2122,12 +2227,16 @@ int comd_get_cheb(comd_ctx *c, double *coef6)
 // Loss and gradient over every replica of the context (one replica for
 // comd_eam_cheb_grad): L = sum_r [ sum_{i in r} |f_i - t_i|^2 / (3 N_r)
 // + w_e (E_r - e_target[r])^2 ], the replicas' moments summed by the same
-// kernels in one launch per stage.
+// kernels in one launch per stage.  Energy mode (e_weight non-null): L =
+// sum_r e_weight[r] E_r, the same stages with no force term (dL/df = 0)
+// and dL/dE_r = e_weight[r]; energies[r] receives E_r.
 static int cheb_grad(comd_ctx *c, const double *coef, int n, const double *f_target, const double *e_target,
-                     double w_e, double *loss, double *grad)
+                     double w_e, double *loss, double *grad, const double *e_weight = nullptr,
+                     double *energies = nullptr)
 {
+    const bool emode = e_weight != nullptr;
     if (!c->have_atoms || c->ntot < 1) return set_err(c, COMD_ERR_STATE, "no atoms");
-    if (!f_target && !c->adj_have_target) return set_err(c, COMD_ERR_STATE, "no force target");
+    if (!emode && !f_target && !c->adj_have_target) return set_err(c, COMD_ERR_STATE, "no force target");
     if (n > cm_adjoint_max_coefs())
         return set_err(c, COMD_ERR_ARG, "the adjoint supports at most %d coefficients per series",
                        cm_adjoint_max_coefs());
@@ -2164,7 +2273,7 @@ static int cheb_grad(comd_ctx *c, const double *coef, int n, const double *f_tar
     cheb_derivative(m, R0.pot.ca[2], R0.pot.cb[2], d1.data(), d2.data());
     d2.resize((size_t)R0.pot.cstride, 0.0);
     HIPCHK(c, hipMemcpy(c->adj_f2, d2.data(), d2.size() * 8, hipMemcpyHostToDevice));
-    if (f_target) {
+    if (f_target && !emode) {
         HIPCHK(c, hipMemcpy(c->adj_ft, f_target, S * 3 * 8, hipMemcpyHostToDevice));
         c->adj_have_target = 1;
     }
@@ -2190,6 +2299,13 @@ static int cheb_grad(comd_ctx *c, const double *coef, int n, const double *f_tar
     for (int r = 0; r < c->nrep; r++) {
         rc = energy_rep(c, r, &E[r], nullptr, nullptr);
         if (rc) return rc;
+        if (energies) energies[r] = E[r];
+        if (emode) {
+            rep[r] = e_weight[r];
+            rep[c->nrep + r] = 0.0;
+            lossE += e_weight[r] * E[r];
+            continue;
+        }
         rep[r] = 2.0 * w_e * (E[r] - e_target[r]);
         rep[c->nrep + r] = c->reps[r].ntot > 0 ? 1.0 / (3.0 * (double)c->reps[r].ntot) : 0.0;
         lossE += w_e * (E[r] - e_target[r]) * (E[r] - e_target[r]);
@@ -2212,14 +2328,18 @@ static int cheb_grad(comd_ctx *c, const double *coef, int n, const double *f_tar
     PotDev P = R0.pot;  // series, domains and cutoff common to the replicas
     P.eval = COMD_EVAL_CHEB;
     DevState d = dev_state(c);
-    cm_launch_adj_resid(d, J, c->adj_ft, c->stream);
+    if (emode) {  // no force term: dL/df = 0
+        for (int k = 0; k < 3; k++) HIPCHK(c, hipMemsetAsync(c->adj_g[k], 0, S * 8, c->stream));
+    } else {
+        cm_launch_adj_resid(d, J, c->adj_ft, c->stream);
+    }
     for (int stage = 0; stage < 4; stage++) cm_launch_adj_stage(d, P, J, stage, mixed, c->stream);
     HIPCHK(c, hipGetLastError());
     std::vector<double> out((size_t)5 * NM);
     HIPCHK(c, hipMemcpyAsync(out.data(), c->adj_out, out.size() * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
 
-    if (loss) *loss = out[(size_t)4 * NM] + lossE;
+    if (loss) *loss = (emode ? 0.0 : out[(size_t)4 * NM]) + lossE;
     if (!grad) return COMD_OK;
     // B_0 = T_0 / 2 in the series convention of src-flat/cheby.c:309; the
     // energy moments (set 1 of stages 0 and 2) carry each replica's dL/dE
@@ -2260,6 +2380,23 @@ int comd_batch_cheb_grad(comd_ctx *c, const double *coef, int n, const double *f
 {
     if (!c || !coef || !e_target) return COMD_ERR_ARG;
     return cheb_grad(c, coef, n, f_target, e_target, w_e, loss, grad);
+}
+
+int comd_eam_cheb_energy_grad(comd_ctx *c, const double *coef, int n, double *energy, double *grad)
+{
+    if (!c || !coef) return COMD_ERR_ARG;
+    if (c->nrep != 1) return set_err(c, COMD_ERR_ARG, "a batch takes comd_batch_cheb_energy_grad");
+    const double one = 1.0;
+    return cheb_grad(c, coef, n, nullptr, nullptr, 0.0, nullptr, grad, &one, energy);
+}
+
+int comd_batch_cheb_energy_grad(comd_ctx *c, const double *coef, int n, const double *weights, double *energies,
+                                double *grad)
+{
+    if (!c || !coef) return COMD_ERR_ARG;
+    std::vector<double> w((size_t)c->nrep, 1.0);
+    if (weights) w.assign(weights, weights + c->nrep);
+    return cheb_grad(c, coef, n, nullptr, nullptr, 0.0, nullptr, grad, w.data(), energies);
 }
 
 }  // extern "C"

@@ -182,16 +182,32 @@ enum comd_tune_key {
                                       groups per compute unit, else the one with the most groups) */
     COMD_TUNE_LIST_ROWS_LDS = 10,  /* -1 (default, = 1), 0, 1: table rows of the list passes staged in
                                       LDS when they fit beside the halo */
+    COMD_TUNE_LIST_THREADS_DIV = 11, /* 1 (default) or 2..4: home atoms per thread of the list passes
+                                      (threads per block = a group's atoms / this, in waves) */
 };
 int comd_tune(comd_ctx *ctx, int key, int value);
 
-/* Pick the force kernels' launch shape by timing: cells (waves) per
- * workgroup, persistent workgroups per CU, and how an atom's accepted pairs
- * are split over its lanes (balanced or per lane).  Force calls only; the
- * state does not change.  The choice is applied and returned; the drain
- * split changes the summation order, so forces may differ in the last bits
- * between the two splits (both within the parity bar). */
+/* Pick the force kernels' launch shape by timing force calls, for the
+ * kernels that run.  With neighbour lists (the default for periodic boxes
+ * and batches): comd_autotune_lists, and waves, blocks_per_cu and balanced
+ * return the link-cell settings unchanged.  Without: the link-cell kernels'
+ * cells (waves) per workgroup, persistent workgroups per CU, and how an
+ * atom's accepted pairs are split over its lanes (balanced or per lane).
+ * The state does not change; the choice is applied and returned.  A new
+ * shape changes the order of the per-atom sums, so forces may differ in
+ * the last bits (within the parity bar). */
 int comd_autotune(comd_ctx *ctx, int *waves, int *blocks_per_cu, int *balanced, double *best_ms);
+
+/* The list passes' shape by timing: the group of auxiliary cells
+ * (COMD_TUNE_LIST_GROUPS -1..4), the table rows in LDS or global memory
+ * (COMD_TUNE_LIST_ROWS_LDS 0/-1, tried when table rows are used) and home
+ * atoms per thread (COMD_TUNE_LIST_THREADS_DIV 1/2); the halo capacity and
+ * threads per block follow from the group shape.  Each option rebuilds the
+ * lists; ones whose lists overflow are skipped.  The fastest is applied (the
+ * lists rebuilt and the forces of the state computed with it) and
+ * returned; best_ms is the time of one force call with it.  Errors
+ * COMD_ERR_STATE when the context has no lists. */
+int comd_autotune_lists(comd_ctx *ctx, int *group_opt, int *rows_lds, int *threads_div, double *best_ms);
 
 /* State of the Verlet neighbour lists (COMD_TUNE_SKIN_MILLI): whether the
  * context uses them, whether the last build fitted (otherwise the link-cell
@@ -385,6 +401,19 @@ int comd_eam_cheb_grad(comd_ctx *ctx, const double *coef, int n, const double *f
  * reference. */
 int comd_batch_cheb_grad(comd_ctx *ctx, const double *coef, int n, const double *f_target, const double *e_target,
                          double w_e, double *loss, double *grad);
+
+/* Gradient of the potential energy alone with respect to the value
+ * coefficients of phi, rho and F (grad: 3 x n, as comd_eam_cheb_grad's),
+ * at coef, through the same adjoint kernels with no force term: dE/dc_phi
+ * = 1/2 sum_ij T_k(r_ij), dE/dc_rho = sum_ij F'(rho_i) T_k(r_ij), dE/dc_F =
+ * sum_i T_k(rho_i) (T_0 halved as the series convention has it).  energy
+ * receives E.  No counterpart in the reference (the north star's energy
+ * gradient, beside the force-matching loss).  The batch form returns the
+ * gradient of sum_r w_r E_r (weights NULL: all 1) and each E_r in
+ * energies[nrep] (may be NULL). */
+int comd_eam_cheb_energy_grad(comd_ctx *ctx, const double *coef, int n, double *energy, double *grad);
+int comd_batch_cheb_energy_grad(comd_ctx *ctx, const double *coef, int n, const double *weights, double *energies,
+                                double *grad);
 
 #pragma GCC visibility pop
 

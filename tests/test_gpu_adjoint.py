@@ -210,3 +210,100 @@ def test_batch_gradient_equals_sum_of_single_contexts(gpu, cu500):
     assert loss_c == loss_b
     np.testing.assert_array_equal(grad_c, grad_b)
     b.close()
+
+
+# ---------------------------------------------------------------------------
+# gradient of the energy alone (comd_eam_cheb_energy_grad)
+
+def cheb_basis(x, dom, m):
+    """dV/dc_k of the series value sum_{k>=1} c_k T_k(y) + c_0 / 2 at x
+    (clamped below at the domain start, as the evaluation clamps it)."""
+    a, b = dom
+    y = (2.0 * np.maximum(x, a) - a - b) / (b - a)
+    t = np.zeros((m,) + np.shape(x))
+    t[0] = 1.0
+    if m > 1:
+        t[1] = y
+    for k in range(2, m):
+        t[k] = 2.0 * y * t[k - 1] - t[k - 2]
+    t[0] = 0.5
+    return t
+
+
+def test_energy_gradient_closed_form_and_oracle_fd(gpu, cu500):
+    """dE/dc at the fitted coefficients against (1) the closed forms
+    dE/dc_phi = 1/2 sum_{i != j} B_k(r_ij), dE/dc_rho = sum_i F'(rho_i)
+    sum_j B_k(r_ij), dE/dc_F = sum_i B_k(rho_i), from the pair distances of
+    the atoms and the oracle's rho and F', and (2) central differences of
+    the oracle's energy; the energy equals the oracle's."""
+    c0 = cu500.cheb[:3].copy()
+    m = c0.shape[1]
+    sim, xyz, bounds = config(cu500, 5, 13)
+    e, g = sim.cheb_energy_grad(c0)
+    opot = O.OraclePot.eam(POTS, "cu500")
+    opot.set_eval(True)
+    osim = O.OracleSim.from_atoms(opot, bounds, xyz)
+    osim.rebox()
+
+    def oracle_energy(coef):
+        opot.set_cheb3(coef)
+        osim.force()
+        return osim.info()["e"]
+
+    e0 = oracle_energy(c0)
+    print(f"[energy gradient] E gpu {e:.12e} oracle {e0:.12e}", flush=True)
+    assert abs(e - e0) <= 1e-12 * abs(e0)
+    # closed forms over ordered pairs inside the cutoff (minimum image)
+    L = np.array(bounds)
+    d = xyz[:, None, :] - xyz[None, :, :]
+    d -= L * np.rint(d / L)
+    r = np.sqrt((d * d).sum(-1))
+    np.fill_diagonal(r, np.inf)
+    inside = r * r <= cu500.cutoff ** 2
+    ii, jj = np.nonzero(inside)
+    bphi = cheb_basis(r[ii, jj], cu500.cheb_dom[0], m)
+    brho = cheb_basis(r[ii, jj], cu500.cheb_dom[1], m)
+    st = osim.state()
+    occ = (np.arange(64)[None, :] < st["natoms"][:, None]).reshape(-1)
+    rho = np.zeros(len(xyz))
+    fp = np.zeros(len(xyz))
+    rho[st["id"][occ]] = st["rho"][occ]
+    fp[st["id"][occ]] = st["fi"][occ]
+    closed = np.stack([0.5 * bphi.sum(1), (brho * fp[ii]).sum(1), cheb_basis(rho, cu500.cheb_dom[2], m).sum(1)])
+    compare(g, closed, "dE/dc vs closed form", 1e-10)
+    h = steps(c0)
+    fd = np.zeros_like(c0)
+    for k in range(3):
+        for j in range(m):
+            cp, cm = c0.copy(), c0.copy()
+            cp[k, j] += h[k, j]
+            cm[k, j] -= h[k, j]
+            fd[k, j] = (oracle_energy(cp) - oracle_energy(cm)) / (2 * h[k, j])
+    compare(g, fd, "dE/dc vs oracle FD", 2e-6)  # the differences' rounding: ~1e-16 |E| / h
+
+
+def test_batch_energy_gradient_is_weighted_sum(gpu, cu500):
+    """comd_batch_cheb_energy_grad over three configurations: the weighted
+    sum of the single contexts' energy gradients, and each E_r theirs; the
+    mixed path (Chebyshev basis in FP32, sums in FP64) within 2e-5 of FP64
+    relative to each function's largest component."""
+    c0 = cu500.cheb[:3].copy()
+    specs = [(4, 21), (5, 22), (6, 23)]
+    w = np.array([1.0, -0.5, 2.0])
+    reps, singles = [], []
+    for n, seed in specs:
+        sim, xyz, bounds = config(cu500, n, seed)
+        reps.append((xyz, bounds))
+        singles.append(sim.cheb_energy_grad(c0))
+    b = CA.Batch([dict(pot=cu500, bounds=bd, defgrad=0.0, evaluation="cheby") for _, bd in reps], precision="fp64")
+    for r, (xyz, _) in enumerate(reps):
+        b.bin_atoms(r, xyz, None)
+    b.rebox()
+    e, g = b.cheb_energy_grad(c0, w)
+    expect = sum(wk * gs for wk, (_, gs) in zip(w, singles))
+    for r, (es, _) in enumerate(singles):
+        assert abs(e[r] - es) <= 1e-12 * abs(es)
+    compare(g, expect, "batch dE/dc vs weighted singles", 1e-12)
+    sim, _, _ = config(cu500, 6, 23, precision="mixed")
+    e32, g32 = sim.cheb_energy_grad(c0)
+    compare(g32, singles[2][1], "mixed dE/dc vs FP64", 2e-5)

@@ -176,13 +176,17 @@ def test_batch_autotune_and_mixed(gpu, cu500):
     b.rebox()
     b.force()
     f0 = [b.download(r)["f"].copy() for r in range(b.nrep)]
+    lists = b.list_info()["active"]  # replicas this small run on the link-cell kernels
     w, bpc, bal, ms = b.autotune()
-    print(f"[batch autotune] waves {w} blocks/CU {bpc} balanced {bal}: {ms:.3f} ms per force", flush=True)
+    print(f"[batch autotune] lists {lists}: waves {w} blocks/CU {bpc} balanced {bal}: {ms:.3f} ms per force",
+          flush=True)
+    assert b.list_info()["active"] == lists
     b.force()
     for r in range(b.nrep):
         f = b.download(r)["f"]
         m = np.abs(f0[r][:, :3]).max()
-        assert np.abs(f[:, :3] - f0[r][:, :3]).max() <= 1e-12 * m
+        # another list group shape changes the mixed passes' fixed-point frames
+        assert np.abs(f[:, :3] - f0[r][:, :3]).max() <= (1e-6 if lists else 1e-12) * m
     b64 = make_batch(specs, cu500, precision="fp64")
     b64.rebox()
     b64.force()
@@ -269,3 +273,36 @@ def test_batch_neighbour_lists_equal_single_runs(gpu, cu500):
             assert min(occ["blocks_per_cu"]) >= 2, occ
         for x in [b, c] + singles:
             x.close()
+
+
+def test_batch_autotune_tunes_list_passes(gpu, cu500):
+    """A batch whose replicas all take lists: comd_autotune times the list
+    passes' shapes (the per-replica search and the five fixed group
+    options, rows in LDS or not, one or two atoms per thread) and applies
+    the fastest; each replica's forces after it equal its single run's."""
+    lj = CA.copper_lj()
+    specs = [(cu500, 3.615, 7, 1), (cu500, 3.615, 8, 2), (lj, lj.lat, 7, 3), (cu500, 3.615, 9, 4)]
+    reps, atoms = [], []
+    for pot, lat, n, seed in specs:
+        xyz, p, bounds = seeded_atoms(n, lat, pot.mass, temp=300.0, disp=0.05, seed=seed)
+        reps.append(dict(pot=pot, bounds=bounds, defgrad=0.0))
+        atoms.append((xyz, p, bounds, pot))
+    b = CA.Batch(reps, precision="fp64")
+    b.tune(lj_physical=1)
+    for r, (xyz, p, _, _) in enumerate(atoms):
+        b.bin_atoms(r, xyz, p)
+    b.rebox()
+    assert b.list_info()["active"] == 1
+    res = b.autotune_lists()
+    print(f"[batch list autotune] {res} occupancy {b.list_occupancy()}", flush=True)
+    assert b.list_info()["active"] == 1 and b.list_info()["valid"] == 1 and res["ms"] > 0.0
+    for r, (xyz, p, bounds, pot) in enumerate(atoms):
+        s = CA.Simulation(pot, bounds, defgrad=0.0)
+        s.tune(lj_physical=1, skin=0.0)
+        s.bin_atoms(xyz, p)
+        s.rebox()
+        s.force()
+        g, o = compact(b.download(r)), compact(s.download())
+        assert np.array_equal(g["id"], o["id"])
+        assert force_rel_error(g["f"], o["f"]).max() <= 1e-12
+        s.close()

@@ -124,26 +124,27 @@ def test_upload_path_sets_up_lists(gpu, cu500):
     assert_same(c, b, "uploaded state")
 
 
-@pytest.mark.parametrize("groups,rows", [(0, 1), (1, 1), (2, 0), (3, 1), (4, 1)])
+@pytest.mark.parametrize("groups,rows,div", [(0, 1, 1), (1, 1, 2), (2, 0, 1), (3, 1, 2), (4, 1, 1), (0, 0, 2)])
 @pytest.mark.parametrize("n", [5, 7])
-def test_group_options_equal_cells(gpu, cu500, groups, rows, n):
+def test_group_options_equal_cells(gpu, cu500, groups, rows, div, n):
     """Every group size (4x4x3 .. 2x2x1 auxiliary cells), table rows in LDS
-    or not, and boxes of 3 auxiliary cells per dimension (n = 5: a group's
-    halo holds the same cell twice, at two images): the same pairs and
-    forces as the link-cell kernels."""
+    or not, one or two home atoms per thread, and boxes of 3 auxiliary cells
+    per dimension (n = 5: a group's halo holds the same cell twice, at two
+    images): the same pairs and forces as the link-cell kernels."""
     for prec in ("fp64", "mixed"):
         a, b = pair_of_sims(cu500, n, 3.615, 900.0, 0.1, 30 + n, "table", prec)
-        a.tune(list_groups=groups, list_rows_lds=rows)
+        a.tune(list_groups=groups, list_rows_lds=rows, list_threads_div=div)
         if prec == "mixed":
             b.set_precision("fp64")
         a.force()
         b.force()
         info = a.list_info()
         assert info["active"] == 1 and info["valid"] == 1, info
-        assert_same(a, b, f"groups {groups} rows {rows} n {n} {prec}", tol=1e-12 if prec == "fp64" else 1e-6)
+        what = f"groups {groups} rows {rows} div {div} n {n} {prec}"
+        assert_same(a, b, what, tol=1e-12 if prec == "fp64" else 1e-6)
         a.nsteps(20)
         b.nsteps(20)
-        assert_same(a, b, f"groups {groups} rows {rows} n {n} {prec}, 20 steps", tol=1e-10 if prec == "fp64" else 1e-5)
+        assert_same(a, b, what + ", 20 steps", tol=1e-10 if prec == "fp64" else 1e-5)
 
 
 def test_overflow_falls_back_to_cells(gpu, cu500):
@@ -189,3 +190,33 @@ def test_force_after_replayed_step_graph(gpu, cu500):
             s.force()
         assert a.list_info()["active"] == 1
         assert_same(a, b, f"after a {chunk}-step graph chunk", tol=1e-10)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_autotune_tunes_list_passes(gpu, cu500, precision):
+    """comd_autotune on a context with lists times the list passes' shapes:
+    the choice is applied (lists valid, occupancy reported for it), the
+    forces of the state stay those of the link-cell kernels, and steps
+    after it follow them."""
+    a, b = pair_of_sims(cu500, 12, 3.615, 300.0, 0.05, 91, "table", precision)
+    if precision == "mixed":
+        b.set_precision("fp64")
+    a.force()
+    b.force()
+    before = a.download()["f"].copy()
+    res = a.autotune_lists()
+    occ = a.list_occupancy()
+    print(f"[autotune lists {precision}] {res} occupancy {occ}", flush=True)
+    assert res["ms"] > 0.0 and res["groups"] in range(-1, 5) and res["threads_div"] in (1, 2)
+    info = a.list_info()
+    assert info["active"] == 1 and info["valid"] == 1
+    f = a.download()["f"]
+    m = np.abs(before[:, :3]).max()
+    assert np.abs(f[:, :3] - before[:, :3]).max() <= (1e-12 if precision == "fp64" else 1e-6) * m
+    w, bpc, bal, ms = a.autotune()  # with lists: the same tuner
+    assert ms > 0.0
+    tol = 1e-12 if precision == "fp64" else 1e-6
+    assert_same(a, b, f"autotuned {precision}", tol=tol)
+    a.nsteps(10)
+    b.nsteps(10)
+    assert_same(a, b, f"autotuned {precision}, 10 steps", tol=1e-10 if precision == "fp64" else 1e-5)
