@@ -545,7 +545,17 @@ static inline size_t nl_tab_bytes(const PotDev &P, int kind, int mixed)
 // Staged halo bytes per atom: group-frame fixed point x, y, z and float F'
 // (mixed); group-frame double x, y, z and, for the embedding-force pass, F'
 // (FP64).
-__host__ __device__ constexpr int nl_halo_bytes(int kind, int mixed) { return mixed ? 16 : (kind == CM_KIND_EAM3 ? 32 : 24); }
+__host__ __device__ constexpr int nl_halo_bytes(int kind, int mixed)
+{
+    return mixed ? (kind == CM_KIND_EAM3 ? 16 : 12) : (kind == CM_KIND_EAM3 ? 32 : 24);
+}
+// Mixed passes: 32-bit words per staged halo entry (x, y, z and, for the
+// embedding-force pass, F').  The density and LJ passes' 12-byte entries put
+// a wave's random neighbour reads on distinct banks more often than a
+// 16-byte stride (3h mod 64 against 4h mod 64): 4 % off the density pass,
+// though each read takes two LDS instructions.  The embedding-force pass
+// keeps one 16-byte read (a 20-byte stride measured 2 % slower there).
+template <int KIND> __host__ __device__ constexpr int nl_halo_words() { return KIND == CM_KIND_EAM3 ? 4 : 3; }
 // Bytes of the staged halo entries' slot map (one byte per entry).
 __host__ __device__ constexpr int nl_slotmap_bytes(int hcap) { return (hcap + 15) / 16 * 16; }
 
@@ -562,8 +572,14 @@ __device__ __forceinline__ void nl_atom(const DevState &d, const PotDev &P, cons
     const int self = l.selfh[t];
     v4u qi = {0, 0, 0, 0};
     double xi = 0.0, yi = 0.0, zi = 0.0, fia = 0.0;
+    constexpr int HW = nl_halo_words<KIND>();
+    const CM_LDS unsigned *hu = (const CM_LDS unsigned *)hq;
     if (MIXED) {
-        qi = hq[self];
+        if constexpr (HW == 4) {
+            qi = hq[self];
+        } else {
+            qi = v4u{hu[HW * self], hu[HW * self + 1], hu[HW * self + 2], 0u};
+        }
         fia = (double)__uint_as_float(qi.w);
     } else {
         xi = hx[self];
@@ -598,15 +614,15 @@ __device__ __forceinline__ void nl_atom(const DevState &d, const PotDev &P, cons
             if (TAIL && k0 + u >= cnt) hj[u] = 0;
         }
         if constexpr (MIXED) {
-            // x, y, z only (12-byte reads) unless F' is needed
+            // x, y, z only unless F' is needed
             v4u qj[8];
 #pragma unroll
             for (int u = 0; u < 8; u++) {
-                if constexpr (KIND == CM_KIND_EAM3) {
+                if constexpr (HW == 4) {
                     qj[u] = hq[hj[u]];
                 } else {
-                    const v3u q3 = *(const CM_LDS v3u *)&hq[hj[u]];
-                    qj[u] = v4u{q3.x, q3.y, q3.z, 0u};
+                    const CM_LDS unsigned *e = hu + HW * hj[u];
+                    qj[u] = v4u{e[0], e[1], e[2], 0u};
                 }
             }
             float ef = 0.0f, rf = 0.0f;  // the chunk's FP32 value sums (density pass)
@@ -787,7 +803,14 @@ void k_nl_pair(DevState d, PotDev P, CmList l, int tab_bytes, int gbase)
                     q.y = (unsigned)(int)rint(y * R.qscale);
                     q.z = (unsigned)(int)rint(z * R.qscale);
                     q.w = __float_as_uint(fj);
-                    hq[h] = q;
+                    if constexpr (nl_halo_words<KIND>() == 4) {
+                        hq[h] = q;
+                    } else {
+                        CM_LDS unsigned *e = (CM_LDS unsigned *)hq + nl_halo_words<KIND>() * h;
+                        e[0] = q.x;
+                        e[1] = q.y;
+                        e[2] = q.z;
+                    }
                 } else {
                     hx[h] = x;
                     hy[h] = y;
