@@ -33,10 +33,9 @@
 // gradient is deterministic.  MIXED evaluates the Chebyshev basis and rho'
 // in single precision and accumulates in double.
 #include "comd_internal.h"
+#include "adj_basis.h"
 
-#define AMAX 32    // coefficients per series supported by the adjoint
 #define AHMAX 320  // halo atoms staged per batch
-#define ANMOM 64   // moments written per wave (two sets of AMAX)
 
 struct AdjDev {
     const double *gx, *gy, *gz;  // per slot: dL/df
@@ -47,57 +46,6 @@ struct AdjDev {
     double *part;                // per block: ANMOM partial moments
 };
 
-
-// T_0..T_{m-1} at y, each times wa added to a[] (and times wb to b[]).
-template <int MIXED, int TWO>
-__device__ __forceinline__ void add_basis(double y, int m, double wa, double wb, double *a, double *b)
-{
-    if (MIXED) {
-        const float yf = (float)y, y2 = 2.0f * yf;
-        float tm = 1.0f, t = yf;
-        a[0] += wa;
-        if (TWO) b[0] += wb;
-#pragma unroll
-        for (int k = 1; k < AMAX; k++) {
-            if (k < m) {
-                a[k] += wa * (double)t;
-                if (TWO) b[k] += wb * (double)t;
-            }
-            const float tn = y2 * t - tm;
-            tm = t;
-            t = tn;
-        }
-    } else {
-        const double y2 = 2.0 * y;
-        double tm = 1.0, t = y;
-        a[0] += wa;
-        if (TWO) b[0] += wb;
-#pragma unroll
-        for (int k = 1; k < AMAX; k++) {
-            if (k < m) {
-                a[k] += wa * t;
-                if (TWO) b[k] += wb * t;
-            }
-            const double tn = y2 * t - tm;
-            tm = t;
-            t = tn;
-        }
-    }
-}
-
-__device__ __forceinline__ double cheb_arg(double x, double a, double b)
-{
-    if (x < a) x = a;  // lower clamp only (src-flat/cheby.c:290-293)
-    return (2.0 * x - a - b) / (b - a);
-}
-
-// Sum of v over the wave's 64 lanes (result in every lane).
-__device__ __forceinline__ double wave_sum(double v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
 
 template <int TWO>
 __device__ __forceinline__ void write_moments(const double *a, const double *b, double *part)
@@ -436,6 +384,11 @@ void cm_launch_adj_resid(const DevState &d, const CmAdjoint &J, const double *ft
 {
     hipLaunchKernelGGL(k_adj_resid, dim3(J.nblocks), dim3(64), 0, st, d, ft, J.rw, J.gx, J.gy, J.gz, J.part);
     hipLaunchKernelGGL(k_adj_final, dim3(1), dim3(1024), 0, st, J.part, J.nblocks, J.out + (size_t)4 * ANMOM);
+}
+
+void cm_launch_adj_final(const CmAdjoint &J, int nblocks, int stage, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_adj_final, dim3(1), dim3(1024), 0, st, J.part, nblocks, J.out + (size_t)stage * ANMOM);
 }
 
 void cm_launch_adj_stage(const DevState &d, const PotDev &P, const CmAdjoint &J, int stage, int mixed, hipStream_t st)

@@ -122,6 +122,9 @@ struct comd_ctx {
     double *adj_g[3], *adj_lam, *adj_mu, *adj_ft, *adj_f2, *adj_part, *adj_out;
     double *adj_rep;      // per replica: dL/dE, then the residual weight 1 / (3 N)
     int adj_nblocks;
+    double *adj_gb, *adj_part_nl;  // list passes: dL/df in build order, partial moments
+    long adj_gb_cap, adj_part_nl_cap;
+    int adj_lists;        // COMD_TUNE_ADJOINT_LISTS: 0 link cells, 1 lists with a group per CU, 2 lists
     int adj_have_target;  // adj_ft holds a target (comd_eam_cheb_grad with f_target NULL reuses it)
     std::vector<double> cheb_host;  // current value series phi, rho, F (3 x cn) of replica 0
     // force-kernel launch shape (comd_tune / comd_autotune)
@@ -492,7 +495,8 @@ static void free_all(comd_ctx *c)
                     c->perm, c->flags, c->mh, c->plist, c->pcount, c->pvalid,
                     c->partials, c->red_out, c->d_grids, c->d_cell_grid, c->d_pots,
                     c->adj_g[0], c->adj_g[1], c->adj_g[2], c->adj_lam, c->adj_mu,
-                    c->adj_ft, c->adj_f2, c->adj_part, c->adj_out, c->adj_rep, c->obs_part, c->obs_out, c->mon};
+                    c->adj_ft, c->adj_f2, c->adj_part, c->adj_out, c->adj_rep, c->obs_part, c->obs_out, c->mon,
+                    c->adj_gb, c->adj_part_nl};
     for (void *p : ptrs)
         if (p) hipFree(p);
     for (Replica &r : c->reps)
@@ -885,6 +889,7 @@ static int ctx_create(int nrep, const comd_params *prms, comd_ctx **out)
     c->nl_group_opt = -1;
     c->nl_rows_lds = -1;
     c->nl_threads_div = 1;
+    c->adj_lists = 1;
     if (hipStreamSynchronize(c->stream) != hipSuccess) {
         free_all(c);
         delete c;
@@ -1640,6 +1645,10 @@ int comd_tune(comd_ctx *c, int key, int value)
         c->nl_threads_div = value;
         if (c->have_atoms) return nl_setup(c);
         break;
+    case COMD_TUNE_ADJOINT_LISTS:
+        if (value < 0 || value > 2) return set_err(c, COMD_ERR_ARG, "adjoint lists option must be 0, 1 or 2");
+        c->adj_lists = value;
+        break;
     case COMD_TUNE_SKIN_MILLI:
         if (value < 0 || value > 3000) return set_err(c, COMD_ERR_ARG, "skin must be 0..3000 (1e-3 A)");
         c->nl_skin_milli = value;
@@ -2333,7 +2342,59 @@ static int cheb_grad(comd_ctx *c, const double *coef, int n, const double *f_tar
     } else {
         cm_launch_adj_resid(d, J, c->adj_ft, c->stream);
     }
-    for (int stage = 0; stage < 4; stage++) cm_launch_adj_stage(d, P, J, stage, mixed, c->stream);
+    // the pair passes over the neighbour lists when the forward pass used
+    // them (valid after it, every group an EAM one, the staged halo in LDS)
+    // and, by default, there is a group per CU at least (a block per group:
+    // fewer groups leave CUs idle, where the link-cell passes, a wave per
+    // cell, measured faster); else over the link cells
+    bool nl = false;
+    if (c->adj_lists && c->nl_active && c->nl_lj_groups == 0 && (c->adj_lists == 2 || c->nl.pass_blocks >= c->ncu)) {
+        unsigned f = 0;
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipMemcpy(&f, c->nl.flags, 4, hipMemcpyDeviceToHost));
+        nl = !(f & CM_NL_INVALID) && cm_adj_nl_lds(c->nl) + (12 << 10) <= (size_t)c->lds_block;
+    }
+    if (nl) {
+        const CmList &l = c->nl;
+        const long ng = l.pass_blocks;
+        if ((long)l.n * 3 > c->adj_gb_cap) {
+            if (c->adj_gb) hipFree(c->adj_gb);
+            c->adj_gb = nullptr;
+            c->adj_gb_cap = 0;
+            if (hipMalloc((void **)&c->adj_gb, (size_t)l.n * 3 * 8) != hipSuccess)
+                return set_err(c, COMD_ERR_NOMEM, "adjoint list buffers");
+            c->adj_gb_cap = (long)l.n * 3;
+        }
+        if (2 * ng > c->adj_part_nl_cap) {
+            if (c->adj_part_nl) hipFree(c->adj_part_nl);
+            c->adj_part_nl = nullptr;
+            c->adj_part_nl_cap = 0;
+            if (hipMalloc((void **)&c->adj_part_nl, (size_t)2 * ng * NM * 8) != hipSuccess)
+                return set_err(c, COMD_ERR_NOMEM, "adjoint list buffers");
+            c->adj_part_nl_cap = 2 * ng;
+        }
+        CmAdjNl A;
+        A.gb = c->adj_gb;
+        A.gx = J.gx;
+        A.gy = J.gy;
+        A.gz = J.gz;
+        A.mu = J.mu;
+        A.lam = J.lam;
+        A.dlde = J.dlde;
+        A.part = c->adj_part_nl;
+        CmAdjoint Jn = J;
+        Jn.part = c->adj_part_nl;
+        cm_launch_adj_nl_gather(l, J.gx, J.gy, J.gz, c->adj_gb, c->stream);
+        cm_launch_adj_nl(d, P, l, A, 1, mixed, c->stream);  // loss and energy moments: 2 ng blocks
+        cm_launch_adj_final(Jn, (int)(2 * ng), 0, c->stream);
+        cm_launch_adj_nl(d, P, l, A, 2, mixed, c->stream);
+        cm_launch_adj_final(Jn, (int)ng, 1, c->stream);
+        cm_launch_adj_stage(d, P, J, 2, mixed, c->stream);  // the atom pass, per slot
+        cm_launch_adj_nl(d, P, l, A, 3, mixed, c->stream);
+        cm_launch_adj_final(Jn, (int)ng, 3, c->stream);
+    } else {
+        for (int stage = 0; stage < 4; stage++) cm_launch_adj_stage(d, P, J, stage, mixed, c->stream);
+    }
     HIPCHK(c, hipGetLastError());
     std::vector<double> out((size_t)5 * NM);
     HIPCHK(c, hipMemcpyAsync(out.data(), c->adj_out, out.size() * 8, hipMemcpyDeviceToHost, c->stream));

@@ -147,6 +147,7 @@ int cm_adjoint_moments();
 void cm_launch_adj_resid(const DevState &d, const CmAdjoint &J, const double *ft, hipStream_t st);
 void cm_launch_adj_stage(const DevState &d, const PotDev &P, const CmAdjoint &J, int stage, int mixed,
                          hipStream_t st);
+void cm_launch_adj_final(const CmAdjoint &J, int nblocks, int stage, hipStream_t st);
 
 // Observables (observe.hip): per-block partial sums of CM_OBS_N doubles
 // over one replica's cells (at most CM_OBS_MAXBLOCKS blocks, in part), then
@@ -245,3 +246,21 @@ void cm_nl_force(const DevState &d, const PotDev &P, const CmList &l, int kind, 
 // Resident blocks per CU of a list pass (-1 if the runtime cannot tell) and
 // its dynamic LDS bytes.
 int cm_nl_occupancy(const PotDev &P, const CmList &l, int kind, int mixed, int *lds_bytes);
+
+// Adjoint pair passes over the lists (nlist.hip): pass 1 (phi moments and
+// the energy moments), 2 (rho' moments and lambda), 3 (rho moments), one
+// block per group writing ANMOM partial moments to part[block]; gb holds
+// dL/df in build order (cm_launch_adj_nl_gather fills it).
+struct CmAdjNl {
+    const double *gb;            // build order: dL/df, 3 per atom
+    const double *gx, *gy, *gz;  // per slot: dL/df
+    const double *mu;            // per slot: mu (pass 3)
+    double *lam;                 // per slot: lambda (pass 2)
+    const double *dlde;          // per replica: dL/dE
+    double *part;                // per block: partial moments
+};
+size_t cm_adj_nl_lds(const CmList &l);
+void cm_launch_adj_nl_gather(const CmList &l, const double *gx, const double *gy, const double *gz, double *gb,
+                             hipStream_t st);
+void cm_launch_adj_nl(const DevState &d, const PotDev &P, const CmList &l, const CmAdjNl &A, int pass, int mixed,
+                      hipStream_t st);

@@ -29,6 +29,7 @@
 // least three auxiliary cells per dimension; the host falls back to the
 // link-cell kernels otherwise.
 #include "comd_internal.h"
+#include "adj_basis.h"
 #include "pair_terms.h"
 
 #include <type_traits>
@@ -828,6 +829,120 @@ void k_nl_pair(DevState d, PotDev P, CmList l, int tab_bytes, int gbase)
 }
 
 // ---------------------------------------------------------------------------
+// Adjoint pair passes over the lists (the sums are stated in
+// csrc/adjoint.hip, whose link-cell kernels k_adj_pair compute the same
+// moments): a block per group with the group's halo positions staged in LDS
+// (FP64, group frame), each thread a home atom that evaluates its own
+// accepted list entries -- about four in five entries are inside the
+// cutoff, so the basis sums run on nearly full waves without the link-cell
+// kernels' pair queue.  dL/df and F' of the neighbours come from
+// build-order copies (gb, l.fq) through the L2; each thread keeps its
+// moments in registers and the block sums them over its waves in a fixed
+// order (block_moments), so the gradient stays deterministic.  PASS 4 is
+// the second set of pass 1 (the energy moments 1/2 dL/dE sum B_k(r_ij)), in
+// a launch of its own: both sets in registers at once would not fit.
+
+// dL/df in build order (gb[3t + k]) from the slot arrays.
+__global__ __launch_bounds__(256) void k_adj_nl_gather(CmList l, const double *gx, const double *gy,
+                                                       const double *gz, double *gb)
+{
+    const int n = l.gstart[l.ngroups];
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        const int s = l.slot[l.order[t]];
+        gb[3L * t + 0] = gx[s];
+        gb[3L * t + 1] = gy[s];
+        gb[3L * t + 2] = gz[s];
+    }
+}
+
+template <int PASS, int MIXED>
+__global__ __launch_bounds__(768) void k_adj_nl(DevState d, PotDev P, CmList l, CmAdjNl A, int pbase)
+{
+    extern __shared__ __align__(16) char nls[];
+    __shared__ GroupLds S;
+    __shared__ double red[12][ANMOM];
+    double *hx = (double *)nls, *hy = hx + l.hcap, *hz = hy + l.hcap;
+    CM_LDS uint8_t *slot_of = (CM_LDS uint8_t *)(nls + (size_t)l.hcap * 24);
+    const int m = P.cn;
+    const int dom = PASS == 1 || PASS == 4 ? 0 : 1;  // phi's series domain, else rho's
+    const double ca = P.ca[dom], cb = P.cb[dom];
+    const double rc2 = P.rc2;
+    const int m8 = P.cstride;
+    const double *dser = P.cheb + 4 * m8;  // rho' series
+    const float *dser32 = P.cheb32 + 4 * m8;
+    double m0[AMAX];
+#pragma unroll
+    for (int k = 0; k < AMAX; k++) m0[k] = 0.0;
+
+    // the group of this block, XCD-contiguous as in the force passes
+    const int nb = gridDim.x, qb = nb / 8, rem = nb % 8;
+    const int xcd = blockIdx.x % 8, idx = blockIdx.x / 8;
+    const int g = nb < 8 ? (int)blockIdx.x : xcd * qb + min(xcd, rem) + idx;
+    const NlRep R = nl_rep(l, l.group_rep[g]);
+    const GroupGeo G = group_geo(R, g - R.group_base);
+    group_setup(l, R, G, g, S, slot_of);
+    const double ox = (G.x0 + 0.5 * G.lx) * R.ab0, oy = (G.y0 + 0.5 * G.ly) * R.ab1,
+                 oz = (G.z0 + 0.5 * G.lz) * R.ab2;
+    const int H = min(S.off[S.ns], l.hcap);
+    for (int h = threadIdx.x; h < H; h += blockDim.x) {
+        const int sl = slot_of[h];
+        const CmPos p = load_pos(l.pos, S.start[sl] + (h - S.off[sl]));
+        frame_pos(R, G, S.uxyz[sl], p, ox, oy, oz, hx[h], hy[h], hz[h]);
+    }
+    __syncthreads();
+    const double de = PASS == 4 ? A.dlde[l.group_rep[g]] : 0.0;  // weight of the energy moments
+    for (int t = S.t0 + threadIdx.x; t < S.t1; t += blockDim.x) {
+        const int s = l.slot[l.order[t]];
+        const int self = l.selfh[t];
+        const double xi = hx[self], yi = hy[self], zi = hz[self];
+        double gix = 0.0, giy = 0.0, giz = 0.0, wi = 0.0;
+        if (PASS != 3) {
+            gix = A.gx[s];
+            giy = A.gy[s];
+            giz = A.gz[s];
+        }
+        if (PASS == 2) wi = l.fq[t];
+        if (PASS == 3) wi = A.mu[s];
+        if (PASS == 4) wi = 0.5 * de;
+        double lam = 0.0;
+        const int cnt = l.ncnt[t];
+        const CM_GLOBAL v4u *row = (const CM_GLOBAL v4u *)l.nbl + t;
+        for (int k0 = 0; k0 < cnt; k0 += 8) {
+            const v4u e = row[(size_t)(k0 >> 3) * l.n];
+            const unsigned ew[4] = {e.x, e.y, e.z, e.w};
+            for (int u = 0; u < 8 && k0 + u < cnt; u++) {
+                const int h = (int)((ew[u >> 1] >> (16 * (u & 1))) & 0xffffu);
+                const double dx = xi - hx[h], dy = yi - hy[h], dz = zi - hz[h];
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                // pair predicates of src-flat/eam.c:519 (phi, rho) and 633 (rho')
+                const bool in = PASS == 2 ? (r2 < rc2) : (r2 <= rc2);
+                if (!in || !(r2 > 0.0)) continue;
+                const double r = sqrt(r2);
+                const double y = cheb_arg(r, ca, cb);
+                if (PASS >= 3) {
+                    add_basis<MIXED, 0>(y, m, wi, 0.0, m0, m0);
+                    continue;
+                }
+                const int sl = slot_of[h];
+                const int q = S.start[sl] + (h - S.off[sl]);
+                const double w = ((gix - A.gb[3L * q + 0]) * dx + (giy - A.gb[3L * q + 1]) * dy +
+                                  (giz - A.gb[3L * q + 2]) * dz) / r;
+                if (PASS == 1) {
+                    add_basis<MIXED, 0>(y, m, 0.5 * w, 0.0, m0, m0);
+                } else {
+                    add_basis<MIXED, 0>(y, m, 0.5 * (wi + l.fq[q]) * w, 0.0, m0, m0);
+                    const double drho = MIXED ? (double)cheb32(dser32, m8, (float)ca, (float)cb, (float)r)
+                                              : cheb64(dser, m8, ca, cb, r);
+                    lam += drho * w;
+                }
+            }
+        }
+        if (PASS == 2) A.lam[s] = lam;
+    }
+    block_moments<PASS == 4 ? 1 : 0>(m0, red, A.part + (size_t)(pbase + blockIdx.x) * ANMOM);
+}
+
+// ---------------------------------------------------------------------------
 // host launchers
 
 static unsigned grid_for(long n, int block)
@@ -941,4 +1056,37 @@ void cm_nl_force(const DevState &d, const PotDev &P, const CmList &l, int kind, 
         if (kind == CM_KIND_EAM1) nl_launch2<CM_KIND_EAM1, COMD_EVAL_TABLE>(d, P, l, mixed, st, g0, ng);
         else nl_launch2<CM_KIND_EAM3, COMD_EVAL_TABLE>(d, P, l, mixed, st, g0, ng);
     }
+}
+
+size_t cm_adj_nl_lds(const CmList &l) { return (size_t)l.hcap * 24 + nl_slotmap_bytes(l.hcap); }
+
+template <int MIXED>
+static void adj_nl_launch(const DevState &d, const PotDev &P, const CmList &l, const CmAdjNl &A, int pass,
+                          hipStream_t st)
+{
+    const dim3 grid(l.pass_blocks), block(l.pass_threads);
+    const size_t lds = cm_adj_nl_lds(l);
+    const int nb = l.pass_blocks;
+    switch (pass) {
+    case 1:  // the loss moments, then the energy moments into blocks nb .. 2 nb - 1
+        hipLaunchKernelGGL((k_adj_nl<1, MIXED>), grid, block, lds, st, d, P, l, A, 0);
+        hipLaunchKernelGGL((k_adj_nl<4, MIXED>), grid, block, lds, st, d, P, l, A, nb);
+        break;
+    case 2: hipLaunchKernelGGL((k_adj_nl<2, MIXED>), grid, block, lds, st, d, P, l, A, 0); break;
+    case 3: hipLaunchKernelGGL((k_adj_nl<3, MIXED>), grid, block, lds, st, d, P, l, A, 0); break;
+    default: break;
+    }
+}
+
+void cm_launch_adj_nl_gather(const CmList &l, const double *gx, const double *gy, const double *gz, double *gb,
+                             hipStream_t st)
+{
+    hipLaunchKernelGGL(k_adj_nl_gather, dim3(grid_for(l.n, 256)), dim3(256), 0, st, l, gx, gy, gz, gb);
+}
+
+void cm_launch_adj_nl(const DevState &d, const PotDev &P, const CmList &l, const CmAdjNl &A, int pass, int mixed,
+                      hipStream_t st)
+{
+    if (mixed) adj_nl_launch<1>(d, P, l, A, pass, st);
+    else adj_nl_launch<0>(d, P, l, A, pass, st);
 }

@@ -184,6 +184,9 @@ enum comd_tune_key {
                                       LDS when they fit beside the halo */
     COMD_TUNE_LIST_THREADS_DIV = 11, /* 1 (default) or 2..4: home atoms per thread of the list passes
                                       (threads per block = a group's atoms / this, in waves) */
+    COMD_TUNE_ADJOINT_LISTS = 12,  /* the adjoint's pair passes over the neighbour lists when the
+                                      context uses them: 1 (default) if it has a list group per CU at
+                                      least, 2 always, 0 never (link cells) */
 };
 int comd_tune(comd_ctx *ctx, int key, int value);
 

@@ -209,6 +209,14 @@ def test_batch_gradient_equals_sum_of_single_contexts(gpu, cu500):
     loss_c, grad_c = b.cheb_grad(c0, None, ets, w_e)
     assert loss_c == loss_b
     np.testing.assert_array_equal(grad_c, grad_b)
+    # the pair passes over the batch's lists (groups of every replica in one
+    # launch) give the same
+    assert b.list_info()["active"] == 1
+    b.tune(adjoint_lists=2)
+    loss_l, grad_l = b.cheb_grad(c0, None, ets, w_e)
+    print(f"[batch adjoint over lists] loss {loss_l:.12e}", flush=True)
+    assert abs(loss_l - loss_b) <= 1e-12 * abs(loss_b)
+    compare(grad_l, grad_b, "batch adjoint, lists vs link cells", 1e-12)
     b.close()
 
 
@@ -307,3 +315,37 @@ def test_batch_energy_gradient_is_weighted_sum(gpu, cu500):
     sim, _, _ = config(cu500, 6, 23, precision="mixed")
     e32, g32 = sim.cheb_energy_grad(c0)
     compare(g32, singles[2][1], "mixed dE/dc vs FP64", 2e-5)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_adjoint_over_lists_equals_link_cells(gpu, cu500, precision):
+    """The adjoint's pair passes over the neighbour lists (COMD_TUNE_ADJOINT_LISTS
+    2; by default they run where the context has a list group per CU) and
+    over the link cells (0) give the same loss and gradient, and the same
+    energy gradient: FP64 to 1e-12 of each function's largest component,
+    mixed (FP32 basis values of separations rounded differently) to 1e-6;
+    both deterministic."""
+    import time
+    c0 = cu500.cheb[:3].copy()
+    sim, xyz, bounds = config(cu500, 10, 17, precision)
+    sim.force()
+    assert sim.list_info()["active"] == 1
+    ft, et = target_of(sim, perturbed(c0, 3))
+    out = {}
+    for lists in (2, 0):
+        sim.tune(adjoint_lists=lists)
+        sim.cheb_grad(c0, ft, et, 1e-4)  # warm-up
+        t0 = time.perf_counter()
+        l, g = sim.cheb_grad(c0, None, et, 1e-4)
+        dt = time.perf_counter() - t0
+        l2, g2 = sim.cheb_grad(c0, None, et, 1e-4)
+        assert l2 == l and np.array_equal(g2, g)
+        e, ge = sim.cheb_energy_grad(c0)
+        out[lists] = (l, g, e, ge)
+        print(f"[adjoint {precision} lists={lists}] {1e3 * dt:.2f} ms per gradient, loss {l:.12e}", flush=True)
+    tol = 1e-12 if precision == "fp64" else 1e-6
+    (l1, g1, e1, ge1), (l0, g0, e0, ge0) = out[2], out[0]
+    assert abs(l1 - l0) <= tol * abs(l0)
+    assert abs(e1 - e0) <= 1e-12 * abs(e0)
+    compare(g1, g0, f"{precision} loss gradient, lists vs link cells", tol)
+    compare(ge1, ge0, f"{precision} energy gradient, lists vs link cells", tol)
