@@ -64,6 +64,8 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--cpu-n", type=int, default=40, help="FCC cells per edge of the CPU reference's sample")
     ap.add_argument("--no-extra", action="store_true", help="skip the other configurations' timings")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default list-pass shapes (for profiles whose kernels should not vary)")
     ap.add_argument("--workload", choices=("eam", "batch", "adjoint"), default="eam",
                     help="eam: EAM Cu 2.05M atoms, FP64 and mixed (the headline line); batch: 64 LJ/EAM replicas "
                          "in one context; adjoint: Chebyshev-coefficient gradients of 16 Cu 20^3 configurations")
@@ -77,14 +79,14 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def run_precision(sim, precision, steps, warmup, n_atoms, timed_steps):
+def run_precision(sim, precision, steps, warmup, n_atoms, timed_steps, autotune=True):
     """steps timed as the library runs them (the step replayed as a HIP
     graph), the energy drift over those steps, then timed_steps more with
     HIP events around every kernel class on the simulation's stream (the
     events break the graph into separate launches, so these steps are not
     the throughput figure)."""
     sim.set_precision(precision)
-    tuned = sim.autotune_lists() if sim.list_info()["active"] else None  # setup, not timed
+    tuned = sim.autotune_lists() if autotune and sim.list_info()["active"] else None  # setup, not timed
     log(f"{precision}: list passes autotuned: {tuned}")
     sim.steps(warmup)
     sim.sync()
@@ -298,7 +300,8 @@ def main():
         if prec != "fp64":
             sim.upload(s0["natoms"], s0["id"], s0["itype"], s0["r"], s0["p"])
             sim.rebox()
-        res[prec] = run_precision(sim, prec, args.steps, args.warmup, n_atoms, min(args.steps, 200))
+        res[prec] = run_precision(sim, prec, args.steps, args.warmup, n_atoms, min(args.steps, 200),
+                                  autotune=not args.no_autotune)
         log(f"{prec}: {res[prec]['atom_steps_per_s'] / 1e9:.3f} G atom-steps/s, "
             f"{res[prec]['ms_per_step']:.3f} ms/step, kernel ms {['%.2f' % v for v in res[prec]['kernel_ms']]}")
         if prec == "fp64":
@@ -453,8 +456,10 @@ def run_batch(args, steps):
     log(f"batch: binned and first force in {t_setup:.2f} s")
     sizes = np.array([a[0].shape[0] for a in atoms])
     total = int(sizes.sum())
-    tuned = b.autotune_lists()  # the list passes' group shape, rows in LDS, atoms per thread
-    tuned["occupancy"] = b.list_occupancy()
+    tuned = None
+    if not getattr(args, "no_autotune", False):
+        tuned = b.autotune_lists()  # the list passes' group shape, rows in LDS, atoms per thread
+        tuned["occupancy"] = b.list_occupancy()
     log(f"batch: autotune {tuned}")
     b.steps(args.warmup)
     b.sync()
