@@ -24,10 +24,14 @@ def both(pot, opot, xyz, p, bounds):
     return g[m], o[m]
 
 
-@pytest.mark.parametrize("case", ["perfect", "thermal", "vacancy", "lj", "cu20"])
+@pytest.mark.parametrize("case", ["perfect", "thermal", "vacancy", "lj", "cu20", "sparse"])
 def test_centrosymmetry_equals_oracle(gpu, cu500, ocu500, case):
     print(f"[case] centrosymmetry {case}", flush=True)
-    if case == "lj":
+    if case == "sparse":  # fewer than 12 neighbours in reach: 1000 for every atom
+        pot, opot = cu500, ocu500
+        xyz = np.array([[1.0, 1.0, 1.0], [3.5, 1.0, 1.0], [1.0, 3.5, 1.0], [9.0, 9.0, 9.0], [15.0, 4.0, 12.0]])
+        p, bounds = np.zeros_like(xyz), (20.0, 20.0, 20.0)
+    elif case == "lj":
         pot = CA.copper_lj()
         opot = O.OraclePot.lj(pot.sigma, pot.epsilon, pot.cutoff, pot.mass)
         xyz, p, bounds = seeded_atoms(6, pot.lat, pot.mass, temp=600.0, disp=0.1, seed=5)
@@ -43,6 +47,8 @@ def test_centrosymmetry_equals_oracle(gpu, cu500, ocu500, case):
     assert np.array_equal(g, o)
     if case == "perfect":
         assert np.abs(g).max() < 1e-10
+    if case == "sparse":
+        assert np.array_equal(g, np.full(5, 1000.0))
 
 
 def test_batch_centrosymmetry_equals_single(gpu, cu500):

@@ -679,6 +679,15 @@ def test_centrosymmetry_perfect_lattice_and_vacancy():
     np.testing.assert_allclose(c, ref, rtol=1e-5, atol=2e-6)
 
 
+def test_centrosymmetry_fewer_than_12_neighbours():
+    """Atoms with fewer than 12 others in the 27 cells around them get 1000
+    (src-viz/AtomVisualize.c:508): five atoms in a 20 A box."""
+    pot = O.OraclePot.eam(POTS, "cu500")
+    xyz = np.array([[1.0, 1.0, 1.0], [3.5, 1.0, 1.0], [1.0, 3.5, 1.0], [9.0, 9.0, 9.0], [15.0, 4.0, 12.0]])
+    s = O.OracleSim.from_atoms(pot, (20.0, 20.0, 20.0), xyz)
+    assert np.array_equal(_centro_by_id(s), np.full(5, 1000.0))
+
+
 def test_centrosymmetry_matches_brute_force_thermal():
     pot = O.OraclePot.eam(POTS, "cu500")
     n, a = 5, 3.615
