@@ -472,7 +472,7 @@ int comd_grid_for(comd_params *p, const double bounds[3], double defgrad, double
 
 static void free_nl(comd_ctx *c)
 {
-    void *p[] = {c->nl.pos,      c->nl.selfh,           c->nl.slot,    c->nl.order,    c->nl.fq,
+    void *p[] = {c->nl.pos,      c->nl.selfh,           c->nl.slot,    c->nl.order,    c->nl.fq,    c->nl.f1,
                  c->nl.nbl,      c->nl.ncnt,     c->nl.aux_cnt,   c->nl.aux_n,   c->nl.aux_start,
                  c->nl.aux_mem,  c->nl.gstart,   c->nl.nonempty,  c->nl.count,   c->nl.flags,
                  (void *)c->nl.cell_order, (void *)c->nl.group_q0, (void *)c->nl.group_rep,
@@ -1074,6 +1074,7 @@ static int nl_setup(comd_ctx *c)
         alloc((void **)&c->nl.slot, n * 4);
         alloc((void **)&c->nl.order, n * 4);
         alloc((void **)&c->nl.fq, n * 8);
+        alloc((void **)&c->nl.f1, n * 24);
         alloc((void **)&c->nl.nbl, n * (size_t)kmax * 2);
         alloc((void **)&c->nl.ncnt, n * 4);
         alloc((void **)&c->nl.aux_cnt, (size_t)naux * 4);

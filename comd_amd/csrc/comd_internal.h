@@ -210,6 +210,8 @@ struct CmList {
     const int *group_q0;     // [ngroups + 1]: a group's first position in cell_order
     CmPos *pos;              // per build position: absolute position (kept by the drift)
     double *fq;              // per position in build order: F' of the last density pass
+    double *f1;              // per position in build order: the density pass's pair forces (x, y, z),
+                             // completed by the embedding-force pass into the slots
     uint16_t *selfh;         // per t: the atom's own index in its group's staged halo
     int *slot;               // per id: current slot, -1 when absent
     int *order;              // atoms (ids) in build order

@@ -695,18 +695,20 @@ __device__ __forceinline__ void nl_atom(const DevState &d, const PotDev &P, cons
         } else {
             interp64(P.tab[2], P.x0[2], P.xn[2], P.invdx[2], P.n[2], A.rho, fv, fp);
         }
-        d.fx[s] = A.x;
-        d.fy[s] = A.y;
-        d.fz[s] = A.z;
+        // the pair forces stay in build order (coalesced) for the
+        // embedding-force pass, which writes the sum to the slots once
+        l.f1[3L * t + 0] = A.x;
+        l.f1[3L * t + 1] = A.y;
+        l.f1[3L * t + 2] = A.z;
         d.fe[s] = A.e + fv;
         d.rho[s] = A.rho;
         d.fi[s] = fp;
         d.nnb[s] = npair;
         l.fq[t] = fp;  // staged with the position by the embedding-force pass (coalesced: t is the thread's)
     } else if (KIND == CM_KIND_EAM3) {
-        d.fx[s] += A.x;
-        d.fy[s] += A.y;
-        d.fz[s] += A.z;
+        d.fx[s] = l.f1[3L * t + 0] + A.x;
+        d.fy[s] = l.f1[3L * t + 1] + A.y;
+        d.fz[s] = l.f1[3L * t + 2] + A.z;
         d.vir[s] = A.vir;
     } else {
         const double px = d.px[s];
