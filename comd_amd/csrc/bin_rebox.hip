@@ -41,12 +41,23 @@ __device__ __forceinline__ void move_atom(const DevState &d, const Grid &g, int 
                                           double dtk, double dtd, int kick, int drift, int rebin)
 {
     const double rmass = g.rmass;
+    // every load first (the arrays may alias as far as the compiler can
+    // tell, so a load after a store would wait for it)
     double x = d.rx[s], y = d.ry[s], z = d.rz[s];
     double px = d.px[s], py = d.py[s], pz = d.pz[s];
+    double fx = 0.0, fy = 0.0, fz = 0.0;
     if (kick) {
-        px = px - dtk * d.fx[s];
-        py = py - dtk * d.fy[s];
-        pz = pz - dtk * d.fz[s];
+        fx = d.fx[s];
+        fy = d.fy[s];
+        fz = d.fz[s];
+    }
+    const bool upkeep = drift && d.nl_pos;
+    float4 q = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (upkeep) q = d.nlr[s];  // position at the last build, build position
+    if (kick) {
+        px = px - dtk * fx;
+        py = py - dtk * fy;
+        pz = pz - dtk * fz;
         d.px[s] = px;
         d.py[s] = py;
         d.pz[s] = pz;
@@ -58,12 +69,11 @@ __device__ __forceinline__ void move_atom(const DevState &d, const Grid &g, int 
         d.rx[s] = x;
         d.ry[s] = y;
         d.rz[s] = z;
-        if (d.nl_pos) {
+        if (upkeep) {
             // absolute position for the neighbour lists, and the rebuild
             // request once the atom has moved half the skin (nearest image)
             double cx, cy, cz;
             corner_of(g, ix, iy, iz, cx, cy, cz);
-            const float4 q = d.nlr[s];  // position at the last build, build position
             const double ax = cx + x, ay = cy + y, az = cz + z;
             d.nl_pos[__float_as_int(q.w)] = CmPos{ax, ay, az};
             double ex = ax - q.x, ey = ay - q.y, ez = az - q.z;

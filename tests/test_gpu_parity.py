@@ -137,6 +137,28 @@ def test_rebox_multi_hop_serial_pass(gpu, cu500, ocu500, case):
         assert abs(sim.energy()[0] - osim.info()["e"]) <= 1e-10 * abs(osim.info()["e"])
 
 
+def test_upload_rejects_bad_ids(gpu, cu500):
+    """comd_upload checks the ids the neighbour lists index by: a negative id
+    and an id given twice are COMD_ERR_ARG, and the context keeps working
+    with a valid upload afterwards."""
+    xyz, p, bounds = seeded_atoms(6, 3.615, 63.55, temp=300.0, disp=0.05, seed=2)
+    sim = CA.Simulation(cu500, bounds, defgrad=0.0)
+    sim.bin_atoms(xyz, p)
+    st = sim.download()
+    occ = np.flatnonzero((np.arange(64)[None, :] < st["natoms"][:, None]).reshape(-1))
+    for what, change in (("negative", lambda ids: ids.__setitem__(occ[3], -5)),
+                         ("repeated", lambda ids: ids.__setitem__(occ[7], ids[occ[2]]))):
+        ids = st["id"].copy()
+        change(ids)
+        with pytest.raises(CA.ComdError) as ei:
+            sim.upload(st["natoms"], ids, st["itype"], st["r"], st["p"])
+        print(f"[bad ids] {what}: {ei.value}", flush=True)
+        assert ei.value.code == 1
+    sim.upload(st["natoms"], st["id"], st["itype"], st["r"], st["p"])
+    sim.force()
+    assert sim.list_info()["active"] == 1 and sim.list_info()["valid"] == 1
+
+
 def test_overflow_reported(gpu, cu500):
     """65 atoms in one cell: the library reports COMD_ERR_OVERFLOW (the
     reference writes past the cell silently, src-flat/domains.c:269-275)."""
