@@ -131,7 +131,7 @@ def _load():
     dropin_path = os.path.join(PKG_DIR, "libcomd_dropin.so")
     for p in (hip_path, dropin_path, host_path):
         if not os.path.exists(p):
-            raise ImportError(f"{p} is not built; run __graft_entry__.build() or make -C comd_amd")
+            raise ImportError(f"{p} is not built; run make -C comd_amd")
     hip = C.CDLL(hip_path, mode=C.RTLD_GLOBAL)
     host = C.CDLL(host_path, mode=C.RTLD_GLOBAL)
     vp = C.c_void_p

@@ -410,8 +410,8 @@ int comd_batch_cheb_grad(comd_ctx *ctx, const double *coef, int n, const double 
  * at coef, through the same adjoint kernels with no force term: dE/dc_phi
  * = 1/2 sum_ij T_k(r_ij), dE/dc_rho = sum_ij F'(rho_i) T_k(r_ij), dE/dc_F =
  * sum_i T_k(rho_i) (T_0 halved as the series convention has it).  energy
- * receives E.  No counterpart in the reference (the north star's energy
- * gradient, beside the force-matching loss).  The batch form returns the
+ * receives E.  No counterpart in the reference (it serves potential fitting
+ * on energies, beside the force-matching loss).  The batch form returns the
  * gradient of sum_r w_r E_r (weights NULL: all 1) and each E_r in
  * energies[nrep] (may be NULL). */
 int comd_eam_cheb_energy_grad(comd_ctx *ctx, const double *coef, int n, double *energy, double *grad);
