@@ -339,6 +339,38 @@ def test_steps_from_shared_state(gpu, cu500, ocu500, kind):
         assert ef.max() <= FP64_TOL
 
 
+@pytest.mark.parametrize("kind", ["eam", "lj"])
+def test_nonperiodic_cluster(gpu, cu500, ocu500, kind):
+    """The reference's non-periodic mode (PERIODIC = 0): a hot lattice block
+    in a box with room around it; the stencil leaves out cells beyond the
+    faces (src-flat/domains.c:318) and reBoxAll keeps atoms that cross a
+    face in their edge cell (domains.c:375-378).  Forces at 1e-10 and 10
+    steps, each from the oracle's state, with slot order exact."""
+    print(f"[case] non-periodic {kind}", flush=True)
+    if kind == "eam":
+        pot, opot, lat = cu500, ocu500, 3.615
+    else:
+        pot, opot = lj_pot_pair()
+        lat = pot.lat
+    xyz, p, _ = seeded_atoms(5, lat, pot.mass, temp=3000.0, disp=0.05, seed=17)
+    xyz = xyz + 2.0 * lat  # the block away from the low faces
+    bounds = (9 * lat,) * 3
+    sim, osim = make_pair(pot, opot, xyz, p, bounds, periodic=False)
+    sim.rebox()
+    osim.rebox()
+    assert sim.list_info()["active"] == 0  # lists need a periodic box
+    sim.force()
+    osim.force()
+    check_forces(sim.download(), osim.state(), FP64_TOL, f"non-periodic {kind} force", lj=kind == "lj")
+    for step in range(10):
+        osim.set_state(sim.download())
+        sim.nsteps(1)
+        osim.nsteps(1)
+        g, o = assert_same_layout(sim.download(), osim.state(), f"non-periodic {kind} step {step}")
+        np.testing.assert_allclose(g["r"], o["r"], rtol=0, atol=1e-12)
+        assert force_rel_error(g["f"], o["f"]).max() <= FP64_TOL
+
+
 def test_trajectory_energy(gpu, cu500, ocu500):
     """20 steps on both sides: the energies stay together and the DKD
     integrator conserves PE + KE for EAM (f = +grad U)."""
