@@ -181,6 +181,7 @@ struct NlRep {
     double ab0, ab1, ab2, iab0, iab1, iab2;  // auxiliary cell edges and their inverses
     double qscale, qinv;                     // group-frame fixed point of the mixed passes: units per A, A per unit
     double rl2;                              // (cutoff + skin)^2
+    double rc2;                              // cutoff^2
     int na0, na1, na2;                       // auxiliary cells per dimension
     int gd0, gd1, gd2;                       // cells per group (the largest extent)
     int ng0, ng1, ng2;                       // groups per dimension
@@ -206,6 +207,7 @@ struct CmList {
     int lds_avail;           // dynamic LDS a list kernel's block may use
     int rows_lds;            // table rows in LDS: 0 never, else whenever they fit
     int pass_blocks, pass_threads;  // one block per group (passes and build)
+    int tailq;               // build: list entries per thread held back for the list's end (k_nl_build)
     const int *cell_order;   // [naux]: auxiliary cells in build order (group, then cell)
     const int *group_q0;     // [ngroups + 1]: a group's first position in cell_order
     CmPos *pos;              // per build position: absolute position (kept by the drift)

@@ -369,12 +369,20 @@ __global__ __launch_bounds__(256) void k_nl_sort(CmList l)
     }
 }
 
+// Bytes of the staged halo entries' slot map (one byte per entry).
+__host__ __device__ constexpr int nl_slotmap_bytes(int hcap) { return (hcap + 15) / 16 * 16; }
+
 // One block per group (the passes' launch shape): the halo is staged in LDS
 // in the group's frame, in single precision; each home atom (a thread) tests
 // the atoms of its own 27-cell stencil, as 9 runs of 3 x-adjacent cells
-// (contiguous in the halo; the order of the list entries only orders the
-// passes' sums), id order within a cell, and lists the halo indices within
-// cutoff + skin.  The single-precision test compares with
+// (contiguous in the halo), id order within a cell, and lists the halo
+// indices within cutoff + skin.  Entries beyond the cutoff at the build go
+// to the end of the list (up to tailq of them per thread are held in LDS
+// until the stencil is done; any more stay in place): in a lattice those
+// are the shell just beyond the cutoff, which the passes test and reject,
+// and at the same list positions across a wave a rejection by every lane
+// skips the pair term's evaluation.  The order of the entries only orders
+// the passes' sums.  The single-precision test compares with
 // (cutoff + skin)^2 x (1 + 4e-6), more than its rounding (frame coordinates
 // within 20 A: at most 1.5e-6 of r^2), so no pair within cutoff + skin is
 // left out; the few extra pairs are rejected by the passes' cutoff test.
@@ -386,6 +394,8 @@ __global__ __launch_bounds__(1024) void k_nl_build(DevState d, CmList l)
     if (!nl_rebuild(l)) return;
     extern __shared__ float4 hl4[];  // [hcap] frame x, y, z; then the slot map
     CM_LDS uint8_t *slot_of = (CM_LDS uint8_t *)((CM_LDS char *)hl4 + (size_t)l.hcap * 16);
+    // then the tail queue: entry j of thread x at [j * blockDim.x + x]
+    CM_LDS uint16_t *tq = (CM_LDS uint16_t *)(slot_of + nl_slotmap_bytes(l.hcap)) + threadIdx.x;
     __shared__ GroupLds S;
     __shared__ int last;
     bool ovf = false;
@@ -395,7 +405,7 @@ __global__ __launch_bounds__(1024) void k_nl_build(DevState d, CmList l)
     const int g = nb < 8 ? (int)blockIdx.x : xcd * qb + min(xcd, rem) + idx;
     const NlRep R = nl_rep(l, l.group_rep[g]);
     const GroupGeo G = group_geo(R, g - R.group_base);
-    const float rl2 = (float)(R.rl2 * (1.0 + 4e-6));
+    const float rl2 = (float)(R.rl2 * (1.0 + 4e-6)), rc2 = (float)R.rc2;
     // the group's replica: the grid of every atom it stages
     const Grid gr = d.ngrids == 1 ? d.g0 : d.grids[l.group_rep[g]];
     group_setup(l, R, G, g, S, slot_of);
@@ -447,29 +457,35 @@ __global__ __launch_bounds__(1024) void k_nl_build(DevState d, CmList l)
             // the 27 stencil cells as 9 runs of three x-adjacent halo slots,
             // whose atoms are contiguous in the halo; entries gathered 8 at a
             // time and stored as one 16-byte group
-            int k = 0;
+            int k = 0, nt = 0;
             unsigned long long lo = 0, hi = 0;
+            auto push = [&](int h) {
+                const int c8 = k & 7;
+                if (c8 < 4) lo |= (unsigned long long)h << (16 * c8);
+                else hi |= (unsigned long long)h << (16 * (c8 - 4));
+                if (c8 == 7 && k < l.kmax) {
+                    ulonglong2 w;
+                    w.x = lo;
+                    w.y = hi;
+                    *(ulonglong2 *)&l.nbl[nl_entry(k - 7, t, l.n)] = w;
+                    lo = hi = 0;
+                }
+                k++;
+            };
             for (int e = 0; e < 9; e++) {
                 const int s0 = (cx - 1) + shx * ((cy + e / 3 - 1) + shy * (cz + e % 3 - 1));
                 const int h1 = S.off[s0 + 3];
                 for (int h = S.off[s0]; h < h1; h++) {
                     const float4 qh = hl4[h];
                     const float dx = pi.x - qh.x, dy = pi.y - qh.y, dz = pi.z - qh.z;
-                    if (fmaf(dx, dx, fmaf(dy, dy, dz * dz)) <= rl2 && h != self) {
-                        const int c8 = k & 7;
-                        if (c8 < 4) lo |= (unsigned long long)h << (16 * c8);
-                        else hi |= (unsigned long long)h << (16 * (c8 - 4));
-                        if (c8 == 7 && k < l.kmax) {
-                            ulonglong2 w;
-                            w.x = lo;
-                            w.y = hi;
-                            *(ulonglong2 *)&l.nbl[nl_entry(k - 7, t, l.n)] = w;
-                            lo = hi = 0;
-                        }
-                        k++;
+                    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                    if (r2 <= rl2 && h != self) {
+                        if (r2 > rc2 && nt < l.tailq) tq[(nt++) * blockDim.x] = (uint16_t)h;
+                        else push(h);
                     }
                 }
             }
+            for (int j = 0; j < nt; j++) push(tq[j * blockDim.x]);
             if ((k & 7) != 0 && k <= l.kmax) {
                 ulonglong2 w;
                 w.x = lo;
@@ -557,8 +573,6 @@ __host__ __device__ constexpr int nl_halo_bytes(int kind, int mixed)
 // though each read takes two LDS instructions.  The embedding-force pass
 // keeps one 16-byte read (a 20-byte stride measured 2 % slower there).
 template <int KIND> __host__ __device__ constexpr int nl_halo_words() { return KIND == CM_KIND_EAM3 ? 4 : 3; }
-// Bytes of the staged halo entries' slot map (one byte per entry).
-__host__ __device__ constexpr int nl_slotmap_bytes(int hcap) { return (hcap + 15) / 16 * 16; }
 
 // One home atom t over its list.  TB: the table pointers (compact rows in
 // LDS when COMPACT, else the full tables in global memory); hq (mixed) or
@@ -964,7 +978,10 @@ void cm_nl_sync(const DevState &d, const CmList &l, hipStream_t st)
     hipLaunchKernelGGL(k_nl_from_slots, dim3(grid_for(d.ncells, 4)), dim3(64), 0, st, d, l, 1);
 }
 
-size_t cm_nl_build_lds(const CmList &l) { return (size_t)l.hcap * 16 + nl_slotmap_bytes(l.hcap); }
+size_t cm_nl_build_lds(const CmList &l)
+{
+    return (size_t)l.hcap * 16 + nl_slotmap_bytes(l.hcap) + (size_t)l.tailq * l.pass_threads * 2;
+}
 
 void cm_nl_build(const DevState &d, const CmList &l, hipStream_t st)
 {
