@@ -1112,15 +1112,7 @@ static int nl_setup(comd_ctx *c)
     l.pass_blocks = (int)ngroups;
     // threads per pass block: a group's home atoms with 10 % to spare
     l.pass_threads = (int)std::min<long>(768, threads);
-    {
-        // the build's tail queue: up to 16 entries per thread in the LDS
-        // left after its halo and slot map, without costing a resident
-        // block per CU
-        l.tailq = 0;
-        const long base = (long)cm_nl_build_lds(l), half = (long)c->lds_block / 2 - CM_NL_STATIC_LDS;
-        const long room = (base <= half ? half : lds_avail) - base;
-        l.tailq = (int)std::max<long>(0, std::min<long>(16, room / (2L * l.pass_threads)));
-    }
+    l.tailq = cm_nl_pick_tailq(l);
     std::vector<NlRep> reps(nrep);
     std::vector<int> corder((size_t)naux), gq0((size_t)ngroups + 1), grep((size_t)ngroups), agroup((size_t)naux);
     long q = 0, g = 0, abase = 0, ibase = 0;

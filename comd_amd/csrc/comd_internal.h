@@ -56,8 +56,9 @@ enum { CM_KIND_EAM1 = 0, CM_KIND_EAM3 = 1, CM_KIND_LJ = 2 };
 // Clenshaw sum with the lower clamp only, src-flat/cheby.c:276-310, over a
 // series zero-padded to m8 coefficients (a multiple of 8; leading zeros
 // leave the sum exactly as it is).  The coefficients, in global memory at an
-// address common to the wave, are loaded 8 at a time ahead of their
-// iterations.
+// address common to the wave, are read by scalar loads 8 at a time, each
+// block's loads issued before the previous block's iterations so that their
+// latency is hidden behind them.
 __device__ __forceinline__ double cheb64(const double *cg, int m8, double a, double b, double x)
 {
     const CM_CONST double *c = (const CM_CONST double *)cg;
@@ -65,10 +66,14 @@ __device__ __forceinline__ double cheb64(const double *cg, int m8, double a, dou
     double d = 0.0, dd = 0.0;
     const double y = (2.0 * x - a - b) / (b - a);
     const double y2 = 2.0 * y;
-    for (int jb = m8 - 8; jb >= 0; jb -= 8) {
-        double cc[8];
+    double cc[8];
 #pragma unroll
-        for (int u = 0; u < 8; u++) cc[u] = c[jb + u];
+    for (int u = 0; u < 8; u++) cc[u] = c[m8 - 8 + u];
+    for (int jb = m8 - 8; jb >= 0; jb -= 8) {
+        double cn[8];
+        const int jn = jb >= 8 ? jb - 8 : 0;  // the next block (block 0 again after the last)
+#pragma unroll
+        for (int u = 0; u < 8; u++) cn[u] = c[jn + u];
 #pragma unroll
         for (int u = 7; u >= 0; u--) {
             if (jb + u > 0) {
@@ -77,8 +82,10 @@ __device__ __forceinline__ double cheb64(const double *cg, int m8, double a, dou
                 dd = sv;
             }
         }
+#pragma unroll
+        for (int u = 0; u < 8; u++) cc[u] = cn[u];
     }
-    return y * d - dd + 0.5 * c[0];
+    return y * d - dd + 0.5 * cc[0];
 }
 
 __device__ __forceinline__ float cheb32(const float *cg, int m8, float a, float b, float x)
@@ -88,10 +95,14 @@ __device__ __forceinline__ float cheb32(const float *cg, int m8, float a, float 
     float d = 0.0f, dd = 0.0f;
     const float y = (2.0f * x - a - b) / (b - a);
     const float y2 = 2.0f * y;
-    for (int jb = m8 - 8; jb >= 0; jb -= 8) {
-        float cc[8];
+    float cc[8];
 #pragma unroll
-        for (int u = 0; u < 8; u++) cc[u] = c[jb + u];
+    for (int u = 0; u < 8; u++) cc[u] = c[m8 - 8 + u];
+    for (int jb = m8 - 8; jb >= 0; jb -= 8) {
+        float cn[8];
+        const int jn = jb >= 8 ? jb - 8 : 0;
+#pragma unroll
+        for (int u = 0; u < 8; u++) cn[u] = c[jn + u];
 #pragma unroll
         for (int u = 7; u >= 0; u--) {
             if (jb + u > 0) {
@@ -100,8 +111,10 @@ __device__ __forceinline__ float cheb32(const float *cg, int m8, float a, float 
                 dd = sv;
             }
         }
+#pragma unroll
+        for (int u = 0; u < 8; u++) cc[u] = cn[u];
     }
-    return fmaf(y, d, 0.5f * c[0] - dd);
+    return fmaf(y, d, 0.5f * cc[0] - dd);
 }
 
 
@@ -240,6 +253,9 @@ void cm_nl_init(const DevState &d, const CmList &l, hipStream_t st);
 void cm_nl_build(const DevState &d, const CmList &l, hipStream_t st);
 void cm_nl_sync(const DevState &d, const CmList &l, hipStream_t st);
 size_t cm_nl_build_lds(const CmList &l);
+// Entries per thread of the build's tail queue: the most (16, 12, 8 or 4)
+// that fit without fewer resident build blocks per CU than with none.
+int cm_nl_pick_tailq(CmList l);
 size_t cm_nl_pass_lds(const PotDev &P, const CmList &l, int kind, int mixed, int *tables_in_lds);
 size_t cm_nl_tab_bytes(const PotDev &P, int kind, int mixed);
 // The list pass of one kind over groups [g0, g0 + ng) (a single

@@ -378,7 +378,8 @@ __host__ __device__ constexpr int nl_slotmap_bytes(int hcap) { return (hcap + 15
 // (contiguous in the halo), id order within a cell, and lists the halo
 // indices within cutoff + skin.  Entries beyond the cutoff at the build go
 // to the end of the list (up to tailq of them per thread are held in LDS
-// until the stencil is done; any more stay in place): in a lattice those
+// until the stencil is done, as many as fit without fewer resident blocks
+// per CU; any more stay in place): in a lattice those
 // are the shell just beyond the cutoff, which the passes test and reject,
 // and at the same list positions across a wave a rejection by every lane
 // skips the pair term's evaluation.  The order of the entries only orders
@@ -981,6 +982,26 @@ void cm_nl_sync(const DevState &d, const CmList &l, hipStream_t st)
 size_t cm_nl_build_lds(const CmList &l)
 {
     return (size_t)l.hcap * 16 + nl_slotmap_bytes(l.hcap) + (size_t)l.tailq * l.pass_threads * 2;
+}
+
+int cm_nl_pick_tailq(CmList l)
+{
+    l.tailq = 0;
+    int nb0 = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb0, k_nl_build, l.pass_threads, cm_nl_build_lds(l)) !=
+            hipSuccess ||
+        nb0 < 1)
+        return 0;
+    for (int q = 16; q >= 4; q -= 4) {
+        l.tailq = q;
+        const size_t lds = cm_nl_build_lds(l);
+        int nb = 0;
+        if ((long)lds <= (long)l.lds_avail &&
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_nl_build, l.pass_threads, lds) == hipSuccess &&
+            nb >= nb0)
+            return q;
+    }
+    return 0;
 }
 
 void cm_nl_build(const DevState &d, const CmList &l, hipStream_t st)
