@@ -550,6 +550,92 @@ __global__ void k_nl_nonempty(DevState d, CmList l, int g0)
     l.nonempty[c] = n;
 }
 
+// Chebyshev passes: N pairs of a list chunk evaluated together, each series
+// as N interleaved Clenshaw recurrences (src-flat/cheby.c:276-310) that
+// share every coefficient load: evaluated pair by pair, each pair reloaded
+// all coefficients and ran one serial recurrence at a time.  x[u] is
+// clamped below at a as in cheb64; the series' argument is formed with
+// 1 / (b - a) computed once rather than a division per pair.
+template <int N>
+__device__ __forceinline__ void cheb64_n(const double *cg, int m8, double a, double b, const double (&x)[N],
+                                         double (&out)[N])
+{
+    const CM_CONST double *c = (const CM_CONST double *)cg;
+    const double inv = 1.0 / (b - a);
+    double y2[N], d[N], dd[N];
+#pragma unroll
+    for (int u = 0; u < N; u++) {
+        const double xu = x[u] < a ? a : x[u];
+        y2[u] = 2.0 * ((2.0 * xu - a - b) * inv);
+        d[u] = dd[u] = 0.0;
+    }
+    double cc[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) cc[k] = c[m8 - 8 + k];
+    for (int jb = m8 - 8; jb >= 0; jb -= 8) {
+        double cn[8];
+        const int jn = jb >= 8 ? jb - 8 : 0;  // the next block (block 0 again after the last)
+#pragma unroll
+        for (int k = 0; k < 8; k++) cn[k] = c[jn + k];
+#pragma unroll
+        for (int k = 7; k >= 0; k--) {
+            if (jb + k > 0) {
+#pragma unroll
+                for (int u = 0; u < N; u++) {
+                    const double sv = d[u];
+                    d[u] = y2[u] * d[u] - dd[u] + cc[k];
+                    dd[u] = sv;
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) cc[k] = cn[k];
+    }
+#pragma unroll
+    for (int u = 0; u < N; u++) out[u] = 0.5 * y2[u] * d[u] - dd[u] + 0.5 * cc[0];
+}
+
+// The same in single precision (cheb32's recurrence) for the values of the
+// mixed density pass.
+template <int N>
+__device__ __forceinline__ void cheb32_n(const float *cg, int m8, float a, float b, const float (&x)[N],
+                                         float (&out)[N])
+{
+    const CM_CONST float *c = (const CM_CONST float *)cg;
+    const float inv = 1.0f / (b - a);
+    float y2[N], d[N], dd[N];
+#pragma unroll
+    for (int u = 0; u < N; u++) {
+        const float xu = fmaxf(x[u], a);
+        y2[u] = 2.0f * ((2.0f * xu - a - b) * inv);
+        d[u] = dd[u] = 0.0f;
+    }
+    float cc[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) cc[k] = c[m8 - 8 + k];
+    for (int jb = m8 - 8; jb >= 0; jb -= 8) {
+        float cn[8];
+        const int jn = jb >= 8 ? jb - 8 : 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) cn[k] = c[jn + k];
+#pragma unroll
+        for (int k = 7; k >= 0; k--) {
+            if (jb + k > 0) {
+#pragma unroll
+                for (int u = 0; u < N; u++) {
+                    const float sv = d[u];
+                    d[u] = fmaf(y2[u], d[u], cc[k] - dd[u]);
+                    dd[u] = sv;
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) cc[k] = cn[k];
+    }
+#pragma unroll
+    for (int u = 0; u < N; u++) out[u] = fmaf(0.5f * y2[u], d[u], 0.5f * cc[0] - dd[u]);
+}
+
 // Compact table rows a pass reads (density: phi and rho, 8 values per
 // interval, 64 bytes or 40 in mixed precision; embedding force: rho', 2
 // doubles), staged per block in LDS: every
@@ -628,6 +714,113 @@ __device__ __forceinline__ void nl_atom(const DevState &d, const PotDev &P, cons
         for (int u = 0; u < 8; u++) {
             hj[u] = (int)((ew[u >> 1] >> (16 * (u & 1))) & 0xffffu);
             if (TAIL && k0 + u >= cnt) hj[u] = 0;
+        }
+        // separation of pair u (angstrom) from the staged halo
+        auto sep = [&](int u, double &dx, double &dy, double &dz) {
+            if constexpr (MIXED) {
+                const CM_LDS unsigned *q = hu + HW * hj[u];
+                dx = (double)(int)(qi.x - q[0]) * qinv;
+                dy = (double)(int)(qi.y - q[1]) * qinv;
+                dz = (double)(int)(qi.z - q[2]) * qinv;
+            } else {
+                dx = xi - hx[hj[u]];
+                dy = yi - hy[hj[u]];
+                dz = zi - hz[hj[u]];
+            }
+        };
+        if constexpr (EVAL == COMD_EVAL_CHEB && KIND != CM_KIND_LJ) {
+            // the chunk's pairs NB at a time (cheb64_n): r of each, the
+            // series, then the sums in list order, the separations read
+            // again from the halo.  Four in FP64, two in mixed precision,
+            // where the registers of more recurrences cost more resident
+            // waves than the wider interleave gains (measured, 256 000
+            // atoms: FP64 passes 0.333 + 0.147 -> 0.240 + 0.118 ms, mixed
+            // 0.245 + 0.113 -> 0.186 + 0.096 ms).
+            constexpr int NB = MIXED ? 2 : 4;
+#pragma unroll
+            for (int h = 0; h < 8; h += NB) {
+                double rr[NB];
+                unsigned msk = 0;
+#pragma unroll
+                for (int j = 0; j < NB; j++) {
+                    const int u = h + j;
+                    double r2;
+                    bool in;
+                    if constexpr (MIXED) {
+                        const CM_LDS unsigned *q = hu + HW * hj[u];
+                        const double dxu = (double)(int)(qi.x - q[0]), dyu = (double)(int)(qi.y - q[1]),
+                                     dzu = (double)(int)(qi.z - q[2]);
+                        const double r2u = dxu * dxu + dyu * dyu + dzu * dzu;
+                        in = KIND == CM_KIND_EAM3 ? (r2u < rc2u) : (r2u <= rc2u);
+                        r2 = r2u * (qinv * qinv);
+                    } else {
+                        double dx, dy, dz;
+                        sep(u, dx, dy, dz);
+                        r2 = dx * dx + dy * dy + dz * dz;
+                        in = KIND == CM_KIND_EAM3 ? (r2 < rc2) : (r2 <= rc2);
+                    }
+                    in = in && (!TAIL || k0 + u < cnt);
+                    rr[j] = in ? r2 * (MIXED ? rsqrt_mixed(r2) : rsqrt64(r2)) : 0.0;  // outside: the lower end
+                    msk |= (unsigned)in << j;
+                }
+                if (msk == 0) continue;
+                npair += __builtin_popcount(msk);
+                const int m = P.cstride;
+                double v[NB];
+                if constexpr (KIND == CM_KIND_EAM1) {
+                    if constexpr (MIXED) {
+                        float rf[NB], vf[NB];
+#pragma unroll
+                        for (int j = 0; j < NB; j++) rf[j] = (float)rr[j];
+                        cheb32_n<NB>(P.cheb32 + 0 * m, m, (float)P.ca[0], (float)P.cb[0], rf, vf);
+#pragma unroll
+                        for (int j = 0; j < NB; j++)
+                            if ((msk >> j) & 1u) A.e += 0.5 * (double)vf[j];
+                        cheb32_n<NB>(P.cheb32 + 1 * m, m, (float)P.ca[1], (float)P.cb[1], rf, vf);
+#pragma unroll
+                        for (int j = 0; j < NB; j++)
+                            if ((msk >> j) & 1u) A.rho += (double)vf[j];
+                    } else {
+                        cheb64_n<NB>(P.cheb + 0 * m, m, P.ca[0], P.cb[0], rr, v);
+#pragma unroll
+                        for (int j = 0; j < NB; j++)
+                            if ((msk >> j) & 1u) A.e += 0.5 * v[j];
+                        cheb64_n<NB>(P.cheb + 1 * m, m, P.ca[1], P.cb[1], rr, v);
+#pragma unroll
+                        for (int j = 0; j < NB; j++)
+                            if ((msk >> j) & 1u) A.rho += v[j];
+                    }
+                    cheb64_n<NB>(P.cheb + 3 * m, m, P.ca[0], P.cb[0], rr, v);  // phi', FP64 in both modes
+#pragma unroll
+                    for (int j = 0; j < NB; j++) {
+                        if ((msk >> j) & 1u) {
+                            double dx, dy, dz;
+                            sep(h + j, dx, dy, dz);
+                            const double sc = v[j] / rr[j];
+                            A.x += sc * dx;
+                            A.y += sc * dy;
+                            A.z += sc * dz;
+                        }
+                    }
+                } else {
+                    cheb64_n<NB>(P.cheb + 4 * m, m, P.ca[1], P.cb[1], rr, v);  // rho', FP64 in both modes
+#pragma unroll
+                    for (int j = 0; j < NB; j++) {
+                        if ((msk >> j) & 1u) {
+                            double dx, dy, dz, fij;
+                            sep(h + j, dx, dy, dz);
+                            if constexpr (MIXED) fij = (double)__uint_as_float(hq[hj[h + j]].w);
+                            else fij = hf[hj[h + j]];
+                            const double g = (fia + fij) * v[j] / rr[j];
+                            A.x += g * dx;
+                            A.y += g * dy;
+                            A.z += g * dz;
+                            A.vir += g * dx * dx;
+                        }
+                    }
+                }
+            }
+            return;
         }
         if constexpr (MIXED) {
             // x, y, z only unless F' is needed
@@ -742,8 +935,12 @@ __device__ __forceinline__ void nl_atom(const DevState &d, const PotDev &P, cons
 // read from global memory (0); a compile-time choice, so the row reads are
 // LDS or global loads.  Dynamic LDS: [rows (TLDS)] [halo, hcap atoms]
 // [slot of each halo entry, hcap bytes].
+// Blocks of at most 768 threads (a group's home atoms); the Chebyshev passes
+// declare that bound, so that their four-pair recurrences (cheb64_n) get the
+// registers three waves per SIMD allow.
 template <int KIND, int EVAL, int MIXED, int TLDS>
-__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(MIXED && KIND != CM_KIND_LJ && TLDS ? 6 : 1)))
+__global__ __attribute__((amdgpu_flat_work_group_size(1, EVAL == COMD_EVAL_CHEB ? 768 : 1024)))
+__attribute__((amdgpu_waves_per_eu(MIXED && KIND != CM_KIND_LJ && TLDS ? 6 : 1)))
 void k_nl_pair(DevState d, PotDev P, CmList l, int tab_bytes, int gbase)
 {
     extern __shared__ __align__(16) char nls[];
@@ -1035,10 +1232,13 @@ static void nl_launch(const DevState &d, const PotDev &P, const CmList &l, hipSt
     const size_t lds = cm_nl_pass_lds(P, l, KIND, MIXED, &tl);
     const int tab = tl ? (int)nl_tab_bytes(P, KIND, MIXED) : 0;
     const dim3 grid(ng), block(l.pass_threads);
-    if (EVAL == COMD_EVAL_TABLE && KIND != CM_KIND_LJ && tl)
-        hipLaunchKernelGGL((k_nl_pair<KIND, EVAL, MIXED, 1>), grid, block, lds, st, d, P, l, tab, g0);
-    else
-        hipLaunchKernelGGL((k_nl_pair<KIND, EVAL, MIXED, 0>), grid, block, lds, st, d, P, l, 0, g0);
+    if constexpr (EVAL == COMD_EVAL_TABLE && KIND != CM_KIND_LJ) {
+        if (tl) {
+            hipLaunchKernelGGL((k_nl_pair<KIND, EVAL, MIXED, 1>), grid, block, lds, st, d, P, l, tab, g0);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_nl_pair<KIND, EVAL, MIXED, 0>), grid, block, lds, st, d, P, l, 0, g0);
 }
 
 // Resident blocks per CU of the pass nl_launch would start, and its dynamic LDS.
@@ -1047,9 +1247,9 @@ static int nl_occupancy(const PotDev &P, const CmList &l, int *lds_bytes)
 {
     int tl = 0;
     const size_t lds = cm_nl_pass_lds(P, l, KIND, MIXED, &tl);
-    const void *fn = (EVAL == COMD_EVAL_TABLE && KIND != CM_KIND_LJ && tl)
-                         ? (const void *)k_nl_pair<KIND, EVAL, MIXED, 1>
-                         : (const void *)k_nl_pair<KIND, EVAL, MIXED, 0>;
+    const void *fn = (const void *)k_nl_pair<KIND, EVAL, MIXED, 0>;
+    if constexpr (EVAL == COMD_EVAL_TABLE && KIND != CM_KIND_LJ)
+        if (tl) fn = (const void *)k_nl_pair<KIND, EVAL, MIXED, 1>;
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, l.pass_threads, lds) != hipSuccess) nb = -1;
     if (lds_bytes) *lds_bytes = (int)lds;
