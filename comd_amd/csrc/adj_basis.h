@@ -45,6 +45,46 @@ __device__ __forceinline__ void add_basis(double y, int m, double wa, double wb,
     }
 }
 
+// add_basis<MIXED, 0> that also returns the series c at y from the same
+// basis values: c_0 / 2 + sum_k c_k T_k(y) (the convention of
+// src-flat/cheby.c:290-310; c zero-padded past m, at an address common to
+// the wave), in place of a separate Clenshaw sum over the same argument.
+template <int MIXED>
+__device__ __forceinline__ double add_basis_sum(double y, int m, double wa, double *a, const double *cg)
+{
+    const CM_CONST double *c = (const CM_CONST double *)cg;
+    double s = 0.5 * c[0];
+    a[0] += wa;
+    if (MIXED) {
+        const float yf = (float)y, y2 = 2.0f * yf;
+        float tm = 1.0f, t = yf;
+#pragma unroll
+        for (int k = 1; k < AMAX; k++) {
+            if (k < m) {
+                a[k] += wa * (double)t;
+                s += c[k] * (double)t;
+            }
+            const float tn = y2 * t - tm;
+            tm = t;
+            t = tn;
+        }
+    } else {
+        const double y2 = 2.0 * y;
+        double tm = 1.0, t = y;
+#pragma unroll
+        for (int k = 1; k < AMAX; k++) {
+            if (k < m) {
+                a[k] += wa * t;
+                s += c[k] * t;
+            }
+            const double tn = y2 * t - tm;
+            tm = t;
+            t = tn;
+        }
+    }
+    return s;
+}
+
 __device__ __forceinline__ double cheb_arg(double x, double a, double b)
 {
     if (x < a) x = a;  // lower clamp only (src-flat/cheby.c:290-293)

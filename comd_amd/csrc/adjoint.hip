@@ -106,7 +106,6 @@ __global__ __launch_bounds__(64) void k_adj_pair(DevState d, PotDev P, AdjDev A)
     const double rc2 = P.rc2;
     const int m8 = P.cstride;
     const double *dser = P.cheb + 4 * m8;      // rho' series
-    const float *dser32 = P.cheb32 + 4 * m8;
 
     double m0[AMAX], m1[AMAX];
 #pragma unroll
@@ -127,9 +126,8 @@ __global__ __launch_bounds__(64) void k_adj_pair(DevState d, PotDev P, AdjDev A)
                 if (PASS == 1) {
                     add_basis<MIXED, 1>(y, m, 0.5 * w, 0.5 * de, m0, m1);
                 } else {
-                    add_basis<MIXED, 0>(y, m, 0.5 * qf[L] * w, 0.0, m0, m1);
-                    const double drho = MIXED ? (double)cheb32(dser32, m8, (float)ca, (float)cb, (float)r)
-                                              : cheb64(dser, m8, ca, cb, r);
+                    // rho'(r) from the same basis values (rho's domain)
+                    const double drho = add_basis_sum<MIXED>(y, m, 0.5 * qf[L] * w, m0, dser);
                     v = drho * w;
                     key = qh[L];
                 }

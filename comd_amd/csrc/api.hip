@@ -126,6 +126,11 @@ struct comd_ctx {
     long adj_gb_cap, adj_part_nl_cap;
     int adj_lists;        // COMD_TUNE_ADJOINT_LISTS: 0 link cells, 1 lists with a group per CU, 2 lists
     int adj_have_target;  // adj_ft holds a target (comd_eam_cheb_grad with f_target NULL reuses it)
+    double *adj_E;        // per replica: the energy reduction's output (3 doubles each)
+    // staging for small per-call uploads (series, adjoint scalars): pinned
+    // host memory and its device copy, stage_cap bytes each
+    char *stage_h, *stage_d;
+    size_t stage_cap;
     std::vector<double> cheb_host;  // current value series phi, rho, F (3 x cn) of replica 0
     // force-kernel launch shape (comd_tune / comd_autotune)
     int ncu, max_lds;
@@ -496,9 +501,10 @@ static void free_all(comd_ctx *c)
                     c->partials, c->red_out, c->d_grids, c->d_cell_grid, c->d_pots,
                     c->adj_g[0], c->adj_g[1], c->adj_g[2], c->adj_lam, c->adj_mu,
                     c->adj_ft, c->adj_f2, c->adj_part, c->adj_out, c->adj_rep, c->obs_part, c->obs_out, c->mon,
-                    c->adj_gb, c->adj_part_nl};
+                    c->adj_gb, c->adj_part_nl, c->adj_E, c->stage_d};
     for (void *p : ptrs)
         if (p) hipFree(p);
+    if (c->stage_h) hipHostFree(c->stage_h);
     for (Replica &r : c->reps)
         for (void *p : r.owned)
             if (p) hipFree(p);
@@ -2192,6 +2198,27 @@ int comd_run_monitored(comd_ctx *c, int nblocks, int block_steps, double dt_seco
 
 // The same value series (3 x n coefficients) for every EAM replica, with the
 // derivative series (src-flat/cheby.c:219-270), zero-padded to the stride.
+// The staging buffers with room for bytes at least; the stream is idle on
+// return (the host may write the pinned buffer).
+static int stage_reserve(comd_ctx *c, size_t bytes)
+{
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (bytes <= c->stage_cap) return COMD_OK;
+    if (c->stage_h) hipHostFree(c->stage_h);
+    if (c->stage_d) hipFree(c->stage_d);
+    c->stage_h = c->stage_d = nullptr;
+    c->stage_cap = 0;
+    bytes = (bytes + 4095) / 4096 * 4096;
+    if (hipHostMalloc((void **)&c->stage_h, bytes, 0) != hipSuccess || hipMalloc((void **)&c->stage_d, bytes) != hipSuccess)
+        return set_err(c, COMD_ERR_NOMEM, "staging buffers");
+    c->stage_cap = bytes;
+    return COMD_OK;
+}
+
+// New value series for every replica, with their derivative series and FP32
+// copies: formed on the host (once per distinct domain), staged in pinned
+// memory and copied to each replica's series by one upload and one kernel,
+// without waiting for either.
 static int set_cheb_all(comd_ctx *c, const double *coef, int n)
 {
     for (Replica &R : c->reps) {
@@ -2199,19 +2226,33 @@ static int set_cheb_all(comd_ctx *c, const double *coef, int n)
             return set_err(c, COMD_ERR_ARG, "a replica has no Chebyshev series");
         if (n != R.pot.cn) return set_err(c, COMD_ERR_ARG, "expected %d coefficients per series, got %d", R.pot.cn, n);
     }
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    for (Replica &R : c->reps) {
-        const int m = R.pot.cn, m8 = R.pot.cstride;
-        std::vector<double> all((size_t)6 * m8, 0.0);
-        std::vector<float> all32((size_t)6 * m8, 0.0f);
+    const int m = n, m8 = c->reps[0].pot.cstride;
+    const size_t per = (size_t)6 * m8 * 12;  // 6 m8 doubles, then 6 m8 floats
+    int rc = stage_reserve(c, per * c->nrep);
+    if (rc) return rc;
+    for (int r = 0; r < c->nrep; r++) {
+        const PotDev &P = c->reps[r].pot;
+        double *all = (double *)(c->stage_h + per * r);
+        float *all32 = (float *)(all + 6 * m8);
+        if (r > 0) {
+            const PotDev &Q = c->reps[r - 1].pot;
+            bool same = true;
+            for (int k = 0; k < 3; k++) same = same && P.ca[k] == Q.ca[k] && P.cb[k] == Q.cb[k];
+            if (same) {
+                memcpy(all, c->stage_h + per * (r - 1), per);
+                continue;
+            }
+        }
+        for (int i = 0; i < 6 * m8; i++) all[i] = 0.0;
         for (int k = 0; k < 3; k++) {
             for (int j = 0; j < m; j++) all[(size_t)k * m8 + j] = coef[(size_t)k * m + j];
-            cheb_derivative(m, R.pot.ca[k], R.pot.cb[k], coef + (size_t)k * m, all.data() + (size_t)(3 + k) * m8);
+            cheb_derivative(m, P.ca[k], P.cb[k], coef + (size_t)k * m, all + (size_t)(3 + k) * m8);
         }
-        for (size_t i = 0; i < all.size(); i++) all32[i] = (float)all[i];
-        HIPCHK(c, hipMemcpy(R.d_cheb, all.data(), all.size() * 8, hipMemcpyHostToDevice));
-        HIPCHK(c, hipMemcpy(R.d_cheb32, all32.data(), all32.size() * 4, hipMemcpyHostToDevice));
+        for (int i = 0; i < 6 * m8; i++) all32[i] = (float)all[i];
     }
+    HIPCHK(c, hipMemcpyAsync(c->stage_d, c->stage_h, per * c->nrep, hipMemcpyHostToDevice, c->stream));
+    cm_launch_scatter_series(c->d_pots, c->nrep, c->stage_d, m8, c->stream);
+    HIPCHK(c, hipGetLastError());
     c->cheb_host.assign(coef, coef + 3 * n);
     return COMD_OK;
 }
@@ -2257,8 +2298,7 @@ static int cheb_grad(comd_ctx *c, const double *coef, int n, const double *f_tar
         for (int k = 0; k < 3; k++)
             if (R.pot.ca[k] != R0.pot.ca[k] || R.pot.cb[k] != R0.pot.cb[k] || R.pot.rc2 != R0.pot.rc2)
                 return set_err(c, COMD_ERR_ARG, "the replicas' series domains or cutoffs differ");
-    int rc = set_cheb_all(c, coef, n);
-    if (rc) return rc;
+    int rc = COMD_OK;
     const int m = n;
     const int NM = cm_adjoint_moments();
     const size_t S = (size_t)c->nslots;
@@ -2277,52 +2317,66 @@ static int cheb_grad(comd_ctx *c, const double *coef, int n, const double *f_tar
         alloc((void **)&c->adj_part, (size_t)c->adj_nblocks * NM * 8);
         alloc((void **)&c->adj_out, (size_t)5 * NM * 8);
         alloc((void **)&c->adj_rep, (size_t)2 * c->nrep * 8);
+        alloc((void **)&c->adj_E, (size_t)3 * c->nrep * 8);
         if (!ok) return set_err(c, COMD_ERR_NOMEM, "adjoint buffers");
     }
-    // F'' series: the derivative recurrence applied twice, zero-padded
-    std::vector<double> d1(m), d2(m);
-    cheb_derivative(m, R0.pot.ca[2], R0.pot.cb[2], coef + 2 * m, d1.data());
-    cheb_derivative(m, R0.pot.ca[2], R0.pot.cb[2], d1.data(), d2.data());
-    d2.resize((size_t)R0.pot.cstride, 0.0);
-    HIPCHK(c, hipMemcpy(c->adj_f2, d2.data(), d2.size() * 8, hipMemcpyHostToDevice));
     if (f_target && !emode) {
         HIPCHK(c, hipMemcpy(c->adj_ft, f_target, S * 3 * 8, hipMemcpyHostToDevice));
         c->adj_have_target = 1;
     }
+    // staged after the series (set_cheb_all's upload reads the staging
+    // buffer's head): F'' series (the derivative recurrence applied twice,
+    // zero-padded), then per replica the energy target or weight and the
+    // residual weight 1 / (3 N)
+    const int m8 = R0.pot.cstride, nrep = c->nrep;
+    const size_t off = (size_t)6 * m8 * 12 * nrep;
+    rc = stage_reserve(c, off + ((size_t)m8 + 2 * nrep) * 8);
+    if (rc) return rc;
+    rc = set_cheb_all(c, coef, n);
+    if (rc) return rc;
+    {
+        double *h = (double *)(c->stage_h + off);
+        std::vector<double> d1(m);
+        for (int j = 0; j < m8; j++) h[j] = 0.0;
+        cheb_derivative(m, R0.pot.ca[2], R0.pot.cb[2], coef + 2 * m, d1.data());
+        cheb_derivative(m, R0.pot.ca[2], R0.pot.cb[2], d1.data(), h);
+        for (int r = 0; r < nrep; r++) {
+            h[m8 + r] = emode ? e_weight[r] : e_target[r];
+            h[m8 + nrep + r] = c->reps[r].ntot > 0 ? 1.0 / (3.0 * (double)c->reps[r].ntot) : 0.0;
+        }
+        HIPCHK(c, hipMemcpyAsync(c->stage_d + off, h, ((size_t)m8 + 2 * nrep) * 8, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->adj_f2, c->stage_d + off, (size_t)m8 * 8, hipMemcpyDeviceToDevice, c->stream));
+    }
+    const double *d_w = (const double *)(c->stage_d + off) + m8, *d_rw = d_w + nrep;
 
-    // forward: forces and energies through the series
-    std::vector<int> eval_saved(c->nrep);
-    for (int r = 0; r < c->nrep; r++) {
+    // forward: forces and energies through the series (the descriptors
+    // switched to the series and back only for replicas that use tables)
+    bool tables = false;
+    for (const Replica &R : c->reps) tables = tables || R.pot.eval != COMD_EVAL_CHEB;
+    std::vector<int> eval_saved(nrep);
+    for (int r = 0; r < nrep; r++) {
         eval_saved[r] = c->reps[r].pot.eval;
         c->reps[r].pot.eval = COMD_EVAL_CHEB;
     }
-    if (c->nrep > 1) {
+    if (tables) {
         rc = upload_descriptors(c);
         if (rc) return rc;
     }
     op_force(c);
-    for (int r = 0; r < c->nrep; r++) c->reps[r].pot.eval = eval_saved[r];
-    if (c->nrep > 1) {
+    for (int r = 0; r < nrep; r++) c->reps[r].pot.eval = eval_saved[r];
+    if (tables) {
         rc = upload_descriptors(c);
         if (rc) return rc;
     }
-    std::vector<double> E(c->nrep), rep((size_t)2 * c->nrep);
-    double lossE = 0.0;
-    for (int r = 0; r < c->nrep; r++) {
-        rc = energy_rep(c, r, &E[r], nullptr, nullptr);
-        if (rc) return rc;
-        if (energies) energies[r] = E[r];
-        if (emode) {
-            rep[r] = e_weight[r];
-            rep[c->nrep + r] = 0.0;
-            lossE += e_weight[r] * E[r];
-            continue;
+    // each replica's energy into adj_E, then dL/dE on the device
+    {
+        DevState d = dev_state(c);
+        for (int r = 0; r < nrep; r++) {
+            cm_launch_reduce(d, c->reps[r].g, c->partials, c->nparts, c->stream);
+            cm_launch_reduce_final(c->partials, c->nparts, c->adj_E + 3 * r, c->stream);
         }
-        rep[r] = 2.0 * w_e * (E[r] - e_target[r]);
-        rep[c->nrep + r] = c->reps[r].ntot > 0 ? 1.0 / (3.0 * (double)c->reps[r].ntot) : 0.0;
-        lossE += w_e * (E[r] - e_target[r]) * (E[r] - e_target[r]);
+        cm_launch_adj_rep(c->adj_E, d_w, d_rw, nrep, emode, w_e, c->adj_rep, c->stream);
     }
-    HIPCHK(c, hipMemcpy(c->adj_rep, rep.data(), rep.size() * 8, hipMemcpyHostToDevice));
 
     const int mixed = c->precision == COMD_PREC_MIXED;
     CmAdjoint J;
@@ -2399,9 +2453,16 @@ static int cheb_grad(comd_ctx *c, const double *coef, int n, const double *f_tar
         for (int stage = 0; stage < 4; stage++) cm_launch_adj_stage(d, P, J, stage, mixed, c->stream);
     }
     HIPCHK(c, hipGetLastError());
-    std::vector<double> out((size_t)5 * NM);
+    std::vector<double> out((size_t)5 * NM), E3((size_t)3 * nrep);
     HIPCHK(c, hipMemcpyAsync(out.data(), c->adj_out, out.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(E3.data(), c->adj_E, E3.size() * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    double lossE = 0.0;
+    for (int r = 0; r < nrep; r++) {
+        const double E = E3[(size_t)3 * r];
+        if (energies) energies[r] = E;
+        lossE += emode ? e_weight[r] * E : w_e * (E - e_target[r]) * (E - e_target[r]);
+    }
 
     if (loss) *loss = (emode ? 0.0 : out[(size_t)4 * NM]) + lossE;
     if (!grad) return COMD_OK;

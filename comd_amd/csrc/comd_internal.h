@@ -141,6 +141,14 @@ void cm_launch_force_batch(const DevState &d, const PotDev *pots, int pass, int 
 // Energies of one replica's cells: per-block partial (PE, KE, virial).
 void cm_launch_reduce(const DevState &d, const Grid &g, double *partials, int nparts, hipStream_t st);
 void cm_launch_reduce_final(const double *partials, int nparts, double *out, hipStream_t st);
+// Staged value and derivative series (FP64, then FP32; 72 m8 bytes per
+// replica) copied to every replica's series buffers (pots: the device
+// descriptors).
+void cm_launch_scatter_series(const PotDev *pots, int nrep, const char *src, int m8, hipStream_t st);
+// The adjoint's per-replica dL/dE and residual weights from the energies E
+// (3 doubles per replica) on the device.
+void cm_launch_adj_rep(const double *E, const double *w, const double *rw, int nrep, int emode, double w_e,
+                       double *rep, hipStream_t st);
 
 // Device buffers and scalars of the Chebyshev-coefficient adjoint
 // (adjoint.hip).  out holds 5 x cm_adjoint_moments() doubles: the moments of

@@ -1083,7 +1083,6 @@ __global__ __launch_bounds__(768) void k_adj_nl(DevState d, PotDev P, CmList l, 
     const double rc2 = P.rc2;
     const int m8 = P.cstride;
     const double *dser = P.cheb + 4 * m8;  // rho' series
-    const float *dser32 = P.cheb32 + 4 * m8;
     double m0[AMAX];
 #pragma unroll
     for (int k = 0; k < AMAX; k++) m0[k] = 0.0;
@@ -1144,9 +1143,8 @@ __global__ __launch_bounds__(768) void k_adj_nl(DevState d, PotDev P, CmList l, 
                 if (PASS == 1) {
                     add_basis<MIXED, 0>(y, m, 0.5 * w, 0.0, m0, m0);
                 } else {
-                    add_basis<MIXED, 0>(y, m, 0.5 * (wi + l.fq[q]) * w, 0.0, m0, m0);
-                    const double drho = MIXED ? (double)cheb32(dser32, m8, (float)ca, (float)cb, (float)r)
-                                              : cheb64(dser, m8, ca, cb, r);
+                    // rho'(r) from the same basis values (rho's domain)
+                    const double drho = add_basis_sum<MIXED>(y, m, 0.5 * (wi + l.fq[q]) * w, m0, dser);
                     lam += drho * w;
                 }
             }

@@ -58,3 +58,42 @@ void cm_launch_reduce_final(const double *partials, int nparts, double *out, hip
 {
     hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, st, partials, nparts, out);
 }
+
+// Block r copies replica r's staged series (6 m8 doubles, then 6 m8 floats,
+// at src + r * 72 m8 bytes) to its series buffers.
+__global__ __launch_bounds__(256) void k_scatter_series(const PotDev *pots, const char *src, int m8)
+{
+    const PotDev &P = pots[blockIdx.x];
+    const double *s64 = (const double *)(src + (size_t)blockIdx.x * 72 * m8);
+    const float *s32 = (const float *)(s64 + 6 * m8);
+    double *d64 = (double *)P.cheb;
+    float *d32 = (float *)P.cheb32;
+    for (int i = threadIdx.x; i < 6 * m8; i += 256) {
+        d64[i] = s64[i];
+        d32[i] = s32[i];
+    }
+}
+
+void cm_launch_scatter_series(const PotDev *pots, int nrep, const char *src, int m8, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_scatter_series, dim3((unsigned)nrep), dim3(256), 0, st, pots, src, m8);
+}
+
+// Per replica r of a gradient call: dL/dE (emode: its weight w[r]; else
+// 2 w_e (E_r - E_target,r) with E_r = E[3 r], the potential energy of the
+// energy reduction) and the force-residual weight (emode: 0; else rw[r]).
+__global__ __launch_bounds__(256) void k_adj_rep(const double *E, const double *w, const double *rw, int nrep,
+                                                 int emode, double w_e, double *rep)
+{
+    for (int r = blockIdx.x * 256 + threadIdx.x; r < nrep; r += gridDim.x * 256) {
+        rep[r] = emode ? w[r] : 2.0 * w_e * (E[3 * r] - w[r]);
+        rep[nrep + r] = emode ? 0.0 : rw[r];
+    }
+}
+
+void cm_launch_adj_rep(const double *E, const double *w, const double *rw, int nrep, int emode, double w_e,
+                       double *rep, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_adj_rep, dim3((unsigned)((nrep + 255) / 256)), dim3(256), 0, st, E, w, rw, nrep, emode,
+                       w_e, rep);
+}
