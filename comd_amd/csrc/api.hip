@@ -127,6 +127,7 @@ struct comd_ctx {
     int adj_lists;        // COMD_TUNE_ADJOINT_LISTS: 0 link cells, 1 lists with a group per CU, 2 lists
     int adj_have_target;  // adj_ft holds a target (comd_eam_cheb_grad with f_target NULL reuses it)
     double *adj_E;        // per replica: the energy reduction's output (3 doubles each)
+    double *adj_epart;    // per replica: its energy reduction's partials (nparts x 3 doubles)
     // staging for small per-call uploads (series, adjoint scalars): pinned
     // host memory and its device copy, stage_cap bytes each
     char *stage_h, *stage_d;
@@ -501,7 +502,7 @@ static void free_all(comd_ctx *c)
                     c->partials, c->red_out, c->d_grids, c->d_cell_grid, c->d_pots,
                     c->adj_g[0], c->adj_g[1], c->adj_g[2], c->adj_lam, c->adj_mu,
                     c->adj_ft, c->adj_f2, c->adj_part, c->adj_out, c->adj_rep, c->obs_part, c->obs_out, c->mon,
-                    c->adj_gb, c->adj_part_nl, c->adj_E, c->stage_d};
+                    c->adj_gb, c->adj_part_nl, c->adj_E, c->adj_epart, c->stage_d};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (c->stage_h) hipHostFree(c->stage_h);
@@ -2318,6 +2319,7 @@ static int cheb_grad(comd_ctx *c, const double *coef, int n, const double *f_tar
         alloc((void **)&c->adj_out, (size_t)5 * NM * 8);
         alloc((void **)&c->adj_rep, (size_t)2 * c->nrep * 8);
         alloc((void **)&c->adj_E, (size_t)3 * c->nrep * 8);
+        alloc((void **)&c->adj_epart, (size_t)3 * c->nrep * c->nparts * 8);
         if (!ok) return set_err(c, COMD_ERR_NOMEM, "adjoint buffers");
     }
     if (f_target && !emode) {
@@ -2368,15 +2370,10 @@ static int cheb_grad(comd_ctx *c, const double *coef, int n, const double *f_tar
         rc = upload_descriptors(c);
         if (rc) return rc;
     }
-    // each replica's energy into adj_E, then dL/dE on the device
-    {
-        DevState d = dev_state(c);
-        for (int r = 0; r < nrep; r++) {
-            cm_launch_reduce(d, c->reps[r].g, c->partials, c->nparts, c->stream);
-            cm_launch_reduce_final(c->partials, c->nparts, c->adj_E + 3 * r, c->stream);
-        }
-        cm_launch_adj_rep(c->adj_E, d_w, d_rw, nrep, emode, w_e, c->adj_rep, c->stream);
-    }
+    // each replica's energy into adj_E (the sums of comd_energy), then
+    // dL/dE on the device
+    cm_launch_reduce_batch(dev_state(c), c->d_grids, nrep, c->adj_epart, c->nparts, c->adj_E, c->stream);
+    cm_launch_adj_rep(c->adj_E, d_w, d_rw, nrep, emode, w_e, c->adj_rep, c->stream);
 
     const int mixed = c->precision == COMD_PREC_MIXED;
     CmAdjoint J;

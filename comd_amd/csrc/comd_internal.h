@@ -141,6 +141,11 @@ void cm_launch_force_batch(const DevState &d, const PotDev *pots, int pass, int 
 // Energies of one replica's cells: per-block partial (PE, KE, virial).
 void cm_launch_reduce(const DevState &d, const Grid &g, double *partials, int nparts, hipStream_t st);
 void cm_launch_reduce_final(const double *partials, int nparts, double *out, hipStream_t st);
+// Every replica's energies at once (grids: the device descriptors), the same
+// sums as cm_launch_reduce + cm_launch_reduce_final per replica: partials
+// (nrep * nparts * 3 doubles), then out + 3 r for replica r.
+void cm_launch_reduce_batch(const DevState &d, const Grid *grids, int nrep, double *partials, int nparts, double *out,
+                            hipStream_t st);
 // Staged value and derivative series (FP64, then FP32; 72 m8 bytes per
 // replica) copied to every replica's series buffers (pots: the device
 // descriptors).

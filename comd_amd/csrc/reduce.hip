@@ -3,13 +3,16 @@
 // block sums the partials in a fixed order.
 #include "comd_internal.h"
 
-__global__ __launch_bounds__(256) void k_reduce(DevState d, double rmass, int cell0, int ncells, double *part)
+// Block b of nb: the partial (PE, KE, virial) of the slots s0 + b * 256 +
+// thread, stepping by nb * 256, written to part[0..2].
+__device__ __forceinline__ void reduce_part(const DevState &d, double rmass, int cell0, int ncells, int b, int nb,
+                                            double *part)
 {
     __shared__ double sh[3][256];
     double pe = 0, ke = 0, vi = 0;
     const long s0 = (long)cell0 * CM_SLOTS;
     const long ns = (long)ncells * CM_SLOTS;
-    for (long s = s0 + blockIdx.x * 256L + threadIdx.x; s < s0 + ns; s += (long)gridDim.x * 256L) {
+    for (long s = s0 + b * 256L + threadIdx.x; s < s0 + ns; s += (long)nb * 256L) {
         const int c = (int)(s / CM_SLOTS);
         const int i = (int)(s % CM_SLOTS);
         if (i < d.natoms[c]) {
@@ -29,10 +32,24 @@ __global__ __launch_bounds__(256) void k_reduce(DevState d, double rmass, int ce
         __syncthreads();
     }
     if (threadIdx.x == 0)
-        for (int k = 0; k < 3; k++) part[3 * blockIdx.x + k] = sh[k][0];
+        for (int k = 0; k < 3; k++) part[k] = sh[k][0];
 }
 
-__global__ __launch_bounds__(256) void k_reduce_final(const double *part, int nparts, double *out)
+__global__ __launch_bounds__(256) void k_reduce(DevState d, double rmass, int cell0, int ncells, double *part)
+{
+    reduce_part(d, rmass, cell0, ncells, blockIdx.x, gridDim.x, part + 3 * blockIdx.x);
+}
+
+// Every replica at once: block (b, r) is block b of replica r's k_reduce
+// (the same partition, so the same sums), partials at part + 3 (r nb + b).
+__global__ __launch_bounds__(256) void k_reduce_batch(DevState d, const Grid *grids, double *part)
+{
+    const Grid &g = grids[blockIdx.y];
+    reduce_part(d, g.rmass, g.cell_base, g.ncells, blockIdx.x, gridDim.x,
+                part + 3 * ((size_t)blockIdx.y * gridDim.x + blockIdx.x));
+}
+
+__device__ __forceinline__ void reduce_final(const double *part, int nparts, double *out)
 {
     __shared__ double sh[3][256];
     double a[3] = {0, 0, 0};
@@ -49,6 +66,17 @@ __global__ __launch_bounds__(256) void k_reduce_final(const double *part, int np
         for (int k = 0; k < 3; k++) out[k] = sh[k][0];
 }
 
+__global__ __launch_bounds__(256) void k_reduce_final(const double *part, int nparts, double *out)
+{
+    reduce_final(part, nparts, out);
+}
+
+// Block r: replica r's k_reduce_final, out at out + 3 r.
+__global__ __launch_bounds__(256) void k_reduce_final_batch(const double *part, int nparts, double *out)
+{
+    reduce_final(part + (size_t)3 * nparts * blockIdx.x, nparts, out + 3 * blockIdx.x);
+}
+
 void cm_launch_reduce(const DevState &d, const Grid &g, double *partials, int nparts, hipStream_t st)
 {
     hipLaunchKernelGGL(k_reduce, dim3(nparts), dim3(256), 0, st, d, g.rmass, g.cell_base, g.ncells, partials);
@@ -57,6 +85,13 @@ void cm_launch_reduce(const DevState &d, const Grid &g, double *partials, int np
 void cm_launch_reduce_final(const double *partials, int nparts, double *out, hipStream_t st)
 {
     hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, st, partials, nparts, out);
+}
+
+void cm_launch_reduce_batch(const DevState &d, const Grid *grids, int nrep, double *partials, int nparts, double *out,
+                            hipStream_t st)
+{
+    hipLaunchKernelGGL(k_reduce_batch, dim3((unsigned)nparts, (unsigned)nrep), dim3(256), 0, st, d, grids, partials);
+    hipLaunchKernelGGL(k_reduce_final_batch, dim3((unsigned)nrep), dim3(256), 0, st, partials, nparts, out);
 }
 
 // Block r copies replica r's staged series (6 m8 doubles, then 6 m8 floats,
