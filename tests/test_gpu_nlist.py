@@ -30,7 +30,9 @@ def pair_of_sims(pot, n, lat, temp, disp, seed, evaluation="table", precision="f
     return out
 
 
-def assert_same(a, b, what, tol=1e-12, lj=False):
+def assert_same(a, b, what, tol=1e-12, lj=False, stol=None):
+    """Forces to tol; energies, rho and F' to stol (default tol)."""
+    stol = tol if stol is None else stol
     ga, gb = a.download(), b.download()
     assert np.array_equal(ga["natoms"], gb["natoms"]), what
     ca, cb = compact(ga), compact(gb)
@@ -38,19 +40,19 @@ def assert_same(a, b, what, tol=1e-12, lj=False):
     ef = force_rel_error(ca["f"], cb["f"]).max()
     ee = scalar_rel_error(ca["f"][:, 3], cb["f"][:, 3]).max()
     print(f"[{what}] lists vs cells: force {ef:.2e} energy {ee:.2e}", flush=True)
-    assert ef <= tol and ee <= tol, what
+    assert ef <= tol and ee <= stol, what
     if not lj:
-        assert scalar_rel_error(ca["rho"], cb["rho"]).max() <= tol
-        assert scalar_rel_error(ca["fi"], cb["fi"]).max() <= tol
+        assert scalar_rel_error(ca["rho"], cb["rho"]).max() <= stol
+        assert scalar_rel_error(ca["fi"], cb["fi"]).max() <= stol
     na, _ = a.download_aux()
     nb, _ = b.download_aux()
     assert np.array_equal(na, nb), f"{what}: neighbour counts"
     pa, pb = a.energy(), b.energy()
-    assert abs(pa[0] - pb[0]) <= tol * abs(pb[0])
-    assert abs(pa[2] - pb[2]) <= max(tol, 1e-10) * max(abs(pb[2]), 1e-30)
+    assert abs(pa[0] - pb[0]) <= stol * abs(pb[0])
+    assert abs(pa[2] - pb[2]) <= max(stol, 1e-10) * max(abs(pb[2]), 1e-30)
 
 
-@pytest.mark.parametrize("evaluation,precision", [("table", "fp64"), ("cheby", "fp64"), ("table", "mixed")])
+@pytest.mark.parametrize("evaluation,precision", [("table", "fp64"), ("cheby", "fp64"), ("table", "mixed"), ("cheby", "mixed")])
 def test_eam_lists_equal_cells(gpu, cu500, evaluation, precision):
     a, b = pair_of_sims(cu500, 8, 3.615, 600.0, 0.1, 3, evaluation, precision)
     if precision == "mixed":
@@ -61,7 +63,13 @@ def test_eam_lists_equal_cells(gpu, cu500, evaluation, precision):
     print(f"[lists] {info}", flush=True)
     assert info["active"] == 1 and info["valid"] == 1 and info["builds"] == 1
     assert b.list_info()["active"] == 0
-    assert_same(a, b, f"eam {evaluation} {precision}", tol=1e-12 if precision == "fp64" else 1e-6)
+    # mixed Chebyshev: the value series (phi, rho) are FP32 Clenshaw sums
+    # from FP32 coefficients, so energies and densities carry their rounding,
+    # amplified by the series' cancellation (per atom about 5e-6 of the FP64
+    # energy, the total about 2e-6); forces stay within the 1e-6 bar
+    tol = 1e-12 if precision == "fp64" else 1e-6
+    stol = 2e-5 if (evaluation, precision) == ("cheby", "mixed") else tol
+    assert_same(a, b, f"eam {evaluation} {precision}", tol=tol, stol=stol)
 
 
 def test_mixed_lists_after_fp64_steps(gpu, cu500):
