@@ -37,7 +37,7 @@ PREC_FP64, PREC_MIXED = 0, 1
 K_REBIN, K_DENSITY, K_EMBED_FORCE, K_REDUCE, K_LIST_BUILD, K_CELL_FALLBACK, K_NCLASS = 0, 1, 2, 3, 4, 5, 6
 TUNE_WAVES, TUNE_BLOCKS_PER_CU, TUNE_TABLES_LDS, TUNE_BALANCED, TUNE_GRAPH, TUNE_PAIR_LISTS, TUNE_SKIN = 1, 2, 3, 4, 5, 6, 7
 TUNE_LJ_PHYSICAL, TUNE_LIST_GROUPS, TUNE_LIST_ROWS_LDS, TUNE_LIST_THREADS_DIV = 8, 9, 10, 11
-TUNE_ADJOINT_LISTS = 12
+TUNE_ADJOINT_LISTS, TUNE_LIST_LANES = 12, 13
 
 # unit constants of src-flat/include/constants.h:61-65
 BOHR_PER_ATU_TO_A_PER_S = 0.529 / 2.418e-17
@@ -507,19 +507,20 @@ class Simulation:
 
     def tune(self, waves_per_block=None, blocks_per_cu=None, tables_in_lds=None, balanced=None, graph=None,
              pair_lists=None, skin=None, lj_physical=None, list_groups=None, list_rows_lds=None,
-             list_threads_div=None, adjoint_lists=None):
+             list_threads_div=None, adjoint_lists=None, list_lanes=None):
         """Launch shape of the force kernels, step replay as a HIP graph,
         pair-list replay in the EAM embedding-force pass, the Verlet skin in A
         (0 turns the neighbour lists off) and the Lennard-Jones force sign
         (lj_physical: atoms move down the potential), the list passes' shape
-        (list_groups, list_rows_lds, list_threads_div) and whether the
-        adjoint's pair passes use the lists (adjoint_lists) (comd_tune)."""
+        (list_groups, list_rows_lds, list_threads_div, list_lanes: lanes per
+        home atom, 0 by size) and whether the adjoint's pair passes use the
+        lists (adjoint_lists) (comd_tune)."""
         for key, val in ((TUNE_WAVES, waves_per_block), (TUNE_BLOCKS_PER_CU, blocks_per_cu),
                          (TUNE_TABLES_LDS, tables_in_lds), (TUNE_BALANCED, balanced), (TUNE_GRAPH, graph),
                          (TUNE_PAIR_LISTS, pair_lists), (TUNE_SKIN, None if skin is None else round(1000 * skin)),
                          (TUNE_LJ_PHYSICAL, lj_physical), (TUNE_LIST_GROUPS, list_groups),
                          (TUNE_LIST_ROWS_LDS, list_rows_lds), (TUNE_LIST_THREADS_DIV, list_threads_div),
-                         (TUNE_ADJOINT_LISTS, adjoint_lists)):
+                         (TUNE_ADJOINT_LISTS, adjoint_lists), (TUNE_LIST_LANES, list_lanes)):
             if val is not None:
                 self._check(self._hip.comd_tune(self.ctx, key, int(val)))
 

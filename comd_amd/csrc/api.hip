@@ -146,6 +146,7 @@ struct comd_ctx {
     int nl_group_opt;   // COMD_TUNE_LIST_GROUPS (-1: by precision)
     int nl_rows_lds;    // COMD_TUNE_LIST_ROWS_LDS (-1: whenever they fit)
     int nl_threads_div; // COMD_TUNE_LIST_THREADS_DIV: home atoms per thread of the list passes
+    int nl_lanes;       // COMD_TUNE_LIST_LANES: lanes per home atom of the list passes (0: by size)
     long nl_alloc_n, nl_alloc_k, nl_alloc_aux, nl_alloc_groups, nl_alloc_rep;
     int nl_eam_groups, nl_lj_groups;  // the list groups of the EAM replicas (first), then of the LJ ones
     int nl_eam_rep, nl_lj_rep;        // a replica of each kind (their potentials stand for the kind's)
@@ -896,6 +897,7 @@ static int ctx_create(int nrep, const comd_params *prms, comd_ctx **out)
     c->nl_group_opt = -1;
     c->nl_rows_lds = -1;
     c->nl_threads_div = 1;
+    c->nl_lanes = 0;
     c->adj_lists = 1;
     if (hipStreamSynchronize(c->stream) != hipSuccess) {
         free_all(c);
@@ -966,6 +968,14 @@ static int nl_setup(comd_ctx *c)
     const long hb = c->precision == COMD_PREC_MIXED ? 16 : 32;
     std::vector<std::array<int, 3>> gd(nrep), na(nrep), ng(nrep);
     long hcap = 0, naux = 0, ngroups = 0, threads = 64;
+    // lanes per home atom in the force passes: as tuned, else two when the
+    // atoms would fill fewer than two waves per CU at one lane each (each
+    // lane then walks half the list: 32 000 Lennard-Jones atoms, pass
+    // 0.034 -> 0.026 ms; at 256 000 atoms and above two lanes measured no
+    // better or slower)
+    long natoms_all = 0;
+    for (const Replica &R : c->reps) natoms_all += R.ntot;
+    const int lanes = c->nl_lanes > 0 ? c->nl_lanes : (natoms_all < 128L * c->ncu ? 2 : 1);
     // o >= 0: that option for every replica; o < 0 (batches): per replica
     // the shape with the fewest groups (then the fewest staged halo atoms)
     // among those whose groups' atoms (10 % spare) fit 768 threads and whose
@@ -1033,7 +1043,8 @@ static int nl_setup(comd_ctx *c)
             const long cells = (long)gd[r][0] * gd[r][1] * gd[r][2];
             const long slots = (long)(gd[r][0] + 2) * (gd[r][1] + 2) * (gd[r][2] + 2);
             hcap = std::max<long>(hcap, hcap_of(occ, slots));
-            threads = std::max<long>(threads, ((long)(1.1 * occ * (double)cells / c->nl_threads_div) + 63) / 64 * 64);
+            threads = std::max<long>(
+                threads, ((long)(1.1 * occ * (double)cells * lanes / c->nl_threads_div) + 63) / 64 * 64);
             naux += nr;
             ngroups += (long)ng[r][0] * ng[r][1] * ng[r][2];
         }
@@ -1119,6 +1130,7 @@ static int nl_setup(comd_ctx *c)
     l.pass_blocks = (int)ngroups;
     // threads per pass block: a group's home atoms with 10 % to spare
     l.pass_threads = (int)std::min<long>(768, threads);
+    l.lanes = lanes;
     l.tailq = cm_nl_pick_tailq(l);
     std::vector<NlRep> reps(nrep);
     std::vector<int> corder((size_t)naux), gq0((size_t)ngroups + 1), grep((size_t)ngroups), agroup((size_t)naux);
@@ -1653,6 +1665,12 @@ int comd_tune(comd_ctx *c, int key, int value)
     case COMD_TUNE_LIST_THREADS_DIV:
         if (value < 1 || value > 4) return set_err(c, COMD_ERR_ARG, "list threads divisor must be 1..4");
         c->nl_threads_div = value;
+        if (c->have_atoms) return nl_setup(c);
+        break;
+    case COMD_TUNE_LIST_LANES:
+        if (value != 0 && value != 1 && value != 2 && value != 4)
+            return set_err(c, COMD_ERR_ARG, "list lanes must be 0 (by size), 1, 2 or 4");
+        c->nl_lanes = value;
         if (c->have_atoms) return nl_setup(c);
         break;
     case COMD_TUNE_ADJOINT_LISTS:

@@ -234,6 +234,8 @@ struct CmList {
     int rows_lds;            // table rows in LDS: 0 never, else whenever they fit
     int pass_blocks, pass_threads;  // one block per group (passes and build)
     int tailq;               // build: list entries per thread held back for the list's end (k_nl_build)
+    int lanes;               // force passes: lanes per home atom (1, 2 or 4), each taking every
+                             // lanes-th chunk of 8 entries of the atom's list
     const int *cell_order;   // [naux]: auxiliary cells in build order (group, then cell)
     const int *group_q0;     // [ngroups + 1]: a group's first position in cell_order
     CmPos *pos;              // per build position: absolute position (kept by the drift)

@@ -664,10 +664,11 @@ template <int KIND> __host__ __device__ constexpr int nl_halo_words() { return K
 // One home atom t over its list.  TB: the table pointers (compact rows in
 // LDS when COMPACT, else the full tables in global memory); hq (mixed) or
 // hx, hy, hz, hf (FP64): the staged halo, which holds the atom itself too.
-template <int KIND, int EVAL, int MIXED, int COMPACT, class TB>
+template <int KIND, int EVAL, int MIXED, int COMPACT, int LANES, class TB>
 __device__ __forceinline__ void nl_atom(const DevState &d, const PotDev &P, const CmList &l, const TB &T, int t,
-                                        const CM_LDS v4u *hq, const CM_LDS double *hx, const CM_LDS double *hy,
-                                        const CM_LDS double *hz, const CM_LDS double *hf, double qscale, double qinv)
+                                        int sub, const CM_LDS v4u *hq, const CM_LDS double *hx,
+                                        const CM_LDS double *hy, const CM_LDS double *hz, const CM_LDS double *hf,
+                                        double qscale, double qinv)
 {
     const int i = l.order[t];
     const int s = l.slot[i];
@@ -883,17 +884,32 @@ __device__ __forceinline__ void nl_atom(const DevState &d, const PotDev &P, cons
             }
         }
     };
-    // the next chunk's index clamped to the list capacity: always a valid load
-    const int clast = l.kmax / 8 - 1;
-    v4u e = row[0];
-    int k0 = 0;
-    for (; k0 + 8 <= cnt; k0 += 8) {
-        const v4u en = row[(size_t)min((k0 >> 3) + 1, clast) * l.n];
+    // LANES: lane sub of the atom's l.lanes takes chunks sub, sub + lanes,
+    // ... (else one lane takes them all); the next chunk's index clamped to
+    // the list capacity: always a valid load
+    const int clast = l.kmax / 8 - 1, step = LANES ? l.lanes : 1;
+    if (!LANES) sub = 0;
+    v4u e = row[(size_t)min(sub, clast) * l.n];
+    int k0 = 8 * sub;
+    for (; k0 + 8 <= cnt; k0 += 8 * step) {
+        const v4u en = row[(size_t)min((k0 >> 3) + step, clast) * l.n];
         chunk(e, k0, std::false_type());
         e = en;
     }
     if (k0 < cnt) chunk(e, k0, std::true_type());
     if (UNITS) A.vir *= qinv;
+    // the atom's lanes (adjacent, the same t) add their sums; the first
+    // writes the results
+    for (int o = 1; LANES && o < step; o <<= 1) {
+        A.x += __shfl_xor(A.x, o);
+        A.y += __shfl_xor(A.y, o);
+        A.z += __shfl_xor(A.z, o);
+        A.e += __shfl_xor(A.e, o);
+        A.rho += __shfl_xor(A.rho, o);
+        A.vir += __shfl_xor(A.vir, o);
+        npair += __shfl_xor(npair, o);
+    }
+    if (LANES && sub != 0) return;
     if (KIND == CM_KIND_EAM1) {
         // pass 2, src-flat/eam.c:576-594
         double fv, fp;
@@ -938,7 +954,7 @@ __device__ __forceinline__ void nl_atom(const DevState &d, const PotDev &P, cons
 // Blocks of at most 768 threads (a group's home atoms); the Chebyshev passes
 // declare that bound, so that their four-pair recurrences (cheb64_n) get the
 // registers three waves per SIMD allow.
-template <int KIND, int EVAL, int MIXED, int TLDS>
+template <int KIND, int EVAL, int MIXED, int TLDS, int LANES>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, EVAL == COMD_EVAL_CHEB ? 768 : 1024)))
 __attribute__((amdgpu_waves_per_eu(MIXED && KIND != CM_KIND_LJ && TLDS ? 6 : 1)))
 void k_nl_pair(DevState d, PotDev P, CmList l, int tab_bytes, int gbase)
@@ -1035,9 +1051,13 @@ void k_nl_pair(DevState d, PotDev P, CmList l, int tab_bytes, int gbase)
             }
         }
         __syncthreads();
-        for (int t = S.t0 + threadIdx.x; t < S.t1; t += blockDim.x) {
-            if (TLDS) nl_atom<KIND, EVAL, MIXED, 1>(d, P, l, TL, t, hq, hx, hy, hz, hf, R.qscale, R.qinv);
-            else nl_atom<KIND, EVAL, MIXED, 0>(d, P, l, TG, t, hq, hx, hy, hz, hf, R.qscale, R.qinv);
+        // LANES: l.lanes adjacent lanes per home atom
+        const int lanes = LANES ? l.lanes : 1;
+        const int sub = (int)threadIdx.x % lanes;
+        const int per = (int)blockDim.x / lanes;
+        for (int t = S.t0 + (int)threadIdx.x / lanes; t < S.t1; t += per) {
+            if (TLDS) nl_atom<KIND, EVAL, MIXED, 1, LANES>(d, P, l, TL, t, sub, hq, hx, hy, hz, hf, R.qscale, R.qinv);
+            else nl_atom<KIND, EVAL, MIXED, 0, LANES>(d, P, l, TG, t, sub, hq, hx, hy, hz, hf, R.qscale, R.qinv);
         }
     }
 }
@@ -1232,11 +1252,13 @@ static void nl_launch(const DevState &d, const PotDev &P, const CmList &l, hipSt
     const dim3 grid(ng), block(l.pass_threads);
     if constexpr (EVAL == COMD_EVAL_TABLE && KIND != CM_KIND_LJ) {
         if (tl) {
-            hipLaunchKernelGGL((k_nl_pair<KIND, EVAL, MIXED, 1>), grid, block, lds, st, d, P, l, tab, g0);
+            if (l.lanes > 1) hipLaunchKernelGGL((k_nl_pair<KIND, EVAL, MIXED, 1, 1>), grid, block, lds, st, d, P, l, tab, g0);
+            else hipLaunchKernelGGL((k_nl_pair<KIND, EVAL, MIXED, 1, 0>), grid, block, lds, st, d, P, l, tab, g0);
             return;
         }
     }
-    hipLaunchKernelGGL((k_nl_pair<KIND, EVAL, MIXED, 0>), grid, block, lds, st, d, P, l, 0, g0);
+    if (l.lanes > 1) hipLaunchKernelGGL((k_nl_pair<KIND, EVAL, MIXED, 0, 1>), grid, block, lds, st, d, P, l, 0, g0);
+    else hipLaunchKernelGGL((k_nl_pair<KIND, EVAL, MIXED, 0, 0>), grid, block, lds, st, d, P, l, 0, g0);
 }
 
 // Resident blocks per CU of the pass nl_launch would start, and its dynamic LDS.
@@ -1245,9 +1267,11 @@ static int nl_occupancy(const PotDev &P, const CmList &l, int *lds_bytes)
 {
     int tl = 0;
     const size_t lds = cm_nl_pass_lds(P, l, KIND, MIXED, &tl);
-    const void *fn = (const void *)k_nl_pair<KIND, EVAL, MIXED, 0>;
+    const bool ln = l.lanes > 1;
+    const void *fn = ln ? (const void *)k_nl_pair<KIND, EVAL, MIXED, 0, 1> : (const void *)k_nl_pair<KIND, EVAL, MIXED, 0, 0>;
     if constexpr (EVAL == COMD_EVAL_TABLE && KIND != CM_KIND_LJ)
-        if (tl) fn = (const void *)k_nl_pair<KIND, EVAL, MIXED, 1>;
+        if (tl)
+            fn = ln ? (const void *)k_nl_pair<KIND, EVAL, MIXED, 1, 1> : (const void *)k_nl_pair<KIND, EVAL, MIXED, 1, 0>;
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, l.pass_threads, lds) != hipSuccess) nb = -1;
     if (lds_bytes) *lds_bytes = (int)lds;

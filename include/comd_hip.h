@@ -187,6 +187,9 @@ enum comd_tune_key {
     COMD_TUNE_ADJOINT_LISTS = 12,  /* the adjoint's pair passes over the neighbour lists when the
                                       context uses them: 1 (default) if it has a list group per CU at
                                       least, 2 always, 0 never (link cells) */
+    COMD_TUNE_LIST_LANES = 13,     /* 0 (default): by the system's size; 1, 2 or 4: adjacent lanes per
+                                      home atom in the list force passes, each taking every lanes-th
+                                      chunk of the atom's list (more waves for small systems) */
 };
 int comd_tune(comd_ctx *ctx, int key, int value);
 

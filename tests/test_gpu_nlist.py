@@ -72,6 +72,39 @@ def test_eam_lists_equal_cells(gpu, cu500, evaluation, precision):
     assert_same(a, b, f"eam {evaluation} {precision}", tol=tol, stol=stol)
 
 
+@pytest.mark.parametrize("lanes", [1, 4])
+@pytest.mark.parametrize("evaluation,precision", [("table", "fp64"), ("cheby", "fp64"), ("table", "mixed")])
+def test_lanes_per_atom_equal_cells(gpu, cu500, lanes, evaluation, precision):
+    """Small systems default to two lanes per home atom in the list passes
+    (COMD_TUNE_LIST_LANES); one and four must give the link-cell results
+    too (each lane takes every lanes-th chunk of the atom's list and the
+    lanes add their sums)."""
+    a, b = pair_of_sims(cu500, 8, 3.615, 600.0, 0.1, 5, evaluation, precision)
+    a.tune(list_lanes=lanes)
+    if precision == "mixed":
+        b.set_precision("fp64")
+    a.force()
+    b.force()
+    assert a.list_info()["active"] == 1 and a.list_info()["valid"] == 1
+    assert_same(a, b, f"eam {evaluation} {precision}, {lanes} lanes", tol=1e-12 if precision == "fp64" else 1e-6)
+    a.nsteps(10)
+    b.nsteps(10)
+    ca, cb = compact(a.download()), compact(b.download())
+    assert np.array_equal(ca["id"], cb["id"])
+    assert force_rel_error(ca["f"], cb["f"]).max() <= (1e-10 if precision == "fp64" else 5e-5)
+
+
+def test_lj_lanes_equal_cells(gpu):
+    lj = CA.copper_lj()
+    for lanes in (1, 2, 4):
+        a, b = pair_of_sims(lj, 7, lj.lat, 600.0, 0.05, 6)
+        a.tune(list_lanes=lanes)
+        a.force()
+        b.force()
+        assert a.list_info()["active"] == 1
+        assert_same(a, b, f"lj, {lanes} lanes", lj=True)
+
+
 def test_mixed_lists_after_fp64_steps(gpu, cu500):
     """Steps in FP64 keep only the double list positions current; switching
     to mixed precision must refresh the fixed-point ones."""
