@@ -85,6 +85,49 @@ __device__ __forceinline__ double add_basis_sum(double y, int m, double wa, doub
     return s;
 }
 
+// T_k(y) times wt plus the derivative of T_k with respect to y, k U_{k-1}(y)
+// (U: the second-kind polynomials, U_0 = 1, U_1 = 2y), times wd, added to
+// a[k], k < m.  With wd = w / (b - a) (the domain's dy/dr = 2 / (b - a)
+// times w / 2) this is, for the series' basis B_k (B_0 = T_0 / 2 applied by
+// the caller), the pair's term of the energy moments and of the derivative
+// operator applied to the force moments in one set.
+template <int MIXED>
+__device__ __forceinline__ void add_basis_deriv(double y, int m, double wt, double wd, double *a)
+{
+    a[0] += wt;
+    if (MIXED) {
+        const float yf = (float)y, y2 = 2.0f * yf;
+        float tm = 1.0f, t = yf, um = 1.0f, u = y2;  // T_{k-1}, T_k, U_{k-1}, U_k at k = 1
+#pragma unroll
+        for (int k = 1; k < AMAX; k++) {
+            if (k < m) {
+                a[k] += wt * (double)t;
+                a[k] += wd * (double)k * (double)um;
+            }
+            const float tn = y2 * t - tm, un = y2 * u - um;
+            tm = t;
+            t = tn;
+            um = u;
+            u = un;
+        }
+    } else {
+        const double y2 = 2.0 * y;
+        double tm = 1.0, t = y, um = 1.0, u = y2;
+#pragma unroll
+        for (int k = 1; k < AMAX; k++) {
+            if (k < m) {
+                a[k] += wt * t;
+                a[k] += wd * (double)k * um;
+            }
+            const double tn = y2 * t - tm, un = y2 * u - um;
+            tm = t;
+            t = tn;
+            um = u;
+            u = un;
+        }
+    }
+}
+
 __device__ __forceinline__ double cheb_arg(double x, double a, double b)
 {
     if (x < a) x = a;  // lower clamp only (src-flat/cheby.c:290-293)

@@ -2457,8 +2457,8 @@ static int cheb_grad(comd_ctx *c, const double *coef, int n, const double *f_tar
         CmAdjoint Jn = J;
         Jn.part = c->adj_part_nl;
         cm_launch_adj_nl_gather(l, J.gx, J.gy, J.gz, c->adj_gb, c->stream);
-        cm_launch_adj_nl(d, P, l, A, 1, mixed, c->stream);  // loss and energy moments: 2 ng blocks
-        cm_launch_adj_final(Jn, (int)(2 * ng), 0, c->stream);
+        cm_launch_adj_nl(d, P, l, A, 1, mixed, c->stream);  // the phi gradient's moments, one set
+        cm_launch_adj_final(Jn, (int)ng, 0, c->stream);
         cm_launch_adj_nl(d, P, l, A, 2, mixed, c->stream);
         cm_launch_adj_final(Jn, (int)ng, 1, c->stream);
         cm_launch_adj_stage(d, P, J, 2, mixed, c->stream);  // the atom pass, per slot
@@ -2494,8 +2494,14 @@ static int cheb_grad(comd_ctx *c, const double *coef, int n, const double *f_tar
     std::vector<double> col;
     for (int j = 0; j < m; j++) {
         double gphi = mom(0, 1, j), grho = mom(3, 0, j), gF = mom(2, 1, j);
-        dcol(0, j, col);
-        for (int k = 0; k < m; k++) gphi += col[k] * mom(0, 0, k);
+        if (nl) {
+            // the list pass applied the derivative operator per pair and
+            // added the energy moments: stage 0 holds dL/dc_phi itself
+            gphi += mom(0, 0, j);
+        } else {
+            dcol(0, j, col);
+            for (int k = 0; k < m; k++) gphi += col[k] * mom(0, 0, k);
+        }
         dcol(1, j, col);
         for (int k = 0; k < m; k++) grho += col[k] * mom(1, 0, k);
         dcol(2, j, col);

@@ -1072,9 +1072,14 @@ void k_nl_pair(DevState d, PotDev P, CmList l, int tab_bytes, int gbase)
 // kernels' pair queue.  dL/df and F' of the neighbours come from
 // build-order copies (gb, l.fq) through the L2; each thread keeps its
 // moments in registers and the block sums them over its waves in a fixed
-// order (block_moments), so the gradient stays deterministic.  PASS 4 is
-// the second set of pass 1 (the energy moments 1/2 dL/dE sum B_k(r_ij)), in
-// a launch of its own: both sets in registers at once would not fit.
+// order (block_moments), so the gradient stays deterministic.  Pass 1
+// differs from the link-cell kernels': instead of the force moments
+// (1/2 w B_k, contracted with the derivative operator on the host) and
+// the energy moments (1/2 dL/dE B_k) as two sets, it sums
+// 1/2 dL/dE T_k + w / (b - a) k U_{k-1} per pair, the energy term and the
+// derivative operator's result in one set of 32 registers
+// (add_basis_deriv): the derivative series of basis function k is
+// d B_k(y(r)) / dr = (2 / (b - a)) k U_{k-1}(y).
 
 // dL/df in build order (gb[3t + k]) from the slot arrays.
 __global__ __launch_bounds__(256) void k_adj_nl_gather(CmList l, const double *gx, const double *gy,
@@ -1098,7 +1103,7 @@ __global__ __launch_bounds__(768) void k_adj_nl(DevState d, PotDev P, CmList l, 
     double *hx = (double *)nls, *hy = hx + l.hcap, *hz = hy + l.hcap;
     CM_LDS uint8_t *slot_of = (CM_LDS uint8_t *)(nls + (size_t)l.hcap * 24);
     const int m = P.cn;
-    const int dom = PASS == 1 || PASS == 4 ? 0 : 1;  // phi's series domain, else rho's
+    const int dom = PASS == 1 ? 0 : 1;  // phi's series domain, else rho's
     const double ca = P.ca[dom], cb = P.cb[dom];
     const double rc2 = P.rc2;
     const int m8 = P.cstride;
@@ -1123,7 +1128,11 @@ __global__ __launch_bounds__(768) void k_adj_nl(DevState d, PotDev P, CmList l, 
         frame_pos(R, G, S.uxyz[sl], p, ox, oy, oz, hx[h], hy[h], hz[h]);
     }
     __syncthreads();
-    const double de = PASS == 4 ? A.dlde[l.group_rep[g]] : 0.0;  // weight of the energy moments
+    // pass 1: each pair's half dL/dE (its replica's) weights the energy
+    // moments, and w / (b - a) the derivative of the basis (the force
+    // moments through the derivative operator)
+    const double de = PASS == 1 ? 0.5 * A.dlde[l.group_rep[g]] : 0.0;
+    const double inv_ba = 1.0 / (cb - ca);
     for (int t = S.t0 + threadIdx.x; t < S.t1; t += blockDim.x) {
         const int s = l.slot[l.order[t]];
         const int self = l.selfh[t];
@@ -1136,7 +1145,6 @@ __global__ __launch_bounds__(768) void k_adj_nl(DevState d, PotDev P, CmList l, 
         }
         if (PASS == 2) wi = l.fq[t];
         if (PASS == 3) wi = A.mu[s];
-        if (PASS == 4) wi = 0.5 * de;
         double lam = 0.0;
         const int cnt = l.ncnt[t];
         const CM_GLOBAL v4u *row = (const CM_GLOBAL v4u *)l.nbl + t;
@@ -1161,7 +1169,7 @@ __global__ __launch_bounds__(768) void k_adj_nl(DevState d, PotDev P, CmList l, 
                 const double w = ((gix - A.gb[3L * q + 0]) * dx + (giy - A.gb[3L * q + 1]) * dy +
                                   (giz - A.gb[3L * q + 2]) * dz) / r;
                 if (PASS == 1) {
-                    add_basis<MIXED, 0>(y, m, 0.5 * w, 0.0, m0, m0);
+                    add_basis_deriv<MIXED>(y, m, de, w * inv_ba, m0);
                 } else {
                     // rho'(r) from the same basis values (rho's domain)
                     const double drho = add_basis_sum<MIXED>(y, m, 0.5 * (wi + l.fq[q]) * w, m0, dser);
@@ -1171,7 +1179,7 @@ __global__ __launch_bounds__(768) void k_adj_nl(DevState d, PotDev P, CmList l, 
         }
         if (PASS == 2) A.lam[s] = lam;
     }
-    block_moments<PASS == 4 ? 1 : 0>(m0, red, A.part + (size_t)(pbase + blockIdx.x) * ANMOM);
+    block_moments<0>(m0, red, A.part + (size_t)(pbase + blockIdx.x) * ANMOM);
 }
 
 // ---------------------------------------------------------------------------
@@ -1328,12 +1336,8 @@ static void adj_nl_launch(const DevState &d, const PotDev &P, const CmList &l, c
 {
     const dim3 grid(l.pass_blocks), block(l.pass_threads);
     const size_t lds = cm_adj_nl_lds(l);
-    const int nb = l.pass_blocks;
     switch (pass) {
-    case 1:  // the loss moments, then the energy moments into blocks nb .. 2 nb - 1
-        hipLaunchKernelGGL((k_adj_nl<1, MIXED>), grid, block, lds, st, d, P, l, A, 0);
-        hipLaunchKernelGGL((k_adj_nl<4, MIXED>), grid, block, lds, st, d, P, l, A, nb);
-        break;
+    case 1: hipLaunchKernelGGL((k_adj_nl<1, MIXED>), grid, block, lds, st, d, P, l, A, 0); break;
     case 2: hipLaunchKernelGGL((k_adj_nl<2, MIXED>), grid, block, lds, st, d, P, l, A, 0); break;
     case 3: hipLaunchKernelGGL((k_adj_nl<3, MIXED>), grid, block, lds, st, d, P, l, A, 0); break;
     default: break;
