@@ -2091,7 +2091,7 @@ int comd_snapshot_save(comd_ctx *c, const char *path)
     const CmList &l = c->nl;
     int hdr[SNAP_NHDR] = {SNAP_VERSION, c->ncells, c->precision, R.pot.eval, c->nl_active, c->nl_skin_milli,
                           l.n, l.kmax, 0, l.naux, l.ngroups, l.hcap, R.ntot, (int)R.nl_ids,
-                          R.pot.lj_fsign < 0.0, 0};
+                          R.pot.lj_fsign < 0.0, c->nl_active ? l.lanes : 0};
     const size_t S = (size_t)c->nslots, C = (size_t)c->ncells;
     bool ok = fwrite(kSnapMagic, 1, 8, fp) == 8 && fwrite(hdr, sizeof(int), SNAP_NHDR, fp) == SNAP_NHDR;
     void *arr[] = {c->natoms, c->id, c->itype, c->rx, c->ry, c->rz, c->px, c->py, c->pz, c->fx, c->fy, c->fz,
@@ -2122,10 +2122,13 @@ int comd_snapshot_load(comd_ctx *c, const char *path)
         fclose(fp);
         return set_err(c, COMD_ERR_ARG, "snapshot of %d cells, context of %d", hdr[1], c->ncells);
     }
-    // the settings that shape the state: precision, evaluation, skin, LJ sign
+    // the settings that shape the state: precision, evaluation, skin, LJ
+    // sign, and the list passes' lanes per atom (the order of their sums;
+    // 0 in files that predate them: the context's own choice)
     int rc = comd_set_precision(c, hdr[2]);
     if (!rc) rc = comd_set_eval(c, hdr[3]);
     if (!rc && c->nl_skin_milli != hdr[5]) rc = comd_tune(c, COMD_TUNE_SKIN_MILLI, hdr[5]);
+    if (!rc && (hdr[15] == 1 || hdr[15] == 2 || hdr[15] == 4)) c->nl_lanes = hdr[15];
     if (!rc && c->reps[0].prm.potential == COMD_POT_LJ) rc = comd_tune(c, COMD_TUNE_LJ_PHYSICAL, hdr[14]);
     if (rc) {
         fclose(fp);

@@ -12,8 +12,11 @@ from helpers import seeded_atoms
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_restart_continues_bit_for_bit(gpu, cu500, tmp_path, precision):
+@pytest.mark.parametrize("precision,lanes", [("fp64", None), ("mixed", None), ("fp64", 4)])
+def test_restart_continues_bit_for_bit(gpu, cu500, tmp_path, precision, lanes):
+    """lanes: the saving context's lanes per atom in the list passes (the
+    order of their sums), which the snapshot carries to a context left at
+    its default."""
     xyz, p, bounds = seeded_atoms(8, 3.615, 63.55, temp=3000.0, disp=0.05, seed=61)
 
     def fresh():
@@ -23,6 +26,8 @@ def test_restart_continues_bit_for_bit(gpu, cu500, tmp_path, precision):
         return s
 
     a = fresh()
+    if lanes:
+        a.tune(list_lanes=lanes)
     a.nsteps(20)
     builds_before = a.list_info()["builds"]
     path = tmp_path / "state.snap"
