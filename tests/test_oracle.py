@@ -251,6 +251,30 @@ def test_chebyshev_clenshaw_and_derivative_series(cu):
     cu.set_eval(False)
 
 
+def test_chebyshev_derivative_basis_identity():
+    """The derivative series (cheby.c:219-270) of basis function k,
+    evaluated in the series convention (c_0 / 2), is
+    (2 / (b - a)) k U_{k-1}(y): the identity by which the adjoint's list
+    pass applies the derivative operator per pair instead of on the summed
+    moments (csrc/adj_basis.h add_basis_deriv)."""
+    from scipy.special import eval_chebyu
+
+    pot = O.OraclePot.eam(POTS, "cu500")
+    m = pot.cheb().shape[1]
+    y = np.linspace(-1.0, 1.0, 41)
+    for k in (1, 2, 5, 16, m - 1):
+        coef = np.zeros((3, m))
+        coef[:, k] = 1.0
+        pot.set_cheb3(coef)
+        d = pot.cheb()[3:6]
+        for which in (0, 1, 2):
+            t = pot.table(which)
+            a, b = t["x0"], t["xn"]
+            got = np.polynomial.chebyshev.chebval(y, d[which]) - 0.5 * d[which][0]
+            want = 2.0 / (b - a) * k * eval_chebyu(k - 1, y)
+            assert np.abs(got - want).max() <= 1e-12 * (2.0 / (b - a)) * k * k, (k, which)
+
+
 # ---------------------------------------------------------------------------
 # force closed forms and invariants
 
