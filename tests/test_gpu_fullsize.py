@@ -17,6 +17,9 @@ counts, halo capacities and list capacity.
   convention (after about 50 steps the reference's lattice flies apart and
   its rebinning aborts, src-flat/domains.c:229), and for 100 steps in the
   physical convention.
+* the 64-replica batch (about 1.2 M atoms, half Lennard-Jones, half EAM,
+  mixed precision): sampled replicas against single contexts of their own,
+  after binning and after 50 graph-replayed steps.
 """
 import numpy as np
 import pytest
@@ -264,3 +267,87 @@ def test_lj20_physical_convention_100_steps(gpu):
     worst, ke0, ke = lj_per_step(True, 100, 0.4)
     print(f"[lj20 physical] worst {worst} KE {ke0:.4f} -> {ke:.4f} Ha", flush=True)
     assert 0.3 * ke0 < ke < 1.2 * ke0
+
+
+# ---------------------------------------------------------------------------
+# the 64-replica batch at its size
+
+
+def test_batch64_replicas_at_size(gpu, cu500):
+    """The batch configuration at its size: 64 replicas, FCC n^3 with n
+    uniform in 8..24, x-stretch uniform in [1.00, 1.10], even replicas
+    Lennard-Jones copper (physical force convention), odd ones EAM Cu,
+    300 K, mixed precision with neighbour lists, in one context (about
+    1.4 M atoms; the lattices displaced by 0.05 A so that the forces
+    compared at the start are not the perfect lattice's zeros).  Sampled replicas -- the smallest, the largest, the most
+    stretched of each kind -- against a single context of their own: slot
+    order exact and forces within 1e-6 after binning (the single one in
+    FP64 over link cells), and after 50 steps replayed from the batch's
+    step graph, against the single context stepped alike in mixed
+    precision (the same trajectory up to the two contexts' roundings: their
+    group frames quantise positions differently): slot order exact,
+    positions within 5e-8 A, forces and energies within 2e-6 (measured:
+    4e-9 A, 4e-7, 2e-7)."""
+    rng = np.random.default_rng(7)
+    lj = CA.copper_lj()
+    reps, atoms = [], []
+    for r in range(64):
+        n = int(rng.integers(8, 25))
+        stretch = float(rng.uniform(1.0, 1.10))
+        pot, lat = (lj, lj.lat) if r % 2 == 0 else (cu500, 3.615)
+        xyz = CA.fcc_positions(n, n, n, lat, stretch)
+        xyz, p = CA.seed_state(xyz, pot.mass, 300.0, 0.05, 7000 + r)
+        reps.append(dict(pot=pot, bounds=(n * lat * stretch, n * lat, n * lat), defgrad=0.0))
+        atoms.append((xyz, p, n, stretch))
+    total = sum(a[0].shape[0] for a in atoms)
+    print(f"[case] batch of 64 replicas, {total} atoms", flush=True)
+    b = CA.Batch(reps, precision="mixed")
+    b.tune(lj_physical=1)
+    for r, (xyz, p, _, _) in enumerate(atoms):
+        b.bin_atoms(r, xyz, p)
+    b.rebox()
+    b.force()
+    assert b.list_info()["active"] == 1
+
+    sizes = [a[2] for a in atoms]
+    pick = set()
+    for kind in (0, 1):
+        idx = [r for r in range(64) if r % 2 == kind]
+        pick |= {min(idx, key=lambda r: sizes[r]), max(idx, key=lambda r: sizes[r]),
+                 max(idx, key=lambda r: atoms[r][3])}
+
+    def single(r, precision, skin):
+        s = CA.Simulation(reps[r]["pot"], reps[r]["bounds"], defgrad=0.0, precision=precision)
+        s.tune(skin=skin)
+        if r % 2 == 0:
+            s.tune(lj_physical=1)
+        s.bin_atoms(atoms[r][0], atoms[r][1])
+        s.rebox()
+        s.force()
+        return s
+
+    for r in sorted(pick):
+        s = single(r, "fp64", 0.0)
+        gb, gs = compact(b.download(r)), compact(s.download())
+        assert np.array_equal(gb["id"], gs["id"]), r
+        err = force_rel_error(gb["f"], gs["f"]).max()
+        print(f"[batch64] replica {r} n={sizes[r]} {'lj' if r % 2 == 0 else 'eam'}: force vs FP64 cells {err:.2e}",
+              flush=True)
+        assert err <= MIXED_TOL, r
+        s.close()
+
+    singles = {r: single(r, "mixed", 0.4) for r in sorted(pick)}
+    b.steps(50)
+    b.force()
+    for r, s in singles.items():
+        s.steps(50)
+        s.force()
+        gb, gs = compact(b.download(r)), compact(s.download())
+        assert np.array_equal(gb["id"], gs["id"]), r
+        dr = np.abs(gb["r"] - gs["r"]).max()
+        ef = force_rel_error(gb["f"], gs["f"]).max()
+        ee = scalar_rel_error(gb["f"][:, 3], gs["f"][:, 3]).max()
+        print(f"[batch64] replica {r} after 50 steps: |dr| {dr:.2e} force {ef:.2e} energy {ee:.2e}", flush=True)
+        assert dr <= 5e-8 and ef <= 2e-6 and ee <= 2e-6, r
+        s.close()
+    b.close()
