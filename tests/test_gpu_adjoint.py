@@ -349,3 +349,48 @@ def test_adjoint_over_lists_equals_link_cells(gpu, cu500, precision):
     assert abs(e1 - e0) <= 1e-12 * abs(e0)
     compare(g1, g0, f"{precision} loss gradient, lists vs link cells", tol)
     compare(ge1, ge0, f"{precision} energy gradient, lists vs link cells", tol)
+
+
+def test_batch_16x32k_gradient_at_size(gpu, cu500):
+    """The differentiation case at its size: 16 configurations of Cu FCC
+    20^3 (32 000 atoms each, displacements sigma 0.1 A) in one batch, whose
+    adjoint runs over the neighbour lists; targets at 1 %-perturbed
+    coefficients.  The FP64 gradient against central differences of the
+    batch's own loss on 12 coefficients, and the mixed gradient against the
+    FP64 one per function."""
+    c0 = cu500.cheb[:3].copy()
+    nconf, n = 16, 20
+    reps, states = [], []
+    for k in range(nconf):
+        xyz, p, bounds = seeded_atoms(n, 3.615, 63.55, temp=0.0, disp=0.1, seed=500 + k)
+        reps.append(dict(pot=cu500, bounds=bounds, defgrad=0.0, evaluation="cheby"))
+        states.append((xyz, p))
+    b = CA.Batch(reps, precision="fp64")
+    for r, (xyz, p) in enumerate(states):
+        b.bin_atoms(r, xyz, p)
+    b.rebox()
+    b.set_cheb(perturbed(c0, 77))
+    b.force()
+    fts = [b.download(r)["f"][:, :3].copy() for r in range(nconf)]
+    ets = [b.energy_of(r)[0] for r in range(nconf)]
+    w_e = 1.0 / (4 * n ** 3) ** 2
+    loss, g = b.cheb_grad(c0, fts, ets, w_e)
+    assert b.list_info()["active"] == 1
+    h = steps(c0)
+    for k, j in [(k, j) for k in range(3) for j in (0, 1, 4, 11)]:
+        cp, cm = c0.copy(), c0.copy()
+        cp[k, j] += h[k, j]
+        cm[k, j] -= h[k, j]
+        lp, _ = b.cheb_grad(cp, None, ets, w_e)
+        lm, _ = b.cheb_grad(cm, None, ets, w_e)
+        fd = (lp - lm) / (2 * h[k, j])
+        print(f"[16x32k] coef ({k},{j}) adjoint {g[k, j]:+.10e} fd {fd:+.10e}", flush=True)
+        assert abs(g[k, j] - fd) <= 1e-5 * np.abs(g[k]).max()
+    b.set_precision("mixed")
+    lm32, g32 = b.cheb_grad(c0, None, ets, w_e)
+    rel = np.abs(g32 - g).max(axis=1) / np.abs(g).max(axis=1)
+    print(f"[16x32k] mixed: loss rel err {abs(lm32 - loss) / loss:.2e}, gradient rel err per function {rel}",
+          flush=True)
+    assert abs(lm32 - loss) <= 1e-5 * loss
+    assert (rel <= 1e-4).all()
+    b.close()
