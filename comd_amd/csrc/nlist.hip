@@ -1,3 +1,5 @@
+
+
 // nlist.hip -- Verlet neighbour lists for the force passes of a single
 // periodic simulation.
 //
@@ -661,18 +663,38 @@ __host__ __device__ constexpr int nl_halo_bytes(int kind, int mixed)
 // keeps one 16-byte read (a 20-byte stride measured 2 % slower there).
 template <int KIND> __host__ __device__ constexpr int nl_halo_words() { return KIND == CM_KIND_EAM3 ? 4 : 3; }
 
-// One home atom t over its list.  TB: the table pointers (compact rows in
-// LDS when COMPACT, else the full tables in global memory); hq (mixed) or
-// hx, hy, hz, hf (FP64): the staged halo, which holds the atom itself too.
+// What a pass reads of home atom t before its pair terms: its atom index,
+// its own halo entry, its list length and the first 8-entry chunk of its
+// list (lane sub's, with several lanes per atom).  Read by nl_head before the
+// block stages its halo, so that these loads wait alongside the staging.
+struct NlHead {
+    int i, self, cnt;
+    v4u e;
+};
+
+__device__ __forceinline__ NlHead nl_head(const CmList &l, int t, int sub)
+{
+    NlHead h;
+    h.i = l.order[t];
+    h.self = l.selfh[t];
+    h.cnt = l.ncnt[t];
+    h.e = ((const CM_GLOBAL v4u *)l.nbl)[(size_t)min(sub, l.kmax / 8 - 1) * l.n + t];
+    return h;
+}
+
+// One home atom t over its list (hd: its head, nl_head(l, t, sub)).  TB: the
+// table pointers (compact rows in LDS when COMPACT, else the full tables in
+// global memory); hq (mixed) or hx, hy, hz, hf (FP64): the staged halo,
+// which holds the atom itself too.
 template <int KIND, int EVAL, int MIXED, int COMPACT, int LANES, class TB>
 __device__ __forceinline__ void nl_atom(const DevState &d, const PotDev &P, const CmList &l, const TB &T, int t,
-                                        int sub, const CM_LDS v4u *hq, const CM_LDS double *hx,
+                                        int sub, const NlHead &hd, const CM_LDS v4u *hq, const CM_LDS double *hx,
                                         const CM_LDS double *hy, const CM_LDS double *hz, const CM_LDS double *hf,
                                         double qscale, double qinv)
 {
-    const int i = l.order[t];
+    const int i = hd.i;
     const int s = l.slot[i];
-    const int self = l.selfh[t];
+    const int self = hd.self;
     v4u qi = {0, 0, 0, 0};
     double xi = 0.0, yi = 0.0, zi = 0.0, fia = 0.0;
     constexpr int HW = nl_halo_words<KIND>();
@@ -698,7 +720,7 @@ __device__ __forceinline__ void nl_atom(const DevState &d, const PotDev &P, cons
     constexpr bool UNITS = MIXED && COMPACT && KIND != CM_KIND_LJ;
     const double rc2u = rc2 * (qscale * qscale);
     const double iu = qinv * P.invdx[0], cu = -P.x0[0] * P.invdx[0];
-    const int cnt = l.ncnt[t];
+    const int cnt = hd.cnt;
     Acc A = {0, 0, 0, 0, 0, 0};
     int npair = 0;
     // 8 entries per 16-byte load, then the 8 halo reads from LDS, all issued
@@ -889,7 +911,7 @@ __device__ __forceinline__ void nl_atom(const DevState &d, const PotDev &P, cons
     // the list capacity: always a valid load
     const int clast = l.kmax / 8 - 1, step = LANES ? l.lanes : 1;
     if (!LANES) sub = 0;
-    v4u e = row[(size_t)min(sub, clast) * l.n];
+    v4u e = hd.e;
     int k0 = 8 * sub;
     for (; k0 + 8 <= cnt; k0 += 8 * step) {
         const v4u en = row[(size_t)min((k0 >> 3) + step, clast) * l.n];
@@ -995,15 +1017,25 @@ void k_nl_pair(DevState d, PotDev P, CmList l, int tab_bytes, int gbase)
         const int g = gbase + (nb < 8 ? (int)blockIdx.x : xcd * qb + min(xcd, rem) + idx);
         const NlRep R = nl_rep(l, l.group_rep[g]);
         const GroupGeo G = group_geo(R, g - R.group_base);
+        // LANES: l.lanes adjacent lanes per home atom.  The head of the
+        // thread's first atom is read now, its wait shared with the staging
+        const int lanes = LANES ? l.lanes : 1;
+        const int sub = (int)threadIdx.x % lanes;
+        const int per = (int)blockDim.x / lanes;
+        const int ta = l.gstart[g] + (int)threadIdx.x / lanes, tb = l.gstart[g + 1];
+        NlHead hd = {0, 0, 0, {0u, 0u, 0u, 0u}};
+        if (ta < tb) hd = nl_head(l, ta, sub);
         group_setup(l, R, G, g, S, slot_of);  // its first barrier also covers the table staging
         const double ox = (G.x0 + 0.5 * G.lx) * R.ab0, oy = (G.y0 + 0.5 * G.ly) * R.ab1,
                      oz = (G.z0 + 0.5 * G.lz) * R.ab2;
         const int H = min(S.off[S.ns], l.hcap);  // never above hcap while the lists are valid
         // up to SB entries per thread in flight: their id and position loads
         // are issued together (a thread stages about hcap / blockDim of
-        // them; one at a time in the mixed embedding-force pass, which keeps
-        // to 80 registers for six waves per SIMD)
-        constexpr int SB = MIXED && KIND == CM_KIND_EAM3 ? 1 : 6;
+        // them, up to 3.8 with 4 x 4 x 3 groups; one at a time in the mixed
+        // embedding-force pass and four in the other mixed table and
+        // Lennard-Jones passes, which keep to 80 registers for six waves
+        // per SIMD with the atom's head held meanwhile)
+        constexpr int SB = MIXED ? (KIND == CM_KIND_EAM3 ? 1 : EVAL == COMD_EVAL_CHEB ? 6 : 4) : 6;
         for (int h0 = threadIdx.x; h0 < H; h0 += SB * blockDim.x) {
             // slots first, then the position loads (build order: a
             // halo slot's atoms are consecutive)
@@ -1051,14 +1083,16 @@ void k_nl_pair(DevState d, PotDev P, CmList l, int tab_bytes, int gbase)
             }
         }
         __syncthreads();
-        // LANES: l.lanes adjacent lanes per home atom
-        const int lanes = LANES ? l.lanes : 1;
-        const int sub = (int)threadIdx.x % lanes;
-        const int per = (int)blockDim.x / lanes;
-        for (int t = S.t0 + (int)threadIdx.x / lanes; t < S.t1; t += per) {
-            if (TLDS) nl_atom<KIND, EVAL, MIXED, 1, LANES>(d, P, l, TL, t, sub, hq, hx, hy, hz, hf, R.qscale, R.qinv);
-            else nl_atom<KIND, EVAL, MIXED, 0, LANES>(d, P, l, TG, t, sub, hq, hx, hy, hz, hf, R.qscale, R.qinv);
-        }
+        // the first atom with its head read above, then any further ones
+        // (more home atoms than threads) with theirs read in turn
+        auto atom = [&](int t, const NlHead &h) {
+            if (TLDS)
+                nl_atom<KIND, EVAL, MIXED, 1, LANES>(d, P, l, TL, t, sub, h, hq, hx, hy, hz, hf, R.qscale, R.qinv);
+            else
+                nl_atom<KIND, EVAL, MIXED, 0, LANES>(d, P, l, TG, t, sub, h, hq, hx, hy, hz, hf, R.qscale, R.qinv);
+        };
+        if (ta < tb) atom(ta, hd);
+        for (int t = ta + per; t < tb; t += per) atom(t, nl_head(l, t, sub));
     }
 }
 
