@@ -126,6 +126,7 @@ def run_precision(sim, precision, steps, warmup, n_atoms, timed_steps, autotune=
         lists=bool(info["active"] and info["valid"]),
         builds=info["builds"] - b0,
         kmax=info["kmax"],
+        lanes=sim.list_lanes(),
         autotune=tuned,
     )
 
@@ -330,8 +331,9 @@ def main():
     # while the neighbour lists are in use, else the link-cell pass
     lists = mixed["lists"]
     rows = int(mixed["autotune"] is None or mixed["autotune"]["rows_lds"] != 0)  # table rows in LDS
-    kd = f"k_nl_pair<0, 0, 1, {rows}>" if lists else "k_pair<0, 0, 1>"
-    ke = f"k_nl_pair<1, 0, 1, {rows}>" if lists else "k_pair<1, 0, 1>"
+    multi = int(mixed["lanes"] > 1)  # the passes' variant with several lanes per home atom
+    kd = f"k_nl_pair<0, 0, 1, {rows}, {multi}>" if lists else "k_pair<0, 0, 1>"
+    ke = f"k_nl_pair<1, 0, 1, {rows}, {multi}>" if lists else "k_pair<1, 0, 1>"
     passes = {
         CA.K_DENSITY: (kd, "density + embedding pass (src-flat/eam.c:484-594), table, mixed",
                        lambda ph: (FLOP_PAIR_DENSITY * ph + FLOP_ATOM_EMBED) * n_atoms),
@@ -423,7 +425,7 @@ def main():
                           "note": "HIP events around each kernel class on the simulation's stream, over steps run "
                                   "after the timed ones (the events split the step graph into separate launches)"},
         "neighbour_lists": {"active": mixed["lists"], "skin_A": 0.4, "builds_in_timed_steps": mixed["builds"],
-                            "list_capacity": mixed["kmax"],
+                            "list_capacity": mixed["kmax"], "lanes_per_atom": mixed["lanes"],
                             "autotuned_shape": {"mixed": mixed["autotune"], "fp64": fp64["autotune"]}},
         "energy_drift_per_atom_ha": (mixed["e1"] - mixed["e0"]) / n_atoms,
         "mixed_error_vs_fp64": dict(force_err, note="per-atom |df| / max(|f|, rms |f|) on the thermalised state "

@@ -172,6 +172,7 @@ def _load():
     hip.comd_autotune_lists.argtypes = [vp, _ip, _ip, _ip, _dp]
     hip.comd_list_info.argtypes = [vp, _ip, _ip, _ip, _ip]
     hip.comd_list_occupancy.argtypes = [vp, _ip, _ip, _ip]
+    hip.comd_list_lanes.argtypes = [vp, _ip]
     hip.comd_set_cheb.argtypes = [vp, _dp, C.c_int]
     hip.comd_get_cheb.argtypes = [vp, _dp]
     hip.comd_eam_cheb_grad.argtypes = [vp, _dp, C.c_int, _dp, C.c_double, C.c_double, _dp, _dp]
@@ -536,6 +537,12 @@ class Simulation:
         b, l, t = (C.c_int * 2)(), (C.c_int * 2)(), C.c_int()
         self._check(self._hip.comd_list_occupancy(self.ctx, b, l, C.byref(t)))
         return dict(blocks_per_cu=list(b), lds_bytes=list(l), threads=t.value)
+
+    def list_lanes(self):
+        """Lanes per home atom of the list passes (comd_list_lanes); 0 without lists."""
+        n = C.c_int()
+        self._check(self._hip.comd_list_lanes(self.ctx, C.byref(n)))
+        return n.value
 
     def autotune_lists(self):
         """Time the list passes over group shapes, table rows in LDS or not

@@ -1883,6 +1883,13 @@ int comd_list_occupancy(comd_ctx *c, int *blocks_per_cu, int *lds_bytes, int *th
     return COMD_OK;
 }
 
+int comd_list_lanes(comd_ctx *c, int *lanes)
+{
+    if (!c || !lanes) return COMD_ERR_ARG;
+    *lanes = c->nl_active ? c->nl.lanes : 0;
+    return COMD_OK;
+}
+
 int comd_download_aux(comd_ctx *c, int *nnb, double *vir)
 {
     if (!c) return COMD_ERR_ARG;

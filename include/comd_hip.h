@@ -228,6 +228,12 @@ int comd_list_info(comd_ctx *ctx, int *active, int *valid, int *builds, int *kma
  * tuning the group shape; no counterpart in the reference. */
 int comd_list_occupancy(comd_ctx *ctx, int blocks_per_cu[2], int lds_bytes[2], int *threads);
 
+/* Lanes per home atom of the neighbour-list passes (1, 2 or 4: as tuned with
+ * COMD_TUNE_LIST_LANES, else the context's default for its size); 0 while
+ * the lists are off.  A diagnostic beside comd_list_occupancy; no
+ * counterpart in the reference. */
+int comd_list_lanes(comd_ctx *ctx, int *lanes);
+
 /* Number of atom pairs inside the cutoff found by the last EAM density or
  * LJ pass (each unordered pair once: the reference's half-pair count). */
 int comd_pair_count(comd_ctx *ctx, long long *half_pairs);

@@ -86,6 +86,7 @@ def test_lanes_per_atom_equal_cells(gpu, cu500, lanes, evaluation, precision):
     a.force()
     b.force()
     assert a.list_info()["active"] == 1 and a.list_info()["valid"] == 1
+    assert a.list_lanes() == lanes and b.list_lanes() == 0
     assert_same(a, b, f"eam {evaluation} {precision}, {lanes} lanes", tol=1e-12 if precision == "fp64" else 1e-6)
     a.nsteps(10)
     b.nsteps(10)
@@ -98,10 +99,12 @@ def test_lj_lanes_equal_cells(gpu):
     lj = CA.copper_lj()
     for lanes in (1, 2, 4):
         a, b = pair_of_sims(lj, 7, lj.lat, 600.0, 0.05, 6)
+        a.force()
+        assert a.list_lanes() == 2  # the default below 128 atoms per CU
         a.tune(list_lanes=lanes)
         a.force()
         b.force()
-        assert a.list_info()["active"] == 1
+        assert a.list_info()["active"] == 1 and a.list_lanes() == lanes
         assert_same(a, b, f"lj, {lanes} lanes", lj=True)
 
 
