@@ -1,5 +1,3 @@
-
-
 // nlist.hip -- Verlet neighbour lists for the force passes of a single
 // periodic simulation.
 //
