@@ -498,22 +498,26 @@ __global__ __launch_bounds__(1024) void k_nl_build(DevState d, CmList l)
             d.nlr[ts] = make_float4((float)p.x, (float)p.y, (float)p.z, __int_as_float(t));
         }
     }
-    if (ovf) atomicOr(l.flags, (unsigned)CM_NL_OVERFLOW);
-    // the last block to finish closes the build
-    __syncthreads();
+    // the last block to finish closes the build.  The ticket word carries
+    // the overflow too (bit 30, set before the block's ticket is taken): two
+    // atomics on one address by one thread stay in order, so the last
+    // ticket sees every overflow without a fence per block (each fence
+    // writes back the L2 on this part, and the other blocks' lists are read
+    // only by later kernels)
+    const int bovf = __syncthreads_or((int)ovf);
     if (threadIdx.x == 0) {
-        __threadfence();
-        last = atomicAdd(&l.count[2], 1) == (int)gridDim.x - 1;
+        if (bovf) atomicOr(&l.count[2], 1 << 30);
+        const int old = atomicAdd(&l.count[2], 1);
+        last = (old & 0xffffff) == (int)gridDim.x - 1 ? 1 + ((old >> 30) & 1) : 0;
     }
     __syncthreads();
     if (!last) return;
-    __threadfence();
     for (int a = threadIdx.x; a < l.naux; a += blockDim.x) l.aux_cnt[a] = 0;
     if (threadIdx.x == 0) {
         l.count[2] = 0;
         l.count[1] += 1;  // builds so far
-        const unsigned f = atomicOr(l.flags, 0u);
-        if (f & CM_NL_OVERFLOW) atomicOr(l.flags, (unsigned)CM_NL_INVALID);
+        const unsigned f = atomicOr(l.flags, 0u);  // and the earlier kernels' overflow
+        if ((f & CM_NL_OVERFLOW) || last == 2) atomicOr(l.flags, (unsigned)CM_NL_INVALID);
         else atomicAnd(l.flags, ~(unsigned)CM_NL_INVALID);
         atomicAnd(l.flags, ~(unsigned)(CM_NL_OVERFLOW | CM_NL_REBUILD));
     }
