@@ -954,17 +954,22 @@ __device__ __forceinline__ void nl_atom(const DevState &d, const PotDev &P, cons
         d.nnb[s] = npair;
         l.fq[t] = fp;  // staged with the position by the embedding-force pass (coalesced: t is the thread's)
     } else if (KIND == CM_KIND_EAM3) {
-        d.fx[s] = l.f1[3L * t + 0] + A.x;
-        d.fy[s] = l.f1[3L * t + 1] + A.y;
-        d.fz[s] = l.f1[3L * t + 2] + A.z;
+        // the three loads before any store: the slot arrays and the
+        // build-order pair forces may alias as far as the compiler knows,
+        // which would otherwise put a load's wait after each store
+        const double px = l.f1[3L * t + 0], py = l.f1[3L * t + 1], pz = l.f1[3L * t + 2];
+        d.fx[s] = px + A.x;
+        d.fy[s] = py + A.y;
+        d.fz[s] = pz + A.z;
         d.vir[s] = A.vir;
     } else {
         const double px = d.px[s];
+        const int ne = l.nonempty[s / CM_SLOTS];  // read before the stores, as above
         d.fx[s] = P.lj_fsign * A.x;
         d.fy[s] = P.lj_fsign * A.y;
         d.fz[s] = P.lj_fsign * A.z;
         d.fe[s] = A.e;
-        d.vir[s] = A.vir - (double)l.nonempty[s / CM_SLOTS] * (px * px / P.mass_amu);
+        d.vir[s] = A.vir - (double)ne * (px * px / P.mass_amu);
         d.nnb[s] = npair;
     }
 }
