@@ -247,20 +247,33 @@ __device__ __forceinline__ void frame_pos(const NlRep &R, const GroupGeo &G, int
 // kept small, with strided loops)
 
 // Each atom into its auxiliary cell (counts start at zero: the previous
-// build's last block cleared them).
+// build's last block cleared them).  Atoms taken in slot order, 16 lanes per
+// link cell (its atoms beyond 16 in further rounds), so that the positions
+// and ids are read in whole lines; the order within an auxiliary cell is
+// set by k_nl_sort.  Ids outside the replica's range are left out, as
+// k_nl_from_slots leaves them out of the id -> slot map (and marks the
+// lists invalid).
 __global__ __launch_bounds__(256) void k_nl_key(DevState d, CmList l)
 {
     if (!nl_rebuild(l)) return;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < l.n; i += gridDim.x * blockDim.x) {
-        const int s = l.slot[i];
-        if (s < 0) continue;
-        const NlRep R = nl_rep(l, rep_of(d, s / CM_SLOTS));
-        int ax, ay, az;
-        aux_coords(R, slot_pos(grid_of(d, s / CM_SLOTS), d, s), ax, ay, az);
-        const int a = R.aux_base + ax + R.na0 * (ay + R.na1 * az);
-        const int k = atomicAdd(&l.aux_cnt[a], 1);
-        if (k < CM_NL_AUXCAP) l.aux_mem[a * CM_NL_AUXCAP + k] = i;
-        else atomicOr(l.flags, (unsigned)CM_NL_OVERFLOW);
+    const int sub = threadIdx.x & 15;
+    const int c0 = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 4), dc = (int)((gridDim.x * blockDim.x) >> 4);
+    for (int c = c0; c < d.ncells; c += dc) {
+        const int n = d.natoms[c];
+        if (sub >= n) continue;
+        const NlRep R = nl_rep(l, rep_of(d, c));
+        const Grid g = grid_of(d, c);
+        for (int t = sub; t < n; t += 16) {
+            const int s = c * CM_SLOTS + t;
+            const int id = d.id[s];
+            if (id < 0 || id >= R.nid) continue;
+            int ax, ay, az;
+            aux_coords(R, slot_pos(g, d, s), ax, ay, az);
+            const int a = R.aux_base + ax + R.na0 * (ay + R.na1 * az);
+            const int k = atomicAdd(&l.aux_cnt[a], 1);
+            if (k < CM_NL_AUXCAP) l.aux_mem[a * CM_NL_AUXCAP + k] = R.id_base + id;
+            else atomicOr(l.flags, (unsigned)CM_NL_OVERFLOW);
+        }
     }
 }
 
